@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""ME+MC throughput of the batched fractional-refinement path at 1080p.
+
+Workload (one step = one frame): 1920x1080 10-bit 4:2:0, seeded smooth-field
+pictures, per-64x64-CTU motion uniform in +/-32 px at 1/16 pel, seeded QTMT
+partition of every CTU (4x4..64x64 incl. 1:2/1:4), 32 fractional candidates
+per block: 8-tap luma + 4-tap chroma interpolation, SATD (8x8 / 4x4 / 2x2
+Hadamard), (SATD<<8) + bits*lambda; then motion compensation of each block's
+best candidate into the prediction picture (luma + chroma).
+
+Reported: batched form (persistent kernels replayed from a HIP graph) as the
+headline value, the plain form (one launch per block shape) beside it, the
+per-frame latency p50/p99, the refinement kernel against the HBM roofline
+(kernel time from HIP events on the session stream), and the CPU oracle
+timed on a bounded sample on this host.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+"""
+import argparse
+import concurrent.futures
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12     # lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+METRIC = "ME+MC Mpixel/s at 1080p (batched vs plain, bit-exact); p50/p99 frame latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(fp, seconds, offs, lam):
+    """The oracle on a bounded sample of the same workload, host threads."""
+    from tests.oracle_lib import OrRefinePrm, load_oracle
+    oracle = load_oracle()
+    pad = 80
+    W, H = fp.width, fp.height
+    refp = [np.ascontiguousarray(np.pad(p, pad if i == 0 else pad // 2, mode="edge")) for i, p in enumerate(fp.ref)]
+    blk = np.ascontiguousarray(fp.blocks.astype(np.int32))
+    nc = len(offs)
+    offx = np.ascontiguousarray(np.array([o[0] for o in offs], np.int16))
+    offy = np.ascontiguousarray(np.array([o[1] for o in offs], np.int16))
+    prm = OrRefinePrm(nc, offx.ctypes.data, offy.ctypes.data, lam, 64, 1)
+    rsy, rsc = W + 2 * pad, W // 2 + pad
+    P = lambda a, off=0: ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    chunk = 256
+    n = len(blk)
+
+    def work(start):
+        cnt = min(chunk, n - start)
+        cost = np.zeros(cnt * nc, np.uint32)
+        a = [np.zeros(cnt, t) for t in (np.int32, np.uint32, np.int32, np.int32)]
+        pred = [np.zeros((H, W), np.uint16), np.zeros((H // 2, W // 2), np.uint16), np.zeros((H // 2, W // 2), np.uint16)]
+        oracle.or_RefineFracBlocks(P(fp.cur[0]), P(fp.cur[1]), P(fp.cur[2]), W, W // 2,
+                                   P(refp[0], pad * rsy + pad), P(refp[1], (pad // 2) * rsc + pad // 2),
+                                   P(refp[2], (pad // 2) * rsc + pad // 2), rsy, rsc, W, H,
+                                   P(blk, start * 8), cnt, ctypes.byref(prm), P(cost), P(a[0]), P(a[1]),
+                                   P(a[2]), P(a[3]), P(pred[0]), P(pred[1]), P(pred[2]), W, W // 2)
+        return int((blk[start:start + cnt, 2] * blk[start:start + cnt, 3]).sum())
+
+    px = 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        pos = 0
+        while time.perf_counter() - t0 < seconds:
+            starts = [((pos + k) * chunk) % n for k in range(threads * 4)]
+            pos += threads * 4
+            px += sum(ex.map(work, starts))
+    dt = time.perf_counter() - t0
+    return {"value": round(px / dt / 1e6, 3), "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C, -O3) refinement+MC of the workload's blocks in chunks of {chunk}, "
+                      f"{threads} threads, {dt:.1f} s, {px / 1e6:.1f} Mpixel of blocks"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "The path would shard by streams or CTU-row bands; see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from xin26x_amd import capi, workloads as wl
+    from xin26x_amd.device import DevicePicture, blocks_to_struct, refine_cfg
+
+    torch.cuda.set_device(0)
+    lib = capi.load()
+    log("generating the 1080p workload")
+    fp = wl.frame_pair_1080p()
+    W, H = fp.width, fp.height
+    n = len(fp.blocks)
+    offs = [tuple(int(v) for v in o) for o in wl.CAND32]
+    nc = len(offs)
+    cfg = refine_cfg(offs, wl.LAMBDA_REFINE)
+
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), n), "session")
+    cur, ref = DevicePicture(fp.cur, 80), DevicePicture(fp.ref, 80)
+    pred = DevicePicture.empty(W, H)
+    capi.check(lib.xg_session_set_picture(s, 0, cur.ref()), "picture")
+    capi.check(lib.xg_session_set_picture(s, 1, ref.ref()), "picture")
+    barr = blocks_to_struct(fp.blocks)
+    capi.check(lib.xg_session_set_blocks(s, barr.ctypes.data, n), "blocks")
+    if not args.no_autotune:
+        log("autotuning candidates-per-task per block shape")
+        capi.check(lib.xg_session_autotune(s, ctypes.byref(cfg), 2, None), "autotune")
+
+    def run(mode, steps):
+        for _ in range(steps):
+            capi.check(lib.xg_refine_frac(s, ctypes.byref(cfg), pred.ref(), mode), "refine")
+
+    def timed(mode):
+        run(mode, args.warmup)
+        capi.check(lib.xg_session_sync(s), "sync")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(mode, args.steps)
+        capi.check(lib.xg_session_sync(s), "sync")
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    def outputs():
+        c = np.empty(n * nc, np.uint32)
+        b = np.empty(n * 4, np.int32)
+        capi.check(lib.xg_session_read(s, c.ctypes.data, b.ctypes.data), "read")
+        return c, b, pred.planes()
+
+    log("plain form")
+    t_plain = timed(capi.MODE_PLAIN)
+    out_plain = outputs()
+    log("batched form (graph)")
+    t_graph = timed(capi.MODE_GRAPH)
+    out_graph = outputs()
+    exact = all(np.array_equal(a, b) for a, b in zip(out_plain[:2], out_graph[:2])) and \
+        all(np.array_equal(a, b) for a, b in zip(out_plain[2], out_graph[2]))
+
+    # per-frame latency: submit one frame, wait for it
+    lat = []
+    for _ in range(max(args.steps, 20)):
+        t0 = time.perf_counter()
+        run(capi.MODE_GRAPH, 1)
+        capi.check(lib.xg_session_sync(s), "sync")
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat)
+
+    # dominant kernel (fractional refinement) timed with HIP events on the session stream
+    capi.check(lib.xg_session_set_timing(s, 1), "timing")
+    capi.check(lib.xg_session_reset_timers(s), "timers")
+    run(capi.MODE_BATCHED, args.steps)
+    ms = ctypes.c_float()
+    cnt = ctypes.c_int32()
+    capi.check(lib.xg_session_kernel_time(s, 0, ctypes.byref(ms), ctypes.byref(cnt)), "kernel time")
+    capi.check(lib.xg_session_set_timing(s, 0), "timing")
+    lib.xg_session_destroy(s)
+
+    frame_px = W * H
+    pic_bytes = 2 * (W * H + 2 * (W // 2) * (H // 2))
+    alg_bytes = 2 * pic_bytes + n * nc * 4          # current + reference read once, costs written once
+    k_s = ms.value / 1e3
+    achieved = alg_bytes / k_s / 1e9
+    cand_px = int((fp.blocks[:, 2] * fp.blocks[:, 3]).sum()) * nc
+    value = frame_px / t_graph / 1e6
+    plain = frame_px / t_plain / 1e6
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU oracle baseline")
+        cpu = cpu_baseline(fp, args.cpu_seconds, offs, wl.LAMBDA_REFINE)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_graph * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded smooth-field 10-bit frames, per-CTU motion, QTMT blocks)",
+        "config": {"workload": "1080p fractional refinement + MC: 1920x1080 10-bit 4:2:0, "
+                               f"{n} QTMT blocks, {nc} fractional candidates/block, 8-tap luma + 4-tap "
+                               "chroma interpolation + SATD, MC of the best candidate",
+                   "frame": f"{W}x{H}", "blocks": n, "candidates": nc, "global_batch": 1,
+                   "parallelism": "single GPU"},
+        "plain_value": round(plain, 2),
+        "batched_vs_plain": round(value / plain, 3),
+        "bit_exact_batched_vs_plain": bool(exact),
+        "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                             "p99": round(float(np.percentile(lat, 99)), 4)},
+        "block_cand_evals_per_s": round(n * nc / t_graph, 1),
+        "luma_cand_px_per_s": round(cand_px / t_graph, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_refine", "kernel_ms": round(ms.value, 4), "launches": cnt.value,
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,378 @@
+/*
+ * xg266.h -- C-ABI of the MI355X (gfx950) VVC block-matching library.
+ *
+ * Two layers:
+ *
+ *  1. Per-call primitives with exactly the signatures of xin26x's pixel
+ *     primitive table `xin_func_struct` (h266_encoder/include/h266_func_struct.h:719-799)
+ *     and of its motion-estimation entry points.  Pointers are HOST pointers
+ *     into the caller's buffers, as in the reference; each call stages the
+ *     bytes it needs to HBM, runs one HIP kernel and returns synchronously.
+ *     A C encoder assigns these to its table after Xin266FuncInit; the
+ *     Xg266FuncInit helper fills the mirror table below.
+ *
+ *  2. Batched entry points over DEVICE pointers: all blocks of a frame (or of
+ *     many frames of many streams) in one persistent kernel per stage, fed by
+ *     per-shape work queues, optionally replayed from a captured HIP graph.
+ *
+ * Types: PIXEL is uint16 (10-bit samples in [0,1023]), COEFF is int32; these
+ * match the reference's ENABLE_10BIT_ENCODER build
+ * (video_api/video_common/include/xin_typedef.h:72-94).  Strides are in
+ * elements.  All functions are thread-safe (one HIP stream per calling host
+ * thread).  Per-call primitives return void like the reference; a failure
+ * sets the value returned by xg_last_error() and reports through the logger.
+ */
+#ifndef XG266_H
+#define XG266_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef uint16_t xg_pixel;
+typedef int32_t  xg_coeff;
+
+/* ------------------------------------------------------------------------
+ * Status, errors, logging
+ * ---------------------------------------------------------------------- */
+typedef enum xg_status {
+    XG_OK = 0,
+    XG_ERR_ARG = 1,        /* invalid argument (size, pointer, range)  */
+    XG_ERR_HIP = 2,        /* a HIP runtime call failed                 */
+    XG_ERR_NOMEM = 3,      /* device or pinned allocation failed        */
+    XG_ERR_BUDGET = 4,     /* admission refused: HBM budget exhausted   */
+    XG_ERR_STATE = 5       /* call not valid in the object's state      */
+} xg_status;
+
+/* Last error of the calling thread (XG_OK if none since xg_clear_error). */
+int32_t xg_last_error(void);
+void    xg_clear_error(void);
+/* Logger callback: level 0 error, 1 warning, 2 info.  NULL restores stderr. */
+typedef void (*xg_log_fn)(int32_t level, const char *msg);
+void    xg_set_logger(xg_log_fn fn);
+/* Library version string and the device architecture it was built for. */
+const char *xg_version(void);
+
+/* ------------------------------------------------------------------------
+ * 1. Per-call primitives (host pointers, synchronous)
+ * ---------------------------------------------------------------------- */
+
+/* XinComputeDist (h266_func_struct.h:56-63).  pfXinComputeSad[] slot;
+ * replaces the closed Xin26xComputeSad (h26x_core/include/h26x_compute_dist.h:27). */
+void Xg266ComputeSad(xg_pixel *input, intptr_t inputStride, xg_pixel *pred, intptr_t predStride,
+                     uint32_t width, uint32_t height, uint32_t *sad);
+/* pfXinComputeSatd[] slot; replaces the closed Xin26xComputeSatd
+ * (h26x_compute_dist.h:806-937).  Hadamard 8x8 tiles, 4x4 when a side is 4. */
+void Xg266ComputeSatd(xg_pixel *input, intptr_t inputStride, xg_pixel *pred, intptr_t predStride,
+                      uint32_t width, uint32_t height, uint32_t *satd);
+/* XinComputeSadx8 (h266_func_struct.h:87-94); replaces Xin26xComputeSadx8
+ * (h26x_compute_dist.h:400-735). */
+void Xg266ComputeSadx8(xg_pixel *input, intptr_t inputStride, xg_pixel *ref[8], intptr_t refStride,
+                       uint32_t width, uint32_t height, uint32_t *sad);
+/* XinComputeSadx5 (h266_func_struct.h:106-117). */
+void Xg266ComputeSadx5(xg_pixel *input, intptr_t inputStride, xg_pixel *ref0, xg_pixel *ref1,
+                       xg_pixel *ref2, xg_pixel *ref3, xg_pixel *ref4, intptr_t refStride,
+                       uint32_t width, uint32_t height, uint32_t *sad);
+/* XinComputeSadx3 (h266_func_struct.h:119-128). */
+void Xg266ComputeSadx3(xg_pixel *input, intptr_t inputStride, xg_pixel *ref0, xg_pixel *ref1,
+                       xg_pixel *ref2, intptr_t refStride, uint32_t width, uint32_t height,
+                       uint32_t *sad);
+/* XinComputeHada8x8 (h266_func_struct.h:448-451); replaces
+ * Xin26xComputeHada8x8 (h26x_compute_dist.h:937). */
+void Xg266ComputeHada8x8(xg_pixel *input, intptr_t inputStride, uint32_t *satd);
+
+/* XinMotionInterp (h266_func_struct.h:130-138): luma slots of
+ * pfXinLumaInterp[!!fx][!!fy][] (h266_func_init.c:424-458), replacing
+ * h266_core/source/h266_luma_mc.c:72-428. */
+void Xg266InterpCopy(const xg_pixel *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                     int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266LumaInterpHor(const xg_pixel *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                        int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266LumaInterpVet(const xg_pixel *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                        int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266LumaInterpHorVet(const xg_pixel *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                           int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+/* XinMotionInterpS16 (h266_func_struct.h:140-148): pfXinLumaInterpS16[][][]
+ * (h266_func_init.c:460-494), replacing h266_luma_mc.c:216-273, 430-559. */
+void Xg266InterpCopyU8S16(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride,
+                          int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266LumaInterpHorU8S16(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride,
+                             int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266LumaInterpVetU8S16(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride,
+                             int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266LumaInterpHorVetU8S16(const xg_pixel *src, intptr_t srcStride, int16_t *dst,
+                                intptr_t dstStride, int32_t frac, uint32_t width, uint32_t height,
+                                uint32_t filterIndex);
+/* Second-pass helpers with S16 input (h266_luma_mc.c:275-392). */
+void Xg266LumaInterpVetS16U8(const int16_t *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                             int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266LumaInterpVetS16S16(const int16_t *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride,
+                              int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+/* Chroma: pfXinChromaInterp / pfXinChromaInterpS16 (h266_func_struct.h:741,744),
+ * replacing h266_core/source/h266_chroma_mc.c:62-384 (frac = (fy<<5)|fx). */
+void Xg266ChromaInterpHor(const xg_pixel *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                          int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266ChromaInterpVet(const xg_pixel *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                          int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266ChromaInterpHorVet(const xg_pixel *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                             int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266ChromaInterpHorU8S16(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride,
+                               int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266ChromaInterpVetU8S16(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride,
+                               int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266ChromaInterpHorVetU8S16(const xg_pixel *src, intptr_t srcStride, int16_t *dst,
+                                  intptr_t dstStride, int32_t frac, uint32_t width, uint32_t height,
+                                  uint32_t filterIndex);
+void Xg266ChromaInterpVetS16U8(const int16_t *src, intptr_t srcStride, xg_pixel *dst, intptr_t dstStride,
+                               int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+void Xg266ChromaInterpVetS16S16(const int16_t *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride,
+                                int32_t frac, uint32_t width, uint32_t height, uint32_t filterIndex);
+/* XinMotionInterpAvg / XinMotionInterpWeight (h266_func_struct.h:150-173):
+ * pfXinInterpAvg / pfXinInterpWeight, replacing h266_luma_mc.c:561-629. */
+void Xg266InterpAvgS16U8(const int16_t *src0, intptr_t src0Stride, const int16_t *src1,
+                         intptr_t src1Stride, xg_pixel *dst, intptr_t dstStride, uint32_t width,
+                         uint32_t height);
+void Xg266InterpWeightS16U8(const int16_t *src0, intptr_t src0Stride, const int16_t *src1,
+                            intptr_t src1Stride, xg_pixel *dst, intptr_t dstStride, uint32_t width,
+                            uint32_t height, int32_t weightA, int32_t weightB);
+
+/* XinFor1dDct (h266_func_struct.h:198-204): pfXinFor1dDct[type][lgN]
+ * (h266_func_init.c:349-374), replacing the closed Xin26xFDct2P2..P64,
+ * Xin26xFDst7P4..P32, Xin26xFDct8P4..P32 (h26x_core/include/h26x_forward_1d_trans.h:31-141).
+ *   out[k*outputStride + j] = (sum_n T[k][n]*in[j*inputStride + n] + 2^(shift-1)) >> shift,
+ *   j < line; outputs past the VVC zero-out (32 for DCT-II-64, 16 for 32-point
+ *   DST-VII/DCT-VIII) are written as 0. */
+void Xg26xFDct2P2(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct2P4(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct2P8(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct2P16(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct2P32(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct2P64(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDst7P4(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDst7P8(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDst7P16(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDst7P32(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct8P4(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct8P8(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct8P16(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+void Xg26xFDct8P32(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t shift);
+/* XinFor2dDct2 (h266_func_struct.h:206-215): pfXinFor2dDct2[lgW][lgH]
+ * (h266_func_init.c:72-85), replacing the closed Xin26xFDct2WxH
+ * (h26x_core/include/h26x_forward_2d_trans.h:23).  tempBuf is unused. */
+void Xg26xFDct2WxH(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                   uint32_t width, uint32_t height, xg_coeff *tempBuf, int32_t shift1st,
+                   int32_t shift2nd);
+/* XinBlockSubForDct: pfXinBlockSubForDct[] (h266_sub_dct.c:74-82), replacing
+ * the closed Xin26xBlockSubForDct (h26x_core/include/h26x_block_utility.h:151). */
+void Xg26xBlockSubForDct(xg_pixel *input, intptr_t inputStride, xg_pixel *recon, intptr_t reconStride,
+                         xg_coeff *diff, intptr_t diffStride, uint32_t width, uint32_t height);
+/* Residual + transform of one TU with the dispatch of Xin266FDctWxH
+ * (h266_encoder/source/h266_forward_trans.c:45-152): mtsIdx per
+ * h26x_trans_context.h:42-51, implicit MTS for intra luma, shifts
+ * lgW+bitDepth-9 and lgH+6.  Transform skip (mtsIdx 1) returns XG_ERR_ARG. */
+int32_t Xg266FDctWxH(const xg_coeff *input, intptr_t inputStride, xg_coeff *output,
+                     intptr_t outputStride, uint32_t width, uint32_t height, uint32_t mtsIdx,
+                     int32_t isIntraLuma, int32_t enableMts);
+
+/* Mirror of the hot-path slots of xin_func_struct (h266_func_struct.h:719-799),
+ * same names, same index conventions (index = log2 of the width). */
+typedef void (*XgComputeDist)(xg_pixel *, intptr_t, xg_pixel *, intptr_t, uint32_t, uint32_t, uint32_t *);
+typedef void (*XgComputeSadx8)(xg_pixel *, intptr_t, xg_pixel *[8], intptr_t, uint32_t, uint32_t, uint32_t *);
+typedef void (*XgComputeSadx5)(xg_pixel *, intptr_t, xg_pixel *, xg_pixel *, xg_pixel *, xg_pixel *,
+                               xg_pixel *, intptr_t, uint32_t, uint32_t, uint32_t *);
+typedef void (*XgComputeSadx3)(xg_pixel *, intptr_t, xg_pixel *, xg_pixel *, xg_pixel *, intptr_t,
+                               uint32_t, uint32_t, uint32_t *);
+typedef void (*XgMotionInterp)(const xg_pixel *, intptr_t, xg_pixel *, intptr_t, int32_t, uint32_t,
+                               uint32_t, uint32_t);
+typedef void (*XgMotionInterpS16)(const xg_pixel *, intptr_t, int16_t *, intptr_t, int32_t, uint32_t,
+                                  uint32_t, uint32_t);
+typedef void (*XgMotionInterpAvg)(const int16_t *, intptr_t, const int16_t *, intptr_t, xg_pixel *,
+                                  intptr_t, uint32_t, uint32_t);
+typedef void (*XgMotionInterpWeight)(const int16_t *, intptr_t, const int16_t *, intptr_t, xg_pixel *,
+                                     intptr_t, uint32_t, uint32_t, int32_t, int32_t);
+typedef void (*XgFor1dDct)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, int32_t, int32_t);
+typedef void (*XgFor2dDct2)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, uint32_t, uint32_t, xg_coeff *,
+                            int32_t, int32_t);
+typedef void (*XgBlockSubForDct)(xg_pixel *, intptr_t, xg_pixel *, intptr_t, xg_coeff *, intptr_t,
+                                 uint32_t, uint32_t);
+typedef void (*XgComputeHada8x8)(xg_pixel *, intptr_t, uint32_t *);
+
+#define XG_BLOCK_NUM 8
+#define XG_TRANS_TYPE 3        /* XIN_DCT2 = 0, XIN_DCT8 = 1, XIN_DST7 = 2 */
+
+typedef struct xg_func_struct {
+    XgComputeDist        pfXinComputeSad[XG_BLOCK_NUM];
+    XgComputeDist        pfXinComputeSatd[XG_BLOCK_NUM];
+    XgComputeDist        pfXinComputeDist[XG_BLOCK_NUM];
+    XgComputeSadx8       pfXinComputeSadx8[XG_BLOCK_NUM];
+    XgComputeSadx5       pfXinComputeSadx5[XG_BLOCK_NUM];
+    XgComputeSadx3       pfXinComputeSadx3[XG_BLOCK_NUM];
+    XgMotionInterp       pfXinLumaInterp[2][2][XG_BLOCK_NUM];
+    XgMotionInterp       pfXinChromaInterp[2][2][XG_BLOCK_NUM];
+    XgMotionInterpS16    pfXinLumaInterpS16[2][2][XG_BLOCK_NUM];
+    XgMotionInterpS16    pfXinChromaInterpS16[2][2][XG_BLOCK_NUM];
+    XgMotionInterpAvg    pfXinInterpAvg[XG_BLOCK_NUM];
+    XgMotionInterpWeight pfXinInterpWeight[XG_BLOCK_NUM];
+    XgFor2dDct2          pfXinFor2dDct2[XG_BLOCK_NUM][XG_BLOCK_NUM];
+    XgFor1dDct           pfXinFor1dDct[XG_TRANS_TYPE][XG_BLOCK_NUM];
+    XgBlockSubForDct     pfXinBlockSubForDct[XG_BLOCK_NUM];
+    XgComputeHada8x8     pfXinComputeHada8x8;
+} xg_func_struct;
+
+/* Fills every slot above with this library's functions (NULL where the
+ * reference table holds NULL: SATD widths 1-2, DST-VII/DCT-VIII outside
+ * 4..32, DCT-II 1 and 128), then binds pfXinComputeDist to SAD or SATD as
+ * Xin266FuncUpdate does (h266_encoder_create.c:3332-3400).  Initialises the
+ * HIP device on the first call. */
+int32_t Xg266FuncInit(xg_func_struct *funcSet, int32_t satdMd);
+
+/* Motion-estimation boundary.  Field-for-field the inputs and outputs the
+ * reference's closed Xin26xMotionEst* read and write in xin_me_struct
+ * (h26x_core/include/h26x_me_struct.h:60-131); MVs: minMv/maxMv/meCand in
+ * integer pel, predMv/bestMv in quarter pel.  `ref` points at the co-located
+ * block in a padded reference picture (the caller guarantees padding for
+ * [minMv, maxMv] plus 4 samples of filter reach). */
+typedef struct xg_mv { int16_t mvX, mvY; } xg_mv;
+typedef struct xg_me_struct {
+    uint32_t  width, height;
+    xg_mv     minMv, maxMv;
+    xg_mv     meCand[12];
+    uint32_t  candNum;
+    xg_mv     bestMv, predMv;
+    uint32_t  bestCost, bestSad;
+    int32_t   searchWidth, searchHeight;
+    uint32_t  sadLambda;
+    xg_pixel *input;  intptr_t inputStride;
+    xg_pixel *ref;    intptr_t refStride;
+} xg_me_struct;
+
+/* Replaces Xin26xMotionEstFull (h26x_core/include/h26x_motion_est.h:26):
+ * exhaustive raster search of +/-searchWidth x +/-searchHeight around the
+ * rounded predictor, clipped to [minMv, maxMv]; cost (SAD<<8) + bits*lambda. */
+void Xg26xMotionEstFull(xg_me_struct *meSet);
+/* Replaces Xin26xMotionEstDia (h26x_motion_est.h:35): start candidates then
+ * small-diamond descent. */
+void Xg26xMotionEstDia(xg_me_struct *meSet);
+/* Replaces Xin266FullSubPelMe (h266_encoder/source/h266_sub_pel_refine.c:81-212):
+ * half-pel then quarter-pel ring with 8-tap interpolation and SAD. */
+void Xg266FullSubPelMe(xg_me_struct *meSet);
+
+/* ------------------------------------------------------------------------
+ * 2. Batched entry points (device pointers)
+ * ---------------------------------------------------------------------- */
+
+/* One plane: `data` points at sample (0,0) of a buffer padded by at least
+ * `pad` samples on every side (edge-replicated by the caller). */
+typedef struct xg_plane {
+    xg_pixel *data;
+    intptr_t  stride;
+    int32_t   width, height, pad;
+} xg_plane;
+
+/* 4:2:0 picture: plane[0] luma, plane[1] Cb, plane[2] Cr. */
+typedef struct xg_picture { xg_plane plane[3]; } xg_picture;
+
+/* One coding block: luma position/size and its motion (1/16 pel). */
+typedef struct xg_block {
+    int16_t x, y;          /* luma position                                   */
+    uint8_t w, h;          /* 4..64, powers of two                            */
+    uint8_t pic, ref;      /* picture-set slot of the current / reference pic */
+    int32_t mvx, mvy;      /* base MV of the candidate pattern, 1/16 pel      */
+    int32_t mvpx, mvpy;    /* MV predictor for the rate term, 1/16 pel        */
+} xg_block;
+
+/* Fractional-refinement configuration (per call). */
+#define XG_MAX_CAND 64
+typedef struct xg_refine_cfg {
+    int32_t  ncand;                  /* candidates per block, <= XG_MAX_CAND    */
+    int16_t  offx[XG_MAX_CAND];      /* candidate offsets around the base MV, */
+    int16_t  offy[XG_MAX_CAND];      /*   1/16 pel, |off| <= 64               */
+    uint32_t lambda;                 /* sadLambda (h266_enc_init.c:86-105)    */
+    int32_t  ctuSize;                /* MV clip window (h266_enc_init.c:1462) */
+    int32_t  useChroma;              /* add Cb+Cr SATD into each cost          */
+} xg_refine_cfg;
+
+/* Result of one block. */
+typedef struct xg_block_result {
+    int32_t  mvx, mvy;     /* best MV after clipping, 1/16 pel */
+    uint32_t cost;         /* (SATD << 8) + bits * lambda      */
+    int32_t  cand;         /* index of the best candidate      */
+} xg_block_result;
+
+typedef enum xg_mode {
+    XG_MODE_PLAIN = 0,     /* one kernel launch per block shape                  */
+    XG_MODE_BATCHED = 1,   /* one persistent kernel per stage over all shapes    */
+    XG_MODE_GRAPH = 2      /* the batched stages replayed from a captured graph  */
+} xg_mode;
+
+typedef struct xg_session xg_session;
+
+/* A session owns a HIP stream, the per-shape work queues and the result
+ * buffers for up to maxBlocks blocks.  Pictures are passed per call through a
+ * picture set of up to 16 slots (xg_block.pic / .ref index it). */
+int32_t xg_session_create(xg_session **out, int32_t maxBlocks);
+int32_t xg_session_destroy(xg_session *s);
+/* HIP stream of the session (hipStream_t) so callers can order their work. */
+void   *xg_session_stream(xg_session *s);
+/* Installs a picture in slot `slot` (0..15).  Device pointers. */
+int32_t xg_session_set_picture(xg_session *s, int32_t slot, const xg_picture *pic);
+/* Uploads the block list (host memory) and builds per-shape work queues:
+ * tasks sorted by shape, largest first, each covering `candPerTask`
+ * candidates of one block (per-shape tile table, see xg_session_set_tile). */
+int32_t xg_session_set_blocks(xg_session *s, const xg_block *blocks, int32_t n);
+/* Per-shape tile choice: candidates per task for shape (lgW-2, lgH-2). */
+int32_t xg_session_set_tile(xg_session *s, int32_t lgW, int32_t lgH, int32_t candPerTask);
+int32_t xg_session_get_tile(xg_session *s, int32_t lgW, int32_t lgH);
+/* Times every legal tile per shape on the installed blocks/pictures and keeps
+ * the fastest; writes the table to `path` when not NULL. */
+int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t reps, const char *path);
+int32_t xg_session_load_tiles(xg_session *s, const char *path);
+
+/* Fractional refinement (per block: ncand candidates, luma 8-tap + chroma
+ * 4-tap interpolation, SATD, cost) followed by motion compensation of the best
+ * candidate into `pred` (device planes, unpadded sizes of the current
+ * picture).  Enqueued on the session stream; returns immediately. */
+int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture *pred,
+                       int32_t mode);
+/* Device pointers of the last results: candCost[block*ncand + c], best[block]. */
+int32_t xg_session_outputs(xg_session *s, uint32_t **candCost, xg_block_result **best);
+int32_t xg_session_sync(xg_session *s);
+/* Waits for the session stream and copies the last results to host memory
+ * (either pointer may be NULL): candCost[nblocks*ncand], best[nblocks]. */
+int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best);
+/* Average device time (ms) of the refinement-stage kernel over the launches
+ * since the last reset, measured with HIP events on the session stream. */
+int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
+int32_t xg_session_reset_timers(xg_session *s);
+int32_t xg_session_set_timing(xg_session *s, int32_t enable);
+
+/* Integer full search for a grid of bw x bh blocks over a whole frame
+ * (device planes; ref padded by >= range + 4).  Cost and order as
+ * Xg26xMotionEstFull with predictor (0,0) and the CU MV window of
+ * h266_enc_init.c:1462-1465.  Outputs per block (raster order): best MV in
+ * quarter pel (int16 x,y pairs), cost and SAD. */
+int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane *ref,
+                             int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda,
+                             int32_t ctuSize, int16_t *bestMv, uint32_t *bestCost,
+                             uint32_t *bestSad);
+
+/* Batched forward transform of residual blocks on int8 MFMA.
+ * xg_fdct_plan takes HOST arrays: block i's residual is at resid + offset[i]
+ * (int16, row-major, stride = width) and its coefficients go to
+ * coeff + offset[i] (int32, row-major [ver freq][hor freq], stride = width);
+ * desc[i] = lgW | lgH << 4 | mtsIdx << 8 (sides 2..64; mtsIdx 0 or 2..5 as
+ * h26x_trans_context.h:42-51, DST-VII/DCT-VIII only for sides 4..32).  The plan
+ * groups blocks by shape and stays on the device until replaced.
+ * xg_fdct_run enqueues the transform of every planned block on the session
+ * stream (device pointers).  Shifts for 10-bit samples as Xg266FDctWxH;
+ * coefficients past the VVC zero-out are written as 0. */
+int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t nblocks);
+int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

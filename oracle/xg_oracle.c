@@ -1,0 +1,933 @@
+/*
+ * xg_oracle.c -- CPU oracle (checker) for the VVC block-matching hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  The product library never uses it.
+ *
+ * Citations are file:line in the xin26x source tree.  Parity status per part:
+ *   interpolation (luma/chroma, S16 variants, bi-avg/weight): pinned against
+ *       golden vectors written by the reference C itself (tests/golden/);
+ *   SAD / SAD-xN: closed in the reference; closed form, pinned by brute force
+ *       and invariants;
+ *   SATD / Hadamard 8x8: closed in the reference; HM's published Hadamard
+ *       method -- parity unpinned vs the reference;
+ *   Exp-Golomb MV bits: closed table; textbook se(v) length -- parity unpinned;
+ *   integer search order / ties, 1/16-pel pass: closed or new; build-defined;
+ *   forward transforms: closed; H.266 matrices + VTM rounding -- parity
+ *       unpinned vs the reference, pinned by closed forms and float DCT/DST.
+ */
+#include "xg_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#define BITDEPTH       10
+#define PIX_MAX        1023
+/* h266_core/include/h266_inter_pred_context.h:31-39 */
+#define IF_SHIFT       6
+#define IF_PREC        14
+#define IF_PREC_OFS    (1 << (IF_PREC - 1))            /* 8192 */
+#define IF_HEADROOM    ((IF_PREC - BITDEPTH) > 2 ? (IF_PREC - BITDEPTH) : 2) /* 4 */
+
+static inline int32_t clip_pix(int32_t v) { return v < 0 ? 0 : (v > PIX_MAX ? PIX_MAX : v); }
+
+/* ---------------------------------------------------------------------------
+ * Filter tables.  h266_core/source/h266_luma_mc.c:27-70 (VVC luma 1/16-pel
+ * 8-tap, 4x4-affine variant, alternative half-pel) and
+ * h266_core/source/h266_chroma_mc.c:26-60 (VVC chroma 1/32-pel 4-tap).  These
+ * are normative H.266 tables (Tables 27-29), so their values must be equal.
+ * ------------------------------------------------------------------------- */
+static const int16_t kLuma[16][8] = {
+    { 0, 0,   0, 64,  0,   0, 0,  0}, { 0, 1,  -3, 63,  4,  -2, 1,  0},
+    {-1, 2,  -5, 62,  8,  -3, 1,  0}, {-1, 3,  -8, 60, 13,  -4, 1,  0},
+    {-1, 4, -10, 58, 17,  -5, 1,  0}, {-1, 4, -11, 52, 26,  -8, 3, -1},
+    {-1, 3,  -9, 47, 31, -10, 4, -1}, {-1, 4, -11, 45, 34, -10, 4, -1},
+    {-1, 4, -11, 40, 40, -11, 4, -1}, {-1, 4, -10, 34, 45, -11, 4, -1},
+    {-1, 4, -10, 31, 47,  -9, 3, -1}, {-1, 3,  -8, 26, 52, -11, 4, -1},
+    { 0, 1,  -5, 17, 58, -10, 4, -1}, { 0, 1,  -4, 13, 60,  -8, 3, -1},
+    { 0, 1,  -3,  8, 62,  -5, 2, -1}, { 0, 1,  -2,  4, 63,  -3, 1,  0}};
+static const int16_t kLuma4x4[16][8] = {
+    {0, 0,   0, 64,  0,   0, 0, 0}, {0, 1,  -3, 63,  4,  -2, 1, 0},
+    {0, 1,  -5, 62,  8,  -3, 1, 0}, {0, 2,  -8, 60, 13,  -4, 1, 0},
+    {0, 3, -10, 58, 17,  -5, 1, 0}, {0, 3, -11, 52, 26,  -8, 2, 0},
+    {0, 2,  -9, 47, 31, -10, 3, 0}, {0, 3, -11, 45, 34, -10, 3, 0},
+    {0, 3, -11, 40, 40, -11, 3, 0}, {0, 3, -10, 34, 45, -11, 3, 0},
+    {0, 3, -10, 31, 47,  -9, 2, 0}, {0, 2,  -8, 26, 52, -11, 3, 0},
+    {0, 1,  -5, 17, 58, -10, 3, 0}, {0, 1,  -4, 13, 60,  -8, 2, 0},
+    {0, 1,  -3,  8, 62,  -5, 1, 0}, {0, 1,  -2,  4, 63,  -3, 1, 0}};
+static const int16_t kLumaAltHalf[8] = {0, 3, 9, 20, 20, 9, 3, 0};
+static const int16_t kChroma[32][4] = {
+    { 0, 64,  0,  0}, {-1, 63,  2,  0}, {-2, 62,  4,  0}, {-2, 60,  7, -1},
+    {-2, 58, 10, -2}, {-3, 57, 12, -2}, {-4, 56, 14, -2}, {-4, 55, 15, -2},
+    {-4, 54, 16, -2}, {-5, 53, 18, -2}, {-6, 52, 20, -2}, {-6, 49, 24, -3},
+    {-6, 46, 28, -4}, {-5, 44, 29, -4}, {-4, 42, 30, -4}, {-4, 39, 33, -4},
+    {-4, 36, 36, -4}, {-4, 33, 39, -4}, {-4, 30, 42, -4}, {-4, 29, 44, -5},
+    {-4, 28, 46, -6}, {-3, 24, 49, -6}, {-2, 20, 52, -6}, {-2, 18, 53, -5},
+    {-2, 16, 54, -4}, {-2, 15, 55, -4}, {-2, 14, 56, -4}, {-2, 12, 57, -3},
+    {-2, 10, 58, -2}, {-1,  7, 60, -2}, { 0,  4, 62, -2}, { 0,  2, 63, -1}};
+
+/* Coefficient choice: h266_luma_mc.c:117-128 (same rule in every variant). */
+static const int16_t *luma_taps(int32_t phase, uint32_t fi)
+{
+    if (fi == 1 && phase == 8) return kLumaAltHalf;
+    if (fi == 2) return kLuma4x4[phase];
+    return kLuma[phase];
+}
+
+/* Generic FIR over `taps` samples spaced by `step`, starting at p. */
+static inline int32_t fir_u16(const or_pixel *p, intptr_t step, const int16_t *c, int taps)
+{
+    int32_t s = 0;
+    for (int k = 0; k < taps; k++) s += (int32_t)p[k * step] * c[k];
+    return s;
+}
+static inline int32_t fir_s16(const int16_t *p, intptr_t step, const int16_t *c, int taps)
+{
+    int32_t s = 0;
+    for (int k = 0; k < taps; k++) s += (int32_t)p[k * step] * c[k];
+    return s;
+}
+
+/* ---- PIXEL -> PIXEL single-direction passes ------------------------------
+ * h266_luma_mc.c:100-155 (Hor) and :157-214 (Vet):
+ *   clip((sum + 32) >> 6).  `step` selects the direction. */
+static void pass_u16_to_pix(const or_pixel *src, intptr_t ss, intptr_t step, or_pixel *dst,
+                            intptr_t ds, const int16_t *c, int taps, uint32_t w, uint32_t h)
+{
+    src -= (taps / 2 - 1) * step;
+    for (uint32_t y = 0; y < h; y++, src += ss, dst += ds)
+        for (uint32_t x = 0; x < w; x++)
+            dst[x] = (or_pixel)clip_pix((fir_u16(src + x, step, c, taps) + (1 << (IF_SHIFT - 1))) >> IF_SHIFT);
+}
+
+/* ---- PIXEL -> S16 first pass ---------------------------------------------
+ * h266_luma_mc.c:216-273 (HorU8S16) and :465-523 (VetU8S16):
+ *   (sum - (8192 << hs)) >> hs, hs = 6 - max(2, 14 - bitDepth), stored as int16. */
+static void pass_u16_to_s16(const or_pixel *src, intptr_t ss, intptr_t step, int16_t *dst,
+                            intptr_t ds, const int16_t *c, int taps, uint32_t w, uint32_t h)
+{
+    const int32_t hs = IF_SHIFT - IF_HEADROOM;
+    const int32_t ofs = -IF_PREC_OFS * (1 << hs);
+    src -= (taps / 2 - 1) * step;
+    for (uint32_t y = 0; y < h; y++, src += ss, dst += ds)
+        for (uint32_t x = 0; x < w; x++)
+            dst[x] = (int16_t)((fir_u16(src + x, step, c, taps) + ofs) >> hs);
+}
+
+/* ---- S16 -> PIXEL vertical second pass -----------------------------------
+ * h266_luma_mc.c:275-336 (VetS16U8):
+ *   clip((sum + (1 << (vs-1)) + (8192 << 6)) >> vs), vs = 6 + max(2, 14-bd). */
+static void pass_s16_to_pix(const int16_t *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                            const int16_t *c, int taps, uint32_t w, uint32_t h)
+{
+    const int32_t vs = IF_SHIFT + IF_HEADROOM;
+    const int32_t ofs = (1 << (vs - 1)) + (IF_PREC_OFS << IF_SHIFT);
+    src -= (taps / 2 - 1) * ss;
+    for (uint32_t y = 0; y < h; y++, src += ss, dst += ds)
+        for (uint32_t x = 0; x < w; x++)
+            dst[x] = (or_pixel)clip_pix((fir_s16(src + x, ss, c, taps) + ofs) >> vs);
+}
+
+/* ---- S16 -> S16 vertical second pass -------------------------------------
+ * h266_luma_mc.c:338-392 (VetS16S16): sum >> 6, no rounding offset. */
+static void pass_s16_to_s16(const int16_t *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                            const int16_t *c, int taps, uint32_t w, uint32_t h)
+{
+    src -= (taps / 2 - 1) * ss;
+    for (uint32_t y = 0; y < h; y++, src += ss, dst += ds)
+        for (uint32_t x = 0; x < w; x++)
+            dst[x] = (int16_t)(fir_s16(src + x, ss, c, taps) >> IF_SHIFT);
+}
+
+/* h266_luma_mc.c:72-98 */
+void or_InterpCopy(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                   int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)frac; (void)fi;
+    for (uint32_t y = 0; y < h; y++, src += ss, dst += ds) memcpy(dst, src, w * sizeof(or_pixel));
+}
+
+/* h266_luma_mc.c:100-155: horizontal phase = frac & 15 */
+void or_LumaInterpHor(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                      int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    pass_u16_to_pix(src, ss, 1, dst, ds, luma_taps(frac & 15, fi), 8, w, h);
+}
+
+/* h266_luma_mc.c:157-214: vertical phase = frac >> 4 */
+void or_LumaInterpVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                      int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    pass_u16_to_pix(src, ss, ss, dst, ds, luma_taps(frac >> 4, fi), 8, w, h);
+}
+
+/* h266_luma_mc.c:216-273 */
+void or_LumaInterpHorU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    pass_u16_to_s16(src, ss, 1, dst, ds, luma_taps(frac & 15, fi), 8, w, h);
+}
+
+/* h266_luma_mc.c:465-523 */
+void or_LumaInterpVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    pass_u16_to_s16(src, ss, ss, dst, ds, luma_taps(frac >> 4, fi), 8, w, h);
+}
+
+/* h266_luma_mc.c:275-336 */
+void or_LumaInterpVetS16U8(const int16_t *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    pass_s16_to_pix(src, ss, dst, ds, luma_taps(frac >> 4, fi), 8, w, h);
+}
+
+/* h266_luma_mc.c:338-392 */
+void or_LumaInterpVetS16S16(const int16_t *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                            int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    pass_s16_to_s16(src, ss, dst, ds, luma_taps(frac >> 4, fi), 8, w, h);
+}
+
+/* h266_luma_mc.c:394-428: first pass over h+7 rows from 3 rows above into an
+ * int16 buffer of row stride w, then the S16->PIXEL vertical pass. */
+void or_LumaInterpHorVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                         int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    int16_t *tmp = (int16_t *)malloc(sizeof(int16_t) * w * (h + 7));
+    pass_u16_to_s16(src - 3 * ss, ss, 1, tmp, w, luma_taps(frac & 15, fi), 8, w, h + 7);
+    pass_s16_to_pix(tmp + 3 * w, w, dst, ds, luma_taps(frac >> 4, fi), 8, w, h);
+    free(tmp);
+}
+
+/* h266_luma_mc.c:525-559 */
+void or_LumaInterpHorVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                              int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    int16_t *tmp = (int16_t *)malloc(sizeof(int16_t) * w * (h + 7));
+    pass_u16_to_s16(src - 3 * ss, ss, 1, tmp, w, luma_taps(frac & 15, fi), 8, w, h + 7);
+    pass_s16_to_s16(tmp + 3 * w, w, dst, ds, luma_taps(frac >> 4, fi), 8, w, h);
+    free(tmp);
+}
+
+/* h266_luma_mc.c:430-463: (s << 4) - 8192 as int16 */
+void or_InterpCopyU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                        int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)frac; (void)fi;
+    for (uint32_t y = 0; y < h; y++, src += ss, dst += ds)
+        for (uint32_t x = 0; x < w; x++)
+            dst[x] = (int16_t)((int16_t)(src[x] << IF_HEADROOM) - IF_PREC_OFS);
+}
+
+/* h266_luma_mc.c:561-593: shift = max(2, 14-bd) + 1, offset = (1<<(shift-1)) + 2*8192 */
+void or_InterpAvgS16U8(const int16_t *s0, intptr_t st0, const int16_t *s1, intptr_t st1,
+                       or_pixel *dst, intptr_t ds, uint32_t w, uint32_t h)
+{
+    const int32_t sh = IF_HEADROOM + 1;
+    const int32_t ofs = (1 << (sh - 1)) + 2 * IF_PREC_OFS;
+    for (uint32_t y = 0; y < h; y++, s0 += st0, s1 += st1, dst += ds)
+        for (uint32_t x = 0; x < w; x++)
+            dst[x] = (or_pixel)clip_pix(((int32_t)s0[x] + s1[x] + ofs) >> sh);
+}
+
+/* h266_luma_mc.c:595-629 with XIN_BCW_LOG_WGT_BASE = 3 (h266_core/include/h266_constant.h:24) */
+void or_InterpWeightS16U8(const int16_t *s0, intptr_t st0, const int16_t *s1, intptr_t st1,
+                          or_pixel *dst, intptr_t ds, uint32_t w, uint32_t h,
+                          int32_t wA, int32_t wB)
+{
+    const int32_t sh = IF_HEADROOM + 3;
+    const int32_t ofs = (1 << (sh - 1)) + (IF_PREC_OFS << 3);
+    for (uint32_t y = 0; y < h; y++, s0 += st0, s1 += st1, dst += ds)
+        for (uint32_t x = 0; x < w; x++)
+            dst[x] = (or_pixel)clip_pix(((int32_t)s0[x] * wA + (int32_t)s1[x] * wB + ofs) >> sh);
+}
+
+/* ---- chroma: h266_chroma_mc.c, same structure with 4 taps and 32 phases ---- */
+/* :62-100 */
+void or_ChromaInterpHor(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                        int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    pass_u16_to_pix(src, ss, 1, dst, ds, kChroma[frac & 31], 4, w, h);
+}
+/* :102-140 */
+void or_ChromaInterpVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                        int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    pass_u16_to_pix(src, ss, ss, dst, ds, kChroma[frac >> 5], 4, w, h);
+}
+/* :142-183 */
+void or_ChromaInterpHorU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    pass_u16_to_s16(src, ss, 1, dst, ds, kChroma[frac & 31], 4, w, h);
+}
+/* :305-346 */
+void or_ChromaInterpVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    pass_u16_to_s16(src, ss, ss, dst, ds, kChroma[frac >> 5], 4, w, h);
+}
+/* :185-228 */
+void or_ChromaInterpVetS16U8(const int16_t *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    pass_s16_to_pix(src, ss, dst, ds, kChroma[frac >> 5], 4, w, h);
+}
+/* :230-267 */
+void or_ChromaInterpVetS16S16(const int16_t *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                              int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    pass_s16_to_s16(src, ss, dst, ds, kChroma[frac >> 5], 4, w, h);
+}
+/* :269-303: first pass over h+3 rows from 1 row above */
+void or_ChromaInterpHorVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    int16_t *tmp = (int16_t *)malloc(sizeof(int16_t) * w * (h + 3));
+    pass_u16_to_s16(src - ss, ss, 1, tmp, w, kChroma[frac & 31], 4, w, h + 3);
+    pass_s16_to_pix(tmp + w, w, dst, ds, kChroma[frac >> 5], 4, w, h);
+    free(tmp);
+}
+/* :348-384 */
+void or_ChromaInterpHorVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                                int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    int16_t *tmp = (int16_t *)malloc(sizeof(int16_t) * w * (h + 3));
+    pass_u16_to_s16(src - ss, ss, 1, tmp, w, kChroma[frac & 31], 4, w, h + 3);
+    pass_s16_to_s16(tmp + w, w, dst, ds, kChroma[frac >> 5], 4, w, h);
+    free(tmp);
+}
+
+/* ---- dispatch as the encoder does it ---- */
+/* h266_inter_prediction.c:143-172 with the table of h266_func_init.c:424-458:
+ * [0][0] Copy, [1][0] Hor, [0][1] Vet, [1][1] HorVet. */
+void or_LumaInterpolation(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                          int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    const int fx = (frac & 15) != 0, fy = ((frac >> 4) & 15) != 0;
+    if (fx && fy) or_LumaInterpHorVet(src, ss, dst, ds, frac, w, h, fi);
+    else if (fx)  or_LumaInterpHor(src, ss, dst, ds, frac, w, h, fi);
+    else if (fy)  or_LumaInterpVet(src, ss, dst, ds, frac, w, h, fi);
+    else          or_InterpCopy(src, ss, dst, ds, frac, w, h, fi);
+}
+
+/* h266_inter_prediction.c:78-141 */
+void or_LumaInterpolationS16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi)
+{
+    if (!frac)              or_InterpCopyU8S16(src, ss, dst, ds, frac, w, h, fi);
+    else if (!(frac & 15))  or_LumaInterpVetU8S16(src, ss, dst, ds, frac, w, h, fi);
+    else if (!(frac & 240)) or_LumaInterpHorU8S16(src, ss, dst, ds, frac, w, h, fi);
+    else                    or_LumaInterpHorVetU8S16(src, ss, dst, ds, frac, w, h, fi);
+}
+
+/* h266_inter_prediction.c:174-236 */
+void or_ChromaInterpolation(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                            int32_t frac, uint32_t w, uint32_t h)
+{
+    if (!frac)                  or_InterpCopy(src, ss, dst, ds, frac, w, h, 0);
+    else if (!(frac & (31 << 5))) or_ChromaInterpHor(src, ss, dst, ds, frac, w, h, 0);
+    else if (!(frac & 31))      or_ChromaInterpVet(src, ss, dst, ds, frac, w, h, 0);
+    else                        or_ChromaInterpHorVet(src, ss, dst, ds, frac, w, h, 0);
+}
+
+/* h266_inter_prediction.c:238-300 */
+void or_ChromaInterpolationS16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                               int32_t frac, uint32_t w, uint32_t h)
+{
+    if (!frac)                  or_InterpCopyU8S16(src, ss, dst, ds, frac, w, h, 0);
+    else if (!(frac & (31 << 5))) or_ChromaInterpHorU8S16(src, ss, dst, ds, frac, w, h, 0);
+    else if (!(frac & 31))      or_ChromaInterpVetU8S16(src, ss, dst, ds, frac, w, h, 0);
+    else                        or_ChromaInterpHorVetU8S16(src, ss, dst, ds, frac, w, h, 0);
+}
+
+/* ---------------------------------------------------------------------------
+ * Distortion.  The reference binds these to closed functions
+ * (h26x_core/include/h26x_compute_dist.h:27-150, 400-735, 806-937); the
+ * signature follows XinComputeDist (h266_encoder/include/h266_func_struct.h:56-63).
+ * ------------------------------------------------------------------------- */
+uint32_t or_ComputeSad(const or_pixel *in, intptr_t is, const or_pixel *pr, intptr_t ps,
+                       uint32_t w, uint32_t h)
+{
+    uint32_t s = 0;
+    for (uint32_t y = 0; y < h; y++, in += is, pr += ps)
+        for (uint32_t x = 0; x < w; x++) s += (uint32_t)abs((int)in[x] - (int)pr[x]);
+    return s;
+}
+
+/* XinComputeSadx8 (h266_func_struct.h:87-94): one input against 8 positions. */
+void or_ComputeSadx8(const or_pixel *in, intptr_t is, const or_pixel *const ref[8], intptr_t rs,
+                     uint32_t w, uint32_t h, uint32_t sad[8])
+{
+    for (int k = 0; k < 8; k++) sad[k] = or_ComputeSad(in, is, ref[k], rs, w, h);
+}
+
+/* XinComputeSadx5 (h266_func_struct.h:106-117) */
+void or_ComputeSadx5(const or_pixel *in, intptr_t is, const or_pixel *r0, const or_pixel *r1,
+                     const or_pixel *r2, const or_pixel *r3, const or_pixel *r4, intptr_t rs,
+                     uint32_t w, uint32_t h, uint32_t sad[5])
+{
+    const or_pixel *r[5] = {r0, r1, r2, r3, r4};
+    for (int k = 0; k < 5; k++) sad[k] = or_ComputeSad(in, is, r[k], rs, w, h);
+}
+
+/* XinComputeSadx3 (h266_func_struct.h:119-128) */
+void or_ComputeSadx3(const or_pixel *in, intptr_t is, const or_pixel *r0, const or_pixel *r1,
+                     const or_pixel *r2, intptr_t rs, uint32_t w, uint32_t h, uint32_t sad[3])
+{
+    const or_pixel *r[3] = {r0, r1, r2};
+    for (int k = 0; k < 3; k++) sad[k] = or_ComputeSad(in, is, r[k], rs, w, h);
+}
+
+/* In-place 1-D Walsh-Hadamard transform of n (power of two) values spaced by step. */
+static void wht_1d(int32_t *v, int n, int step)
+{
+    for (int len = 1; len < n; len <<= 1)
+        for (int i = 0; i < n; i += len << 1)
+            for (int j = i; j < i + len; j++) {
+                int32_t a = v[j * step], b = v[(j + len) * step];
+                v[j * step] = a + b;
+                v[(j + len) * step] = a - b;
+            }
+}
+
+/* Sum of |2-D WHT coefficients| of an n x n difference tile. */
+static uint32_t hadamard_tile(const or_pixel *in, intptr_t is, const or_pixel *pr, intptr_t ps, int n)
+{
+    int32_t d[64];
+    for (int y = 0; y < n; y++)
+        for (int x = 0; x < n; x++) d[y * n + x] = (int32_t)in[y * is + x] - (int32_t)pr[y * ps + x];
+    for (int y = 0; y < n; y++) wht_1d(d + y * n, n, 1);
+    for (int x = 0; x < n; x++) wht_1d(d + x, n, n);
+    uint32_t s = 0;
+    for (int i = 0; i < n * n; i++) s += (uint32_t)abs(d[i]);
+    return s;
+}
+
+/* SATD: HM's published method (TComRdCost::xCalcHADs8x8/4x4/2x2 and getHADs):
+ * 8x8 tiles when both sides are multiples of 8, else 4x4, else 2x2; per-tile
+ * normalisation (s+2)>>2 (8x8), (s+1)>>1 (4x4), none (2x2).  The coefficient
+ * order of the transform does not matter because only |coef| is summed.
+ * Parity unpinned vs the reference's closed Xin26xComputeSatd. */
+uint32_t or_ComputeSatd(const or_pixel *in, intptr_t is, const or_pixel *pr, intptr_t ps,
+                        uint32_t w, uint32_t h)
+{
+    uint32_t s = 0;
+    if (w % 8 == 0 && h % 8 == 0) {
+        for (uint32_t y = 0; y < h; y += 8)
+            for (uint32_t x = 0; x < w; x += 8)
+                s += (hadamard_tile(in + y * is + x, is, pr + y * ps + x, ps, 8) + 2) >> 2;
+    } else if (w % 4 == 0 && h % 4 == 0) {
+        for (uint32_t y = 0; y < h; y += 4)
+            for (uint32_t x = 0; x < w; x += 4)
+                s += (hadamard_tile(in + y * is + x, is, pr + y * ps + x, ps, 4) + 1) >> 1;
+    } else {
+        for (uint32_t y = 0; y < h; y += 2)
+            for (uint32_t x = 0; x < w; x += 2)
+                s += hadamard_tile(in + y * is + x, is, pr + y * ps + x, ps, 2);
+    }
+    return s;
+}
+
+/* XinComputeHada8x8 (h266_func_struct.h:448-451): Hadamard magnitude of one
+ * 8x8 block of samples, defined here as the 8x8 SATD against a zero block.
+ * Parity unpinned vs the closed Xin26xComputeHada8x8. */
+uint32_t or_ComputeHada8x8(const or_pixel *in, intptr_t is)
+{
+    static const or_pixel zero[8] = {0};
+    return (hadamard_tile(in, is, zero, 0, 8) + 2) >> 2;
+}
+
+/* ---- rate terms ----------------------------------------------------------
+ * CALC_MVD_BIT (common/include/h26x_definition.h:176) indexes the closed
+ * ExpGolombBits table.  Textbook signed Exp-Golomb length se(v):
+ * k = v > 0 ? 2v-1 : -2v, length = 2*floor(log2(k+1)) + 1.  Parity unpinned. */
+uint32_t or_ExpGolombBits(int32_t v)
+{
+    uint32_t k = v > 0 ? (uint32_t)(2 * v - 1) : (uint32_t)(-2 * v);
+    uint32_t n = k + 1, lg = 0;
+    while (n >>= 1) lg++;
+    return 2 * lg + 1;
+}
+
+uint32_t or_MvdBits(int32_t dx, int32_t dy) { return or_ExpGolombBits(dx) + or_ExpGolombBits(dy); }
+
+/* sadLambda table (h266_encoder/source/h266_enc_init.c:86-105) indexed by
+ * qp + 12*is10Bit (:384).  Every entry is round(64 * 2^(i/6)). */
+uint32_t or_SadLambda(int32_t qp, int32_t is10bit)
+{
+    static const uint32_t tab[64] = {
+        64, 72, 81, 91, 102, 114, 128, 144, 161, 181, 203, 228, 256, 287, 323, 362,
+        406, 456, 512, 575, 645, 724, 813, 912, 1024, 1149, 1290, 1448, 1625, 1825, 2048, 2299,
+        2580, 2896, 3251, 3649, 4096, 4598, 5161, 5793, 6502, 7298, 8192, 9195, 10321, 11585, 13004, 14596,
+        16384, 18390, 20643, 23170, 26008, 29193, 32768, 36781, 41285, 46341, 52016, 58386, 65536, 73562, 82570, 92682};
+    int32_t i = qp + 12 * (is10bit ? 1 : 0);
+    if (i < 0) i = 0;
+    if (i > 63) i = 63;
+    return tab[i];
+}
+
+/* h266_get_neighbour_mv.c:245-290 (shift < 0 branch): rounding right shift
+ * v >= 0 ? (v + ofs - 1) >> s : (v + ofs) >> s, ofs = 1 << (s-1). */
+int32_t or_MvRoundShift(int32_t v, int32_t s)
+{
+    if (s <= 0) return v * (1 << (-s));
+    int32_t ofs = 1 << (s - 1);
+    return v >= 0 ? (v + ofs - 1) >> s : (v + ofs) >> s;
+}
+
+/* ---- motion search -------------------------------------------------------- */
+static inline uint32_t me_cost(uint32_t sad, int32_t mvq_x, int32_t mvq_y, const or_me *me)
+{
+    /* cost = (SAD << XIN_COST_FRACTION) + bits(mvd) * sadLambda,
+     * h266_sub_pel_refine.c:155-156, XIN_COST_FRACTION = 8 (h26x_definition.h:88) */
+    return (sad << 8) + or_MvdBits(mvq_x - me->predMvX, mvq_y - me->predMvY) * me->sadLambda;
+}
+
+/* Full search (closed Xin26xMotionEstFull, h26x_core/include/h26x_motion_est.h:26).
+ * Build-defined: window centred on the predictor rounded to integer pel the way
+ * Xin266FillMeCand rounds (h266_analyze_inter_cu.c:271-272), +/- searchWidth x
+ * searchHeight intersected with [minMv, maxMv]; raster scan, dy outer and dx
+ * inner ascending; strict < keeps the earliest.  Parity unpinned. */
+void or_MotionEstFull(or_me *me)
+{
+    const int32_t cx = (me->predMvX + 2) >> 2, cy = (me->predMvY + 2) >> 2;
+    int32_t x0 = cx - me->searchWidth, x1 = cx + me->searchWidth;
+    int32_t y0 = cy - me->searchHeight, y1 = cy + me->searchHeight;
+    if (x0 < me->minMvX) x0 = me->minMvX;
+    if (x1 > me->maxMvX) x1 = me->maxMvX;
+    if (y0 < me->minMvY) y0 = me->minMvY;
+    if (y1 > me->maxMvY) y1 = me->maxMvY;
+    uint32_t best = UINT32_MAX, bestSad = UINT32_MAX;
+    int32_t bx = me->bestMvX >> 2, by = me->bestMvY >> 2;
+    for (int32_t my = y0; my <= y1; my++)
+        for (int32_t mx = x0; mx <= x1; mx++) {
+            uint32_t sad = or_ComputeSad(me->input, me->inputStride,
+                                         me->ref + my * me->refStride + mx, me->refStride,
+                                         me->width, me->height);
+            uint32_t c = me_cost(sad, mx * 4, my * 4, me);
+            if (c < best) { best = c; bestSad = sad; bx = mx; by = my; }
+        }
+    me->bestMvX = (int16_t)(bx * 4);
+    me->bestMvY = (int16_t)(by * 4);
+    me->bestCost = best;
+    me->bestSad = bestSad;
+}
+
+/* Diamond search (closed Xin26xMotionEstDia, h26x_motion_est.h:35).
+ * Build-defined: evaluate meCand[0..candNum) in list order (each clipped to
+ * [minMv, maxMv]), then repeat a small-diamond step around the best point with
+ * the neighbour order L, R, T, B of mvOffset (h266_sub_pel_refine.c:57-60),
+ * keeping points inside [minMv, maxMv] and inside the search window around the
+ * rounded predictor, until no neighbour is strictly better or
+ * searchWidth + searchHeight steps were taken.  Parity unpinned. */
+void or_MotionEstDia(or_me *me)
+{
+    static const int8_t dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};
+    const int32_t cx = (me->predMvX + 2) >> 2, cy = (me->predMvY + 2) >> 2;
+    uint32_t best = UINT32_MAX, bestSad = UINT32_MAX;
+    int32_t bx = 0, by = 0;
+    for (uint32_t i = 0; i < me->candNum; i++) {
+        int32_t mx = me->candX[i], my = me->candY[i];
+        mx = mx < me->minMvX ? me->minMvX : (mx > me->maxMvX ? me->maxMvX : mx);
+        my = my < me->minMvY ? me->minMvY : (my > me->maxMvY ? me->maxMvY : my);
+        uint32_t sad = or_ComputeSad(me->input, me->inputStride,
+                                     me->ref + my * me->refStride + mx, me->refStride,
+                                     me->width, me->height);
+        uint32_t c = me_cost(sad, mx * 4, my * 4, me);
+        if (c < best) { best = c; bestSad = sad; bx = mx; by = my; }
+    }
+    const int32_t steps = me->searchWidth + me->searchHeight;
+    for (int32_t it = 0; it < steps; it++) {
+        int32_t nx = bx, ny = by;
+        uint32_t nb = best, nsad = bestSad;
+        for (int k = 0; k < 4; k++) {
+            int32_t mx = bx + dxs[k], my = by + dys[k];
+            if (mx < me->minMvX || mx > me->maxMvX || my < me->minMvY || my > me->maxMvY) continue;
+            if (mx < cx - me->searchWidth || mx > cx + me->searchWidth ||
+                my < cy - me->searchHeight || my > cy + me->searchHeight) continue;
+            uint32_t sad = or_ComputeSad(me->input, me->inputStride,
+                                         me->ref + my * me->refStride + mx, me->refStride,
+                                         me->width, me->height);
+            uint32_t c = me_cost(sad, mx * 4, my * 4, me);
+            if (c < nb) { nb = c; nsad = sad; nx = mx; ny = my; }
+        }
+        if (nx == bx && ny == by) break;
+        bx = nx; by = ny; best = nb; bestSad = nsad;
+    }
+    me->bestMvX = (int16_t)(bx * 4);
+    me->bestMvY = (int16_t)(by * 4);
+    me->bestCost = best;
+    me->bestSad = bestSad;
+}
+
+/* h266_sub_pel_refine.c:47-55 neighbour order L, R, T, B, TL, TR, BL, BR. */
+static const int8_t kRingX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+static const int8_t kRingY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+
+/* SAD of the block predicted at a 1/16-pel MV relative to me->ref, through the
+ * luma interpolation table (h266_sub_pel_refine.c:134-153). */
+static uint32_t sad_at_mv16(const or_me *me, int32_t mv16x, int32_t mv16y, or_pixel *buf)
+{
+    int32_t frac = ((mv16y & 15) << 4) + (mv16x & 15);
+    or_LumaInterpolation(me->ref + me->refStride * (mv16y >> 4) + (mv16x >> 4), me->refStride,
+                         buf, 128, frac, me->width, me->height, 0);
+    return or_ComputeSad(me->input, me->inputStride, buf, 128, me->width, me->height);
+}
+
+/* h266_sub_pel_refine.c:81-212 (Xin266FullSubPelMe): half-pel ring (+/-2
+ * quarter-pel) then quarter-pel ring (+/-1) around the integer best, strict <.
+ * filterIndex is XIN_INTERP_DEF_FILTER (0): the reference call site passes 8
+ * arguments to a 9-parameter function, leaving filterIndex undefined. */
+void or_FullSubPelMe(or_me *me)
+{
+    or_pixel *buf = (or_pixel *)malloc(sizeof(or_pixel) * 128 * 128);
+    int32_t mx = me->bestMvX, my = me->bestMvY;
+    uint32_t best = me->bestCost;
+    if ((mx >> 2) < me->minMvX || (my >> 2) < me->minMvY || (mx >> 2) > me->maxMvX ||
+        (my >> 2) > me->maxMvY) {
+        free(buf);
+        return;
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        const int32_t step = pass == 0 ? 2 : 1;
+        const int32_t cx = mx, cy = my;
+        for (int k = 0; k < 8; k++) {
+            int32_t tx = cx + kRingX[k] * step, ty = cy + kRingY[k] * step;
+            uint32_t sad = sad_at_mv16(me, tx * 4, ty * 4, buf);
+            uint32_t c = me_cost(sad, tx, ty, me);
+            if (c < best) { best = c; mx = tx; my = ty; }
+        }
+    }
+    me->bestMvX = (int16_t)mx;
+    me->bestMvY = (int16_t)my;
+    me->bestCost = best;
+    free(buf);
+}
+
+/* 1/16-pel pass (build extension of the A7 pattern): re-cost the current
+ * quarter-pel best in 1/16-pel MVD units, then the +/-1 (1/16) ring in the
+ * order L, R, T, B, TL, TR, BL, BR, strict <.  bestCost/bestSad are updated. */
+void or_SixteenthPelMe(or_me *me, int16_t *bx16, int16_t *by16)
+{
+    or_pixel *buf = (or_pixel *)malloc(sizeof(or_pixel) * 128 * 128);
+    const int32_t px = me->predMvX * 4, py = me->predMvY * 4;
+    int32_t cx = me->bestMvX * 4, cy = me->bestMvY * 4;
+    uint32_t sad0 = sad_at_mv16(me, cx, cy, buf);
+    uint32_t best = (sad0 << 8) + or_MvdBits(cx - px, cy - py) * me->sadLambda, bestSad = sad0;
+    int32_t bx = cx, by = cy;
+    for (int k = 0; k < 8; k++) {
+        int32_t tx = cx + kRingX[k], ty = cy + kRingY[k];
+        uint32_t sad = sad_at_mv16(me, tx, ty, buf);
+        uint32_t c = (sad << 8) + or_MvdBits(tx - px, ty - py) * me->sadLambda;
+        if (c < best) { best = c; bestSad = sad; bx = tx; by = ty; }
+    }
+    *bx16 = (int16_t)bx;
+    *by16 = (int16_t)by;
+    me->bestCost = best;
+    me->bestSad = bestSad;
+    free(buf);
+}
+
+/* h266_analyze_inter_cu.c:133-293 (Xin266FillMeCand): meCand[0] = (0,0); the
+ * predictor and the extra quarter-pel candidates are rounded to integer pel by
+ * (v + 2) >> 2 (:271-272) and appended unless already present (:265-291). */
+void or_FillMeCand(int16_t predX, int16_t predY, const int16_t *qx, const int16_t *qy,
+                   uint32_t n, int16_t *outX, int16_t *outY, uint32_t *outNum)
+{
+    uint32_t cnt = 1;
+    outX[0] = 0;
+    outY[0] = 0;
+    for (uint32_t i = 0; i <= n && i < 11; i++) {
+        int32_t vx = i == 0 ? predX : qx[i - 1];
+        int32_t vy = i == 0 ? predY : qy[i - 1];
+        int16_t cx = (int16_t)((vx + 2) >> 2), cy = (int16_t)((vy + 2) >> 2);
+        int dup = 0;
+        for (uint32_t j = 0; j < cnt; j++)
+            if (outX[j] == cx && outY[j] == cy) { dup = 1; break; }
+        if (!dup) { outX[cnt] = cx; outY[cnt] = cy; cnt++; }
+    }
+    *outNum = cnt;
+}
+
+/* ---------------------------------------------------------------------------
+ * Forward transforms.  The reference's matrices and kernels are closed
+ * (h26x_core/include/h26x_trans_context.h:53-67, h26x_forward_1d_trans.h).
+ * Matrices: ITU-T H.266 clause 8.7.4.  DCT-II of every size takes rows of the
+ * 64-point matrix (T_N[k][n] = T_64[k * 64/N][n]); the 64-point matrix is built
+ * from the cosine-magnitude tables of each dyadic level with the DCT-II sign
+ * pattern.  DST-VII from its sine-magnitude table; DCT-VIII as the
+ * row-alternating, column-reversed DST-VII.  Parity unpinned vs the reference.
+ * ------------------------------------------------------------------------- */
+static const int32_t kCos64[32] = {91, 90, 90, 90, 88, 87, 86, 84, 83, 81, 79, 77, 73, 71, 69, 65,
+                                   62, 59, 56, 52, 48, 44, 41, 37, 33, 28, 24, 20, 15, 11, 7, 2};
+static const int32_t kCos32[16] = {90, 90, 88, 85, 82, 78, 73, 67, 61, 54, 46, 38, 31, 22, 13, 4};
+static const int32_t kCos16[8] = {90, 87, 80, 70, 57, 43, 25, 9};
+static const int32_t kCos8[4] = {89, 75, 50, 18};
+static const int32_t kCos4[2] = {83, 36};
+static const int32_t kCos2[1] = {64};
+static const int32_t kSin4[4] = {29, 55, 74, 84};
+static const int32_t kSin8[8] = {17, 32, 46, 60, 71, 78, 85, 86};
+static const int32_t kSin16[16] = {8, 17, 25, 33, 40, 48, 55, 62, 68, 73, 77, 81, 85, 87, 88, 88};
+static const int32_t kSin32[32] = {4, 9, 13, 17, 21, 26, 30, 34, 38, 42, 46, 50, 53, 56, 60, 63,
+                                   66, 68, 72, 74, 77, 78, 80, 82, 84, 85, 86, 87, 88, 89, 90, 90};
+
+/* Entry (k, n) of the 64-point DCT-II: 64 for k = 0; otherwise write
+ * k = 2^e * k' (k' odd).  The level size L = 64 / 2^e holds magnitudes of
+ * cos((2i+1) pi / (2L)), i < L/2; the angle index m = (2n+1) k' mod 4L picks
+ * the magnitude and the quadrant picks the sign. */
+static int32_t dct2_64(int k, int n)
+{
+    if (k == 0) return 64;
+    int kk = k, L = 64;
+    while (!(kk & 1)) { kk >>= 1; L >>= 1; }
+    const int32_t *mag = L == 64 ? kCos64 : L == 32 ? kCos32 : L == 16 ? kCos16
+                       : L == 8 ? kCos8 : L == 4 ? kCos4 : kCos2;
+    int m = ((2 * n + 1) * kk) % (4 * L);
+    int q = m / L;
+    switch (q) {
+    case 0: return mag[(m - 1) / 2];
+    case 1: return -mag[(2 * L - m - 1) / 2];
+    case 2: return -mag[(m - 2 * L - 1) / 2];
+    default: return mag[(4 * L - m - 1) / 2];
+    }
+}
+
+/* DST-VII (k, n) of an N-point transform: sign and magnitude of
+ * sin(pi (2k+1)(n+1) / (2N+1)); mag[j-1] holds |sin(j pi / (2N+1))| scaled. */
+static int32_t dst7(int N, int k, int n)
+{
+    const int32_t *mag = N == 4 ? kSin4 : N == 8 ? kSin8 : N == 16 ? kSin16 : kSin32;
+    const int P = 2 * N + 1;
+    int m = ((2 * k + 1) * (n + 1)) % (2 * P);
+    int sgn = 1;
+    if (m == 0 || m == P) return 0;
+    if (m > P) { m -= P; sgn = -1; }
+    int j = m <= N ? m : P - m;
+    return sgn * mag[j - 1];
+}
+
+int or_TrMatrix(int type, int n, int32_t *out)
+{
+    if (type == OR_DCT2) {
+        if (n < 2 || n > 64 || (n & (n - 1))) return -1;
+        for (int k = 0; k < n; k++)
+            for (int i = 0; i < n; i++) out[k * n + i] = dct2_64(k * (64 / n), i);
+        return 0;
+    }
+    if (n != 4 && n != 8 && n != 16 && n != 32) return -1;
+    for (int k = 0; k < n; k++)
+        for (int i = 0; i < n; i++)
+            out[k * n + i] = type == OR_DST7 ? dst7(n, k, i)
+                           : ((k & 1) ? -1 : 1) * dst7(n, k, n - 1 - i);
+    return 0;
+}
+
+/* Number of leading outputs a 1-D forward pass keeps (VVC zero-out; the
+ * quantiser reads only min(W,32) x min(H,32), h266_trans_quant.c:208-223). */
+static int kept_outputs(int type, int n)
+{
+    if (type == OR_DCT2) return n == 64 ? 32 : n;
+    return n == 32 ? 16 : n;
+}
+
+/* 1-D forward pass with the XinFor1dDct contract (h266_func_struct.h:198-204,
+ * layout from the two calls at h266_forward_trans.c:114-128):
+ *   out[k*os + j] = (sum_i T[k][i] * in[j*is + i] + 2^(shift-1)) >> shift
+ * for j < line, k < kept; zero for k >= kept. */
+void or_FDct1d(int type, int n, const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+               int32_t line, int32_t shift)
+{
+    int32_t T[64 * 64];
+    if (or_TrMatrix(type, n, T)) return;
+    const int keep = kept_outputs(type, n);
+    const int64_t rnd = shift > 0 ? ((int64_t)1 << (shift - 1)) : 0;
+    for (int32_t j = 0; j < line; j++)
+        for (int k = 0; k < n; k++) {
+            if (k >= keep) { out[k * os + j] = 0; continue; }
+            int64_t s = 0;
+            for (int i = 0; i < n; i++) s += (int64_t)T[k * n + i] * in[j * is + i];
+            out[k * os + j] = (or_coeff)((s + rnd) >> shift);
+        }
+}
+
+/* XinFor2dDct2 (h266_func_struct.h:206-215): two DCT-II passes through tmp. */
+void or_FDct2WxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+                 uint32_t w, uint32_t h, or_coeff *tmp, int32_t s1, int32_t s2)
+{
+    or_FDct1d(OR_DCT2, (int)w, in, is, tmp, h, (int32_t)h, s1);
+    or_FDct1d(OR_DCT2, (int)h, tmp, h, out, os, (int32_t)w, s2);
+}
+
+/* Residual as COEFF, XinBlockSubForDct (h26x_core/include/h26x_block_utility.h:151-159),
+ * called at h266_sub_dct.c:74-82. */
+void or_BlockSubForDct(const or_pixel *in, intptr_t is, const or_pixel *pred, intptr_t ps,
+                       or_coeff *diff, intptr_t dstride, uint32_t w, uint32_t h)
+{
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++)
+            diff[y * dstride + x] = (or_coeff)in[y * is + x] - (or_coeff)pred[y * ps + x];
+}
+
+/* mtsIdx2txType (closed, h26x_trans_context.h:66) as the H.266 mts_idx
+ * semantics fix it: [hor, ver] for DCT2_DCT2, SKIP, DST7_DST7, DCT8_DST7,
+ * DST7_DCT8, DCT8_DCT8 (enum order h26x_trans_context.h:42-51). */
+void or_MtsTypes(uint32_t mtsIdx, int32_t *horTx, int32_t *verTx)
+{
+    static const int8_t tab[6][2] = {{OR_DCT2, OR_DCT2}, {OR_SKIP, OR_SKIP}, {OR_DST7, OR_DST7},
+                                     {OR_DCT8, OR_DST7}, {OR_DST7, OR_DCT8}, {OR_DCT8, OR_DCT8}};
+    *horTx = tab[mtsIdx < 6 ? mtsIdx : 0][0];
+    *verTx = tab[mtsIdx < 6 ? mtsIdx : 0][1];
+}
+
+/* h266_forward_trans.c:45-152 (Xin266FDctWxH) without the transform-skip branch. */
+void or_FDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+                uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts, int bitDepth)
+{
+    int32_t hor, ver;
+    int lgW = 0, lgH = 0;
+    while ((1u << lgW) < w) lgW++;
+    while ((1u << lgH) < h) lgH++;
+    or_MtsTypes(mtsIdx, &hor, &ver);
+    const int32_t s1 = lgW + bitDepth + 6 - 15, s2 = lgH + 6;   /* :72-73 */
+    if (isIntraLuma && enableMts && mtsIdx == 0) {               /* :75-86 */
+        if (w >= 4 && w <= 16) hor = OR_DST7;
+        if (h >= 4 && h <= 16) ver = OR_DST7;
+    }
+    if (hor == OR_SKIP) return;
+    or_coeff *tmp = (or_coeff *)malloc(sizeof(or_coeff) * w * h);
+    if (lgW > 0 && lgH > 0) {                                    /* :88-130 */
+        or_FDct1d(hor, (int)w, in, is, tmp, h, (int32_t)h, s1);
+        or_FDct1d(ver, (int)h, tmp, h, out, os, (int32_t)w, s2);
+    } else if (lgW > 0) {
+        or_FDct1d(hor, (int)w, in, is, out, os, (int32_t)h, s1);
+    } else {
+        or_FDct1d(ver, (int)h, in, is, out, os, (int32_t)w, s2);
+    }
+    free(tmp);
+}
+
+/* ---------------------------------------------------------------------------
+ * Frame-level drivers (checker for the batched kernels).
+ * ------------------------------------------------------------------------- */
+
+/* MV window of a CU, h266_encoder/source/h266_enc_init.c:1462-1465 with
+ * XIN_PADDING_OFFSET_X/Y = 16 (h26x_definition.h:103-104). */
+static void cu_mv_window(int32_t x, int32_t y, int32_t w, int32_t h, int32_t fw, int32_t fh,
+                         int32_t ctu, int32_t *mnx, int32_t *mny, int32_t *mxx, int32_t *mxy)
+{
+    *mnx = (int16_t)(-x - ctu - (16 - 8));
+    *mny = (int16_t)(-y - ctu - (16 - 8));
+    *mxx = (int16_t)(fw + (ctu - w + 16 - 8) - x);
+    *mxy = (int16_t)(fh + (ctu - h + 16 - 8) - y);
+}
+
+static inline int32_t clip3(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+void or_RefineFracBlocks(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV,
+                         intptr_t csY, intptr_t csC,
+                         const or_pixel *refY, const or_pixel *refU, const or_pixel *refV,
+                         intptr_t rsY, intptr_t rsC, int32_t fw, int32_t fh,
+                         const or_block *blk, int32_t nblk, const or_refine_prm *prm,
+                         uint32_t *candCost, int32_t *bestIdx, uint32_t *bestCost,
+                         int32_t *bestMvx, int32_t *bestMvy,
+                         or_pixel *predY, or_pixel *predU, or_pixel *predV,
+                         intptr_t psY, intptr_t psC)
+{
+    or_pixel *bufY = (or_pixel *)malloc(sizeof(or_pixel) * 64 * 64);
+    or_pixel *bufU = (or_pixel *)malloc(sizeof(or_pixel) * 32 * 32);
+    or_pixel *bufV = (or_pixel *)malloc(sizeof(or_pixel) * 32 * 32);
+    for (int32_t b = 0; b < nblk; b++) {
+        const or_block *B = &blk[b];
+        int32_t mnx, mny, mxx, mxy;
+        cu_mv_window(B->x, B->y, B->w, B->h, fw, fh, prm->ctuSize, &mnx, &mny, &mxx, &mxy);
+        uint32_t best = UINT32_MAX;
+        int32_t bi = 0, bmx = 0, bmy = 0;
+        for (int32_t pass = 0; pass < 2; pass++) {
+            /* pass 0 costs every candidate; pass 1 writes the MC of the best. */
+            int32_t c0 = pass == 0 ? 0 : bi, c1 = pass == 0 ? prm->ncand : bi + 1;
+            for (int32_t c = c0; c < c1; c++) {
+                /* MV clip and address split: h266_inter_prediction.c:332-342 (luma),
+                 * :664-674 (chroma, 1/32 of chroma sample). */
+                int32_t mvx = clip3(B->mvx + prm->offx[c], mnx * 16, mxx * 16);
+                int32_t mvy = clip3(B->mvy + prm->offy[c], mny * 16, mxy * 16);
+                int32_t rx = (B->x << 4) + mvx, ry = (B->y << 4) + mvy;
+                int32_t fracY = ((ry & 15) << 4) + (rx & 15);
+                or_pixel *dY = pass ? predY + B->y * psY + B->x : bufY;
+                intptr_t sdY = pass ? psY : 64;
+                or_LumaInterpolation(refY + (ry >> 4) * rsY + (rx >> 4), rsY, dY, sdY, fracY,
+                                     (uint32_t)B->w, (uint32_t)B->h, 0);
+                uint32_t satd = 0;
+                if (pass == 0)
+                    satd = or_ComputeSatd(curY + B->y * csY + B->x, csY, bufY, 64, B->w, B->h);
+                if (prm->useChroma || pass) {
+                    int32_t fracC = ((ry & 31) << 5) + (rx & 31);
+                    intptr_t off = (ry >> 5) * rsC + (rx >> 5);
+                    int32_t cw = B->w / 2, ch = B->h / 2;
+                    or_pixel *dU = pass ? predU + (B->y / 2) * psC + B->x / 2 : bufU;
+                    or_pixel *dV = pass ? predV + (B->y / 2) * psC + B->x / 2 : bufV;
+                    intptr_t sdC = pass ? psC : 32;
+                    or_ChromaInterpolation(refU + off, rsC, dU, sdC, fracC, cw, ch);
+                    or_ChromaInterpolation(refV + off, rsC, dV, sdC, fracC, cw, ch);
+                    if (pass == 0) {
+                        intptr_t co = (B->y / 2) * csC + B->x / 2;
+                        satd += or_ComputeSatd(curU + co, csC, bufU, 32, cw, ch);
+                        satd += or_ComputeSatd(curV + co, csC, bufV, 32, cw, ch);
+                    }
+                }
+                if (pass == 0) {
+                    uint32_t cost = (satd << 8) + or_MvdBits(mvx - B->mvpx, mvy - B->mvpy) * prm->lambda;
+                    candCost[(size_t)b * prm->ncand + c] = cost;
+                    if (cost < best) { best = cost; bi = c; bmx = mvx; bmy = mvy; }
+                }
+            }
+        }
+        bestIdx[b] = bi;
+        bestCost[b] = best;
+        bestMvx[b] = bmx;
+        bestMvy[b] = bmy;
+    }
+    free(bufY);
+    free(bufU);
+    free(bufV);
+}
+
+void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs,
+                        int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range,
+                        uint32_t sadLambda, int32_t ctuSize, int32_t blk0, int32_t blk1,
+                        int16_t *bestMvX, int16_t *bestMvY, uint32_t *bestCost, uint32_t *bestSad)
+{
+    const int32_t nbx = fw / bw;
+    for (int32_t b = blk0; b < blk1; b++) {
+        int32_t x = (b % nbx) * bw, y = (b / nbx) * bh;
+        int32_t mnx, mny, mxx, mxy;
+        cu_mv_window(x, y, bw, bh, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
+        or_me me;
+        memset(&me, 0, sizeof(me));
+        me.width = (uint32_t)bw;
+        me.height = (uint32_t)bh;
+        me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny;
+        me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
+        me.searchWidth = range;
+        me.searchHeight = range;
+        me.sadLambda = sadLambda;
+        me.input = cur + y * cs + x;
+        me.inputStride = cs;
+        me.ref = ref + y * rs + x;
+        me.refStride = rs;
+        or_MotionEstFull(&me);
+        bestMvX[b] = me.bestMvX;
+        bestMvY[b] = me.bestMvY;
+        bestCost[b] = me.bestCost;
+        bestSad[b] = me.bestSad;
+    }
+}

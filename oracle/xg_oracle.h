@@ -1,0 +1,188 @@
+/*
+ * xg_oracle.h -- CPU oracle for the VVC block-matching hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
+ * The product library (libxg266.so) never links, loads or calls it.
+ *
+ * Every function restates the reference algorithm in plain scalar C.  Each
+ * definition in xg_oracle.c cites the reference file:line it follows (paths
+ * relative to the xin26x source tree).  Where the reference implementation
+ * is a closed binary, the function states the public method it follows and
+ * is marked "parity unpinned vs the reference"; such functions are pinned
+ * by closed forms, invariants and brute force in tests/.
+ *
+ * 10-bit build: PIXEL = uint16 in [0,1023], COEFF = int32.
+ */
+#ifndef XG_ORACLE_H
+#define XG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef uint16_t or_pixel;
+typedef int32_t  or_coeff;
+
+/* ---- T1: luma 8-tap interpolation (h266_core/source/h266_luma_mc.c) ---- */
+void or_InterpCopy(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                   int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpHor(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                      int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                      int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpHorVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                         int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_InterpCopyU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                        int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpHorU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpHorVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                              int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpVetS16U8(const int16_t *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_LumaInterpVetS16S16(const int16_t *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                            int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+
+/* ---- chroma 4-tap interpolation (h266_core/source/h266_chroma_mc.c) ---- */
+void or_ChromaInterpHor(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                        int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_ChromaInterpVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                        int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_ChromaInterpHorVet(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                           int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_ChromaInterpHorU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_ChromaInterpVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_ChromaInterpHorVetU8S16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                                int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_ChromaInterpVetS16U8(const int16_t *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+void or_ChromaInterpVetS16S16(const int16_t *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                              int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+
+/* ---- bi-prediction combine (h266_luma_mc.c:561-629) ---- */
+void or_InterpAvgS16U8(const int16_t *s0, intptr_t st0, const int16_t *s1, intptr_t st1,
+                       or_pixel *dst, intptr_t ds, uint32_t w, uint32_t h);
+void or_InterpWeightS16U8(const int16_t *s0, intptr_t st0, const int16_t *s1, intptr_t st1,
+                          or_pixel *dst, intptr_t ds, uint32_t w, uint32_t h,
+                          int32_t wA, int32_t wB);
+
+/* ---- table dispatch as the encoder does it ---- */
+/* h266_inter_prediction.c:143-172 (luma, PIXEL out, table index [!!fx][!!fy]) */
+void or_LumaInterpolation(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                          int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+/* h266_inter_prediction.c:78-141 (luma, S16 out) */
+void or_LumaInterpolationS16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                             int32_t frac, uint32_t w, uint32_t h, uint32_t fi);
+/* h266_inter_prediction.c:174-236 (chroma, PIXEL out; frac = (fy<<5)|fx) */
+void or_ChromaInterpolation(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds,
+                            int32_t frac, uint32_t w, uint32_t h);
+/* h266_inter_prediction.c:238-300 (chroma, S16 out) */
+void or_ChromaInterpolationS16(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds,
+                               int32_t frac, uint32_t w, uint32_t h);
+
+/* ---- distortion (closed in the reference: closed forms / HM methods) ---- */
+uint32_t or_ComputeSad(const or_pixel *in, intptr_t is, const or_pixel *pr, intptr_t ps,
+                       uint32_t w, uint32_t h);
+void or_ComputeSadx8(const or_pixel *in, intptr_t is, const or_pixel *const ref[8], intptr_t rs,
+                     uint32_t w, uint32_t h, uint32_t sad[8]);
+void or_ComputeSadx5(const or_pixel *in, intptr_t is, const or_pixel *r0, const or_pixel *r1,
+                     const or_pixel *r2, const or_pixel *r3, const or_pixel *r4, intptr_t rs,
+                     uint32_t w, uint32_t h, uint32_t sad[5]);
+void or_ComputeSadx3(const or_pixel *in, intptr_t is, const or_pixel *r0, const or_pixel *r1,
+                     const or_pixel *r2, intptr_t rs, uint32_t w, uint32_t h, uint32_t sad[3]);
+uint32_t or_ComputeSatd(const or_pixel *in, intptr_t is, const or_pixel *pr, intptr_t ps,
+                        uint32_t w, uint32_t h);
+uint32_t or_ComputeHada8x8(const or_pixel *in, intptr_t is);
+
+/* ---- rate terms ---- */
+uint32_t or_ExpGolombBits(int32_t v);
+uint32_t or_MvdBits(int32_t dx, int32_t dy);
+uint32_t or_SadLambda(int32_t qp, int32_t is10bit);
+
+/* ---- MV helpers (h266_get_neighbour_mv.c:245-320) ---- */
+int32_t or_MvRoundShift(int32_t v, int32_t shift);
+
+/* ---- integer and fractional motion search ---- */
+typedef struct or_me {
+    uint32_t width, height;
+    int16_t  minMvX, minMvY, maxMvX, maxMvY;   /* int-pel */
+    int16_t  candX[12], candY[12];             /* int-pel start candidates */
+    uint32_t candNum;
+    int16_t  bestMvX, bestMvY;                 /* quarter-pel */
+    int16_t  predMvX, predMvY;                 /* quarter-pel */
+    uint32_t bestCost, bestSad;
+    int32_t  searchWidth, searchHeight;
+    uint32_t sadLambda;
+    const or_pixel *input; intptr_t inputStride;
+    const or_pixel *ref;   intptr_t refStride; /* co-located block in padded ref */
+} or_me;
+
+void or_MotionEstFull(or_me *me);
+void or_MotionEstDia(or_me *me);
+void or_FullSubPelMe(or_me *me);
+void or_SixteenthPelMe(or_me *me, int16_t *bestMv16X, int16_t *bestMv16Y);
+void or_FillMeCand(int16_t predX, int16_t predY, const int16_t *qx, const int16_t *qy,
+                   uint32_t n, int16_t *outX, int16_t *outY, uint32_t *outNum);
+
+/* ---- forward transforms (closed in the reference; H.266 matrices) ---- */
+enum { OR_DCT2 = 0, OR_DCT8 = 1, OR_DST7 = 2, OR_SKIP = 4 };
+int  or_TrMatrix(int type, int n, int32_t *out);
+void or_FDct1d(int type, int n, const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+               int32_t line, int32_t shift);
+void or_FDct2WxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+                 uint32_t w, uint32_t h, or_coeff *tmp, int32_t s1, int32_t s2);
+void or_BlockSubForDct(const or_pixel *in, intptr_t is, const or_pixel *pred, intptr_t ps,
+                       or_coeff *diff, intptr_t dstride, uint32_t w, uint32_t h);
+void or_FDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+                uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts,
+                int bitDepth);
+void or_MtsTypes(uint32_t mtsIdx, int32_t *horTx, int32_t *verTx);
+
+/* ---- frame-level drivers used as the batched-path checker ---- */
+typedef struct or_block {
+    int32_t x, y, w, h;        /* luma position and size */
+    int32_t mvx, mvy;          /* base MV, 1/16 pel */
+    int32_t mvpx, mvpy;        /* predictor, 1/16 pel */
+} or_block;
+
+typedef struct or_refine_prm {
+    int32_t ncand;
+    const int16_t *offx, *offy;    /* candidate offsets, 1/16 pel */
+    uint32_t lambda;
+    int32_t  ctuSize;              /* MV clip window (h266_enc_init.c:1462-1465) */
+    int32_t  useChroma;            /* add Cb/Cr SATD into the candidate cost */
+} or_refine_prm;
+
+/* Config-2 stage: per block and candidate, luma 8-tap (+ chroma 4-tap)
+ * interpolation and SATD; returns per-candidate costs, the best candidate,
+ * and the motion-compensated prediction of the best candidate. */
+void or_RefineFracBlocks(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV,
+                         intptr_t curStrideY, intptr_t curStrideC,
+                         const or_pixel *refY, const or_pixel *refU, const or_pixel *refV,
+                         intptr_t refStrideY, intptr_t refStrideC,
+                         int32_t frameW, int32_t frameH,
+                         const or_block *blk, int32_t nblk, const or_refine_prm *prm,
+                         uint32_t *candCost, int32_t *bestIdx, uint32_t *bestCost,
+                         int32_t *bestMvx, int32_t *bestMvy,
+                         or_pixel *predY, or_pixel *predU, or_pixel *predV,
+                         intptr_t predStrideY, intptr_t predStrideC);
+
+/* Config-1 stage: integer full search for every block of a frame. */
+void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs,
+                        int32_t frameW, int32_t frameH, int32_t bw, int32_t bh,
+                        int32_t range, uint32_t sadLambda, int32_t ctuSize,
+                        int32_t blk0, int32_t blk1,
+                        int16_t *bestMvX, int16_t *bestMvY, uint32_t *bestCost,
+                        uint32_t *bestSad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

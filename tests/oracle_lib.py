@@ -1,0 +1,82 @@
+"""ctypes loader for the CPU oracle (oracle/liboracle.so): the checker.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+
+P = ctypes.c_void_p
+SS = ctypes.c_ssize_t
+I32 = ctypes.c_int32
+U32 = ctypes.c_uint32
+
+
+class OrBlock(ctypes.Structure):
+    _fields_ = [("x", I32), ("y", I32), ("w", I32), ("h", I32),
+                ("mvx", I32), ("mvy", I32), ("mvpx", I32), ("mvpy", I32)]
+
+
+class OrRefinePrm(ctypes.Structure):
+    _fields_ = [("ncand", I32), ("offx", P), ("offy", P), ("lambda_", U32),
+                ("ctuSize", I32), ("useChroma", I32)]
+
+
+class OrMe(ctypes.Structure):
+    _fields_ = [("width", U32), ("height", U32),
+                ("minMvX", ctypes.c_int16), ("minMvY", ctypes.c_int16),
+                ("maxMvX", ctypes.c_int16), ("maxMvY", ctypes.c_int16),
+                ("candX", ctypes.c_int16 * 12), ("candY", ctypes.c_int16 * 12),
+                ("candNum", U32),
+                ("bestMvX", ctypes.c_int16), ("bestMvY", ctypes.c_int16),
+                ("predMvX", ctypes.c_int16), ("predMvY", ctypes.c_int16),
+                ("bestCost", U32), ("bestSad", U32),
+                ("searchWidth", I32), ("searchHeight", I32),
+                ("sadLambda", U32),
+                ("input", P), ("inputStride", SS),
+                ("ref", P), ("refStride", SS)]
+
+
+def load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
+    lib = ctypes.CDLL(ORACLE_SO)
+    sig = {
+        "or_ComputeSad": (U32, [P, SS, P, SS, U32, U32]),
+        "or_ComputeSatd": (U32, [P, SS, P, SS, U32, U32]),
+        "or_ComputeHada8x8": (U32, [P, SS]),
+        "or_ComputeSadx8": (None, [P, SS, P, SS, U32, U32, P]),
+        "or_ComputeSadx5": (None, [P, SS, P, P, P, P, P, SS, U32, U32, P]),
+        "or_ComputeSadx3": (None, [P, SS, P, P, P, SS, U32, U32, P]),
+        "or_ExpGolombBits": (U32, [I32]),
+        "or_MvdBits": (U32, [I32, I32]),
+        "or_SadLambda": (U32, [I32, I32]),
+        "or_MvRoundShift": (I32, [I32, I32]),
+        "or_MotionEstFull": (None, [ctypes.POINTER(OrMe)]),
+        "or_MotionEstDia": (None, [ctypes.POINTER(OrMe)]),
+        "or_FullSubPelMe": (None, [ctypes.POINTER(OrMe)]),
+        "or_SixteenthPelMe": (None, [ctypes.POINTER(OrMe), P, P]),
+        "or_FillMeCand": (None, [ctypes.c_int16, ctypes.c_int16, P, P, U32, P, P, P]),
+        "or_TrMatrix": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P]),
+        "or_FDct1d": (None, [ctypes.c_int, ctypes.c_int, P, SS, P, SS, I32, I32]),
+        "or_FDct2WxH": (None, [P, SS, P, SS, U32, U32, P, I32, I32]),
+        "or_FDctWxH": (None, [P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "or_BlockSubForDct": (None, [P, SS, P, SS, P, SS, U32, U32]),
+        "or_MtsTypes": (None, [U32, P, P]),
+        "or_LumaInterpolation": (None, [P, SS, P, SS, I32, U32, U32, U32]),
+        "or_LumaInterpolationS16": (None, [P, SS, P, SS, I32, U32, U32, U32]),
+        "or_ChromaInterpolation": (None, [P, SS, P, SS, I32, U32, U32]),
+        "or_ChromaInterpolationS16": (None, [P, SS, P, SS, I32, U32, U32]),
+        "or_RefineFracBlocks": (None, [P, P, P, SS, SS, P, P, P, SS, SS, I32, I32, P, I32,
+                                       ctypes.POINTER(OrRefinePrm), P, P, P, P, P, P, P, P, SS, SS]),
+        "or_FullSearchFrame": (None, [P, SS, P, SS, I32, I32, I32, I32, I32, U32, I32, I32, I32,
+                                      P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib

@@ -1,0 +1,150 @@
+"""ctypes bindings of libxg266.so (include/xg266.h).
+
+load() returns the library with every prototype set.  It raises if the HIP
+library has not been built: there is no CPU fallback anywhere in the product.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libxg266.so")
+
+P = ctypes.c_void_p
+SS = ctypes.c_ssize_t
+I16 = ctypes.c_int16
+I32 = ctypes.c_int32
+U32 = ctypes.c_uint32
+U8 = ctypes.c_uint8
+
+XG_MAX_CAND = 64
+XG_OK, XG_ERR_ARG, XG_ERR_HIP, XG_ERR_NOMEM, XG_ERR_BUDGET, XG_ERR_STATE = range(6)
+MODE_PLAIN, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
+
+
+class XgPlane(ctypes.Structure):
+    _fields_ = [("data", P), ("stride", SS), ("width", I32), ("height", I32), ("pad", I32)]
+
+
+class XgPicture(ctypes.Structure):
+    _fields_ = [("plane", XgPlane * 3)]
+
+
+class XgBlock(ctypes.Structure):
+    _fields_ = [("x", I16), ("y", I16), ("w", U8), ("h", U8), ("pic", U8), ("ref", U8),
+                ("mvx", I32), ("mvy", I32), ("mvpx", I32), ("mvpy", I32)]
+
+
+class XgRefineCfg(ctypes.Structure):
+    _fields_ = [("ncand", I32), ("offx", I16 * XG_MAX_CAND), ("offy", I16 * XG_MAX_CAND),
+                ("lambda_", U32), ("ctuSize", I32), ("useChroma", I32)]
+
+
+class XgBlockResult(ctypes.Structure):
+    _fields_ = [("mvx", I32), ("mvy", I32), ("cost", U32), ("cand", I32)]
+
+
+class XgMv(ctypes.Structure):
+    _fields_ = [("mvX", I16), ("mvY", I16)]
+
+
+class XgMeStruct(ctypes.Structure):
+    _fields_ = [("width", U32), ("height", U32), ("minMv", XgMv), ("maxMv", XgMv),
+                ("meCand", XgMv * 12), ("candNum", U32), ("bestMv", XgMv), ("predMv", XgMv),
+                ("bestCost", U32), ("bestSad", U32), ("searchWidth", I32), ("searchHeight", I32),
+                ("sadLambda", U32), ("input", P), ("inputStride", SS), ("ref", P), ("refStride", SS)]
+
+
+class XgFuncStruct(ctypes.Structure):
+    _fields_ = [("pfXinComputeSad", P * 8), ("pfXinComputeSatd", P * 8), ("pfXinComputeDist", P * 8),
+                ("pfXinComputeSadx8", P * 8), ("pfXinComputeSadx5", P * 8), ("pfXinComputeSadx3", P * 8),
+                ("pfXinLumaInterp", P * 32), ("pfXinChromaInterp", P * 32),
+                ("pfXinLumaInterpS16", P * 32), ("pfXinChromaInterpS16", P * 32),
+                ("pfXinInterpAvg", P * 8), ("pfXinInterpWeight", P * 8),
+                ("pfXinFor2dDct2", P * 64), ("pfXinFor1dDct", P * 24),
+                ("pfXinBlockSubForDct", P * 8), ("pfXinComputeHada8x8", P)]
+
+
+# Every symbol include/xg266.h declares, with its prototype (restype, argtypes).
+INTERP = (None, [P, SS, P, SS, I32, U32, U32, U32])
+FDCT1D = (None, [P, SS, P, SS, I32, I32])
+PROTOS = {
+    "xg_last_error": (I32, []),
+    "xg_clear_error": (None, []),
+    "xg_set_logger": (None, [P]),
+    "xg_version": (ctypes.c_char_p, []),
+    "Xg266ComputeSad": (None, [P, SS, P, SS, U32, U32, P]),
+    "Xg266ComputeSatd": (None, [P, SS, P, SS, U32, U32, P]),
+    "Xg266ComputeSadx8": (None, [P, SS, P, SS, U32, U32, P]),
+    "Xg266ComputeSadx5": (None, [P, SS, P, P, P, P, P, SS, U32, U32, P]),
+    "Xg266ComputeSadx3": (None, [P, SS, P, P, P, SS, U32, U32, P]),
+    "Xg266ComputeHada8x8": (None, [P, SS, P]),
+    "Xg266InterpCopy": INTERP, "Xg266LumaInterpHor": INTERP, "Xg266LumaInterpVet": INTERP,
+    "Xg266LumaInterpHorVet": INTERP, "Xg266InterpCopyU8S16": INTERP,
+    "Xg266LumaInterpHorU8S16": INTERP, "Xg266LumaInterpVetU8S16": INTERP,
+    "Xg266LumaInterpHorVetU8S16": INTERP, "Xg266LumaInterpVetS16U8": INTERP,
+    "Xg266LumaInterpVetS16S16": INTERP, "Xg266ChromaInterpHor": INTERP,
+    "Xg266ChromaInterpVet": INTERP, "Xg266ChromaInterpHorVet": INTERP,
+    "Xg266ChromaInterpHorU8S16": INTERP, "Xg266ChromaInterpVetU8S16": INTERP,
+    "Xg266ChromaInterpHorVetU8S16": INTERP, "Xg266ChromaInterpVetS16U8": INTERP,
+    "Xg266ChromaInterpVetS16S16": INTERP,
+    "Xg266InterpAvgS16U8": (None, [P, SS, P, SS, P, SS, U32, U32]),
+    "Xg266InterpWeightS16U8": (None, [P, SS, P, SS, P, SS, U32, U32, I32, I32]),
+    "Xg26xFDct2P2": FDCT1D, "Xg26xFDct2P4": FDCT1D, "Xg26xFDct2P8": FDCT1D,
+    "Xg26xFDct2P16": FDCT1D, "Xg26xFDct2P32": FDCT1D, "Xg26xFDct2P64": FDCT1D,
+    "Xg26xFDst7P4": FDCT1D, "Xg26xFDst7P8": FDCT1D, "Xg26xFDst7P16": FDCT1D, "Xg26xFDst7P32": FDCT1D,
+    "Xg26xFDct8P4": FDCT1D, "Xg26xFDct8P8": FDCT1D, "Xg26xFDct8P16": FDCT1D, "Xg26xFDct8P32": FDCT1D,
+    "Xg26xFDct2WxH": (None, [P, SS, P, SS, U32, U32, P, I32, I32]),
+    "Xg26xBlockSubForDct": (None, [P, SS, P, SS, P, SS, U32, U32]),
+    "Xg266FDctWxH": (I32, [P, SS, P, SS, U32, U32, U32, I32, I32]),
+    "Xg266FuncInit": (I32, [ctypes.POINTER(XgFuncStruct), I32]),
+    "Xg26xMotionEstFull": (None, [ctypes.POINTER(XgMeStruct)]),
+    "Xg26xMotionEstDia": (None, [ctypes.POINTER(XgMeStruct)]),
+    "Xg266FullSubPelMe": (None, [ctypes.POINTER(XgMeStruct)]),
+    "xg_session_create": (I32, [ctypes.POINTER(P), I32]),
+    "xg_session_destroy": (I32, [P]),
+    "xg_session_stream": (P, [P]),
+    "xg_session_set_picture": (I32, [P, I32, ctypes.POINTER(XgPicture)]),
+    "xg_session_set_blocks": (I32, [P, P, I32]),
+    "xg_session_set_tile": (I32, [P, I32, I32, I32]),
+    "xg_session_get_tile": (I32, [P, I32, I32]),
+    "xg_session_autotune": (I32, [P, ctypes.POINTER(XgRefineCfg), I32, ctypes.c_char_p]),
+    "xg_session_load_tiles": (I32, [P, ctypes.c_char_p]),
+    "xg_refine_frac": (I32, [P, ctypes.POINTER(XgRefineCfg), ctypes.POINTER(XgPicture), I32]),
+    "xg_session_outputs": (I32, [P, ctypes.POINTER(P), ctypes.POINTER(P)]),
+    "xg_session_sync": (I32, [P]),
+    "xg_session_read": (I32, [P, P, P]),
+    "xg_session_kernel_time": (I32, [P, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(I32)]),
+    "xg_session_reset_timers": (I32, [P]),
+    "xg_session_set_timing": (I32, [P, I32]),
+    "xg_full_search_frame": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, I32,
+                                   U32, I32, P, P, P]),
+    "xg_fdct_plan": (I32, [P, P, P, I32]),
+    "xg_fdct_run": (I32, [P, P, P]),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load libxg266.so and set every prototype.  Raises when it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -m xin26x_amd.build` "
+                           "(there is no CPU fallback)")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in PROTOS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != XG_OK:
+        raise RuntimeError(f"{what} failed with xg status {rc}")
+    return rc

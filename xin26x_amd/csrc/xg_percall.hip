@@ -1,0 +1,655 @@
+// xg_percall.hip -- the per-call primitives of the xin_func_struct mirror.
+//
+// Each entry point takes the reference's host pointers, stages exactly the
+// samples the reference function reads into pinned memory (samples it does
+// not read are zero-filled and only ever meet zero filter taps), copies them
+// to HBM on the calling thread's stream, runs one kernel and copies the
+// result back before returning.
+//
+// Interpolation: every table slot (Copy, Hor, Vet, HorVet and their S16
+// variants) runs the same separable kernel with a phase-0 filter in the
+// direction the slot does not filter.  For 10-bit samples that is
+// bit-identical to the reference's separate code paths
+// (h266_core/source/h266_luma_mc.c:72-559, h266_chroma_mc.c:62-384):
+//   Hor:  clip((S+32)>>6)        == clip((64*((S-32768)>>2) + 524800) >> 10)
+//   Vet:  clip((S+32)>>6)        with first-pass value 16*s - 8192 (exact)
+//   Copy: s                      (both passes phase 0)
+//   S16:  (S-32768)>>2 == (64*t)>>6, (16*S - 2^19)>>6 == (S-32768)>>2
+// so the kernels have no per-slot branches.
+#include <mutex>
+#include <string.h>
+
+#include "xg_device.hpp"
+#include "xg_runtime.hpp"
+#include "xg_trmatrix.hpp"
+
+namespace xg {
+
+static __constant__ TrTables c_tr = kTr;
+
+static std::once_flag g_init_once;
+static bool g_init_ok = false;
+
+bool device_init()
+{
+    std::call_once(g_init_once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            set_error(XG_ERR_HIP, "no HIP device available");
+            return;
+        }
+        hipDeviceProp_t prop;
+        if (!XG_HIP(hipGetDeviceProperties(&prop, 0))) return;
+        if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            log_msg(1, "device is %s; this library is built for gfx950", prop.gcnArchName);
+        g_init_ok = true;
+    });
+    if (!g_init_ok) set_error(XG_ERR_HIP, "HIP device initialisation failed");
+    return g_init_ok;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Separable interpolation of a packed region: src holds (h+TAPS-1) rows of
+// (w+TAPS-1) samples starting TAPS/2-1 samples above/left of the block.
+template <int TAPS, bool S16>
+__global__ void k_interp(const uint16_t *__restrict__ src, int w, int h, int fx, int fy, int fi,
+                         void *__restrict__ dst)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const int ss = w + TAPS - 1;
+    const int8_t *ch = taps_for<TAPS>(fx, fi);
+    const int8_t *cv = taps_for<TAPS>(fy, fi);
+    const uint16_t *p = src + y * ss + x;
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < TAPS; k++) acc += (int)cv[k] * hpass<TAPS>(p + k * ss, ch);
+    if constexpr (S16) ((int16_t *)dst)[y * w + x] = (int16_t)vfinal_s16(acc);
+    else ((uint16_t *)dst)[y * w + x] = (uint16_t)vfinal_pix(acc);
+}
+
+// Vertical pass over an S16 first-pass buffer (VetS16U8 / VetS16S16):
+// src holds (h+TAPS-1) rows of w values starting TAPS/2-1 rows above.
+template <int TAPS, bool S16>
+__global__ void k_vet_s16(const int16_t *__restrict__ src, int w, int h, int fy, int fi,
+                          void *__restrict__ dst)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const int8_t *cv = taps_for<TAPS>(fy, fi);
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < TAPS; k++) acc += (int)cv[k] * (int)src[(y + k) * w + x];
+    if constexpr (S16) ((int16_t *)dst)[y * w + x] = (int16_t)vfinal_s16(acc);
+    else ((uint16_t *)dst)[y * w + x] = (uint16_t)vfinal_pix(acc);
+}
+
+// Bi-prediction average (h266_luma_mc.c:561-593) or BCW weighting (:595-629).
+__global__ void k_bipred(const int16_t *__restrict__ a, const int16_t *__restrict__ b, int n,
+                         int weighted, int wa, int wb, uint16_t *__restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v;
+    if (weighted) v = ((int)a[i] * wa + (int)b[i] * wb + (1 << 6) + (8192 << 3)) >> 7;
+    else v = ((int)a[i] + (int)b[i] + (1 << 4) + 2 * 8192) >> 5;
+    dst[i] = (uint16_t)clip_pix(v);
+}
+
+// Block-reduce helper for one workgroup of 256 threads.
+__device__ __forceinline__ uint32_t block_sum256(uint32_t v)
+{
+    __shared__ uint32_t red[4];
+    v = (uint32_t)group_sum<64>((int)v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    uint32_t t = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return t;
+}
+
+// SAD of `n` packed reference blocks against one packed input block; one
+// workgroup per reference block.
+__global__ void __launch_bounds__(256) k_sad(const uint16_t *__restrict__ in,
+                                             const uint16_t *__restrict__ refs, int npix,
+                                             uint32_t *__restrict__ out)
+{
+    const uint16_t *r = refs + (size_t)blockIdx.x * npix;
+    uint32_t s = 0;
+    for (int i = threadIdx.x; i < npix; i += 256) s += (uint32_t)abs((int)in[i] - (int)r[i]);
+    s = block_sum256(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// SATD of a packed w x h block pair; one thread per Hadamard tile.
+template <int N>
+__device__ uint32_t tile_satd_serial(const uint16_t *a, const uint16_t *b, int stride)
+{
+    int d[N * N];
+#pragma unroll
+    for (int y = 0; y < N; y++)
+#pragma unroll
+        for (int x = 0; x < N; x++) d[y * N + x] = (int)a[y * stride + x] - (int)b[y * stride + x];
+#pragma unroll
+    for (int y = 0; y < N; y++) wht_reg<N>(d + y * N);
+#pragma unroll
+    for (int x = 0; x < N; x++) {
+        int col[N];
+#pragma unroll
+        for (int y = 0; y < N; y++) col[y] = d[y * N + x];
+        wht_reg<N>(col);
+#pragma unroll
+        for (int y = 0; y < N; y++) d[y * N + x] = col[y];
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < N * N; i++) s += (uint32_t)abs(d[i]);
+    if (N == 8) return (s + 2) >> 2;
+    if (N == 4) return (s + 1) >> 1;
+    return s;
+}
+
+__global__ void __launch_bounds__(256) k_satd(const uint16_t *__restrict__ a,
+                                              const uint16_t *__restrict__ b, int w, int h,
+                                              uint32_t *__restrict__ out)
+{
+    const int n = (w % 8 == 0 && h % 8 == 0) ? 8 : (w % 4 == 0 && h % 4 == 0) ? 4 : 2;
+    const int tw = w / n, tiles = tw * (h / n);
+    uint32_t s = 0;
+    for (int t = threadIdx.x; t < tiles; t += 256) {
+        const int off = (t / tw) * n * w + (t % tw) * n;
+        if (n == 8) s += tile_satd_serial<8>(a + off, b + off, w);
+        else if (n == 4) s += tile_satd_serial<4>(a + off, b + off, w);
+        else s += tile_satd_serial<2>(a + off, b + off, w);
+    }
+    s = block_sum256(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// 1-D forward transform: in holds `line` vectors of N values (packed),
+// out[k*line + j] (packed).  int64 accumulation, round-half-up shift.
+__global__ void k_fdct1d(const int32_t *__restrict__ in, int type, int N, int line, int shift,
+                         int32_t *__restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (j >= line || k >= N) return;
+    int32_t v = 0;
+    if (k < tr_kept(type, N)) {
+        long long s = 0;
+        for (int n = 0; n < N; n++) s += (long long)tr_entry(c_tr, type, N, k, n) * in[(size_t)j * N + n];
+        const long long rnd = shift > 0 ? (1LL << (shift - 1)) : 0;
+        v = (int32_t)((s + rnd) >> shift);
+    }
+    out[(size_t)k * line + j] = v;
+}
+
+__global__ void k_blocksub(const uint16_t *__restrict__ a, const uint16_t *__restrict__ b, int n,
+                           int32_t *__restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (int32_t)a[i] - (int32_t)b[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host wrappers
+// ---------------------------------------------------------------------------
+enum Dir { kCopy = 0, kHor = 1, kVet = 2, kHorVet = 3 };
+
+static bool check_wh(uint32_t w, uint32_t h, uint32_t maxw, uint32_t maxh, const char *fn)
+{
+    if (w == 0 || h == 0 || w > maxw || h > maxh) {
+        set_error(XG_ERR_ARG, "%s: unsupported block %ux%u", fn, w, h);
+        return false;
+    }
+    return true;
+}
+
+static bool finish(CallCtx *c) { return XG_HIP(hipStreamSynchronize(c->stream)); }
+
+// PIXEL source, PIXEL or S16 destination.
+static void interp_u16(int taps, bool outS16, Dir dir, const uint16_t *src, intptr_t ss, void *dst,
+                       intptr_t ds, int fx, int fy, int fi, uint32_t w, uint32_t h, const char *fn)
+{
+    if (!check_wh(w, h, 128, 128, fn)) return;
+    if (fi > 2) fi = 0;
+    const int hb = taps / 2 - 1, rw = (int)w + taps - 1, rh = (int)h + taps - 1;
+    const size_t inB = align_up((size_t)rw * rh * 2, 256), outB = (size_t)w * h * 2;
+    CallCtx *c = call_ctx(inB + outB, inB + outB);
+    if (!c) return;
+    uint16_t *hin = (uint16_t *)c->host;
+    memset(hin, 0, (size_t)rw * rh * 2);
+    const bool rows = dir == kVet || dir == kHorVet, cols = dir == kHor || dir == kHorVet;
+    const int r0 = rows ? 0 : hb, r1 = rows ? rh : hb + (int)h;
+    const int c0 = cols ? 0 : hb, c1 = cols ? rw : hb + (int)w;
+    for (int r = r0; r < r1; r++)
+        memcpy(hin + (size_t)r * rw + c0, src + (intptr_t)(r - hb) * ss + (c0 - hb), (size_t)(c1 - c0) * 2);
+    char *din = c->dev, *dout = c->dev + inB;
+    if (!XG_HIP(hipMemcpyAsync(din, hin, (size_t)rw * rh * 2, hipMemcpyHostToDevice, c->stream))) return;
+    dim3 grid((w + 63) / 64, h), block(64);
+    if (taps == 8) {
+        if (outS16) k_interp<8, true><<<grid, block, 0, c->stream>>>((uint16_t *)din, w, h, fx, fy, fi, dout);
+        else k_interp<8, false><<<grid, block, 0, c->stream>>>((uint16_t *)din, w, h, fx, fy, fi, dout);
+    } else {
+        if (outS16) k_interp<4, true><<<grid, block, 0, c->stream>>>((uint16_t *)din, w, h, fx, fy, 0, dout);
+        else k_interp<4, false><<<grid, block, 0, c->stream>>>((uint16_t *)din, w, h, fx, fy, 0, dout);
+    }
+    if (!XG_HIP(hipGetLastError())) return;
+    char *hout = c->host + inB;
+    if (!XG_HIP(hipMemcpyAsync(hout, dout, outB, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(dst, ds, hout, (int)w, (int)h, 2);
+}
+
+// S16 source (first-pass buffer), vertical pass only.
+static void interp_s16(int taps, bool outS16, const int16_t *src, intptr_t ss, void *dst, intptr_t ds,
+                       int fy, int fi, uint32_t w, uint32_t h, const char *fn)
+{
+    if (!check_wh(w, h, 128, 128, fn)) return;
+    if (fi > 2) fi = 0;
+    const int hb = taps / 2 - 1, rh = (int)h + taps - 1;
+    const size_t inB = align_up((size_t)w * rh * 2, 256), outB = (size_t)w * h * 2;
+    CallCtx *c = call_ctx(inB + outB, inB + outB);
+    if (!c) return;
+    pack2d(c->host, src - (intptr_t)hb * ss, ss, (int)w, rh, 2);
+    char *din = c->dev, *dout = c->dev + inB;
+    if (!XG_HIP(hipMemcpyAsync(din, c->host, (size_t)w * rh * 2, hipMemcpyHostToDevice, c->stream))) return;
+    dim3 grid((w + 63) / 64, h), block(64);
+    if (taps == 8) {
+        if (outS16) k_vet_s16<8, true><<<grid, block, 0, c->stream>>>((int16_t *)din, w, h, fy, fi, dout);
+        else k_vet_s16<8, false><<<grid, block, 0, c->stream>>>((int16_t *)din, w, h, fy, fi, dout);
+    } else {
+        if (outS16) k_vet_s16<4, true><<<grid, block, 0, c->stream>>>((int16_t *)din, w, h, fy, 0, dout);
+        else k_vet_s16<4, false><<<grid, block, 0, c->stream>>>((int16_t *)din, w, h, fy, 0, dout);
+    }
+    if (!XG_HIP(hipGetLastError())) return;
+    char *hout = c->host + inB;
+    if (!XG_HIP(hipMemcpyAsync(hout, dout, outB, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(dst, ds, hout, (int)w, (int)h, 2);
+}
+
+static void bipred(const int16_t *a, intptr_t sa, const int16_t *b, intptr_t sb, uint16_t *dst,
+                   intptr_t ds, uint32_t w, uint32_t h, int weighted, int wa, int wb, const char *fn)
+{
+    if (!check_wh(w, h, 128, 128, fn)) return;
+    const size_t n = (size_t)w * h, blk = align_up(n * 2, 256);
+    CallCtx *c = call_ctx(3 * blk, 3 * blk);
+    if (!c) return;
+    pack2d(c->host, a, sa, (int)w, (int)h, 2);
+    pack2d(c->host + blk, b, sb, (int)w, (int)h, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, 2 * blk, hipMemcpyHostToDevice, c->stream))) return;
+    k_bipred<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>((int16_t *)c->dev, (int16_t *)(c->dev + blk),
+                                                               (int)n, weighted, wa, wb,
+                                                               (uint16_t *)(c->dev + 2 * blk));
+    if (!XG_HIP(hipGetLastError())) return;
+    if (!XG_HIP(hipMemcpyAsync(c->host + 2 * blk, c->dev + 2 * blk, n * 2, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(dst, ds, c->host + 2 * blk, (int)w, (int)h, 2);
+}
+
+// SAD of one input block against nref reference blocks.
+static void sad_multi(const uint16_t *in, intptr_t is, const uint16_t *const *refs, int nref,
+                      intptr_t rs, uint32_t w, uint32_t h, uint32_t *out, const char *fn)
+{
+    for (int k = 0; k < nref; k++) out[k] = UINT32_MAX;
+    if (!check_wh(w, h, 128, 128, fn)) return;
+    const size_t npix = (size_t)w * h, blk = align_up(npix * 2, 256);
+    const size_t bytes = blk * (1 + nref) + 256;
+    CallCtx *c = call_ctx(bytes, bytes);
+    if (!c) return;
+    pack2d(c->host, in, is, (int)w, (int)h, 2);
+    for (int k = 0; k < nref; k++) pack2d(c->host + blk * (1 + k), refs[k], rs, (int)w, (int)h, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, blk * (1 + nref), hipMemcpyHostToDevice, c->stream))) return;
+    uint32_t *dres = (uint32_t *)(c->dev + blk * (1 + nref));
+    k_sad<<<nref, 256, 0, c->stream>>>((uint16_t *)c->dev, (uint16_t *)(c->dev + blk), (int)npix, dres);
+    if (!XG_HIP(hipGetLastError())) return;
+    uint32_t *hres = (uint32_t *)(c->host + blk * (1 + nref));
+    if (!XG_HIP(hipMemcpyAsync(hres, dres, 4 * nref, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    for (int k = 0; k < nref; k++) out[k] = hres[k];
+}
+
+static void satd_call(const uint16_t *in, intptr_t is, const uint16_t *pr, intptr_t ps, uint32_t w,
+                      uint32_t h, uint32_t *out, const char *fn)
+{
+    *out = UINT32_MAX;
+    if (!check_wh(w, h, 128, 128, fn)) return;
+    if ((w & 1) || (h & 1)) {
+        set_error(XG_ERR_ARG, "%s: SATD needs even sizes (%ux%u)", fn, w, h);
+        return;
+    }
+    const size_t npix = (size_t)w * h, blk = align_up(npix * 2, 256);
+    CallCtx *c = call_ctx(2 * blk + 256, 2 * blk + 256);
+    if (!c) return;
+    pack2d(c->host, in, is, (int)w, (int)h, 2);
+    pack2d(c->host + blk, pr, ps, (int)w, (int)h, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, 2 * blk, hipMemcpyHostToDevice, c->stream))) return;
+    k_satd<<<1, 256, 0, c->stream>>>((uint16_t *)c->dev, (uint16_t *)(c->dev + blk), (int)w, (int)h,
+                                     (uint32_t *)(c->dev + 2 * blk));
+    if (!XG_HIP(hipGetLastError())) return;
+    if (!XG_HIP(hipMemcpyAsync(c->host + 2 * blk, c->dev + 2 * blk, 4, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    *out = *(uint32_t *)(c->host + 2 * blk);
+}
+
+// Device-side 1-D pass: `in` packed [line][N] -> `out` packed [N][line].
+static bool fdct1d_dev(CallCtx *c, const int32_t *din, int type, int N, int line, int shift, int32_t *dout)
+{
+    dim3 grid((line + 63) / 64, N), block(64);
+    k_fdct1d<<<grid, block, 0, c->stream>>>(din, type, N, line, shift, dout);
+    return XG_HIP(hipGetLastError());
+}
+
+static void fdct1d_call(int type, int N, const int32_t *in, intptr_t is, int32_t *out, intptr_t os,
+                        int32_t line, int32_t shift)
+{
+    if (line <= 0 || line > 128 || shift < 0 || shift > 30) {
+        set_error(XG_ERR_ARG, "FDct 1-D: bad line %d or shift %d", line, shift);
+        return;
+    }
+    const size_t inB = align_up((size_t)line * N * 4, 256), outB = (size_t)line * N * 4;
+    CallCtx *c = call_ctx(inB + outB, inB + outB);
+    if (!c) return;
+    pack2d(c->host, in, is, N, line, 4);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, (size_t)line * N * 4, hipMemcpyHostToDevice, c->stream))) return;
+    if (!fdct1d_dev(c, (int32_t *)c->dev, type, N, line, shift, (int32_t *)(c->dev + inB))) return;
+    if (!XG_HIP(hipMemcpyAsync(c->host + inB, c->dev + inB, outB, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(out, os, c->host + inB, line, N, 4);
+}
+
+static int lg2(uint32_t v)
+{
+    int l = 0;
+    while ((1u << l) < v) l++;
+    return l;
+}
+
+// Two-pass transform on device with the layout of h266_forward_trans.c:114-128.
+static int32_t fdct2d_call(int hor, int ver, const int32_t *in, intptr_t is, int32_t *out, intptr_t os,
+                           uint32_t w, uint32_t h, int32_t s1, int32_t s2)
+{
+    const size_t n = (size_t)w * h, blk = align_up(n * 4, 256);
+    CallCtx *c = call_ctx(3 * blk, 2 * blk);
+    if (!c) return XG_ERR_HIP;
+    pack2d(c->host, in, is, (int)w, (int)h, 4);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, n * 4, hipMemcpyHostToDevice, c->stream))) return XG_ERR_HIP;
+    int32_t *d0 = (int32_t *)c->dev, *d1 = (int32_t *)(c->dev + blk), *d2 = (int32_t *)(c->dev + 2 * blk);
+    if (!fdct1d_dev(c, d0, hor, (int)w, (int)h, s1, d1)) return XG_ERR_HIP;   // tmp[k*h + y]
+    if (!fdct1d_dev(c, d1, ver, (int)h, (int)w, s2, d2)) return XG_ERR_HIP;   // out[j*w + k]
+    if (!XG_HIP(hipMemcpyAsync(c->host + blk, d2, n * 4, hipMemcpyDeviceToHost, c->stream))) return XG_ERR_HIP;
+    if (!finish(c)) return XG_ERR_HIP;
+    unpack2d(out, os, c->host + blk, (int)w, (int)h, 4);
+    return XG_OK;
+}
+
+}  // namespace xg
+
+using namespace xg;
+
+extern "C" {
+
+// ---- distortion ----
+void Xg266ComputeSad(xg_pixel *input, intptr_t inputStride, xg_pixel *pred, intptr_t predStride,
+                     uint32_t width, uint32_t height, uint32_t *sad)
+{
+    const uint16_t *r[1] = {pred};
+    sad_multi(input, inputStride, r, 1, predStride, width, height, sad, "ComputeSad");
+}
+
+void Xg266ComputeSatd(xg_pixel *input, intptr_t inputStride, xg_pixel *pred, intptr_t predStride,
+                      uint32_t width, uint32_t height, uint32_t *satd)
+{
+    satd_call(input, inputStride, pred, predStride, width, height, satd, "ComputeSatd");
+}
+
+void Xg266ComputeSadx8(xg_pixel *input, intptr_t inputStride, xg_pixel *ref[8], intptr_t refStride,
+                       uint32_t width, uint32_t height, uint32_t *sad)
+{
+    sad_multi(input, inputStride, (const uint16_t *const *)ref, 8, refStride, width, height, sad, "ComputeSadx8");
+}
+
+void Xg266ComputeSadx5(xg_pixel *input, intptr_t inputStride, xg_pixel *ref0, xg_pixel *ref1,
+                       xg_pixel *ref2, xg_pixel *ref3, xg_pixel *ref4, intptr_t refStride,
+                       uint32_t width, uint32_t height, uint32_t *sad)
+{
+    const uint16_t *r[5] = {ref0, ref1, ref2, ref3, ref4};
+    sad_multi(input, inputStride, r, 5, refStride, width, height, sad, "ComputeSadx5");
+}
+
+void Xg266ComputeSadx3(xg_pixel *input, intptr_t inputStride, xg_pixel *ref0, xg_pixel *ref1,
+                       xg_pixel *ref2, intptr_t refStride, uint32_t width, uint32_t height, uint32_t *sad)
+{
+    const uint16_t *r[3] = {ref0, ref1, ref2};
+    sad_multi(input, inputStride, r, 3, refStride, width, height, sad, "ComputeSadx3");
+}
+
+void Xg266ComputeHada8x8(xg_pixel *input, intptr_t inputStride, uint32_t *satd)
+{
+    static const xg_pixel zero[8] = {0};
+    satd_call(input, inputStride, (xg_pixel *)zero, 0, 8, 8, satd, "ComputeHada8x8");
+}
+
+// ---- luma interpolation ----
+#define XG_LUMA_U16(NAME, DIR, FX, FY)                                                              \
+    void NAME(const xg_pixel *src, intptr_t ss, xg_pixel *dst, intptr_t ds, int32_t frac,          \
+              uint32_t w, uint32_t h, uint32_t fi)                                                  \
+    {                                                                                               \
+        interp_u16(8, false, DIR, src, ss, dst, ds, FX, FY, (int)fi, w, h, #NAME);                  \
+    }
+#define XG_LUMA_S16(NAME, DIR, FX, FY)                                                              \
+    void NAME(const xg_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,           \
+              uint32_t w, uint32_t h, uint32_t fi)                                                  \
+    {                                                                                               \
+        interp_u16(8, true, DIR, src, ss, dst, ds, FX, FY, (int)fi, w, h, #NAME);                   \
+    }
+XG_LUMA_U16(Xg266InterpCopy, kCopy, 0, 0)
+XG_LUMA_U16(Xg266LumaInterpHor, kHor, frac & 15, 0)
+XG_LUMA_U16(Xg266LumaInterpVet, kVet, 0, (frac >> 4) & 15)
+XG_LUMA_U16(Xg266LumaInterpHorVet, kHorVet, frac & 15, (frac >> 4) & 15)
+XG_LUMA_S16(Xg266InterpCopyU8S16, kCopy, 0, 0)
+XG_LUMA_S16(Xg266LumaInterpHorU8S16, kHor, frac & 15, 0)
+XG_LUMA_S16(Xg266LumaInterpVetU8S16, kVet, 0, (frac >> 4) & 15)
+XG_LUMA_S16(Xg266LumaInterpHorVetU8S16, kHorVet, frac & 15, (frac >> 4) & 15)
+
+void Xg266LumaInterpVetS16U8(const int16_t *src, intptr_t ss, xg_pixel *dst, intptr_t ds, int32_t frac,
+                             uint32_t w, uint32_t h, uint32_t fi)
+{
+    interp_s16(8, false, src, ss, dst, ds, (frac >> 4) & 15, (int)fi, w, h, "Xg266LumaInterpVetS16U8");
+}
+void Xg266LumaInterpVetS16S16(const int16_t *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                              uint32_t w, uint32_t h, uint32_t fi)
+{
+    interp_s16(8, true, src, ss, dst, ds, (frac >> 4) & 15, (int)fi, w, h, "Xg266LumaInterpVetS16S16");
+}
+
+// ---- chroma interpolation (frac = (fy << 5) | fx) ----
+#define XG_CHROMA_U16(NAME, DIR, FX, FY)                                                            \
+    void NAME(const xg_pixel *src, intptr_t ss, xg_pixel *dst, intptr_t ds, int32_t frac,          \
+              uint32_t w, uint32_t h, uint32_t fi)                                                  \
+    {                                                                                               \
+        (void)fi;                                                                                   \
+        interp_u16(4, false, DIR, src, ss, dst, ds, FX, FY, 0, w, h, #NAME);                        \
+    }
+#define XG_CHROMA_S16(NAME, DIR, FX, FY)                                                            \
+    void NAME(const xg_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,           \
+              uint32_t w, uint32_t h, uint32_t fi)                                                  \
+    {                                                                                               \
+        (void)fi;                                                                                   \
+        interp_u16(4, true, DIR, src, ss, dst, ds, FX, FY, 0, w, h, #NAME);                         \
+    }
+XG_CHROMA_U16(Xg266ChromaInterpHor, kHor, frac & 31, 0)
+XG_CHROMA_U16(Xg266ChromaInterpVet, kVet, 0, (frac >> 5) & 31)
+XG_CHROMA_U16(Xg266ChromaInterpHorVet, kHorVet, frac & 31, (frac >> 5) & 31)
+XG_CHROMA_S16(Xg266ChromaInterpHorU8S16, kHor, frac & 31, 0)
+XG_CHROMA_S16(Xg266ChromaInterpVetU8S16, kVet, 0, (frac >> 5) & 31)
+XG_CHROMA_S16(Xg266ChromaInterpHorVetU8S16, kHorVet, frac & 31, (frac >> 5) & 31)
+
+void Xg266ChromaInterpVetS16U8(const int16_t *src, intptr_t ss, xg_pixel *dst, intptr_t ds, int32_t frac,
+                               uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    interp_s16(4, false, src, ss, dst, ds, (frac >> 5) & 31, 0, w, h, "Xg266ChromaInterpVetS16U8");
+}
+void Xg266ChromaInterpVetS16S16(const int16_t *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                                uint32_t w, uint32_t h, uint32_t fi)
+{
+    (void)fi;
+    interp_s16(4, true, src, ss, dst, ds, (frac >> 5) & 31, 0, w, h, "Xg266ChromaInterpVetS16S16");
+}
+
+void Xg266InterpAvgS16U8(const int16_t *src0, intptr_t src0Stride, const int16_t *src1, intptr_t src1Stride,
+                         xg_pixel *dst, intptr_t dstStride, uint32_t width, uint32_t height)
+{
+    bipred(src0, src0Stride, src1, src1Stride, dst, dstStride, width, height, 0, 0, 0, "Xg266InterpAvgS16U8");
+}
+
+void Xg266InterpWeightS16U8(const int16_t *src0, intptr_t src0Stride, const int16_t *src1, intptr_t src1Stride,
+                            xg_pixel *dst, intptr_t dstStride, uint32_t width, uint32_t height,
+                            int32_t weightA, int32_t weightB)
+{
+    bipred(src0, src0Stride, src1, src1Stride, dst, dstStride, width, height, 1, weightA, weightB,
+           "Xg266InterpWeightS16U8");
+}
+
+// ---- forward transforms ----
+#define XG_FDCT(NAME, TYPE, N)                                                                      \
+    void NAME(xg_coeff *in, intptr_t is, xg_coeff *out, intptr_t os, int32_t line, int32_t shift)  \
+    {                                                                                               \
+        fdct1d_call(TYPE, N, in, is, out, os, line, shift);                                         \
+    }
+XG_FDCT(Xg26xFDct2P2, kDCT2, 2)
+XG_FDCT(Xg26xFDct2P4, kDCT2, 4)
+XG_FDCT(Xg26xFDct2P8, kDCT2, 8)
+XG_FDCT(Xg26xFDct2P16, kDCT2, 16)
+XG_FDCT(Xg26xFDct2P32, kDCT2, 32)
+XG_FDCT(Xg26xFDct2P64, kDCT2, 64)
+XG_FDCT(Xg26xFDst7P4, kDST7, 4)
+XG_FDCT(Xg26xFDst7P8, kDST7, 8)
+XG_FDCT(Xg26xFDst7P16, kDST7, 16)
+XG_FDCT(Xg26xFDst7P32, kDST7, 32)
+XG_FDCT(Xg26xFDct8P4, kDCT8, 4)
+XG_FDCT(Xg26xFDct8P8, kDCT8, 8)
+XG_FDCT(Xg26xFDct8P16, kDCT8, 16)
+XG_FDCT(Xg26xFDct8P32, kDCT8, 32)
+
+void Xg26xFDct2WxH(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                   uint32_t width, uint32_t height, xg_coeff *tempBuf, int32_t shift1st, int32_t shift2nd)
+{
+    (void)tempBuf;
+    if (width < 2 || height < 2 || width > 64 || height > 64 || (width & (width - 1)) || (height & (height - 1))) {
+        set_error(XG_ERR_ARG, "Xg26xFDct2WxH: unsupported %ux%u", width, height);
+        return;
+    }
+    fdct2d_call(kDCT2, kDCT2, input, inputStride, output, outputStride, width, height, shift1st, shift2nd);
+}
+
+void Xg26xBlockSubForDct(xg_pixel *input, intptr_t inputStride, xg_pixel *recon, intptr_t reconStride,
+                         xg_coeff *diff, intptr_t diffStride, uint32_t width, uint32_t height)
+{
+    if (!check_wh(width, height, 128, 128, "Xg26xBlockSubForDct")) return;
+    const size_t n = (size_t)width * height, blk = align_up(n * 4, 256);
+    CallCtx *c = call_ctx(3 * blk, 3 * blk);
+    if (!c) return;
+    pack2d(c->host, input, inputStride, (int)width, (int)height, 2);
+    pack2d(c->host + blk, recon, reconStride, (int)width, (int)height, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, 2 * blk, hipMemcpyHostToDevice, c->stream))) return;
+    k_blocksub<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>((uint16_t *)c->dev, (uint16_t *)(c->dev + blk),
+                                                                 (int)n, (int32_t *)(c->dev + 2 * blk));
+    if (!XG_HIP(hipGetLastError())) return;
+    if (!XG_HIP(hipMemcpyAsync(c->host + 2 * blk, c->dev + 2 * blk, n * 4, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(diff, diffStride, c->host + 2 * blk, (int)width, (int)height, 4);
+}
+
+int32_t Xg266FDctWxH(const xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                     uint32_t width, uint32_t height, uint32_t mtsIdx, int32_t isIntraLuma, int32_t enableMts)
+{
+    static const int8_t tab[6][2] = {{kDCT2, kDCT2}, {kSKIP, kSKIP}, {kDST7, kDST7},
+                                     {kDCT8, kDST7}, {kDST7, kDCT8}, {kDCT8, kDCT8}};
+    if (width < 2 || height < 2 || width > 64 || height > 64 || (width & (width - 1)) ||
+        (height & (height - 1)) || mtsIdx > 5 || mtsIdx == 1) {
+        set_error(XG_ERR_ARG, "Xg266FDctWxH: unsupported %ux%u mts %u", width, height, mtsIdx);
+        return XG_ERR_ARG;
+    }
+    int hor = tab[mtsIdx][0], ver = tab[mtsIdx][1];
+    if (isIntraLuma && enableMts && mtsIdx == 0) {         // h266_forward_trans.c:75-86
+        if (width >= 4 && width <= 16) hor = kDST7;
+        if (height >= 4 && height <= 16) ver = kDST7;
+    }
+    if ((hor != kDCT2 && (width < 4 || width > 32)) || (ver != kDCT2 && (height < 4 || height > 32))) {
+        set_error(XG_ERR_ARG, "Xg266FDctWxH: DST-VII/DCT-VIII need sizes 4..32");
+        return XG_ERR_ARG;
+    }
+    const int lw = lg2(width), lh = lg2(height);
+    return fdct2d_call(hor, ver, input, inputStride, output, outputStride, width, height,
+                       lw + 10 + 6 - 15, lh + 6);                         // :72-73
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Table population following h266_encoder/source/h266_func_init.c:87-690 (C
+// defaults: SATD NULL for widths 1 and 2 at :648-649; DST-VII/DCT-VIII 4..32
+// and DCT-II 2..64 at :349-374; 2-D DCT-II for every [lgW][lgH] at :72-85)
+// followed by the SAD/SATD binding of Xin266FuncUpdate
+// (h266_encoder/source/h266_encoder_create.c:3332-3400).
+int32_t Xg266FuncInit(xg_func_struct *f, int32_t satdMd)
+{
+    if (!f) {
+        set_error(XG_ERR_ARG, "Xg266FuncInit: NULL table");
+        return XG_ERR_ARG;
+    }
+    memset(f, 0, sizeof(*f));
+    if (!device_init()) return XG_ERR_HIP;
+    static const XgFor1dDct dct2[XG_BLOCK_NUM] = {nullptr, Xg26xFDct2P2, Xg26xFDct2P4, Xg26xFDct2P8,
+                                                  Xg26xFDct2P16, Xg26xFDct2P32, Xg26xFDct2P64, nullptr};
+    static const XgFor1dDct dct8[XG_BLOCK_NUM] = {nullptr, nullptr, Xg26xFDct8P4, Xg26xFDct8P8,
+                                                  Xg26xFDct8P16, Xg26xFDct8P32, nullptr, nullptr};
+    static const XgFor1dDct dst7[XG_BLOCK_NUM] = {nullptr, nullptr, Xg26xFDst7P4, Xg26xFDst7P8,
+                                                  Xg26xFDst7P16, Xg26xFDst7P32, nullptr, nullptr};
+    for (int i = 0; i < XG_BLOCK_NUM; i++) {
+        f->pfXinComputeSad[i] = Xg266ComputeSad;
+        f->pfXinComputeSatd[i] = i >= 2 ? Xg266ComputeSatd : nullptr;
+        f->pfXinComputeDist[i] = satdMd ? f->pfXinComputeSatd[i] : f->pfXinComputeSad[i];
+        f->pfXinComputeSadx8[i] = Xg266ComputeSadx8;
+        f->pfXinComputeSadx5[i] = Xg266ComputeSadx5;
+        f->pfXinComputeSadx3[i] = Xg266ComputeSadx3;
+        f->pfXinLumaInterp[0][0][i] = Xg266InterpCopy;
+        f->pfXinLumaInterp[1][0][i] = Xg266LumaInterpHor;
+        f->pfXinLumaInterp[0][1][i] = Xg266LumaInterpVet;
+        f->pfXinLumaInterp[1][1][i] = Xg266LumaInterpHorVet;
+        f->pfXinLumaInterpS16[0][0][i] = Xg266InterpCopyU8S16;
+        f->pfXinLumaInterpS16[1][0][i] = Xg266LumaInterpHorU8S16;
+        f->pfXinLumaInterpS16[0][1][i] = Xg266LumaInterpVetU8S16;
+        f->pfXinLumaInterpS16[1][1][i] = Xg266LumaInterpHorVetU8S16;
+        f->pfXinChromaInterp[0][0][i] = Xg266InterpCopy;
+        f->pfXinChromaInterp[1][0][i] = Xg266ChromaInterpHor;
+        f->pfXinChromaInterp[0][1][i] = Xg266ChromaInterpVet;
+        f->pfXinChromaInterp[1][1][i] = Xg266ChromaInterpHorVet;
+        f->pfXinChromaInterpS16[0][0][i] = Xg266InterpCopyU8S16;
+        f->pfXinChromaInterpS16[1][0][i] = Xg266ChromaInterpHorU8S16;
+        f->pfXinChromaInterpS16[0][1][i] = Xg266ChromaInterpVetU8S16;
+        f->pfXinChromaInterpS16[1][1][i] = Xg266ChromaInterpHorVetU8S16;
+        f->pfXinInterpAvg[i] = Xg266InterpAvgS16U8;
+        f->pfXinInterpWeight[i] = Xg266InterpWeightS16U8;
+        f->pfXinFor1dDct[0][i] = dct2[i];   // XIN_DCT2
+        f->pfXinFor1dDct[1][i] = dct8[i];   // XIN_DCT8
+        f->pfXinFor1dDct[2][i] = dst7[i];   // XIN_DST7
+        f->pfXinBlockSubForDct[i] = Xg26xBlockSubForDct;
+        for (int j = 0; j < XG_BLOCK_NUM - 1; j++)
+            if (i < XG_BLOCK_NUM - 1) f->pfXinFor2dDct2[i][j] = Xg26xFDct2WxH;
+    }
+    f->pfXinComputeHada8x8 = Xg266ComputeHada8x8;
+    return XG_OK;
+}
+
+}  // extern "C"

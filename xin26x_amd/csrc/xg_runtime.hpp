@@ -1,0 +1,48 @@
+// xg_runtime.hpp -- host runtime shared by the C-ABI entry points: error
+// state, logging, HIP checks and the per-thread call context (stream plus
+// device and pinned staging buffers) used by the synchronous per-call
+// primitives.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/xg266.h"
+
+namespace xg {
+
+void set_error(int32_t code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void log_msg(int32_t level, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// Returns true when e == hipSuccess; otherwise records XG_ERR_HIP.
+bool hip_ok(hipError_t e, const char *what);
+
+#define XG_HIP(call) ::xg::hip_ok((call), #call)
+
+// Per-thread context of the synchronous per-call primitives.  The reference
+// calls its primitives from every worker thread concurrently
+// (h266_encode_section.c:63), so each host thread owns its stream and
+// staging buffers; nothing mutable is shared between threads.
+struct CallCtx {
+    hipStream_t stream = nullptr;
+    char *dev = nullptr;   size_t devCap = 0;
+    char *host = nullptr;  size_t hostCap = 0;
+};
+
+// Returns the calling thread's context with at least devBytes of device
+// scratch and hostBytes of pinned host staging, or nullptr (error recorded).
+CallCtx *call_ctx(size_t devBytes, size_t hostBytes);
+
+// Ensures the HIP runtime and device are usable and the device-side constant
+// tables (transform matrices) are initialised.  Idempotent, thread-safe.
+bool device_init();
+
+// 2-D copies between strided host memory and packed staging memory.
+void pack2d(void *dst, const void *src, intptr_t srcStride, int w, int h, size_t elem);
+void unpack2d(void *dst, intptr_t dstStride, const void *src, int w, int h, size_t elem);
+
+// Releases the forward-transform plan a session owns (xg_fdct.hip).
+void fdct_release(xg_session *s);
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace xg

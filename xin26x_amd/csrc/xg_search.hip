@@ -1,0 +1,394 @@
+// xg_search.hip -- integer and fractional motion search.
+//
+//  * k_full_search: exhaustive integer search of one block per workgroup; the
+//    block's reference window (every candidate position plus the block
+//    extent) and the current block are staged once in LDS; each thread costs
+//    a strided subset of candidates; the argmin is a 64-bit min over
+//    (cost << 32 | raster index), so ties resolve to the earliest candidate
+//    in raster order (dy outer, dx inner) whatever the thread schedule.
+//    Cost = (SAD << 8) + (EG(4*dx - pmx) + EG(4*dy - pmy)) * lambda
+//    (XIN_COST_FRACTION 8, common/include/h26x_definition.h:88; MVD in
+//    quarter pel as CALC_MVD_BIT is used at h266_sub_pel_refine.c:155-156).
+//  * Xg26xMotionEstDia: start candidates then small-diamond descent, one
+//    workgroup, SADs reduced across its 256 threads.
+//  * Xg266FullSubPelMe: the half-pel and quarter-pel rings of
+//    h266_encoder/source/h266_sub_pel_refine.c:81-212 with the 8-tap luma
+//    interpolation (filterIndex 0) and SAD.
+#include <algorithm>
+#include <string.h>
+
+#include "xg_device.hpp"
+#include "xg_runtime.hpp"
+
+namespace xg {
+
+__device__ __forceinline__ unsigned long long block_min_u64(unsigned long long v, unsigned long long *red)
+{
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const unsigned long long o = (unsigned long long)__shfl_xor((long long)v, m, 64);
+        v = o < v ? o : v;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    unsigned long long r = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) r = red[i] < r ? red[i] : r;
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *red)
+{
+    v = (uint32_t)group_sum<64>((int)v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    uint32_t r = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += red[i];
+    __syncthreads();
+    return r;
+}
+
+// One block's full search over window [x0,x1] x [y0,y1] (integer MVs).
+// win: reference samples from (x0, y0) relative to the block, stride ws;
+// cur: the block, stride bw.  Both in LDS.
+__device__ unsigned long long full_search_core(const uint16_t *win, int ws, const uint16_t *cur, int bw, int bh,
+                                               int x0, int x1, int y0, int y1, uint32_t lambda, int pmx, int pmy,
+                                               unsigned long long *red)
+{
+    const int nx = x1 - x0 + 1, n = nx * (y1 - y0 + 1);
+    unsigned long long best = ~0ull;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int ox = i % nx, oy = i / nx;
+        const uint16_t *w = win + oy * ws + ox;
+        uint32_t sad = 0;
+        for (int r = 0; r < bh; r++) {
+            const uint16_t *wr = w + r * ws, *cr = cur + r * bw;
+            for (int c = 0; c < bw; c++) sad += (uint32_t)abs((int)cr[c] - (int)wr[c]);
+        }
+        const int mx = x0 + ox, my = y0 + oy;
+        const uint32_t cost = (sad << 8) + (eg_bits(4 * mx - pmx) + eg_bits(4 * my - pmy)) * lambda;
+        const unsigned long long key = ((unsigned long long)cost << 32) | (uint32_t)i;
+        best = key < best ? key : best;
+    }
+    return block_min_u64(best, red);
+}
+
+struct FsFrameArgs {
+    const uint16_t *cur; intptr_t cs;
+    const uint16_t *ref; intptr_t rs;
+    int fw, fh, bw, bh, range, ctu, nbx;
+    uint32_t lambda;
+    int16_t *bestMv; uint32_t *bestCost; uint32_t *bestSad;
+};
+
+// Frame form: block b of the bw x bh grid, predictor (0,0), window
+// [-range, range]^2 clipped to the CU MV window (h266_enc_init.c:1462-1465).
+__global__ void __launch_bounds__(256) k_full_search_frame(FsFrameArgs a)
+{
+    extern __shared__ uint16_t smem[];
+    __shared__ unsigned long long red[4];
+    const int b = blockIdx.x;
+    const int x = (b % a.nbx) * a.bw, y = (b / a.nbx) * a.bh;
+    const int mnx = (int16_t)(-x - a.ctu - 8), mny = (int16_t)(-y - a.ctu - 8);
+    const int mxx = (int16_t)(a.fw + (a.ctu - a.bw + 8) - x), mxy = (int16_t)(a.fh + (a.ctu - a.bh + 8) - y);
+    const int x0 = max(-a.range, mnx), x1 = min(a.range, mxx);
+    const int y0 = max(-a.range, mny), y1 = min(a.range, mxy);
+    const int ws = x1 - x0 + a.bw, wh = y1 - y0 + a.bh;
+    uint16_t *win = smem, *cur = smem + ws * wh;
+    const uint16_t *rp = a.ref + (intptr_t)(y + y0) * a.rs + (x + x0);
+    for (int i = threadIdx.x; i < ws * wh; i += blockDim.x) win[i] = rp[(intptr_t)(i / ws) * a.rs + (i % ws)];
+    const uint16_t *cp = a.cur + (intptr_t)y * a.cs + x;
+    for (int i = threadIdx.x; i < a.bw * a.bh; i += blockDim.x) cur[i] = cp[(intptr_t)(i / a.bw) * a.cs + (i % a.bw)];
+    __syncthreads();
+    const unsigned long long k = full_search_core(win, ws, cur, a.bw, a.bh, x0, x1, y0, y1, a.lambda, 0, 0, red);
+    if (threadIdx.x == 0) {
+        const int i = (int)(uint32_t)k, nx = x1 - x0 + 1;
+        const int mx = x0 + i % nx, my = y0 + i / nx;
+        a.bestMv[2 * b] = (int16_t)(4 * mx);
+        a.bestMv[2 * b + 1] = (int16_t)(4 * my);
+        a.bestCost[b] = (uint32_t)(k >> 32);
+        uint32_t sad = 0;
+        const uint16_t *w = win + (my - y0) * ws + (mx - x0);
+        for (int r = 0; r < a.bh; r++)
+            for (int c = 0; c < a.bw; c++) sad += (uint32_t)abs((int)cur[r * a.bw + c] - (int)w[r * ws + c]);
+        a.bestSad[b] = sad;
+    }
+}
+
+// Per-call form: packed window and block already in global memory.
+__global__ void __launch_bounds__(256) k_full_search_one(const uint16_t *gwin, int ws, int wh, const uint16_t *gcur,
+                                                         int bw, int bh, int x0, int x1, int y0, int y1,
+                                                         uint32_t lambda, int pmx, int pmy, uint32_t *out)
+{
+    extern __shared__ uint16_t smem[];
+    __shared__ unsigned long long red[4];
+    uint16_t *win = smem, *cur = smem + ws * wh;
+    for (int i = threadIdx.x; i < ws * wh; i += blockDim.x) win[i] = gwin[i];
+    for (int i = threadIdx.x; i < bw * bh; i += blockDim.x) cur[i] = gcur[i];
+    __syncthreads();
+    const unsigned long long k = full_search_core(win, ws, cur, bw, bh, x0, x1, y0, y1, lambda, pmx, pmy, red);
+    if (threadIdx.x == 0) {
+        const int i = (int)(uint32_t)k, nx = x1 - x0 + 1;
+        const int mx = x0 + i % nx, my = y0 + i / nx;
+        uint32_t sad = 0;
+        const uint16_t *w = win + (my - y0) * ws + (mx - x0);
+        for (int r = 0; r < bh; r++)
+            for (int c = 0; c < bw; c++) sad += (uint32_t)abs((int)cur[r * bw + c] - (int)w[r * ws + c]);
+        out[0] = (uint32_t)mx;
+        out[1] = (uint32_t)my;
+        out[2] = (uint32_t)(k >> 32);
+        out[3] = sad;
+    }
+}
+
+// SAD of the block at integer offset (mx, my) inside a packed region whose
+// origin is (rx0, ry0) relative to the co-located position; block-wide sum.
+__device__ uint32_t region_sad(const uint16_t *reg, int rstride, int rx0, int ry0, const uint16_t *cur, int bw,
+                               int bh, int mx, int my, uint32_t *red)
+{
+    const uint16_t *w = reg + (my - ry0) * rstride + (mx - rx0);
+    uint32_t s = 0;
+    for (int i = threadIdx.x; i < bw * bh; i += blockDim.x) {
+        const int r = i / bw, c = i % bw;
+        s += (uint32_t)abs((int)cur[i] - (int)w[r * rstride + c]);
+    }
+    return block_sum_u32(s, red);
+}
+
+// Diamond search in one workgroup; the region covers every position the
+// search can read.  out: bestX, bestY (integer pel), cost, sad.
+__global__ void __launch_bounds__(256) k_dia_one(const uint16_t *reg, int rstride, int rx0, int ry0,
+                                                 const uint16_t *cur, int bw, int bh, const int16_t *cand,
+                                                 int ncand, int mnx, int mny, int mxx, int mxy, int cx, int cy,
+                                                 int sw, int sh, uint32_t lambda, int pmx, int pmy, uint32_t *out)
+{
+    __shared__ uint32_t red[4];
+    uint32_t best = 0xFFFFFFFFu, bestSad = 0xFFFFFFFFu;
+    int bx = 0, by = 0;
+    for (int i = 0; i < ncand; i++) {
+        const int mx = min(max((int)cand[2 * i], mnx), mxx), my = min(max((int)cand[2 * i + 1], mny), mxy);
+        const uint32_t sad = region_sad(reg, rstride, rx0, ry0, cur, bw, bh, mx, my, red);
+        const uint32_t c = (sad << 8) + (eg_bits(4 * mx - pmx) + eg_bits(4 * my - pmy)) * lambda;
+        if (c < best) { best = c; bestSad = sad; bx = mx; by = my; }
+    }
+    const int dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};
+    for (int it = 0; it < sw + sh; it++) {
+        int nx = bx, ny = by;
+        uint32_t nb = best, ns = bestSad;
+        for (int k = 0; k < 4; k++) {
+            const int mx = bx + dxs[k], my = by + dys[k];
+            if (mx < mnx || mx > mxx || my < mny || my > mxy) continue;
+            if (mx < cx - sw || mx > cx + sw || my < cy - sh || my > cy + sh) continue;
+            const uint32_t sad = region_sad(reg, rstride, rx0, ry0, cur, bw, bh, mx, my, red);
+            const uint32_t c = (sad << 8) + (eg_bits(4 * mx - pmx) + eg_bits(4 * my - pmy)) * lambda;
+            if (c < nb) { nb = c; ns = sad; nx = mx; ny = my; }
+        }
+        if (nx == bx && ny == by) break;
+        bx = nx; by = ny; best = nb; bestSad = ns;
+    }
+    if (threadIdx.x == 0) {
+        out[0] = (uint32_t)bx;
+        out[1] = (uint32_t)by;
+        out[2] = best;
+        out[3] = bestSad;
+    }
+}
+
+// Half-pel then quarter-pel ring around (qx, qy) (quarter pel).  reg: luma
+// samples whose first row/column is (rx0, ry0) relative to the co-located
+// block position (the host stages integer positions -1..+1 plus 8-tap reach).
+__global__ void __launch_bounds__(256) k_subpel_one(const uint16_t *reg, int rstride, int rx0, int ry0,
+                                                    const uint16_t *cur, int bw, int bh, int qx, int qy,
+                                                    uint32_t bestCost, int pmx, int pmy, uint32_t lambda,
+                                                    uint32_t *out)
+{
+    __shared__ uint32_t red[4];
+    const int ring_x[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, ring_y[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+    int mx = qx, my = qy;
+    uint32_t best = bestCost;
+    for (int pass = 0; pass < 2; pass++) {
+        const int step = pass == 0 ? 2 : 1;
+        const int cx = mx, cy = my;
+        for (int k = 0; k < 8; k++) {
+            const int tx = cx + ring_x[k] * step, ty = cy + ring_y[k] * step;
+            const int m16x = tx * 4, m16y = ty * 4;
+            const int8_t *ch = c_luma[0][m16x & 15], *cv = c_luma[0][m16y & 15];
+            const int ix = (m16x >> 4) - 3 - rx0, iy = (m16y >> 4) - 3 - ry0;
+            uint32_t s = 0;
+            for (int i = threadIdx.x; i < bw * bh; i += blockDim.x) {
+                const int r = i / bw, c = i % bw;
+                const uint16_t *p = reg + (iy + r) * rstride + ix + c;
+                int acc = 0;
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc += (int)cv[j] * hpass<8>(p + j * rstride, ch);
+                s += (uint32_t)abs((int)cur[i] - vfinal_pix(acc));
+            }
+            const uint32_t sad = block_sum_u32(s, red);
+            const uint32_t cost = (sad << 8) + (eg_bits(tx - pmx) + eg_bits(ty - pmy)) * lambda;
+            if (cost < best) { best = cost; mx = tx; my = ty; }
+        }
+    }
+    if (threadIdx.x == 0) {
+        out[0] = (uint32_t)mx;
+        out[1] = (uint32_t)my;
+        out[2] = best;
+    }
+}
+
+static inline int clampi_h(int v, int lo, int hi) { return std::min(std::max(v, lo), hi); }
+
+}  // namespace xg
+
+using namespace xg;
+
+extern "C" {
+
+void Xg26xMotionEstFull(xg_me_struct *me)
+{
+    if (!me || !me->input || !me->ref || me->width == 0 || me->height == 0 || me->width > 128 || me->height > 128) {
+        set_error(XG_ERR_ARG, "Xg26xMotionEstFull: bad arguments");
+        return;
+    }
+    const int bw = (int)me->width, bh = (int)me->height;
+    const int cx = (me->predMv.mvX + 2) >> 2, cy = (me->predMv.mvY + 2) >> 2;
+    const int x0 = std::max(cx - me->searchWidth, (int)me->minMv.mvX), x1 = std::min(cx + me->searchWidth, (int)me->maxMv.mvX);
+    const int y0 = std::max(cy - me->searchHeight, (int)me->minMv.mvY), y1 = std::min(cy + me->searchHeight, (int)me->maxMv.mvY);
+    if (x1 < x0 || y1 < y0) {
+        set_error(XG_ERR_ARG, "Xg26xMotionEstFull: empty search window");
+        return;
+    }
+    const int ws = x1 - x0 + bw, wh = y1 - y0 + bh;
+    const size_t lds = ((size_t)ws * wh + (size_t)bw * bh) * 2;
+    if (lds > 60 * 1024) {
+        set_error(XG_ERR_ARG, "Xg26xMotionEstFull: window %dx%d exceeds the LDS budget", ws, wh);
+        return;
+    }
+    const size_t wB = align_up((size_t)ws * wh * 2, 256), cB = align_up((size_t)bw * bh * 2, 256);
+    CallCtx *c = call_ctx(wB + cB + 256, wB + cB + 256);
+    if (!c) return;
+    pack2d(c->host, me->ref + (intptr_t)y0 * me->refStride + x0, me->refStride, ws, wh, 2);
+    pack2d(c->host + wB, me->input, me->inputStride, bw, bh, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, wB + cB, hipMemcpyHostToDevice, c->stream))) return;
+    uint32_t *dout = (uint32_t *)(c->dev + wB + cB);
+    k_full_search_one<<<1, 256, lds, c->stream>>>((uint16_t *)c->dev, ws, wh, (uint16_t *)(c->dev + wB), bw, bh,
+                                                  x0, x1, y0, y1, me->sadLambda, me->predMv.mvX,
+                                                  me->predMv.mvY, dout);
+    if (!XG_HIP(hipGetLastError())) return;
+    uint32_t *hout = (uint32_t *)(c->host + wB + cB);
+    if (!XG_HIP(hipMemcpyAsync(hout, dout, 16, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!XG_HIP(hipStreamSynchronize(c->stream))) return;
+    me->bestMv.mvX = (int16_t)(4 * (int)hout[0]);
+    me->bestMv.mvY = (int16_t)(4 * (int)hout[1]);
+    me->bestCost = hout[2];
+    me->bestSad = hout[3];
+}
+
+void Xg26xMotionEstDia(xg_me_struct *me)
+{
+    if (!me || !me->input || !me->ref || me->width == 0 || me->height == 0 || me->width > 128 ||
+        me->height > 128 || me->candNum == 0 || me->candNum > 12) {
+        set_error(XG_ERR_ARG, "Xg26xMotionEstDia: bad arguments");
+        return;
+    }
+    const int bw = (int)me->width, bh = (int)me->height;
+    const int cx = (me->predMv.mvX + 2) >> 2, cy = (me->predMv.mvY + 2) >> 2;
+    const int mnx = me->minMv.mvX, mny = me->minMv.mvY, mxx = me->maxMv.mvX, mxy = me->maxMv.mvY;
+    // bounding box of every position the search may read
+    int x0 = std::max(cx - me->searchWidth, mnx), x1 = std::min(cx + me->searchWidth, mxx);
+    int y0 = std::max(cy - me->searchHeight, mny), y1 = std::min(cy + me->searchHeight, mxy);
+    int16_t cand[24];
+    for (uint32_t i = 0; i < me->candNum; i++) {
+        const int mx = clampi_h(me->meCand[i].mvX, mnx, mxx), my = clampi_h(me->meCand[i].mvY, mny, mxy);
+        cand[2 * i] = me->meCand[i].mvX;
+        cand[2 * i + 1] = me->meCand[i].mvY;
+        x0 = std::min(x0, mx); x1 = std::max(x1, mx);
+        y0 = std::min(y0, my); y1 = std::max(y1, my);
+        // a descent step can leave the window only from a candidate inside it
+    }
+    const int rw = x1 - x0 + bw, rh = y1 - y0 + bh;
+    const size_t rB = align_up((size_t)rw * rh * 2, 256), cB = align_up((size_t)bw * bh * 2, 256);
+    CallCtx *c = call_ctx(rB + cB + 512, rB + cB + 512);
+    if (!c) return;
+    pack2d(c->host, me->ref + (intptr_t)y0 * me->refStride + x0, me->refStride, rw, rh, 2);
+    pack2d(c->host + rB, me->input, me->inputStride, bw, bh, 2);
+    memcpy(c->host + rB + cB, cand, sizeof(cand));
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, rB + cB + sizeof(cand), hipMemcpyHostToDevice, c->stream))) return;
+    uint32_t *dout = (uint32_t *)(c->dev + rB + cB + 256);
+    k_dia_one<<<1, 256, 0, c->stream>>>((uint16_t *)c->dev, rw, x0, y0, (uint16_t *)(c->dev + rB), bw, bh,
+                                        (int16_t *)(c->dev + rB + cB), (int)me->candNum, mnx, mny, mxx, mxy, cx, cy,
+                                        me->searchWidth, me->searchHeight, me->sadLambda, me->predMv.mvX,
+                                        me->predMv.mvY, dout);
+    if (!XG_HIP(hipGetLastError())) return;
+    uint32_t *hout = (uint32_t *)(c->host + rB + cB + 256);
+    if (!XG_HIP(hipMemcpyAsync(hout, dout, 16, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!XG_HIP(hipStreamSynchronize(c->stream))) return;
+    me->bestMv.mvX = (int16_t)(4 * (int)hout[0]);
+    me->bestMv.mvY = (int16_t)(4 * (int)hout[1]);
+    me->bestCost = hout[2];
+    me->bestSad = hout[3];
+}
+
+void Xg266FullSubPelMe(xg_me_struct *me)
+{
+    if (!me || !me->input || !me->ref || me->width == 0 || me->height == 0 || me->width > 128 || me->height > 128) {
+        set_error(XG_ERR_ARG, "Xg266FullSubPelMe: bad arguments");
+        return;
+    }
+    const int qx = me->bestMv.mvX, qy = me->bestMv.mvY;
+    // h266_sub_pel_refine.c:125-128: leave the MV alone when it is out of range
+    if ((qx >> 2) < me->minMv.mvX || (qy >> 2) < me->minMv.mvY || (qx >> 2) > me->maxMv.mvX ||
+        (qy >> 2) > me->maxMv.mvY)
+        return;
+    const int bw = (int)me->width, bh = (int)me->height;
+    // reads span integer positions (qx>>2)-1 .. (qx>>2)+1 plus the 8-tap reach
+    const int rx0 = (qx >> 2) - 1 - 3, ry0 = (qy >> 2) - 1 - 3;
+    const int rw = bw + 10, rh = bh + 10;
+    const size_t rB = align_up((size_t)rw * rh * 2, 256), cB = align_up((size_t)bw * bh * 2, 256);
+    CallCtx *c = call_ctx(rB + cB + 256, rB + cB + 256);
+    if (!c) return;
+    pack2d(c->host, me->ref + (intptr_t)ry0 * me->refStride + rx0, me->refStride, rw, rh, 2);
+    pack2d(c->host + rB, me->input, me->inputStride, bw, bh, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, rB + cB, hipMemcpyHostToDevice, c->stream))) return;
+    uint32_t *dout = (uint32_t *)(c->dev + rB + cB);
+    k_subpel_one<<<1, 256, 0, c->stream>>>((uint16_t *)c->dev, rw, rx0, ry0, (uint16_t *)(c->dev + rB), bw, bh,
+                                           qx, qy, me->bestCost, me->predMv.mvX, me->predMv.mvY, me->sadLambda, dout);
+    if (!XG_HIP(hipGetLastError())) return;
+    uint32_t *hout = (uint32_t *)(c->host + rB + cB);
+    if (!XG_HIP(hipMemcpyAsync(hout, dout, 12, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!XG_HIP(hipStreamSynchronize(c->stream))) return;
+    me->bestMv.mvX = (int16_t)hout[0];
+    me->bestMv.mvY = (int16_t)hout[1];
+    me->bestCost = hout[2];
+}
+
+int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t bw, int32_t bh,
+                             int32_t range, uint32_t sadLambda, int32_t ctuSize, int16_t *bestMv,
+                             uint32_t *bestCost, uint32_t *bestSad)
+{
+    if (!s || !cur || !ref || !bestMv || !bestCost || !bestSad || bw < 4 || bh < 4 || bw > 64 || bh > 64 ||
+        range < 0 || range > 64 || cur->width != ref->width || cur->height != ref->height ||
+        ref->pad < range + 4) {
+        set_error(XG_ERR_ARG, "xg_full_search_frame: bad arguments");
+        return XG_ERR_ARG;
+    }
+    const int nbx = cur->width / bw, nby = cur->height / bh;
+    if (nbx <= 0 || nby <= 0) return XG_OK;
+    const size_t lds = ((size_t)(2 * range + bw) * (2 * range + bh) + (size_t)bw * bh) * 2;
+    if (lds > 64 * 1024) {
+        set_error(XG_ERR_ARG, "xg_full_search_frame: window exceeds the LDS budget");
+        return XG_ERR_ARG;
+    }
+    FsFrameArgs a;
+    a.cur = cur->data; a.cs = cur->stride;
+    a.ref = ref->data; a.rs = ref->stride;
+    a.fw = cur->width; a.fh = cur->height; a.bw = bw; a.bh = bh; a.range = range; a.ctu = ctuSize; a.nbx = nbx;
+    a.lambda = sadLambda;
+    a.bestMv = bestMv; a.bestCost = bestCost; a.bestSad = bestSad;
+    hipStream_t st = (hipStream_t)xg_session_stream(s);
+    k_full_search_frame<<<nbx * nby, 256, lds, st>>>(a);
+    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
+}
+
+}  // extern "C"
