@@ -90,6 +90,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="profiling run: batched form only (no plain form, latency loop or CPU leg)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -102,6 +106,15 @@ def main():
 
     torch.cuda.set_device(0)
     lib = capi.load()
+    if args.calib_copy_mib:
+        nbytes = args.calib_copy_mib << 20
+        a = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
+        del a, b
     log("generating the 1080p workload")
     fp = wl.frame_pair_1080p()
     W, H = fp.width, fp.height
@@ -142,23 +155,27 @@ def main():
         capi.check(lib.xg_session_read(s, c.ctypes.data, b.ctypes.data), "read")
         return c, b, pred.planes()
 
-    log("plain form")
-    t_plain = timed(capi.MODE_PLAIN)
-    out_plain = outputs()
-    log("batched form (graph)")
-    t_graph = timed(capi.MODE_GRAPH)
-    out_graph = outputs()
-    exact = all(np.array_equal(a, b) for a, b in zip(out_plain[:2], out_graph[:2])) and \
-        all(np.array_equal(a, b) for a, b in zip(out_plain[2], out_graph[2]))
+    if args.quick:
+        t_graph = timed(capi.MODE_GRAPH)
+        t_plain, exact, lat = float("nan"), None, np.array([float("nan")])
+    else:
+        log("plain form")
+        t_plain = timed(capi.MODE_PLAIN)
+        out_plain = outputs()
+        log("batched form (graph)")
+        t_graph = timed(capi.MODE_GRAPH)
+        out_graph = outputs()
+        exact = all(np.array_equal(a, b) for a, b in zip(out_plain[:2], out_graph[:2])) and \
+            all(np.array_equal(a, b) for a, b in zip(out_plain[2], out_graph[2]))
 
-    # per-frame latency: submit one frame, wait for it
-    lat = []
-    for _ in range(max(args.steps, 20)):
-        t0 = time.perf_counter()
-        run(capi.MODE_GRAPH, 1)
-        capi.check(lib.xg_session_sync(s), "sync")
-        lat.append((time.perf_counter() - t0) * 1e3)
-    lat = np.array(lat)
+        # per-frame latency: submit one frame, wait for it
+        lat = []
+        for _ in range(max(args.steps, 20)):
+            t0 = time.perf_counter()
+            run(capi.MODE_GRAPH, 1)
+            capi.check(lib.xg_session_sync(s), "sync")
+            lat.append((time.perf_counter() - t0) * 1e3)
+        lat = np.array(lat)
 
     # dominant kernel (fractional refinement) timed with HIP events on the session stream
     capi.check(lib.xg_session_set_timing(s, 1), "timing")
@@ -180,7 +197,7 @@ def main():
     plain = frame_px / t_plain / 1e6
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.quick:
         log("CPU oracle baseline")
         cpu = cpu_baseline(fp, args.cpu_seconds, offs, wl.LAMBDA_REFINE)
 
@@ -204,7 +221,7 @@ def main():
                    "parallelism": "single GPU"},
         "plain_value": round(plain, 2),
         "batched_vs_plain": round(value / plain, 3),
-        "bit_exact_batched_vs_plain": bool(exact),
+        "bit_exact_batched_vs_plain": exact,
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "block_cand_evals_per_s": round(n * nc / t_graph, 1),

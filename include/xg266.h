@@ -326,8 +326,16 @@ int32_t xg_session_set_blocks(xg_session *s, const xg_block *blocks, int32_t n);
 /* Per-shape tile choice: candidates per task for shape (lgW-2, lgH-2). */
 int32_t xg_session_set_tile(xg_session *s, int32_t lgW, int32_t lgH, int32_t candPerTask);
 int32_t xg_session_get_tile(xg_session *s, int32_t lgW, int32_t lgH);
-/* Times every legal tile per shape on the installed blocks/pictures and keeps
- * the fastest; writes the table to `path` when not NULL. */
+/* Scheduling of the batched form: `regions` spatial work queues (1..8; each
+ * workgroup drains the queue of its XCD's region first, then steals) and
+ * persistent workgroups per CU. */
+int32_t xg_session_set_sched(xg_session *s, int32_t regions, int32_t wgPerCU);
+/* Tunes the batched form on the installed blocks and pictures: first the
+ * scheduling (spatial queues x workgroups per CU), then candidates per task
+ * for each block shape present by coordinate descent, heaviest shape first;
+ * every choice is timed on whole-frame launches (costs only) with HIP events.
+ * Writes the schedule and tile table to `path` when not NULL
+ * (xg_session_load_tiles reads it back).  Results never depend on the tiles. */
 int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t reps, const char *path);
 int32_t xg_session_load_tiles(xg_session *s, const char *path);
 

@@ -131,14 +131,25 @@ def test_refine_edges_clipping_and_luma_only(oracle, frame):
     g = Gpu(frame, blocks)
     try:
         cfg = refine_cfg(offs, 12345, use_chroma=0)
-        got = g.run(cfg, capi.MODE_BATCHED)
-        cost, best, pred = got
+        cost, best, _ = g.run(cfg, capi.MODE_BATCHED)
         assert np.array_equal(cost, exp[0])
         assert np.array_equal(best[:, 3], exp[1])
-        # the MC stage writes chroma too; compare luma prediction where blocks lie
-        for b in blocks:
-            x, y, w, h = (int(v) for v in b[:4])
-            assert np.array_equal(pred[0][y:y + h, x:x + w], exp[5][0][y:y + h, x:x + w])
+        assert np.array_equal(best[:, 0], exp[3]) and np.array_equal(best[:, 1], exp[4])
+        # Blocks at one position overlap, so the prediction picture is checked
+        # per group of non-overlapping blocks (one size and MV per position).
+        idx = np.arange(len(blocks)).reshape(6, 25)
+        for gi in range(25):
+            sub = blocks[idx[:, gi]]
+            e = oracle_refine(oracle, frame, sub, offs, 12345, use_chroma=0)
+            g.set_blocks(sub)
+            g.pred.tensors[0].zero_()
+            _, _, pred = g.run(cfg, capi.MODE_BATCHED)
+            for b in sub:
+                x, y, w, h = (int(v) for v in b[:4])
+                assert np.array_equal(pred[0][y:y + h, x:x + w], e[5][0][y:y + h, x:x + w])
+                for p in (1, 2):
+                    assert np.array_equal(pred[p][y // 2:(y + h) // 2, x // 2:(x + w) // 2],
+                                          e[5][p][y // 2:(y + h) // 2, x // 2:(x + w) // 2])
     finally:
         g.close()
 

@@ -107,6 +107,7 @@ PROTOS = {
     "xg_session_set_blocks": (I32, [P, P, I32]),
     "xg_session_set_tile": (I32, [P, I32, I32, I32]),
     "xg_session_get_tile": (I32, [P, I32, I32]),
+    "xg_session_set_sched": (I32, [P, I32, I32]),
     "xg_session_autotune": (I32, [P, ctypes.POINTER(XgRefineCfg), I32, ctypes.c_char_p]),
     "xg_session_load_tiles": (I32, [P, ctypes.c_char_p]),
     "xg_refine_frac": (I32, [P, ctypes.POINTER(XgRefineCfg), ctypes.POINTER(XgPicture), I32]),
@@ -131,6 +132,15 @@ def load(path=None):
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    # torch wheels bundle their own HIP runtime (same soname as /opt/rocm's).
+    # Whichever is loaded first serves the whole process; if libxg266.so were
+    # loaded first, torch would later bind to a runtime it was not built
+    # against and report no device.  Load torch's first when it is installed,
+    # so the device buffers torch allocates and this library share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(p):
         raise RuntimeError(f"{p} is missing: build it with `python -m xin26x_amd.build` "
                            "(there is no CPU fallback)")

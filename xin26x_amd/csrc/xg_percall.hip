@@ -114,13 +114,13 @@ __device__ __forceinline__ uint32_t block_sum256(uint32_t v)
     return t;
 }
 
-// SAD of `n` packed reference blocks against one packed input block; one
-// workgroup per reference block.
+// SAD of packed reference blocks (one every refPitch samples) against one
+// packed input block; one workgroup per reference block.
 __global__ void __launch_bounds__(256) k_sad(const uint16_t *__restrict__ in,
-                                             const uint16_t *__restrict__ refs, int npix,
+                                             const uint16_t *__restrict__ refs, int npix, int refPitch,
                                              uint32_t *__restrict__ out)
 {
-    const uint16_t *r = refs + (size_t)blockIdx.x * npix;
+    const uint16_t *r = refs + (size_t)blockIdx.x * refPitch;
     uint32_t s = 0;
     for (int i = threadIdx.x; i < npix; i += 256) s += (uint32_t)abs((int)in[i] - (int)r[i]);
     s = block_sum256(s);
@@ -308,7 +308,8 @@ static void sad_multi(const uint16_t *in, intptr_t is, const uint16_t *const *re
     for (int k = 0; k < nref; k++) pack2d(c->host + blk * (1 + k), refs[k], rs, (int)w, (int)h, 2);
     if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, blk * (1 + nref), hipMemcpyHostToDevice, c->stream))) return;
     uint32_t *dres = (uint32_t *)(c->dev + blk * (1 + nref));
-    k_sad<<<nref, 256, 0, c->stream>>>((uint16_t *)c->dev, (uint16_t *)(c->dev + blk), (int)npix, dres);
+    k_sad<<<nref, 256, 0, c->stream>>>((uint16_t *)c->dev, (uint16_t *)(c->dev + blk), (int)npix, (int)(blk / 2),
+                                       dres);
     if (!XG_HIP(hipGetLastError())) return;
     uint32_t *hres = (uint32_t *)(c->host + blk * (1 + nref));
     if (!XG_HIP(hipMemcpyAsync(hres, dres, 4 * nref, hipMemcpyDeviceToHost, c->stream))) return;

@@ -1,17 +1,16 @@
-// xg_refine.hip -- batched fractional motion refinement + motion compensation.
+// xg_refine.hip -- batched fractional motion refinement fused with motion
+// compensation.
 //
-// Stage 1 (k_refine): for every block and each of its ncand candidate MVs
-// (base MV + offset, 1/16 pel, clipped to the CU MV window of
-// h266_encoder/source/h266_enc_init.c:1462-1465), the luma 8-tap prediction
+// For every block and each of its ncand candidate MVs (base MV + offset,
+// 1/16 pel, clipped to the CU MV window of
+// h266_encoder/source/h266_enc_init.c:1462-1465) the luma 8-tap prediction
 // (h266_core/source/h266_luma_mc.c, table dispatch of
 // h266_encoder/source/h266_inter_prediction.c:143-172, filterIndex 0) and,
 // when enabled, the Cb/Cr 4-tap predictions (h266_chroma_mc.c, address split
 // of h266_inter_prediction.c:664-674) are formed and compared with the
 // current block by SATD; cost = (SATD << 8) + (EG(dx) + EG(dy)) * lambda with
-// the MVD in 1/16 pel.
-// Stage 2 (k_mc): per block the lowest cost (earliest candidate on ties), its
-// result record, and the luma + chroma prediction of that candidate written
-// into the prediction picture.
+// the MVD in 1/16 pel.  The lowest cost wins (earliest candidate on ties) and
+// its luma + chroma prediction is written into the prediction picture.
 //
 // Mapping (one wavefront = one workgroup of 64 lanes):
 //  * a task is (block, first candidate, candidate count); tasks are grouped
@@ -24,8 +23,15 @@
 //  * SATD tiles (8x8, 4x4 or 2x2 as the block shape allows) are transformed
 //    vertically in registers and horizontally across lanes with DPP and
 //    ds_swizzle butterflies; sums reduce within the wavefront;
-//  * batched mode: one persistent launch per stage pulls tasks from an atomic
-//    counter; plain mode: one launch per block shape.
+//  * each task folds its best (cost << 32 | candidate) into a per-block
+//    64-bit atomic minimum and counts its candidates into a per-block
+//    counter; the task that completes a block performs the motion
+//    compensation of the winner from a freshly staged LDS window and resets
+//    the block's counters, so one launch does refinement and MC;
+//  * batched mode: one persistent launch pulls tasks from an atomic queue
+//    head, and the last workgroup to leave resets the queue, so a frame is a
+//    single kernel node (graph-capturable, no memset); plain mode: one launch
+//    per block shape.
 #include <algorithm>
 #include <stdio.h>
 #include <string.h>
@@ -43,34 +49,28 @@ constexpr int kChromaLH = 40;      // rows: H/2 + 3 + 5 at most
 constexpr int kMaxOff = 64;        // |candidate offset| limit, 1/16 pel
 constexpr int kNumShapes = 25;     // (lgW-2)*5 + (lgH-2), W, H in 4..64
 constexpr int kPicSlots = 16;
+constexpr int kRegions = 8;        // spatial work queues, one per XCD of the MI355X
 
 struct RefineArgs {
     const xg_picture *pics;
     const xg_block *blocks;
     const int2 *tasks;             // x: block index, y: c0 | (count << 16)
-    int taskBegin, taskEnd;        // plain mode: this launch's range
-    int *counter;                  // batched mode: work-queue head (nullptr in plain mode)
+    int taskBegin, taskEnd;        // this launch's task range
+    int *queue;                    // batched mode: [0..7] region queue heads, [8] exit count; nullptr in plain mode
+    const int *regionBegin;        // batched mode: task offsets of the spatial queues (nregions + 1)
+    int nregions;                  // batched mode: number of spatial queues (1..kRegions)
     uint32_t *candCost;
+    xg_block_result *best;
+    unsigned long long *bestKey;   // per block: min over tasks of (cost << 32 | cand); all-ones when idle
+    int *done;                     // per block: candidates costed so far; 0 when idle
     int ncand;
     int offMinX, offMaxX, offMinY, offMaxY;
     uint32_t lambda;
     int ctuSize;
     int useChroma;
     const int16_t *off;            // device: offx[XG_MAX_CAND] then offy[XG_MAX_CAND]
-};
-
-struct McArgs {
-    const xg_picture *pics;
-    const xg_block *blocks;
-    const int *order;              // block indices grouped by shape
-    int begin, end;
-    int *counter;
-    const uint32_t *candCost;
-    xg_block_result *best;
+    int doMc;                      // 0: costs only (tile timing), no block completion or MC
     xg_picture pred;               // output planes (unpadded)
-    int ncand;
-    int ctuSize;
-    const int16_t *off;
 };
 
 // CU MV window in 1/16 pel (h266_enc_init.c:1462-1465, int16 as stored there).
@@ -191,9 +191,9 @@ __device__ __forceinline__ void chroma_pass(const RefineArgs &a, const xg_block 
 }
 
 template <int W, int TS>
-__device__ __forceinline__ void luma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw, const uint16_t *lds,
-                          int wx0, int wy0, const uint16_t *cur, intptr_t cs, int c0, int nc, uint32_t *csum,
-                          const int16_t *sOff)
+__device__ __forceinline__ void luma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw,
+                                          const uint16_t *lds, int wx0, int wy0, const uint16_t *cur,
+                                          intptr_t cs, int c0, int nc, uint32_t *csum, const int16_t *sOff)
 {
     constexpr int CP = 64 / W;
     const int cl = threadIdx.x / W;
@@ -205,10 +205,10 @@ __device__ __forceinline__ void luma_task(const RefineArgs &a, const xg_block &B
 }
 
 template <int WC, int TC>
-__device__ __forceinline__ void chroma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw, const uint16_t *ldsU,
-                            const uint16_t *ldsV, int cx0, int cy0, const uint16_t *curU,
-                            const uint16_t *curV, intptr_t cs, int c0, int nc, uint32_t *csum,
-                            const int16_t *sOff)
+__device__ __forceinline__ void chroma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw,
+                                            const uint16_t *ldsU, const uint16_t *ldsV, int cx0, int cy0,
+                                            const uint16_t *curU, const uint16_t *curV, intptr_t cs, int c0,
+                                            int nc, uint32_t *csum, const int16_t *sOff)
 {
     constexpr int CP = 64 / WC;
     const int cl = threadIdx.x / WC;
@@ -220,7 +220,94 @@ __device__ __forceinline__ void chroma_task(const RefineArgs &a, const xg_block 
     }
 }
 
-__device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16_t *lds, uint32_t *csum, const int16_t *sOff)
+// Motion compensation of one plane from an LDS window whose first row and
+// column are TAPS/2-1 samples above/left of the block's integer position.
+// Lanes [lane0, lane0 + nlanes) split the block into column x row-chunk
+// units; each unit walks its rows with a sliding window of first-pass values.
+template <int TAPS>
+__device__ __forceinline__ void mc_plane(const uint16_t *win, int ls, int fx, int fy, int W, int H,
+                                         uint16_t *dst, intptr_t ds, int lane0, int nlanes)
+{
+    const int lid = (int)threadIdx.x - lane0;
+    if (lid < 0 || lid >= nlanes) return;
+    const int chunks = max(1, min(nlanes / W, H));
+    const int x = lid % W, ck = lid / W;
+    if (ck >= chunks) return;
+    const int rp = H / chunks, y0 = ck * rp;
+    int hc[TAPS], vc[TAPS];
+    const int8_t *ch = taps_for<TAPS>(fx, 0), *cv = taps_for<TAPS>(fy, 0);
+#pragma unroll
+    for (int j = 0; j < TAPS; j++) {
+        hc[j] = ch[j];
+        vc[j] = cv[j];
+    }
+    const uint16_t *p = win + y0 * ls + x;
+    int t[TAPS];
+#pragma unroll
+    for (int k = 0; k < TAPS - 1; k++) t[k] = hrow_lds<TAPS>(p + k * ls, hc);
+    for (int y = 0; y < rp; y++) {
+        t[TAPS - 1] = hrow_lds<TAPS>(p + (y + TAPS - 1) * ls, hc);
+        int s = 0;
+#pragma unroll
+        for (int j = 0; j < TAPS; j++) s += vc[j] * t[j];
+        dst[(intptr_t)(y0 + y) * ds + x] = (uint16_t)vfinal_pix(s);
+#pragma unroll
+        for (int k = 0; k < TAPS - 1; k++) t[k] = t[k + 1];
+    }
+}
+
+// Best-candidate record and luma + chroma prediction of block b.
+__device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_block &B, const MvWin &mw,
+                                         unsigned long long key, uint16_t *lds, const int16_t *sOff)
+{
+    const int lane = threadIdx.x;
+    const int c = (int)(uint32_t)key;
+    const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
+    const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
+    if (lane == 0) {
+        xg_block_result r;
+        r.mvx = mvx;
+        r.mvy = mvy;
+        r.cost = (uint32_t)(key >> 32);
+        r.cand = c;
+        a.best[b] = r;
+    }
+    const xg_picture &R = a.pics[B.ref];
+    const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
+    {   // luma: (H+7) x (W+7) samples from 3 above / 3 left of the integer position
+        const intptr_t rs = R.plane[0].stride;
+        const uint16_t *src = R.plane[0].data + (intptr_t)((ry >> 4) - 3) * rs + ((rx >> 4) - 3);
+        const int ww = B.w + 7, hh = B.h + 7;
+        for (int r = 0; r < hh; r++)
+            for (int cc = lane; cc < ww; cc += 64) lds[r * kLumaLW + cc] = src[(intptr_t)r * rs + cc];
+        __syncthreads();
+        const xg_plane &P0 = a.pred.plane[0];
+        mc_plane<8>(lds, kLumaLW, rx & 15, ry & 15, B.w, B.h, P0.data + (intptr_t)B.y * P0.stride + B.x,
+                    P0.stride, 0, 64);
+        __syncthreads();
+    }
+    {   // chroma: (H/2+3) x (W/2+3) samples per plane from 1 above / 1 left
+        const int wc = B.w >> 1, hc = B.h >> 1, ww = wc + 3, hh = hc + 3;
+        uint16_t *lu = lds, *lv = lds + kChromaLW * kChromaLH;
+        const intptr_t rs = R.plane[1].stride;
+        const intptr_t so = (intptr_t)((ry >> 5) - 1) * rs + ((rx >> 5) - 1);
+        for (int r = 0; r < hh; r++)
+            for (int cc = lane; cc < 2 * ww; cc += 64) {
+                const int pl = cc >= ww, col = cc - pl * ww;
+                (pl ? lv : lu)[r * kChromaLW + col] = R.plane[1 + pl].data[so + (intptr_t)r * rs + col];
+            }
+        __syncthreads();
+        const xg_plane &P1 = a.pred.plane[1], &P2 = a.pred.plane[2];
+        const intptr_t o1 = (intptr_t)(B.y >> 1) * P1.stride + (B.x >> 1);
+        const intptr_t o2 = (intptr_t)(B.y >> 1) * P2.stride + (B.x >> 1);
+        mc_plane<4>(lu, kChromaLW, rx & 31, ry & 31, wc, hc, P1.data + o1, P1.stride, 0, 32);
+        mc_plane<4>(lv, kChromaLW, rx & 31, ry & 31, wc, hc, P2.data + o2, P2.stride, 32, 32);
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16_t *lds, uint32_t *csum,
+                                           const int16_t *sOff, int *sLast, unsigned long long *sKey)
 {
     const int lane = threadIdx.x;
     const int2 tk = a.tasks[task];
@@ -302,13 +389,43 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16
         __syncthreads();
     }
 
+    // ---- costs, task-local best, block completion ----
+    unsigned long long key = ~0ull;
     if (lane < nc) {
         const int c = c0 + lane;
         const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
         const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
         const uint32_t bits = eg_bits(mvx - B.mvpx) + eg_bits(mvy - B.mvpy);
-        a.candCost[(size_t)b * a.ncand + c] = (csum[lane] << 8) + bits * a.lambda;
+        const uint32_t cost = (csum[lane] << 8) + bits * a.lambda;
+        a.candCost[(size_t)b * a.ncand + c] = cost;
+        key = ((unsigned long long)cost << 32) | (uint32_t)c;
     }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const unsigned long long o = (unsigned long long)__shfl_xor((long long)key, m, 64);
+        key = o < key ? o : key;
+    }
+    if (lane == 0) {
+        int last = 0;
+        if (!a.doMc) {
+            last = 0;
+        } else if (nc == a.ncand) {
+            last = 1;                                // the task covers the whole block
+        } else {
+            atomicMin(&a.bestKey[b], key);
+            __threadfence();
+            last = atomicAdd(&a.done[b], nc) + nc == a.ncand;
+            if (last) {
+                __threadfence();
+                key = atomicExch(&a.bestKey[b], ~0ull);   // read and reset for the next launch
+                atomicExch(&a.done[b], 0);
+            }
+        }
+        *sLast = last;
+        *sKey = key;
+    }
+    __syncthreads();
+    if (*sLast) mc_block(a, b, B, mw, *sKey, lds, sOff);
     __syncthreads();
 }
 
@@ -317,125 +434,49 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a)
     __shared__ uint16_t lds[kLumaLW * kLumaLH];
     __shared__ uint32_t csum[64];
     __shared__ int16_t sOff[2 * XG_MAX_CAND];
-    __shared__ int s_task;
+    __shared__ int s_task, s_last;
+    __shared__ unsigned long long s_key;
     for (int i = threadIdx.x; i < 2 * XG_MAX_CAND; i += 64) sOff[i] = a.off[i];
     __syncthreads();
-    if (a.counter) {
+    if (a.queue) {
+        // Workgroups are dispatched to the XCDs round-robin, so workgroup i
+        // runs on XCD i % 8.  Each drains the queue of its own spatial region
+        // first -- neighbouring blocks share reference samples in that XCD's
+        // L2 -- then steals from the other regions in turn.
+        int region = (int)(blockIdx.x % kRegions) % a.nregions, exhausted = 0;
         for (;;) {
-            if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
+            if (threadIdx.x == 0) {
+                int task = -1;
+                while (exhausted < a.nregions) {
+                    const int beg = a.regionBegin[region], cnt = a.regionBegin[region + 1] - beg;
+                    const int i = cnt > 0 ? atomicAdd(&a.queue[region], 1) : 0;
+                    if (i < cnt) {
+                        task = beg + i;
+                        break;
+                    }
+                    region = (region + 1) % a.nregions;
+                    exhausted++;
+                }
+                s_task = task;
+            }
             __syncthreads();
             const int t = s_task;
             __syncthreads();
-            if (t >= a.taskEnd) break;
-            refine_one(a, t, lds, csum, sOff);
+            if (t < 0) break;
+            refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
+        }
+        // the last workgroup to leave resets the queues for the next launch
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(&a.queue[kRegions], 1) == (int)gridDim.x - 1) {
+                for (int r = 0; r < a.nregions; r++) atomicExch(&a.queue[r], 0);
+                atomicExch(&a.queue[kRegions], 0);
+            }
         }
     } else {
         const int t = a.taskBegin + (int)blockIdx.x;
-        if (t < a.taskEnd) refine_one(a, t, lds, csum, sOff);
+        if (t < a.taskEnd) refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
     }
-}
-
-// ---------------------------------------------------------------------------
-// Stage 2: argmin + motion compensation of the best candidate.
-// ---------------------------------------------------------------------------
-template <int TAPS, int FBITS>
-__device__ __forceinline__ void mc_column(const uint16_t *src, intptr_t ss, uint16_t *dst, intptr_t ds,
-                                          int fx, int fy, int h)
-{
-    // src: top-left of the integer position of this column's block; the
-    // column walks down h rows with a sliding window of first-pass values.
-    const int8_t *ch = taps_for<TAPS>(fx, 0);
-    const int8_t *cv = taps_for<TAPS>(fy, 0);
-    int hc[TAPS], vc[TAPS];
-#pragma unroll
-    for (int j = 0; j < TAPS; j++) {
-        hc[j] = ch[j];
-        vc[j] = cv[j];
-    }
-    const uint16_t *p = src - (TAPS / 2 - 1) * ss - (TAPS / 2 - 1);
-    int t[TAPS];
-#pragma unroll
-    for (int k = 0; k < TAPS - 1; k++) t[k] = hrow_lds<TAPS>(p + k * ss, hc);
-    for (int y = 0; y < h; y++) {
-        t[TAPS - 1] = hrow_lds<TAPS>(p + (y + TAPS - 1) * ss, hc);
-        int s = 0;
-#pragma unroll
-        for (int j = 0; j < TAPS; j++) s += vc[j] * t[j];
-        dst[(intptr_t)y * ds] = (uint16_t)vfinal_pix(s);
-#pragma unroll
-        for (int k = 0; k < TAPS - 1; k++) t[k] = t[k + 1];
-    }
-}
-
-__device__ __forceinline__ void mc_one(const McArgs &a, int b, const int16_t *sOff)
-{
-    const int lane = threadIdx.x;
-    const xg_block B = a.blocks[b];
-    const xg_picture &P = a.pics[B.pic];
-    const xg_picture &R = a.pics[B.ref];
-    const MvWin mw = mv_window(B, P.plane[0].width, P.plane[0].height, a.ctuSize);
-    uint32_t cost = lane < a.ncand ? a.candCost[(size_t)b * a.ncand + lane] : 0xFFFFFFFFu;
-    int idx = lane;
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        const uint32_t oc = (uint32_t)__shfl_xor((int)cost, m, 64);
-        const int oi = __shfl_xor(idx, m, 64);
-        if (oc < cost || (oc == cost && oi < idx)) {
-            cost = oc;
-            idx = oi;
-        }
-    }
-    const int mvx = clampi(B.mvx + sOff[idx], mw.mnx, mw.mxx);
-    const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + idx], mw.mny, mw.mxy);
-    if (lane == 0) {
-        xg_block_result r;
-        r.mvx = mvx;
-        r.mvy = mvy;
-        r.cost = cost;
-        r.cand = idx;
-        a.best[b] = r;
-    }
-    const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
-    if (lane < B.w) {
-        const intptr_t rs = R.plane[0].stride;
-        const uint16_t *src = R.plane[0].data + (intptr_t)(ry >> 4) * rs + (rx >> 4) + lane;
-        uint16_t *dst = a.pred.plane[0].data + (intptr_t)B.y * a.pred.plane[0].stride + B.x + lane;
-        mc_column<8, 4>(src, rs, dst, a.pred.plane[0].stride, rx & 15, ry & 15, B.h);
-    }
-    const int wc = B.w >> 1;
-    if (lane < 2 * wc) {
-        const int pl = 1 + (lane >= wc), xl = lane - (pl - 1) * wc;
-        const intptr_t rs = R.plane[pl].stride;
-        const uint16_t *src = R.plane[pl].data + (intptr_t)(ry >> 5) * rs + (rx >> 5) + xl;
-        uint16_t *dst = a.pred.plane[pl].data + (intptr_t)(B.y >> 1) * a.pred.plane[pl].stride + (B.x >> 1) + xl;
-        mc_column<4, 5>(src, rs, dst, a.pred.plane[pl].stride, rx & 31, ry & 31, B.h >> 1);
-    }
-}
-
-__global__ void __launch_bounds__(64) k_mc(McArgs a)
-{
-    __shared__ int s_idx;
-    __shared__ int16_t sOff[2 * XG_MAX_CAND];
-    for (int i = threadIdx.x; i < 2 * XG_MAX_CAND; i += 64) sOff[i] = a.off[i];
-    __syncthreads();
-    if (a.counter) {
-        for (;;) {
-            if (threadIdx.x == 0) s_idx = atomicAdd(a.counter, 1);
-            __syncthreads();
-            const int i = s_idx;
-            __syncthreads();
-            if (i >= a.end) break;
-            mc_one(a, a.order[i], sOff);
-        }
-    } else {
-        const int i = a.begin + (int)blockIdx.x;
-        if (i < a.end) mc_one(a, a.order[i], sOff);
-    }
-}
-
-__global__ void k_zero_counters(int *c, int n)
-{
-    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
 }  // namespace xg
@@ -449,7 +490,8 @@ struct xg_session {
     hipStream_t stream = nullptr;
     int maxBlocks = 0;
     int numCUs = 256;
-    int persistFactor = 16;            // persistent workgroups per CU
+    int persistFactor = 11;            // persistent workgroups per CU (LDS allows 11)
+    int nregions = 1;                  // spatial work queues of the batched form
 
     xg_picture pics[kPicSlots];
     bool picSet[kPicSlots] = {};
@@ -464,12 +506,12 @@ struct xg_session {
     // per-shape work queues
     int tile[kNumShapes];              // candidates per task
     std::vector<int2> tasks;
-    int2 *dTasks = nullptr;
+    int2 *dTasks = nullptr;            // shape-major (plain mode)
+    int2 *dTasksR = nullptr;           // region-major, heaviest first within a region (batched mode)
+    int *dRegionBegin = nullptr;
     size_t taskCap = 0;
     int shapeTaskBegin[kNumShapes + 1] = {};
     std::vector<int> shapeOrder;       // shapes, heaviest tasks first
-    int *dOrder = nullptr;             // block indices grouped by shape
-    int shapeBlockBegin[kNumShapes + 1] = {};
     uint64_t tasksKey = 0;             // (blocksVersion, ncand, tile table) the tasks were built for
 
     uint32_t *dCandCost = nullptr;
@@ -477,7 +519,9 @@ struct xg_session {
     int16_t *dOff = nullptr;           // candidate offsets of the last configuration
     int16_t hOff[2 * XG_MAX_CAND];
     bool offValid = false;
-    int *dCounters = nullptr;
+    int *dQueue = nullptr;             // persistent work-queue head and exit count
+    int *dDone = nullptr;              // per-block candidate counters
+    unsigned long long *dBestKey = nullptr;  // per-block best (cost << 32 | cand)
     void *hStage = nullptr;            // pinned upload staging
     size_t hStageCap = 0;
 
@@ -530,10 +574,19 @@ static uint64_t mix(uint64_t h, uint64_t v)
     return h;
 }
 
+// Region of a block for the XCD-local queues: horizontal bands of its picture.
+static int region_of(const xg_session *s, const xg_block &B)
+{
+    const int h = std::max(1, s->pics[B.pic].plane[0].height);
+    return std::min(s->nregions - 1, (int)((long)B.y * s->nregions / h));
+}
+
 static bool build_tasks(xg_session *s, int ncand)
 {
     uint64_t key = mix(mix(s->blocksVersion, (uint64_t)ncand), 0);
     for (int i = 0; i < kNumShapes; i++) key = mix(key, (uint64_t)s->tile[i]);
+    for (int i = 0; i < kPicSlots; i++) key = mix(key, s->picSet[i] ? (uint64_t)s->pics[i].plane[0].height : 0);
+    key = mix(key, (uint64_t)s->nregions);
     if (key == s->tasksKey) return true;
     std::vector<std::vector<int>> per(kNumShapes);
     for (int b = 0; b < s->nblocks; b++) per[shape_of(s->blocks[b].w, s->blocks[b].h)].push_back(b);
@@ -546,37 +599,46 @@ static bool build_tasks(xg_session *s, int ncand)
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return weight(x) > weight(y); });
     s->tasks.clear();
-    std::vector<int> blockOrder;
-    blockOrder.reserve(s->nblocks);
-    int sb[kNumShapes + 1], st[kNumShapes + 1];
+    std::vector<std::vector<int2>> perRegion(s->nregions);
+    int st[kNumShapes + 1];
     for (int oi = 0; oi < kNumShapes; oi++) {
         const int sh = order[oi];
         st[oi] = (int)s->tasks.size();
-        sb[oi] = (int)blockOrder.size();
         const int cpt = std::max(1, std::min(s->tile[sh], ncand));
         for (int b : per[sh]) {
-            blockOrder.push_back(b);
-            for (int c0 = 0; c0 < ncand; c0 += cpt) s->tasks.push_back(make_int2(b, c0 | (std::min(cpt, ncand - c0) << 16)));
+            const int r = region_of(s, s->blocks[b]);
+            for (int c0 = 0; c0 < ncand; c0 += cpt) {
+                const int2 t = make_int2(b, c0 | (std::min(cpt, ncand - c0) << 16));
+                s->tasks.push_back(t);
+                perRegion[r].push_back(t);
+            }
         }
     }
     st[kNumShapes] = (int)s->tasks.size();
-    sb[kNumShapes] = (int)blockOrder.size();
     memcpy(s->shapeTaskBegin, st, sizeof(st));
-    memcpy(s->shapeBlockBegin, sb, sizeof(sb));
     s->shapeOrder = order;
+    std::vector<int2> tasksR;
+    tasksR.reserve(s->tasks.size());
+    int rb[kRegions + 1];
+    for (int r = 0; r <= kRegions; r++) {
+        rb[r] = (int)tasksR.size();
+        if (r < s->nregions) tasksR.insert(tasksR.end(), perRegion[r].begin(), perRegion[r].end());
+    }
     if (s->tasks.size() > s->taskCap) {
-        if (s->dTasks) {
-            if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
-            hipFree(s->dTasks);
-        }
-        s->dTasks = nullptr;
+        if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
+        if (s->dTasks) hipFree(s->dTasks);
+        if (s->dTasksR) hipFree(s->dTasksR);
+        s->dTasks = s->dTasksR = nullptr;
         s->taskCap = 0;
         size_t cap = std::max<size_t>(s->tasks.size(), 1024);
-        if (!XG_HIP(hipMalloc((void **)&s->dTasks, cap * sizeof(int2)))) return false;
+        if (!XG_HIP(hipMalloc((void **)&s->dTasks, cap * sizeof(int2))) ||
+            !XG_HIP(hipMalloc((void **)&s->dTasksR, cap * sizeof(int2))))
+            return false;
         s->taskCap = cap;
     }
     if (!s->tasks.empty() && !stage_upload(s, s->dTasks, s->tasks.data(), s->tasks.size() * sizeof(int2))) return false;
-    if (!blockOrder.empty() && !stage_upload(s, s->dOrder, blockOrder.data(), blockOrder.size() * sizeof(int))) return false;
+    if (!tasksR.empty() && !stage_upload(s, s->dTasksR, tasksR.data(), tasksR.size() * sizeof(int2))) return false;
+    if (!stage_upload(s, s->dRegionBegin, rb, sizeof(rb))) return false;
     if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
     s->tasksKey = key;
     return true;
@@ -657,7 +719,7 @@ static bool validate_blocks(xg_session *s, const xg_picture *pred, int ctu)
     return true;
 }
 
-static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, McArgs &ma, const xg_picture *pred)
+static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, const xg_picture *pred)
 {
     int16_t off[2 * XG_MAX_CAND] = {};
     memset(&ra, 0, sizeof(ra));
@@ -665,6 +727,9 @@ static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, M
     ra.blocks = s->dBlocks;
     ra.tasks = s->dTasks;
     ra.candCost = s->dCandCost;
+    ra.best = s->dBest;
+    ra.bestKey = s->dBestKey;
+    ra.done = s->dDone;
     ra.ncand = cfg->ncand;
     ra.offMinX = ra.offMinY = 1 << 30;
     ra.offMaxX = ra.offMaxY = -(1 << 30);
@@ -680,16 +745,8 @@ static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, M
     ra.ctuSize = cfg->ctuSize;
     ra.useChroma = cfg->useChroma;
     ra.off = s->dOff;
-    memset(&ma, 0, sizeof(ma));
-    ma.pics = s->dPics;
-    ma.blocks = s->dBlocks;
-    ma.order = s->dOrder;
-    ma.candCost = s->dCandCost;
-    ma.best = s->dBest;
-    if (pred) ma.pred = *pred;
-    ma.ncand = cfg->ncand;
-    ma.ctuSize = cfg->ctuSize;
-    ma.off = s->dOff;
+    ra.doMc = pred != nullptr;
+    if (pred) ra.pred = *pred;
     if (!s->offValid || memcmp(off, s->hOff, sizeof(off)) != 0) {
         if (!stage_upload(s, s->dOff, off, sizeof(off))) return false;
         memcpy(s->hOff, off, sizeof(off));
@@ -698,12 +755,14 @@ static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, M
     return true;
 }
 
-// Enqueues both stages in batched (persistent) form on the session stream.
-static bool enqueue_batched(xg_session *s, RefineArgs ra, McArgs ma, bool timed)
+// One persistent launch over every task (refinement + MC) on the session stream.
+static bool enqueue_batched(xg_session *s, RefineArgs ra, bool timed)
 {
-    const int grid = s->numCUs * s->persistFactor;
-    k_zero_counters<<<1, 64, 0, s->stream>>>(s->dCounters, 2);
-    ra.counter = s->dCounters;
+    const int grid = std::min(s->numCUs * s->persistFactor, std::max(1, (int)s->tasks.size()));
+    ra.queue = s->dQueue;
+    ra.tasks = s->dTasksR;
+    ra.regionBegin = s->dRegionBegin;
+    ra.nregions = s->nregions;
     ra.taskBegin = 0;
     ra.taskEnd = (int)s->tasks.size();
     int e0 = -1, e1 = -1;
@@ -713,19 +772,16 @@ static bool enqueue_batched(xg_session *s, RefineArgs ra, McArgs ma, bool timed)
         if (e0 < 0 || e1 < 0) timed = false;
     }
     if (timed) hipEventRecord(s->evPool[e0], s->stream);
-    k_refine<<<std::min(grid, std::max(1, ra.taskEnd)), 64, 0, s->stream>>>(ra);
+    k_refine<<<grid, 64, 0, s->stream>>>(ra);
     if (timed) {
         hipEventRecord(s->evPool[e1], s->stream);
         s->evPending[0].push_back({e0, e1});
     }
-    ma.counter = s->dCounters + 1;
-    ma.begin = 0;
-    ma.end = s->nblocks;
-    k_mc<<<std::min(grid, std::max(1, s->nblocks)), 64, 0, s->stream>>>(ma);
     return XG_HIP(hipGetLastError());
 }
 
-static bool enqueue_plain(xg_session *s, RefineArgs ra, McArgs ma, bool timed)
+// One launch per block shape, one workgroup per task.
+static bool enqueue_plain(xg_session *s, RefineArgs ra, bool timed)
 {
     int e0 = -1, e1 = -1;
     if (timed) {
@@ -737,7 +793,8 @@ static bool enqueue_plain(xg_session *s, RefineArgs ra, McArgs ma, bool timed)
     for (int oi = 0; oi < kNumShapes; oi++) {
         const int b0 = s->shapeTaskBegin[oi], b1 = s->shapeTaskBegin[oi + 1];
         if (b1 <= b0) continue;
-        ra.counter = nullptr;
+        ra.queue = nullptr;
+        ra.tasks = s->dTasks;
         ra.taskBegin = b0;
         ra.taskEnd = b1;
         k_refine<<<b1 - b0, 64, 0, s->stream>>>(ra);
@@ -745,14 +802,6 @@ static bool enqueue_plain(xg_session *s, RefineArgs ra, McArgs ma, bool timed)
     if (timed) {
         hipEventRecord(s->evPool[e1], s->stream);
         s->evPending[0].push_back({e0, e1});
-    }
-    for (int oi = 0; oi < kNumShapes; oi++) {
-        const int b0 = s->shapeBlockBegin[oi], b1 = s->shapeBlockBegin[oi + 1];
-        if (b1 <= b0) continue;
-        ma.counter = nullptr;
-        ma.begin = b0;
-        ma.end = b1;
-        k_mc<<<b1 - b0, 64, 0, s->stream>>>(ma);
     }
     return XG_HIP(hipGetLastError());
 }
@@ -783,10 +832,16 @@ int32_t xg_session_create(xg_session **out, int32_t maxBlocks)
     bool ok = XG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) &&
               XG_HIP(hipMalloc((void **)&s->dPics, sizeof(xg_picture) * kPicSlots)) &&
               XG_HIP(hipMalloc((void **)&s->dBlocks, sizeof(xg_block) * (size_t)maxBlocks)) &&
-              XG_HIP(hipMalloc((void **)&s->dOrder, sizeof(int) * (size_t)maxBlocks)) &&
               XG_HIP(hipMalloc((void **)&s->dCandCost, sizeof(uint32_t) * (size_t)maxBlocks * XG_MAX_CAND)) &&
               XG_HIP(hipMalloc((void **)&s->dBest, sizeof(xg_block_result) * (size_t)maxBlocks)) &&
-              XG_HIP(hipMalloc((void **)&s->dCounters, 64 * sizeof(int))) &&
+              XG_HIP(hipMalloc((void **)&s->dQueue, 64 * sizeof(int))) &&
+              XG_HIP(hipMalloc((void **)&s->dRegionBegin, (kRegions + 1) * sizeof(int))) &&
+              XG_HIP(hipMemset(s->dRegionBegin, 0, (kRegions + 1) * sizeof(int))) &&
+              XG_HIP(hipMalloc((void **)&s->dDone, sizeof(int) * (size_t)maxBlocks)) &&
+              XG_HIP(hipMalloc((void **)&s->dBestKey, sizeof(unsigned long long) * (size_t)maxBlocks)) &&
+              XG_HIP(hipMemset(s->dQueue, 0, 64 * sizeof(int))) &&
+              XG_HIP(hipMemset(s->dDone, 0, sizeof(int) * (size_t)maxBlocks)) &&
+              XG_HIP(hipMemset(s->dBestKey, 0xFF, sizeof(unsigned long long) * (size_t)maxBlocks)) &&
               XG_HIP(hipMalloc((void **)&s->dOff, 2 * XG_MAX_CAND * sizeof(int16_t)));
     if (!ok) {
         xg_session_destroy(s);
@@ -806,11 +861,14 @@ int32_t xg_session_destroy(xg_session *s)
     for (auto e : s->evPool) hipEventDestroy(e);
     hipFree(s->dPics);
     hipFree(s->dBlocks);
-    hipFree(s->dOrder);
     hipFree(s->dTasks);
+    hipFree(s->dTasksR);
+    hipFree(s->dRegionBegin);
     hipFree(s->dCandCost);
     hipFree(s->dBest);
-    hipFree(s->dCounters);
+    hipFree(s->dQueue);
+    hipFree(s->dDone);
+    hipFree(s->dBestKey);
     hipFree(s->dOff);
     if (s->hStage) hipHostFree(s->hStage);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -834,6 +892,10 @@ int32_t xg_session_set_picture(xg_session *s, int32_t slot, const xg_picture *pi
             set_error(XG_ERR_ARG, "xg_session_set_picture: plane %d malformed", p);
             return XG_ERR_ARG;
         }
+    }
+    if (pic->plane[1].stride != pic->plane[2].stride || pic->plane[1].pad != pic->plane[2].pad) {
+        set_error(XG_ERR_ARG, "xg_session_set_picture: Cb and Cr planes must share stride and padding");
+        return XG_ERR_ARG;
     }
     s->pics[slot] = *pic;
     s->picSet[slot] = true;
@@ -859,6 +921,9 @@ int32_t xg_session_set_blocks(xg_session *s, const xg_block *blocks, int32_t n)
     s->blocks.assign(blocks, blocks + n);
     s->nblocks = n;
     s->blocksVersion++;
+    if (n && (!XG_HIP(hipMemsetAsync(s->dDone, 0, sizeof(int) * (size_t)n, s->stream)) ||
+              !XG_HIP(hipMemsetAsync(s->dBestKey, 0xFF, sizeof(unsigned long long) * (size_t)n, s->stream))))
+        return XG_ERR_HIP;
     if (n && !stage_upload(s, s->dBlocks, blocks, sizeof(xg_block) * (size_t)n)) return XG_ERR_HIP;
     return XG_OK;
 }
@@ -870,6 +935,17 @@ int32_t xg_session_set_tile(xg_session *s, int32_t lgW, int32_t lgH, int32_t can
         return XG_ERR_ARG;
     }
     s->tile[(lgW - 2) * 5 + (lgH - 2)] = candPerTask;
+    return XG_OK;
+}
+
+int32_t xg_session_set_sched(xg_session *s, int32_t regions, int32_t wgPerCU)
+{
+    if (!s || regions < 1 || regions > kRegions || wgPerCU < 1 || wgPerCU > 64) {
+        set_error(XG_ERR_ARG, "xg_session_set_sched: regions 1..%d, workgroups per CU 1..64", kRegions);
+        return XG_ERR_ARG;
+    }
+    s->nregions = regions;
+    s->persistFactor = wgPerCU;
     return XG_OK;
 }
 
@@ -889,11 +965,10 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
     if (!validate_blocks(s, pred, cfg->ctuSize)) return XG_ERR_ARG;
     if (!build_tasks(s, cfg->ncand)) return XG_ERR_HIP;
     RefineArgs ra;
-    McArgs ma;
-    if (!fill_args(s, cfg, ra, ma, pred)) return XG_ERR_HIP;
+    if (!fill_args(s, cfg, ra, pred)) return XG_ERR_HIP;
     s->lastNcand = cfg->ncand;
-    if (mode == XG_MODE_PLAIN) return enqueue_plain(s, ra, ma, s->timing) ? XG_OK : XG_ERR_HIP;
-    if (mode == XG_MODE_BATCHED) return enqueue_batched(s, ra, ma, s->timing) ? XG_OK : XG_ERR_HIP;
+    if (mode == XG_MODE_PLAIN) return enqueue_plain(s, ra, s->timing) ? XG_OK : XG_ERR_HIP;
+    if (mode == XG_MODE_BATCHED) return enqueue_batched(s, ra, s->timing) ? XG_OK : XG_ERR_HIP;
     if (mode != XG_MODE_GRAPH) {
         set_error(XG_ERR_ARG, "xg_refine_frac: unknown mode %d", mode);
         return XG_ERR_ARG;
@@ -904,6 +979,7 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
     key = mix(key, (uint64_t)(uintptr_t)pred->plane[2].data);
     key = mix(key, ((uint64_t)cfg->lambda << 32) | (uint32_t)cfg->ctuSize);
     key = mix(key, (uint64_t)cfg->useChroma);
+    key = mix(key, ((uint64_t)s->nregions << 16) | (uint64_t)s->persistFactor);
     for (int i = 0; i < cfg->ncand; i++) key = mix(key, ((uint64_t)(uint16_t)cfg->offx[i] << 16) | (uint16_t)cfg->offy[i]);
     if (!s->graphExec || key != s->graphKey) {
         if (s->graphExec) hipGraphExecDestroy(s->graphExec);
@@ -911,7 +987,7 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
         s->graphExec = nullptr;
         s->graph = nullptr;
         if (!XG_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal))) return XG_ERR_HIP;
-        const bool ok = enqueue_batched(s, ra, ma, false);
+        const bool ok = enqueue_batched(s, ra, false);
         hipGraph_t g = nullptr;
         const bool ended = XG_HIP(hipStreamEndCapture(s->stream, &g));
         if (!ok || !ended) return XG_ERR_HIP;
@@ -992,61 +1068,85 @@ int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32
     return XG_OK;
 }
 
+// Device time of `reps` batched launches (costs only, no MC), best of 3.
+static float time_frame(xg_session *s, const xg_refine_cfg *cfg, int reps, hipEvent_t e0, hipEvent_t e1, bool *ok)
+{
+    *ok = false;
+    if (!build_tasks(s, cfg->ncand)) return 0.f;
+    RefineArgs ra;
+    if (!fill_args(s, cfg, ra, nullptr)) return 0.f;
+    float best = 1e30f;
+    for (int trial = 0; trial < 4; trial++) {
+        if (!XG_HIP(hipEventRecord(e0, s->stream))) return 0.f;
+        for (int r = 0; r < reps; r++)
+            if (!enqueue_batched(s, ra, false)) return 0.f;
+        if (!XG_HIP(hipEventRecord(e1, s->stream)) || !XG_HIP(hipEventSynchronize(e1))) return 0.f;
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (trial > 0) best = std::min(best, ms);   // the first trial warms up
+    }
+    *ok = true;
+    return best;
+}
+
 int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t reps, const char *path)
 {
     if (!s || !validate_cfg(cfg) || reps < 1) return XG_ERR_ARG;
     if (s->nblocks == 0) return XG_OK;
-    // Time each shape's refinement launch alone for every legal tile.
-    std::vector<xg_block> all = s->blocks;
+    if (!validate_blocks(s, nullptr, cfg->ctuSize)) return XG_ERR_ARG;
     hipEvent_t e0, e1;
     if (!XG_HIP(hipEventCreate(&e0)) || !XG_HIP(hipEventCreate(&e1))) return XG_ERR_HIP;
     int32_t rc = XG_OK;
-    for (int sh = 0; sh < kNumShapes && rc == XG_OK; sh++) {
-        const int w = 4 << (sh / 5), h = 4 << (sh % 5);
-        std::vector<xg_block> sub;
-        for (auto &B : all)
-            if (B.w == w && B.h == h) sub.push_back(B);
-        if (sub.empty()) continue;
-        if (xg_session_set_blocks(s, sub.data(), (int)sub.size()) != XG_OK) { rc = XG_ERR_HIP; break; }
-        if (!validate_blocks(s, nullptr, cfg->ctuSize)) { rc = XG_ERR_ARG; break; }
+    bool ok = true;
+    // 1. scheduling: spatial queues x persistent workgroups per CU, whole frame
+    {
         float bestMs = 1e30f;
-        int bestT = s->tile[sh];
-        for (int cpt = 1; cpt <= cfg->ncand; cpt <<= 1) {
-            s->tile[sh] = cpt;
-            if (!build_tasks(s, cfg->ncand)) { rc = XG_ERR_HIP; break; }
-            RefineArgs ra;
-            McArgs ma;
-            if (!fill_args(s, cfg, ra, ma, nullptr)) { rc = XG_ERR_HIP; break; }
-            ra.counter = s->dCounters;
-            ra.taskBegin = 0;
-            ra.taskEnd = (int)s->tasks.size();
-            const int grid = std::min(s->numCUs * s->persistFactor, std::max(1, ra.taskEnd));
-            float tot = 0;
-            for (int r = 0; r <= reps; r++) {
-                k_zero_counters<<<1, 64, 0, s->stream>>>(s->dCounters, 2);
-                hipEventRecord(e0, s->stream);
-                k_refine<<<grid, 64, 0, s->stream>>>(ra);
-                hipEventRecord(e1, s->stream);
-                if (!XG_HIP(hipEventSynchronize(e1))) { rc = XG_ERR_HIP; break; }
-                float ms = 0;
-                hipEventElapsedTime(&ms, e0, e1);
-                if (r > 0) tot += ms;    // first run warms up
+        int bestR = s->nregions, bestF = s->persistFactor;
+        for (int r : {1, 2, 4, 8})
+            for (int f : {8, 11, 16}) {
+                s->nregions = r;
+                s->persistFactor = f;
+                const float ms = time_frame(s, cfg, reps, e0, e1, &ok);
+                if (!ok) { rc = XG_ERR_HIP; break; }
+                if (ms < bestMs) { bestMs = ms; bestR = r; bestF = f; }
             }
-            if (rc != XG_OK) break;
-            if (tot < bestMs) { bestMs = tot; bestT = cpt; }
+        s->nregions = bestR;
+        s->persistFactor = bestF;
+    }
+    // 2. candidates per task: coordinate descent over the shapes present,
+    //    heaviest total work first, each choice timed on the whole frame
+    long work[kNumShapes] = {};
+    for (int b = 0; b < s->nblocks && rc == XG_OK; b++) work[shape_of(s->blocks[b].w, s->blocks[b].h)] += (long)s->blocks[b].w * s->blocks[b].h;
+    std::vector<int> order;
+    for (int sh = 0; sh < kNumShapes; sh++)
+        if (work[sh]) order.push_back(sh);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    float cur = rc == XG_OK ? time_frame(s, cfg, reps, e0, e1, &ok) : 0.f;
+    if (!ok) rc = XG_ERR_HIP;
+    for (int sh : order) {
+        if (rc != XG_OK) break;
+        int bestT = s->tile[sh];
+        for (int cpt = 1; cpt <= cfg->ncand && rc == XG_OK; cpt <<= 1) {
+            if (cpt == bestT) continue;
+            const int keep = s->tile[sh];
+            s->tile[sh] = cpt;
+            const float ms = time_frame(s, cfg, reps, e0, e1, &ok);
+            if (!ok) { rc = XG_ERR_HIP; break; }
+            if (ms < cur * 0.995f) { cur = ms; bestT = cpt; }
+            s->tile[sh] = keep;
         }
         s->tile[sh] = bestT;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (xg_session_set_blocks(s, all.data(), (int)all.size()) != XG_OK && rc == XG_OK) rc = XG_ERR_HIP;
     if (rc == XG_OK && path) {
         FILE *f = fopen(path, "w");
         if (!f) {
             set_error(XG_ERR_ARG, "xg_session_autotune: cannot write %s", path);
             return XG_ERR_ARG;
         }
-        fprintf(f, "# xg266 refine tiles: lgW lgH candidates_per_task (ncand %d)\n", cfg->ncand);
+        fprintf(f, "# xg266 refine schedule: regions %d wg_per_cu %d\n", s->nregions, s->persistFactor);
+        fprintf(f, "# lgW lgH candidates_per_task (ncand %d)\n", cfg->ncand);
         for (int sh = 0; sh < kNumShapes; sh++) fprintf(f, "%d %d %d\n", sh / 5 + 2, sh % 5 + 2, s->tile[sh]);
         fclose(f);
     }
@@ -1065,6 +1165,10 @@ int32_t xg_session_load_tiles(xg_session *s, const char *path)
     int n = 0;
     while (fgets(line, sizeof(line), f)) {
         int lw, lh, t;
+        if (sscanf(line, "# xg266 refine schedule: regions %d wg_per_cu %d", &lw, &lh) == 2) {
+            xg_session_set_sched(s, lw, lh);
+            continue;
+        }
         if (line[0] == '#') continue;
         if (sscanf(line, "%d %d %d", &lw, &lh, &t) == 3 && xg_session_set_tile(s, lw, lh, t) == XG_OK) n++;
     }
