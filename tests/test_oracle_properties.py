@@ -1,0 +1,309 @@
+"""Pins the oracle's functions whose reference implementation is a closed
+binary, by independent routes: brute force, library routines, closed forms and
+invariants -- each chosen so that a plausible mistake (a dropped term, a
+transposed matrix, a wrong sign, shift or rounding) fails.  CPU only."""
+import ctypes
+import fractions
+
+import numpy as np
+import pytest
+import scipy.fft
+import scipy.linalg
+
+from tests.helpers import ptr, rand_pixels
+from tests.oracle_lib import OrMe
+
+RNG = np.random.default_rng(2024)
+
+
+# ---------------------------------------------------------------- SAD -------
+def test_sad_brute_force_and_invariants(oracle):
+    for (w, h) in ((1, 1), (4, 4), (8, 16), (13, 7), (64, 64), (128, 3)):
+        a = rand_pixels(RNG, (h + 5, w + 9))
+        b = rand_pixels(RNG, (h + 5, w + 9))
+        exp = int(np.abs(a[:h, :w].astype(np.int64) - b[:h, :w]).sum())
+        assert oracle.or_ComputeSad(ptr(a), w + 9, ptr(b), w + 9, w, h) == exp
+        assert oracle.or_ComputeSad(ptr(a), w + 9, ptr(a), w + 9, w, h) == 0
+        assert oracle.or_ComputeSad(ptr(b), w + 9, ptr(a), w + 9, w, h) == exp
+    a = rand_pixels(RNG, (16, 16))
+    b = rand_pixels(RNG, (16, 16))
+    whole = oracle.or_ComputeSad(ptr(a), 16, ptr(b), 16, 16, 16)
+    parts = sum(oracle.or_ComputeSad(ptr(a, y * 16 + x), 16, ptr(b, y * 16 + x), 16, 8, 8)
+                for y in (0, 8) for x in (0, 8))
+    assert whole == parts
+
+
+def test_sadxn_are_individual_sads(oracle):
+    plane = rand_pixels(RNG, (64, 64))
+    inp = rand_pixels(RNG, (8, 8))
+    refs = [int(plane.ctypes.data + 2 * (r * 64 + c)) for r, c in RNG.integers(0, 50, size=(8, 2))]
+    exp = [oracle.or_ComputeSad(ptr(inp), 8, ctypes.c_void_p(r), 64, 8, 8) for r in refs]
+    out = np.zeros(8, np.uint32)
+    oracle.or_ComputeSadx8(ptr(inp), 8, ctypes.cast((ctypes.c_void_p * 8)(*refs), ctypes.c_void_p), 64, 8, 8, ptr(out))
+    assert out.tolist() == exp
+    out5 = np.zeros(5, np.uint32)
+    oracle.or_ComputeSadx5(ptr(inp), 8, *[ctypes.c_void_p(r) for r in refs[:5]], 64, 8, 8, ptr(out5))
+    assert out5.tolist() == exp[:5]
+
+
+# ---------------------------------------------------------------- SATD ------
+def _satd_matrix(a, b):
+    """Sum over HM-style tiles of |H d H^T| with the per-tile normalisation,
+    via scipy's Hadamard matrix (a different coefficient order than the
+    oracle's butterflies -- the sum of magnitudes does not depend on it)."""
+    h, w = a.shape
+    d = a.astype(np.int64) - b.astype(np.int64)
+    n = 8 if (w % 8 == 0 and h % 8 == 0) else 4 if (w % 4 == 0 and h % 4 == 0) else 2
+    H = scipy.linalg.hadamard(n).astype(np.int64)
+    tot = 0
+    for y in range(0, h, n):
+        for x in range(0, w, n):
+            s = int(np.abs(H @ d[y:y + n, x:x + n] @ H.T).sum())
+            tot += (s + 2) >> 2 if n == 8 else (s + 1) >> 1 if n == 4 else s
+    return tot
+
+
+def test_satd_against_hadamard_matrix(oracle):
+    for (w, h) in ((8, 8), (16, 8), (64, 32), (4, 4), (4, 16), (32, 4), (2, 2), (2, 8), (6, 2), (12, 4)):
+        a = rand_pixels(RNG, (h, w))
+        b = rand_pixels(RNG, (h, w))
+        assert oracle.or_ComputeSatd(ptr(a), w, ptr(b), w, w, h) == _satd_matrix(a, b), (w, h)
+
+
+def test_satd_closed_forms(oracle):
+    # identical blocks: 0
+    a = rand_pixels(RNG, (16, 16))
+    assert oracle.or_ComputeSatd(ptr(a), 16, ptr(a), 16, 16, 16) == 0
+    # constant difference d over an 8x8 tile: only DC = 64|d|, normalised (64|d|+2)>>2 = 16|d|
+    for d in (1, 7, 1023):
+        x = np.full((8, 8), 1023, np.uint16)
+        y = (x - d).astype(np.uint16)
+        assert oracle.or_ComputeSatd(ptr(x), 8, ptr(y), 8, 8, 8) == (64 * d + 2) >> 2
+        # 4x4 tile: DC = 16|d|, (16|d|+1)>>1
+        assert oracle.or_ComputeSatd(ptr(x), 8, ptr(y), 8, 4, 4) == (16 * d + 1) >> 1
+    # a single-sample difference spreads to |d| in all 64 coefficients
+    x = np.zeros((8, 8), np.uint16)
+    y = np.zeros((8, 8), np.uint16)
+    x[3, 5] = 100
+    assert oracle.or_ComputeSatd(ptr(x), 8, ptr(y), 8, 8, 8) == (64 * 100 + 2) >> 2
+    # Hadamard 8x8 of a block is its SATD against zero
+    z = np.zeros((8, 8), np.uint16)
+    assert oracle.or_ComputeHada8x8(ptr(a), 16) == oracle.or_ComputeSatd(ptr(a), 16, ptr(z), 8, 8, 8)
+
+
+# ---------------------------------------------------------------- rate ------
+def _se_codeword(v):
+    """Explicit signed Exp-Golomb codeword (H.266 clause 9.2): codeNum
+    2v-1 for v > 0, -2v otherwise; prefix of M zeros, a one, M info bits."""
+    k = 2 * v - 1 if v > 0 else -2 * v
+    info = bin(k + 1)[3:]
+    return "0" * len(info) + "1" + info
+
+
+def test_exp_golomb_lengths(oracle):
+    for v in range(-600, 601):
+        assert oracle.or_ExpGolombBits(v) == len(_se_codeword(v)), v
+    for v in (-16383, 16383, 4096, -4097):
+        assert oracle.or_ExpGolombBits(v) == len(_se_codeword(v))
+    assert oracle.or_MvdBits(5, -9) == len(_se_codeword(5)) + len(_se_codeword(-9))
+
+
+def test_sad_lambda_table(oracle):
+    # every entry of h266_enc_init.c:86-105 is round(64 * 2^(i/6)); index qp + 12 for 10-bit
+    for i in range(64):
+        assert oracle.or_SadLambda(i, 0) == round(64 * 2 ** (i / 6))
+    assert oracle.or_SadLambda(30, 1) == 8192 and oracle.or_SadLambda(32, 1) == 10321
+
+
+def test_mv_round_shift_is_round_half_toward_zero(oracle):
+    for s in (1, 2, 3):
+        for v in range(-70, 71):
+            q = fractions.Fraction(v, 1 << s)
+            fl = q.numerator // q.denominator
+            rem = q - fl
+            if rem > fractions.Fraction(1, 2):
+                exp = fl + 1
+            elif rem < fractions.Fraction(1, 2):
+                exp = fl
+            else:                                  # tie: toward zero
+                exp = fl + 1 if v < 0 else fl
+            assert oracle.or_MvRoundShift(v, s) == exp, (v, s)
+
+
+def test_fill_me_cand_rounds_and_dedups(oracle):
+    qx = np.array([6, 7, -5, 100, 8], np.int16)
+    qy = np.array([-2, -3, 9, 1, -2], np.int16)
+    ox = np.zeros(12, np.int16)
+    oy = np.zeros(12, np.int16)
+    n = ctypes.c_uint32()
+    oracle.or_FillMeCand(5, -2, ptr(qx), ptr(qy), 5, ptr(ox), ptr(oy), ctypes.byref(n))
+    got = list(zip(ox[:n.value].tolist(), oy[:n.value].tolist()))
+    # (0,0) first; predictor (5,-2) -> (1,0); (6,-2) -> (2,0); (7,-3) -> (2,-1);
+    # (-5,9) -> (-1,2); (100,1) -> (25,0); (8,-2) -> (2,0) duplicate
+    assert got == [(0, 0), (1, 0), (2, 0), (2, -1), (-1, 2), (25, 0)]
+
+
+# ---------------------------------------------------------------- search ----
+def _me(cur, ref, stride, x, y, w, h, pred, sw, lam, mn, mx, cands=()):
+    m = OrMe()
+    m.width, m.height = w, h
+    m.minMvX, m.minMvY = mn
+    m.maxMvX, m.maxMvY = mx
+    m.predMvX, m.predMvY = pred
+    m.searchWidth = m.searchHeight = sw
+    m.sadLambda = lam
+    m.input = cur.ctypes.data + 2 * (y * stride + x)
+    m.inputStride = stride
+    m.ref = ref.ctypes.data + 2 * (y * stride + x)
+    m.refStride = stride
+    m.candNum = len(cands)
+    for i, (a, b) in enumerate(cands):
+        m.candX[i], m.candY[i] = a, b
+    return m
+
+
+def _eg(v):
+    return len(_se_codeword(int(v)))
+
+
+def test_full_search_brute_force(oracle):
+    ref = rand_pixels(RNG, (96, 96))
+    cur = np.roll(ref, (3, -2), axis=(0, 1)).copy()
+    cur[40:56, 40:56] = np.clip(cur[40:56, 40:56].astype(np.int64) + RNG.integers(-20, 21, (16, 16)), 0, 1023)
+    for (pred, sw, lam, mn, mx) in (((0, 0), 8, 1024, (-20, -20), (20, 20)), ((9, -6), 6, 4096, (-4, -20), (20, 3)),
+                                    ((0, 0), 10, 0, (-10, -10), (10, 10))):
+        m = _me(cur, ref, 96, 40, 40, 16, 16, pred, sw, lam, mn, mx)
+        oracle.or_MotionEstFull(ctypes.byref(m))
+        cx, cy = (pred[0] + 2) >> 2, (pred[1] + 2) >> 2
+        best = None
+        for my in range(max(cy - sw, mn[1]), min(cy + sw, mx[1]) + 1):
+            for mxx in range(max(cx - sw, mn[0]), min(cx + sw, mx[0]) + 1):
+                sad = int(np.abs(cur[40:56, 40:56].astype(np.int64) - ref[40 + my:56 + my, 40 + mxx:56 + mxx]).sum())
+                cost = (sad << 8) + (_eg(4 * mxx - pred[0]) + _eg(4 * my - pred[1])) * lam
+                if best is None or cost < best[0]:
+                    best = (cost, sad, mxx, my)
+        assert (m.bestCost, m.bestSad, m.bestMvX, m.bestMvY) == (best[0], best[1], 4 * best[2], 4 * best[3])
+
+
+def test_diamond_reaches_a_local_minimum(oracle):
+    ref = rand_pixels(RNG, (128, 128))
+    cur = np.roll(ref, (5, 7), axis=(0, 1)).copy()
+    for cands in ([(0, 0)], [(0, 0), (-7, -5)], [(3, 3), (-2, -9)]):
+        m = _me(cur, ref, 128, 48, 48, 16, 16, (0, 0), 16, 1024, (-40, -40), (40, 40), cands)
+        oracle.or_MotionEstDia(ctypes.byref(m))
+        bx, by = m.bestMvX // 4, m.bestMvY // 4
+
+        def cost(mx_, my_):
+            sad = int(np.abs(cur[48:64, 48:64].astype(np.int64) - ref[48 + my_:64 + my_, 48 + mx_:64 + mx_]).sum())
+            return (sad << 8) + (_eg(4 * mx_) + _eg(4 * my_)) * 1024
+        assert m.bestCost == cost(bx, by)
+        for (cx, cy) in cands:
+            assert m.bestCost <= cost(cx, cy)
+        for (dx, dy) in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+            assert m.bestCost <= cost(bx + dx, by + dy)
+
+
+def test_subpel_never_worse_and_stays_near(oracle):
+    ref = np.pad(rand_pixels(RNG, (80, 80)), 8, mode="edge")
+    cur = np.roll(ref, (1, 2), axis=(0, 1)).copy()
+    m = _me(cur, ref, 96, 40, 40, 16, 16, (0, 0), 4, 1024, (-20, -20), (20, 20))
+    oracle.or_MotionEstFull(ctypes.byref(m))
+    ic, ix, iy = m.bestCost, m.bestMvX, m.bestMvY
+    oracle.or_FullSubPelMe(ctypes.byref(m))
+    assert m.bestCost <= ic
+    assert abs(m.bestMvX - ix) <= 3 and abs(m.bestMvY - iy) <= 3
+
+
+# ---------------------------------------------------------------- transforms
+def _matrix(oracle, typ, n):
+    T = np.zeros(n * n, np.int32)
+    assert oracle.or_TrMatrix(typ, n, ptr(T)) == 0
+    return T.reshape(n, n).astype(np.int64)
+
+
+def _float_basis(typ, n):
+    k = np.arange(n)[:, None]
+    i = np.arange(n)[None, :]
+    if typ == 0:
+        return scipy.fft.dct(np.eye(n), type=2, norm="ortho", axis=0) * 64 * np.sqrt(n)
+    if typ == 2:   # DST-VII
+        return np.sqrt(4 / (2 * n + 1)) * np.sin(np.pi * (2 * k + 1) * (i + 1) / (2 * n + 1)) * 64 * np.sqrt(n)
+    # DCT-VIII
+    return np.sqrt(4 / (2 * n + 1)) * np.cos(np.pi * (2 * k + 1) * (2 * i + 1) / (4 * n + 2)) * 64 * np.sqrt(n)
+
+
+@pytest.mark.parametrize("typ,n", [(0, 2), (0, 4), (0, 8), (0, 16), (0, 32), (0, 64),
+                                   (2, 4), (2, 8), (2, 16), (2, 32), (1, 4), (1, 8), (1, 16), (1, 32)])
+def test_matrix_against_float_basis(oracle, typ, n):
+    """Rows are frequencies, signs and orientation as the float transform;
+    every entry within 1.6 of the ideal value (the H.266 tables are tuned
+    integers, so not exactly rounded)."""
+    T = _matrix(oracle, typ, n)
+    F = _float_basis(typ, n)
+    assert np.max(np.abs(T - F)) <= 1.6, np.max(np.abs(T - F))
+    assert np.abs(T).max() <= 91
+    G = T @ T.T
+    diag = 64 * 64 * n
+    assert np.max(np.abs(np.diag(G) - diag)) <= 0.02 * diag
+    off = G - np.diag(np.diag(G))
+    assert np.max(np.abs(off)) <= 0.02 * diag
+
+
+def test_dct2_rows_sum_to_zero(oracle):
+    for n in (4, 8, 16, 32, 64):
+        T = _matrix(oracle, 0, n)
+        assert (T[0] == 64).all()
+        assert (T[1:].sum(axis=1) == 0).all()
+
+
+def test_fdct1d_shift_rounding_and_zero_out(oracle):
+    for (typ, n, keep) in ((0, 64, 32), (0, 32, 32), (2, 32, 16), (1, 32, 16), (2, 16, 16)):
+        T = _matrix(oracle, typ, n)
+        x = RNG.integers(-1023, 1024, size=(5, n)).astype(np.int32)
+        for shift in (0, 1, 6, 11):
+            out = np.zeros((n, 5), np.int32)
+            oracle.or_FDct1d(typ, n, ptr(x), n, ptr(out), 5, 5, shift)
+            full = T @ x.T.astype(np.int64)
+            exp = (full + ((1 << shift) >> 1)) >> shift
+            assert np.array_equal(out[:keep], exp[:keep])
+            assert (out[keep:] == 0).all()
+
+
+def test_fdct2d_dc_closed_form(oracle):
+    for (w, h) in ((4, 4), (8, 32), (64, 16), (64, 64)):
+        for c in (-1023, -7, 5, 1023):
+            res = np.full((h, w), c, np.int32)
+            out = np.zeros((h, w), np.int32)
+            tmp = np.zeros(w * h, np.int32)
+            lw, lh = w.bit_length() - 1, h.bit_length() - 1
+            oracle.or_FDct2WxH(ptr(res), w, ptr(out), w, w, h, ptr(tmp), lw + 1, lh + 6)
+            t0 = (64 * w * c + (1 << lw)) >> (lw + 1)
+            dc = (64 * h * t0 + (1 << (lh + 5))) >> (lh + 6)
+            exp = np.zeros((h, w), np.int64)
+            exp[0, 0] = dc
+            assert np.array_equal(out, exp), (w, h, c)
+
+
+def test_dispatcher_types_and_implicit_mts(oracle):
+    """mtsIdx -> (hor, ver) and the implicit intra-luma DST-VII rule, checked by
+    composing pinned 1-D passes: out = V (H r^T rounded)^T rounded."""
+    def two_pass(r, hor, ver):
+        h, w = r.shape
+        lw, lh = w.bit_length() - 1, h.bit_length() - 1
+        tmp = np.zeros((w, h), np.int32)
+        out = np.zeros((h, w), np.int32)
+        oracle.or_FDct1d(hor, w, ptr(np.ascontiguousarray(r)), w, ptr(tmp), h, h, lw + 1)
+        oracle.or_FDct1d(ver, h, ptr(tmp), h, ptr(out), w, w, lh + 6)
+        return out
+    r = RNG.integers(-1023, 1024, size=(8, 16)).astype(np.int32)
+    for mts, (hor, ver) in {0: (0, 0), 2: (2, 2), 3: (1, 2), 4: (2, 1), 5: (1, 1)}.items():
+        out = np.zeros((8, 16), np.int32)
+        oracle.or_FDctWxH(ptr(r), 16, ptr(out), 16, 16, 8, mts, 0, 1, 10)
+        assert np.array_equal(out, two_pass(r, hor, ver)), mts
+    out = np.zeros((8, 16), np.int32)
+    oracle.or_FDctWxH(ptr(r), 16, ptr(out), 16, 16, 8, 0, 1, 1, 10)     # intra luma, MTS on
+    assert np.array_equal(out, two_pass(r, 2, 2))
+    r2 = RNG.integers(-1023, 1024, size=(32, 8)).astype(np.int32)
+    out2 = np.zeros((32, 8), np.int32)
+    oracle.or_FDctWxH(ptr(r2), 8, ptr(out2), 8, 8, 32, 0, 1, 1, 10)     # height 32: vertical stays DCT-II
+    assert np.array_equal(out2, two_pass(r2, 2, 0))

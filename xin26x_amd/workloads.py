@@ -37,7 +37,7 @@ CHROMA_FILTER = np.array([
 _RING = [(-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1)]
 CAND32 = np.array([(dx * r, dy * r) for r in (8, 4, 12, 2) for (dx, dy) in _RING], dtype=np.int16)
 CTU = 64
-LAMBDA_REFINE = 8192            # sadLambda at QP 32, 10-bit (h266_enc_init.c:86-105, :384)
+LAMBDA_REFINE = 8192            # sadLambda at QP 30, 10-bit: index 30 + 12 (h266_enc_init.c:86-105, :384)
 
 
 def splitmix64(seed, n, stream=0):
