@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="profiling run: batched form only (no plain form, latency loop or CPU leg)")
+    ap.add_argument("--serve-frames", type=int, default=90,
+                    help="frames per stream of the 16-stream serving run at 30 fps (0 = skip serving)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -196,6 +198,20 @@ def main():
     value = frame_px / t_graph / 1e6
     plain = frame_px / t_plain / 1e6
 
+    serving = None
+    if args.serve_frames > 0 and not args.quick:
+        from xin26x_amd import serve
+        log("serving: generating the 16-stream mix")
+        srcs = serve.mix_sources()
+        log(f"serving: 16 streams at 30 fps, {args.serve_frames} frames each")
+        rt = serve.run_mix(frames=args.serve_frames, fps=30.0, srcs=srcs)
+        log("serving: saturated (all frames offered at once)")
+        sat = serve.run_mix(frames=max(10, args.serve_frames // 3), fps=0, srcs=srcs)
+        serving = {"workload": "16 streams (6x 640x360, 6x 1280x720, 4x 1920x1080) 10-bit 4:2:0; per frame: "
+                               "integer full search +/-16 around the co-located block, 32-candidate "
+                               "fractional refinement, MC; reference = previous frame",
+                   "realtime_30fps": rt, "saturated": sat}
+
     cpu = None
     if not args.no_cpu and not args.quick:
         log("CPU oracle baseline")
@@ -231,6 +247,7 @@ def main():
                      "kernel": "k_refine", "kernel_ms": round(ms.value, 4), "launches": cnt.value,
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

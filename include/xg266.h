@@ -339,10 +339,22 @@ int32_t xg_session_set_sched(xg_session *s, int32_t regions, int32_t wgPerCU);
 int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t reps, const char *path);
 int32_t xg_session_load_tiles(xg_session *s, const char *path);
 
+/* Prediction target of current-picture slot `slot` (device planes of the
+ * picture's size, unpadded).  Used by xg_refine_frac when its `pred` is NULL. */
+int32_t xg_session_set_pred(xg_session *s, int32_t slot, const xg_picture *pred);
+/* Integer full search of every installed block (one workgroup per block):
+ * +/-range around the block's predictor (mvpx, mvpy rounded to integer pel),
+ * clipped to the CU MV window (h266_enc_init.c:1462-1465), cost
+ * (SAD << 8) + (EG(4*dx - px) + EG(4*dy - py)) * sadLambda with the predictor
+ * in quarter pel (mvp >> 2), raster order, earliest wins ties.  Writes each
+ * block's base MV (mvx, mvy, 1/16 pel) in the session's device block list,
+ * where xg_refine_frac then starts.  Enqueued on the session stream. */
+int32_t xg_search_blocks(xg_session *s, int32_t range, uint32_t sadLambda, int32_t ctuSize);
 /* Fractional refinement (per block: ncand candidates, luma 8-tap + chroma
  * 4-tap interpolation, SATD, cost) followed by motion compensation of the best
- * candidate into `pred` (device planes, unpadded sizes of the current
- * picture).  Enqueued on the session stream; returns immediately. */
+ * candidate into the prediction picture: `pred` for every block when not NULL,
+ * else each block's slot target set by xg_session_set_pred.  Enqueued on the
+ * session stream; returns immediately. */
 int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture *pred,
                        int32_t mode);
 /* Device pointers of the last results: candCost[block*ncand + c], best[block]. */
@@ -356,6 +368,69 @@ int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
 int32_t xg_session_reset_timers(xg_session *s);
 int32_t xg_session_set_timing(xg_session *s, int32_t enable);
+
+/* ------------------------------------------------------------------------
+ * 3. Serving: continuous batching of frames from many concurrent streams
+ * ---------------------------------------------------------------------- */
+
+/* A server keeps each admitted stream's reference picture resident in HBM and
+ * a dispatcher thread that, whenever a device session is free, batches the
+ * oldest pending frame of every stream without a frame in flight into one
+ * integer-search launch and one refinement+MC launch.  Frames that arrive
+ * while a batch is on the GPU join the next batch; two sessions alternate so
+ * consecutive batches overlap on the device.  A stream's first frame uses
+ * itself as reference; afterwards each processed frame becomes the reference
+ * of the next. */
+typedef struct xg_server xg_server;
+
+typedef struct xg_server_cfg {
+    uint64_t      hbmBudget;       /* bytes of stream state the server may hold      */
+    int32_t       maxStreams;      /* <= 32                                            */
+    int32_t       maxBatchBlocks;  /* blocks per batch (sizes the device sessions)     */
+    int32_t       searchRange;     /* integer search range in pel (<= 32)              */
+    uint32_t      searchLambda;    /* sadLambda of the integer search                  */
+    int32_t       maxDelayUs;      /* a free session waits up to this long after the
+                                      oldest waiting frame arrived, to gather more
+                                      frames into its batch (0 = launch at once)      */
+    xg_refine_cfg refine;          /* fractional refinement + MC                       */
+} xg_server_cfg;
+
+typedef struct xg_frame_result {
+    int32_t  stream;               /* stream id                                        */
+    int32_t  status;               /* XG_OK, or XG_ERR_STATE if dropped by close       */
+    int32_t  nblocks;
+    int32_t  batchFrames;          /* frames in the batch this frame ran in            */
+    uint64_t tag;                  /* caller's tag from xg_server_submit               */
+    int64_t  submitNs, startNs, doneNs;   /* steady-clock times                       */
+} xg_frame_result;
+
+typedef struct xg_server_stats {
+    int64_t  framesDone, framesDropped, batches;
+    uint64_t bytesInUse, bytesBudget;
+    int32_t  streamsOpen;
+} xg_server_stats;
+
+int32_t xg_server_create(xg_server **out, const xg_server_cfg *cfg);
+int32_t xg_server_destroy(xg_server *srv);
+/* Admission by HBM budget: XG_ERR_BUDGET if the stream's state (two padded
+ * pictures, one prediction picture, block buffers) does not fit. */
+int32_t xg_server_open_stream(xg_server *srv, int32_t width, int32_t height, int32_t *streamId);
+/* Waits for the stream's frame in flight, drops its pending frames (reported
+ * with status XG_ERR_STATE) and releases its memory. */
+int32_t xg_server_close_stream(xg_server *srv, int32_t streamId);
+/* Queues one frame (host planes Y, Cb, Cr with strides in samples, copied
+ * before returning) and its blocks (host; pic/ref fields are ignored).  On
+ * completion the per-block results are written to `results` (host, nblocks
+ * entries, caller-owned until the frame is reported) and, when `pred` is not
+ * NULL, the prediction planes to pred[0..2] (host, strides predStride[]). */
+int32_t xg_server_submit(xg_server *srv, int32_t streamId, const xg_pixel *const planes[3],
+                         const intptr_t strides[3], const xg_block *blocks, int32_t nblocks,
+                         xg_block_result *results, xg_pixel *const pred[3], const intptr_t predStride[3],
+                         uint64_t tag);
+/* Up to `cap` completed frames; waits up to timeoutMs for the first one.
+ * Returns the number written (>= 0) or a negative xg_status. */
+int32_t xg_server_poll(xg_server *srv, xg_frame_result *out, int32_t cap, int32_t timeoutMs);
+int32_t xg_server_stats_get(xg_server *srv, xg_server_stats *stats);
 
 /* Integer full search for a grid of bw x bh blocks over a whole frame
  * (device planes; ref padded by >= range + 4).  Cost and order as

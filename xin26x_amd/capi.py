@@ -43,6 +43,22 @@ class XgBlockResult(ctypes.Structure):
     _fields_ = [("mvx", I32), ("mvy", I32), ("cost", U32), ("cand", I32)]
 
 
+class XgServerCfg(ctypes.Structure):
+    _fields_ = [("hbmBudget", ctypes.c_uint64), ("maxStreams", I32), ("maxBatchBlocks", I32),
+                ("searchRange", I32), ("searchLambda", U32), ("maxDelayUs", I32), ("refine", XgRefineCfg)]
+
+
+class XgFrameResult(ctypes.Structure):
+    _fields_ = [("stream", I32), ("status", I32), ("nblocks", I32), ("batchFrames", I32),
+                ("tag", ctypes.c_uint64), ("submitNs", ctypes.c_int64), ("startNs", ctypes.c_int64),
+                ("doneNs", ctypes.c_int64)]
+
+
+class XgServerStats(ctypes.Structure):
+    _fields_ = [("framesDone", ctypes.c_int64), ("framesDropped", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("bytesInUse", ctypes.c_uint64), ("bytesBudget", ctypes.c_uint64), ("streamsOpen", I32)]
+
+
 class XgMv(ctypes.Structure):
     _fields_ = [("mvX", I16), ("mvY", I16)]
 
@@ -119,6 +135,15 @@ PROTOS = {
     "xg_session_set_timing": (I32, [P, I32]),
     "xg_full_search_frame": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, I32,
                                    U32, I32, P, P, P]),
+    "xg_session_set_pred": (I32, [P, I32, ctypes.POINTER(XgPicture)]),
+    "xg_search_blocks": (I32, [P, I32, U32, I32]),
+    "xg_server_create": (I32, [ctypes.POINTER(P), ctypes.POINTER(XgServerCfg)]),
+    "xg_server_destroy": (I32, [P]),
+    "xg_server_open_stream": (I32, [P, I32, I32, ctypes.POINTER(I32)]),
+    "xg_server_close_stream": (I32, [P, I32]),
+    "xg_server_submit": (I32, [P, I32, P, P, P, I32, P, P, P, ctypes.c_uint64]),
+    "xg_server_poll": (I32, [P, ctypes.POINTER(XgFrameResult), I32, I32]),
+    "xg_server_stats_get": (I32, [P, ctypes.POINTER(XgServerStats)]),
     "xg_fdct_plan": (I32, [P, P, P, I32]),
     "xg_fdct_run": (I32, [P, P, P]),
 }

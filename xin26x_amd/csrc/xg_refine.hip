@@ -48,7 +48,7 @@ constexpr int kChromaLW = 42;      // per-plane chroma window stride (u16): 21 w
 constexpr int kChromaLH = 40;      // rows: H/2 + 3 + 5 at most
 constexpr int kMaxOff = 64;        // |candidate offset| limit, 1/16 pel
 constexpr int kNumShapes = 25;     // (lgW-2)*5 + (lgH-2), W, H in 4..64
-constexpr int kPicSlots = 16;
+constexpr int kPicSlots = kSessionPicSlots;
 constexpr int kRegions = 8;        // spatial work queues, one per XCD of the MI355X
 
 struct RefineArgs {
@@ -70,7 +70,7 @@ struct RefineArgs {
     int useChroma;
     const int16_t *off;            // device: offx[XG_MAX_CAND] then offy[XG_MAX_CAND]
     int doMc;                      // 0: costs only (tile timing), no block completion or MC
-    xg_picture pred;               // output planes (unpadded)
+    const xg_picture *predTab;     // per picture slot: prediction planes (unpadded) the MC writes
 };
 
 // CU MV window in 1/16 pel (h266_enc_init.c:1462-1465, int16 as stored there).
@@ -281,7 +281,7 @@ __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_bl
         for (int r = 0; r < hh; r++)
             for (int cc = lane; cc < ww; cc += 64) lds[r * kLumaLW + cc] = src[(intptr_t)r * rs + cc];
         __syncthreads();
-        const xg_plane &P0 = a.pred.plane[0];
+        const xg_plane &P0 = a.predTab[B.pic].plane[0];
         mc_plane<8>(lds, kLumaLW, rx & 15, ry & 15, B.w, B.h, P0.data + (intptr_t)B.y * P0.stride + B.x,
                     P0.stride, 0, 64);
         __syncthreads();
@@ -297,7 +297,7 @@ __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_bl
                 (pl ? lv : lu)[r * kChromaLW + col] = R.plane[1 + pl].data[so + (intptr_t)r * rs + col];
             }
         __syncthreads();
-        const xg_plane &P1 = a.pred.plane[1], &P2 = a.pred.plane[2];
+        const xg_plane &P1 = a.predTab[B.pic].plane[1], &P2 = a.predTab[B.pic].plane[2];
         const intptr_t o1 = (intptr_t)(B.y >> 1) * P1.stride + (B.x >> 1);
         const intptr_t o2 = (intptr_t)(B.y >> 1) * P2.stride + (B.x >> 1);
         mc_plane<4>(lu, kChromaLW, rx & 31, ry & 31, wc, hc, P1.data + o1, P1.stride, 0, 32);
@@ -495,6 +495,11 @@ struct xg_session {
 
     xg_picture pics[kPicSlots];
     bool picSet[kPicSlots] = {};
+    xg_picture preds[kPicSlots];       // prediction target of each current-picture slot
+    bool predSet[kPicSlots] = {};
+    xg_picture *dPred = nullptr;       // device copy of preds
+    xg_picture predUploaded[kPicSlots];
+    bool predValid = false;
     xg_picture *dPics = nullptr;
 
     std::vector<xg_block> blocks;
@@ -540,6 +545,20 @@ struct xg_session {
 };
 
 namespace xg {
+
+bool session_view(xg_session *s, SessionView *v)
+{
+    if (!s || !v) return false;
+    v->stream = s->stream;
+    v->dPics = s->dPics;
+    v->dBlocks = s->dBlocks;
+    v->hBlocks = s->blocks.data();
+    v->nblocks = s->nblocks;
+    v->hPics = s->pics;
+    v->picSet = s->picSet;
+    v->numCUs = s->numCUs;
+    return true;
+}
 
 static int shape_of(int w, int h)
 {
@@ -683,7 +702,7 @@ static bool validate_cfg(const xg_refine_cfg *cfg)
     return true;
 }
 
-static bool validate_blocks(xg_session *s, const xg_picture *pred, int ctu)
+static bool validate_blocks(xg_session *s, bool pred, int ctu)
 {
     for (int i = 0; i < s->nblocks; i++) {
         const xg_block &B = s->blocks[i];
@@ -706,9 +725,14 @@ static bool validate_blocks(xg_session *s, const xg_picture *pred, int ctu)
     if (pred)
         for (int i = 0; i < s->nblocks; i++) {
             const xg_block &B = s->blocks[i];
+            if (!s->predSet[B.pic]) {
+                set_error(XG_ERR_ARG, "block %d: no prediction picture for slot %d", i, B.pic);
+                return false;
+            }
+            const xg_picture &pp = s->preds[B.pic];
             for (int p = 0; p < 3; p++) {
                 const int sub = p ? 1 : 0;
-                const xg_plane &pl = pred->plane[p];
+                const xg_plane &pl = pp.plane[p];
                 if (!pl.data || pl.stride < pl.width || ((B.x + B.w) >> sub) > pl.width ||
                     ((B.y + B.h) >> sub) > pl.height) {
                     set_error(XG_ERR_ARG, "prediction plane %d does not cover block %d", p, i);
@@ -719,7 +743,7 @@ static bool validate_blocks(xg_session *s, const xg_picture *pred, int ctu)
     return true;
 }
 
-static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, const xg_picture *pred)
+static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, bool doMc)
 {
     int16_t off[2 * XG_MAX_CAND] = {};
     memset(&ra, 0, sizeof(ra));
@@ -745,8 +769,13 @@ static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, c
     ra.ctuSize = cfg->ctuSize;
     ra.useChroma = cfg->useChroma;
     ra.off = s->dOff;
-    ra.doMc = pred != nullptr;
-    if (pred) ra.pred = *pred;
+    ra.doMc = doMc;
+    ra.predTab = s->dPred;
+    if (doMc && (!s->predValid || memcmp(s->predUploaded, s->preds, sizeof(s->preds)) != 0)) {
+        if (!stage_upload(s, s->dPred, s->preds, sizeof(s->preds))) return false;
+        memcpy(s->predUploaded, s->preds, sizeof(s->preds));
+        s->predValid = true;
+    }
     if (!s->offValid || memcmp(off, s->hOff, sizeof(off)) != 0) {
         if (!stage_upload(s, s->dOff, off, sizeof(off))) return false;
         memcpy(s->hOff, off, sizeof(off));
@@ -831,6 +860,7 @@ int32_t xg_session_create(xg_session **out, int32_t maxBlocks)
         s->numCUs = prop.multiProcessorCount;
     bool ok = XG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) &&
               XG_HIP(hipMalloc((void **)&s->dPics, sizeof(xg_picture) * kPicSlots)) &&
+              XG_HIP(hipMalloc((void **)&s->dPred, sizeof(xg_picture) * kPicSlots)) &&
               XG_HIP(hipMalloc((void **)&s->dBlocks, sizeof(xg_block) * (size_t)maxBlocks)) &&
               XG_HIP(hipMalloc((void **)&s->dCandCost, sizeof(uint32_t) * (size_t)maxBlocks * XG_MAX_CAND)) &&
               XG_HIP(hipMalloc((void **)&s->dBest, sizeof(xg_block_result) * (size_t)maxBlocks)) &&
@@ -860,6 +890,7 @@ int32_t xg_session_destroy(xg_session *s)
     if (s->graph) hipGraphDestroy(s->graph);
     for (auto e : s->evPool) hipEventDestroy(e);
     hipFree(s->dPics);
+    hipFree(s->dPred);
     hipFree(s->dBlocks);
     hipFree(s->dTasks);
     hipFree(s->dTasksR);
@@ -900,6 +931,17 @@ int32_t xg_session_set_picture(xg_session *s, int32_t slot, const xg_picture *pi
     s->pics[slot] = *pic;
     s->picSet[slot] = true;
     if (!stage_upload(s, s->dPics + slot, pic, sizeof(xg_picture))) return XG_ERR_HIP;
+    return XG_OK;
+}
+
+int32_t xg_session_set_pred(xg_session *s, int32_t slot, const xg_picture *pred)
+{
+    if (!s || !pred || slot < 0 || slot >= kPicSlots) {
+        set_error(XG_ERR_ARG, "xg_session_set_pred: bad arguments");
+        return XG_ERR_ARG;
+    }
+    s->preds[slot] = *pred;
+    s->predSet[slot] = true;
     return XG_OK;
 }
 
@@ -957,15 +999,17 @@ int32_t xg_session_get_tile(xg_session *s, int32_t lgW, int32_t lgH)
 
 int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture *pred, int32_t mode)
 {
-    if (!s || !validate_cfg(cfg) || !pred) {
-        if (s && !pred) set_error(XG_ERR_ARG, "xg_refine_frac: prediction picture required");
-        return XG_ERR_ARG;
-    }
+    if (!s || !validate_cfg(cfg)) return XG_ERR_ARG;
     if (s->nblocks == 0) return XG_OK;
-    if (!validate_blocks(s, pred, cfg->ctuSize)) return XG_ERR_ARG;
+    if (pred)   // one prediction picture for every current-picture slot in use
+        for (int i = 0; i < s->nblocks; i++) {
+            s->preds[s->blocks[i].pic] = *pred;
+            s->predSet[s->blocks[i].pic] = true;
+        }
+    if (!validate_blocks(s, true, cfg->ctuSize)) return XG_ERR_ARG;
     if (!build_tasks(s, cfg->ncand)) return XG_ERR_HIP;
     RefineArgs ra;
-    if (!fill_args(s, cfg, ra, pred)) return XG_ERR_HIP;
+    if (!fill_args(s, cfg, ra, true)) return XG_ERR_HIP;
     s->lastNcand = cfg->ncand;
     if (mode == XG_MODE_PLAIN) return enqueue_plain(s, ra, s->timing) ? XG_OK : XG_ERR_HIP;
     if (mode == XG_MODE_BATCHED) return enqueue_batched(s, ra, s->timing) ? XG_OK : XG_ERR_HIP;
@@ -974,9 +1018,7 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
         return XG_ERR_ARG;
     }
     // graph key: everything captured by value in the kernel arguments
-    uint64_t key = mix(s->tasksKey, (uint64_t)(uintptr_t)pred->plane[0].data);
-    key = mix(key, (uint64_t)(uintptr_t)pred->plane[1].data);
-    key = mix(key, (uint64_t)(uintptr_t)pred->plane[2].data);
+    uint64_t key = s->tasksKey;
     key = mix(key, ((uint64_t)cfg->lambda << 32) | (uint32_t)cfg->ctuSize);
     key = mix(key, (uint64_t)cfg->useChroma);
     key = mix(key, ((uint64_t)s->nregions << 16) | (uint64_t)s->persistFactor);
@@ -1074,7 +1116,7 @@ static float time_frame(xg_session *s, const xg_refine_cfg *cfg, int reps, hipEv
     *ok = false;
     if (!build_tasks(s, cfg->ncand)) return 0.f;
     RefineArgs ra;
-    if (!fill_args(s, cfg, ra, nullptr)) return 0.f;
+    if (!fill_args(s, cfg, ra, false)) return 0.f;
     float best = 1e30f;
     for (int trial = 0; trial < 4; trial++) {
         if (!XG_HIP(hipEventRecord(e0, s->stream))) return 0.f;
@@ -1093,7 +1135,7 @@ int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t rep
 {
     if (!s || !validate_cfg(cfg) || reps < 1) return XG_ERR_ARG;
     if (s->nblocks == 0) return XG_OK;
-    if (!validate_blocks(s, nullptr, cfg->ctuSize)) return XG_ERR_ARG;
+    if (!validate_blocks(s, false, cfg->ctuSize)) return XG_ERR_ARG;
     hipEvent_t e0, e1;
     if (!XG_HIP(hipEventCreate(&e0)) || !XG_HIP(hipEventCreate(&e1))) return XG_ERR_HIP;
     int32_t rc = XG_OK;

@@ -43,6 +43,20 @@ void unpack2d(void *dst, intptr_t dstStride, const void *src, int w, int h, size
 // Releases the forward-transform plan a session owns (xg_fdct.hip).
 void fdct_release(xg_session *s);
 
+// What other translation units need of a session (xg_refine.hip owns it).
+struct SessionView {
+    hipStream_t stream;
+    const xg_picture *dPics;       // device picture table
+    xg_block *dBlocks;             // device block list
+    const xg_block *hBlocks;       // host copy of the block list
+    int nblocks;
+    const xg_picture *hPics;       // host picture table
+    const bool *picSet;
+    int numCUs;
+};
+bool session_view(xg_session *s, SessionView *v);
+constexpr int kSessionPicSlots = 64;
+
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 }  // namespace xg

@@ -239,6 +239,44 @@ __global__ void __launch_bounds__(256) k_subpel_one(const uint16_t *reg, int rst
 
 static inline int clampi_h(int v, int lo, int hi) { return std::min(std::max(v, lo), hi); }
 
+// Integer search of every block of a session (one workgroup per block): the
+// window +/-range around the rounded predictor, clipped to the CU MV window,
+// staged in LDS with the block; result written back as the block's base MV.
+__global__ void __launch_bounds__(256) k_search_blocks(const xg_picture *pics, xg_block *blocks, int range,
+                                                       uint32_t lambda, int ctu)
+{
+    extern __shared__ uint16_t smem[];
+    __shared__ unsigned long long red[4];
+    const int b = blockIdx.x;
+    xg_block B = blocks[b];
+    const xg_picture &P = pics[B.pic], &R = pics[B.ref];
+    const int fw = P.plane[0].width, fh = P.plane[0].height;
+    const int mnx = (int16_t)(-B.x - ctu - 8), mny = (int16_t)(-B.y - ctu - 8);
+    const int mxx = (int16_t)(fw + (ctu - B.w + 8) - B.x), mxy = (int16_t)(fh + (ctu - B.h + 8) - B.y);
+    // predictor: 1/16 pel -> quarter pel (>> 2) -> integer rounded as Xin266FillMeCand does
+    // ((v + 2) >> 2, h266_analyze_inter_cu.c:271-272), kept inside the MV window
+    const int cx = min(max(((B.mvpx >> 2) + 2) >> 2, mnx), mxx), cy = min(max(((B.mvpy >> 2) + 2) >> 2, mny), mxy);
+    const int x0 = max(cx - range, mnx), x1 = min(cx + range, mxx);
+    const int y0 = max(cy - range, mny), y1 = min(cy + range, mxy);
+    const int ws = x1 - x0 + B.w, wh = y1 - y0 + B.h;
+    uint16_t *win = smem, *cur = smem + ws * wh;
+    const intptr_t rs = R.plane[0].stride, cs = P.plane[0].stride;
+    const uint16_t *rp = R.plane[0].data + (intptr_t)(B.y + y0) * rs + (B.x + x0);
+    for (int i = threadIdx.x; i < ws * wh; i += blockDim.x) win[i] = rp[(intptr_t)(i / ws) * rs + (i % ws)];
+    const uint16_t *cp = P.plane[0].data + (intptr_t)B.y * cs + B.x;
+    for (int i = threadIdx.x; i < B.w * B.h; i += blockDim.x) cur[i] = cp[(intptr_t)(i / B.w) * cs + (i % B.w)];
+    __syncthreads();
+    const unsigned long long k = full_search_core(win, ws, cur, B.w, B.h, x0, x1, y0, y1, lambda, B.mvpx >> 2,
+                                                  B.mvpy >> 2, red);
+    if (threadIdx.x == 0) {
+        const int i = (int)(uint32_t)k, nx = x1 - x0 + 1;
+        B.mvx = 16 * (x0 + i % nx);
+        B.mvy = 16 * (y0 + i / nx);
+        blocks[b].mvx = B.mvx;
+        blocks[b].mvy = B.mvy;
+    }
+}
+
 }  // namespace xg
 
 using namespace xg;
@@ -361,6 +399,40 @@ void Xg266FullSubPelMe(xg_me_struct *me)
     me->bestMv.mvX = (int16_t)hout[0];
     me->bestMv.mvY = (int16_t)hout[1];
     me->bestCost = hout[2];
+}
+
+int32_t xg_search_blocks(xg_session *s, int32_t range, uint32_t sadLambda, int32_t ctuSize)
+{
+    SessionView v;
+    if (!session_view(s, &v) || range < 0 || range > 32 || ctuSize < 4 || ctuSize > 256) {
+        set_error(XG_ERR_ARG, "xg_search_blocks: bad arguments");
+        return XG_ERR_ARG;
+    }
+    if (v.nblocks == 0) return XG_OK;
+    int maxw = 0, maxh = 0;
+    for (int i = 0; i < v.nblocks; i++) {
+        const xg_block &B = v.hBlocks[i];
+        if (B.pic >= kSessionPicSlots || B.ref >= kSessionPicSlots || !v.picSet[B.pic] || !v.picSet[B.ref]) {
+            set_error(XG_ERR_ARG, "xg_search_blocks: block %d picture slot not set", i);
+            return XG_ERR_ARG;
+        }
+        const xg_picture &P = v.hPics[B.pic], &R = v.hPics[B.ref];
+        if (B.x < 0 || B.y < 0 || B.x + B.w > P.plane[0].width || B.y + B.h > P.plane[0].height ||
+            R.plane[0].width != P.plane[0].width || R.plane[0].height != P.plane[0].height ||
+            R.plane[0].pad < ctuSize + 16) {
+            set_error(XG_ERR_ARG, "xg_search_blocks: block %d outside its picture or reference under-padded", i);
+            return XG_ERR_ARG;
+        }
+        maxw = std::max(maxw, (int)B.w);
+        maxh = std::max(maxh, (int)B.h);
+    }
+    const size_t lds = ((size_t)(2 * range + maxw) * (2 * range + maxh) + (size_t)maxw * maxh) * 2;
+    if (lds > 64 * 1024) {
+        set_error(XG_ERR_ARG, "xg_search_blocks: window exceeds the LDS budget");
+        return XG_ERR_ARG;
+    }
+    k_search_blocks<<<v.nblocks, 256, lds, v.stream>>>(v.dPics, v.dBlocks, range, sadLambda, ctuSize);
+    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
 int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t bw, int32_t bh,
