@@ -29,6 +29,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+TUNED = os.path.join(ROOT, "xin26x_amd", "tuned", "refine_1080p_cand32.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_refine_1080p_pmc.md")
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12     # lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 METRIC = "ME+MC Mpixel/s at 1080p (batched vs plain, bit-exact); p50/p99 frame latency"
@@ -89,7 +91,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default tiles and schedule")
+    ap.add_argument("--autotune", action="store_true", help="tune live even when a stored table exists")
+    ap.add_argument("--tiles", default=TUNED, help="stored schedule + tile table (written by --save-tiles)")
+    ap.add_argument("--save-tiles", default=None, help="write the live-tuned table here")
     ap.add_argument("--quick", action="store_true",
                     help="profiling run: batched form only (no plain form, latency loop or CPU leg)")
     ap.add_argument("--serve-frames", type=int, default=90,
@@ -133,9 +138,17 @@ def main():
     capi.check(lib.xg_session_set_picture(s, 1, ref.ref()), "picture")
     barr = blocks_to_struct(fp.blocks)
     capi.check(lib.xg_session_set_blocks(s, barr.ctypes.data, n), "blocks")
-    if not args.no_autotune:
-        log("autotuning candidates-per-task per block shape")
-        capi.check(lib.xg_session_autotune(s, ctypes.byref(cfg), 2, None), "autotune")
+    tuning = "default"
+    if args.no_autotune:
+        pass
+    elif args.autotune or args.save_tiles or not os.path.exists(args.tiles):
+        log("autotuning the schedule and candidates-per-task per block shape")
+        path = args.save_tiles.encode() if args.save_tiles else None
+        capi.check(lib.xg_session_autotune(s, ctypes.byref(cfg), 2, path), "autotune")
+        tuning = "live"
+    else:
+        capi.check(lib.xg_session_load_tiles(s, args.tiles.encode()), "load tiles")
+        tuning = os.path.relpath(args.tiles, ROOT)
 
     def run(mode, steps):
         for _ in range(steps):
@@ -191,7 +204,8 @@ def main():
 
     frame_px = W * H
     pic_bytes = 2 * (W * H + 2 * (W // 2) * (H // 2))
-    alg_bytes = 2 * pic_bytes + n * nc * 4          # current + reference read once, costs written once
+    # current + reference read once; candidate costs, best records (16 B) and the prediction written once
+    alg_bytes = 2 * pic_bytes + n * nc * 4 + n * 16 + pic_bytes
     k_s = ms.value / 1e3
     achieved = alg_bytes / k_s / 1e9
     cand_px = int((fp.blocks[:, 2] * fp.blocks[:, 3]).sum()) * nc
@@ -211,6 +225,16 @@ def main():
                                "integer full search +/-16 around the co-located block, 32-candidate "
                                "fractional refinement, MC; reference = previous frame",
                    "realtime_30fps": rt, "saturated": sat}
+
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (calibrated)
+    traffic, traffic_src = None, None
+    try:
+        with open(PMC_SUMMARY) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    traffic, traffic_src = int(ln.split(":")[1]), os.path.relpath(PMC_SUMMARY, ROOT)
+    except OSError:
+        pass
 
     cpu = None
     if not args.no_cpu and not args.quick:
@@ -233,7 +257,7 @@ def main():
         "config": {"workload": "1080p fractional refinement + MC: 1920x1080 10-bit 4:2:0, "
                                f"{n} QTMT blocks, {nc} fractional candidates/block, 8-tap luma + 4-tap "
                                "chroma interpolation + SATD, MC of the best candidate",
-                   "frame": f"{W}x{H}", "blocks": n, "candidates": nc, "global_batch": 1,
+                   "frame": f"{W}x{H}", "blocks": n, "candidates": nc, "global_batch": 1, "tiles": tuning,
                    "parallelism": "single GPU"},
         "plain_value": round(plain, 2),
         "batched_vs_plain": round(value / plain, 3),
@@ -243,8 +267,8 @@ def main():
         "block_cand_evals_per_s": round(n * nc / t_graph, 1),
         "luma_cand_px_per_s": round(cand_px / t_graph, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_refine", "kernel_ms": round(ms.value, 4), "launches": cnt.value,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "k_refine_persistent", "kernel_ms": round(ms.value, 4), "launches": cnt.value,
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -343,8 +343,9 @@ int32_t xg_session_load_tiles(xg_session *s, const char *path);
  * picture's size, unpadded).  Used by xg_refine_frac when its `pred` is NULL. */
 int32_t xg_session_set_pred(xg_session *s, int32_t slot, const xg_picture *pred);
 /* Integer full search of every installed block (one workgroup per block):
- * +/-range around the block's predictor (mvpx, mvpy rounded to integer pel),
- * clipped to the CU MV window (h266_enc_init.c:1462-1465), cost
+ * +/-range around the centre ((mvp >> 2) + 2) >> 2 (the predictor in quarter
+ * pel rounded to integer pel as h266_analyze_inter_cu.c:271-272, clamped into
+ * the window), clipped to the CU MV window (h266_enc_init.c:1462-1465), cost
  * (SAD << 8) + (EG(4*dx - px) + EG(4*dy - py)) * sadLambda with the predictor
  * in quarter pel (mvp >> 2), raster order, earliest wins ties.  Writes each
  * block's base MV (mvx, mvy, 1/16 pel) in the session's device block list,

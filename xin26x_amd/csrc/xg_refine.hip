@@ -429,55 +429,63 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(64) k_refine(RefineArgs a)
-{
-    __shared__ uint16_t lds[kLumaLW * kLumaLH];
-    __shared__ uint32_t csum[64];
-    __shared__ int16_t sOff[2 * XG_MAX_CAND];
-    __shared__ int s_task, s_last;
-    __shared__ unsigned long long s_key;
-    for (int i = threadIdx.x; i < 2 * XG_MAX_CAND; i += 64) sOff[i] = a.off[i];
+#define XG_REFINE_LDS                                        \
+    __shared__ uint16_t lds[kLumaLW * kLumaLH];              \
+    __shared__ uint32_t csum[64];                            \
+    __shared__ int16_t sOff[2 * XG_MAX_CAND];                \
+    __shared__ int s_task, s_last;                           \
+    __shared__ unsigned long long s_key;                     \
+    for (int i = threadIdx.x; i < 2 * XG_MAX_CAND; i += 64) sOff[i] = a.off[i]; \
     __syncthreads();
-    if (a.queue) {
-        // Workgroups are dispatched to the XCDs round-robin, so workgroup i
-        // runs on XCD i % 8.  Each drains the queue of its own spatial region
-        // first -- neighbouring blocks share reference samples in that XCD's
-        // L2 -- then steals from the other regions in turn.
-        int region = (int)(blockIdx.x % kRegions) % a.nregions, exhausted = 0;
-        for (;;) {
-            if (threadIdx.x == 0) {
-                int task = -1;
-                while (exhausted < a.nregions) {
-                    const int beg = a.regionBegin[region], cnt = a.regionBegin[region + 1] - beg;
-                    const int i = cnt > 0 ? atomicAdd(&a.queue[region], 1) : 0;
-                    if (i < cnt) {
-                        task = beg + i;
-                        break;
-                    }
-                    region = (region + 1) % a.nregions;
-                    exhausted++;
-                }
-                s_task = task;
-            }
-            __syncthreads();
-            const int t = s_task;
-            __syncthreads();
-            if (t < 0) break;
-            refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
-        }
-        // the last workgroup to leave resets the queues for the next launch
+
+// Batched form: a persistent grid drains the per-region task queues.
+__global__ void __launch_bounds__(64) k_refine_persistent(RefineArgs a)
+{
+    XG_REFINE_LDS
+    // Workgroups are dispatched to the XCDs round-robin, so workgroup i runs on
+    // XCD i % 8.  Each drains the queue of its own spatial region first --
+    // neighbouring blocks share reference samples in that XCD's L2 -- then
+    // steals from the other regions in turn.
+    int region = (int)(blockIdx.x % kRegions) % a.nregions, exhausted = 0;
+    for (;;) {
         if (threadIdx.x == 0) {
-            __threadfence();
-            if (atomicAdd(&a.queue[kRegions], 1) == (int)gridDim.x - 1) {
-                for (int r = 0; r < a.nregions; r++) atomicExch(&a.queue[r], 0);
-                atomicExch(&a.queue[kRegions], 0);
+            int task = -1;
+            while (exhausted < a.nregions) {
+                const int beg = a.regionBegin[region], cnt = a.regionBegin[region + 1] - beg;
+                const int i = cnt > 0 ? atomicAdd(&a.queue[region], 1) : 0;
+                if (i < cnt) {
+                    task = beg + i;
+                    break;
+                }
+                region = (region + 1) % a.nregions;
+                exhausted++;
             }
+            s_task = task;
         }
-    } else {
-        const int t = a.taskBegin + (int)blockIdx.x;
-        if (t < a.taskEnd) refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
+        __syncthreads();
+        const int t = s_task;
+        __syncthreads();
+        if (t < 0) break;
+        refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
+    }
+    // the last workgroup to leave resets the queues for the next launch
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&a.queue[kRegions], 1) == (int)gridDim.x - 1) {
+            for (int r = 0; r < a.nregions; r++) atomicExch(&a.queue[r], 0);
+            atomicExch(&a.queue[kRegions], 0);
+        }
     }
 }
+
+// Plain form: one launch per block shape, one workgroup per task.
+__global__ void __launch_bounds__(64) k_refine_shape(RefineArgs a)
+{
+    XG_REFINE_LDS
+    const int t = a.taskBegin + (int)blockIdx.x;
+    if (t < a.taskEnd) refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
+}
+#undef XG_REFINE_LDS
 
 }  // namespace xg
 
@@ -801,7 +809,7 @@ static bool enqueue_batched(xg_session *s, RefineArgs ra, bool timed)
         if (e0 < 0 || e1 < 0) timed = false;
     }
     if (timed) hipEventRecord(s->evPool[e0], s->stream);
-    k_refine<<<grid, 64, 0, s->stream>>>(ra);
+    k_refine_persistent<<<grid, 64, 0, s->stream>>>(ra);
     if (timed) {
         hipEventRecord(s->evPool[e1], s->stream);
         s->evPending[0].push_back({e0, e1});
@@ -826,7 +834,7 @@ static bool enqueue_plain(xg_session *s, RefineArgs ra, bool timed)
         ra.tasks = s->dTasks;
         ra.taskBegin = b0;
         ra.taskEnd = b1;
-        k_refine<<<b1 - b0, 64, 0, s->stream>>>(ra);
+        k_refine_shape<<<b1 - b0, 64, 0, s->stream>>>(ra);
     }
     if (timed) {
         hipEventRecord(s->evPool[e1], s->stream);
