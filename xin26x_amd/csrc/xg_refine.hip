@@ -17,12 +17,14 @@
 //    per block shape and ordered largest first (per-shape work queues);
 //  * the task's reference window -- every sample any of its candidates'
 //    filters touch -- is staged once into LDS and reused by all candidates;
-//  * lane = candidate * W + column: each lane walks its column down the block,
-//    keeping the 8 (luma) / 4 (chroma) first-pass values of the separable
-//    filter in registers, so the intermediate never leaves the VGPRs;
+//  * lane = unit * (W / 2) + m: each lane walks columns 2m and 2m+1 of one
+//    candidate (chroma: of one candidate's Cb or Cr) down the block, keeping
+//    the first-pass values of the separable filter in registers as packed
+//    16-bit row pairs, so the intermediate never leaves the VGPRs; both
+//    filter passes are v_dot2_i32_i16 on sample pairs;
 //  * SATD tiles (8x8, 4x4 or 2x2 as the block shape allows) are transformed
-//    vertically in registers and horizontally across lanes with DPP and
-//    ds_swizzle butterflies; sums reduce within the wavefront;
+//    on packed int16: vertically in registers, horizontally inside the
+//    register and across lanes with DPP; sums reduce within the wavefront;
 //  * each task folds its best (cost << 32 | candidate) into a per-block
 //    64-bit atomic minimum and counts its candidates into a per-block
 //    counter; the task that completes a block performs the motion
@@ -96,127 +98,254 @@ __device__ __forceinline__ int hrow_lds(const uint16_t *p, const int *hc)
     return (int)(int16_t)((s + kHOffset) >> kHShift);
 }
 
-// Luma SATD of one pass of candidates; lane = cand_local * W + x.
-template <int W, int TS>
-__device__ __forceinline__ void luma_pass(const RefineArgs &a, const xg_block &B, const MvWin &mw,
-                                          const uint16_t *lds, int wx0, int wy0,
-                                          const uint16_t *cur, intptr_t cs, int c, bool valid, int ci,
-                                          uint32_t *csum, const int16_t *sOff)
+// ---------------------------------------------------------------------------
+// Interpolation + SATD of one plane for a pass of candidates, two adjacent
+// columns per lane.
+//
+// Each lane owns columns 2m and 2m+1 of one unit (a candidate, or for chroma
+// a candidate's Cb or Cr plane).  A window row is fetched as dword-aligned
+// LDS words and realigned with v_alignbit, so every 16-bit sample pair
+// (p[i], p[i+1]) the filters need is one register; the first pass is then
+// TAPS/2 v_dot2_i32_i16 per column.  First-pass values are kept as packed
+// consecutive-row pairs (t[r], t[r+1]), so the second pass is TAPS/2 more
+// dot2.  Differences, the vertical Walsh-Hadamard stages and the first
+// horizontal stages run on packed int16 (the largest magnitude before the
+// last stage is 8 * 4 * 1023 < 32768); the last stage uses
+// |a + b| + |a - b| = 2 * max(|a|, |b|), so no value ever needs 17 bits.
+// ---------------------------------------------------------------------------
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) uint32_t gdword;
+
+__device__ __forceinline__ v2s as_v2s(uint32_t v) { return __builtin_bit_cast(v2s, v); }
+__device__ __forceinline__ uint32_t as_u32(v2s v) { return __builtin_bit_cast(uint32_t, v); }
+// (lo & 0xFFFF) | (hi << 16) in one v_perm_b32: selector bytes 0-1 take lo's
+// low bytes, 4-5 hi's.
+__device__ __forceinline__ uint32_t pack16(int lo, int hi)
 {
-    const int lane = threadIdx.x;
-    const int xl = lane & (W - 1);
-    const int H = B.h;
-    const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
-    const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
-    const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
-    int hc[8], vc[8];
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
+}
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c)
+{
+    return __builtin_amdgcn_sdot2(as_v2s(a), as_v2s(b), c, false);
+}
+__device__ __forceinline__ uint32_t align16(uint32_t hi, uint32_t lo, uint32_t sh)
+{
+    return __builtin_amdgcn_alignbit(hi, lo, sh);
+}
+
+// Lane exchange of a packed pair with lane ^ M (M = 1, 2: DPP quad permutes).
+template <int M>
+__device__ __forceinline__ v2s xlane(v2s v)
+{
+    return as_v2s((uint32_t)shfl_xor<M>((int)as_u32(v)));
+}
+
+// Butterfly (a, b) -> (a + b, a - b) between the two halves of a register.
+__device__ __forceinline__ v2s bfly_in(v2s v)
+{
+    const v2s sw = __builtin_shufflevector(v, v, 1, 0);
+    return v * (v2s){1, -1} + sw;
+}
+
+// Butterfly across lanes: with a the value of the lane whose bit M is clear
+// and b its partner's, that lane keeps a + b and the partner keeps a - b.
+template <int M>
+__device__ __forceinline__ v2s bfly_lane(v2s v, int lane)
+{
+    const short sg = (lane & M) ? -1 : 1;
+    return v * (v2s){sg, sg} + xlane<M>(v);
+}
+
+__device__ __forceinline__ v2s absv(v2s v) { return __builtin_elementwise_max(v, (v2s){0, 0} - v); }
+
+// The last horizontal stage (across lane ^ LM, or inside the register when
+// LM == 0) is folded into the absolute sum: each lane adds max(|own|,
+// |partner|) per half, so the pair together adds |a + b| + |a - b|.
+// SATD (HM normalisation) of one TS x TS tile whose rows d[0..TS-1] hold, in
+// each of the TS/2 lanes of the tile, the differences of two adjacent
+// columns.  Every lane of the tile receives the tile's SATD.
+template <int TS>
+__device__ __forceinline__ int tile_satd2(v2s *d, int lane)
+{
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        hc[j] = c_luma[0][rx & 15][j];
-        vc[j] = c_luma[0][ry & 15][j];
+    for (int len = 1; len < TS; len <<= 1)          // vertical transform in registers
+#pragma unroll
+        for (int i = 0; i < TS; i += len << 1)
+#pragma unroll
+            for (int j = i; j < i + len; j++) {
+                const v2s a = d[j], b = d[j + len];
+                d[j] = a + b;
+                d[j + len] = a - b;
+            }
+    // horizontal stages, each over all rows before the next (independent DPP
+    // sources between a write and its cross-lane read)
+    if constexpr (TS >= 4) {
+#pragma unroll
+        for (int r = 0; r < TS; r++) d[r] = bfly_in(d[r]);
     }
-    const uint16_t *wp = lds + ((ry >> 4) - 3 - wy0) * kLumaLW + ((rx >> 4) - 3 - wx0) + xl;
-    const uint16_t *cp = cur + xl;
-    int t[TS + 7];
+    if constexpr (TS >= 8) {
 #pragma unroll
-    for (int k = 0; k < 7; k++) t[k] = hrow_lds<8>(wp + k * kLumaLW, hc);
+        for (int r = 0; r < TS; r++) d[r] = bfly_lane<1>(d[r], lane);
+    }
+    constexpr int LM = TS == 2 ? 0 : TS == 4 ? 1 : 2;
+    int acc = 0;
+#pragma unroll
+    for (int r = 0; r < TS; r++) d[r] = absv(d[r]);
+#pragma unroll
+    for (int r = 0; r < TS; r++) {
+        v2s mx;
+        if constexpr (LM == 0) mx = __builtin_elementwise_max(d[r], __builtin_shufflevector(d[r], d[r], 1, 0));
+        else mx = __builtin_elementwise_max(d[r], xlane<LM>(d[r]));
+        acc = dot2(as_u32(mx), 0x00010001u, acc);
+    }
+    if constexpr (TS >= 4) acc += shfl_xor<1>(acc);
+    if constexpr (TS >= 8) acc += shfl_xor<2>(acc);
+    if constexpr (TS == 8) return (acc + 2) >> 2;
+    else if constexpr (TS == 4) return (acc + 1) >> 1;
+    else return acc;
+}
+
+// First-pass values of window row `row` (sample index of the lane's first
+// column in the window, any parity) for the lane's two columns.
+template <int TAPS>
+__device__ __forceinline__ void hrow2(const uint16_t *win, int idx, const uint32_t *hp, int &t0, int &t1)
+{
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(win) + (idx >> 1);
+    const uint32_t sh = (uint32_t)(idx & 1) << 4;
+    constexpr int ND = TAPS / 2 + 2;                // dwords covering TAPS + 1 samples at any parity
+    uint32_t D[ND], E[ND - 1];
+#pragma unroll
+    for (int j = 0; j < ND; j++) D[j] = q[j];
+#pragma unroll
+    for (int j = 0; j < ND - 1; j++) E[j] = align16(D[j + 1], D[j], sh);      // (p[2j], p[2j+1])
+    int s0 = kHOffset, s1 = kHOffset;
+#pragma unroll
+    for (int j = 0; j < TAPS / 2; j++) {
+        s0 = dot2(E[j], hp[j], s0);
+        s1 = dot2(align16(E[j + 1], E[j], 16), hp[j], s1);                  // (p[2j+1], p[2j+2])
+    }
+    t0 = s0 >> kHShift;
+    t1 = s1 >> kHShift;
+}
+
+// One pass over up to 64 / (W / 2) units of one block.  unitPlane(u) and the
+// window origin are supplied per lane by the caller.
+//   win:   LDS window of the lane's plane; ls its row stride (even)
+//   wx, wy: the lane's candidate's first tap position in the window
+//   cur:   the plane's current block (global, dword aligned), cs its stride
+template <int TAPS, int W, int TS>
+__device__ __forceinline__ int interp_satd_pass(const uint16_t *win, int ls, int wx, int wy, int fx, int fy,
+                                                const uint16_t *cur, intptr_t cs, int H, int m, int lane)
+{
+    uint32_t hp[TAPS / 2], vp[TAPS / 2];
+    {
+        const int8_t *ch = taps_for<TAPS>(fx, 0), *cv = taps_for<TAPS>(fy, 0);
+#pragma unroll
+        for (int j = 0; j < TAPS / 2; j++) {
+            hp[j] = pack16(ch[2 * j], ch[2 * j + 1]);
+            vp[j] = pack16(cv[2 * j], cv[2 * j + 1]);
+        }
+    }
+    constexpr int NP = TS + TAPS - 2;               // packed row pairs per column in a tile
+    uint32_t P0[NP], P1[NP];
+    int tl0, tl1;
+    const int base = wy * ls + wx + 2 * m;
+    {
+        int a0, a1;
+        hrow2<TAPS>(win, base, hp, a0, a1);
+#pragma unroll
+        for (int r = 1; r < TAPS - 1; r++) {
+            int b0, b1;
+            hrow2<TAPS>(win, base + r * ls, hp, b0, b1);
+            P0[r - 1] = pack16(a0, b0);
+            P1[r - 1] = pack16(a1, b1);
+            a0 = b0;
+            a1 = b1;
+        }
+        tl0 = a0;
+        tl1 = a1;
+    }
+    const gdword *gc = (const gdword *)(cur + 2 * m);
+    const intptr_t csw = cs >> 1;
     int acc = 0;
     for (int y0 = 0; y0 < H; y0 += TS) {
+        uint32_t cw[TS];
 #pragma unroll
-        for (int k = 0; k < TS; k++) t[7 + k] = hrow_lds<8>(wp + (y0 + 7 + k) * kLumaLW, hc);
-        int d[TS];
+        for (int k = 0; k < TS; k++) cw[k] = gc[(intptr_t)(y0 + k) * csw];
 #pragma unroll
         for (int k = 0; k < TS; k++) {
-            int s = 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++) s += vc[j] * t[k + j];
-            d[k] = (int)cp[(intptr_t)(y0 + k) * cs] - vfinal_pix(s);
+            int b0, b1;
+            hrow2<TAPS>(win, base + (y0 + TAPS - 1 + k) * ls, hp, b0, b1);
+            P0[TAPS - 2 + k] = pack16(tl0, b0);
+            P1[TAPS - 2 + k] = pack16(tl1, b1);
+            tl0 = b0;
+            tl1 = b1;
         }
-        const int ts = tile_satd<TS>(d, lane);
-        if ((xl & (TS - 1)) == 0) acc += ts;
+        v2s d[TS];
 #pragma unroll
-        for (int k = 0; k < 7; k++) t[k] = t[TS + k];
-    }
-    acc = group_sum<W>(acc);
-    if (xl == 0 && valid) csum[ci] = (uint32_t)acc;
-}
-
-// Chroma SATD (Cb + Cr) of one pass; lane = cand_local * WC + x.
-template <int WC, int TC>
-__device__ __forceinline__ void chroma_pass(const RefineArgs &a, const xg_block &B, const MvWin &mw,
-                                            const uint16_t *ldsU, const uint16_t *ldsV, int cx0, int cy0,
-                                            const uint16_t *curU, const uint16_t *curV, intptr_t cs,
-                                            int c, bool valid, int ci, uint32_t *csum, const int16_t *sOff)
-{
-    const int lane = threadIdx.x;
-    const int xl = lane & (WC - 1);
-    const int HC = B.h >> 1;
-    const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
-    const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
-    const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
-    int hc[4], vc[4];
+        for (int k = 0; k < TS; k++) {
+            int s0 = kVOffset, s1 = kVOffset;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        hc[j] = c_chroma[rx & 31][j];
-        vc[j] = c_chroma[ry & 31][j];
-    }
-    const int off = ((ry >> 5) - 1 - cy0) * kChromaLW + ((rx >> 5) - 1 - cx0) + xl;
-    int acc = 0;
-#pragma unroll
-    for (int pl = 0; pl < 2; pl++) {
-        const uint16_t *wp = (pl ? ldsV : ldsU) + off;
-        const uint16_t *cp = (pl ? curV : curU) + xl;
-        int t[TC + 3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) t[k] = hrow_lds<4>(wp + k * kChromaLW, hc);
-        for (int y0 = 0; y0 < HC; y0 += TC) {
-#pragma unroll
-            for (int k = 0; k < TC; k++) t[3 + k] = hrow_lds<4>(wp + (y0 + 3 + k) * kChromaLW, hc);
-            int d[TC];
-#pragma unroll
-            for (int k = 0; k < TC; k++) {
-                int s = 0;
-#pragma unroll
-                for (int j = 0; j < 4; j++) s += vc[j] * t[k + j];
-                d[k] = (int)cp[(intptr_t)(y0 + k) * cs] - vfinal_pix(s);
+            for (int j = 0; j < TAPS / 2; j++) {
+                s0 = dot2(P0[k + 2 * j], vp[j], s0);
+                s1 = dot2(P1[k + 2 * j], vp[j], s1);
             }
-            const int ts = tile_satd<TC>(d, lane);
-            if ((xl & (TC - 1)) == 0) acc += ts;
+            d[k] = as_v2s(cw[k]) - as_v2s(pack16(clip_pix(s0 >> kVShift), clip_pix(s1 >> kVShift)));
+        }
+        const int ts = tile_satd2<TS>(d, lane);
+        if ((m & (TS / 2 - 1)) == 0) acc += ts;
 #pragma unroll
-            for (int k = 0; k < 3; k++) t[k] = t[TC + k];
+        for (int k = 0; k < TAPS - 2; k++) {
+            P0[k] = P0[TS + k];
+            P1[k] = P1[TS + k];
         }
     }
-    acc = group_sum<WC>(acc);
-    if (xl == 0 && valid) csum[ci] += (uint32_t)acc;
+    return acc;
 }
 
+// Luma SATD of every candidate of a task; lane = unit * (W / 2) + m.
 template <int W, int TS>
 __device__ __forceinline__ void luma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw,
                                           const uint16_t *lds, int wx0, int wy0, const uint16_t *cur,
                                           intptr_t cs, int c0, int nc, uint32_t *csum, const int16_t *sOff)
 {
-    constexpr int CP = 64 / W;
-    const int cl = threadIdx.x / W;
+    constexpr int L = W / 2, CP = 64 / L;
+    const int lane = threadIdx.x, m = lane & (L - 1), u = lane / L;
     for (int base = 0; base < nc; base += CP) {
-        const int ci = base + cl;
-        const bool valid = ci < nc;
-        luma_pass<W, TS>(a, B, mw, lds, wx0, wy0, cur, cs, c0 + min(ci, nc - 1), valid, ci, csum, sOff);
+        const int ci = base + u;
+        const int c = c0 + min(ci, nc - 1);
+        const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
+        const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
+        const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
+        int acc = interp_satd_pass<8, W, TS>(lds, kLumaLW, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0, rx & 15,
+                                             ry & 15, cur, cs, B.h, m, lane);
+        acc = group_sum<L>(acc);
+        if (m == 0 && ci < nc) csum[ci] = (uint32_t)acc;
     }
 }
 
+// Chroma SATD (Cb + Cr) of every candidate of a task; unit = 2 * candidate +
+// plane, lane = unit * (WC / 2) + m, so a candidate's two planes are adjacent.
 template <int WC, int TC>
 __device__ __forceinline__ void chroma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw,
                                             const uint16_t *ldsU, const uint16_t *ldsV, int cx0, int cy0,
                                             const uint16_t *curU, const uint16_t *curV, intptr_t cs, int c0,
                                             int nc, uint32_t *csum, const int16_t *sOff)
 {
-    constexpr int CP = 64 / WC;
-    const int cl = threadIdx.x / WC;
-    for (int base = 0; base < nc; base += CP) {
-        const int ci = base + cl;
-        const bool valid = ci < nc;
-        chroma_pass<WC, TC>(a, B, mw, ldsU, ldsV, cx0, cy0, curU, curV, cs, c0 + min(ci, nc - 1), valid,
-                            ci, csum, sOff);
+    constexpr int L = WC / 2 > 0 ? WC / 2 : 1, UP = 64 / L;
+    const int lane = threadIdx.x, m = lane & (L - 1), u = lane / L;
+    for (int base = 0; base < 2 * nc; base += UP) {
+        const int ui = base + u;
+        const int ci = ui >> 1, pl = ui & 1;
+        const int c = c0 + min(ci, nc - 1);
+        const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
+        const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
+        const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
+        int acc = interp_satd_pass<4, WC, TC>(pl ? ldsV : ldsU, kChromaLW, (rx >> 5) - 1 - cx0,
+                                              (ry >> 5) - 1 - cy0, rx & 31, ry & 31, pl ? curV : curU, cs,
+                                              B.h >> 1, m, lane);
+        acc = group_sum<2 * L>(acc);
+        if (m == 0 && pl == 0 && ci < nc) csum[ci] += (uint32_t)acc;
     }
 }
 
@@ -430,7 +559,7 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16
 }
 
 #define XG_REFINE_LDS                                        \
-    __shared__ uint16_t lds[kLumaLW * kLumaLH];              \
+    __shared__ __align__(16) uint16_t lds[kLumaLW * kLumaLH]; \
     __shared__ uint32_t csum[64];                            \
     __shared__ int16_t sOff[2 * XG_MAX_CAND];                \
     __shared__ int s_task, s_last;                           \
@@ -463,7 +592,7 @@ __global__ void __launch_bounds__(64) k_refine_persistent(RefineArgs a)
             s_task = task;
         }
         __syncthreads();
-        const int t = s_task;
+        const int t = __builtin_amdgcn_readfirstlane(s_task);
         __syncthreads();
         if (t < 0) break;
         refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
@@ -720,8 +849,9 @@ static bool validate_blocks(xg_session *s, bool pred, int ctu)
         }
         const xg_picture &P = s->pics[B.pic], &R = s->pics[B.ref];
         if (B.x < 0 || B.y < 0 || B.x + B.w > P.plane[0].width || B.y + B.h > P.plane[0].height ||
-            (B.x & 1) || (B.y & 1)) {
-            set_error(XG_ERR_ARG, "block %d (%d,%d %dx%d) outside its picture", i, B.x, B.y, B.w, B.h);
+            (B.x & 3) || (B.y & 3)) {
+            set_error(XG_ERR_ARG, "block %d (%d,%d %dx%d) outside its picture or not on the 4-sample grid", i, B.x,
+                      B.y, B.w, B.h);
             return false;
         }
         if (R.plane[0].width != P.plane[0].width || R.plane[0].height != P.plane[0].height ||
@@ -927,8 +1057,9 @@ int32_t xg_session_set_picture(xg_session *s, int32_t slot, const xg_picture *pi
         const xg_plane &pl = pic->plane[p];
         const int sub = p ? 1 : 0;
         if (!pl.data || pl.width != (pic->plane[0].width >> sub) || pl.height != (pic->plane[0].height >> sub) ||
-            pl.stride < pl.width + 2 * pl.pad || pl.pad < 0) {
-            set_error(XG_ERR_ARG, "xg_session_set_picture: plane %d malformed", p);
+            pl.stride < pl.width + 2 * pl.pad || pl.pad < 0 || (pl.stride & 1) || ((uintptr_t)pl.data & 3)) {
+            set_error(XG_ERR_ARG, "xg_session_set_picture: plane %d malformed (strides even, samples 4-byte aligned)",
+                      p);
             return XG_ERR_ARG;
         }
     }

@@ -480,7 +480,7 @@ int32_t xg_server_submit(xg_server *srv, int32_t streamId, const xg_pixel *const
     for (int i = 0; i < nblocks; i++) {   // malformed descriptors are rejected before anything is queued
         const xg_block &B = blocks[i];
         const bool okw = B.w >= 4 && B.w <= 64 && !(B.w & (B.w - 1)), okh = B.h >= 4 && B.h <= 64 && !(B.h & (B.h - 1));
-        if (!okw || !okh || B.x < 0 || B.y < 0 || (B.x & 1) || (B.y & 1) || B.x + B.w > w || B.y + B.h > h) {
+        if (!okw || !okh || B.x < 0 || B.y < 0 || (B.x & 3) || (B.y & 3) || B.x + B.w > w || B.y + B.h > h) {
             set_error(XG_ERR_ARG, "xg_server_submit: block %d (%d,%d %dx%d) is malformed", i, B.x, B.y, B.w, B.h);
             return XG_ERR_ARG;
         }
