@@ -9,7 +9,9 @@ Hadamard), (SATD<<8) + bits*lambda; then motion compensation of each block's
 best candidate into the prediction picture (luma + chroma).
 
 Reported: batched form (persistent kernels replayed from a HIP graph) as the
-headline value, the plain form (one launch per block shape) beside it, the
+headline value, the plain form (one launch per block shape, one workgroup
+per block) beside it and, for reference, the plain form with the tuned
+candidates-per-workgroup table, the
 per-frame latency p50/p99, the refinement kernel against the HBM roofline
 (kernel time from HIP events on the session stream), and the CPU oracle
 timed on a bounded sample on this host.
@@ -172,11 +174,21 @@ def main():
 
     if args.quick:
         t_graph = timed(capi.MODE_GRAPH)
-        t_plain, exact, lat = float("nan"), None, np.array([float("nan")])
+        t_plain = t_plain_tuned = float("nan")
+        exact, lat = None, np.array([float("nan")])
     else:
-        log("plain form")
+        # plain form: one launch per block shape, one workgroup per block (all of its candidates)
+        shapes = [(lw, lh) for lw in range(2, 7) for lh in range(2, 7)]
+        tuned = {sh: lib.xg_session_get_tile(s, *sh) for sh in shapes}
+        for sh in shapes:
+            capi.check(lib.xg_session_set_tile(s, sh[0], sh[1], nc), "tile")
+        log("plain form (one workgroup per block)")
         t_plain = timed(capi.MODE_PLAIN)
         out_plain = outputs()
+        for sh in shapes:
+            capi.check(lib.xg_session_set_tile(s, sh[0], sh[1], tuned[sh]), "tile")
+        log("plain form with the tuned candidates-per-workgroup table")
+        t_plain_tuned = timed(capi.MODE_PLAIN)
         log("batched form (graph)")
         t_graph = timed(capi.MODE_GRAPH)
         out_graph = outputs()
@@ -211,6 +223,7 @@ def main():
     cand_px = int((fp.blocks[:, 2] * fp.blocks[:, 3]).sum()) * nc
     value = frame_px / t_graph / 1e6
     plain = frame_px / t_plain / 1e6
+    plain_tuned = frame_px / t_plain_tuned / 1e6
 
     serving = None
     if args.serve_frames > 0 and not args.quick:
@@ -260,6 +273,7 @@ def main():
                    "frame": f"{W}x{H}", "blocks": n, "candidates": nc, "global_batch": 1, "tiles": tuning,
                    "parallelism": "single GPU"},
         "plain_value": round(plain, 2),
+        "plain_tuned_value": round(plain_tuned, 2),
         "batched_vs_plain": round(value / plain, 3),
         "bit_exact_batched_vs_plain": exact,
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),

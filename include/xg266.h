@@ -13,7 +13,7 @@
  *
  *  2. Batched entry points over DEVICE pointers: all blocks of a frame (or of
  *     many frames of many streams) in one persistent kernel per stage, fed by
- *     per-shape work queues, optionally replayed from a captured HIP graph.
+ *     per-shape task lists, optionally replayed from a captured HIP graph.
  *
  * Types: PIXEL is uint16 (10-bit samples in [0,1023]), COEFF is int32; these
  * match the reference's ENABLE_10BIT_ENCODER build
@@ -310,7 +310,7 @@ typedef enum xg_mode {
 
 typedef struct xg_session xg_session;
 
-/* A session owns a HIP stream, the per-shape work queues and the result
+/* A session owns a HIP stream, the per-shape task lists and the result
  * buffers for up to maxBlocks blocks.  Pictures are passed per call through a
  * picture set of up to 16 slots (xg_block.pic / .ref index it). */
 int32_t xg_session_create(xg_session **out, int32_t maxBlocks);
@@ -319,16 +319,18 @@ int32_t xg_session_destroy(xg_session *s);
 void   *xg_session_stream(xg_session *s);
 /* Installs a picture in slot `slot` (0..15).  Device pointers. */
 int32_t xg_session_set_picture(xg_session *s, int32_t slot, const xg_picture *pic);
-/* Uploads the block list (host memory) and builds per-shape work queues:
+/* Uploads the block list (host memory) and builds per-shape task lists:
  * tasks sorted by shape, largest first, each covering `candPerTask`
  * candidates of one block (per-shape tile table, see xg_session_set_tile). */
 int32_t xg_session_set_blocks(xg_session *s, const xg_block *blocks, int32_t n);
 /* Per-shape tile choice: candidates per task for shape (lgW-2, lgH-2). */
 int32_t xg_session_set_tile(xg_session *s, int32_t lgW, int32_t lgH, int32_t candPerTask);
 int32_t xg_session_get_tile(xg_session *s, int32_t lgW, int32_t lgH);
-/* Scheduling of the batched form: `regions` spatial work queues (1..8; each
- * workgroup drains the queue of its XCD's region first, then steals) and
- * persistent workgroups per CU. */
+/* Scheduling of the batched form: persistent workgroups per CU, and
+ * `regions` (1..8) horizontal bands of equal estimated cost, band r run by the
+ * workgroups g with g % regions == r (workgroup g runs on XCD g % 8).  Tasks
+ * are assigned to workgroups on the host (heaviest first, least loaded
+ * workgroup), so the launch needs no work-queue atomics. */
 int32_t xg_session_set_sched(xg_session *s, int32_t regions, int32_t wgPerCU);
 /* Tunes the batched form on the installed blocks and pictures: first the
  * scheduling (spatial queues x workgroups per CU), then candidates per task

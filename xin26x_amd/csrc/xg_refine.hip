@@ -30,11 +30,13 @@
 //    counter; the task that completes a block performs the motion
 //    compensation of the winner from a freshly staged LDS window and resets
 //    the block's counters, so one launch does refinement and MC;
-//  * batched mode: one persistent launch pulls tasks from an atomic queue
-//    head, and the last workgroup to leave resets the queue, so a frame is a
-//    single kernel node (graph-capturable, no memset); plain mode: one launch
-//    per block shape.
+//  * batched mode: one persistent launch; the host assigns the tasks to its
+//    workgroups ahead of time (heaviest first to the least loaded workgroup,
+//    optionally in XCD-aligned picture bands), so scheduling costs no atomics
+//    and a frame is a single kernel node (graph-capturable, no memset);
+//    plain mode: one launch per block shape.
 #include <algorithm>
+#include <queue>
 #include <stdio.h>
 #include <string.h>
 #include <vector>
@@ -44,23 +46,21 @@
 
 namespace xg {
 
-constexpr int kLumaLW = 82;        // LDS row stride (u16) of the luma window: 41 words
-constexpr int kLumaLH = 80;        // rows: H + 7 + 9 at most
-constexpr int kChromaLW = 42;      // per-plane chroma window stride (u16): 21 words
-constexpr int kChromaLH = 40;      // rows: H/2 + 3 + 5 at most
+constexpr int kLumaLW = 80;        // LDS row stride (samples) of the luma window: W + 8 + 8 at most
+constexpr int kLumaLH = 80;        // rows: H + 7 + 8 at most
+constexpr int kChromaLW = 40;      // per-plane chroma window stride: W/2 + 4 + 4 at most
+constexpr int kChromaLH = 40;      // rows: H/2 + 3 + 4 at most
 constexpr int kMaxOff = 64;        // |candidate offset| limit, 1/16 pel
 constexpr int kNumShapes = 25;     // (lgW-2)*5 + (lgH-2), W, H in 4..64
 constexpr int kPicSlots = kSessionPicSlots;
-constexpr int kRegions = 8;        // spatial work queues, one per XCD of the MI355X
+constexpr int kRegions = 8;        // picture bands of the batched form: at most one per XCD of the MI355X
 
 struct RefineArgs {
     const xg_picture *pics;
     const xg_block *blocks;
     const int2 *tasks;             // x: block index, y: c0 | (count << 16)
     int taskBegin, taskEnd;        // this launch's task range
-    int *queue;                    // batched mode: [0..7] region queue heads, [8] exit count; nullptr in plain mode
-    const int *regionBegin;        // batched mode: task offsets of the spatial queues (nregions + 1)
-    int nregions;                  // batched mode: number of spatial queues (1..kRegions)
+    const int *wgBegin;            // batched mode: task range [wgBegin[g], wgBegin[g+1]) of workgroup g
     uint32_t *candCost;
     xg_block_result *best;
     unsigned long long *bestKey;   // per block: min over tasks of (cost << 32 | cand); all-ones when idle
@@ -88,15 +88,6 @@ __device__ __forceinline__ MvWin mv_window(const xg_block &B, int fw, int fh, in
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
-
-template <int TAPS>
-__device__ __forceinline__ int hrow_lds(const uint16_t *p, const int *hc)
-{
-    int s = 0;
-#pragma unroll
-    for (int j = 0; j < TAPS; j++) s += (int)p[j] * hc[j];
-    return (int)(int16_t)((s + kHOffset) >> kHShift);
-}
 
 // ---------------------------------------------------------------------------
 // Interpolation + SATD of one plane for a pass of candidates, two adjacent
@@ -205,58 +196,100 @@ __device__ __forceinline__ int tile_satd2(v2s *d, int lane)
     else return acc;
 }
 
-// First-pass values of window row `row` (sample index of the lane's first
-// column in the window, any parity) for the lane's two columns.
-template <int TAPS>
-__device__ __forceinline__ void hrow2(const uint16_t *win, int idx, const uint32_t *hp, int &t0, int &t1)
+// Shared memory of one refinement workgroup.  The luma and both chroma
+// windows have their own regions, so one staging round trip fills all three
+// and the motion compensation of the winner reads them in place.
+struct RefineLds {
+    uint32_t luma[kLumaLH * kLumaLW / 2];
+    uint32_t cb[kChromaLH * kChromaLW / 2];
+    uint32_t cr[kChromaLH * kChromaLW / 2];
+    uint32_t tapL[16 * 4];         // luma filter (index 0), packed tap pairs per phase
+    uint32_t tapC[32 * 2];         // chroma filter, packed tap pairs per phase
+    uint32_t csum[64];
+    int16_t off[2 * XG_MAX_CAND];
+    unsigned long long key;
+    int last;
+};
+
+__device__ __forceinline__ void lds_init(RefineLds &L, const RefineArgs &a)
 {
-    const uint32_t *q = reinterpret_cast<const uint32_t *>(win) + (idx >> 1);
-    const uint32_t sh = (uint32_t)(idx & 1) << 4;
-    constexpr int ND = TAPS / 2 + 2;                // dwords covering TAPS + 1 samples at any parity
-    uint32_t D[ND], E[ND - 1];
-#pragma unroll
-    for (int j = 0; j < ND; j++) D[j] = q[j];
-#pragma unroll
-    for (int j = 0; j < ND - 1; j++) E[j] = align16(D[j + 1], D[j], sh);      // (p[2j], p[2j+1])
-    int s0 = kHOffset, s1 = kHOffset;
-#pragma unroll
-    for (int j = 0; j < TAPS / 2; j++) {
-        s0 = dot2(E[j], hp[j], s0);
-        s1 = dot2(align16(E[j + 1], E[j], 16), hp[j], s1);                  // (p[2j+1], p[2j+2])
-    }
-    t0 = s0 >> kHShift;
-    t1 = s1 >> kHShift;
+    const int lane = threadIdx.x;
+    L.tapL[lane] = pack16(c_luma[0][lane >> 2][2 * (lane & 3)], c_luma[0][lane >> 2][2 * (lane & 3) + 1]);
+    L.tapC[lane] = pack16(c_chroma[lane >> 1][2 * (lane & 1)], c_chroma[lane >> 1][2 * (lane & 1) + 1]);
+    for (int i = lane; i < 2 * XG_MAX_CAND; i += 64) L.off[i] = a.off[i];
+    __syncthreads();
 }
 
-// One pass over up to 64 / (W / 2) units of one block.  unitPlane(u) and the
-// window origin are supplied per lane by the caller.
-//   win:   LDS window of the lane's plane; ls its row stride (even)
-//   wx, wy: the lane's candidate's first tap position in the window
-//   cur:   the plane's current block (global, dword aligned), cs its stride
-template <int TAPS, int W, int TS>
-__device__ __forceinline__ int interp_satd_pass(const uint16_t *win, int ls, int wx, int wy, int fx, int fy,
-                                                const uint16_t *cur, intptr_t cs, int H, int m, int lane)
-{
-    uint32_t hp[TAPS / 2], vp[TAPS / 2];
+// First pass of one window row for the lane's two columns, split into the
+// LDS fetch (the dwords covering TAPS + 1 samples from the dword holding the
+// first tap) and the arithmetic, so a row can be fetched while the previous
+// one is filtered.  sh = 16 when the first tap is the high half of its dword.
+template <int TAPS>
+struct HRow {
+    static constexpr int ND = TAPS / 2 + 2;
+    uint32_t D[ND];
+    __device__ __forceinline__ void fetch(const uint32_t *q)
     {
-        const int8_t *ch = taps_for<TAPS>(fx, 0), *cv = taps_for<TAPS>(fy, 0);
+#pragma unroll
+        for (int j = 0; j < ND; j++) D[j] = q[j];
+    }
+    __device__ __forceinline__ void filter(uint32_t sh, const uint32_t *hp, int &t0, int &t1) const
+    {
+        uint32_t E[ND - 1];
+#pragma unroll
+        for (int j = 0; j < ND - 1; j++) E[j] = align16(D[j + 1], D[j], sh);  // (p[2j], p[2j+1])
+        int s0 = kHOffset, s1 = kHOffset;
 #pragma unroll
         for (int j = 0; j < TAPS / 2; j++) {
-            hp[j] = pack16(ch[2 * j], ch[2 * j + 1]);
-            vp[j] = pack16(cv[2 * j], cv[2 * j + 1]);
+            s0 = dot2(E[j], hp[j], s0);
+            s1 = dot2(align16(E[j + 1], E[j], 16), hp[j], s1);              // (p[2j+1], p[2j+2])
         }
+        t0 = s0 >> kHShift;
+        t1 = s1 >> kHShift;
     }
+};
+
+template <int TAPS>
+__device__ __forceinline__ void hrow2(const uint32_t *win, int idx, const uint32_t *hp, int &t0, int &t1)
+{
+    HRow<TAPS> h;
+    h.fetch(win + (idx >> 1));
+    h.filter((uint32_t)(idx & 1) << 4, hp, t0, t1);
+}
+
+// Interpolation + SATD of one unit (a candidate's plane) of W x H samples for
+// the lane's columns 2m, 2m+1:
+//   win, ls: the plane's LDS window and its row stride in samples (even)
+//   wx, wy:  window position of the unit's first tap
+//   hp, vp:  packed tap pairs of the horizontal and vertical phases (LDS)
+//   cur, cs: the current block (global, dword aligned) and its stride
+// Window rows are fetched one row ahead of the row being filtered, current
+// rows one tile ahead.
+template <int TAPS, int W, int TS>
+__device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int ls, int wx, int wy, const uint32_t *hp,
+                                                const uint32_t *vp, const uint16_t *cur, intptr_t cs, int H, int m,
+                                                int lane)
+{
     constexpr int NP = TS + TAPS - 2;               // packed row pairs per column in a tile
+    const int lw = ls >> 1;                         // row stride in dwords
+    const int base = wy * ls + wx + 2 * m;
+    const uint32_t sh = (uint32_t)(base & 1) << 4;
+    const uint32_t *q = win + (base >> 1);          // dword of the first tap of the current row
     uint32_t P0[NP], P1[NP];
     int tl0, tl1;
-    const int base = wy * ls + wx + 2 * m;
+    HRow<TAPS> cur_row, next_row;
+    cur_row.fetch(q);
     {
         int a0, a1;
-        hrow2<TAPS>(win, base, hp, a0, a1);
+        next_row.fetch(q + lw);
+        cur_row.filter(sh, hp, a0, a1);
+        cur_row = next_row;
 #pragma unroll
         for (int r = 1; r < TAPS - 1; r++) {
             int b0, b1;
-            hrow2<TAPS>(win, base + r * ls, hp, b0, b1);
+            next_row.fetch(q + (r + 1) * lw);
+            cur_row.filter(sh, hp, b0, b1);
+            cur_row = next_row;
             P0[r - 1] = pack16(a0, b0);
             P1[r - 1] = pack16(a1, b1);
             a0 = b0;
@@ -265,22 +298,30 @@ __device__ __forceinline__ int interp_satd_pass(const uint16_t *win, int ls, int
         tl0 = a0;
         tl1 = a1;
     }
+    q += (TAPS - 1) * lw;                           // cur_row holds row TAPS - 1
     const gdword *gc = (const gdword *)(cur + 2 * m);
     const intptr_t csw = cs >> 1;
+    uint32_t cw[TS], cn[TS];
+#pragma unroll
+    for (int k = 0; k < TS; k++) cw[k] = gc[(intptr_t)k * csw];
     int acc = 0;
     for (int y0 = 0; y0 < H; y0 += TS) {
-        uint32_t cw[TS];
+        if (y0 + TS < H) {                          // the next tile's current rows, in flight meanwhile
 #pragma unroll
-        for (int k = 0; k < TS; k++) cw[k] = gc[(intptr_t)(y0 + k) * csw];
+            for (int k = 0; k < TS; k++) cn[k] = gc[(intptr_t)(y0 + TS + k) * csw];
+        }
 #pragma unroll
         for (int k = 0; k < TS; k++) {
             int b0, b1;
-            hrow2<TAPS>(win, base + (y0 + TAPS - 1 + k) * ls, hp, b0, b1);
+            next_row.fetch(q + (k + 1) * lw);       // one row past the window at the very end: unused
+            cur_row.filter(sh, hp, b0, b1);
+            cur_row = next_row;
             P0[TAPS - 2 + k] = pack16(tl0, b0);
             P1[TAPS - 2 + k] = pack16(tl1, b1);
             tl0 = b0;
             tl1 = b1;
         }
+        q += TS * lw;
         v2s d[TS];
 #pragma unroll
         for (int k = 0; k < TS; k++) {
@@ -299,27 +340,28 @@ __device__ __forceinline__ int interp_satd_pass(const uint16_t *win, int ls, int
             P0[k] = P0[TS + k];
             P1[k] = P1[TS + k];
         }
+#pragma unroll
+        for (int k = 0; k < TS; k++) cw[k] = cn[k];
     }
     return acc;
 }
 
 // Luma SATD of every candidate of a task; lane = unit * (W / 2) + m.
 template <int W, int TS>
-__device__ __forceinline__ void luma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw,
-                                          const uint16_t *lds, int wx0, int wy0, const uint16_t *cur,
-                                          intptr_t cs, int c0, int nc, uint32_t *csum, const int16_t *sOff)
+__device__ __forceinline__ void luma_task(const xg_block &B, const MvWin &mw, const RefineLds &L, int wx0, int wy0,
+                                          const uint16_t *cur, intptr_t cs, int c0, int nc, uint32_t *csum)
 {
-    constexpr int L = W / 2, CP = 64 / L;
-    const int lane = threadIdx.x, m = lane & (L - 1), u = lane / L;
+    constexpr int LN = W / 2, CP = 64 / LN;
+    const int lane = threadIdx.x, m = lane & (LN - 1), u = lane / LN;
     for (int base = 0; base < nc; base += CP) {
         const int ci = base + u;
         const int c = c0 + min(ci, nc - 1);
-        const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
-        const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
+        const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
+        const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
         const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
-        int acc = interp_satd_pass<8, W, TS>(lds, kLumaLW, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0, rx & 15,
-                                             ry & 15, cur, cs, B.h, m, lane);
-        acc = group_sum<L>(acc);
+        int acc = interp_satd_pass<8, W, TS>(L.luma, kLumaLW, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0,
+                                             L.tapL + 4 * (rx & 15), L.tapL + 4 * (ry & 15), cur, cs, B.h, m, lane);
+        acc = group_sum<LN>(acc);
         if (m == 0 && ci < nc) csum[ci] = (uint32_t)acc;
     }
 }
@@ -327,72 +369,137 @@ __device__ __forceinline__ void luma_task(const RefineArgs &a, const xg_block &B
 // Chroma SATD (Cb + Cr) of every candidate of a task; unit = 2 * candidate +
 // plane, lane = unit * (WC / 2) + m, so a candidate's two planes are adjacent.
 template <int WC, int TC>
-__device__ __forceinline__ void chroma_task(const RefineArgs &a, const xg_block &B, const MvWin &mw,
-                                            const uint16_t *ldsU, const uint16_t *ldsV, int cx0, int cy0,
-                                            const uint16_t *curU, const uint16_t *curV, intptr_t cs, int c0,
-                                            int nc, uint32_t *csum, const int16_t *sOff)
+__device__ __forceinline__ void chroma_task(const xg_block &B, const MvWin &mw, const RefineLds &L, int cx0, int cy0,
+                                            const uint16_t *curU, const uint16_t *curV, intptr_t cs, int c0, int nc,
+                                            uint32_t *csum)
 {
-    constexpr int L = WC / 2 > 0 ? WC / 2 : 1, UP = 64 / L;
-    const int lane = threadIdx.x, m = lane & (L - 1), u = lane / L;
+    constexpr int LN = WC / 2 > 0 ? WC / 2 : 1, UP = 64 / LN;
+    const int lane = threadIdx.x, m = lane & (LN - 1), u = lane / LN;
     for (int base = 0; base < 2 * nc; base += UP) {
         const int ui = base + u;
         const int ci = ui >> 1, pl = ui & 1;
         const int c = c0 + min(ci, nc - 1);
-        const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
-        const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
+        const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
+        const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
         const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
-        int acc = interp_satd_pass<4, WC, TC>(pl ? ldsV : ldsU, kChromaLW, (rx >> 5) - 1 - cx0,
-                                              (ry >> 5) - 1 - cy0, rx & 31, ry & 31, pl ? curV : curU, cs,
+        int acc = interp_satd_pass<4, WC, TC>(pl ? L.cr : L.cb, kChromaLW, (rx >> 5) - 1 - cx0, (ry >> 5) - 1 - cy0,
+                                              L.tapC + 2 * (rx & 31), L.tapC + 2 * (ry & 31), pl ? curV : curU, cs,
                                               B.h >> 1, m, lane);
-        acc = group_sum<2 * L>(acc);
+        acc = group_sum<2 * LN>(acc);
         if (m == 0 && pl == 0 && ci < nc) csum[ci] += (uint32_t)acc;
     }
 }
 
-// Motion compensation of one plane from an LDS window whose first row and
-// column are TAPS/2-1 samples above/left of the block's integer position.
-// Lanes [lane0, lane0 + nlanes) split the block into column x row-chunk
-// units; each unit walks its rows with a sliding window of first-pass values.
-template <int TAPS>
-__device__ __forceinline__ void mc_plane(const uint16_t *win, int ls, int fx, int fy, int W, int H,
-                                         uint16_t *dst, intptr_t ds, int lane0, int nlanes)
+// Dword pointer to sample (x, y) of a plane; x even.
+__device__ __forceinline__ const gdword *plane_dw(const xg_plane &pl, int x, int y)
 {
-    const int lid = (int)threadIdx.x - lane0;
-    if (lid < 0 || lid >= nlanes) return;
-    const int chunks = max(1, min(nlanes / W, H));
-    const int x = lid % W, ck = lid / W;
-    if (ck >= chunks) return;
-    const int rp = H / chunks, y0 = ck * rp;
-    int hc[TAPS], vc[TAPS];
-    const int8_t *ch = taps_for<TAPS>(fx, 0), *cv = taps_for<TAPS>(fy, 0);
+    return (const gdword *)(pl.data + (intptr_t)y * pl.stride + x);
+}
+
+// Global -> LDS copy plan of the windows of up to two planes with one
+// geometry: `rows` rows of `wd` dwords (wd <= 64) from s0/s1 (row pitch sp
+// dwords) to d0/d1 (row pitch dp dwords).  Each lane owns one dword column and
+// walks rows with a stride of 64 / wd; loads go in batches of 8 per lane, all
+// issued before the batch's first store.
+struct StagePlan {
+    const gdword *s0, *s1;
+    uint32_t *d0, *d1;
+    intptr_t sp;
+    int dp, rows, total, per, c, r0, batches;
+};
+
+__device__ __forceinline__ StagePlan stage_plan(uint32_t *d0, uint32_t *d1, int dp, const gdword *s0,
+                                                const gdword *s1, intptr_t sp, int rows, int wd, int planes)
+{
+    StagePlan P;
+    P.s0 = s0;
+    P.s1 = s1;
+    P.d0 = d0;
+    P.d1 = d1;
+    P.sp = sp;
+    P.dp = dp;
+    P.rows = rows;
+    P.total = planes * rows;
+    P.per = 64 / wd;
+    P.c = (int)threadIdx.x % wd;
+    P.r0 = (int)threadIdx.x / wd;
+    P.batches = (P.total + 8 * P.per - 1) / (8 * P.per);
+    return P;
+}
+
+__device__ __forceinline__ void stage_load(const StagePlan &P, int k, uint32_t *v)
+{
+    if (P.r0 >= P.per) return;
 #pragma unroll
-    for (int j = 0; j < TAPS; j++) {
-        hc[j] = ch[j];
-        vc[j] = cv[j];
-    }
-    const uint16_t *p = win + y0 * ls + x;
-    int t[TAPS];
-#pragma unroll
-    for (int k = 0; k < TAPS - 1; k++) t[k] = hrow_lds<TAPS>(p + k * ls, hc);
-    for (int y = 0; y < rp; y++) {
-        t[TAPS - 1] = hrow_lds<TAPS>(p + (y + TAPS - 1) * ls, hc);
-        int s = 0;
-#pragma unroll
-        for (int j = 0; j < TAPS; j++) s += vc[j] * t[j];
-        dst[(intptr_t)(y0 + y) * ds + x] = (uint16_t)vfinal_pix(s);
-#pragma unroll
-        for (int k = 0; k < TAPS - 1; k++) t[k] = t[k + 1];
+    for (int j = 0; j < 8; j++) {
+        const int rr = min(P.r0 + (8 * k + j) * P.per, P.total - 1);
+        const bool second = rr >= P.rows;
+        v[j] = (second ? P.s1 : P.s0)[(intptr_t)(second ? rr - P.rows : rr) * P.sp + P.c];
     }
 }
 
-// Best-candidate record and luma + chroma prediction of block b.
+__device__ __forceinline__ void stage_store(const StagePlan &P, int k, const uint32_t *v)
+{
+    if (P.r0 >= P.per) return;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int rr = P.r0 + (8 * k + j) * P.per;
+        if (rr < P.total) {
+            const bool second = rr >= P.rows;
+            (second ? P.d1 : P.d0)[(second ? rr - P.rows : rr) * P.dp + P.c] = v[j];
+        }
+    }
+}
+
+// Motion compensation of one plane from its LDS window: lanes [lane0,
+// lane0 + nlanes) split the W x H block into row groups of W / 2 lanes; each
+// lane forms columns 2m, 2m+1 of its rows and stores them as one dword.
+template <int TAPS>
+__device__ __forceinline__ void mc_pass(const uint32_t *win, int ls, int wx, int wy, const uint32_t *hp, int fy, int W,
+                                        int H, uint16_t *dst, intptr_t ds, int lane0, int nlanes)
+{
+    const int lid = (int)threadIdx.x - lane0;
+    const int LN = W > 2 ? W >> 1 : 1;
+    const int groups = min(nlanes / LN, H);
+    const int m = lid & (LN - 1), g = lid / LN;
+    if (lid < 0 || lid >= nlanes || g >= groups) return;
+    const int rows = H / groups, y0 = g * rows;
+    int vc[TAPS];
+    const int8_t *cv = taps_for<TAPS>(fy, 0);
+#pragma unroll
+    for (int j = 0; j < TAPS; j++) vc[j] = cv[j];
+    int t0[TAPS], t1[TAPS];
+    const int base = (wy + y0) * ls + wx + 2 * m;
+#pragma unroll
+    for (int k = 0; k < TAPS - 1; k++) hrow2<TAPS>(win, base + k * ls, hp, t0[k], t1[k]);
+    uint16_t *o = dst + (intptr_t)y0 * ds + 2 * m;
+    for (int y = 0; y < rows; y++) {
+        hrow2<TAPS>(win, base + (y + TAPS - 1) * ls, hp, t0[TAPS - 1], t1[TAPS - 1]);
+        int s0 = kVOffset, s1 = kVOffset;
+#pragma unroll
+        for (int j = 0; j < TAPS; j++) {
+            s0 += vc[j] * t0[j];
+            s1 += vc[j] * t1[j];
+        }
+        *reinterpret_cast<uint32_t *>(o + (intptr_t)y * ds) = pack16(clip_pix(s0 >> kVShift), clip_pix(s1 >> kVShift));
+#pragma unroll
+        for (int k = 0; k < TAPS - 1; k++) {
+            t0[k] = t0[k + 1];
+            t1[k] = t1[k + 1];
+        }
+    }
+}
+
+// Best-candidate record and luma + chroma prediction of block b, from the
+// task's staged windows (they hold every candidate's footprint).
 __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_block &B, const MvWin &mw,
-                                         unsigned long long key, uint16_t *lds, const int16_t *sOff)
+                                         unsigned long long key, const RefineLds &L, int wx0, int wy0, int cx0,
+                                         int cy0)
 {
     const int lane = threadIdx.x;
     const int c = (int)(uint32_t)key;
-    const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
-    const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
+    const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
+    const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
     if (lane == 0) {
         xg_block_result r;
         r.mvx = mvx;
@@ -401,119 +508,90 @@ __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_bl
         r.cand = c;
         a.best[b] = r;
     }
-    const xg_picture &R = a.pics[B.ref];
     const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
-    {   // luma: (H+7) x (W+7) samples from 3 above / 3 left of the integer position
-        const intptr_t rs = R.plane[0].stride;
-        const uint16_t *src = R.plane[0].data + (intptr_t)((ry >> 4) - 3) * rs + ((rx >> 4) - 3);
-        const int ww = B.w + 7, hh = B.h + 7;
-        for (int r = 0; r < hh; r++)
-            for (int cc = lane; cc < ww; cc += 64) lds[r * kLumaLW + cc] = src[(intptr_t)r * rs + cc];
-        __syncthreads();
-        const xg_plane &P0 = a.predTab[B.pic].plane[0];
-        mc_plane<8>(lds, kLumaLW, rx & 15, ry & 15, B.w, B.h, P0.data + (intptr_t)B.y * P0.stride + B.x,
-                    P0.stride, 0, 64);
-        __syncthreads();
-    }
-    {   // chroma: (H/2+3) x (W/2+3) samples per plane from 1 above / 1 left
-        const int wc = B.w >> 1, hc = B.h >> 1, ww = wc + 3, hh = hc + 3;
-        uint16_t *lu = lds, *lv = lds + kChromaLW * kChromaLH;
-        const intptr_t rs = R.plane[1].stride;
-        const intptr_t so = (intptr_t)((ry >> 5) - 1) * rs + ((rx >> 5) - 1);
-        for (int r = 0; r < hh; r++)
-            for (int cc = lane; cc < 2 * ww; cc += 64) {
-                const int pl = cc >= ww, col = cc - pl * ww;
-                (pl ? lv : lu)[r * kChromaLW + col] = R.plane[1 + pl].data[so + (intptr_t)r * rs + col];
-            }
-        __syncthreads();
-        const xg_plane &P1 = a.predTab[B.pic].plane[1], &P2 = a.predTab[B.pic].plane[2];
-        const intptr_t o1 = (intptr_t)(B.y >> 1) * P1.stride + (B.x >> 1);
-        const intptr_t o2 = (intptr_t)(B.y >> 1) * P2.stride + (B.x >> 1);
-        mc_plane<4>(lu, kChromaLW, rx & 31, ry & 31, wc, hc, P1.data + o1, P1.stride, 0, 32);
-        mc_plane<4>(lv, kChromaLW, rx & 31, ry & 31, wc, hc, P2.data + o2, P2.stride, 32, 32);
-        __syncthreads();
-    }
+    const xg_picture &PT = a.predTab[B.pic];
+    const xg_plane &P0 = PT.plane[0], &P1 = PT.plane[1], &P2 = PT.plane[2];
+    mc_pass<8>(L.luma, kLumaLW, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0, L.tapL + 4 * (rx & 15), ry & 15, B.w, B.h,
+               P0.data + (intptr_t)B.y * P0.stride + B.x, P0.stride, 0, 64);
+    const int wc = B.w >> 1, hc = B.h >> 1, ox = (rx >> 5) - 1 - cx0, oy = (ry >> 5) - 1 - cy0;
+    const uint32_t *hc4 = L.tapC + 2 * (rx & 31);
+    mc_pass<4>(L.cb, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
+               P1.data + (intptr_t)(B.y >> 1) * P1.stride + (B.x >> 1), P1.stride, 0, 64);
+    mc_pass<4>(L.cr, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
+               P2.data + (intptr_t)(B.y >> 1) * P2.stride + (B.x >> 1), P2.stride, 0, 64);
 }
 
-__device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16_t *lds, uint32_t *csum,
-                                           const int16_t *sOff, int *sLast, unsigned long long *sKey)
+__device__ __forceinline__ void refine_one(const RefineArgs &a, int2 tk, const xg_block &B, RefineLds &L)
 {
     const int lane = threadIdx.x;
-    const int2 tk = a.tasks[task];
     const int b = tk.x, c0 = tk.y & 0xFFFF, nc = tk.y >> 16;
-    const xg_block B = a.blocks[b];
     const xg_picture &P = a.pics[B.pic];
     const xg_picture &R = a.pics[B.ref];
     const MvWin mw = mv_window(B, P.plane[0].width, P.plane[0].height, a.ctuSize);
 
-    // ---- luma window: every sample the task's candidates read ----
-    const int ixmin = ((B.x << 4) + clampi(B.mvx + a.offMinX, mw.mnx, mw.mxx)) >> 4;
-    const int ixmax = ((B.x << 4) + clampi(B.mvx + a.offMaxX, mw.mnx, mw.mxx)) >> 4;
-    const int iymin = ((B.y << 4) + clampi(B.mvy + a.offMinY, mw.mny, mw.mxy)) >> 4;
-    const int iymax = ((B.y << 4) + clampi(B.mvy + a.offMaxY, mw.mny, mw.mxy)) >> 4;
-    const int wx0 = ixmin - 3, wy0 = iymin - 3;
-    const int WW = ixmax - ixmin + B.w + 7, HH = iymax - iymin + B.h + 7;
+    // ---- windows: every luma / chroma sample any candidate's filters touch,
+    // from the even column at or left of the first tap (dword-aligned rows) ----
+    const int qx0 = (B.x << 4) + clampi(B.mvx + a.offMinX, mw.mnx, mw.mxx);
+    const int qx1 = (B.x << 4) + clampi(B.mvx + a.offMaxX, mw.mnx, mw.mxx);
+    const int qy0 = (B.y << 4) + clampi(B.mvy + a.offMinY, mw.mny, mw.mxy);
+    const int qy1 = (B.y << 4) + clampi(B.mvy + a.offMaxY, mw.mny, mw.mxy);
+    const int wx0 = ((qx0 >> 4) - 3) & ~1, wy0 = (qy0 >> 4) - 3;
+    const int WW = (qx1 >> 4) + B.w + 4 - wx0, HH = (qy1 >> 4) - (qy0 >> 4) + B.h + 7;
+    const int cx0 = ((qx0 >> 5) - 1) & ~1, cy0 = (qy0 >> 5) - 1;
+    const int CWW = (qx1 >> 5) + (B.w >> 1) + 2 - cx0, CHH = (qy1 >> 5) - (qy0 >> 5) + (B.h >> 1) + 3;
+    const intptr_t cs = P.plane[0].stride, ccs = P.plane[1].stride;
+    const uint16_t *cur = P.plane[0].data + (intptr_t)B.y * cs + B.x;
+    const intptr_t co = (intptr_t)(B.y >> 1) * ccs + (B.x >> 1);
+    const uint16_t *cu = P.plane[1].data + co, *cv = P.plane[2].data + co;
     {
-        const intptr_t rs = R.plane[0].stride;
-        const uint16_t *rp = R.plane[0].data + (intptr_t)wy0 * rs + wx0;
-        for (int r = 0; r < HH; r++)
-            for (int cc = lane; cc < WW; cc += 64) lds[r * kLumaLW + cc] = rp[(intptr_t)r * rs + cc];
+        const StagePlan pl = stage_plan(L.luma, L.luma, kLumaLW / 2, plane_dw(R.plane[0], wx0, wy0), nullptr,
+                                        R.plane[0].stride >> 1, HH, (WW + 1) >> 1, 1);
+        const StagePlan pc = stage_plan(L.cb, L.cr, kChromaLW / 2, plane_dw(R.plane[1], cx0, cy0),
+                                        plane_dw(R.plane[2], cx0, cy0), R.plane[1].stride >> 1, CHH, (CWW + 1) >> 1,
+                                        2);
+        const int nb = max(pl.batches, pc.batches);
+        for (int k = 0; k < nb; k++) {
+            uint32_t vl[8], vc[8];
+            if (k < pl.batches) stage_load(pl, k, vl);
+            if (k < pc.batches) stage_load(pc, k, vc);
+            if (k < pl.batches) stage_store(pl, k, vl);
+            if (k < pc.batches) stage_store(pc, k, vc);
+        }
     }
-    csum[lane] = 0;
+    L.csum[lane] = 0;
     __syncthreads();
 
-    const intptr_t cs = P.plane[0].stride;
-    const uint16_t *cur = P.plane[0].data + (intptr_t)B.y * cs + B.x;
     const bool t4 = B.w == 4 || B.h == 4;
+    uint32_t *csum = L.csum;
     switch (B.w) {
-    case 4: luma_task<4, 4>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff); break;
-    case 8: t4 ? luma_task<8, 4>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff)
-               : luma_task<8, 8>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff); break;
-    case 16: t4 ? luma_task<16, 4>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff)
-                : luma_task<16, 8>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff); break;
-    case 32: t4 ? luma_task<32, 4>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff)
-                : luma_task<32, 8>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff); break;
-    default: t4 ? luma_task<64, 4>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff)
-                : luma_task<64, 8>(a, B, mw, lds, wx0, wy0, cur, cs, c0, nc, csum, sOff); break;
+    case 4: luma_task<4, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
+    case 8: t4 ? luma_task<8, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
+               : luma_task<8, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
+    case 16: t4 ? luma_task<16, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
+                : luma_task<16, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
+    case 32: t4 ? luma_task<32, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
+                : luma_task<32, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
+    default: t4 ? luma_task<64, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
+                : luma_task<64, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
     }
     __syncthreads();
 
     if (a.useChroma) {
-        // ---- chroma windows (Cb, Cr share the address split) ----
-        const int cxmin = ((B.x << 4) + clampi(B.mvx + a.offMinX, mw.mnx, mw.mxx)) >> 5;
-        const int cxmax = ((B.x << 4) + clampi(B.mvx + a.offMaxX, mw.mnx, mw.mxx)) >> 5;
-        const int cymin = ((B.y << 4) + clampi(B.mvy + a.offMinY, mw.mny, mw.mxy)) >> 5;
-        const int cymax = ((B.y << 4) + clampi(B.mvy + a.offMaxY, mw.mny, mw.mxy)) >> 5;
-        const int cx0 = cxmin - 1, cy0 = cymin - 1;
-        const int CWW = cxmax - cxmin + (B.w >> 1) + 3, CHH = cymax - cymin + (B.h >> 1) + 3;
-        uint16_t *ldsU = lds, *ldsV = lds + kChromaLW * kChromaLH;
-        const intptr_t rs = R.plane[1].stride;
-        const uint16_t *ru = R.plane[1].data + (intptr_t)cy0 * rs + cx0;
-        const uint16_t *rv = R.plane[2].data + (intptr_t)cy0 * rs + cx0;
-        for (int r = 0; r < CHH; r++)
-            for (int cc = lane; cc < 2 * CWW; cc += 64) {
-                const int pl = cc >= CWW, col = cc - pl * CWW;
-                (pl ? ldsV : ldsU)[r * kChromaLW + col] = (pl ? rv : ru)[(intptr_t)r * rs + col];
-            }
-        __syncthreads();
-        const intptr_t ccs = P.plane[1].stride;
-        const intptr_t co = (intptr_t)(B.y >> 1) * ccs + (B.x >> 1);
-        const uint16_t *cu = P.plane[1].data + co, *cv = P.plane[2].data + co;
         const int wc = B.w >> 1, hc = B.h >> 1;
         const int tc = (wc % 8 == 0 && hc % 8 == 0) ? 8 : (wc % 4 == 0 && hc % 4 == 0) ? 4 : 2;
         switch (wc) {
-        case 2: chroma_task<2, 2>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff); break;
-        case 4: tc == 4 ? chroma_task<4, 4>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff)
-                        : chroma_task<4, 2>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff); break;
-        case 8: tc == 8 ? chroma_task<8, 8>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff)
-              : tc == 4 ? chroma_task<8, 4>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff)
-                        : chroma_task<8, 2>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff); break;
-        case 16: tc == 8 ? chroma_task<16, 8>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff)
-               : tc == 4 ? chroma_task<16, 4>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff)
-                         : chroma_task<16, 2>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff); break;
-        default: tc == 8 ? chroma_task<32, 8>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff)
-               : tc == 4 ? chroma_task<32, 4>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff)
-                         : chroma_task<32, 2>(a, B, mw, ldsU, ldsV, cx0, cy0, cu, cv, ccs, c0, nc, csum, sOff); break;
+        case 2: chroma_task<2, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
+        case 4: tc == 4 ? chroma_task<4, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
+                        : chroma_task<4, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
+        case 8: tc == 8 ? chroma_task<8, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
+              : tc == 4 ? chroma_task<8, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
+                        : chroma_task<8, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
+        case 16: tc == 8 ? chroma_task<16, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
+               : tc == 4 ? chroma_task<16, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
+                         : chroma_task<16, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
+        default: tc == 8 ? chroma_task<32, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
+               : tc == 4 ? chroma_task<32, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
+                         : chroma_task<32, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
         }
         __syncthreads();
     }
@@ -522,8 +600,8 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16
     unsigned long long key = ~0ull;
     if (lane < nc) {
         const int c = c0 + lane;
-        const int mvx = clampi(B.mvx + sOff[c], mw.mnx, mw.mxx);
-        const int mvy = clampi(B.mvy + sOff[XG_MAX_CAND + c], mw.mny, mw.mxy);
+        const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
+        const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
         const uint32_t bits = eg_bits(mvx - B.mvpx) + eg_bits(mvy - B.mvpy);
         const uint32_t cost = (csum[lane] << 8) + bits * a.lambda;
         a.candCost[(size_t)b * a.ncand + c] = cost;
@@ -550,71 +628,39 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int task, uint16
                 atomicExch(&a.done[b], 0);
             }
         }
-        *sLast = last;
-        *sKey = key;
+        L.last = last;
+        L.key = key;
     }
     __syncthreads();
-    if (*sLast) mc_block(a, b, B, mw, *sKey, lds, sOff);
+    if (L.last) mc_block(a, b, B, mw, L.key, L, wx0, wy0, cx0, cy0);
     __syncthreads();
 }
 
-#define XG_REFINE_LDS                                        \
-    __shared__ __align__(16) uint16_t lds[kLumaLW * kLumaLH]; \
-    __shared__ uint32_t csum[64];                            \
-    __shared__ int16_t sOff[2 * XG_MAX_CAND];                \
-    __shared__ int s_task, s_last;                           \
-    __shared__ unsigned long long s_key;                     \
-    for (int i = threadIdx.x; i < 2 * XG_MAX_CAND; i += 64) sOff[i] = a.off[i]; \
-    __syncthreads();
-
-// Batched form: a persistent grid drains the per-region task queues.
-__global__ void __launch_bounds__(64) k_refine_persistent(RefineArgs a)
+// Batched form: a persistent grid; workgroup g runs the tasks the host
+// assigned to it (balanced by estimated cost, heaviest first), so the launch
+// takes no atomics to schedule and leaves no state behind.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_refine_persistent(RefineArgs a)
 {
-    XG_REFINE_LDS
-    // Workgroups are dispatched to the XCDs round-robin, so workgroup i runs on
-    // XCD i % 8.  Each drains the queue of its own spatial region first --
-    // neighbouring blocks share reference samples in that XCD's L2 -- then
-    // steals from the other regions in turn.
-    int region = (int)(blockIdx.x % kRegions) % a.nregions, exhausted = 0;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            int task = -1;
-            while (exhausted < a.nregions) {
-                const int beg = a.regionBegin[region], cnt = a.regionBegin[region + 1] - beg;
-                const int i = cnt > 0 ? atomicAdd(&a.queue[region], 1) : 0;
-                if (i < cnt) {
-                    task = beg + i;
-                    break;
-                }
-                region = (region + 1) % a.nregions;
-                exhausted++;
-            }
-            s_task = task;
-        }
-        __syncthreads();
-        const int t = __builtin_amdgcn_readfirstlane(s_task);
-        __syncthreads();
-        if (t < 0) break;
-        refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
-    }
-    // the last workgroup to leave resets the queues for the next launch
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&a.queue[kRegions], 1) == (int)gridDim.x - 1) {
-            for (int r = 0; r < a.nregions; r++) atomicExch(&a.queue[r], 0);
-            atomicExch(&a.queue[kRegions], 0);
-        }
+    __shared__ RefineLds L;
+    lds_init(L, a);
+    const int e = a.wgBegin[blockIdx.x + 1];
+    for (int t = a.wgBegin[blockIdx.x]; t < e; t++) {
+        const int2 tk = a.tasks[t];
+        refine_one(a, tk, a.blocks[tk.x], L);
     }
 }
 
 // Plain form: one launch per block shape, one workgroup per task.
-__global__ void __launch_bounds__(64) k_refine_shape(RefineArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_refine_shape(RefineArgs a)
 {
-    XG_REFINE_LDS
+    __shared__ RefineLds L;
+    lds_init(L, a);
     const int t = a.taskBegin + (int)blockIdx.x;
-    if (t < a.taskEnd) refine_one(a, t, lds, csum, sOff, &s_last, &s_key);
+    if (t < a.taskEnd) {
+        const int2 tk = a.tasks[t];
+        refine_one(a, tk, a.blocks[tk.x], L);
+    }
 }
-#undef XG_REFINE_LDS
 
 }  // namespace xg
 
@@ -627,8 +673,8 @@ struct xg_session {
     hipStream_t stream = nullptr;
     int maxBlocks = 0;
     int numCUs = 256;
-    int persistFactor = 11;            // persistent workgroups per CU (LDS allows 11)
-    int nregions = 1;                  // spatial work queues of the batched form
+    int persistFactor = 8;             // persistent workgroups per CU (LDS allows 8)
+    int nregions = 1;                  // spatial partitions of the batched form (XCD-aligned)
 
     xg_picture pics[kPicSlots];
     bool picSet[kPicSlots] = {};
@@ -649,8 +695,10 @@ struct xg_session {
     int tile[kNumShapes];              // candidates per task
     std::vector<int2> tasks;
     int2 *dTasks = nullptr;            // shape-major (plain mode)
-    int2 *dTasksR = nullptr;           // region-major, heaviest first within a region (batched mode)
-    int *dRegionBegin = nullptr;
+    int2 *dTasksW = nullptr;           // workgroup-major (batched mode)
+    int *dWgBegin = nullptr;           // task range of each persistent workgroup
+    int wgCap = 0;
+    int gridW = 0;                     // persistent workgroups of the current assignment
     size_t taskCap = 0;
     int shapeTaskBegin[kNumShapes + 1] = {};
     std::vector<int> shapeOrder;       // shapes, heaviest tasks first
@@ -661,7 +709,6 @@ struct xg_session {
     int16_t *dOff = nullptr;           // candidate offsets of the last configuration
     int16_t hOff[2 * XG_MAX_CAND];
     bool offValid = false;
-    int *dQueue = nullptr;             // persistent work-queue head and exit count
     int *dDone = nullptr;              // per-block candidate counters
     unsigned long long *dBestKey = nullptr;  // per-block best (cost << 32 | cand)
     void *hStage = nullptr;            // pinned upload staging
@@ -730,19 +777,77 @@ static uint64_t mix(uint64_t h, uint64_t v)
     return h;
 }
 
-// Region of a block for the XCD-local queues: horizontal bands of its picture.
-static int region_of(const xg_session *s, const xg_block &B)
+// Estimated cost of a task in row-passes of the refinement kernel: luma and
+// chroma passes (units of W/2 lanes, 64 lanes per pass) times the rows each
+// filters, plus a fixed per-task share (setup, staging latency, costs, MC).
+// Measured on MI355X at 1080p: ~0.2 us per row-pass and ~8 us per task of
+// workgroup time, hence the fixed share of 42.
+static long task_cost(const xg_block &B, int nc)
 {
-    const int h = std::max(1, s->pics[B.pic].plane[0].height);
-    return std::min(s->nregions - 1, (int)((long)B.y * s->nregions / h));
+    const long lp = (nc * (long)B.w / 2 + 63) / 64, cp = (2L * nc * std::max(1, B.w / 4) + 63) / 64;
+    return lp * (B.h + 7) + cp * (B.h / 2 + 3) + 42;
+}
+
+// Assignment of the tasks to the persistent workgroups: the picture area is
+// split into nregions horizontal bands of equal estimated cost, band r goes to
+// the workgroups g with g % nregions == r (workgroup g runs on XCD g % 8, so a
+// band's reference rows stay in one XCD's L2), and inside a band each task,
+// heaviest first, goes to the least loaded workgroup.
+static void assign_tasks(const xg_session *s, const std::vector<int2> &tasks, int G, std::vector<int2> &out,
+                         std::vector<int> &begin)
+{
+    const int R = std::max(1, std::min(s->nregions, G));
+    const size_t n = tasks.size();
+    std::vector<long> cost(n);
+    long total = 0;
+    for (size_t i = 0; i < n; i++) {
+        cost[i] = task_cost(s->blocks[tasks[i].x], tasks[i].y >> 16);
+        total += cost[i];
+    }
+    // bands: tasks ordered by picture position, cut at equal cost
+    std::vector<int> band(n, 0);
+    if (R > 1) {
+        std::vector<size_t> pos(n);
+        for (size_t i = 0; i < n; i++) pos[i] = i;
+        std::stable_sort(pos.begin(), pos.end(), [&](size_t x, size_t y) {
+            const xg_block &A = s->blocks[tasks[x].x], &B = s->blocks[tasks[y].x];
+            return A.pic != B.pic ? A.pic < B.pic : A.y != B.y ? A.y < B.y : A.x < B.x;
+        });
+        long acc = 0;
+        for (size_t k = 0; k < n; k++) {
+            band[pos[k]] = (int)std::min<long>(R - 1, acc * R / std::max(1L, total));
+            acc += cost[pos[k]];
+        }
+    }
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<std::vector<int>> lists(G);
+    std::vector<std::priority_queue<std::pair<long, int>, std::vector<std::pair<long, int>>, std::greater<>>> heaps(R);
+    for (int g = 0; g < G; g++) heaps[g % R].push({0, g});
+    for (size_t i : order) {
+        auto &h = heaps[band[i]];
+        auto top = h.top();
+        h.pop();
+        lists[top.second].push_back((int)i);
+        h.push({top.first + cost[i], top.second});
+    }
+    out.clear();
+    out.reserve(n);
+    begin.assign(G + 1, 0);
+    for (int g = 0; g < G; g++) {
+        begin[g] = (int)out.size();
+        for (int i : lists[g]) out.push_back(tasks[i]);
+    }
+    begin[G] = (int)out.size();
 }
 
 static bool build_tasks(xg_session *s, int ncand)
 {
+    const int G = std::max(1, s->numCUs * s->persistFactor);
     uint64_t key = mix(mix(s->blocksVersion, (uint64_t)ncand), 0);
     for (int i = 0; i < kNumShapes; i++) key = mix(key, (uint64_t)s->tile[i]);
-    for (int i = 0; i < kPicSlots; i++) key = mix(key, s->picSet[i] ? (uint64_t)s->pics[i].plane[0].height : 0);
-    key = mix(key, (uint64_t)s->nregions);
+    key = mix(key, ((uint64_t)s->nregions << 32) | (uint64_t)G);
     if (key == s->tasksKey) return true;
     std::vector<std::vector<int>> per(kNumShapes);
     for (int b = 0; b < s->nblocks; b++) per[shape_of(s->blocks[b].w, s->blocks[b].h)].push_back(b);
@@ -755,47 +860,44 @@ static bool build_tasks(xg_session *s, int ncand)
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return weight(x) > weight(y); });
     s->tasks.clear();
-    std::vector<std::vector<int2>> perRegion(s->nregions);
     int st[kNumShapes + 1];
     for (int oi = 0; oi < kNumShapes; oi++) {
         const int sh = order[oi];
         st[oi] = (int)s->tasks.size();
         const int cpt = std::max(1, std::min(s->tile[sh], ncand));
-        for (int b : per[sh]) {
-            const int r = region_of(s, s->blocks[b]);
-            for (int c0 = 0; c0 < ncand; c0 += cpt) {
-                const int2 t = make_int2(b, c0 | (std::min(cpt, ncand - c0) << 16));
-                s->tasks.push_back(t);
-                perRegion[r].push_back(t);
-            }
-        }
+        for (int b : per[sh])
+            for (int c0 = 0; c0 < ncand; c0 += cpt)
+                s->tasks.push_back(make_int2(b, c0 | (std::min(cpt, ncand - c0) << 16)));
     }
     st[kNumShapes] = (int)s->tasks.size();
     memcpy(s->shapeTaskBegin, st, sizeof(st));
     s->shapeOrder = order;
-    std::vector<int2> tasksR;
-    tasksR.reserve(s->tasks.size());
-    int rb[kRegions + 1];
-    for (int r = 0; r <= kRegions; r++) {
-        rb[r] = (int)tasksR.size();
-        if (r < s->nregions) tasksR.insert(tasksR.end(), perRegion[r].begin(), perRegion[r].end());
-    }
-    if (s->tasks.size() > s->taskCap) {
+    std::vector<int2> tasksW;
+    std::vector<int> wgBegin;
+    assign_tasks(s, s->tasks, G, tasksW, wgBegin);
+    if (s->tasks.size() > s->taskCap || G + 1 > s->wgCap) {
         if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
-        if (s->dTasks) hipFree(s->dTasks);
-        if (s->dTasksR) hipFree(s->dTasksR);
-        s->dTasks = s->dTasksR = nullptr;
+        hipFree(s->dTasks);
+        hipFree(s->dTasksW);
+        hipFree(s->dWgBegin);
+        s->dTasks = s->dTasksW = nullptr;
+        s->dWgBegin = nullptr;
         s->taskCap = 0;
-        size_t cap = std::max<size_t>(s->tasks.size(), 1024);
+        s->wgCap = 0;
+        const size_t cap = std::max<size_t>(s->tasks.size(), 1024);
+        const int wcap = std::max(G + 1, 4096);
         if (!XG_HIP(hipMalloc((void **)&s->dTasks, cap * sizeof(int2))) ||
-            !XG_HIP(hipMalloc((void **)&s->dTasksR, cap * sizeof(int2))))
+            !XG_HIP(hipMalloc((void **)&s->dTasksW, cap * sizeof(int2))) ||
+            !XG_HIP(hipMalloc((void **)&s->dWgBegin, wcap * sizeof(int))))
             return false;
         s->taskCap = cap;
+        s->wgCap = wcap;
     }
     if (!s->tasks.empty() && !stage_upload(s, s->dTasks, s->tasks.data(), s->tasks.size() * sizeof(int2))) return false;
-    if (!tasksR.empty() && !stage_upload(s, s->dTasksR, tasksR.data(), tasksR.size() * sizeof(int2))) return false;
-    if (!stage_upload(s, s->dRegionBegin, rb, sizeof(rb))) return false;
+    if (!tasksW.empty() && !stage_upload(s, s->dTasksW, tasksW.data(), tasksW.size() * sizeof(int2))) return false;
+    if (!stage_upload(s, s->dWgBegin, wgBegin.data(), wgBegin.size() * sizeof(int))) return false;
     if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
+    s->gridW = G;
     s->tasksKey = key;
     return true;
 }
@@ -872,8 +974,9 @@ static bool validate_blocks(xg_session *s, bool pred, int ctu)
                 const int sub = p ? 1 : 0;
                 const xg_plane &pl = pp.plane[p];
                 if (!pl.data || pl.stride < pl.width || ((B.x + B.w) >> sub) > pl.width ||
-                    ((B.y + B.h) >> sub) > pl.height) {
-                    set_error(XG_ERR_ARG, "prediction plane %d does not cover block %d", p, i);
+                    ((B.y + B.h) >> sub) > pl.height || (pl.stride & 1) || ((uintptr_t)pl.data & 3)) {
+                    set_error(XG_ERR_ARG, "prediction plane %d does not cover block %d (or is not dword aligned)", p,
+                              i);
                     return false;
                 }
             }
@@ -925,11 +1028,9 @@ static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, b
 // One persistent launch over every task (refinement + MC) on the session stream.
 static bool enqueue_batched(xg_session *s, RefineArgs ra, bool timed)
 {
-    const int grid = std::min(s->numCUs * s->persistFactor, std::max(1, (int)s->tasks.size()));
-    ra.queue = s->dQueue;
-    ra.tasks = s->dTasksR;
-    ra.regionBegin = s->dRegionBegin;
-    ra.nregions = s->nregions;
+    const int grid = s->gridW;
+    ra.wgBegin = s->dWgBegin;
+    ra.tasks = s->dTasksW;
     ra.taskBegin = 0;
     ra.taskEnd = (int)s->tasks.size();
     int e0 = -1, e1 = -1;
@@ -960,7 +1061,7 @@ static bool enqueue_plain(xg_session *s, RefineArgs ra, bool timed)
     for (int oi = 0; oi < kNumShapes; oi++) {
         const int b0 = s->shapeTaskBegin[oi], b1 = s->shapeTaskBegin[oi + 1];
         if (b1 <= b0) continue;
-        ra.queue = nullptr;
+        ra.wgBegin = nullptr;
         ra.tasks = s->dTasks;
         ra.taskBegin = b0;
         ra.taskEnd = b1;
@@ -1002,12 +1103,8 @@ int32_t xg_session_create(xg_session **out, int32_t maxBlocks)
               XG_HIP(hipMalloc((void **)&s->dBlocks, sizeof(xg_block) * (size_t)maxBlocks)) &&
               XG_HIP(hipMalloc((void **)&s->dCandCost, sizeof(uint32_t) * (size_t)maxBlocks * XG_MAX_CAND)) &&
               XG_HIP(hipMalloc((void **)&s->dBest, sizeof(xg_block_result) * (size_t)maxBlocks)) &&
-              XG_HIP(hipMalloc((void **)&s->dQueue, 64 * sizeof(int))) &&
-              XG_HIP(hipMalloc((void **)&s->dRegionBegin, (kRegions + 1) * sizeof(int))) &&
-              XG_HIP(hipMemset(s->dRegionBegin, 0, (kRegions + 1) * sizeof(int))) &&
               XG_HIP(hipMalloc((void **)&s->dDone, sizeof(int) * (size_t)maxBlocks)) &&
               XG_HIP(hipMalloc((void **)&s->dBestKey, sizeof(unsigned long long) * (size_t)maxBlocks)) &&
-              XG_HIP(hipMemset(s->dQueue, 0, 64 * sizeof(int))) &&
               XG_HIP(hipMemset(s->dDone, 0, sizeof(int) * (size_t)maxBlocks)) &&
               XG_HIP(hipMemset(s->dBestKey, 0xFF, sizeof(unsigned long long) * (size_t)maxBlocks)) &&
               XG_HIP(hipMalloc((void **)&s->dOff, 2 * XG_MAX_CAND * sizeof(int16_t)));
@@ -1031,11 +1128,10 @@ int32_t xg_session_destroy(xg_session *s)
     hipFree(s->dPred);
     hipFree(s->dBlocks);
     hipFree(s->dTasks);
-    hipFree(s->dTasksR);
-    hipFree(s->dRegionBegin);
+    hipFree(s->dTasksW);
+    hipFree(s->dWgBegin);
     hipFree(s->dCandCost);
     hipFree(s->dBest);
-    hipFree(s->dQueue);
     hipFree(s->dDone);
     hipFree(s->dBestKey);
     hipFree(s->dOff);
@@ -1284,7 +1380,7 @@ int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t rep
         float bestMs = 1e30f;
         int bestR = s->nregions, bestF = s->persistFactor;
         for (int r : {1, 2, 4, 8})
-            for (int f : {8, 11, 16}) {
+            for (int f : {4, 6, 8}) {
                 s->nregions = r;
                 s->persistFactor = f;
                 const float ms = time_frame(s, cfg, reps, e0, e1, &ok);
