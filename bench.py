@@ -86,6 +86,160 @@ def cpu_baseline(fp, seconds, offs, lam):
                       f"{threads} threads, {dt:.1f} s, {px / 1e6:.1f} Mpixel of blocks"}
 
 
+MFMA_I8_PEAK_TOPS = 5000.0    # dense int8 MFMA, MI355X
+
+
+def session_time(lib, s, stage, fn, steps, warmup):
+    """Average device time (ms) of `stage`'s kernel over `steps` calls of fn,
+    from the session's HIP events on its stream."""
+    from xin26x_amd import capi
+    for _ in range(warmup):
+        fn()
+    capi.check(lib.xg_session_sync(s), "sync")
+    capi.check(lib.xg_session_set_timing(s, 1), "timing")
+    capi.check(lib.xg_session_reset_timers(s), "timers")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    capi.check(lib.xg_session_sync(s), "sync")
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    ms = ctypes.c_float()
+    cnt = ctypes.c_int32()
+    capi.check(lib.xg_session_kernel_time(s, stage, ctypes.byref(ms), ctypes.byref(cnt)), "kernel time")
+    capi.check(lib.xg_session_set_timing(s, 0), "timing")
+    return ms.value, wall
+
+
+def transforms_leg(lib, steps, warmup, cpu_seconds):
+    """200,000 residual blocks, shapes 4..64 x 4..64, DCT-II or DST-VII/DCT-VIII:
+    the batched int8-MFMA forward transform, against the HBM and MFMA rooflines."""
+    import torch
+    from xin26x_amd import capi, workloads as wl
+    desc, offs, resid = wl.residual_blocks(n=200000)
+    n = len(desc)
+    w = 1 << (desc & 15).astype(np.int64)
+    h = 1 << ((desc >> 4) & 15).astype(np.int64)
+    mts = (desc >> 8) & 15
+    keep = lambda side: np.where(mts > 0, np.minimum(side, 16 * (side == 32) + side * (side < 32)), np.minimum(side, 32))
+    wk, hk = keep(w), keep(h)
+    macs = int((h * w * wk + wk * hk * h).sum())          # stage 1 rows, then the kept columns
+    samples = int((w * h).sum())
+    alg_bytes = samples * (2 + 4)                          # int16 residual in, int32 coefficients out
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        dres = torch.from_numpy(resid).cuda()
+        dco = torch.empty(resid.size, dtype=torch.int32, device="cuda")
+        capi.check(lib.xg_fdct_plan(s, offs.ctypes.data, desc.ctypes.data, n), "fdct plan")
+        run = lambda: capi.check(lib.xg_fdct_run(s, dres.data_ptr(), dco.data_ptr()), "fdct run")
+        k_ms, wall_ms = session_time(lib, s, 2, run, steps, warmup)
+        coeff = dco.cpu().numpy()
+    finally:
+        lib.xg_session_destroy(s)
+    k_s = k_ms / 1e3
+    out = {"workload": "200,000 residual blocks, sides 4..64 (VVC-like weights), Laplacian(12) int16 residuals, "
+                       "DCT-II or DST-VII/DCT-VIII (sides <= 32), 10-bit shifts, zero-out",
+           "value": round(n / (wall_ms / 1e3), 1), "unit": "blocks/s",
+           "coefficients_per_s": round(samples / (wall_ms / 1e3), 1), "ms_per_batch": round(wall_ms, 4),
+           "kernel": "k_fdct_mfma", "kernel_ms": round(k_ms, 4),
+           "roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
+                        "traffic": None, "alg_bytes_per_launch": alg_bytes},
+           "mfma": {"achieved": round(2 * macs / k_s / 1e12, 3), "peak": MFMA_I8_PEAK_TOPS, "unit": "TOP/s",
+                    "frac": round(2 * macs / k_s / 1e12 / MFMA_I8_PEAK_TOPS, 6),
+                    "useful_macs_per_launch": macs}}
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = transforms_cpu(desc, offs, resid, coeff, cpu_seconds)
+    return out
+
+
+def transforms_cpu(desc, offs, resid, gpu_coeff, seconds):
+    """The oracle's integer transform on the host, chunks of the same blocks;
+    the chunks it computes are also compared with the GPU's coefficients."""
+    from tests.oracle_lib import load_oracle
+    oracle = load_oracle()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = np.zeros(resid.size, np.int32)
+    chunk, n = 2000, len(desc)
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    done = []
+
+    def work(i0):
+        i1 = min(n, i0 + chunk)
+        oracle.or_FDctBatch(P(resid), P(offs), P(desc), i0, i1, P(out))
+        return i0, i1
+
+    blocks = 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        pos = 0
+        while time.perf_counter() - t0 < seconds and pos < n:
+            starts = [pos + k * chunk for k in range(threads * 2) if pos + k * chunk < n]
+            pos += threads * 2 * chunk
+            for i0, i1 in ex.map(work, starts):
+                done.append((i0, i1))
+                blocks += i1 - i0
+    dt = time.perf_counter() - t0
+    exact = all(np.array_equal(out[offs[i0]:offs[i1 - 1] + (1 << (desc[i1 - 1] & 15)) * (1 << ((desc[i1 - 1] >> 4) & 15))],
+                               gpu_coeff[offs[i0]:offs[i1 - 1] + (1 << (desc[i1 - 1] & 15)) * (1 << ((desc[i1 - 1] >> 4) & 15))])
+                for i0, i1 in done)
+    return {"value": round(blocks / dt, 1), "unit": "blocks/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C, int64 matrix products, -O3) on the first {blocks} blocks in chunks of "
+                      f"{chunk}, {threads} host threads, {dt:.1f} s",
+            "bit_exact_vs_gpu_on_sample": bool(exact)}
+
+
+def search_leg(lib, steps, warmup, cpu_seconds):
+    """The plain 416x240 integer full search: 390 16x16 blocks, +/-16, lambda 4."""
+    import torch
+    from xin26x_amd import capi, workloads as wl
+    from xin26x_amd.device import DevicePicture
+    cur, ref = wl.frame_pair_416x240()
+    W, H = 416, 240
+    nb = (W // 16) * (H // 16)
+    dc = DevicePicture([cur, cur[::2, ::2].copy(), cur[::2, ::2].copy()], pad=0)
+    dr = DevicePicture([ref, ref[::2, ::2].copy(), ref[::2, ::2].copy()], pad=80)
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        mv = torch.zeros(2 * nb, dtype=torch.int16, device="cuda")
+        cost = torch.zeros(nb, dtype=torch.int32, device="cuda")
+        sad = torch.zeros(nb, dtype=torch.int32, device="cuda")
+        run = lambda: capi.check(lib.xg_full_search_frame(s, ctypes.byref(dc.pic.plane[0]),
+                                                          ctypes.byref(dr.pic.plane[0]), 16, 16, 16, 1024, 64,
+                                                          mv.data_ptr(), cost.data_ptr(), sad.data_ptr()), "search")
+        k_ms, wall_ms = session_time(lib, s, 3, run, steps, warmup)
+        gmv = mv.cpu().numpy()
+    finally:
+        lib.xg_session_destroy(s)
+    evals = nb * 33 * 33
+    out = {"workload": "416x240 10-bit, 390 16x16 blocks, integer full search +/-16, (SAD<<8)+bits*1024",
+           "value": round(W * H / (wall_ms / 1e3) / 1e6, 3), "unit": "Mpixel/s", "ms_per_frame": round(wall_ms, 4),
+           "sad_evals_per_s": round(evals / (wall_ms / 1e3), 1), "kernel": "k_full_search_frame",
+           "kernel_ms": round(k_ms, 4),
+           "kernel_abs_diff_ops_per_s": round(evals * 256 / (k_ms / 1e3), 1)}
+    if cpu_seconds > 0:
+        from tests.oracle_lib import load_oracle
+        oracle = load_oracle()
+        refp = np.ascontiguousarray(np.pad(ref, 80, mode="edge"))
+        rs = W + 160
+        emx, emy = np.zeros(nb, np.int16), np.zeros(nb, np.int16)
+        ec, es = np.zeros(nb, np.uint32), np.zeros(nb, np.uint32)
+        P = lambda a, off=0: ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
+        frames, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            oracle.or_FullSearchFrame(P(cur), W, P(refp, 80 * rs + 80), rs, W, H, 16, 16, 16, 1024, 64, 0, nb,
+                                      P(emx), P(emy), P(ec), P(es))
+            frames += 1
+        dt = time.perf_counter() - t0
+        exact = bool(np.array_equal(gmv.reshape(nb, 2)[:, 0], emx) and np.array_equal(gmv.reshape(nb, 2)[:, 1], emy))
+        out["cpu_baseline"] = {"value": round(W * H * frames / dt / 1e6, 3), "unit": "Mpixel/s", "cores": 1,
+                               "kind": "port", "sample": f"oracle (scalar C, -O3) full frames, 1 thread, {frames} "
+                                                         f"frames in {dt:.1f} s",
+                               "bit_exact_vs_gpu": exact}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +253,7 @@ def main():
     ap.add_argument("--save-tiles", default=None, help="write the live-tuned table here")
     ap.add_argument("--quick", action="store_true",
                     help="profiling run: batched form only (no plain form, latency loop or CPU leg)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the transform and 416x240 search legs")
     ap.add_argument("--serve-frames", type=int, default=90,
                     help="frames per stream of the 16-stream serving run at 30 fps (0 = skip serving)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -225,6 +380,13 @@ def main():
     plain = frame_px / t_plain / 1e6
     plain_tuned = frame_px / t_plain_tuned / 1e6
 
+    transforms = search = None
+    if not args.quick and not args.no_extra:
+        log("transforms: 200,000 residual blocks")
+        transforms = transforms_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
+        log("plain 416x240 full search")
+        search = search_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
+
     serving = None
     if args.serve_frames > 0 and not args.quick:
         from xin26x_amd import serve
@@ -285,6 +447,8 @@ def main():
                      "kernel": "k_refine_persistent", "kernel_ms": round(ms.value, 4), "launches": cnt.value,
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "transforms": transforms,
+        "plain_search_416x240": search,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -366,8 +366,11 @@ int32_t xg_session_sync(xg_session *s);
 /* Waits for the session stream and copies the last results to host memory
  * (either pointer may be NULL): candCost[nblocks*ncand], best[nblocks]. */
 int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best);
-/* Average device time (ms) of the refinement-stage kernel over the launches
- * since the last reset, measured with HIP events on the session stream. */
+/* Average device time (ms) per launch since the last reset, measured with HIP
+ * events on the session stream while timing is enabled (xg_session_set_timing):
+ * stage 0 the refinement kernel(s) of the eager modes, 1 a whole graph replay,
+ * 2 the forward-transform kernel (xg_fdct_run), 3 the integer-search kernel
+ * (xg_search_blocks, xg_full_search_frame). */
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
 int32_t xg_session_reset_timers(xg_session *s);
 int32_t xg_session_set_timing(xg_session *s, int32_t enable);

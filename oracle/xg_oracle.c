@@ -931,3 +931,20 @@ void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, i
         bestSad[b] = me.bestSad;
     }
 }
+
+/* Forward transform of a batch of residual blocks laid out as the batched
+ * GPU entry takes them (xg_fdct_plan): block i is int16 at resid + offset[i],
+ * row-major with stride = width, desc[i] = lgW | lgH << 4 | mtsIdx << 8;
+ * coefficients go to coeff + offset[i], stride = width.  Each block is
+ * or_FDctWxH (h266_forward_trans.c:45-152) with the explicit mtsIdx, 10-bit. */
+void or_FDctBatch(const int16_t *resid, const int64_t *offset, const uint32_t *desc, int32_t i0, int32_t i1,
+                  or_coeff *coeff)
+{
+    or_coeff tmp[64 * 64];
+    for (int32_t i = i0; i < i1; i++) {
+        const uint32_t w = 1u << (desc[i] & 15), h = 1u << ((desc[i] >> 4) & 15), mts = (desc[i] >> 8) & 15;
+        const int16_t *r = resid + offset[i];
+        for (uint32_t k = 0; k < w * h; k++) tmp[k] = r[k];
+        or_FDctWxH(tmp, w, coeff + offset[i], w, w, h, mts, 0, 0, 10);
+    }
+}

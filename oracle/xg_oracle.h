@@ -144,6 +144,10 @@ void or_FDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
                 uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts,
                 int bitDepth);
 void or_MtsTypes(uint32_t mtsIdx, int32_t *horTx, int32_t *verTx);
+/* Blocks [i0, i1) of a batch in the layout of the batched GPU entry
+ * (int16 residuals at resid + offset[i], desc = lgW | lgH << 4 | mtsIdx << 8). */
+void or_FDctBatch(const int16_t *resid, const int64_t *offset, const uint32_t *desc, int32_t i0, int32_t i1,
+                  or_coeff *coeff);
 
 /* ---- frame-level drivers used as the batched-path checker ---- */
 typedef struct or_block {

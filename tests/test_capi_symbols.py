@@ -36,3 +36,10 @@ def test_library_has_gfx950_code_object():
     # the offload bundle of a library built here names its target
     data = open(capi.LIB_PATH, "rb").read()
     assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_no_unresolved_internal_symbols():
+    # every symbol of the library's own namespace is defined inside it
+    out = subprocess.run(["nm", "-D", "--undefined-only", capi.LIB_PATH], capture_output=True, text=True).stdout
+    internal = [ln.split()[-1] for ln in out.splitlines() if "_ZN2xg" in ln or " xg_" in ln or " Xg2" in ln]
+    assert not internal, internal

@@ -307,3 +307,18 @@ def test_dispatcher_types_and_implicit_mts(oracle):
     out2 = np.zeros((32, 8), np.int32)
     oracle.or_FDctWxH(ptr(r2), 8, ptr(out2), 8, 8, 32, 0, 1, 1, 10)     # height 32: vertical stays DCT-II
     assert np.array_equal(out2, two_pass(r2, 2, 0))
+
+
+def test_fdct_batch_matches_per_block_calls(oracle):
+    """The batched helper (the transform CPU baseline) is per-block or_FDctWxH
+    on the int16 layout of the batched GPU entry."""
+    from xin26x_amd import workloads as wl
+    desc, offs, resid = wl.residual_blocks(n=300, seed=11)
+    got = np.zeros(resid.size, np.int32)
+    oracle.or_FDctBatch(ptr(resid), ptr(offs), ptr(desc), 0, len(desc), ptr(got))
+    exp = np.zeros(resid.size, np.int32)
+    for d, o in zip(desc.tolist(), offs.tolist()):
+        w, h, mts = 1 << (d & 15), 1 << ((d >> 4) & 15), (d >> 8) & 15
+        r32 = resid[o:o + w * h].astype(np.int32)
+        oracle.or_FDctWxH(ptr(r32), w, ptr(exp, o), w, w, h, mts, 0, 0, 10)
+    assert np.array_equal(got, exp)

@@ -297,7 +297,10 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
         return 256;
     }();
     const int grid = std::min(p->n, cus * 16);
+    int ev[2];
+    const bool timed = session_timer_begin(s, 2, ev);
     k_fdct_mfma<<<grid, 64, 0, (hipStream_t)xg_session_stream(s)>>>(a);
+    if (timed) session_timer_end(s, 2, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 

@@ -722,10 +722,10 @@ struct xg_session {
     // timing
     bool timing = false;
     std::vector<hipEvent_t> evPool;
-    std::vector<std::pair<int, int>> evPending[2];  // per stage: (start, stop) indices
+    std::vector<std::pair<int, int>> evPending[kTimerStages];  // per stage: (start, stop) indices
     int evNext = 0;
-    double accMs[2] = {0, 0};
-    int accN[2] = {0, 0};
+    double accMs[kTimerStages] = {};
+    int accN[kTimerStages] = {};
 };
 
 namespace xg {
@@ -914,7 +914,7 @@ static int ev_get(xg_session *s)
 
 static void ev_collect(xg_session *s)
 {
-    for (int st = 0; st < 2; st++) {
+    for (int st = 0; st < kTimerStages; st++) {
         for (auto &pr : s->evPending[st]) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, s->evPool[pr.first], s->evPool[pr.second]) == hipSuccess) {
@@ -925,6 +925,20 @@ static void ev_collect(xg_session *s)
         s->evPending[st].clear();
     }
     s->evNext = 0;
+}
+
+bool session_timer_begin(xg_session *s, int stage, int ev[2])
+{
+    if (!s->timing || stage < 0 || stage >= kTimerStages) return false;
+    ev[0] = ev_get(s);
+    ev[1] = ev_get(s);
+    if (ev[0] < 0 || ev[1] < 0) return false;
+    return hipEventRecord(s->evPool[ev[0]], s->stream) == hipSuccess;
+}
+
+void session_timer_end(xg_session *s, int stage, const int ev[2])
+{
+    if (hipEventRecord(s->evPool[ev[1]], s->stream) == hipSuccess) s->evPending[stage].push_back({ev[0], ev[1]});
 }
 
 static bool validate_cfg(const xg_refine_cfg *cfg)
@@ -1330,15 +1344,18 @@ int32_t xg_session_reset_timers(xg_session *s)
 {
     if (!s) return XG_ERR_ARG;
     xg_session_sync(s);
-    s->accMs[0] = s->accMs[1] = 0;
-    s->accN[0] = s->accN[1] = 0;
+    for (int st = 0; st < kTimerStages; st++) {
+        s->accMs[st] = 0;
+        s->accN[st] = 0;
+    }
     return XG_OK;
 }
 
-// stage 0: refinement kernel(s) of the eager modes; stage 1: whole graph replay.
+// stage 0: refinement kernel(s) of the eager modes; 1: whole graph replay;
+// 2: forward-transform kernel; 3: integer-search kernel.
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count)
 {
-    if (!s || stage < 0 || stage > 1) return XG_ERR_ARG;
+    if (!s || stage < 0 || stage >= kTimerStages) return XG_ERR_ARG;
     if (xg_session_sync(s) != XG_OK) return XG_ERR_HIP;
     if (avgMs) *avgMs = s->accN[stage] ? (float)(s->accMs[stage] / s->accN[stage]) : 0.f;
     if (count) *count = s->accN[stage];

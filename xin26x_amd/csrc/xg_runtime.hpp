@@ -43,6 +43,12 @@ void unpack2d(void *dst, intptr_t dstStride, const void *src, int w, int h, size
 // Releases the forward-transform plan a session owns (xg_fdct.hip).
 void fdct_release(xg_session *s);
 
+// Session kernel timers (HIP events on the session stream), by stage:
+// 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search.
+constexpr int kTimerStages = 4;
+bool session_timer_begin(xg_session *s, int stage, int ev[2]);   // false when timing is off
+void session_timer_end(xg_session *s, int stage, const int ev[2]);
+
 // What other translation units need of a session (xg_refine.hip owns it).
 struct SessionView {
     hipStream_t stream;

@@ -431,7 +431,10 @@ int32_t xg_search_blocks(xg_session *s, int32_t range, uint32_t sadLambda, int32
         set_error(XG_ERR_ARG, "xg_search_blocks: window exceeds the LDS budget");
         return XG_ERR_ARG;
     }
+    int ev[2];
+    const bool timed = session_timer_begin(s, 3, ev);
     k_search_blocks<<<v.nblocks, 256, lds, v.stream>>>(v.dPics, v.dBlocks, range, sadLambda, ctuSize);
+    if (timed) session_timer_end(s, 3, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
@@ -459,7 +462,10 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
     a.lambda = sadLambda;
     a.bestMv = bestMv; a.bestCost = bestCost; a.bestSad = bestSad;
     hipStream_t st = (hipStream_t)xg_session_stream(s);
+    int ev[2];
+    const bool timed = session_timer_begin(s, 3, ev);
     k_full_search_frame<<<nbx * nby, 256, lds, st>>>(a);
+    if (timed) session_timer_end(s, 3, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
