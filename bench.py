@@ -32,7 +32,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 TUNED = os.path.join(ROOT, "xin26x_amd", "tuned", "refine_1080p_cand32.txt")
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_refine_1080p_pmc.md")
+PMC_REFINE = os.path.join(ROOT, "profiles", "k_refine_1080p.md")
+PMC_FDCT = os.path.join(ROOT, "profiles", "k_fdct_mfma_200k_blocks.md")
+
+
+def profiled_traffic(path):
+    """HBM bytes per launch from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (calibrated against a copy kernel), or (None, None)."""
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return int(ln.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12     # lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 METRIC = "ME+MC Mpixel/s at 1080p (batched vs plain, bit-exact); p50/p99 frame latency"
@@ -144,7 +158,8 @@ def transforms_leg(lib, steps, warmup, cpu_seconds):
            "kernel": "k_fdct_mfma", "kernel_ms": round(k_ms, 4),
            "roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
-                        "traffic": None, "alg_bytes_per_launch": alg_bytes},
+                        "traffic": profiled_traffic(PMC_FDCT)[0], "traffic_source": profiled_traffic(PMC_FDCT)[1],
+                        "alg_bytes_per_launch": alg_bytes},
            "mfma": {"achieved": round(2 * macs / k_s / 1e12, 3), "peak": MFMA_I8_PEAK_TOPS, "unit": "TOP/s",
                     "frac": round(2 * macs / k_s / 1e12 / MFMA_I8_PEAK_TOPS, 6),
                     "useful_macs_per_launch": macs}}
@@ -401,15 +416,7 @@ def main():
                                "fractional refinement, MC; reference = previous frame",
                    "realtime_30fps": rt, "saturated": sat}
 
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (calibrated)
-    traffic, traffic_src = None, None
-    try:
-        with open(PMC_SUMMARY) as f:
-            for ln in f:
-                if ln.startswith("traffic_bytes_per_launch:"):
-                    traffic, traffic_src = int(ln.split(":")[1]), os.path.relpath(PMC_SUMMARY, ROOT)
-    except OSError:
-        pass
+    traffic, traffic_src = profiled_traffic(PMC_REFINE)
 
     cpu = None
     if not args.no_cpu and not args.quick:
