@@ -255,6 +255,108 @@ def search_leg(lib, steps, warmup, cpu_seconds):
     return out
 
 
+SAD_PAIR_PEAK = 256 * 64 * 2.4e9 * 2 / 1e12   # abs-diffs/s: one v_sad_u16 (two samples) per lane per cycle
+
+
+def pipeline_leg(lib, steps, warmup, cpu_seconds):
+    """3840x2160, two references: full search +/-64, predictor + candidates +
+    diamond, half/quarter-pel rings, reference choice, 1/16-pel refinement +
+    MC, replayed per frame from one captured HIP graph."""
+    from xin26x_amd import capi, workloads as wl
+    from xin26x_amd.device import DevicePicture, refine_cfg
+    REFINE_OFFS = wl.CAND17_SIXTEENTH
+    W, H, R, LAM = 3840, 2160, 64, 1024
+    log("pipeline: generating the 4K layered scene")
+    fr = wl.layered_frames(W, H)
+    cur, ref0, ref1 = fr[2], fr[1], fr[0]
+    nbx, nby = W // 16, H // 16
+    nb = nbx * nby
+    evals = 0                                   # SAD evaluations of the full search (windows clipped at the frame)
+    for by in range(nby):
+        for bx in range(nbx):
+            x, y = bx * 16, by * 16
+            nx = min(R, W + 56 - x) - max(-R, -x - 72) + 1
+            ny = min(R, H + 56 - y) - max(-R, -y - 72) + 1
+            evals += 2 * nx * ny
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), nb), "session")
+    try:
+        dc, d0, d1 = DevicePicture(cur, 80), DevicePicture(ref0, 80), DevicePicture(ref1, 80)
+        pred = DevicePicture.empty(W, H)
+        cfg = capi.XgPipelineCfg()
+        cfg.range, cfg.sadLambda, cfg.ctuSize = R, LAM, 64
+        cfg.refine = refine_cfg(REFINE_OFFS, wl.LAMBDA_REFINE)
+        run = lambda mode: capi.check(lib.xg_me_pipeline(s, ctypes.byref(cfg), dc.ref(), d0.ref(), d1.ref(),
+                                                         pred.ref(), mode), "pipeline")
+        log("pipeline: eager frames (per-kernel timing)")
+        k_ms, eager_ms = session_time(lib, s, 3, lambda: run(capi.MODE_BATCHED), steps, warmup)
+        frame_ms = ctypes.c_float()
+        cnt = ctypes.c_int32()
+        capi.check(lib.xg_session_kernel_time(s, 4, ctypes.byref(frame_ms), ctypes.byref(cnt)), "frame time")
+        log("pipeline: graph replays")
+        for _ in range(warmup):
+            run(capi.MODE_GRAPH)
+        capi.check(lib.xg_session_sync(s), "sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run(capi.MODE_GRAPH)
+        capi.check(lib.xg_session_sync(s), "sync")
+        graph_ms = (time.perf_counter() - t0) / steps * 1e3
+        q = np.zeros(4 * nb, np.int16)
+        capi.check(lib.xg_me_pipeline_read(s, None, None, None, None, q.ctypes.data, None), "read")
+    finally:
+        lib.xg_session_destroy(s)
+    k_s = k_ms / 1e3
+    absdiff = evals * 256
+    out = {"workload": "3840x2160 10-bit 4:2:0, layered scene (pan (+5.5,+2.25) px + 12 moving rectangles, "
+                       "MVs up to +/-64 px, occlusion), 2 references, 32,400 16x16 blocks: full search +/-64, "
+                       "median predictor + candidates + diamond, half/quarter-pel rings, reference choice, "
+                       f"{len(REFINE_OFFS)}-candidate 1/16-pel refinement + MC; one HIP graph per frame",
+           "value": round(W * H / (graph_ms / 1e3) / 1e6, 2), "unit": "Mpixel/s", "ms_per_frame": round(graph_ms, 3),
+           "eager_ms_per_frame": round(eager_ms, 3), "eager_device_ms_per_frame": round(frame_ms.value, 3),
+           "sad_evals_per_frame": evals, "kernel": "k_pipe_full", "kernel_ms": round(k_ms, 3),
+           "roofline": {"bound": "valu", "achieved": round(absdiff / k_s / 1e12, 2), "peak": round(SAD_PAIR_PEAK, 1),
+                        "unit": "T abs-diff/s", "frac": round(absdiff / k_s / 1e12 / SAD_PAIR_PEAK, 4),
+                        "note": "v_sad_u16 sums two packed absolute differences per lane and cycle"},
+           "median_quarter_mv_ref0": [int(np.median(q.reshape(-1, 2, 2)[:, 0, 0])),
+                                      int(np.median(q.reshape(-1, 2, 2)[:, 0, 1]))]}
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = pipeline_cpu(cur, ref0, ref1, cpu_seconds)
+    return out
+
+
+def pipeline_cpu(cur, ref0, ref1, seconds):
+    """The oracle's pipeline on 128x128 crops of the same frames (edge-padded
+    crops, so the search windows match the interior of the frame), one crop per
+    host thread."""
+    from tests.oracle_lib import load_oracle
+    from tests.pipeline_common import oracle_pipeline
+    oracle = load_oracle()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    cw = 128
+    H, W = cur[0].shape
+    crops = []
+    for k in range(64):
+        x = (k * 977) % (W - 2 * cw) // 16 * 16 + cw
+        y = (k * 571) % (H - 2 * cw) // 16 * 16 + cw
+        crop = lambda f: tuple(np.ascontiguousarray(p[(y >> (i > 0)):(y + cw) >> (i > 0), (x >> (i > 0)):(x + cw) >> (i > 0)])
+                               for i, p in enumerate(f))
+        crops.append((crop(cur), crop(ref0), crop(ref1)))
+    px = 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        pos = 0
+        while time.perf_counter() - t0 < seconds:
+            batch = [crops[(pos + k) % len(crops)] for k in range(threads)]
+            pos += threads
+            list(ex.map(lambda c: oracle_pipeline(oracle, *c), batch))
+            px += threads * cw * cw
+    dt = time.perf_counter() - t0
+    return {"value": round(px / dt / 1e6, 4), "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "sample": f"oracle pipeline (scalar C, -O3) on {px // (cw * cw)} 128x128 crops of the same frames, "
+                      f"one per thread, {dt:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -395,8 +497,10 @@ def main():
     plain = frame_px / t_plain / 1e6
     plain_tuned = frame_px / t_plain_tuned / 1e6
 
-    transforms = search = None
+    transforms = search = pipeline = None
     if not args.quick and not args.no_extra:
+        log("full motion-search pipeline, 4K, two references")
+        pipeline = pipeline_leg(lib, max(5, args.steps // 4), 2, 0 if args.no_cpu else 10.0)
         log("transforms: 200,000 residual blocks")
         transforms = transforms_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
         log("plain 416x240 full search")
@@ -456,6 +560,7 @@ def main():
         "cpu_baseline": cpu,
         "transforms": transforms,
         "plain_search_416x240": search,
+        "pipeline_4k": pipeline,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

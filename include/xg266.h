@@ -370,7 +370,8 @@ int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best
  * events on the session stream while timing is enabled (xg_session_set_timing):
  * stage 0 the refinement kernel(s) of the eager modes, 1 a whole graph replay,
  * 2 the forward-transform kernel (xg_fdct_run), 3 the integer-search kernel
- * (xg_search_blocks, xg_full_search_frame). */
+ * (xg_search_blocks, xg_full_search_frame, the full search of xg_me_pipeline),
+ * 4 a whole xg_me_pipeline frame. */
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
 int32_t xg_session_reset_timers(xg_session *s);
 int32_t xg_session_set_timing(xg_session *s, int32_t enable);
@@ -447,6 +448,39 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
                              int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda,
                              int32_t ctuSize, int16_t *bestMv, uint32_t *bestCost,
                              uint32_t *bestSad);
+
+/* Full motion-search pipeline of one frame with two reference pictures, on a
+ * raster grid of 16x16 blocks (frame sides multiples of 16): per block and
+ * reference, integer full search +/-range around (0,0) (as Xg26xMotionEstFull,
+ * predictor (0,0)); predictor = component-wise median of the left, top and
+ * top-right blocks' full-search MVs; start candidates as Xin266FillMeCand
+ * (h266_analyze_inter_cu.c:133-293) from that predictor and the own, left,
+ * top and top-right MVs; diamond search +/-range around the rounded predictor
+ * (as Xg26xMotionEstDia); half- then quarter-pel rings (Xin266FullSubPelMe,
+ * h266_sub_pel_refine.c:81-212, filterIndex 0); then the reference with the
+ * lower quarter-pel cost (ties: ref0) is refined at 1/16 pel by
+ * xg_refine_frac's kernel (candidate offsets around 4x the quarter-pel MV,
+ * predictor 4x the median) and its prediction written to `pred`.
+ * Pictures are installed in session slots 0 (cur), 1 (ref0) and 2 (ref1); the
+ * session's block list becomes the grid.  mode: XG_MODE_BATCHED (kernels
+ * enqueued in order) or XG_MODE_GRAPH (the same captured once in one HIP
+ * graph and replayed).  Results: xg_session_read / xg_session_outputs for the
+ * refinement, xg_me_pipeline_read for the earlier stages. */
+typedef struct xg_pipeline_cfg {
+    int32_t       range;           /* integer search range in pel (<= 64)          */
+    uint32_t      sadLambda;       /* integer and quarter-pel stages               */
+    int32_t       ctuSize;         /* CU MV window (h266_enc_init.c:1462-1465)     */
+    xg_refine_cfg refine;          /* 1/16-pel stage and motion compensation        */
+} xg_pipeline_cfg;
+
+int32_t xg_me_pipeline(xg_session *s, const xg_pipeline_cfg *cfg, const xg_picture *cur,
+                       const xg_picture *ref0, const xg_picture *ref1, const xg_picture *pred,
+                       int32_t mode);
+/* Host copies of the stage outputs of the last pipeline run, per block b and
+ * reference r (any pointer may be NULL): fullMv / intMv [(b*2+r)*2 + {x,y}] in
+ * integer pel, predMv / qMv likewise in quarter pel, intCost / qCost [b*2+r]. */
+int32_t xg_me_pipeline_read(xg_session *s, int16_t *fullMv, int16_t *predMv, int16_t *intMv,
+                            uint32_t *intCost, int16_t *qMv, uint32_t *qCost);
 
 /* Batched forward transform of residual blocks on int8 MFMA.
  * xg_fdct_plan takes HOST arrays: block i's residual is at resid + offset[i]

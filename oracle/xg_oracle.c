@@ -948,3 +948,105 @@ void or_FDctBatch(const int16_t *resid, const int64_t *offset, const uint32_t *d
         or_FDctWxH(tmp, w, coeff + offset[i], w, w, h, mts, 0, 0, 10);
     }
 }
+
+static int32_t median3(int32_t a, int32_t b, int32_t c)
+{
+    int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    return c < lo ? lo : (c > hi ? hi : c);
+}
+
+/* Full motion-search pipeline over a raster grid of bw x bh blocks with two
+ * reference pictures (a build composition of the reference's stages):
+ *  1. full search +/-range around (0,0), predictor (0,0), per block and
+ *     reference (or_MotionEstFull = closed Xin26xMotionEstFull);
+ *  2. predictor = component-wise median of the pass-1 MVs of the left, top and
+ *     top-right blocks (missing neighbours count as (0,0)), in quarter pel;
+ *     start candidates = Xin266FillMeCand(predictor; own, left, top, top-right
+ *     pass-1 MVs) (h266_analyze_inter_cu.c:133-293); diamond search +/-range
+ *     around the rounded predictor (or_MotionEstDia = closed Xin26xMotionEstDia);
+ *  3. half- then quarter-pel ring from the diamond result
+ *     (h266_sub_pel_refine.c:81-212, or_FullSubPelMe);
+ *  4. reference with the lower quarter-pel cost (ties: reference 0);
+ *  5. 1/16-pel refinement + MC of the chosen reference (or_RefineFracBlocks),
+ *     base MV = quarter-pel MV, predictor = that reference's predictor.
+ * Pictures point at sample (0,0) of padded planes (luma stride sy, chroma sc).
+ * Per block b and reference r: fullMv/intMv/qMv[(b*2+r)*2 + {0,1}] (int pel,
+ * int pel, quarter pel), predMv likewise (quarter pel), intCost/qCost[b*2+r];
+ * refIdx[b]; candCost[b*ncand + c], bestIdx/bestCost/bestMvx/bestMvy[b]. */
+void or_MePipeline(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV, intptr_t csy, intptr_t csc,
+                   const or_pixel *r0Y, const or_pixel *r0U, const or_pixel *r0V,
+                   const or_pixel *r1Y, const or_pixel *r1U, const or_pixel *r1V, intptr_t rsy, intptr_t rsc,
+                   int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda,
+                   int32_t ctuSize, const or_refine_prm *prm,
+                   int16_t *fullMv, int16_t *predMv, int16_t *intMv, uint32_t *intCost, int16_t *qMv,
+                   uint32_t *qCost, int32_t *refIdx, uint32_t *candCost, int32_t *bestIdx, uint32_t *bestCost,
+                   int32_t *bestMvx, int32_t *bestMvy, or_pixel *predY, or_pixel *predU, or_pixel *predV,
+                   intptr_t psy, intptr_t psc)
+{
+    const int32_t nbx = fw / bw, nby = fh / bh, nb = nbx * nby;
+    const or_pixel *refs[2] = {r0Y, r1Y};
+    for (int32_t r = 0; r < 2; r++)
+        for (int32_t b = 0; b < nb; b++) {
+            const int32_t x = (b % nbx) * bw, y = (b / nbx) * bh;
+            int32_t mnx, mny, mxx, mxy;
+            cu_mv_window(x, y, bw, bh, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
+            or_me me;
+            memset(&me, 0, sizeof(me));
+            me.width = (uint32_t)bw; me.height = (uint32_t)bh;
+            me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny; me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
+            me.searchWidth = me.searchHeight = range;
+            me.sadLambda = sadLambda;
+            me.input = curY + y * csy + x; me.inputStride = csy;
+            me.ref = refs[r] + y * rsy + x; me.refStride = rsy;
+            or_MotionEstFull(&me);
+            fullMv[(b * 2 + r) * 2] = (int16_t)(me.bestMvX >> 2);
+            fullMv[(b * 2 + r) * 2 + 1] = (int16_t)(me.bestMvY >> 2);
+        }
+    for (int32_t r = 0; r < 2; r++)
+        for (int32_t b = 0; b < nb; b++) {
+            const int32_t bx = b % nbx, by = b / nbx, x = bx * bw, y = by * bh;
+            int16_t nq[4][2];                       /* own, left, top, top-right, quarter pel */
+            const int32_t nbi[4] = {b, bx > 0 ? b - 1 : -1, by > 0 ? b - nbx : -1,
+                                    (by > 0 && bx + 1 < nbx) ? b - nbx + 1 : -1};
+            for (int k = 0; k < 4; k++)
+                for (int c = 0; c < 2; c++)
+                    nq[k][c] = nbi[k] < 0 ? 0 : (int16_t)(fullMv[(nbi[k] * 2 + r) * 2 + c] * 4);
+            const int16_t px = (int16_t)median3(nq[1][0], nq[2][0], nq[3][0]);
+            const int16_t py = (int16_t)median3(nq[1][1], nq[2][1], nq[3][1]);
+            predMv[(b * 2 + r) * 2] = px;
+            predMv[(b * 2 + r) * 2 + 1] = py;
+            int32_t mnx, mny, mxx, mxy;
+            cu_mv_window(x, y, bw, bh, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
+            or_me me;
+            memset(&me, 0, sizeof(me));
+            const int16_t ex[4] = {nq[0][0], nq[1][0], nq[2][0], nq[3][0]};
+            const int16_t ey[4] = {nq[0][1], nq[1][1], nq[2][1], nq[3][1]};
+            or_FillMeCand(px, py, ex, ey, 4, me.candX, me.candY, &me.candNum);
+            me.width = (uint32_t)bw; me.height = (uint32_t)bh;
+            me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny; me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
+            me.predMvX = px; me.predMvY = py;
+            me.searchWidth = me.searchHeight = range;
+            me.sadLambda = sadLambda;
+            me.input = curY + y * csy + x; me.inputStride = csy;
+            me.ref = refs[r] + y * rsy + x; me.refStride = rsy;
+            or_MotionEstDia(&me);
+            intMv[(b * 2 + r) * 2] = (int16_t)(me.bestMvX >> 2);
+            intMv[(b * 2 + r) * 2 + 1] = (int16_t)(me.bestMvY >> 2);
+            intCost[b * 2 + r] = me.bestCost;
+            or_FullSubPelMe(&me);
+            qMv[(b * 2 + r) * 2] = me.bestMvX;
+            qMv[(b * 2 + r) * 2 + 1] = me.bestMvY;
+            qCost[b * 2 + r] = me.bestCost;
+        }
+    for (int32_t b = 0; b < nb; b++) {
+        const int32_t r = qCost[b * 2 + 1] < qCost[b * 2] ? 1 : 0;
+        refIdx[b] = r;
+        or_block blk;
+        blk.x = (b % nbx) * bw; blk.y = (b / nbx) * bh; blk.w = bw; blk.h = bh;
+        blk.mvx = qMv[(b * 2 + r) * 2] * 4; blk.mvy = qMv[(b * 2 + r) * 2 + 1] * 4;
+        blk.mvpx = predMv[(b * 2 + r) * 2] * 4; blk.mvpy = predMv[(b * 2 + r) * 2 + 1] * 4;
+        or_RefineFracBlocks(curY, curU, curV, csy, csc, r ? r1Y : r0Y, r ? r1U : r0U, r ? r1V : r0V, rsy, rsc,
+                            fw, fh, &blk, 1, prm, candCost + (size_t)b * prm->ncand, bestIdx + b, bestCost + b,
+                            bestMvx + b, bestMvy + b, predY, predU, predV, psy, psc);
+    }
+}

@@ -48,6 +48,10 @@ class XgServerCfg(ctypes.Structure):
                 ("searchRange", I32), ("searchLambda", U32), ("maxDelayUs", I32), ("refine", XgRefineCfg)]
 
 
+class XgPipelineCfg(ctypes.Structure):
+    _fields_ = [("range", I32), ("sadLambda", U32), ("ctuSize", I32), ("refine", XgRefineCfg)]
+
+
 class XgFrameResult(ctypes.Structure):
     _fields_ = [("stream", I32), ("status", I32), ("nblocks", I32), ("batchFrames", I32),
                 ("tag", ctypes.c_uint64), ("submitNs", ctypes.c_int64), ("startNs", ctypes.c_int64),
@@ -144,6 +148,9 @@ PROTOS = {
     "xg_server_submit": (I32, [P, I32, P, P, P, I32, P, P, P, ctypes.c_uint64]),
     "xg_server_poll": (I32, [P, ctypes.POINTER(XgFrameResult), I32, I32]),
     "xg_server_stats_get": (I32, [P, ctypes.POINTER(XgServerStats)]),
+    "xg_me_pipeline": (I32, [P, ctypes.POINTER(XgPipelineCfg), ctypes.POINTER(XgPicture), ctypes.POINTER(XgPicture),
+                             ctypes.POINTER(XgPicture), ctypes.POINTER(XgPicture), I32]),
+    "xg_me_pipeline_read": (I32, [P, P, P, P, P, P, P]),
     "xg_fdct_plan": (I32, [P, P, P, I32]),
     "xg_fdct_run": (I32, [P, P, P]),
 }

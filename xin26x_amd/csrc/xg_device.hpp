@@ -153,6 +153,29 @@ __device__ __forceinline__ int tile_satd(int *d, int lane)
     else return s;
 }
 
+// ---------------------------------------------------------------------------
+// Packed 16-bit pairs: two samples (or two int16 values) per dword.
+// ---------------------------------------------------------------------------
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) uint32_t gdword;
+
+__device__ __forceinline__ v2s as_v2s(uint32_t v) { return __builtin_bit_cast(v2s, v); }
+__device__ __forceinline__ uint32_t as_u32(v2s v) { return __builtin_bit_cast(uint32_t, v); }
+// (lo & 0xFFFF) | (hi << 16) in one v_perm_b32: selector bytes 0-1 take lo's
+// low bytes, 4-5 hi's.
+__device__ __forceinline__ uint32_t pack16(int lo, int hi)
+{
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
+}
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c)
+{
+    return __builtin_amdgcn_sdot2(as_v2s(a), as_v2s(b), c, false);
+}
+__device__ __forceinline__ uint32_t align16(uint32_t hi, uint32_t lo, uint32_t sh)
+{
+    return __builtin_amdgcn_alignbit(hi, lo, sh);
+}
+
 // Exp-Golomb se(v) length: k = v > 0 ? 2v-1 : -2v; 2*floor(log2(k+1)) + 1.
 __device__ __forceinline__ uint32_t eg_bits(int v)
 {

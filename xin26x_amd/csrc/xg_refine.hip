@@ -104,26 +104,6 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 // last stage is 8 * 4 * 1023 < 32768); the last stage uses
 // |a + b| + |a - b| = 2 * max(|a|, |b|), so no value ever needs 17 bits.
 // ---------------------------------------------------------------------------
-typedef short v2s __attribute__((ext_vector_type(2)));
-typedef const __attribute__((address_space(1))) uint32_t gdword;
-
-__device__ __forceinline__ v2s as_v2s(uint32_t v) { return __builtin_bit_cast(v2s, v); }
-__device__ __forceinline__ uint32_t as_u32(v2s v) { return __builtin_bit_cast(uint32_t, v); }
-// (lo & 0xFFFF) | (hi << 16) in one v_perm_b32: selector bytes 0-1 take lo's
-// low bytes, 4-5 hi's.
-__device__ __forceinline__ uint32_t pack16(int lo, int hi)
-{
-    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
-}
-__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c)
-{
-    return __builtin_amdgcn_sdot2(as_v2s(a), as_v2s(b), c, false);
-}
-__device__ __forceinline__ uint32_t align16(uint32_t hi, uint32_t lo, uint32_t sh)
-{
-    return __builtin_amdgcn_alignbit(hi, lo, sh);
-}
-
 // Lane exchange of a packed pair with lane ^ M (M = 1, 2: DPP quad permutes).
 template <int M>
 __device__ __forceinline__ v2s xlane(v2s v)
@@ -714,6 +694,11 @@ struct xg_session {
     void *hStage = nullptr;            // pinned upload staging
     size_t hStageCap = 0;
 
+    // refinement prepared by refine_prepare for a caller that enqueues it itself
+    RefineArgs prepared;
+    bool preparedValid = false;
+    void *pipe = nullptr;              // motion-search pipeline state (xg_pipeline.hip)
+
     // graph of the batched stages
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -1135,6 +1120,7 @@ int32_t xg_session_destroy(xg_session *s)
     if (!s) return XG_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
     fdct_release(s);
+    pipe_release(s);
     if (s->graphExec) hipGraphExecDestroy(s->graphExec);
     if (s->graph) hipGraphDestroy(s->graph);
     for (auto e : s->evPool) hipEventDestroy(e);
@@ -1301,6 +1287,37 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
     }
     return XG_OK;
 }
+
+}  // extern "C"
+
+namespace xg {
+int32_t refine_prepare(xg_session *s, const xg_refine_cfg *cfg, const xg_picture *pred)
+{
+    s->preparedValid = false;
+    if (!validate_cfg(cfg)) return XG_ERR_ARG;
+    if (s->nblocks == 0) return XG_ERR_STATE;
+    if (pred)
+        for (int i = 0; i < s->nblocks; i++) {
+            s->preds[s->blocks[i].pic] = *pred;
+            s->predSet[s->blocks[i].pic] = true;
+        }
+    if (!validate_blocks(s, true, cfg->ctuSize)) return XG_ERR_ARG;
+    if (!build_tasks(s, cfg->ncand)) return XG_ERR_HIP;
+    if (!fill_args(s, cfg, s->prepared, true)) return XG_ERR_HIP;
+    s->lastNcand = cfg->ncand;
+    s->preparedValid = true;
+    return XG_OK;
+}
+
+bool refine_enqueue_prepared(xg_session *s)
+{
+    return s->preparedValid && enqueue_batched(s, s->prepared, false);
+}
+
+void *&session_pipe(xg_session *s) { return s->pipe; }
+}  // namespace xg
+
+extern "C" {
 
 int32_t xg_session_outputs(xg_session *s, uint32_t **candCost, xg_block_result **best)
 {

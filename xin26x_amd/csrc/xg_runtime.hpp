@@ -43,9 +43,17 @@ void unpack2d(void *dst, intptr_t dstStride, const void *src, int w, int h, size
 // Releases the forward-transform plan a session owns (xg_fdct.hip).
 void fdct_release(xg_session *s);
 
+void pipe_release(xg_session *s);
+// Refinement of the session's current blocks, validated and uploaded but not
+// launched, for callers that enqueue (or capture) it among their own kernels.
+int32_t refine_prepare(xg_session *s, const xg_refine_cfg *cfg, const xg_picture *pred);
+bool refine_enqueue_prepared(xg_session *s);
+void *&session_pipe(xg_session *s);
+
 // Session kernel timers (HIP events on the session stream), by stage:
-// 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search.
-constexpr int kTimerStages = 4;
+// 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search,
+// 4 a whole motion-search pipeline frame.
+constexpr int kTimerStages = 5;
 bool session_timer_begin(xg_session *s, int stage, int ev[2]);   // false when timing is off
 void session_timer_end(xg_session *s, int stage, const int ev[2]);
 

@@ -278,3 +278,76 @@ def residual_blocks(n=200000, seed=3, scale=12.0):
     resid = np.clip(np.rint(lap), -1023, 1023).astype(np.int16)
     desc = (lgw | (lgh << 4) | (mts << 8)).astype(np.uint32)
     return desc, offsets, resid
+
+
+# ---------------------------------------------------------------------------
+# 4K layered scene: the full motion-search pipeline
+# ---------------------------------------------------------------------------
+PAN16 = (88, 36)                   # background pan per frame, 1/16 luma pel: (+5.5, +2.25) px
+# 1/16-pel stage of the pipeline: the quarter-pel MV itself, then the rings of
+# one and two sixteenths in the order L, R, T, B, TL, TR, BL, BR
+CAND17_SIXTEENTH = [(0, 0)] + [(dx * r, dy * r) for r in (1, 2)
+                               for (dx, dy) in ((-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1))]
+
+
+def _layer(plane, tex, ox16, oy16, taps, fbits, margin):
+    """Paint texture `tex` (with `margin` extra samples around it) whose
+    top-left sits at (ox16, oy16) in 1/2^fbits sample units; clipped to the plane."""
+    H, W = plane.shape
+    th, tw = tex.shape[0] - 2 * margin, tex.shape[1] - 2 * margin
+    ix, fx = ox16 >> fbits, ox16 & ((1 << fbits) - 1)
+    iy, fy = oy16 >> fbits, oy16 & ((1 << fbits) - 1)
+    x0, y0 = max(ix, 0), max(iy, 0)
+    x1, y1 = min(ix + tw, W), min(iy + th, H)
+    if x1 <= x0 or y1 <= y0:
+        return
+    # content at plane x is tex(x - ox): interpolate tex at offset -frac
+    blk = _interp(tex, margin + (x0 - ix), margin + (y0 - iy), x1 - x0, y1 - y0, -fx, -fy, taps, fbits)
+    plane[y0:y1, x0:x1] = blk
+
+
+def layered_frames(width=3840, height=2160, seed=4, nobj=12, frames=3, max_mv=64, sides=(64, 512)):
+    """Frames t = 0..frames-1 of a layered 10-bit 4:2:0 scene: a smooth background
+    panned by (+5.5, +2.25) px per frame, and `nobj` seeded rectangles (sides in
+    `sides`, own smooth textures, velocities uniform in +/-max_mv px per frame at
+    quarter-pel precision) drawn in index order, so later ones occlude earlier
+    ones.  Returns [(Y, Cb, Cr)] as uint16 planes, oldest first."""
+    m = 16 + 2 * int(np.ceil(max(PAN16) * frames / 16.0))
+    bgY = smooth_field(height + 2 * m, width + 2 * m, seed, 10)
+    bgU = smooth_field(height // 2 + m, width // 2 + m, seed, 11)
+    bgV = smooth_field(height // 2 + m, width // 2 + m, seed, 12)
+    r = uniform01(seed, 8 * nobj, stream=13).reshape(nobj, 8)
+    objs = []
+    for k in range(nobj):
+        w = int(sides[0] + r[k, 0] * (sides[1] - sides[0])) // 4 * 4
+        h = int(sides[0] + r[k, 1] * (sides[1] - sides[0])) // 4 * 4
+        px, py = int(r[k, 2] * (width - w)) // 2 * 2, int(r[k, 3] * (height - h)) // 2 * 2
+        vx = int(np.rint((r[k, 4] * 2 - 1) * max_mv * 4)) * 4       # 1/16 pel per frame, quarter-pel steps
+        vy = int(np.rint((r[k, 5] * 2 - 1) * max_mv * 4)) * 4
+        tY = smooth_field(h + 16, w + 16, seed, 100 + k)
+        tU = smooth_field(h // 2 + 8, w // 2 + 8, seed, 200 + k)
+        tV = smooth_field(h // 2 + 8, w // 2 + 8, seed, 300 + k)
+        objs.append((px, py, vx, vy, tY, tU, tV))
+    out = []
+    for t in range(frames):
+        Y = _interp(bgY, m, m, width, height, -PAN16[0] * t, -PAN16[1] * t, 8, 4).astype(np.int64)
+        U = _interp(bgU, m // 2, m // 2, width // 2, height // 2, -PAN16[0] * t, -PAN16[1] * t, 4, 5).astype(np.int64)
+        V = _interp(bgV, m // 2, m // 2, width // 2, height // 2, -PAN16[0] * t, -PAN16[1] * t, 4, 5).astype(np.int64)
+        for (px, py, vx, vy, tY, tU, tV) in objs:
+            ox, oy = px * 16 + vx * t, py * 16 + vy * t            # 1/16 luma = 1/32 chroma units
+            _layer(Y, tY, ox, oy, 8, 4, 8)
+            _layer(U, tU, ox, oy, 4, 5, 4)
+            _layer(V, tV, ox, oy, 4, 5, 4)
+        out.append(tuple(np.ascontiguousarray(a.astype(np.uint16)) for a in (Y, U, V)))
+    return out
+
+
+def block_grid(width, height, bw=16, bh=16):
+    """Raster grid of bw x bh blocks as int64 rows (x, y, w, h, 0, 0, 0, 0)."""
+    ys, xs = np.mgrid[0:height // bh, 0:width // bw]
+    b = np.zeros((xs.size, 8), np.int64)
+    b[:, 0] = xs.ravel() * bw
+    b[:, 1] = ys.ravel() * bh
+    b[:, 2] = bw
+    b[:, 3] = bh
+    return b
