@@ -90,7 +90,6 @@ struct FdctArgs {
     int matCols[3][7];
 };
 
-constexpr int kTmpStride = 64 + 4;   // LDS row stride (int32) of TMP[k][y]
 
 __device__ __forceinline__ long pack8(const int *v)
 {
@@ -105,8 +104,13 @@ __device__ __forceinline__ long load_a(const int8_t *p)
     return *(const long *)p;      // 8-byte aligned: rows are multiples of 32, k offsets of 8
 }
 
+// HMAX: the largest block height of the launch; the LDS tile TMP[k][y] has
+// HMAX + 4 int32 per row, so launches of blocks up to 32 high hold half the
+// LDS and run twice the workgroups per CU.
+template <int HMAX>
 __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a)
 {
+    constexpr int kTmpStride = HMAX + 4;
     __shared__ int32_t tmp[32 * kTmpStride];
     const int lane = threadIdx.x;
     const int lr = lane & 15, lq = lane >> 4;
@@ -196,8 +200,8 @@ using namespace xg;
 
 namespace {
 struct FdctPlan {
-    FdctItem *dItems = nullptr;
-    int n = 0, cap = 0;
+    FdctItem *dItems = nullptr;       // blocks up to 32 high first, then the 64-high ones
+    int n = 0, nLow = 0, cap = 0;
 };
 std::mutex g_plan_mu;
 std::vector<std::pair<xg_session *, FdctPlan>> g_plans;
@@ -239,12 +243,17 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
         items[i].desc = (uint32_t)(lgW | (lgH << 4) | (hor << 8) | (ver << 12));
         items[i].pad = 0;
     }
-    // per-shape queues, largest blocks first
+    // two launches: blocks up to 32 high (small LDS tile), then 64-high blocks;
+    // per-shape queues, largest blocks first, inside each
     std::stable_sort(items.begin(), items.end(), [](const FdctItem &x, const FdctItem &y) {
+        const int hx = ((x.desc >> 4) & 15) == 6, hy = ((y.desc >> 4) & 15) == 6;
+        if (hx != hy) return hx < hy;
         const int ax = (x.desc & 15) + ((x.desc >> 4) & 15), ay = (y.desc & 15) + ((y.desc >> 4) & 15);
         if (ax != ay) return ax > ay;
         return (x.desc & 0xFF) < (y.desc & 0xFF);
     });
+    int nLow = 0;
+    while (nLow < n && ((items[nLow].desc >> 4) & 15) != 6) nLow++;
     std::lock_guard<std::mutex> g(g_plan_mu);
     FdctPlan *p = plan_for(s, true);
     if (n > p->cap) {
@@ -262,6 +271,7 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
         return XG_ERR_HIP;
     if (!XG_HIP(hipStreamSynchronize(st))) return XG_ERR_HIP;
     p->n = n;
+    p->nLow = nLow;
     return XG_OK;
 }
 
@@ -296,10 +306,20 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
             return prop.multiProcessorCount;
         return 256;
     }();
-    const int grid = std::min(p->n, cus * 16);
+    hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
     const bool timed = session_timer_begin(s, 2, ev);
-    k_fdct_mfma<<<grid, 64, 0, (hipStream_t)xg_session_stream(s)>>>(a);
+    if (p->nLow > 0) {
+        FdctArgs lo = a;
+        lo.n = p->nLow;
+        k_fdct_mfma<32><<<std::min(lo.n, cus * 32), 64, 0, st>>>(lo);
+    }
+    if (p->n > p->nLow) {
+        FdctArgs hi = a;
+        hi.items = p->dItems + p->nLow;
+        hi.n = p->n - p->nLow;
+        k_fdct_mfma<64><<<std::min(hi.n, cus * 16), 64, 0, st>>>(hi);
+    }
     if (timed) session_timer_end(s, 2, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
