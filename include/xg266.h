@@ -484,7 +484,8 @@ int32_t xg_me_pipeline_read(xg_session *s, int16_t *fullMv, int16_t *predMv, int
 
 /* Batched forward transform of residual blocks on int8 MFMA.
  * xg_fdct_plan takes HOST arrays: block i's residual is at resid + offset[i]
- * (int16, row-major, stride = width) and its coefficients go to
+ * (int16, row-major, stride = width; offset a multiple of min(8, width)
+ * samples, so rows move as whole vectors) and its coefficients go to
  * coeff + offset[i] (int32, row-major [ver freq][hor freq], stride = width);
  * desc[i] = lgW | lgH << 4 | mtsIdx << 8 (sides 2..64; mtsIdx 0 or 2..5 as
  * h26x_trans_context.h:42-51, DST-VII/DCT-VIII only for sides 4..32).  The plan

@@ -37,10 +37,15 @@ def _run(desc, offs, resid):
 
 
 def _layout(shapes):
+    """Blocks back to back, each offset rounded up to a multiple of 8 samples
+    (the batched entry's alignment rule)."""
     desc = np.array([lw | (lh << 4) | (m << 8) for (lw, lh, m) in shapes], np.uint32)
-    sizes = np.array([(1 << lw) * (1 << lh) for (lw, lh, _) in shapes], np.int64)
-    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
-    return desc, offs, int(sizes.sum())
+    offs, o = [], 0
+    for (lw, lh, _) in shapes:
+        o = (o + 7) // 8 * 8
+        offs.append(o)
+        o += (1 << lw) * (1 << lh)
+    return desc, np.array(offs, np.int64), o
 
 
 def test_every_shape_and_type(oracle):
@@ -65,6 +70,11 @@ def test_every_shape_and_type(oracle):
             resid[o:o + n] = np.where(np.arange(n) % 2, 1023, -1023)
     exp = _oracle_blocks(oracle, desc, offs, resid)
     got = _run(desc, offs, resid)
+    # samples between blocks belong to nobody: the transform leaves them as they were
+    covered = np.zeros(total, bool)
+    for d, o in zip(desc.tolist(), offs.tolist()):
+        covered[o:o + (1 << (d & 15)) * (1 << ((d >> 4) & 15))] = True
+    exp[~covered] = -12345
     bad = np.nonzero(got != exp)[0]
     assert bad.size == 0, f"{bad.size} mismatches, first at {bad[0]}"
 
@@ -87,6 +97,10 @@ def test_plan_rejects_bad_descriptors():
             desc = np.array([d], np.uint32)
             offs = np.zeros(1, np.int64)
             assert lib.xg_fdct_plan(s, ptr(offs), ptr(desc), 1) == capi.XG_ERR_ARG
+        # rows move as whole vectors: an 8-wide block must start on a multiple of 8 samples
+        desc = np.array([3 | (3 << 4)], np.uint32)
+        assert lib.xg_fdct_plan(s, ptr(np.array([4], np.int64)), ptr(desc), 1) == capi.XG_ERR_ARG
+        assert lib.xg_fdct_plan(s, ptr(np.array([8], np.int64)), ptr(desc), 1) == 0
     finally:
         lib.xg_clear_error()
         lib.xg_session_destroy(s)

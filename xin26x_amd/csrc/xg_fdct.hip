@@ -13,6 +13,11 @@
 //   t = 16384*l2 + 128*l1 + l0, l0,l1 in [0,127], |l2| <= 3      (3 MFMAs)
 // |partial| <= 91*127*64 = 739,648; |stage-2 sum| <= 91*46.5K*64 < 2^31.
 //
+// Blocks of one descriptor are processed in groups of up to 4 (as many as
+// fill a 16-wide MFMA tile: 16 / min(H, Wk)): stage 1 stacks the rows of all
+// blocks of the group as the N dimension, stage 2 stacks their kept columns,
+// so a 4x4 block costs a quarter of a tile instead of a whole one.
+//
 // Operand layout of v_mfma_i32_16x16x32_i8 (one wavefront): lane l supplies
 // A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16] as 8 packed bytes and
 // receives D[4*(l/16) + r][l%16], r = 0..3.
@@ -74,16 +79,27 @@ static bool mats_init()
     return g_mat_ok;
 }
 
-struct FdctItem {
-    long long off;      // element offset of the block in resid / coeff
-    uint32_t desc;      // lgW | lgH << 4 | horTx << 8 | verTx << 12
-    uint32_t pad;
+constexpr int kGroupMax = 4;
+
+struct FdctGroup {
+    uint32_t desc;                // lgW | lgH << 4 | horTx << 8 | verTx << 12
+    int32_t count;                // blocks in the group, 1..kGroupMax
+    long long off[kGroupMax];     // element offsets of the blocks in resid / coeff
 };
+
+// Blocks per group for a descriptor: enough to fill the 16-wide N dimension of
+// both stages.
+__host__ __device__ inline int group_cap(int H, int Wk)
+{
+    const int m = H < Wk ? H : Wk;
+    const int c = m >= 16 ? 1 : 16 / m;
+    return c < kGroupMax ? c : kGroupMax;
+}
 
 struct FdctArgs {
     const int16_t *resid;
     int32_t *coeff;
-    const FdctItem *items;
+    const FdctGroup *groups;
     int n;
     const int8_t *mats;
     int matOff[3][7];
@@ -104,68 +120,96 @@ __device__ __forceinline__ long load_a(const int8_t *p)
     return *(const long *)p;      // 8-byte aligned: rows are multiples of 32, k offsets of 8
 }
 
-// HMAX: the largest block height of the launch; the LDS tile TMP[k][y] has
-// HMAX + 4 int32 per row, so launches of blocks up to 32 high hold half the
-// LDS and run twice the workgroups per CU.
+// HMAX: the largest block height of the launch; the LDS tile holds TMP[b][k][y]
+// with HMAX + 4 int32 per (b, k) row, so launches of blocks up to 32 high hold
+// half the LDS and run twice the workgroups per CU.
 template <int HMAX>
 __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a)
 {
-    constexpr int kTmpStride = HMAX + 4;
-    __shared__ int32_t tmp[32 * kTmpStride];
+    __shared__ int32_t tmp[32 * (HMAX + 4)];
     const int lane = threadIdx.x;
     const int lr = lane & 15, lq = lane >> 4;
-    for (int i = blockIdx.x; i < a.n; i += gridDim.x) {
-        const FdctItem it = a.items[i];
-        const int lgW = it.desc & 15, lgH = (it.desc >> 4) & 15;
-        const int hor = (it.desc >> 8) & 15, ver = (it.desc >> 12) & 15;
+    for (int gi = blockIdx.x; gi < a.n; gi += gridDim.x) {
+        const FdctGroup &G = a.groups[gi];
+        const uint32_t desc = G.desc;
+        const int G_n = G.count;
+        const int lgW = desc & 15, lgH = (desc >> 4) & 15;
+        const int hor = (desc >> 8) & 15, ver = (desc >> 12) & 15;
         const int W = 1 << lgW, H = 1 << lgH;
         const int Wk = tr_kept(hor, W), Hk = tr_kept(ver, H);
+        const int lgWk = 31 - __builtin_clz(Wk);
+        const int cap = group_cap(H, Wk);
+        const int TS = H + 4;                       // LDS stride of a (b, k) row of TMP
         const int s1 = lgW + 1, s2 = lgH + 6;
-        const int16_t *R = a.resid + it.off;
-        int32_t *O = a.coeff + it.off;
         const int8_t *TW = a.mats + a.matOff[hor][lgW];
         const int8_t *TH = a.mats + a.matOff[ver][lgH];
         const int cw = a.matCols[hor][lgW], ch = a.matCols[ver][lgH];
 
-        // ---- stage 1: TMP[k][y] over M = Wk, N = H, K = W ----
+        // ---- stage 1: TMP[b][k][y] over M = Wk, N = (b, y) rows of the group, K = W ----
+        const int rows1 = cap * H;
+        const int rnd1 = 1 << (s1 - 1);
         for (int m0 = 0; m0 < Wk; m0 += 16)
-            for (int n0 = 0; n0 < H; n0 += 16) {
+            for (int n0 = 0; n0 < rows1; n0 += 16) {
                 v4i dlo = {0, 0, 0, 0}, dhi = {0, 0, 0, 0};
-                const int y = n0 + lr;
+                const int row = n0 + lr, b = row >> lgH, y = row & (H - 1);
+                const bool rowOk = b < G_n;
+                const int16_t *R = a.resid + (rowOk ? G.off[b] : 0) + ((long)y << lgW);
                 for (int k0 = 0; k0 < W; k0 += 32) {
                     const long av = load_a(TW + (m0 + lr) * cw + k0 + 8 * lq);
-                    int lo[8], hi[8];
-                    const int nb = k0 + 8 * lq;
-#pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const int r = (y < H && nb + j < W) ? (int)R[y * W + nb + j] : 0;
-                        lo[j] = r & 127;
-                        hi[j] = r >> 7;
+                    const int x = k0 + 8 * lq;
+                    uint4 v = make_uint4(0, 0, 0, 0);
+                    if (rowOk && x < W) {
+                        if (W >= 8) {
+                            v = *reinterpret_cast<const uint4 *>(R + x);
+                        } else if (W == 4) {
+                            const uint2 h2 = *reinterpret_cast<const uint2 *>(R + x);
+                            v.x = h2.x;
+                            v.y = h2.y;
+                        } else {
+                            v.x = *reinterpret_cast<const uint32_t *>(R + x);
+                        }
                     }
-                    dlo = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, pack8(lo), dlo, 0, 0, 0);
-                    dhi = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, pack8(hi), dhi, 0, 0, 0);
-                }
-                const int rnd = 1 << (s1 - 1);
+                    // int8 limbs: lo = r & 127, hi = r >> 7, bytes in sample order
+                    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+                    uint32_t m[4], hh[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int k = m0 + 4 * lq + r;
-                    if (k < Wk && y < H) tmp[k * kTmpStride + y] = (dlo[r] + (dhi[r] << 7) + rnd) >> s1;
+                    for (int q = 0; q < 4; q++) {
+                        m[q] = w[q] & 0x007F007Fu;
+                        hh[q] = as_u32(as_v2s(w[q]) >> (v2s){7, 7});
+                    }
+                    const long blo = (long)(((unsigned long long)__builtin_amdgcn_perm(m[3], m[2], 0x06040200u) << 32) |
+                                            __builtin_amdgcn_perm(m[1], m[0], 0x06040200u));
+                    const long bhi = (long)(((unsigned long long)__builtin_amdgcn_perm(hh[3], hh[2], 0x06040200u) << 32) |
+                                            __builtin_amdgcn_perm(hh[1], hh[0], 0x06040200u));
+                    dlo = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, blo, dlo, 0, 0, 0);
+                    dhi = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, bhi, dhi, 0, 0, 0);
+                }
+                if (rowOk) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int k = m0 + 4 * lq + r;
+                        if (k < Wk) tmp[((b << lgWk) + k) * TS + y] = (dlo[r] + (dhi[r] << 7) + rnd1) >> s1;
+                    }
                 }
             }
         __syncthreads();
 
-        // ---- stage 2: OUT[j][k] over M = Hk, N = Wk, K = H ----
+        // ---- stage 2: OUT[b][j][k] over M = Hk, N = (b, k) columns of the group, K = H ----
+        const int cols2 = cap * Wk;
+        const int rnd2 = 1 << (s2 - 1);
         for (int m0 = 0; m0 < Hk; m0 += 16)
-            for (int n0 = 0; n0 < Wk; n0 += 16) {
+            for (int n0 = 0; n0 < cols2; n0 += 16) {
                 v4i d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
-                const int k = n0 + lr;
+                const int col = n0 + lr, b = col >> lgWk, k = col & (Wk - 1);
+                const bool colOk = b < G_n;
+                const int32_t *trow = tmp + ((b << lgWk) + k) * TS;
                 for (int k0 = 0; k0 < H; k0 += 32) {
                     const long av = load_a(TH + (m0 + lr) * ch + k0 + 8 * lq);
                     int l0[8], l1[8], l2[8];
                     const int yb = k0 + 8 * lq;
 #pragma unroll
                     for (int j = 0; j < 8; j++) {
-                        const int t = (k < Wk && yb + j < H) ? tmp[k * kTmpStride + yb + j] : 0;
+                        const int t = (colOk && yb + j < H) ? trow[yb + j] : 0;
                         l0[j] = t & 127;
                         l1[j] = (t >> 7) & 127;
                         l2[j] = t >> 14;
@@ -174,18 +218,24 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a)
                     d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, pack8(l1), d1, 0, 0, 0);
                     d2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, pack8(l2), d2, 0, 0, 0);
                 }
-                const int rnd = 1 << (s2 - 1);
+                if (colOk) {
+                    int32_t *O = a.coeff + G.off[b];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int j = m0 + 4 * lq + r;
-                    if (j < Hk && k < Wk) O[j * W + k] = (d0[r] + (d1[r] << 7) + (d2[r] << 14) + rnd) >> s2;
+                    for (int r = 0; r < 4; r++) {
+                        const int j = m0 + 4 * lq + r;
+                        if (j < Hk) O[j * W + k] = (d0[r] + (d1[r] << 7) + (d2[r] << 14) + rnd2) >> s2;
+                    }
                 }
             }
-        // zero-out region
+        // zero-out region (16-byte stores; kept widths are multiples of 4)
         if (Wk < W || Hk < H)
-            for (int idx = lane; idx < W * H; idx += 64) {
-                const int j = idx >> lgW, k = idx & (W - 1);
-                if (j >= Hk || k >= Wk) O[idx] = 0;
+            for (int b = 0; b < G_n; b++) {
+                int32_t *O = a.coeff + G.off[b];
+                const int q4 = W >> 2;
+                for (int idx = lane; idx < (W * H) >> 2; idx += 64) {
+                    const int j = idx / q4, k = (idx - j * q4) << 2;
+                    if (j >= Hk || k >= Wk) *reinterpret_cast<int4 *>(O + j * W + k) = make_int4(0, 0, 0, 0);
+                }
             }
         __syncthreads();
     }
@@ -200,8 +250,8 @@ using namespace xg;
 
 namespace {
 struct FdctPlan {
-    FdctItem *dItems = nullptr;       // blocks up to 32 high first, then the 64-high ones
-    int n = 0, nLow = 0, cap = 0;
+    FdctGroup *dGroups = nullptr;     // groups of blocks up to 32 high first, then the 64-high ones
+    int n = 0, nLow = 0, cap = 0;     // groups
 };
 std::mutex g_plan_mu;
 std::vector<std::pair<xg_session *, FdctPlan>> g_plans;
@@ -227,7 +277,11 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
     if (!mats_init()) return XG_ERR_HIP;
     static const int8_t mts[6][2] = {{kDCT2, kDCT2}, {kSKIP, kSKIP}, {kDST7, kDST7},
                                      {kDCT8, kDST7}, {kDST7, kDCT8}, {kDCT8, kDCT8}};
-    std::vector<FdctItem> items((size_t)n);
+    struct Item {
+        long long off;
+        uint32_t desc;
+    };
+    std::vector<Item> items((size_t)n);
     for (int i = 0; i < n; i++) {
         const int lgW = desc[i] & 15, lgH = (desc[i] >> 4) & 15, m = (desc[i] >> 8) & 15;
         if (lgW < 1 || lgW > 6 || lgH < 1 || lgH > 6 || m > 5 || m == 1) {
@@ -239,37 +293,54 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
             set_error(XG_ERR_ARG, "xg_fdct_plan: block %d: DST-VII/DCT-VIII need sides 4..32", i);
             return XG_ERR_ARG;
         }
+        if (offset[i] % std::min(8, 1 << lgW)) {
+            set_error(XG_ERR_ARG, "xg_fdct_plan: block %d: offset must be a multiple of min(8, width) samples", i);
+            return XG_ERR_ARG;
+        }
         items[i].off = offset[i];
         items[i].desc = (uint32_t)(lgW | (lgH << 4) | (hor << 8) | (ver << 12));
-        items[i].pad = 0;
     }
     // two launches: blocks up to 32 high (small LDS tile), then 64-high blocks;
-    // per-shape queues, largest blocks first, inside each
-    std::stable_sort(items.begin(), items.end(), [](const FdctItem &x, const FdctItem &y) {
+    // inside each, descriptors with the largest blocks first, then grouped
+    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
         const int hx = ((x.desc >> 4) & 15) == 6, hy = ((y.desc >> 4) & 15) == 6;
         if (hx != hy) return hx < hy;
         const int ax = (x.desc & 15) + ((x.desc >> 4) & 15), ay = (y.desc & 15) + ((y.desc >> 4) & 15);
         if (ax != ay) return ax > ay;
-        return (x.desc & 0xFF) < (y.desc & 0xFF);
+        return x.desc < y.desc;
     });
+    std::vector<FdctGroup> groups;
     int nLow = 0;
-    while (nLow < n && ((items[nLow].desc >> 4) & 15) != 6) nLow++;
+    for (int i = 0; i < n;) {
+        const uint32_t d = items[i].desc;
+        const int lgW = d & 15, lgH = (d >> 4) & 15;
+        const int cap = group_cap(1 << lgH, tr_kept((d >> 8) & 15, 1 << lgW));
+        FdctGroup g;
+        memset(&g, 0, sizeof(g));
+        g.desc = d;
+        while (i < n && items[i].desc == d && g.count < cap) g.off[g.count++] = items[i++].off;
+        groups.push_back(g);
+        if (lgH != 6) nLow = (int)groups.size();
+    }
+    const int ng = (int)groups.size();
     std::lock_guard<std::mutex> g(g_plan_mu);
     FdctPlan *p = plan_for(s, true);
-    if (n > p->cap) {
-        if (p->dItems) {
+    if (ng > p->cap) {
+        if (p->dGroups) {
             hipStreamSynchronize((hipStream_t)xg_session_stream(s));
-            hipFree(p->dItems);
+            hipFree(p->dGroups);
         }
-        p->dItems = nullptr;
+        p->dGroups = nullptr;
         p->cap = 0;
-        if (!XG_HIP(hipMalloc((void **)&p->dItems, sizeof(FdctItem) * (size_t)std::max(n, 1)))) return XG_ERR_NOMEM;
-        p->cap = std::max(n, 1);
+        if (!XG_HIP(hipMalloc((void **)&p->dGroups, sizeof(FdctGroup) * (size_t)std::max(ng, 1)))) return XG_ERR_NOMEM;
+        p->cap = std::max(ng, 1);
     }
     hipStream_t st = (hipStream_t)xg_session_stream(s);
-    if (n && !XG_HIP(hipMemcpyAsync(p->dItems, items.data(), sizeof(FdctItem) * (size_t)n, hipMemcpyHostToDevice, st)))
+    if (ng && !XG_HIP(hipMemcpyAsync(p->dGroups, groups.data(), sizeof(FdctGroup) * (size_t)ng, hipMemcpyHostToDevice,
+                                     st)))
         return XG_ERR_HIP;
     if (!XG_HIP(hipStreamSynchronize(st))) return XG_ERR_HIP;
+    n = ng;
     p->n = n;
     p->nLow = nLow;
     return XG_OK;
@@ -294,7 +365,7 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
     FdctArgs a;
     a.resid = resid;
     a.coeff = coeff;
-    a.items = p->dItems;
+    a.groups = p->dGroups;
     a.n = p->n;
     a.mats = g_mats;
     memcpy(a.matOff, g_mi.off, sizeof(a.matOff));
@@ -316,7 +387,7 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
     }
     if (p->n > p->nLow) {
         FdctArgs hi = a;
-        hi.items = p->dItems + p->nLow;
+        hi.groups = p->dGroups + p->nLow;
         hi.n = p->n - p->nLow;
         k_fdct_mfma<64><<<std::min(hi.n, cus * 16), 64, 0, st>>>(hi);
     }
@@ -331,7 +402,7 @@ void xg::fdct_release(xg_session *s)
     std::lock_guard<std::mutex> g(g_plan_mu);
     for (size_t i = 0; i < g_plans.size(); i++)
         if (g_plans[i].first == s) {
-            if (g_plans[i].second.dItems) hipFree(g_plans[i].second.dItems);
+            if (g_plans[i].second.dGroups) hipFree(g_plans[i].second.dGroups);
             g_plans.erase(g_plans.begin() + (long)i);
             return;
         }
