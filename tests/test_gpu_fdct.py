@@ -36,13 +36,14 @@ def _run(desc, offs, resid):
         lib.xg_session_destroy(s)
 
 
-def _layout(shapes):
-    """Blocks back to back, each offset rounded up to a multiple of 8 samples
-    (the batched entry's alignment rule)."""
+def _layout(shapes, tight=False):
+    """Blocks back to back, each offset rounded up to a multiple of 8 samples,
+    or with `tight` to the least the batched entry accepts: min(8, width)."""
     desc = np.array([lw | (lh << 4) | (m << 8) for (lw, lh, m) in shapes], np.uint32)
     offs, o = [], 0
     for (lw, lh, _) in shapes:
-        o = (o + 7) // 8 * 8
+        a = min(8, 1 << lw) if tight else 8
+        o = (o + a - 1) // a * a
         offs.append(o)
         o += (1 << lw) * (1 << lh)
     return desc, np.array(offs, np.int64), o
@@ -71,6 +72,25 @@ def test_every_shape_and_type(oracle):
     exp = _oracle_blocks(oracle, desc, offs, resid)
     got = _run(desc, offs, resid)
     # samples between blocks belong to nobody: the transform leaves them as they were
+    covered = np.zeros(total, bool)
+    for d, o in zip(desc.tolist(), offs.tolist()):
+        covered[o:o + (1 << (d & 15)) * (1 << ((d >> 4) & 15))] = True
+    exp[~covered] = -12345
+    bad = np.nonzero(got != exp)[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[0]}"
+
+
+def test_least_aligned_offsets(oracle):
+    """Narrow blocks at offsets that are multiples of min(8, width) only, mixed
+    with wide ones; many blocks per shape so groups fill up."""
+    rng = np.random.default_rng(43)
+    kinds = [(1, 1, 0), (1, 6, 0), (2, 2, 2), (2, 6, 0), (1, 3, 0), (6, 1, 0), (3, 6, 0), (5, 5, 3), (6, 6, 0)]
+    shapes = [kinds[k] for k in rng.integers(0, len(kinds), 700)]
+    desc, offs, total = _layout(shapes, tight=True)
+    assert np.any(offs % 8 != 0)
+    resid = rng.integers(-1023, 1024, total).astype(np.int16)
+    exp = _oracle_blocks(oracle, desc, offs, resid)
+    got = _run(desc, offs, resid)
     covered = np.zeros(total, bool)
     for d, o in zip(desc.tolist(), offs.tolist()):
         covered[o:o + (1 << (d & 15)) * (1 << ((d >> 4) & 15))] = True

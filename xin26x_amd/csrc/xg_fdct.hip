@@ -79,27 +79,44 @@ static bool mats_init()
     return g_mat_ok;
 }
 
-constexpr int kGroupMax = 4;
+constexpr int kGroupMax = 64;     // blocks per group: one relative offset per lane
 
-struct FdctGroup {
-    uint32_t desc;                // lgW | lgH << 4 | horTx << 8 | verTx << 12
-    int32_t count;                // blocks in the group, 1..kGroupMax
-    long long off[kGroupMax];     // element offsets of the blocks in resid / coeff
+// LDS of a launch whose blocks are at most HMAX high: the group's residual
+// (kStage samples) and its stage-1 output as three int8 limb planes of kPlane
+// bytes each.  A limb plane holds one row of `tmp_stride(H)` bytes per kept
+// column (b, k), the column's H values along the row.
+template <int HMAX>
+struct FdctLds {
+    static constexpr int kStage = 64 * HMAX;
+    static constexpr int kPlane = HMAX == 64 ? 32 * 72 : 2048;
 };
 
-// Blocks per group for a descriptor: enough to fill the 16-wide N dimension of
-// both stages.
-__host__ __device__ inline int group_cap(int H, int Wk)
+__host__ __device__ inline int tmp_stride(int H)
 {
-    const int m = H < Wk ? H : Wk;
-    const int c = m >= 16 ? 1 : 16 / m;
+    return H <= 8 ? 8 : H <= 16 ? 16 : H + 8;   // 8-byte aligned; long rows padded against bank conflicts
+}
+
+struct FdctGroup {
+    long long base;     // element offset the group's relative block offsets start from
+    uint32_t desc;      // lgW | lgH << 4 | horTx << 8 | verTx << 12
+    int32_t count;      // blocks, 1..kGroupMax, all of this descriptor
+    int32_t first;      // index of the group's first relative offset
+    int32_t pad;
+};
+
+// Blocks of one descriptor per group: as many as the launch's LDS holds (the
+// residual W*H int16 and Wk limb rows per block).
+inline int group_cap(int W, int H, int Wk, int stage, int plane)
+{
+    int c = stage / (W * H);
+    const int t = plane / (Wk * tmp_stride(H));
+    if (t < c) c = t;
     return c < kGroupMax ? c : kGroupMax;
 }
 
 struct FdctArgs {
     const int16_t *resid;
     int32_t *coeff;
-    const FdctGroup *groups;
     int n;
     const int8_t *mats;
     int matOff[3][7];
@@ -107,12 +124,12 @@ struct FdctArgs {
 };
 
 
-__device__ __forceinline__ long pack8(const int *v)
+// low bytes of four int32, in order, as one dword
+__device__ __forceinline__ uint32_t bytes4(int a, int b, int c, int d)
 {
-    unsigned long long r = 0;
-#pragma unroll
-    for (int j = 0; j < 8; j++) r |= (unsigned long long)(uint8_t)(int8_t)v[j] << (8 * j);
-    return (long)r;
+    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
 __device__ __forceinline__ long load_a(const int8_t *p)
@@ -120,123 +137,259 @@ __device__ __forceinline__ long load_a(const int8_t *p)
     return *(const long *)p;      // 8-byte aligned: rows are multiples of 32, k offsets of 8
 }
 
-// HMAX: the largest block height of the launch; the LDS tile holds TMP[b][k][y]
-// with HMAX + 4 int32 per (b, k) row, so launches of blocks up to 32 high hold
-// half the LDS and run twice the workgroups per CU.
-template <int HMAX>
-__global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a)
+// Residual chunks of a group: min(8, W) samples per lane and load, so every
+// load is aligned (block offsets are multiples of min(8, W)); chunk c of the
+// group is sample c * cs of the LDS copy, blocks back to back.  Lane b holds
+// block b's offset relative to the group base.  Lanes past the group re-read
+// its last chunk: every load is issued and none waits for another.
+template <typename V, int N>
+__device__ __forceinline__ void stage_group(V (&c)[N], const int16_t *base, int myRel, int lgCpb, int nch, int cs,
+                                            int lane)
 {
-    __shared__ int32_t tmp[32 * (HMAX + 4)];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int ch = min(lane + 64 * i, nch - 1);
+        const int o = __shfl(myRel, ch >> lgCpb);
+        c[i] = *reinterpret_cast<const V *>(base + o + (ch & ((1 << lgCpb) - 1)) * cs);
+    }
+}
+
+// every slot is written (past the group: copies of its last chunk), so the
+// loads stay in one straight run ahead of the stores
+template <typename V, int N>
+__device__ __forceinline__ void stage_store(const V (&c)[N], void *lds, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < N; i++) reinterpret_cast<V *>(lds)[lane + 64 * i] = c[i];
+}
+
+// Zeroes the coefficients outside the kept Hk x Wk corner of every block of a
+// group with V-sized stores (V = int4 for widths >= 4; int2 for 2-wide blocks,
+// whose offsets are only 8-byte aligned).  The zero rows Hk..H-1 of a block are
+// one contiguous run; the columns right of Wk exist only for widths 32 and 64,
+// where W - Wk >= 16.  All counts are powers of two.
+template <typename V>
+__device__ __forceinline__ void zero_out(int32_t *Og, int myRel, int G_n, int W, int H, int Wk, int Hk, int lane)
+{
+    constexpr int g = sizeof(V) / 4;
+    const int tail = (W * (H - Hk)) / g, side = (W - Wk) / g;
+    const int lgSide = side ? 31 - __builtin_clz(side) : 0;
+    V z;
+    memset(&z, 0, sizeof(z));
+    for (int b = 0; b < G_n; b++) {
+        int32_t *O = Og + __builtin_amdgcn_readlane(myRel, b);
+        for (int e = lane; e < tail; e += 64) *reinterpret_cast<V *>(O + Hk * W + g * e) = z;
+        for (int e = lane; e < (Hk << lgSide) && side; e += 64) {
+            const int j = e >> lgSide;
+            *reinterpret_cast<V *>(O + j * W + Wk + g * (e & (side - 1))) = z;
+        }
+    }
+}
+
+// int8 limbs of 8 samples (lo = r & 127, hi = r >> 7), bytes in sample order
+__device__ __forceinline__ void limbs8(uint4 v, long &blo, long &bhi)
+{
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t m[4], hh[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        m[q] = w[q] & 0x007F007Fu;
+        hh[q] = as_u32(as_v2s(w[q]) >> (v2s){7, 7});
+    }
+    blo = (long)(((unsigned long long)__builtin_amdgcn_perm(m[3], m[2], 0x06040200u) << 32) |
+                 __builtin_amdgcn_perm(m[1], m[0], 0x06040200u));
+    bhi = (long)(((unsigned long long)__builtin_amdgcn_perm(hh[3], hh[2], 0x06040200u) << 32) |
+                 __builtin_amdgcn_perm(hh[1], hh[0], 0x06040200u));
+}
+
+// One workgroup = one wave; it walks groups of same-descriptor blocks.  A group
+// issues all its global loads (residual chunks and basis fragments) before its
+// first wait and fills the LDS stage, so one memory latency is exposed per
+// group of up to 64 blocks.  Both stages put the data in the A operand and the
+// basis in B, so D comes out transposed: stage 1 leaves four consecutive rows
+// y of one column k per lane (packed straight into the limb planes), stage 2
+// four consecutive columns k of one output row j (one 16-byte store).
+template <int HMAX>
+__global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *__restrict__ groups,
+                                                   const int32_t *__restrict__ rel)
+{
+    constexpr int NC = FdctLds<HMAX>::kStage / 512;   // 16-byte chunks per lane
+    constexpr int PL = FdctLds<HMAX>::kPlane;
+    __shared__ uint4 stg[64 * NC];
+    __shared__ uint64_t limb[3 * PL / 8];
+    const int16_t *S = reinterpret_cast<const int16_t *>(stg);
+    uint8_t *P0 = reinterpret_cast<uint8_t *>(limb), *P1 = P0 + PL, *P2 = P1 + PL;
     const int lane = threadIdx.x;
     const int lr = lane & 15, lq = lane >> 4;
     for (int gi = blockIdx.x; gi < a.n; gi += gridDim.x) {
-        const FdctGroup &G = a.groups[gi];
-        const uint32_t desc = G.desc;
+        const FdctGroup G = groups[gi];
         const int G_n = G.count;
+        const int myRel = rel[G.first + min(lane, G_n - 1)];
+        const uint32_t desc = G.desc;
         const int lgW = desc & 15, lgH = (desc >> 4) & 15;
         const int hor = (desc >> 8) & 15, ver = (desc >> 12) & 15;
         const int W = 1 << lgW, H = 1 << lgH;
         const int Wk = tr_kept(hor, W), Hk = tr_kept(ver, H);
         const int lgWk = 31 - __builtin_clz(Wk);
-        const int cap = group_cap(H, Wk);
-        const int TS = H + 4;                       // LDS stride of a (b, k) row of TMP
+        const int TS = tmp_stride(H);
         const int s1 = lgW + 1, s2 = lgH + 6;
         const int8_t *TW = a.mats + a.matOff[hor][lgW];
         const int8_t *TH = a.mats + a.matOff[ver][lgH];
         const int cw = a.matCols[hor][lgW], ch = a.matCols[ver][lgH];
+        const int16_t *Rg = a.resid + G.base;
+        int32_t *Og = a.coeff + G.base;
 
-        // ---- stage 1: TMP[b][k][y] over M = Wk, N = (b, y) rows of the group, K = W ----
-        const int rows1 = cap * H;
-        const int rnd1 = 1 << (s1 - 1);
-        for (int m0 = 0; m0 < Wk; m0 += 16)
-            for (int n0 = 0; n0 < rows1; n0 += 16) {
-                v4i dlo = {0, 0, 0, 0}, dhi = {0, 0, 0, 0};
-                const int row = n0 + lr, b = row >> lgH, y = row & (H - 1);
-                const bool rowOk = b < G_n;
-                const int16_t *R = a.resid + (rowOk ? G.off[b] : 0) + ((long)y << lgW);
-                for (int k0 = 0; k0 < W; k0 += 32) {
-                    const long av = load_a(TW + (m0 + lr) * cw + k0 + 8 * lq);
-                    const int x = k0 + 8 * lq;
-                    uint4 v = make_uint4(0, 0, 0, 0);
-                    if (rowOk && x < W) {
-                        if (W >= 8) {
-                            v = *reinterpret_cast<const uint4 *>(R + x);
-                        } else if (W == 4) {
-                            const uint2 h2 = *reinterpret_cast<const uint2 *>(R + x);
-                            v.x = h2.x;
-                            v.y = h2.y;
-                        } else {
-                            v.x = *reinterpret_cast<const uint32_t *>(R + x);
-                        }
-                    }
-                    // int8 limbs: lo = r & 127, hi = r >> 7, bytes in sample order
-                    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-                    uint32_t m[4], hh[4];
+        // ---- loads: basis fragments (B operands of both stages) and the residual ----
+        // unused fragments re-read fragment (0, 0): the loads are unconditional,
+        // so the waits below count them exactly
+        long a1[2][2], a2[2][2];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        m[q] = w[q] & 0x007F007Fu;
-                        hh[q] = as_u32(as_v2s(w[q]) >> (v2s){7, 7});
-                    }
-                    const long blo = (long)(((unsigned long long)__builtin_amdgcn_perm(m[3], m[2], 0x06040200u) << 32) |
-                                            __builtin_amdgcn_perm(m[1], m[0], 0x06040200u));
-                    const long bhi = (long)(((unsigned long long)__builtin_amdgcn_perm(hh[3], hh[2], 0x06040200u) << 32) |
-                                            __builtin_amdgcn_perm(hh[1], hh[0], 0x06040200u));
-                    dlo = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, blo, dlo, 0, 0, 0);
-                    dhi = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, bhi, dhi, 0, 0, 0);
-                }
-                if (rowOk) {
+        for (int mi = 0; mi < 2; mi++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int k = m0 + 4 * lq + r;
-                        if (k < Wk) tmp[((b << lgWk) + k) * TS + y] = (dlo[r] + (dhi[r] << 7) + rnd1) >> s1;
-                    }
-                }
+            for (int ki = 0; ki < 2; ki++) {
+                const bool u1 = 16 * mi < Wk && 32 * ki < W, u2 = 16 * mi < Hk && 32 * ki < H;
+                a1[mi][ki] = load_a(TW + (u1 ? (16 * mi + lr) * cw + 32 * ki : lr * cw) + 8 * lq);
+                a2[mi][ki] = load_a(TH + (u2 ? (16 * mi + lr) * ch + 32 * ki : lr * ch) + 8 * lq);
             }
+        if (W >= 8) {
+            const int lgCpb = lgW + lgH - 3, nch = G_n << lgCpb;
+            uint4 c[NC];
+            stage_group(c, Rg, myRel, lgCpb, nch, 8, lane);
+            stage_store(c, stg, lane);
+        } else if (W == 4) {
+            const int lgCpb = lgH, nch = G_n << lgCpb;
+            uint2 c[2 * NC];
+            stage_group(c, Rg, myRel, lgCpb, nch, 4, lane);
+            stage_store(c, stg, lane);
+        } else {
+            const int lgCpb = lgH, nch = G_n << lgCpb;
+            uint32_t c[4 * NC];
+            stage_group(c, Rg, myRel, lgCpb, nch, 2, lane);
+            stage_store(c, stg, lane);
+        }
         __syncthreads();
 
-        // ---- stage 2: OUT[b][j][k] over M = Hk, N = (b, k) columns of the group, K = H ----
-        const int cols2 = cap * Wk;
+        // ---- stage 1: M = rows (b, y) of the group, N = k < Wk, K = x < W ----
+        const int rows1 = G_n * H;
+        const int rnd1 = 1 << (s1 - 1);
+        for (int n0 = 0; n0 < rows1; n0 += 16) {
+            const int row = n0 + lr;
+            const bool rowOk = row < rows1;
+            const int16_t *R = S + (rowOk ? row << lgW : 0);     // blocks are back to back in the stage
+            long blo[2], bhi[2];
+#pragma unroll
+            for (int ki = 0; ki < 2; ki++) {
+                uint4 v = make_uint4(0, 0, 0, 0);
+                const int x = 32 * ki + 8 * lq;
+                if (x < W) {
+                    if (W >= 8) {
+                        v = *reinterpret_cast<const uint4 *>(R + x);
+                    } else if (W == 4) {
+                        const uint2 h2 = *reinterpret_cast<const uint2 *>(R);
+                        v.x = h2.x;
+                        v.y = h2.y;
+                    } else {
+                        v.x = *reinterpret_cast<const uint32_t *>(R);
+                    }
+                }
+                limbs8(v, blo[ki], bhi[ki]);
+            }
+            // this lane's D rows: n0 + 4 lq + r, r = 0..3
+            const int orow = n0 + 4 * lq, ob = orow >> lgH, oy = orow & (H - 1);
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++) {
+                if (16 * mi >= Wk) break;
+                v4i dlo = {0, 0, 0, 0}, dhi = {0, 0, 0, 0};
+#pragma unroll
+                for (int ki = 0; ki < 2; ki++) {
+                    if (32 * ki >= W) break;
+                    dlo = __builtin_amdgcn_mfma_i32_16x16x32_i8(blo[ki], a1[mi][ki], dlo, 0, 0, 0);
+                    dhi = __builtin_amdgcn_mfma_i32_16x16x32_i8(bhi[ki], a1[mi][ki], dhi, 0, 0, 0);
+                }
+                int t[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) t[r] = (dlo[r] + (dhi[r] << 7) + rnd1) >> s1;
+                const uint32_t L0 = bytes4(t[0], t[1], t[2], t[3]) & 0x7F7F7F7Fu;
+                const uint32_t L1 = bytes4(t[0] >> 7, t[1] >> 7, t[2] >> 7, t[3] >> 7) & 0x7F7F7F7Fu;
+                const uint32_t L2 = bytes4(t[0] >> 14, t[1] >> 14, t[2] >> 14, t[3] >> 14);
+                const int k = 16 * mi + lr;
+                if (k < Wk && ob < G_n) {
+                    const int o = ((ob << lgWk) + k) * TS + oy;
+                    if (H >= 4) {
+                        *reinterpret_cast<uint32_t *>(P0 + o) = L0;
+                        *reinterpret_cast<uint32_t *>(P1 + o) = L1;
+                        *reinterpret_cast<uint32_t *>(P2 + o) = L2;
+                    } else {
+                        // 2-high blocks: rows r = 0, 1 belong to block ob, rows 2, 3 to block ob + 1
+                        *reinterpret_cast<uint16_t *>(P0 + o) = (uint16_t)L0;
+                        *reinterpret_cast<uint16_t *>(P1 + o) = (uint16_t)L1;
+                        *reinterpret_cast<uint16_t *>(P2 + o) = (uint16_t)L2;
+                        if (ob + 1 < G_n) {
+                            const int o2 = o + (Wk * TS);
+                            *reinterpret_cast<uint16_t *>(P0 + o2) = (uint16_t)(L0 >> 16);
+                            *reinterpret_cast<uint16_t *>(P1 + o2) = (uint16_t)(L1 >> 16);
+                            *reinterpret_cast<uint16_t *>(P2 + o2) = (uint16_t)(L2 >> 16);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- stage 2: M = columns (b, k) of the group, N = j < Hk, K = y < H ----
+        const int cols2 = G_n * Wk;
         const int rnd2 = 1 << (s2 - 1);
-        for (int m0 = 0; m0 < Hk; m0 += 16)
-            for (int n0 = 0; n0 < cols2; n0 += 16) {
+        for (int n0 = 0; n0 < cols2; n0 += 16) {
+            const int col = n0 + lr;
+            const int lrow = (col < cols2 ? col : 0) * TS;        // limb row of this column
+            long p0[2], p1[2], p2[2];
+#pragma unroll
+            for (int ki = 0; ki < 2; ki++) {
+                const int yb = 32 * ki + 8 * lq;
+                p0[ki] = p1[ki] = p2[ki] = 0;
+                if (yb < H) {           // bytes past H in the row are multiplied by zero basis columns
+                    p0[ki] = *reinterpret_cast<const long *>(P0 + lrow + yb);
+                    p1[ki] = *reinterpret_cast<const long *>(P1 + lrow + yb);
+                    p2[ki] = *reinterpret_cast<const long *>(P2 + lrow + yb);
+                }
+            }
+            // this lane's D columns: n0 + 4 lq + r, r = 0..3
+            const int ocol = n0 + 4 * lq, obk = ocol >> lgWk, ok0 = ocol & (Wk - 1);
+            const int orel = __shfl(myRel, obk < G_n ? obk : 0);
+            const int orel2 = __shfl(myRel, obk + 1 < G_n ? obk + 1 : 0);
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++) {
+                if (16 * mi >= Hk) break;
                 v4i d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
-                const int col = n0 + lr, b = col >> lgWk, k = col & (Wk - 1);
-                const bool colOk = b < G_n;
-                const int32_t *trow = tmp + ((b << lgWk) + k) * TS;
-                for (int k0 = 0; k0 < H; k0 += 32) {
-                    const long av = load_a(TH + (m0 + lr) * ch + k0 + 8 * lq);
-                    int l0[8], l1[8], l2[8];
-                    const int yb = k0 + 8 * lq;
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const int t = (colOk && yb + j < H) ? trow[yb + j] : 0;
-                        l0[j] = t & 127;
-                        l1[j] = (t >> 7) & 127;
-                        l2[j] = t >> 14;
-                    }
-                    d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, pack8(l0), d0, 0, 0, 0);
-                    d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, pack8(l1), d1, 0, 0, 0);
-                    d2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, pack8(l2), d2, 0, 0, 0);
+                for (int ki = 0; ki < 2; ki++) {
+                    if (32 * ki >= H) break;
+                    d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(p0[ki], a2[mi][ki], d0, 0, 0, 0);
+                    d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(p1[ki], a2[mi][ki], d1, 0, 0, 0);
+                    d2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(p2[ki], a2[mi][ki], d2, 0, 0, 0);
                 }
-                if (colOk) {
-                    int32_t *O = a.coeff + G.off[b];
+                int v[4];
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int j = m0 + 4 * lq + r;
-                        if (j < Hk) O[j * W + k] = (d0[r] + (d1[r] << 7) + (d2[r] << 14) + rnd2) >> s2;
+                for (int r = 0; r < 4; r++) v[r] = (d0[r] + (d1[r] << 7) + (d2[r] << 14) + rnd2) >> s2;
+                const int j = 16 * mi + lr;
+                if (j < Hk && obk < G_n) {
+                    if (Wk >= 4) {
+                        *reinterpret_cast<int4 *>(Og + orel + j * W + ok0) = make_int4(v[0], v[1], v[2], v[3]);
+                    } else {
+                        // 2-wide blocks: columns r = 0, 1 belong to block obk, 2, 3 to block obk + 1
+                        *reinterpret_cast<int2 *>(Og + orel + j * W) = make_int2(v[0], v[1]);
+                        if (obk + 1 < G_n) *reinterpret_cast<int2 *>(Og + orel2 + j * W) = make_int2(v[2], v[3]);
                     }
                 }
             }
-        // zero-out region (16-byte stores; kept widths are multiples of 4)
-        if (Wk < W || Hk < H)
-            for (int b = 0; b < G_n; b++) {
-                int32_t *O = a.coeff + G.off[b];
-                const int q4 = W >> 2;
-                for (int idx = lane; idx < (W * H) >> 2; idx += 64) {
-                    const int j = idx / q4, k = (idx - j * q4) << 2;
-                    if (j >= Hk || k >= Wk) *reinterpret_cast<int4 *>(O + j * W + k) = make_int4(0, 0, 0, 0);
-                }
-            }
+        }
+        // zero-out region: rows Hk..H-1 whole, then columns Wk..W-1 of the kept rows
+        if (Wk < W || Hk < H) {
+            if (W >= 4)
+                zero_out<int4>(Og, myRel, G_n, W, H, Wk, Hk, lane);
+            else
+                zero_out<int2>(Og, myRel, G_n, W, H, Wk, Hk, lane);
+        }
         __syncthreads();
     }
 }
@@ -251,7 +404,10 @@ using namespace xg;
 namespace {
 struct FdctPlan {
     FdctGroup *dGroups = nullptr;     // groups of blocks up to 32 high first, then the 64-high ones
+    int32_t *dRel = nullptr;          // block offsets relative to their group's base
     int n = 0, nLow = 0, cap = 0;     // groups
+    int relCap = 0;
+    bool valid = false;
 };
 std::mutex g_plan_mu;
 std::vector<std::pair<xg_session *, FdctPlan>> g_plans;
@@ -264,7 +420,42 @@ FdctPlan *plan_for(xg_session *s, bool create)
     g_plans.push_back({s, FdctPlan()});
     return &g_plans.back().second;
 }
+
+// a plan that failed validation leaves no runnable plan behind
+int32_t plan_fail(xg_session *s)
+{
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    if (FdctPlan *p = plan_for(s, false)) p->valid = false;
+    return XG_ERR_ARG;
+}
+
+// grows a device array to hold n elements (contents not kept)
+template <typename T>
+bool grow(T *&ptr, int &cap, int n, hipStream_t st)
+{
+    if (n <= cap) return true;
+    if (ptr) {
+        hipStreamSynchronize(st);
+        hipFree(ptr);
+    }
+    ptr = nullptr;
+    cap = 0;
+    if (!XG_HIP(hipMalloc((void **)&ptr, sizeof(T) * (size_t)n))) return false;
+    cap = n;
+    return true;
+}
 }  // namespace
+
+// workgroups of 64 threads that fit on the device at once
+static int resident_slots(const void *kernel)
+{
+    int dev = 0, cus = 256, per = 8;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        cus = prop.multiProcessorCount;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0) != hipSuccess || per < 1) per = 8;
+    return cus * per;
+}
 
 extern "C" {
 
@@ -272,7 +463,7 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
 {
     if (!s || n < 0 || (n > 0 && (!offset || !desc))) {
         set_error(XG_ERR_ARG, "xg_fdct_plan: bad arguments");
-        return XG_ERR_ARG;
+        return s ? plan_fail(s) : XG_ERR_ARG;
     }
     if (!mats_init()) return XG_ERR_HIP;
     static const int8_t mts[6][2] = {{kDCT2, kDCT2}, {kSKIP, kSKIP}, {kDST7, kDST7},
@@ -286,63 +477,70 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
         const int lgW = desc[i] & 15, lgH = (desc[i] >> 4) & 15, m = (desc[i] >> 8) & 15;
         if (lgW < 1 || lgW > 6 || lgH < 1 || lgH > 6 || m > 5 || m == 1) {
             set_error(XG_ERR_ARG, "xg_fdct_plan: block %d has descriptor 0x%x", i, desc[i]);
-            return XG_ERR_ARG;
+            return plan_fail(s);
         }
         const int hor = mts[m][0], ver = mts[m][1];
         if ((hor != kDCT2 && (lgW < 2 || lgW > 5)) || (ver != kDCT2 && (lgH < 2 || lgH > 5)) || offset[i] < 0) {
             set_error(XG_ERR_ARG, "xg_fdct_plan: block %d: DST-VII/DCT-VIII need sides 4..32", i);
-            return XG_ERR_ARG;
+            return plan_fail(s);
         }
         if (offset[i] % std::min(8, 1 << lgW)) {
             set_error(XG_ERR_ARG, "xg_fdct_plan: block %d: offset must be a multiple of min(8, width) samples", i);
-            return XG_ERR_ARG;
+            return plan_fail(s);
         }
         items[i].off = offset[i];
         items[i].desc = (uint32_t)(lgW | (lgH << 4) | (hor << 8) | (ver << 12));
     }
     // two launches: blocks up to 32 high (small LDS tile), then 64-high blocks;
-    // inside each, descriptors with the largest blocks first, then grouped
-    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+    // inside each, descriptors with the largest blocks first, blocks of one
+    // descriptor in memory order
+    std::sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
         const int hx = ((x.desc >> 4) & 15) == 6, hy = ((y.desc >> 4) & 15) == 6;
         if (hx != hy) return hx < hy;
         const int ax = (x.desc & 15) + ((x.desc >> 4) & 15), ay = (y.desc & 15) + ((y.desc >> 4) & 15);
         if (ax != ay) return ax > ay;
-        return x.desc < y.desc;
+        if (x.desc != y.desc) return x.desc < y.desc;
+        return x.off < y.off;
     });
     std::vector<FdctGroup> groups;
+    std::vector<int32_t> rel((size_t)n);
     int nLow = 0;
     for (int i = 0; i < n;) {
         const uint32_t d = items[i].desc;
         const int lgW = d & 15, lgH = (d >> 4) & 15;
-        const int cap = group_cap(1 << lgH, tr_kept((d >> 8) & 15, 1 << lgW));
+        const int W = 1 << lgW, H = 1 << lgH;
+        const int cap = lgH == 6 ? group_cap(W, H, tr_kept((d >> 8) & 15, W), FdctLds<64>::kStage, FdctLds<64>::kPlane)
+                                 : group_cap(W, H, tr_kept((d >> 8) & 15, W), FdctLds<32>::kStage, FdctLds<32>::kPlane);
         FdctGroup g;
         memset(&g, 0, sizeof(g));
         g.desc = d;
-        while (i < n && items[i].desc == d && g.count < cap) g.off[g.count++] = items[i++].off;
+        g.base = items[i].off;
+        g.first = i;
+        // relative offsets are int32: a group spans less than 2^30 samples
+        while (i < n && items[i].desc == d && g.count < cap && items[i].off - g.base < (1LL << 30)) {
+            rel[(size_t)i] = (int32_t)(items[i].off - g.base);
+            g.count++;
+            i++;
+        }
         groups.push_back(g);
         if (lgH != 6) nLow = (int)groups.size();
     }
     const int ng = (int)groups.size();
     std::lock_guard<std::mutex> g(g_plan_mu);
     FdctPlan *p = plan_for(s, true);
-    if (ng > p->cap) {
-        if (p->dGroups) {
-            hipStreamSynchronize((hipStream_t)xg_session_stream(s));
-            hipFree(p->dGroups);
-        }
-        p->dGroups = nullptr;
-        p->cap = 0;
-        if (!XG_HIP(hipMalloc((void **)&p->dGroups, sizeof(FdctGroup) * (size_t)std::max(ng, 1)))) return XG_ERR_NOMEM;
-        p->cap = std::max(ng, 1);
-    }
+    p->valid = false;
     hipStream_t st = (hipStream_t)xg_session_stream(s);
+    if (!grow(p->dGroups, p->cap, std::max(ng, 1), st) || !grow(p->dRel, p->relCap, std::max(n, 1), st))
+        return XG_ERR_NOMEM;
     if (ng && !XG_HIP(hipMemcpyAsync(p->dGroups, groups.data(), sizeof(FdctGroup) * (size_t)ng, hipMemcpyHostToDevice,
                                      st)))
         return XG_ERR_HIP;
+    if (n && !XG_HIP(hipMemcpyAsync(p->dRel, rel.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st)))
+        return XG_ERR_HIP;
     if (!XG_HIP(hipStreamSynchronize(st))) return XG_ERR_HIP;
-    n = ng;
-    p->n = n;
+    p->n = ng;
     p->nLow = nLow;
+    p->valid = true;
     return XG_OK;
 }
 
@@ -357,39 +555,32 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
         std::lock_guard<std::mutex> g(g_plan_mu);
         p = plan_for(s, false);
     }
-    if (!p) {
-        set_error(XG_ERR_STATE, "xg_fdct_run: no plan (call xg_fdct_plan first)");
+    if (!p || !p->valid) {
+        set_error(XG_ERR_STATE, "xg_fdct_run: no plan (call xg_fdct_plan first; a failed plan leaves none)");
         return XG_ERR_STATE;
     }
     if (p->n == 0) return XG_OK;
     FdctArgs a;
     a.resid = resid;
     a.coeff = coeff;
-    a.groups = p->dGroups;
-    a.n = p->n;
     a.mats = g_mats;
     memcpy(a.matOff, g_mi.off, sizeof(a.matOff));
     memcpy(a.matCols, g_mi.cols, sizeof(a.matCols));
-    static int cus = [] {
-        int dev = 0;
-        hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-            return prop.multiProcessorCount;
-        return 256;
-    }();
+    // one resident wave per workgroup slot the chip holds (registers and LDS bound it)
+    static const int slots[2] = {resident_slots((const void *)k_fdct_mfma<32>),
+                                 resident_slots((const void *)k_fdct_mfma<64>)};
     hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
     const bool timed = session_timer_begin(s, 2, ev);
     if (p->nLow > 0) {
         FdctArgs lo = a;
         lo.n = p->nLow;
-        k_fdct_mfma<32><<<std::min(lo.n, cus * 32), 64, 0, st>>>(lo);
+        k_fdct_mfma<32><<<std::min(lo.n, slots[0]), 64, 0, st>>>(lo, p->dGroups, p->dRel);
     }
     if (p->n > p->nLow) {
         FdctArgs hi = a;
-        hi.groups = p->dGroups + p->nLow;
         hi.n = p->n - p->nLow;
-        k_fdct_mfma<64><<<std::min(hi.n, cus * 16), 64, 0, st>>>(hi);
+        k_fdct_mfma<64><<<std::min(hi.n, slots[1]), 64, 0, st>>>(hi, p->dGroups + p->nLow, p->dRel);
     }
     if (timed) session_timer_end(s, 2, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
@@ -403,6 +594,7 @@ void xg::fdct_release(xg_session *s)
     for (size_t i = 0; i < g_plans.size(); i++)
         if (g_plans[i].first == s) {
             if (g_plans[i].second.dGroups) hipFree(g_plans[i].second.dGroups);
+            if (g_plans[i].second.dRel) hipFree(g_plans[i].second.dRel);
             g_plans.erase(g_plans.begin() + (long)i);
             return;
         }
