@@ -23,6 +23,7 @@ ARCH = os.environ.get("XG_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",      # MFMA results in VGPRs: no accumulator-register reads
          "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
 

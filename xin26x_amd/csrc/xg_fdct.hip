@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
 #pragma unroll
             for (int mi = 0; mi < 2; mi++) {
                 if (16 * mi >= Wk) break;
-                v4i dlo = {0, 0, 0, 0}, dhi = {0, 0, 0, 0};
+                v4i dlo = {rnd1, rnd1, rnd1, rnd1}, dhi = {0, 0, 0, 0};     // rounding offset rides in the accumulator
 #pragma unroll
                 for (int ki = 0; ki < 2; ki++) {
                     if (32 * ki >= W) break;
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
                 }
                 int t[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) t[r] = (dlo[r] + (dhi[r] << 7) + rnd1) >> s1;
+                for (int r = 0; r < 4; r++) t[r] = (dlo[r] + (dhi[r] << 7)) >> s1;
                 const uint32_t L0 = bytes4(t[0], t[1], t[2], t[3]) & 0x7F7F7F7Fu;
                 const uint32_t L1 = bytes4(t[0] >> 7, t[1] >> 7, t[2] >> 7, t[3] >> 7) & 0x7F7F7F7Fu;
                 const uint32_t L2 = bytes4(t[0] >> 14, t[1] >> 14, t[2] >> 14, t[3] >> 14);
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
 #pragma unroll
             for (int mi = 0; mi < 2; mi++) {
                 if (16 * mi >= Hk) break;
-                v4i d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
+                v4i d0 = {rnd2, rnd2, rnd2, rnd2}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
 #pragma unroll
                 for (int ki = 0; ki < 2; ki++) {
                     if (32 * ki >= H) break;
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
                 }
                 int v[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) v[r] = (d0[r] + (d1[r] << 7) + (d2[r] << 14) + rnd2) >> s2;
+                for (int r = 0; r < 4; r++) v[r] = (d0[r] + (d1[r] << 7) + (d2[r] << 14)) >> s2;
                 const int j = 16 * mi + lr;
                 if (j < Hk && obk < G_n) {
                     if (Wk >= 4) {
