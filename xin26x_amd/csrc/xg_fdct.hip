@@ -35,8 +35,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 namespace xg {
 
 // Padded int8 matrices: one per (type, lgN); rows padded to max(16, N),
-// columns to max(32, N), zeros in the padding.
-struct MatIndex { int off[3][7]; int cols[3][7]; int total; };
+// columns to max(32, N), zeros in the padding.  For N <= 8 also a
+// block-diagonal 16 x 32 form: 16 / N copies of the N x N matrix along the
+// diagonal, so one MFMA tile transforms 16 / N row sets at once.
+struct MatIndex { int off[3][7]; int cols[3][7]; int diag[3][4]; int total; };
 
 static MatIndex mat_index()
 {
@@ -49,6 +51,12 @@ static MatIndex mat_index()
             m.off[t][lg] = ok ? o : -1;
             m.cols[t][lg] = std::max(32, N);
             if (ok) o += std::max(16, N) * std::max(32, N);
+        }
+    for (int t = 0; t < 3; t++)
+        for (int lg = 1; lg <= 3; lg++) {
+            const bool ok = t == kDCT2 || lg >= 2;
+            m.diag[t][lg] = ok ? o : -1;
+            if (ok) o += 16 * 32;
         }
     m.total = o;
     return m;
@@ -71,6 +79,16 @@ static bool mats_init()
                 for (int k = 0; k < N; k++)
                     for (int n = 0; n < N; n++)
                         h[(size_t)g_mi.off[t][lg] + (size_t)k * cols + n] = (int8_t)tr_entry(kTr, t, N, k, n);
+            }
+        for (int t = 0; t < 3; t++)
+            for (int lg = 1; lg <= 3; lg++) {
+                if (g_mi.diag[t][lg] < 0) continue;
+                const int N = 1 << lg;
+                for (int q = 0; q < 16 / N; q++)
+                    for (int k = 0; k < N; k++)
+                        for (int n = 0; n < N; n++)
+                            h[(size_t)g_mi.diag[t][lg] + (size_t)(q * N + k) * 32 + q * N + n] =
+                                (int8_t)tr_entry(kTr, t, N, k, n);
             }
         if (!XG_HIP(hipMalloc((void **)&g_mats, h.size()))) return;
         if (!XG_HIP(hipMemcpy(g_mats, h.data(), h.size(), hipMemcpyHostToDevice))) return;
@@ -121,6 +139,7 @@ struct FdctArgs {
     const int8_t *mats;
     int matOff[3][7];
     int matCols[3][7];
+    int diagOff[3][4];
 };
 
 
@@ -233,9 +252,12 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
         const int lgWk = 31 - __builtin_clz(Wk);
         const int TS = tmp_stride(H);
         const int s1 = lgW + 1, s2 = lgH + 6;
-        const int8_t *TW = a.mats + a.matOff[hor][lgW];
-        const int8_t *TH = a.mats + a.matOff[ver][lgH];
-        const int cw = a.matCols[hor][lgW], ch = a.matCols[ver][lgH];
+        // sides up to 8: block-diagonal basis, 16 / side row sets (stage 1) or
+        // columns (stage 2) per tile, so no lane of D holds padding
+        const bool dW = W <= 8, dH = H <= 8;
+        const int8_t *TW = a.mats + (dW ? a.diagOff[hor][lgW] : a.matOff[hor][lgW]);
+        const int8_t *TH = a.mats + (dH ? a.diagOff[ver][lgH] : a.matOff[ver][lgH]);
+        const int cw = dW ? 32 : a.matCols[hor][lgW], ch = dH ? 32 : a.matCols[ver][lgH];
         const int16_t *Rg = a.resid + G.base;
         int32_t *Og = a.coeff + G.base;
 
@@ -272,30 +294,53 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
         // ---- stage 1: M = rows (b, y) of the group, N = k < Wk, K = x < W ----
         const int rows1 = G_n * H;
         const int rnd1 = 1 << (s1 - 1);
-        for (int n0 = 0; n0 < rows1; n0 += 16) {
-            const int row = n0 + lr;
-            const bool rowOk = row < rows1;
-            const int16_t *R = S + (rowOk ? row << lgW : 0);     // blocks are back to back in the stage
+        const int lgQ1 = dW ? 4 - lgW : 0;                 // row sets per tile
+        for (int n0 = 0; n0 < rows1; n0 += 16 << lgQ1) {
             long blo[2], bhi[2];
+            if (!dW) {
+                const int row = n0 + lr;
+                const int16_t *R = S + (row < rows1 ? row << lgW : 0);     // blocks are back to back
 #pragma unroll
-            for (int ki = 0; ki < 2; ki++) {
+                for (int ki = 0; ki < 2; ki++) {
+                    const int x = 32 * ki + 8 * lq;
+                    const uint4 v = x < W ? *reinterpret_cast<const uint4 *>(R + x) : make_uint4(0, 0, 0, 0);
+                    limbs8(v, blo[ki], bhi[ki]);
+                }
+            } else {
+                // K slice lq < 2 carries row sets q = lq * 8 / W .. : row n0 + lr + 16 q, W samples each
                 uint4 v = make_uint4(0, 0, 0, 0);
-                const int x = 32 * ki + 8 * lq;
-                if (x < W) {
-                    if (W >= 8) {
-                        v = *reinterpret_cast<const uint4 *>(R + x);
+                if (lq < 2) {
+                    if (W == 8) {
+                        const int r0 = n0 + lr + 16 * lq;
+                        if (r0 < rows1) v = *reinterpret_cast<const uint4 *>(S + r0 * 8);
                     } else if (W == 4) {
-                        const uint2 h2 = *reinterpret_cast<const uint2 *>(R);
-                        v.x = h2.x;
-                        v.y = h2.y;
+                        const int r0 = n0 + lr + 32 * lq, r1 = r0 + 16;
+                        if (r0 < rows1) {
+                            const uint2 h = *reinterpret_cast<const uint2 *>(S + r0 * 4);
+                            v.x = h.x;
+                            v.y = h.y;
+                        }
+                        if (r1 < rows1) {
+                            const uint2 h = *reinterpret_cast<const uint2 *>(S + r1 * 4);
+                            v.z = h.x;
+                            v.w = h.y;
+                        }
                     } else {
-                        v.x = *reinterpret_cast<const uint32_t *>(R);
+                        uint32_t w[4];
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            const int rt = n0 + lr + 16 * (4 * lq + t);
+                            w[t] = rt < rows1 ? *reinterpret_cast<const uint32_t *>(S + rt * 2) : 0;
+                        }
+                        v = make_uint4(w[0], w[1], w[2], w[3]);
                     }
                 }
-                limbs8(v, blo[ki], bhi[ki]);
+                limbs8(v, blo[0], bhi[0]);
+                blo[1] = bhi[1] = 0;
             }
-            // this lane's D rows: n0 + 4 lq + r, r = 0..3
-            const int orow = n0 + 4 * lq, ob = orow >> lgH, oy = orow & (H - 1);
+            // this lane's D rows: n0 + 4 lq + 16 q + r, r = 0..3, of column k
+            const int q = dW ? lr >> lgW : 0;
+            const int orow = n0 + 4 * lq + 16 * q, ob = orow >> lgH, oy = orow & (H - 1);
 #pragma unroll
             for (int mi = 0; mi < 2; mi++) {
                 if (16 * mi >= Wk) break;
@@ -312,7 +357,7 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
                 const uint32_t L0 = bytes4(t[0], t[1], t[2], t[3]) & 0x7F7F7F7Fu;
                 const uint32_t L1 = bytes4(t[0] >> 7, t[1] >> 7, t[2] >> 7, t[3] >> 7) & 0x7F7F7F7Fu;
                 const uint32_t L2 = bytes4(t[0] >> 14, t[1] >> 14, t[2] >> 14, t[3] >> 14);
-                const int k = 16 * mi + lr;
+                const int k = dW ? lr & (W - 1) : 16 * mi + lr;
                 if (k < Wk && ob < G_n) {
                     const int o = ((ob << lgWk) + k) * TS + oy;
                     if (H >= 4) {
@@ -339,22 +384,63 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
         // ---- stage 2: M = columns (b, k) of the group, N = j < Hk, K = y < H ----
         const int cols2 = G_n * Wk;
         const int rnd2 = 1 << (s2 - 1);
-        for (int n0 = 0; n0 < cols2; n0 += 16) {
-            const int col = n0 + lr;
-            const int lrow = (col < cols2 ? col : 0) * TS;        // limb row of this column
+        const int lgQ2 = dH ? 4 - lgH : 0;                 // column sets per tile
+        for (int n0 = 0; n0 < cols2; n0 += 16 << lgQ2) {
             long p0[2], p1[2], p2[2];
+            if (!dH) {
+                const int col = n0 + lr;
+                const int lrow = (col < cols2 ? col : 0) * TS;        // limb row of this column
 #pragma unroll
-            for (int ki = 0; ki < 2; ki++) {
-                const int yb = 32 * ki + 8 * lq;
-                p0[ki] = p1[ki] = p2[ki] = 0;
-                if (yb < H) {           // bytes past H in the row are multiplied by zero basis columns
-                    p0[ki] = *reinterpret_cast<const long *>(P0 + lrow + yb);
-                    p1[ki] = *reinterpret_cast<const long *>(P1 + lrow + yb);
-                    p2[ki] = *reinterpret_cast<const long *>(P2 + lrow + yb);
+                for (int ki = 0; ki < 2; ki++) {
+                    const int yb = 32 * ki + 8 * lq;
+                    p0[ki] = p1[ki] = p2[ki] = 0;
+                    if (yb < H) {
+                        p0[ki] = *reinterpret_cast<const long *>(P0 + lrow + yb);
+                        p1[ki] = *reinterpret_cast<const long *>(P1 + lrow + yb);
+                        p2[ki] = *reinterpret_cast<const long *>(P2 + lrow + yb);
+                    }
                 }
+            } else {
+                // K slice lq < 2 carries column sets q = lq * 8 / H .. : limb row n0 + lr + 16 q, H bytes each
+                unsigned long long x0 = 0, x1 = 0, x2 = 0;
+                if (lq < 2) {
+                    if (H == 8) {
+                        const int c0 = n0 + lr + 16 * lq;
+                        if (c0 < cols2) {
+                            x0 = *reinterpret_cast<const unsigned long long *>(P0 + c0 * 8);
+                            x1 = *reinterpret_cast<const unsigned long long *>(P1 + c0 * 8);
+                            x2 = *reinterpret_cast<const unsigned long long *>(P2 + c0 * 8);
+                        }
+                    } else if (H == 4) {
+#pragma unroll
+                        for (int t = 0; t < 2; t++) {
+                            const int ct = n0 + lr + 16 * (2 * lq + t);
+                            if (ct < cols2) {
+                                x0 |= (unsigned long long)*reinterpret_cast<const uint32_t *>(P0 + ct * 8) << (32 * t);
+                                x1 |= (unsigned long long)*reinterpret_cast<const uint32_t *>(P1 + ct * 8) << (32 * t);
+                                x2 |= (unsigned long long)*reinterpret_cast<const uint32_t *>(P2 + ct * 8) << (32 * t);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            const int ct = n0 + lr + 16 * (4 * lq + t);
+                            if (ct < cols2) {
+                                x0 |= (unsigned long long)*reinterpret_cast<const uint16_t *>(P0 + ct * 8) << (16 * t);
+                                x1 |= (unsigned long long)*reinterpret_cast<const uint16_t *>(P1 + ct * 8) << (16 * t);
+                                x2 |= (unsigned long long)*reinterpret_cast<const uint16_t *>(P2 + ct * 8) << (16 * t);
+                            }
+                        }
+                    }
+                }
+                p0[0] = (long)x0;
+                p1[0] = (long)x1;
+                p2[0] = (long)x2;
+                p0[1] = p1[1] = p2[1] = 0;
             }
-            // this lane's D columns: n0 + 4 lq + r, r = 0..3
-            const int ocol = n0 + 4 * lq, obk = ocol >> lgWk, ok0 = ocol & (Wk - 1);
+            // this lane's D columns: n0 + 4 lq + 16 q + r, r = 0..3, of output row j
+            const int q = dH ? lr >> lgH : 0;
+            const int ocol = n0 + 4 * lq + 16 * q, obk = ocol >> lgWk, ok0 = ocol & (Wk - 1);
             const int orel = __shfl(myRel, obk < G_n ? obk : 0);
             const int orel2 = __shfl(myRel, obk + 1 < G_n ? obk + 1 : 0);
 #pragma unroll
@@ -371,7 +457,7 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
                 int v[4];
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[r] = (d0[r] + (d1[r] << 7) + (d2[r] << 14)) >> s2;
-                const int j = 16 * mi + lr;
+                const int j = dH ? lr & (H - 1) : 16 * mi + lr;
                 if (j < Hk && obk < G_n) {
                     if (Wk >= 4) {
                         *reinterpret_cast<int4 *>(Og + orel + j * W + ok0) = make_int4(v[0], v[1], v[2], v[3]);
@@ -566,6 +652,7 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
     a.mats = g_mats;
     memcpy(a.matOff, g_mi.off, sizeof(a.matOff));
     memcpy(a.matCols, g_mi.cols, sizeof(a.matCols));
+    memcpy(a.diagOff, g_mi.diag, sizeof(a.diagOff));
     // one resident wave per workgroup slot the chip holds (registers and LDS bound it)
     static const int slots[2] = {resident_slots((const void *)k_fdct_mfma<32>),
                                  resident_slots((const void *)k_fdct_mfma<64>)};
