@@ -155,7 +155,8 @@ def transforms_leg(lib, steps, warmup, cpu_seconds):
                        "DCT-II or DST-VII/DCT-VIII (sides <= 32), 10-bit shifts, zero-out",
            "value": round(n / (wall_ms / 1e3), 1), "unit": "blocks/s",
            "coefficients_per_s": round(samples / (wall_ms / 1e3), 1), "ms_per_batch": round(wall_ms, 4),
-           "kernel": "k_fdct_mfma", "kernel_ms": round(k_ms, 4),
+           "kernel": "k_fdct_mfma<32> + k_fdct_mfma<64> (blocks up to 32 high, then 64 high)",
+           "kernel_ms": round(k_ms, 4),
            "roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
                         "traffic": profiled_traffic(PMC_FDCT)[0], "traffic_source": profiled_traffic(PMC_FDCT)[1],
