@@ -46,10 +46,15 @@
 
 namespace xg {
 
-constexpr int kLumaLW = 80;        // LDS row stride (samples) of the luma window: W + 8 + 8 at most
-constexpr int kLumaLH = 80;        // rows: H + 7 + 8 at most
-constexpr int kChromaLW = 40;      // per-plane chroma window stride: W/2 + 4 + 4 at most
-constexpr int kChromaLH = 40;      // rows: H/2 + 3 + 4 at most
+// Blocks are refined in sub-blocks of at most kSub x kSub luma samples (a
+// block with a 64-sample side is four or two of them, one after another;
+// SATD tiles never cross a sub-block edge, so the sum is the same).  The LDS
+// windows are sized for one sub-block, which keeps a workgroup at ~9 KB.
+constexpr int kSub = 32;
+constexpr int kLumaLW = kSub + 16;     // LDS row stride (samples) of the luma window: W + 8 + 8 at most
+constexpr int kLumaLH = kSub + 16;     // rows: H + 7 + 8 at most, plus the one-row fetch-ahead
+constexpr int kChromaLW = kSub / 2 + 8;  // per-plane chroma window stride: W/2 + 4 + 4 at most
+constexpr int kChromaLH = kSub / 2 + 8;  // rows: H/2 + 3 + 4 at most, plus the fetch-ahead
 constexpr int kMaxOff = 64;        // |candidate offset| limit, 1/16 pel
 constexpr int kNumShapes = 25;     // (lgW-2)*5 + (lgH-2), W, H in 4..64
 constexpr int kPicSlots = kSessionPicSlots;
@@ -327,9 +332,11 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int ls, int
 }
 
 // Luma SATD of every candidate of a task; lane = unit * (W / 2) + m.
+// (ox, oy): the sub-block's offset inside the block, H its height; csum accumulates.
 template <int W, int TS>
 __device__ __forceinline__ void luma_task(const xg_block &B, const MvWin &mw, const RefineLds &L, int wx0, int wy0,
-                                          const uint16_t *cur, intptr_t cs, int c0, int nc, uint32_t *csum)
+                                          const uint16_t *cur, intptr_t cs, int H, int ox, int oy, int c0, int nc,
+                                          uint32_t *csum)
 {
     constexpr int LN = W / 2, CP = 64 / LN;
     const int lane = threadIdx.x, m = lane & (LN - 1), u = lane / LN;
@@ -338,11 +345,11 @@ __device__ __forceinline__ void luma_task(const xg_block &B, const MvWin &mw, co
         const int c = c0 + min(ci, nc - 1);
         const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
         const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
-        const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
+        const int rx = ((B.x + ox) << 4) + mvx, ry = ((B.y + oy) << 4) + mvy;
         int acc = interp_satd_pass<8, W, TS>(L.luma, kLumaLW, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0,
-                                             L.tapL + 4 * (rx & 15), L.tapL + 4 * (ry & 15), cur, cs, B.h, m, lane);
+                                             L.tapL + 4 * (rx & 15), L.tapL + 4 * (ry & 15), cur, cs, H, m, lane);
         acc = group_sum<LN>(acc);
-        if (m == 0 && ci < nc) csum[ci] = (uint32_t)acc;
+        if (m == 0 && ci < nc) csum[ci] += (uint32_t)acc;
     }
 }
 
@@ -350,8 +357,8 @@ __device__ __forceinline__ void luma_task(const xg_block &B, const MvWin &mw, co
 // plane, lane = unit * (WC / 2) + m, so a candidate's two planes are adjacent.
 template <int WC, int TC>
 __device__ __forceinline__ void chroma_task(const xg_block &B, const MvWin &mw, const RefineLds &L, int cx0, int cy0,
-                                            const uint16_t *curU, const uint16_t *curV, intptr_t cs, int c0, int nc,
-                                            uint32_t *csum)
+                                            const uint16_t *curU, const uint16_t *curV, intptr_t cs, int HC, int ox,
+                                            int oy, int c0, int nc, uint32_t *csum)
 {
     constexpr int LN = WC / 2 > 0 ? WC / 2 : 1, UP = 64 / LN;
     const int lane = threadIdx.x, m = lane & (LN - 1), u = lane / LN;
@@ -361,10 +368,10 @@ __device__ __forceinline__ void chroma_task(const xg_block &B, const MvWin &mw, 
         const int c = c0 + min(ci, nc - 1);
         const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
         const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
-        const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
+        const int rx = ((B.x + ox) << 4) + mvx, ry = ((B.y + oy) << 4) + mvy;
         int acc = interp_satd_pass<4, WC, TC>(pl ? L.cr : L.cb, kChromaLW, (rx >> 5) - 1 - cx0, (ry >> 5) - 1 - cy0,
                                               L.tapC + 2 * (rx & 31), L.tapC + 2 * (ry & 31), pl ? curV : curU, cs,
-                                              B.h >> 1, m, lane);
+                                              HC, m, lane);
         acc = group_sum<2 * LN>(acc);
         if (m == 0 && pl == 0 && ci < nc) csum[ci] += (uint32_t)acc;
     }
@@ -470,11 +477,66 @@ __device__ __forceinline__ void mc_pass(const uint32_t *win, int ls, int wx, int
     }
 }
 
-// Best-candidate record and luma + chroma prediction of block b, from the
-// task's staged windows (they hold every candidate's footprint).
+// Windows of one sub-block (luma origin sx, sy, size sw x sh) for every
+// candidate of the configuration: every luma / chroma sample any candidate's
+// filters touch, from the even column at or left of the first tap
+// (dword-aligned rows), staged into LDS.
+struct SubWin {
+    int wx0, wy0, cx0, cy0;
+};
+
+__device__ __forceinline__ SubWin stage_sub(const RefineArgs &a, const xg_block &B, const MvWin &mw,
+                                            const xg_picture &R, int sx, int sy, int sw, int sh, RefineLds &L)
+{
+    const int qx0 = (sx << 4) + clampi(B.mvx + a.offMinX, mw.mnx, mw.mxx);
+    const int qx1 = (sx << 4) + clampi(B.mvx + a.offMaxX, mw.mnx, mw.mxx);
+    const int qy0 = (sy << 4) + clampi(B.mvy + a.offMinY, mw.mny, mw.mxy);
+    const int qy1 = (sy << 4) + clampi(B.mvy + a.offMaxY, mw.mny, mw.mxy);
+    SubWin w;
+    w.wx0 = ((qx0 >> 4) - 3) & ~1;
+    w.wy0 = (qy0 >> 4) - 3;
+    const int WW = (qx1 >> 4) + sw + 4 - w.wx0, HH = (qy1 >> 4) - (qy0 >> 4) + sh + 7;
+    w.cx0 = ((qx0 >> 5) - 1) & ~1;
+    w.cy0 = (qy0 >> 5) - 1;
+    const int CWW = (qx1 >> 5) + (sw >> 1) + 2 - w.cx0, CHH = (qy1 >> 5) - (qy0 >> 5) + (sh >> 1) + 3;
+    const StagePlan pl = stage_plan(L.luma, L.luma, kLumaLW / 2, plane_dw(R.plane[0], w.wx0, w.wy0), nullptr,
+                                    R.plane[0].stride >> 1, HH, (WW + 1) >> 1, 1);
+    const StagePlan pc = stage_plan(L.cb, L.cr, kChromaLW / 2, plane_dw(R.plane[1], w.cx0, w.cy0),
+                                    plane_dw(R.plane[2], w.cx0, w.cy0), R.plane[1].stride >> 1, CHH, (CWW + 1) >> 1,
+                                    2);
+    const int nb = max(pl.batches, pc.batches);
+    for (int k = 0; k < nb; k++) {
+        uint32_t vl[8], vc[8];
+        if (k < pl.batches) stage_load(pl, k, vl);
+        if (k < pc.batches) stage_load(pc, k, vc);
+        if (k < pl.batches) stage_store(pl, k, vl);
+        if (k < pc.batches) stage_store(pc, k, vc);
+    }
+    return w;
+}
+
+// Luma + chroma prediction of one sub-block of the winner from the staged windows.
+__device__ __forceinline__ void mc_sub(const RefineArgs &a, const xg_block &B, int mvx, int mvy, int sx, int sy,
+                                       int sw, int sh, const SubWin &w, const RefineLds &L)
+{
+    const xg_picture &PT = a.predTab[B.pic];
+    const xg_plane &P0 = PT.plane[0], &P1 = PT.plane[1], &P2 = PT.plane[2];
+    const int rx = (sx << 4) + mvx, ry = (sy << 4) + mvy;   // the sub-block's reference position
+    mc_pass<8>(L.luma, kLumaLW, (rx >> 4) - 3 - w.wx0, (ry >> 4) - 3 - w.wy0, L.tapL + 4 * (rx & 15), ry & 15, sw, sh,
+               P0.data + (intptr_t)sy * P0.stride + sx, P0.stride, 0, 64);
+    const int wc = sw >> 1, hc = sh >> 1, ox = (rx >> 5) - 1 - w.cx0, oy = (ry >> 5) - 1 - w.cy0;
+    const uint32_t *hc4 = L.tapC + 2 * (rx & 31);
+    mc_pass<4>(L.cb, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
+               P1.data + (intptr_t)(sy >> 1) * P1.stride + (sx >> 1), P1.stride, 0, 64);
+    mc_pass<4>(L.cr, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
+               P2.data + (intptr_t)(sy >> 1) * P2.stride + (sx >> 1), P2.stride, 0, 64);
+}
+
+// Best-candidate record and luma + chroma prediction of block b.  A block of
+// one sub-block predicts from the windows its task staged; a larger block
+// stages each sub-block again (they hold every candidate's footprint).
 __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_block &B, const MvWin &mw,
-                                         unsigned long long key, const RefineLds &L, int wx0, int wy0, int cx0,
-                                         int cy0)
+                                         unsigned long long key, RefineLds &L, const SubWin &last)
 {
     const int lane = threadIdx.x;
     const int c = (int)(uint32_t)key;
@@ -488,17 +550,19 @@ __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_bl
         r.cand = c;
         a.best[b] = r;
     }
-    const int rx = (B.x << 4) + mvx, ry = (B.y << 4) + mvy;
-    const xg_picture &PT = a.predTab[B.pic];
-    const xg_plane &P0 = PT.plane[0], &P1 = PT.plane[1], &P2 = PT.plane[2];
-    mc_pass<8>(L.luma, kLumaLW, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0, L.tapL + 4 * (rx & 15), ry & 15, B.w, B.h,
-               P0.data + (intptr_t)B.y * P0.stride + B.x, P0.stride, 0, 64);
-    const int wc = B.w >> 1, hc = B.h >> 1, ox = (rx >> 5) - 1 - cx0, oy = (ry >> 5) - 1 - cy0;
-    const uint32_t *hc4 = L.tapC + 2 * (rx & 31);
-    mc_pass<4>(L.cb, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
-               P1.data + (intptr_t)(B.y >> 1) * P1.stride + (B.x >> 1), P1.stride, 0, 64);
-    mc_pass<4>(L.cr, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
-               P2.data + (intptr_t)(B.y >> 1) * P2.stride + (B.x >> 1), P2.stride, 0, 64);
+    const int sw = min(B.w, kSub), sh = min(B.h, kSub);
+    if (sw == B.w && sh == B.h) {
+        mc_sub(a, B, mvx, mvy, B.x, B.y, sw, sh, last, L);
+        return;
+    }
+    const xg_picture &R = a.pics[B.ref];
+    for (int sy = B.y; sy < B.y + B.h; sy += sh)
+        for (int sx = B.x; sx < B.x + B.w; sx += sw) {
+            __syncthreads();
+            const SubWin w = stage_sub(a, B, mw, R, sx, sy, sw, sh, L);
+            __syncthreads();
+            mc_sub(a, B, mvx, mvy, sx, sy, sw, sh, w, L);
+        }
 }
 
 __device__ __forceinline__ void refine_one(const RefineArgs &a, int2 tk, const xg_block &B, RefineLds &L)
@@ -509,72 +573,50 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int2 tk, const x
     const xg_picture &R = a.pics[B.ref];
     const MvWin mw = mv_window(B, P.plane[0].width, P.plane[0].height, a.ctuSize);
 
-    // ---- windows: every luma / chroma sample any candidate's filters touch,
-    // from the even column at or left of the first tap (dword-aligned rows) ----
-    const int qx0 = (B.x << 4) + clampi(B.mvx + a.offMinX, mw.mnx, mw.mxx);
-    const int qx1 = (B.x << 4) + clampi(B.mvx + a.offMaxX, mw.mnx, mw.mxx);
-    const int qy0 = (B.y << 4) + clampi(B.mvy + a.offMinY, mw.mny, mw.mxy);
-    const int qy1 = (B.y << 4) + clampi(B.mvy + a.offMaxY, mw.mny, mw.mxy);
-    const int wx0 = ((qx0 >> 4) - 3) & ~1, wy0 = (qy0 >> 4) - 3;
-    const int WW = (qx1 >> 4) + B.w + 4 - wx0, HH = (qy1 >> 4) - (qy0 >> 4) + B.h + 7;
-    const int cx0 = ((qx0 >> 5) - 1) & ~1, cy0 = (qy0 >> 5) - 1;
-    const int CWW = (qx1 >> 5) + (B.w >> 1) + 2 - cx0, CHH = (qy1 >> 5) - (qy0 >> 5) + (B.h >> 1) + 3;
     const intptr_t cs = P.plane[0].stride, ccs = P.plane[1].stride;
-    const uint16_t *cur = P.plane[0].data + (intptr_t)B.y * cs + B.x;
-    const intptr_t co = (intptr_t)(B.y >> 1) * ccs + (B.x >> 1);
-    const uint16_t *cu = P.plane[1].data + co, *cv = P.plane[2].data + co;
-    {
-        const StagePlan pl = stage_plan(L.luma, L.luma, kLumaLW / 2, plane_dw(R.plane[0], wx0, wy0), nullptr,
-                                        R.plane[0].stride >> 1, HH, (WW + 1) >> 1, 1);
-        const StagePlan pc = stage_plan(L.cb, L.cr, kChromaLW / 2, plane_dw(R.plane[1], cx0, cy0),
-                                        plane_dw(R.plane[2], cx0, cy0), R.plane[1].stride >> 1, CHH, (CWW + 1) >> 1,
-                                        2);
-        const int nb = max(pl.batches, pc.batches);
-        for (int k = 0; k < nb; k++) {
-            uint32_t vl[8], vc[8];
-            if (k < pl.batches) stage_load(pl, k, vl);
-            if (k < pc.batches) stage_load(pc, k, vc);
-            if (k < pl.batches) stage_store(pl, k, vl);
-            if (k < pc.batches) stage_store(pc, k, vc);
-        }
-    }
     L.csum[lane] = 0;
-    __syncthreads();
-
-    const bool t4 = B.w == 4 || B.h == 4;
     uint32_t *csum = L.csum;
-    switch (B.w) {
-    case 4: luma_task<4, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
-    case 8: t4 ? luma_task<8, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
-               : luma_task<8, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
-    case 16: t4 ? luma_task<16, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
-                : luma_task<16, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
-    case 32: t4 ? luma_task<32, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
-                : luma_task<32, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
-    default: t4 ? luma_task<64, 4>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum)
-                : luma_task<64, 8>(B, mw, L, wx0, wy0, cur, cs, c0, nc, csum); break;
-    }
-    __syncthreads();
+    const int sw = min(B.w, kSub), sh = min(B.h, kSub);
+    SubWin w{};
+    for (int sy = B.y; sy < B.y + B.h; sy += sh)
+        for (int sx = B.x; sx < B.x + B.w; sx += sw) {
+            __syncthreads();                           // the previous sub-block's readers are done
+            w = stage_sub(a, B, mw, R, sx, sy, sw, sh, L);
+            __syncthreads();
+            const uint16_t *cur = P.plane[0].data + (intptr_t)sy * cs + sx;
+            const int ox = sx - B.x, oy = sy - B.y;     // sub-block offset inside the block
+            const bool t4 = sw == 4 || sh == 4;
+            switch (sw) {
+            case 4: luma_task<4, 4>(B, mw, L, w.wx0, w.wy0, cur, cs, sh, ox, oy, c0, nc, csum); break;
+            case 8: t4 ? luma_task<8, 4>(B, mw, L, w.wx0, w.wy0, cur, cs, sh, ox, oy, c0, nc, csum)
+                       : luma_task<8, 8>(B, mw, L, w.wx0, w.wy0, cur, cs, sh, ox, oy, c0, nc, csum); break;
+            case 16: t4 ? luma_task<16, 4>(B, mw, L, w.wx0, w.wy0, cur, cs, sh, ox, oy, c0, nc, csum)
+                        : luma_task<16, 8>(B, mw, L, w.wx0, w.wy0, cur, cs, sh, ox, oy, c0, nc, csum); break;
+            default: t4 ? luma_task<32, 4>(B, mw, L, w.wx0, w.wy0, cur, cs, sh, ox, oy, c0, nc, csum)
+                        : luma_task<32, 8>(B, mw, L, w.wx0, w.wy0, cur, cs, sh, ox, oy, c0, nc, csum); break;
+            }
+            __syncthreads();
 
-    if (a.useChroma) {
-        const int wc = B.w >> 1, hc = B.h >> 1;
-        const int tc = (wc % 8 == 0 && hc % 8 == 0) ? 8 : (wc % 4 == 0 && hc % 4 == 0) ? 4 : 2;
-        switch (wc) {
-        case 2: chroma_task<2, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
-        case 4: tc == 4 ? chroma_task<4, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
-                        : chroma_task<4, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
-        case 8: tc == 8 ? chroma_task<8, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
-              : tc == 4 ? chroma_task<8, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
-                        : chroma_task<8, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
-        case 16: tc == 8 ? chroma_task<16, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
-               : tc == 4 ? chroma_task<16, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
-                         : chroma_task<16, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
-        default: tc == 8 ? chroma_task<32, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
-               : tc == 4 ? chroma_task<32, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum)
-                         : chroma_task<32, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, c0, nc, csum); break;
+            if (a.useChroma) {
+                const intptr_t co = (intptr_t)(sy >> 1) * ccs + (sx >> 1);
+                const uint16_t *cu = P.plane[1].data + co, *cv = P.plane[2].data + co;
+                const int wc = sw >> 1, hc = sh >> 1;
+                const int tc = (wc % 8 == 0 && hc % 8 == 0) ? 8 : (wc % 4 == 0 && hc % 4 == 0) ? 4 : 2;
+                const int cx0 = w.cx0, cy0 = w.cy0;
+                switch (wc) {
+                case 2: chroma_task<2, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum); break;
+                case 4: tc == 4 ? chroma_task<4, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum)
+                                : chroma_task<4, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum); break;
+                case 8: tc == 8 ? chroma_task<8, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum)
+                      : tc == 4 ? chroma_task<8, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum)
+                                : chroma_task<8, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum); break;
+                default: tc == 8 ? chroma_task<16, 8>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum)
+                       : tc == 4 ? chroma_task<16, 4>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum)
+                                 : chroma_task<16, 2>(B, mw, L, cx0, cy0, cu, cv, ccs, hc, ox, oy, c0, nc, csum); break;
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
-    }
 
     // ---- costs, task-local best, block completion ----
     unsigned long long key = ~0ull;
@@ -612,14 +654,14 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int2 tk, const x
         L.key = key;
     }
     __syncthreads();
-    if (L.last) mc_block(a, b, B, mw, L.key, L, wx0, wy0, cx0, cy0);
+    if (L.last) mc_block(a, b, B, mw, L.key, L, w);
     __syncthreads();
 }
 
 // Batched form: a persistent grid; workgroup g runs the tasks the host
 // assigned to it (balanced by estimated cost, heaviest first), so the launch
 // takes no atomics to schedule and leaves no state behind.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_refine_persistent(RefineArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_refine_persistent(RefineArgs a)
 {
     __shared__ RefineLds L;
     lds_init(L, a);
@@ -631,7 +673,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 }
 
 // Plain form: one launch per block shape, one workgroup per task.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_refine_shape(RefineArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_refine_shape(RefineArgs a)
 {
     __shared__ RefineLds L;
     lds_init(L, a);
@@ -653,7 +695,7 @@ struct xg_session {
     hipStream_t stream = nullptr;
     int maxBlocks = 0;
     int numCUs = 256;
-    int persistFactor = 8;             // persistent workgroups per CU (LDS allows 8)
+    int persistFactor = 12;            // persistent workgroups per CU (registers allow 3 waves per SIMD)
     int nregions = 1;                  // spatial partitions of the batched form (XCD-aligned)
 
     xg_picture pics[kPicSlots];
@@ -1414,7 +1456,7 @@ int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t rep
         float bestMs = 1e30f;
         int bestR = s->nregions, bestF = s->persistFactor;
         for (int r : {1, 2, 4, 8})
-            for (int f : {4, 6, 8}) {
+            for (int f : {8, 10, 12}) {
                 s->nregions = r;
                 s->persistFactor = f;
                 const float ms = time_frame(s, cfg, reps, e0, e1, &ok);
