@@ -231,9 +231,14 @@ def search_leg(lib, steps, warmup, cpu_seconds):
     evals = nb * 33 * 33
     out = {"workload": "416x240 10-bit, 390 16x16 blocks, integer full search +/-16, (SAD<<8)+bits*1024",
            "value": round(W * H / (wall_ms / 1e3) / 1e6, 3), "unit": "Mpixel/s", "ms_per_frame": round(wall_ms, 4),
-           "sad_evals_per_s": round(evals / (wall_ms / 1e3), 1), "kernel": "k_full_search_frame",
+           "sad_evals_per_s": round(evals / (wall_ms / 1e3), 1),
+           "kernel": "k_full_search_frame_p<16> (packed v_sad_u16, four adjacent candidates per lane)",
            "kernel_ms": round(k_ms, 4),
-           "kernel_abs_diff_ops_per_s": round(evals * 256 / (k_ms / 1e3), 1)}
+           "kernel_abs_diff_ops_per_s": round(evals * 256 / (k_ms / 1e3), 1),
+           "sad_roofline": {"achieved": round(evals * 256 / (k_ms / 1e3) / 1e12, 3), "peak": SAD_PAIR_PEAK,
+                            "unit": "T abs-diff/s", "frac": round(evals * 256 / (k_ms / 1e3) / 1e12 / SAD_PAIR_PEAK, 4),
+                            "note": "390 workgroups on 256 CUs: one wave of workgroups, so the frame is "
+                                    "latency-bound (window staging, then 297 items of 16 rows each)"}}
     if cpu_seconds > 0:
         from tests.oracle_lib import load_oracle
         oracle = load_oracle()

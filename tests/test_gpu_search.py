@@ -152,3 +152,39 @@ def test_frame_full_search_416x240(oracle):
     inner = g.reshape(H // 16, W // 16, 2)[2:-2, 2:-2].reshape(-1, 2)
     assert np.median(inner[:, 0]) == 12 and np.median(inner[:, 1]) == -8
     lib.xg_session_destroy(s)
+
+
+@pytest.mark.parametrize("bw,bh,rng", [(8, 8, 7), (32, 16, 24), (16, 32, 3), (8, 16, 60), (64, 8, 10), (4, 4, 5),
+                                       (32, 32, 0)])
+def test_frame_full_search_shapes(oracle, bw, bh, rng):
+    """Packed kernel (widths 8, 16, 32) and the general one (4, 64) against the
+    oracle on a 128x96 crop: costs, SADs, MVs and the raster tie order."""
+    import torch
+    lib = capi.load()
+    cur, ref = wl.frame_pair_416x240()
+    W, H, pad = 128, 96, 72
+    cur = np.ascontiguousarray(cur[40:40 + H, 96:96 + W])
+    ref = np.ascontiguousarray(ref[40:40 + H, 96:96 + W])
+    nb = (W // bw) * (H // bh)
+    dc = DevicePicture([cur, cur[::2, ::2].copy(), cur[::2, ::2].copy()], pad=0)
+    dr = DevicePicture([ref, ref[::2, ::2].copy(), ref[::2, ::2].copy()], pad=pad)
+    s = ctypes.c_void_p()
+    assert lib.xg_session_create(ctypes.byref(s), 1024) == 0
+    mv = torch.zeros(2 * nb, dtype=torch.int16, device="cuda")
+    cost = torch.zeros(nb, dtype=torch.int32, device="cuda")
+    sad = torch.zeros(nb, dtype=torch.int32, device="cuda")
+    rc = lib.xg_full_search_frame(s, ctypes.byref(dc.pic.plane[0]), ctypes.byref(dr.pic.plane[0]), bw, bh, rng,
+                                  1024, 64, mv.data_ptr(), cost.data_ptr(), sad.data_ptr())
+    assert rc == 0 and lib.xg_session_sync(s) == 0
+    refp = np.ascontiguousarray(np.pad(ref, pad, mode="edge"))
+    emx = np.zeros(nb, np.int16)
+    emy = np.zeros(nb, np.int16)
+    ec = np.zeros(nb, np.uint32)
+    es = np.zeros(nb, np.uint32)
+    oracle.or_FullSearchFrame(ptr(cur), W, ptr(refp, pad * (W + 2 * pad) + pad), W + 2 * pad, W, H, bw, bh, rng,
+                              1024, 64, 0, nb, ptr(emx), ptr(emy), ptr(ec), ptr(es))
+    g = mv.cpu().numpy().reshape(nb, 2)
+    assert np.array_equal(g[:, 0], emx) and np.array_equal(g[:, 1], emy)
+    assert np.array_equal(cost.cpu().numpy().view(np.uint32), ec)
+    assert np.array_equal(sad.cpu().numpy().view(np.uint32), es)
+    lib.xg_session_destroy(s)

@@ -117,6 +117,128 @@ __global__ void __launch_bounds__(256) k_full_search_frame(FsFrameArgs a)
     }
 }
 
+// Packed frame form for block widths 8, 16 and 32.  Each lane costs four
+// horizontally adjacent candidates (4q .. 4q + 3) of one search row dyi: a
+// window row is read as BW/2 + 2 dwords D (sample pairs from an even column),
+// F = D realigned by one sample, and v_sad_u16 takes two absolute
+// differences per instruction against the packed current row C.  Window rows
+// are ws samples (ws even, >= nx + BW + 6, so the reads of the last quad stay
+// inside the row); the block follows at the next 16-byte boundary, bh rows of
+// BW/2 dwords.  Same cost, window and
+// tie order as k_full_search_frame.
+template <int BW>
+__global__ void __launch_bounds__(1024) k_full_search_frame_p(FsFrameArgs a, int ws)
+{
+    constexpr int ND = BW / 2;                      // dword pairs per block row
+    extern __shared__ uint32_t smem32[];
+    __shared__ unsigned long long red[16];
+    const int b = blockIdx.x;
+    const int x = (b % a.nbx) * BW, y = (b / a.nbx) * a.bh;
+    const int mnx = (int16_t)(-x - a.ctu - 8), mny = (int16_t)(-y - a.ctu - 8);
+    const int mxx = (int16_t)(a.fw + (a.ctu - BW + 8) - x), mxy = (int16_t)(a.fh + (a.ctu - a.bh + 8) - y);
+    const int x0 = max(-a.range, mnx), x1 = min(a.range, mxx);
+    const int y0 = max(-a.range, mny), y1 = min(a.range, mxy);
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
+    const int rows = ny + a.bh - 1, cols = nx + BW - 1;
+    uint16_t *win = reinterpret_cast<uint16_t *>(smem32);
+    uint32_t *curL = smem32 + (((rows * (ws >> 1)) + 3) & ~3);    // 16-byte aligned
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    {   // window: a wave per row, lanes along it, four rows of loads in flight per lane
+        const uint16_t *src = a.ref + (intptr_t)(y + y0) * a.rs + (x + x0);
+        for (int r0 = wave; r0 < rows; r0 += 4 * nw) {
+            for (int c0 = lane; c0 < cols; c0 += 64) {
+                uint16_t v[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = src[(intptr_t)min(r0 + k * nw, rows - 1) * a.rs + c0];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (r0 + k * nw < rows) win[(r0 + k * nw) * ws + c0] = v[k];
+            }
+        }
+        const uint16_t *cp = a.cur + (intptr_t)y * a.cs + x;
+        for (int i = tid; i < a.bh * ND; i += blockDim.x)
+            curL[i] = *reinterpret_cast<const uint32_t *>(cp + (intptr_t)(i / ND) * a.cs + 2 * (i % ND));
+    }
+    __syncthreads();
+    const int nq = (nx + 3) >> 2, items = ny * nq;
+    const uint32_t *w32 = smem32;
+    unsigned long long best = ~0ull;
+    for (int it = tid; it < items; it += blockDim.x) {
+        const int dyi = it / nq, q = it - dyi * nq;
+        const uint32_t *rp = w32 + ((dyi * ws) >> 1) + 2 * q;
+        uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+#pragma unroll 2
+        for (int i = 0; i < a.bh; i++) {
+            uint32_t D[ND + 2], F[ND + 1], C[ND];
+#pragma unroll
+            for (int j = 0; j < ND + 2; j++) D[j] = rp[i * (ws >> 1) + j];
+            const uint4 *c4 = reinterpret_cast<const uint4 *>(curL + i * ND);
+            if constexpr (ND >= 4) {
+#pragma unroll
+                for (int j = 0; j < ND / 4; j++) {
+                    const uint4 v = c4[j];
+                    C[4 * j] = v.x;
+                    C[4 * j + 1] = v.y;
+                    C[4 * j + 2] = v.z;
+                    C[4 * j + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < ND; j++) C[j] = curL[i * ND + j];
+            }
+#pragma unroll
+            for (int j = 0; j < ND + 1; j++) F[j] = align16(D[j + 1], D[j], 16);    // pairs from odd samples
+#pragma unroll
+            for (int j = 0; j < ND; j++) {
+                acc0 = __builtin_amdgcn_sad_u16(D[j], C[j], acc0);
+                acc1 = __builtin_amdgcn_sad_u16(F[j], C[j], acc1);
+                acc2 = __builtin_amdgcn_sad_u16(D[j + 1], C[j], acc2);
+                acc3 = __builtin_amdgcn_sad_u16(F[j + 1], C[j], acc3);
+            }
+        }
+        const int my = y0 + dyi;
+        const uint32_t by = eg_bits(4 * my);
+        const uint32_t acc[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int mx = x0 + 4 * q + k;
+            if (mx <= x1) {
+                const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * mx) + by) * a.lambda;
+                const unsigned long long key = ((unsigned long long)cost << 32) | (uint32_t)(dyi * nx + 4 * q + k);
+                best = key < best ? key : best;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const unsigned long long o = (unsigned long long)__shfl_xor((long long)best, m, 64);
+        best = o < best ? o : best;
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    if (wave == 0) {
+        best = red[0];
+        for (int i = 1; i < nw; i++) best = red[i] < best ? red[i] : best;
+        const int idx = (int)(uint32_t)best;
+        const int mx = x0 + idx % nx, my = y0 + idx / nx;
+        // the winner's SAD, recomputed by one wave
+        uint32_t sad = 0;
+        const uint16_t *wr = win + (my - y0) * ws + (mx - x0);
+        const uint16_t *cr = reinterpret_cast<const uint16_t *>(curL);
+        for (int e = lane; e < a.bh * BW; e += 64) {
+            const int r = e / BW, c = e % BW;
+            sad += (uint32_t)abs((int)cr[e] - (int)wr[r * ws + c]);
+        }
+        sad = (uint32_t)group_sum<64>((int)sad);
+        if (lane == 0) {
+            a.bestMv[2 * b] = (int16_t)(4 * mx);
+            a.bestMv[2 * b + 1] = (int16_t)(4 * my);
+            a.bestCost[b] = (uint32_t)(best >> 32);
+            a.bestSad[b] = sad;
+        }
+    }
+}
+
 // Per-call form: packed window and block already in global memory.
 __global__ void __launch_bounds__(256) k_full_search_one(const uint16_t *gwin, int ws, int wh, const uint16_t *gcur,
                                                          int bw, int bh, int x0, int x1, int y0, int y1,
@@ -455,6 +577,11 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
         set_error(XG_ERR_ARG, "xg_full_search_frame: window exceeds the LDS budget");
         return XG_ERR_ARG;
     }
+    // packed form: widths 8, 16, 32 with a dword-aligned current plane
+    const int nxm = 2 * range + 1, wsP = (nxm + bw + 7) & ~1;
+    const size_t ldsP = ((size_t)(2 * range + bh) * wsP + (size_t)bw * bh) * 2 + 16;
+    const bool packed = (bw == 8 || bw == 16 || bw == 32) && ((uintptr_t)cur->data & 3) == 0 && (cur->stride & 1) == 0 &&
+                        ldsP <= 64 * 1024;
     FsFrameArgs a;
     a.cur = cur->data; a.cs = cur->stride;
     a.ref = ref->data; a.rs = ref->stride;
@@ -464,7 +591,19 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
     hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
     const bool timed = session_timer_begin(s, 3, ev);
-    k_full_search_frame<<<nbx * nby, 256, lds, st>>>(a);
+    if (packed) {
+        // every (row, quad) item of the largest window at once, 64 to 1024 threads
+        const int items = nxm * ((nxm + 3) >> 2);
+        const int threads = std::min(1024, std::max(64, (items + 63) & ~63));
+        if (bw == 8)
+            k_full_search_frame_p<8><<<nbx * nby, threads, ldsP, st>>>(a, wsP);
+        else if (bw == 16)
+            k_full_search_frame_p<16><<<nbx * nby, threads, ldsP, st>>>(a, wsP);
+        else
+            k_full_search_frame_p<32><<<nbx * nby, threads, ldsP, st>>>(a, wsP);
+    } else {
+        k_full_search_frame<<<nbx * nby, 256, lds, st>>>(a);
+    }
     if (timed) session_timer_end(s, 3, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
