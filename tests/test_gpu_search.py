@@ -188,3 +188,60 @@ def test_frame_full_search_shapes(oracle, bw, bh, rng):
     assert np.array_equal(cost.cpu().numpy().view(np.uint32), ec)
     assert np.array_equal(sad.cpu().numpy().view(np.uint32), es)
     lib.xg_session_destroy(s)
+
+
+def test_search_blocks_every_width(oracle):
+    """The session search of the serving path (packed core) for block widths
+    4..64 and heights 4..64 around non-zero predictors, against the oracle's
+    Xg26xMotionEstFull; the searched MVs are read back through a one-candidate
+    (0, 0) refinement."""
+    from xin26x_amd.device import blocks_to_struct, refine_cfg
+    lib = capi.load()
+    cur, ref = wl.frame_pair_416x240()
+    W, H, pad = 256, 192, 80
+    cur = np.ascontiguousarray(cur[24:24 + H, 80:80 + W])
+    ref = np.ascontiguousarray(ref[24:24 + H, 80:80 + W])
+    chroma = lambda p: np.ascontiguousarray(p[::2, ::2])
+    dc = DevicePicture([cur, chroma(cur), chroma(cur)], pad)
+    dr = DevicePicture([ref, chroma(ref), chroma(ref)], pad)
+    pred = DevicePicture.empty(W, H)
+    rng = np.random.default_rng(5)
+    blocks = []
+    for w in (4, 8, 16, 32, 64):
+        for h in (4, 8, 16, 32, 64):
+            for _ in range(2):
+                x = int(rng.integers(0, (W - w) // 4 + 1)) * 4
+                y = int(rng.integers(0, (H - h) // 4 + 1)) * 4
+                blocks.append((x, y, w, h, 0, 0, int(rng.integers(-300, 301)), int(rng.integers(-300, 301))))
+    n = len(blocks)
+    s = ctypes.c_void_p()
+    assert lib.xg_session_create(ctypes.byref(s), n) == 0
+    try:
+        assert lib.xg_session_set_picture(s, 0, dc.ref()) == 0
+        assert lib.xg_session_set_picture(s, 1, dr.ref()) == 0
+        barr = blocks_to_struct(blocks)
+        assert lib.xg_session_set_blocks(s, barr.ctypes.data, n) == 0
+        assert lib.xg_search_blocks(s, 16, 1024, 64) == 0
+        cfg = refine_cfg([(0, 0)], wl.LAMBDA_REFINE)
+        assert lib.xg_refine_frac(s, ctypes.byref(cfg), pred.ref(), capi.MODE_BATCHED) == 0
+        best = np.zeros(n * 4, np.int32)
+        assert lib.xg_session_read(s, None, best.ctypes.data) == 0
+    finally:
+        lib.xg_session_destroy(s)
+    got = best.reshape(n, 4)[:, :2]
+    refp = np.ascontiguousarray(np.pad(ref, pad, mode="edge"))
+    rs = W + 2 * pad
+    for i, (x, y, w, h, _, _, mvpx, mvpy) in enumerate(blocks):
+        m = OrMe()
+        m.width, m.height = w, h
+        m.minMvX, m.minMvY = -x - 72, -y - 72
+        m.maxMvX, m.maxMvY = W + 72 - w - x, H + 72 - h - y
+        m.predMvX, m.predMvY = mvpx >> 2, mvpy >> 2
+        m.searchWidth = m.searchHeight = 16
+        m.sadLambda = 1024
+        m.input = cur.ctypes.data + 2 * (y * W + x)
+        m.inputStride = W
+        m.ref = refp.ctypes.data + 2 * ((y + pad) * rs + x + pad)
+        m.refStride = rs
+        oracle.or_MotionEstFull(ctypes.byref(m))
+        assert (got[i, 0], got[i, 1]) == (4 * m.bestMvX, 4 * m.bestMvY), (i, blocks[i])

@@ -117,63 +117,63 @@ __global__ void __launch_bounds__(256) k_full_search_frame(FsFrameArgs a)
     }
 }
 
-// Packed frame form for block widths 8, 16 and 32.  Each lane costs four
-// horizontally adjacent candidates (4q .. 4q + 3) of one search row dyi: a
-// window row is read as BW/2 + 2 dwords D (sample pairs from an even column),
-// F = D realigned by one sample, and v_sad_u16 takes two absolute
-// differences per instruction against the packed current row C.  Window rows
-// are ws samples (ws even, >= nx + BW + 6, so the reads of the last quad stay
-// inside the row); the block follows at the next 16-byte boundary, bh rows of
-// BW/2 dwords.  Same cost, window and
-// tie order as k_full_search_frame.
+// ---------------------------------------------------------------------------
+// Packed integer search core (block widths 4..64, powers of two)
+//
+// Each lane costs four horizontally adjacent candidates (4q .. 4q + 3) of one
+// search row dyi: a window row is read as BW/2 + 2 dwords D (sample pairs from
+// an even column), F = D realigned by one sample, and v_sad_u16 takes two
+// absolute differences per instruction against the packed current row C.
+// Window rows are ws samples (ws even, >= nx + BW + 6, so the reads of the
+// last quad stay inside the row); the block is bh rows of BW/2 dwords at a
+// 16-byte boundary.  Cost = (SAD << 8) + (EG(4 mx - pmx) + EG(4 my - pmy)) *
+// lambda; key = cost << 32 | raster index, as in full_search_core.
+// ---------------------------------------------------------------------------
+
+// rows x cols samples from src (row pitch rs) into win (row pitch ws): a wave
+// per row, lanes along it, four rows of loads in flight per lane.
+__device__ __forceinline__ void stage_rows(uint16_t *win, int ws, const uint16_t *src, intptr_t rs, int rows, int cols)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int r0 = wave; r0 < rows; r0 += 4 * nw) {
+        for (int c0 = lane; c0 < cols; c0 += 64) {
+            uint16_t v[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[k] = src[(intptr_t)min(r0 + k * nw, rows - 1) * rs + c0];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (r0 + k * nw < rows) win[(r0 + k * nw) * ws + c0] = v[k];
+        }
+    }
+}
+
+// The block's rows as dword pairs (x even, plane dword aligned).
+__device__ __forceinline__ void stage_cur(uint32_t *curL, const uint16_t *cp, intptr_t cs, int bw, int bh)
+{
+    const int nd = bw >> 1;
+    for (int i = threadIdx.x; i < bh * nd; i += blockDim.x)
+        curL[i] = *reinterpret_cast<const uint32_t *>(cp + (intptr_t)(i / nd) * cs + 2 * (i % nd));
+}
+
 template <int BW>
-__global__ void __launch_bounds__(1024) k_full_search_frame_p(FsFrameArgs a, int ws)
+__device__ __forceinline__ unsigned long long packed_items(const uint32_t *w32, int ws, const uint32_t *curL, int bh,
+                                                           int x0, int x1, int y0, int nx, int ny, uint32_t lambda,
+                                                           int pmx, int pmy)
 {
     constexpr int ND = BW / 2;                      // dword pairs per block row
-    extern __shared__ uint32_t smem32[];
-    __shared__ unsigned long long red[16];
-    const int b = blockIdx.x;
-    const int x = (b % a.nbx) * BW, y = (b / a.nbx) * a.bh;
-    const int mnx = (int16_t)(-x - a.ctu - 8), mny = (int16_t)(-y - a.ctu - 8);
-    const int mxx = (int16_t)(a.fw + (a.ctu - BW + 8) - x), mxy = (int16_t)(a.fh + (a.ctu - a.bh + 8) - y);
-    const int x0 = max(-a.range, mnx), x1 = min(a.range, mxx);
-    const int y0 = max(-a.range, mny), y1 = min(a.range, mxy);
-    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
-    const int rows = ny + a.bh - 1, cols = nx + BW - 1;
-    uint16_t *win = reinterpret_cast<uint16_t *>(smem32);
-    uint32_t *curL = smem32 + (((rows * (ws >> 1)) + 3) & ~3);    // 16-byte aligned
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    {   // window: a wave per row, lanes along it, four rows of loads in flight per lane
-        const uint16_t *src = a.ref + (intptr_t)(y + y0) * a.rs + (x + x0);
-        for (int r0 = wave; r0 < rows; r0 += 4 * nw) {
-            for (int c0 = lane; c0 < cols; c0 += 64) {
-                uint16_t v[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) v[k] = src[(intptr_t)min(r0 + k * nw, rows - 1) * a.rs + c0];
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (r0 + k * nw < rows) win[(r0 + k * nw) * ws + c0] = v[k];
-            }
-        }
-        const uint16_t *cp = a.cur + (intptr_t)y * a.cs + x;
-        for (int i = tid; i < a.bh * ND; i += blockDim.x)
-            curL[i] = *reinterpret_cast<const uint32_t *>(cp + (intptr_t)(i / ND) * a.cs + 2 * (i % ND));
-    }
-    __syncthreads();
     const int nq = (nx + 3) >> 2, items = ny * nq;
-    const uint32_t *w32 = smem32;
     unsigned long long best = ~0ull;
-    for (int it = tid; it < items; it += blockDim.x) {
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
         const int dyi = it / nq, q = it - dyi * nq;
         const uint32_t *rp = w32 + ((dyi * ws) >> 1) + 2 * q;
         uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
 #pragma unroll 2
-        for (int i = 0; i < a.bh; i++) {
+        for (int i = 0; i < bh; i++) {
             uint32_t D[ND + 2], F[ND + 1], C[ND];
 #pragma unroll
             for (int j = 0; j < ND + 2; j++) D[j] = rp[i * (ws >> 1) + j];
-            const uint4 *c4 = reinterpret_cast<const uint4 *>(curL + i * ND);
             if constexpr (ND >= 4) {
+                const uint4 *c4 = reinterpret_cast<const uint4 *>(curL + i * ND);
 #pragma unroll
                 for (int j = 0; j < ND / 4; j++) {
                     const uint4 v = c4[j];
@@ -197,28 +197,71 @@ __global__ void __launch_bounds__(1024) k_full_search_frame_p(FsFrameArgs a, int
             }
         }
         const int my = y0 + dyi;
-        const uint32_t by = eg_bits(4 * my);
+        const uint32_t by = eg_bits(4 * my - pmy);
         const uint32_t acc[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int mx = x0 + 4 * q + k;
             if (mx <= x1) {
-                const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * mx) + by) * a.lambda;
+                const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * mx - pmx) + by) * lambda;
                 const unsigned long long key = ((unsigned long long)cost << 32) | (uint32_t)(dyi * nx + 4 * q + k);
                 best = key < best ? key : best;
             }
         }
     }
+    return best;
+}
+
+// Minimum over the workgroup (up to 16 waves); every thread gets it.
+__device__ __forceinline__ unsigned long long block_min_u64_16(unsigned long long v, unsigned long long *red)
+{
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
-        const unsigned long long o = (unsigned long long)__shfl_xor((long long)best, m, 64);
-        best = o < best ? o : best;
+        const unsigned long long o = (unsigned long long)__shfl_xor((long long)v, m, 64);
+        v = o < v ? o : v;
     }
-    if (lane == 0) red[wave] = best;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (wave == 0) {
-        best = red[0];
-        for (int i = 1; i < nw; i++) best = red[i] < best ? red[i] : best;
+    unsigned long long r = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) r = red[i] < r ? red[i] : r;
+    __syncthreads();
+    return r;
+}
+
+// Window rows needed by the packed core for nx candidates per row.
+__host__ __device__ inline int packed_ws(int nx, int bw) { return (nx + bw + 7) & ~1; }
+
+// Threads for a packed search over an n x n candidate window: every item at once, 64..1024.
+static inline int packed_threads(int n)
+{
+    const int items = n * ((n + 3) >> 2);
+    return std::min(1024, std::max(64, (items + 63) & ~63));
+}
+
+// Packed frame form for block widths 8, 16 and 32; same cost, window and tie
+// order as k_full_search_frame.
+template <int BW>
+__global__ void __launch_bounds__(1024) k_full_search_frame_p(FsFrameArgs a, int ws)
+{
+    extern __shared__ uint32_t smem32[];
+    __shared__ unsigned long long red[16];
+    const int b = blockIdx.x;
+    const int x = (b % a.nbx) * BW, y = (b / a.nbx) * a.bh;
+    const int mnx = (int16_t)(-x - a.ctu - 8), mny = (int16_t)(-y - a.ctu - 8);
+    const int mxx = (int16_t)(a.fw + (a.ctu - BW + 8) - x), mxy = (int16_t)(a.fh + (a.ctu - a.bh + 8) - y);
+    const int x0 = max(-a.range, mnx), x1 = min(a.range, mxx);
+    const int y0 = max(-a.range, mny), y1 = min(a.range, mxy);
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
+    const int rows = ny + a.bh - 1;
+    uint16_t *win = reinterpret_cast<uint16_t *>(smem32);
+    uint32_t *curL = smem32 + (((rows * (ws >> 1)) + 3) & ~3);    // 16-byte aligned
+    stage_rows(win, ws, a.ref + (intptr_t)(y + y0) * a.rs + (x + x0), a.rs, rows, nx + BW - 1);
+    stage_cur(curL, a.cur + (intptr_t)y * a.cs + x, a.cs, BW, a.bh);
+    __syncthreads();
+    unsigned long long best = packed_items<BW>(smem32, ws, curL, a.bh, x0, x1, y0, nx, ny, a.lambda, 0, 0);
+    best = block_min_u64_16(best, red);
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
         const int idx = (int)(uint32_t)best;
         const int mx = x0 + idx % nx, my = y0 + idx / nx;
         // the winner's SAD, recomputed by one wave
@@ -364,11 +407,13 @@ static inline int clampi_h(int v, int lo, int hi) { return std::min(std::max(v, 
 // Integer search of every block of a session (one workgroup per block): the
 // window +/-range around the rounded predictor, clipped to the CU MV window,
 // staged in LDS with the block; result written back as the block's base MV.
-__global__ void __launch_bounds__(256) k_search_blocks(const xg_picture *pics, xg_block *blocks, int range,
-                                                       uint32_t lambda, int ctu)
+// Integer search of every session block around its predictor (packed core;
+// ws: window row pitch for the widest block).
+__global__ void __launch_bounds__(1024) k_search_blocks(const xg_picture *pics, xg_block *blocks, int range,
+                                                        uint32_t lambda, int ctu, int ws)
 {
-    extern __shared__ uint16_t smem[];
-    __shared__ unsigned long long red[4];
+    extern __shared__ uint32_t smem32[];
+    __shared__ unsigned long long red[16];
     const int b = blockIdx.x;
     xg_block B = blocks[b];
     const xg_picture &P = pics[B.pic], &R = pics[B.ref];
@@ -380,18 +425,25 @@ __global__ void __launch_bounds__(256) k_search_blocks(const xg_picture *pics, x
     const int cx = min(max(((B.mvpx >> 2) + 2) >> 2, mnx), mxx), cy = min(max(((B.mvpy >> 2) + 2) >> 2, mny), mxy);
     const int x0 = max(cx - range, mnx), x1 = min(cx + range, mxx);
     const int y0 = max(cy - range, mny), y1 = min(cy + range, mxy);
-    const int ws = x1 - x0 + B.w, wh = y1 - y0 + B.h;
-    uint16_t *win = smem, *cur = smem + ws * wh;
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, rows = ny + B.h - 1;
+    uint16_t *win = reinterpret_cast<uint16_t *>(smem32);
+    uint32_t *curL = smem32 + (((rows * (ws >> 1)) + 3) & ~3);    // 16-byte aligned
     const intptr_t rs = R.plane[0].stride, cs = P.plane[0].stride;
-    const uint16_t *rp = R.plane[0].data + (intptr_t)(B.y + y0) * rs + (B.x + x0);
-    for (int i = threadIdx.x; i < ws * wh; i += blockDim.x) win[i] = rp[(intptr_t)(i / ws) * rs + (i % ws)];
-    const uint16_t *cp = P.plane[0].data + (intptr_t)B.y * cs + B.x;
-    for (int i = threadIdx.x; i < B.w * B.h; i += blockDim.x) cur[i] = cp[(intptr_t)(i / B.w) * cs + (i % B.w)];
+    stage_rows(win, ws, R.plane[0].data + (intptr_t)(B.y + y0) * rs + (B.x + x0), rs, rows, nx + B.w - 1);
+    stage_cur(curL, P.plane[0].data + (intptr_t)B.y * cs + B.x, cs, B.w, B.h);
     __syncthreads();
-    const unsigned long long k = full_search_core(win, ws, cur, B.w, B.h, x0, x1, y0, y1, lambda, B.mvpx >> 2,
-                                                  B.mvpy >> 2, red);
+    const int pmx = B.mvpx >> 2, pmy = B.mvpy >> 2;
+    unsigned long long k;
+    switch (B.w) {
+    case 4: k = packed_items<4>(smem32, ws, curL, B.h, x0, x1, y0, nx, ny, lambda, pmx, pmy); break;
+    case 8: k = packed_items<8>(smem32, ws, curL, B.h, x0, x1, y0, nx, ny, lambda, pmx, pmy); break;
+    case 16: k = packed_items<16>(smem32, ws, curL, B.h, x0, x1, y0, nx, ny, lambda, pmx, pmy); break;
+    case 32: k = packed_items<32>(smem32, ws, curL, B.h, x0, x1, y0, nx, ny, lambda, pmx, pmy); break;
+    default: k = packed_items<64>(smem32, ws, curL, B.h, x0, x1, y0, nx, ny, lambda, pmx, pmy); break;
+    }
+    k = block_min_u64_16(k, red);
     if (threadIdx.x == 0) {
-        const int i = (int)(uint32_t)k, nx = x1 - x0 + 1;
+        const int i = (int)(uint32_t)k;
         B.mvx = 16 * (x0 + i % nx);
         B.mvy = 16 * (y0 + i / nx);
         blocks[b].mvx = B.mvx;
@@ -545,17 +597,26 @@ int32_t xg_search_blocks(xg_session *s, int32_t range, uint32_t sadLambda, int32
             set_error(XG_ERR_ARG, "xg_search_blocks: block %d outside its picture or reference under-padded", i);
             return XG_ERR_ARG;
         }
+        // the packed core reads the current block as dword pairs
+        if ((B.w & (B.w - 1)) || B.w < 4 || B.w > 64 || B.h < 1 || B.h > 64 || (B.x & 1) || (P.plane[0].stride & 1) ||
+            ((uintptr_t)P.plane[0].data & 3)) {
+            set_error(XG_ERR_ARG, "xg_search_blocks: block %d: width must be 4..64 (a power of two) at an even x "
+                      "in a dword-aligned plane", i);
+            return XG_ERR_ARG;
+        }
         maxw = std::max(maxw, (int)B.w);
         maxh = std::max(maxh, (int)B.h);
     }
-    const size_t lds = ((size_t)(2 * range + maxw) * (2 * range + maxh) + (size_t)maxw * maxh) * 2;
+    const int ws = packed_ws(2 * range + 1, maxw);
+    const size_t lds = ((size_t)(2 * range + maxh) * ws + (size_t)maxw * maxh) * 2 + 16;
     if (lds > 64 * 1024) {
         set_error(XG_ERR_ARG, "xg_search_blocks: window exceeds the LDS budget");
         return XG_ERR_ARG;
     }
     int ev[2];
     const bool timed = session_timer_begin(s, 3, ev);
-    k_search_blocks<<<v.nblocks, 256, lds, v.stream>>>(v.dPics, v.dBlocks, range, sadLambda, ctuSize);
+    k_search_blocks<<<v.nblocks, packed_threads(2 * range + 1), lds, v.stream>>>(v.dPics, v.dBlocks, range,
+                                                                                sadLambda, ctuSize, ws);
     if (timed) session_timer_end(s, 3, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
@@ -578,7 +639,7 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
         return XG_ERR_ARG;
     }
     // packed form: widths 8, 16, 32 with a dword-aligned current plane
-    const int nxm = 2 * range + 1, wsP = (nxm + bw + 7) & ~1;
+    const int nxm = 2 * range + 1, wsP = packed_ws(nxm, bw);
     const size_t ldsP = ((size_t)(2 * range + bh) * wsP + (size_t)bw * bh) * 2 + 16;
     const bool packed = (bw == 8 || bw == 16 || bw == 32) && ((uintptr_t)cur->data & 3) == 0 && (cur->stride & 1) == 0 &&
                         ldsP <= 64 * 1024;
@@ -592,9 +653,7 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
     int ev[2];
     const bool timed = session_timer_begin(s, 3, ev);
     if (packed) {
-        // every (row, quad) item of the largest window at once, 64 to 1024 threads
-        const int items = nxm * ((nxm + 3) >> 2);
-        const int threads = std::min(1024, std::max(64, (items + 63) & ~63));
+        const int threads = packed_threads(nxm);
         if (bw == 8)
             k_full_search_frame_p<8><<<nbx * nby, threads, ldsP, st>>>(a, wsP);
         else if (bw == 16)
