@@ -40,7 +40,9 @@ namespace xg {
 
 constexpr int kPB = 16;                          // block side of the pipeline grid
 constexpr int kPipeMaxRange = 64;
-constexpr int kPipeWinW = 2 * kPipeMaxRange + kPB + 10;   // 154: window + over-read of the last candidate quad
+constexpr int kPipeQuads = 3;                              // candidate quads per lane in the full search
+// window row: 2 * range + 16 samples plus the over-read of the last lane's quads
+constexpr int kPipeWinW = 2 * kPipeMaxRange + kPB + 4 * kPipeQuads + 2;   // 158
 constexpr int kPipeWinH = 2 * kPipeMaxRange + kPB;        // 144
 
 struct PipeArgs {
@@ -87,6 +89,11 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long k)
 // ---------------------------------------------------------------------------
 // 1. Full search
 // ---------------------------------------------------------------------------
+// Each lane costs NQ quads (4 * NQ horizontally adjacent candidates) of one
+// search row: the window row's dwords D and their one-sample realignment F are
+// shared by all of them, so the loads and v_alignbit per v_sad_u16 fall as NQ
+// grows.
+template <int NQ>
 __global__ void __launch_bounds__(256) k_pipe_full(PipeArgs a)
 {
     __shared__ __align__(16) uint16_t win[kPipeWinH * kPipeWinW];
@@ -125,18 +132,21 @@ __global__ void __launch_bounds__(256) k_pipe_full(PipeArgs a)
                                                             2 * (tid & 7));
     }
     __syncthreads();
-    const int nq = (nx + 3) >> 2, items = ny * nq;
+    constexpr int ND = 2 * NQ + 8;                  // window dwords per row read by one lane
+    const int ng = (nx + 4 * NQ - 1) / (4 * NQ), items = ny * ng;
     const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win);
     unsigned long long best = ~0ull;
     for (int it = tid; it < items; it += 256) {
-        const int dyi = it / nq, q = it - dyi * nq;
-        const uint32_t *rp = w32 + ((dyi * kPipeWinW) >> 1) + 2 * q;
-        uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-#pragma unroll 4
-        for (int i = 0; i < kPB; i++) {
-            uint32_t D[10], F[9], C[8];
+        const int dyi = it / ng, g = it - dyi * ng;
+        const uint32_t *rp = w32 + ((dyi * kPipeWinW) >> 1) + 2 * NQ * g;
+        uint32_t acc[4 * NQ];
 #pragma unroll
-            for (int j = 0; j < 10; j++) D[j] = rp[i * (kPipeWinW / 2) + j];
+        for (int k = 0; k < 4 * NQ; k++) acc[k] = 0;
+#pragma unroll 2
+        for (int i = 0; i < kPB; i++) {
+            uint32_t D[ND], F[ND - 1], C[8];
+#pragma unroll
+            for (int j = 0; j < ND; j++) D[j] = rp[i * (kPipeWinW / 2) + j];
             const uint4 *c4 = reinterpret_cast<const uint4 *>(curL + i * (kPB / 2));
 #pragma unroll
             for (int j = 0; j < 2; j++) {
@@ -147,24 +157,25 @@ __global__ void __launch_bounds__(256) k_pipe_full(PipeArgs a)
                 C[4 * j + 3] = v.w;
             }
 #pragma unroll
-            for (int j = 0; j < 9; j++) F[j] = align16(D[j + 1], D[j], 16);    // pairs starting at odd samples
+            for (int j = 0; j < ND - 1; j++) F[j] = align16(D[j + 1], D[j], 16);    // pairs starting at odd samples
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                acc0 = sad2(D[j], C[j], acc0);
-                acc1 = sad2(F[j], C[j], acc1);
-                acc2 = sad2(D[j + 1], C[j], acc2);
-                acc3 = sad2(F[j + 1], C[j], acc3);
-            }
+            for (int t = 0; t < NQ; t++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    acc[4 * t] = sad2(D[2 * t + j], C[j], acc[4 * t]);
+                    acc[4 * t + 1] = sad2(F[2 * t + j], C[j], acc[4 * t + 1]);
+                    acc[4 * t + 2] = sad2(D[2 * t + j + 1], C[j], acc[4 * t + 2]);
+                    acc[4 * t + 3] = sad2(F[2 * t + j + 1], C[j], acc[4 * t + 3]);
+                }
         }
         const int my = y0 + dyi;
         const uint32_t by = eg_bits(4 * my);
-        const uint32_t acc[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int mx = x0 + 4 * q + k;
-            if (mx <= x1) {
-                const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * mx) + by) * a.lambda;
-                best = min_u64(best, ((unsigned long long)cost << 32) | (uint32_t)(dyi * nx + 4 * q + k));
+        for (int k = 0; k < 4 * NQ; k++) {
+            const int ox = 4 * NQ * g + k;
+            if (x0 + ox <= x1) {
+                const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * (x0 + ox)) + by) * a.lambda;
+                best = min_u64(best, ((unsigned long long)cost << 32) | (uint32_t)(dyi * nx + ox));
             }
         }
     }
@@ -511,7 +522,7 @@ static bool enqueue_pipe(const PipeState *p, hipStream_t st, xg_session *s, bool
 {
     int ev[2];
     const bool t3 = timed && session_timer_begin(s, 3, ev);
-    k_pipe_full<<<2 * p->nb, 256, 0, st>>>(p->args);
+    k_pipe_full<kPipeQuads><<<2 * p->nb, 256, 0, st>>>(p->args);
     if (t3) session_timer_end(s, 3, ev);
     k_pipe_dia<<<2 * p->nb, 64, 0, st>>>(p->args);
     k_pipe_select<<<(p->nb + 255) / 256, 256, 0, st>>>(p->args);
