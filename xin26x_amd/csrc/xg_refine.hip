@@ -711,6 +711,8 @@ struct xg_session {
     xg_block *dBlocks = nullptr;
     int nblocks = 0;
     uint64_t blocksVersion = 1;
+    uint64_t stateVersion = 1;         // bumped by every picture, prediction and block setter
+    uint64_t validKey = 0;             // (state, prediction argument, CTU size) last validated
     int lastNcand = 0;                 // ncand of the last refinement
 
     // per-shape work queues
@@ -1025,6 +1027,29 @@ static bool validate_blocks(xg_session *s, bool pred, int ctu)
     return true;
 }
 
+// Prediction picture of every block's slot (when given) and block validation.
+// Both depend only on the session state, the prediction argument and the CTU
+// size, so a call with all three as at the last successful check skips the
+// per-block loops (they cost about as much host time as a 1080p frame's kernel).
+static bool prepare_blocks(xg_session *s, const xg_picture *pred, int ctu)
+{
+    uint64_t key = mix(mix(s->stateVersion, (uint64_t)ctu), pred ? 1 : 0);
+    if (pred) {
+        const unsigned char *bytes = reinterpret_cast<const unsigned char *>(pred);
+        for (size_t i = 0; i < sizeof(xg_picture); i++) key = mix(key, bytes[i]);
+    }
+    if (key == s->validKey) return true;
+    s->validKey = 0;
+    if (pred)   // one prediction picture for every current-picture slot in use
+        for (int i = 0; i < s->nblocks; i++) {
+            s->preds[s->blocks[i].pic] = *pred;
+            s->predSet[s->blocks[i].pic] = true;
+        }
+    if (!validate_blocks(s, true, ctu)) return false;
+    s->validKey = key;
+    return true;
+}
+
 static bool fill_args(xg_session *s, const xg_refine_cfg *cfg, RefineArgs &ra, bool doMc)
 {
     int16_t off[2 * XG_MAX_CAND] = {};
@@ -1207,6 +1232,7 @@ int32_t xg_session_set_picture(xg_session *s, int32_t slot, const xg_picture *pi
     }
     s->pics[slot] = *pic;
     s->picSet[slot] = true;
+    s->stateVersion++;
     if (!stage_upload(s, s->dPics + slot, pic, sizeof(xg_picture))) return XG_ERR_HIP;
     return XG_OK;
 }
@@ -1219,6 +1245,7 @@ int32_t xg_session_set_pred(xg_session *s, int32_t slot, const xg_picture *pred)
     }
     s->preds[slot] = *pred;
     s->predSet[slot] = true;
+    s->stateVersion++;
     return XG_OK;
 }
 
@@ -1240,6 +1267,7 @@ int32_t xg_session_set_blocks(xg_session *s, const xg_block *blocks, int32_t n)
     s->blocks.assign(blocks, blocks + n);
     s->nblocks = n;
     s->blocksVersion++;
+    s->stateVersion++;
     if (n && (!XG_HIP(hipMemsetAsync(s->dDone, 0, sizeof(int) * (size_t)n, s->stream)) ||
               !XG_HIP(hipMemsetAsync(s->dBestKey, 0xFF, sizeof(unsigned long long) * (size_t)n, s->stream))))
         return XG_ERR_HIP;
@@ -1278,12 +1306,7 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
 {
     if (!s || !validate_cfg(cfg)) return XG_ERR_ARG;
     if (s->nblocks == 0) return XG_OK;
-    if (pred)   // one prediction picture for every current-picture slot in use
-        for (int i = 0; i < s->nblocks; i++) {
-            s->preds[s->blocks[i].pic] = *pred;
-            s->predSet[s->blocks[i].pic] = true;
-        }
-    if (!validate_blocks(s, true, cfg->ctuSize)) return XG_ERR_ARG;
+    if (!prepare_blocks(s, pred, cfg->ctuSize)) return XG_ERR_ARG;
     if (!build_tasks(s, cfg->ncand)) return XG_ERR_HIP;
     RefineArgs ra;
     if (!fill_args(s, cfg, ra, true)) return XG_ERR_HIP;
@@ -1338,12 +1361,7 @@ int32_t refine_prepare(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
     s->preparedValid = false;
     if (!validate_cfg(cfg)) return XG_ERR_ARG;
     if (s->nblocks == 0) return XG_ERR_STATE;
-    if (pred)
-        for (int i = 0; i < s->nblocks; i++) {
-            s->preds[s->blocks[i].pic] = *pred;
-            s->predSet[s->blocks[i].pic] = true;
-        }
-    if (!validate_blocks(s, true, cfg->ctuSize)) return XG_ERR_ARG;
+    if (!prepare_blocks(s, pred, cfg->ctuSize)) return XG_ERR_ARG;
     if (!build_tasks(s, cfg->ncand)) return XG_ERR_HIP;
     if (!fill_args(s, cfg, s->prepared, true)) return XG_ERR_HIP;
     s->lastNcand = cfg->ncand;
