@@ -172,3 +172,37 @@ def test_refine_rejects_bad_inputs(frame):
     finally:
         lib.xg_clear_error()
         g.close()
+
+
+def test_refine_revalidates_after_state_changes(frame):
+    """Validation is cached per session state: a later bad block list is still
+    rejected, a good one accepted again, and a new prediction picture receives
+    the motion compensation (the cache key covers the prediction argument)."""
+    lib = capi.load()
+    blocks = frame.blocks[:40]
+    g = Gpu(frame, blocks)
+    try:
+        cfg = refine_cfg([tuple(int(v) for v in o) for o in wl.CAND32[:8]], wl.LAMBDA_REFINE)
+        _, best1, pred1 = g.run(cfg, capi.MODE_GRAPH)
+        _, best1b, _ = g.run(cfg, capi.MODE_GRAPH)          # unchanged state: the cached path
+        assert np.array_equal(best1, best1b)
+        bad = blocks[:1].copy()
+        bad[0, 0] = frame.width - 4
+        bad[0, 2] = 8
+        g.set_blocks(bad)
+        assert lib.xg_refine_frac(g.s, ctypes.byref(cfg), g.pred.ref(), capi.MODE_BATCHED) == capi.XG_ERR_ARG
+        lib.xg_clear_error()
+        g.set_blocks(blocks)
+        other = DevicePicture.empty(frame.width, frame.height)
+        assert lib.xg_refine_frac(g.s, ctypes.byref(cfg), other.ref(), capi.MODE_GRAPH) == 0
+        best2 = np.empty(g.n * 4, np.int32)
+        assert lib.xg_session_read(g.s, None, ptr(best2)) == 0
+        assert np.array_equal(best2.reshape(g.n, 4), best1)
+        for p, (a, b) in enumerate(zip(other.planes(), pred1)):
+            sc = 1 if p == 0 else 2
+            for (x, y, w, h) in blocks[:, :4].tolist():
+                sub = (slice(y // sc, (y + h) // sc), slice(x // sc, (x + w) // sc))
+                assert np.array_equal(a[sub], b[sub]), (p, x, y)
+    finally:
+        lib.xg_clear_error()
+        g.close()
