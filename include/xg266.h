@@ -351,7 +351,9 @@ int32_t xg_session_set_pred(xg_session *s, int32_t slot, const xg_picture *pred)
  * (SAD << 8) + (EG(4*dx - px) + EG(4*dy - py)) * sadLambda with the predictor
  * in quarter pel (mvp >> 2), raster order, earliest wins ties.  Writes each
  * block's base MV (mvx, mvy, 1/16 pel) in the session's device block list,
- * where xg_refine_frac then starts.  Enqueued on the session stream. */
+ * where xg_refine_frac then starts.  Enqueued on the session stream.  Block
+ * widths are powers of two 4..64 at even x in dword-aligned planes (the
+ * current block is read as sample pairs); range <= 32. */
 int32_t xg_search_blocks(xg_session *s, int32_t range, uint32_t sadLambda, int32_t ctuSize);
 /* Fractional refinement (per block: ncand candidates, luma 8-tap + chroma
  * 4-tap interpolation, SATD, cost) followed by motion compensation of the best
@@ -443,7 +445,9 @@ int32_t xg_server_stats_get(xg_server *srv, xg_server_stats *stats);
  * (device planes; ref padded by >= range + 4).  Cost and order as
  * Xg26xMotionEstFull with predictor (0,0) and the CU MV window of
  * h266_enc_init.c:1462-1465.  Outputs per block (raster order): best MV in
- * quarter pel (int16 x,y pairs), cost and SAD. */
+ * quarter pel (int16 x,y pairs), cost and SAD.  Block widths 8, 16 and 32 on a
+ * dword-aligned current plane take the packed-SAD kernel; other widths (4..64)
+ * a scalar one with the same results. */
 int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane *ref,
                              int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda,
                              int32_t ctuSize, int16_t *bestMv, uint32_t *bestCost,
