@@ -80,6 +80,19 @@ void Xg266ComputeSadx5(xg_pixel *input, intptr_t inputStride, xg_pixel *ref0, xg
 void Xg266ComputeSadx3(xg_pixel *input, intptr_t inputStride, xg_pixel *ref0, xg_pixel *ref1,
                        xg_pixel *ref2, intptr_t refStride, uint32_t width, uint32_t height,
                        uint32_t *sad);
+/* XinComputeAvgDist (h266_func_struct.h:65-73): pfXinComputeAvgSad[] and
+ * pfXinComputeAvgSatd[] slots, replacing the closed Xin26xComputeAvgSad /
+ * Xin26xComputeAvgSatd (h26x_compute_dist.h:153-243, 824-900).  SAD / SATD
+ * of the input against the rounded average (a + b + 1) >> 1 of two
+ * predictions (the bi-prediction cost of h266_analyze_inter_cu.c:720, 817, 1214). */
+void Xg266ComputeAvgSad(xg_pixel *input, intptr_t inputStride, xg_pixel *refA, xg_pixel *refB,
+                        intptr_t refStride, uint32_t width, uint32_t height, uint32_t *sad);
+void Xg266ComputeAvgSatd(xg_pixel *input, intptr_t inputStride, xg_pixel *refA, xg_pixel *refB,
+                         intptr_t refStride, uint32_t width, uint32_t height, uint32_t *satd);
+/* XinComputeAvgSadx8 (h266_func_struct.h:96-104); replaces Xin26xComputeAvgSadx8
+ * (h26x_compute_dist.h:735-795): the averaged SAD for 8 prediction pairs. */
+void Xg266ComputeAvgSadx8(xg_pixel *input, intptr_t inputStride, xg_pixel *refA[8], xg_pixel *refB[8],
+                          intptr_t refStride, uint32_t width, uint32_t height, uint32_t *sad);
 /* XinComputeHada8x8 (h266_func_struct.h:448-451); replaces
  * Xin26xComputeHada8x8 (h26x_compute_dist.h:937). */
 void Xg266ComputeHada8x8(xg_pixel *input, intptr_t inputStride, uint32_t *satd);
@@ -180,7 +193,11 @@ int32_t Xg266FDctWxH(const xg_coeff *input, intptr_t inputStride, xg_coeff *outp
 /* Mirror of the hot-path slots of xin_func_struct (h266_func_struct.h:719-799),
  * same names, same index conventions (index = log2 of the width). */
 typedef void (*XgComputeDist)(xg_pixel *, intptr_t, xg_pixel *, intptr_t, uint32_t, uint32_t, uint32_t *);
+typedef void (*XgComputeAvgDist)(xg_pixel *, intptr_t, xg_pixel *, xg_pixel *, intptr_t, uint32_t, uint32_t,
+                                 uint32_t *);
 typedef void (*XgComputeSadx8)(xg_pixel *, intptr_t, xg_pixel *[8], intptr_t, uint32_t, uint32_t, uint32_t *);
+typedef void (*XgComputeAvgSadx8)(xg_pixel *, intptr_t, xg_pixel *[8], xg_pixel *[8], intptr_t, uint32_t,
+                                  uint32_t, uint32_t *);
 typedef void (*XgComputeSadx5)(xg_pixel *, intptr_t, xg_pixel *, xg_pixel *, xg_pixel *, xg_pixel *,
                                xg_pixel *, intptr_t, uint32_t, uint32_t, uint32_t *);
 typedef void (*XgComputeSadx3)(xg_pixel *, intptr_t, xg_pixel *, xg_pixel *, xg_pixel *, intptr_t,
@@ -207,7 +224,11 @@ typedef struct xg_func_struct {
     XgComputeDist        pfXinComputeSad[XG_BLOCK_NUM];
     XgComputeDist        pfXinComputeSatd[XG_BLOCK_NUM];
     XgComputeDist        pfXinComputeDist[XG_BLOCK_NUM];
+    XgComputeAvgDist     pfXinComputeAvgSad[XG_BLOCK_NUM];
+    XgComputeAvgDist     pfXinComputeAvgSatd[XG_BLOCK_NUM];
+    XgComputeAvgDist     pfXinComputeAvgDist[XG_BLOCK_NUM];
     XgComputeSadx8       pfXinComputeSadx8[XG_BLOCK_NUM];
+    XgComputeAvgSadx8    pfXinComputeAvgSadx8[XG_BLOCK_NUM];
     XgComputeSadx5       pfXinComputeSadx5[XG_BLOCK_NUM];
     XgComputeSadx3       pfXinComputeSadx3[XG_BLOCK_NUM];
     XgMotionInterp       pfXinLumaInterp[2][2][XG_BLOCK_NUM];
@@ -223,9 +244,10 @@ typedef struct xg_func_struct {
 } xg_func_struct;
 
 /* Fills every slot above with this library's functions (NULL where the
- * reference table holds NULL: SATD widths 1-2, DST-VII/DCT-VIII outside
- * 4..32, DCT-II 1 and 128), then binds pfXinComputeDist to SAD or SATD as
- * Xin266FuncUpdate does (h266_encoder_create.c:3332-3400).  Initialises the
+ * reference table holds NULL: SATD and averaged SAD/SATD widths 1-2,
+ * DST-VII/DCT-VIII outside 4..32, DCT-II 1 and 128), then binds
+ * pfXinComputeDist / pfXinComputeAvgDist to SAD or SATD as Xin266FuncUpdate
+ * does (h266_encoder_create.c:3332-3400).  Initialises the
  * HIP device on the first call. */
 int32_t Xg266FuncInit(xg_func_struct *funcSet, int32_t satdMd);
 

@@ -388,6 +388,38 @@ void or_ComputeSadx3(const or_pixel *in, intptr_t is, const or_pixel *r0, const 
     for (int k = 0; k < 3; k++) sad[k] = or_ComputeSad(in, is, r[k], rs, w, h);
 }
 
+/* Bi-prediction distortions XinComputeAvgDist / XinComputeAvgSadx8
+ * (h266_encoder/include/h266_func_struct.h:65-73, 96-104), bound to the closed
+ * Xin26xComputeAvgSad / AvgSatd / AvgSadx8 (h26x_core/include/h26x_compute_dist.h:
+ * 153-243, 735-795, 824-900) and called on the two uni-directional predictions
+ * of a B-frame PU (h266_encoder/source/h266_analyze_inter_cu.c:720, 817, 1214).
+ * The distortion is that of the input against the rounded average
+ * (a + b + 1) >> 1 of the two predictions.  The averaging rounding is
+ * parity unpinned vs the closed library; with a == b each reduces to
+ * the plain SAD / SATD exactly. */
+static void avg_block(const or_pixel *a, const or_pixel *b, intptr_t rs, uint32_t w, uint32_t h, or_pixel *out)
+{
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++)
+            out[y * w + x] = (or_pixel)(((uint32_t)a[y * rs + x] + b[y * rs + x] + 1) >> 1);
+}
+
+uint32_t or_ComputeAvgSad(const or_pixel *in, intptr_t is, const or_pixel *a, const or_pixel *b, intptr_t rs,
+                          uint32_t w, uint32_t h)
+{
+    uint32_t s = 0;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++)
+            s += (uint32_t)abs((int)in[y * is + x] - (int)(((uint32_t)a[y * rs + x] + b[y * rs + x] + 1) >> 1));
+    return s;
+}
+
+void or_ComputeAvgSadx8(const or_pixel *in, intptr_t is, const or_pixel *const a[8], const or_pixel *const b[8],
+                        intptr_t rs, uint32_t w, uint32_t h, uint32_t sad[8])
+{
+    for (int k = 0; k < 8; k++) sad[k] = or_ComputeAvgSad(in, is, a[k], b[k], rs, w, h);
+}
+
 /* In-place 1-D Walsh-Hadamard transform of n (power of two) values spaced by step. */
 static void wht_1d(int32_t *v, int n, int step)
 {
@@ -441,6 +473,15 @@ uint32_t or_ComputeSatd(const or_pixel *in, intptr_t is, const or_pixel *pr, int
 /* XinComputeHada8x8 (h266_func_struct.h:448-451): Hadamard magnitude of one
  * 8x8 block of samples, defined here as the 8x8 SATD against a zero block.
  * Parity unpinned vs the closed Xin26xComputeHada8x8. */
+uint32_t or_ComputeAvgSatd(const or_pixel *in, intptr_t is, const or_pixel *a, const or_pixel *b, intptr_t rs,
+                           uint32_t w, uint32_t h)
+{
+    or_pixel avg[128 * 128];
+    if (w > 128 || h > 128) return UINT32_MAX;
+    avg_block(a, b, rs, w, h, avg);
+    return or_ComputeSatd(in, is, avg, w, w, h);
+}
+
 uint32_t or_ComputeHada8x8(const or_pixel *in, intptr_t is)
 {
     static const or_pixel zero[8] = {0};

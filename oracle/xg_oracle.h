@@ -100,6 +100,13 @@ void or_ComputeSadx3(const or_pixel *in, intptr_t is, const or_pixel *r0, const 
 uint32_t or_ComputeSatd(const or_pixel *in, intptr_t is, const or_pixel *pr, intptr_t ps,
                         uint32_t w, uint32_t h);
 uint32_t or_ComputeHada8x8(const or_pixel *in, intptr_t is);
+/* bi-prediction distortions against (a + b + 1) >> 1 (h266_func_struct.h:65-73, 96-104) */
+uint32_t or_ComputeAvgSad(const or_pixel *in, intptr_t is, const or_pixel *a, const or_pixel *b, intptr_t rs,
+                          uint32_t w, uint32_t h);
+uint32_t or_ComputeAvgSatd(const or_pixel *in, intptr_t is, const or_pixel *a, const or_pixel *b, intptr_t rs,
+                           uint32_t w, uint32_t h);
+void or_ComputeAvgSadx8(const or_pixel *in, intptr_t is, const or_pixel *const a[8], const or_pixel *const b[8],
+                        intptr_t rs, uint32_t w, uint32_t h, uint32_t sad[8]);
 
 /* ---- rate terms ---- */
 uint32_t or_ExpGolombBits(int32_t v);

@@ -49,6 +49,9 @@ def load_oracle():
         "or_ComputeSatd": (U32, [P, SS, P, SS, U32, U32]),
         "or_ComputeHada8x8": (U32, [P, SS]),
         "or_ComputeSadx8": (None, [P, SS, P, SS, U32, U32, P]),
+        "or_ComputeAvgSad": (U32, [P, SS, P, P, SS, U32, U32]),
+        "or_ComputeAvgSatd": (U32, [P, SS, P, P, SS, U32, U32]),
+        "or_ComputeAvgSadx8": (None, [P, SS, P, P, SS, U32, U32, P]),
         "or_ComputeSadx5": (None, [P, SS, P, P, P, P, P, SS, U32, U32, P]),
         "or_ComputeSadx3": (None, [P, SS, P, P, P, SS, U32, U32, P]),
         "or_ExpGolombBits": (U32, [I32]),

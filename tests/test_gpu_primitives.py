@@ -139,6 +139,31 @@ def test_block_sub(lib, oracle):
     assert np.array_equal(got, exp)
 
 
+def test_avg_distortions(lib, oracle):
+    """Averaged SAD / SATD of bi-prediction (every width class, stride-128
+    prediction buffers as the reference passes them) and the x8 form."""
+    rng = np.random.default_rng(14)
+    plane = rand_pixels(rng, (140, 150))
+    pa = rand_pixels(rng, (128, 128))
+    pb = rand_pixels(rng, (128, 128))
+    for w in SIZES[2:]:
+        for h in SIZES[2:]:
+            out = np.zeros(1, np.uint32)
+            args = (ptr(plane, 5 * 150 + 3), 150, ptr(pa), ptr(pb), 128, w, h)
+            lib.Xg266ComputeAvgSad(*args, ptr(out))
+            assert int(out[0]) == oracle.or_ComputeAvgSad(*args), ("sad", w, h)
+            lib.Xg266ComputeAvgSatd(*args, ptr(out))
+            assert int(out[0]) == oracle.or_ComputeAvgSatd(*args), ("satd", w, h)
+    refs_a = [int(pa.ctypes.data + 2 * (r * 128 + c)) for r, c in rng.integers(0, 60, size=(8, 2))]
+    refs_b = [int(pb.ctypes.data + 2 * (r * 128 + c)) for r, c in rng.integers(0, 60, size=(8, 2))]
+    arr = lambda r: ctypes.cast((ctypes.c_void_p * 8)(*r), ctypes.c_void_p)
+    for (w, h) in ((8, 8), (16, 32), (64, 64), (4, 4)):
+        got, exp = np.zeros(8, np.uint32), np.zeros(8, np.uint32)
+        lib.Xg266ComputeAvgSadx8(ptr(plane), 150, arr(refs_a), arr(refs_b), 128, w, h, ptr(got))
+        oracle.or_ComputeAvgSadx8(ptr(plane), 150, arr(refs_a), arr(refs_b), 128, w, h, ptr(exp))
+        assert got.tolist() == exp.tolist(), (w, h)
+
+
 def test_func_table(lib):
     t = capi.XgFuncStruct()
     assert lib.Xg266FuncInit(ctypes.byref(t), 1) == 0
@@ -154,5 +179,10 @@ def test_func_table(lib):
     assert t.pfXinFor1dDct[0 * 8 + 6] == addr("Xg26xFDct2P64")
     assert t.pfXinFor1dDct[2 * 8 + 2] == addr("Xg26xFDst7P4")
     assert t.pfXinFor1dDct[1 * 8 + 6] is None
+    assert t.pfXinComputeAvgSad[0] is None and t.pfXinComputeAvgSatd[1] is None
+    assert t.pfXinComputeAvgSad[2] == addr("Xg266ComputeAvgSad")
+    assert t.pfXinComputeAvgDist[4] == addr("Xg266ComputeAvgSatd")
+    assert t.pfXinComputeAvgSadx8[0] == addr("Xg266ComputeAvgSadx8")
     assert lib.Xg266FuncInit(ctypes.byref(t), 0) == 0
     assert t.pfXinComputeDist[3] == addr("Xg266ComputeSad")
+    assert t.pfXinComputeAvgDist[4] == addr("Xg266ComputeAvgSad")

@@ -46,6 +46,36 @@ def test_sadxn_are_individual_sads(oracle):
     assert out5.tolist() == exp[:5]
 
 
+def test_avg_distortions_pins(oracle):
+    """Bi-prediction distortions: with equal predictions they are exactly the
+    plain SAD / SATD (pinned); they are symmetric in the two predictions; a
+    pair one apart averages up (a, a + 1 -> a + 1: round half up, the
+    documented choice; rounding parity vs the closed library is unpinned);
+    the x8 form is eight single calls."""
+    plane = rand_pixels(RNG, (64, 64))
+    inp = rand_pixels(RNG, (16, 16))
+    P = lambda r, c: ctypes.c_void_p(plane.ctypes.data + 2 * (r * 64 + c))
+    for (w, h) in ((16, 16), (8, 4), (4, 16)):
+        a, b = P(3, 5), P(20, 7)
+        assert oracle.or_ComputeAvgSad(ptr(inp), 16, a, a, 64, w, h) == oracle.or_ComputeSad(ptr(inp), 16, a, 64, w, h)
+        assert oracle.or_ComputeAvgSatd(ptr(inp), 16, a, a, 64, w, h) == \
+            oracle.or_ComputeSatd(ptr(inp), 16, a, 64, w, h)
+        assert oracle.or_ComputeAvgSad(ptr(inp), 16, a, b, 64, w, h) == \
+            oracle.or_ComputeAvgSad(ptr(inp), 16, b, a, 64, w, h)
+    lo = np.minimum(rand_pixels(RNG, (16, 16)), 1022)
+    hi = lo + 1
+    assert oracle.or_ComputeAvgSad(ptr(inp), 16, ptr(lo), ptr(hi), 16, 16, 16) == \
+        oracle.or_ComputeSad(ptr(inp), 16, ptr(hi), 16, 16, 16)
+    refs_a = [int(plane.ctypes.data + 2 * (r * 64 + c)) for r, c in RNG.integers(0, 40, size=(8, 2))]
+    refs_b = [int(plane.ctypes.data + 2 * (r * 64 + c)) for r, c in RNG.integers(0, 40, size=(8, 2))]
+    exp = [oracle.or_ComputeAvgSad(ptr(inp), 16, ctypes.c_void_p(x), ctypes.c_void_p(y), 64, 16, 8)
+           for x, y in zip(refs_a, refs_b)]
+    out = np.zeros(8, np.uint32)
+    arr = lambda r: ctypes.cast((ctypes.c_void_p * 8)(*r), ctypes.c_void_p)
+    oracle.or_ComputeAvgSadx8(ptr(inp), 16, arr(refs_a), arr(refs_b), 64, 16, 8, ptr(out))
+    assert out.tolist() == exp
+
+
 # ---------------------------------------------------------------- SATD ------
 def _satd_matrix(a, b):
     """Sum over HM-style tiles of |H d H^T| with the per-tile normalisation,

@@ -76,7 +76,9 @@ class XgMeStruct(ctypes.Structure):
 
 class XgFuncStruct(ctypes.Structure):
     _fields_ = [("pfXinComputeSad", P * 8), ("pfXinComputeSatd", P * 8), ("pfXinComputeDist", P * 8),
-                ("pfXinComputeSadx8", P * 8), ("pfXinComputeSadx5", P * 8), ("pfXinComputeSadx3", P * 8),
+                ("pfXinComputeAvgSad", P * 8), ("pfXinComputeAvgSatd", P * 8), ("pfXinComputeAvgDist", P * 8),
+                ("pfXinComputeSadx8", P * 8), ("pfXinComputeAvgSadx8", P * 8),
+                ("pfXinComputeSadx5", P * 8), ("pfXinComputeSadx3", P * 8),
                 ("pfXinLumaInterp", P * 32), ("pfXinChromaInterp", P * 32),
                 ("pfXinLumaInterpS16", P * 32), ("pfXinChromaInterpS16", P * 32),
                 ("pfXinInterpAvg", P * 8), ("pfXinInterpWeight", P * 8),
@@ -98,6 +100,9 @@ PROTOS = {
     "Xg266ComputeSadx5": (None, [P, SS, P, P, P, P, P, SS, U32, U32, P]),
     "Xg266ComputeSadx3": (None, [P, SS, P, P, P, SS, U32, U32, P]),
     "Xg266ComputeHada8x8": (None, [P, SS, P]),
+    "Xg266ComputeAvgSad": (None, [P, SS, P, P, SS, U32, U32, P]),
+    "Xg266ComputeAvgSatd": (None, [P, SS, P, P, SS, U32, U32, P]),
+    "Xg266ComputeAvgSadx8": (None, [P, SS, P, P, SS, U32, U32, P]),
     "Xg266InterpCopy": INTERP, "Xg266LumaInterpHor": INTERP, "Xg266LumaInterpVet": INTERP,
     "Xg266LumaInterpHorVet": INTERP, "Xg266InterpCopyU8S16": INTERP,
     "Xg266LumaInterpHorU8S16": INTERP, "Xg266LumaInterpVetU8S16": INTERP,

@@ -127,6 +127,15 @@ __global__ void __launch_bounds__(256) k_sad(const uint16_t *__restrict__ in,
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// Rounded average (a + b + 1) >> 1 of packed block pairs, in place into a:
+// pair k at a + k * pitch, b + k * pitch (blockIdx.y = k).
+__global__ void k_avg2(uint16_t *__restrict__ a, const uint16_t *__restrict__ b, int n, int pitch)
+{
+    const size_t base = (size_t)blockIdx.y * pitch;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        a[base + i] = (uint16_t)(((uint32_t)a[base + i] + b[base + i] + 1) >> 1);
+}
+
 // SATD of a packed w x h block pair; one thread per Hadamard tile.
 template <int N>
 __device__ uint32_t tile_satd_serial(const uint16_t *a, const uint16_t *b, int stride)
@@ -317,6 +326,44 @@ static void sad_multi(const uint16_t *in, intptr_t is, const uint16_t *const *re
     for (int k = 0; k < nref; k++) out[k] = hres[k];
 }
 
+// Distortion of one input block against the rounded averages of nref
+// prediction pairs (refA[k], refB[k]): SAD for each pair, or the SATD of the
+// single pair (satd, nref == 1).
+static void avg_dist(const uint16_t *in, intptr_t is, const uint16_t *const *ra, const uint16_t *const *rb, int nref,
+                     intptr_t rs, uint32_t w, uint32_t h, bool satd, uint32_t *out, const char *fn)
+{
+    for (int k = 0; k < nref; k++) out[k] = UINT32_MAX;
+    if (!check_wh(w, h, 128, 128, fn)) return;
+    if (satd && ((w & 1) || (h & 1))) {
+        set_error(XG_ERR_ARG, "%s: SATD needs even sizes (%ux%u)", fn, w, h);
+        return;
+    }
+    const size_t npix = (size_t)w * h, blk = align_up(npix * 2, 256);
+    const size_t bytes = blk * (1 + 2 * nref) + 256;
+    CallCtx *c = call_ctx(bytes, bytes);
+    if (!c) return;
+    // input, then the A blocks, then the B blocks
+    pack2d(c->host, in, is, (int)w, (int)h, 2);
+    for (int k = 0; k < nref; k++) {
+        pack2d(c->host + blk * (1 + k), ra[k], rs, (int)w, (int)h, 2);
+        pack2d(c->host + blk * (1 + nref + k), rb[k], rs, (int)w, (int)h, 2);
+    }
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, blk * (1 + 2 * nref), hipMemcpyHostToDevice, c->stream))) return;
+    uint16_t *dA = (uint16_t *)(c->dev + blk), *dB = (uint16_t *)(c->dev + blk * (1 + nref));
+    k_avg2<<<dim3((unsigned)std::min<size_t>((npix + 255) / 256, 64), nref), 256, 0, c->stream>>>(
+        dA, dB, (int)npix, (int)(blk / 2));
+    uint32_t *dres = (uint32_t *)(c->dev + blk * (1 + 2 * nref));
+    if (satd)
+        k_satd<<<1, 256, 0, c->stream>>>((uint16_t *)c->dev, dA, (int)w, (int)h, dres);
+    else
+        k_sad<<<nref, 256, 0, c->stream>>>((uint16_t *)c->dev, dA, (int)npix, (int)(blk / 2), dres);
+    if (!XG_HIP(hipGetLastError())) return;
+    uint32_t *hres = (uint32_t *)(c->host + blk * (1 + 2 * nref));
+    if (!XG_HIP(hipMemcpyAsync(hres, dres, 4 * nref, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    for (int k = 0; k < nref; k++) out[k] = hres[k];
+}
+
 static void satd_call(const uint16_t *in, intptr_t is, const uint16_t *pr, intptr_t ps, uint32_t w,
                       uint32_t h, uint32_t *out, const char *fn)
 {
@@ -430,6 +477,27 @@ void Xg266ComputeSadx3(xg_pixel *input, intptr_t inputStride, xg_pixel *ref0, xg
 {
     const uint16_t *r[3] = {ref0, ref1, ref2};
     sad_multi(input, inputStride, r, 3, refStride, width, height, sad, "ComputeSadx3");
+}
+
+void Xg266ComputeAvgSad(xg_pixel *input, intptr_t inputStride, xg_pixel *refA, xg_pixel *refB,
+                        intptr_t refStride, uint32_t width, uint32_t height, uint32_t *sad)
+{
+    const uint16_t *a[1] = {refA}, *b[1] = {refB};
+    avg_dist(input, inputStride, a, b, 1, refStride, width, height, false, sad, "ComputeAvgSad");
+}
+
+void Xg266ComputeAvgSatd(xg_pixel *input, intptr_t inputStride, xg_pixel *refA, xg_pixel *refB,
+                         intptr_t refStride, uint32_t width, uint32_t height, uint32_t *satd)
+{
+    const uint16_t *a[1] = {refA}, *b[1] = {refB};
+    avg_dist(input, inputStride, a, b, 1, refStride, width, height, true, satd, "ComputeAvgSatd");
+}
+
+void Xg266ComputeAvgSadx8(xg_pixel *input, intptr_t inputStride, xg_pixel *refA[8], xg_pixel *refB[8],
+                          intptr_t refStride, uint32_t width, uint32_t height, uint32_t *sad)
+{
+    avg_dist(input, inputStride, (const uint16_t *const *)refA, (const uint16_t *const *)refB, 8, refStride, width,
+             height, false, sad, "ComputeAvgSadx8");
 }
 
 void Xg266ComputeHada8x8(xg_pixel *input, intptr_t inputStride, uint32_t *satd)
@@ -621,7 +689,11 @@ int32_t Xg266FuncInit(xg_func_struct *f, int32_t satdMd)
         f->pfXinComputeSad[i] = Xg266ComputeSad;
         f->pfXinComputeSatd[i] = i >= 2 ? Xg266ComputeSatd : nullptr;
         f->pfXinComputeDist[i] = satdMd ? f->pfXinComputeSatd[i] : f->pfXinComputeSad[i];
+        f->pfXinComputeAvgSad[i] = i >= 2 ? Xg266ComputeAvgSad : nullptr;
+        f->pfXinComputeAvgSatd[i] = i >= 2 ? Xg266ComputeAvgSatd : nullptr;
+        f->pfXinComputeAvgDist[i] = satdMd ? f->pfXinComputeAvgSatd[i] : f->pfXinComputeAvgSad[i];
         f->pfXinComputeSadx8[i] = Xg266ComputeSadx8;
+        f->pfXinComputeAvgSadx8[i] = Xg266ComputeAvgSadx8;
         f->pfXinComputeSadx5[i] = Xg266ComputeSadx5;
         f->pfXinComputeSadx3[i] = Xg266ComputeSadx3;
         f->pfXinLumaInterp[0][0][i] = Xg266InterpCopy;
