@@ -93,6 +93,16 @@ void Xg266ComputeAvgSatd(xg_pixel *input, intptr_t inputStride, xg_pixel *refA, 
  * (h26x_compute_dist.h:735-795): the averaged SAD for 8 prediction pairs. */
 void Xg266ComputeAvgSadx8(xg_pixel *input, intptr_t inputStride, xg_pixel *refA[8], xg_pixel *refB[8],
                           intptr_t refStride, uint32_t width, uint32_t height, uint32_t *sad);
+/* XinQuantInvQuant (h266_func_struct.h:181-196): pfXinQuantInvQuant[] slot,
+ * replacing Xin266QuantInvQuant (h266_encoder/source/h266_quant_inv_quant.c:80-143).
+ * Per coefficient group of cgWidth x cgHeight (at most 64 groups):
+ * q = sign(t) * ((|t| * qMult + qAdd) >> qShift) and r = (q * iqMult + iqAdd) >> iqShift,
+ * both clipped to int16; bit k of *nzCGBitMapRs is set when group k (raster
+ * order) holds a non-zero q. */
+void Xg266QuantInvQuant(xg_coeff *qCoeff, xg_coeff *tCoeff, xg_coeff *rCoeff, intptr_t coeffStride,
+                        uint32_t width, uint32_t height, uint32_t cgWidth, uint32_t cgHeight, int32_t qMult,
+                        int32_t qAdd, int32_t qShift, int32_t iqMult, int32_t iqAdd, int32_t iqShift,
+                        uint64_t *nzCGBitMapRs);
 /* XinComputeHada8x8 (h266_func_struct.h:448-451); replaces
  * Xin26xComputeHada8x8 (h26x_compute_dist.h:937). */
 void Xg266ComputeHada8x8(xg_pixel *input, intptr_t inputStride, uint32_t *satd);
@@ -216,6 +226,8 @@ typedef void (*XgFor2dDct2)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, uint32_t
 typedef void (*XgBlockSubForDct)(xg_pixel *, intptr_t, xg_pixel *, intptr_t, xg_coeff *, intptr_t,
                                  uint32_t, uint32_t);
 typedef void (*XgComputeHada8x8)(xg_pixel *, intptr_t, uint32_t *);
+typedef void (*XgQuantInvQuant)(xg_coeff *, xg_coeff *, xg_coeff *, intptr_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                int32_t, int32_t, int32_t, int32_t, int32_t, int32_t, uint64_t *);
 
 #define XG_BLOCK_NUM 8
 #define XG_TRANS_TYPE 3        /* XIN_DCT2 = 0, XIN_DCT8 = 1, XIN_DST7 = 2 */
@@ -240,11 +252,13 @@ typedef struct xg_func_struct {
     XgFor2dDct2          pfXinFor2dDct2[XG_BLOCK_NUM][XG_BLOCK_NUM];
     XgFor1dDct           pfXinFor1dDct[XG_TRANS_TYPE][XG_BLOCK_NUM];
     XgBlockSubForDct     pfXinBlockSubForDct[XG_BLOCK_NUM];
+    XgQuantInvQuant      pfXinQuantInvQuant[XG_BLOCK_NUM];
     XgComputeHada8x8     pfXinComputeHada8x8;
 } xg_func_struct;
 
 /* Fills every slot above with this library's functions (NULL where the
  * reference table holds NULL: SATD and averaged SAD/SATD widths 1-2,
+ * quantisation width 128,
  * DST-VII/DCT-VIII outside 4..32, DCT-II 1 and 128), then binds
  * pfXinComputeDist / pfXinComputeAvgDist to SAD or SATD as Xin266FuncUpdate
  * does (h266_encoder_create.c:3332-3400).  Initialises the

@@ -1091,3 +1091,43 @@ void or_MePipeline(const or_pixel *curY, const or_pixel *curU, const or_pixel *c
                             bestMvx + b, bestMvy + b, predY, predU, predV, psy, psc);
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * Quantisation and inverse quantisation of one transform block, coefficient
+ * group by coefficient group: Xin266QuantInvQuant and Xin266QIQCodingGroup
+ * (h266_encoder/source/h266_quant_inv_quant.c:27-143; slot pfXinQuantInvQuant,
+ * typedef XinQuantInvQuant h266_encoder/include/h266_func_struct.h:181-196).
+ * q = sign * ((|t| * qMult + qAdd) >> qShift), clipped to int16; r =
+ * (q * iqMult + iqAdd) >> iqShift, clipped to int16 (MIN/MAX_COEFF_VALUE,
+ * video_api/video_common/include/xin_typedef.h:80-81).  Bit k of *nzMap is set
+ * when coefficient group k (raster order of the cgW x cgH groups) holds a
+ * non-zero q: the reference walks the groups in reverse raster order and
+ * shifts the map left before each, which lands group k on bit k.
+ * ------------------------------------------------------------------------- */
+void or_QuantInvQuant(int32_t *q, const int32_t *t, int32_t *r, intptr_t stride, uint32_t w, uint32_t h,
+                      uint32_t cgw, uint32_t cgh, int32_t qMult, int32_t qAdd, int32_t qShift, int32_t iqMult,
+                      int32_t iqAdd, int32_t iqShift, uint64_t *nzMap)
+{
+    const uint32_t ncx = w / cgw, ncy = h / cgh;
+    uint64_t map = 0;
+    for (uint32_t cy = 0; cy < ncy; cy++)
+        for (uint32_t cx = 0; cx < ncx; cx++) {
+            int nz = 0;
+            for (uint32_t y = cy * cgh; y < (cy + 1) * cgh; y++)
+                for (uint32_t x = cx * cgw; x < (cx + 1) * cgw; x++) {
+                    const intptr_t i = (intptr_t)y * stride + x;
+                    const int32_t tc = t[i], sign = tc < 0 ? -1 : 1, a = tc < 0 ? -tc : tc;
+                    /* int32 arithmetic as the reference's SINT32 expressions */
+                    int32_t qc = (int32_t)(((uint32_t)a * (uint32_t)qMult + (uint32_t)qAdd)) >> qShift;
+                    qc *= sign;
+                    qc = qc < -32768 ? -32768 : qc > 32767 ? 32767 : qc;
+                    q[i] = qc;
+                    int32_t rc = (int32_t)((uint32_t)qc * (uint32_t)iqMult + (uint32_t)iqAdd) >> iqShift;
+                    rc = rc < -32768 ? -32768 : rc > 32767 ? 32767 : rc;
+                    r[i] = rc;
+                    nz |= qc != 0;
+                }
+            if (nz) map |= 1ull << (cy * ncx + cx);
+        }
+    *nzMap = map;
+}

@@ -206,6 +206,11 @@ void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, i
                         int16_t *bestMvX, int16_t *bestMvY, uint32_t *bestCost,
                         uint32_t *bestSad);
 
+/* ---- quantisation (h266_encoder/source/h266_quant_inv_quant.c:27-143) ---- */
+void or_QuantInvQuant(int32_t *q, const int32_t *t, int32_t *r, intptr_t stride, uint32_t w, uint32_t h,
+                      uint32_t cgw, uint32_t cgh, int32_t qMult, int32_t qAdd, int32_t qShift, int32_t iqMult,
+                      int32_t iqAdd, int32_t iqShift, uint64_t *nzMap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -164,6 +164,20 @@ def test_avg_distortions(lib, oracle):
         assert got.tolist() == exp.tolist(), (w, h)
 
 
+def test_quant_inv_quant_reference_golden(lib):
+    """Xg266QuantInvQuant against the reference's own Xin266QuantInvQuant
+    outputs (tests/golden/quant_golden.npz): q, r and the coefficient-group map."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import quant_cases as qc
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "quant_golden.npz"))
+    t = np.ascontiguousarray(g["t"])
+    q, r, maps = qc.replay(qc.resolve(lib, "Xg266QuantInvQuant"), g["cases"], t)
+    assert np.array_equal(q, g["q"]) and np.array_equal(r, g["r"])
+    assert np.array_equal(maps, g["maps"])
+
+
 def test_func_table(lib):
     t = capi.XgFuncStruct()
     assert lib.Xg266FuncInit(ctypes.byref(t), 1) == 0
@@ -183,6 +197,7 @@ def test_func_table(lib):
     assert t.pfXinComputeAvgSad[2] == addr("Xg266ComputeAvgSad")
     assert t.pfXinComputeAvgDist[4] == addr("Xg266ComputeAvgSatd")
     assert t.pfXinComputeAvgSadx8[0] == addr("Xg266ComputeAvgSadx8")
+    assert t.pfXinQuantInvQuant[6] == addr("Xg266QuantInvQuant") and t.pfXinQuantInvQuant[7] is None
     assert lib.Xg266FuncInit(ctypes.byref(t), 0) == 0
     assert t.pfXinComputeDist[3] == addr("Xg266ComputeSad")
     assert t.pfXinComputeAvgDist[4] == addr("Xg266ComputeAvgSad")

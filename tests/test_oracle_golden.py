@@ -6,6 +6,11 @@ generator tests/golden/make_golden.py) on seeded planes: every luma phase pair
 under filterIndex 0/1/2, every chroma phase pair, every table width class
 1..128, clipping extremes (flat 0/1023 and a 0/1023 checkerboard) and the
 bi-prediction average / BCW weighting.
+
+tests/golden/quant_golden.npz holds the outputs of the reference's own build of
+h266_encoder/source/h266_quant_inv_quant.c (generator
+tests/golden/make_quant_golden.py): every coefficient-group shape the encoder
+passes, QPs 0..51 with the RDOQ / intra / inter rounding offsets, int16 extremes.
 """
 import numpy as np
 
@@ -87,3 +92,18 @@ def test_reference_horvet_phase_pairs_cover_all(golden):
     k = gc.KEYS.index("ChromaHorVet")
     fr = set(int(c[1]) for c in cases if c[0] == k)
     assert fr == set(range(1024))
+
+
+def test_oracle_quant_matches_reference_golden(oracle):
+    import os
+    import quant_cases as qc
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "quant_golden.npz"))
+    cases, total = qc.build_cases()
+    assert np.array_equal(cases, g["cases"]) and np.array_equal(qc.make_inputs(total), g["t"])
+    t = np.ascontiguousarray(g["t"])
+    q, r, maps = qc.replay(qc.resolve(oracle, "or_QuantInvQuant"), cases, t)
+    assert np.array_equal(q, g["q"]) and np.array_equal(r, g["r"])
+    assert np.array_equal(maps, g["maps"])
+    # the fixture exercises both clips and empty groups
+    assert (g["q"] == 32767).any() or (g["q"] == -32768).any() or (np.abs(g["r"]) == 32767).any()
+    assert (g["maps"] == 0).any() and (g["maps"] != 0).any()

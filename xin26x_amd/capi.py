@@ -83,7 +83,7 @@ class XgFuncStruct(ctypes.Structure):
                 ("pfXinLumaInterpS16", P * 32), ("pfXinChromaInterpS16", P * 32),
                 ("pfXinInterpAvg", P * 8), ("pfXinInterpWeight", P * 8),
                 ("pfXinFor2dDct2", P * 64), ("pfXinFor1dDct", P * 24),
-                ("pfXinBlockSubForDct", P * 8), ("pfXinComputeHada8x8", P)]
+                ("pfXinBlockSubForDct", P * 8), ("pfXinQuantInvQuant", P * 8), ("pfXinComputeHada8x8", P)]
 
 
 # Every symbol include/xg266.h declares, with its prototype (restype, argtypes).
@@ -100,6 +100,7 @@ PROTOS = {
     "Xg266ComputeSadx5": (None, [P, SS, P, P, P, P, P, SS, U32, U32, P]),
     "Xg266ComputeSadx3": (None, [P, SS, P, P, P, SS, U32, U32, P]),
     "Xg266ComputeHada8x8": (None, [P, SS, P]),
+    "Xg266QuantInvQuant": (None, [P, P, P, SS, U32, U32, U32, U32, I32, I32, I32, I32, I32, I32, P]),
     "Xg266ComputeAvgSad": (None, [P, SS, P, P, SS, U32, U32, P]),
     "Xg266ComputeAvgSatd": (None, [P, SS, P, P, SS, U32, U32, P]),
     "Xg266ComputeAvgSadx8": (None, [P, SS, P, P, SS, U32, U32, P]),

@@ -136,6 +136,37 @@ __global__ void k_avg2(uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
         a[base + i] = (uint16_t)(((uint32_t)a[base + i] + b[base + i] + 1) >> 1);
 }
 
+// Quantisation + inverse quantisation of one packed w x h block
+// (h266_encoder/source/h266_quant_inv_quant.c:27-143): one lane per coefficient
+// group (at most 64), the non-zero map from a wave ballot, so group k (raster
+// order) lands on bit k as in the reference's reverse walk with left shifts.
+// Arithmetic is int32 as the reference's SINT32 expressions (wrapping).
+__global__ void __launch_bounds__(64) k_qiq(const int32_t *__restrict__ t, int32_t *__restrict__ q,
+                                            int32_t *__restrict__ r, int w, int h, int cgw, int cgh, int qMult,
+                                            int qAdd, int qShift, int iqMult, int iqAdd, int iqShift,
+                                            unsigned long long *__restrict__ map)
+{
+    const int ncx = w / cgw, ncg = ncx * (h / cgh), g = threadIdx.x;
+    bool nz = false;
+    if (g < ncg) {
+        const int x0 = (g % ncx) * cgw, y0 = (g / ncx) * cgh;
+        for (int y = y0; y < y0 + cgh; y++)
+            for (int x = x0; x < x0 + cgw; x++) {
+                const int i = y * w + x;
+                const int tc = t[i], a = tc < 0 ? -tc : tc;
+                int qc = (int)((unsigned)a * (unsigned)qMult + (unsigned)qAdd) >> qShift;
+                qc = tc < 0 ? -qc : qc;
+                qc = min(max(qc, -32768), 32767);
+                q[i] = qc;
+                const int rc = (int)((unsigned)qc * (unsigned)iqMult + (unsigned)iqAdd) >> iqShift;
+                r[i] = min(max(rc, -32768), 32767);
+                nz |= qc != 0;
+            }
+    }
+    const unsigned long long m = __ballot(nz);
+    if (g == 0) *map = m;
+}
+
 // SATD of a packed w x h block pair; one thread per Hadamard tile.
 template <int N>
 __device__ uint32_t tile_satd_serial(const uint16_t *a, const uint16_t *b, int stride)
@@ -500,6 +531,35 @@ void Xg266ComputeAvgSadx8(xg_pixel *input, intptr_t inputStride, xg_pixel *refA[
              height, false, sad, "ComputeAvgSadx8");
 }
 
+void Xg266QuantInvQuant(xg_coeff *qCoeff, xg_coeff *tCoeff, xg_coeff *rCoeff, intptr_t coeffStride, uint32_t width,
+                        uint32_t height, uint32_t cgWidth, uint32_t cgHeight, int32_t qMult, int32_t qAdd,
+                        int32_t qShift, int32_t iqMult, int32_t iqAdd, int32_t iqShift, uint64_t *nzCGBitMapRs)
+{
+    const char *fn = "QuantInvQuant";
+    if (!qCoeff || !tCoeff || !rCoeff || !nzCGBitMapRs || !check_wh(width, height, 128, 128, fn)) return;
+    if (cgWidth == 0 || cgHeight == 0 || width % cgWidth || height % cgHeight ||
+        (width / cgWidth) * (height / cgHeight) > 64 || qShift < 0 || qShift > 31 || iqShift < 0 || iqShift > 31) {
+        set_error(XG_ERR_ARG, "%s: %ux%u in %ux%u groups (at most 64) or shifts %d/%d out of range", fn, width, height,
+                  cgWidth, cgHeight, qShift, iqShift);
+        return;
+    }
+    const size_t n = (size_t)width * height, blk = align_up(n * 4, 256);
+    CallCtx *c = call_ctx(3 * blk + 256, 3 * blk + 256);
+    if (!c) return;
+    pack2d(c->host, tCoeff, coeffStride, (int)width, (int)height, 4);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, n * 4, hipMemcpyHostToDevice, c->stream))) return;
+    int32_t *dq = (int32_t *)(c->dev + blk), *dr = (int32_t *)(c->dev + 2 * blk);
+    unsigned long long *dm = (unsigned long long *)(c->dev + 3 * blk);
+    k_qiq<<<1, 64, 0, c->stream>>>((const int32_t *)c->dev, dq, dr, (int)width, (int)height, (int)cgWidth,
+                                   (int)cgHeight, qMult, qAdd, qShift, iqMult, iqAdd, iqShift, dm);
+    if (!XG_HIP(hipGetLastError())) return;
+    if (!XG_HIP(hipMemcpyAsync(c->host + blk, c->dev + blk, 2 * blk + 8, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(qCoeff, coeffStride, c->host + blk, (int)width, (int)height, 4);
+    unpack2d(rCoeff, coeffStride, c->host + 2 * blk, (int)width, (int)height, 4);
+    *nzCGBitMapRs = *(const uint64_t *)(c->host + 3 * blk);
+}
+
 void Xg266ComputeHada8x8(xg_pixel *input, intptr_t inputStride, uint32_t *satd)
 {
     static const xg_pixel zero[8] = {0};
@@ -694,6 +754,7 @@ int32_t Xg266FuncInit(xg_func_struct *f, int32_t satdMd)
         f->pfXinComputeAvgDist[i] = satdMd ? f->pfXinComputeAvgSatd[i] : f->pfXinComputeAvgSad[i];
         f->pfXinComputeSadx8[i] = Xg266ComputeSadx8;
         f->pfXinComputeAvgSadx8[i] = Xg266ComputeAvgSadx8;
+        f->pfXinQuantInvQuant[i] = i < 7 ? Xg266QuantInvQuant : nullptr;
         f->pfXinComputeSadx5[i] = Xg266ComputeSadx5;
         f->pfXinComputeSadx3[i] = Xg266ComputeSadx3;
         f->pfXinLumaInterp[0][0][i] = Xg266InterpCopy;
