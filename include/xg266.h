@@ -200,6 +200,60 @@ int32_t Xg266FDctWxH(const xg_coeff *input, intptr_t inputStride, xg_coeff *outp
                      intptr_t outputStride, uint32_t width, uint32_t height, uint32_t mtsIdx,
                      int32_t isIntraLuma, int32_t enableMts);
 
+/* XinInv1dDct (h266_func_struct.h:280-287): pfXinInv1dDct[type][lgN]
+ * (h266_func_init.c:376-401), replacing the closed Xin26xIDct2P2..P64,
+ * Xin26xIDst7P4..P32, Xin26xIDct8P4..P32 (h26x_core/include/h26x_inverse_1d_trans.h:23-212).
+ *   out[j*outputStride + i] = Clip3(-32768, 32767,
+ *       (sum_{k<nz} T[k][i]*in[k*inputStride + j] + 2^(shift-1)) >> shift)
+ * for j < line - skipLine; 0 for the last skipLine lines, whose input is not
+ * read.  nz is the H.266 zero-out bound (32 for DCT-II-64, 16 for 32-point
+ * DST-VII/DCT-VIII): coefficients past it are not read. */
+void Xg26xIDct2P2(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct2P4(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct2P8(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct2P16(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct2P32(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct2P64(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDst7P4(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDst7P8(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDst7P16(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDst7P32(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct8P4(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct8P8(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct8P16(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+void Xg26xIDct8P32(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride, int32_t line, int32_t skipLine, int32_t shift);
+/* XinInv2dDct2 (h266_func_struct.h:269-278): pfXinInv2dDct2[lgW][lgH]
+ * (h266_func_init.c:72-85), replacing Xin266IDct2WxH
+ * (h266_encoder/source/h266_inverse_trans.c:51-89): vertical then horizontal
+ * DCT-II, coefficient columns past 32 skipped.  tempBuf is unused. */
+void Xg266IDct2WxH(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                   uint32_t width, uint32_t height, xg_coeff *tempBuf, int32_t shift1st, int32_t shift2nd);
+/* XinInv2dSkip (h266_func_struct.h:635-641): pfXinInv2dSkip[] (h266_func_init.c:410-419),
+ * replacing Xin266ISkipWxH (h266_inverse_trans.c:91-113):
+ * out = (int16_t)(in >> (7 - ((lgW + lgH) >> 1))), sizes 1..128. */
+void Xg266ISkipWxH(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                   uint32_t width, uint32_t height);
+/* Inverse transform of one TU with the dispatch of Xin266IDctWxH
+ * (h266_encoder/source/h266_inverse_trans.c:115-229): mtsIdx as
+ * Xg266FDctWxH, implicit intra-luma DST-VII, shifts 7 and 20 - 10, zero-out
+ * skip of 16 lines for a 32-wide DST-VII/DCT-VIII, transform skip for
+ * mtsIdx 1.  Widths or heights of 1 outside transform skip return XG_ERR_ARG
+ * (the reference leaves that result in its section scratch buffer). */
+int32_t Xg266IDctWxH(const xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                     uint32_t width, uint32_t height, uint32_t mtsIdx, int32_t isIntraLuma, int32_t enableMts);
+/* XinBlockAddForDct (h266_func_struct.h:530-538): pfXinBlockAddForDct[]
+ * (h266_func_init.c:161-166), replacing the closed Xin26xBlockRecon:
+ * recon = Clip3(0, 1023, pred + residual). */
+void Xg26xBlockRecon(xg_coeff *input, intptr_t inputStride, xg_pixel *pred, intptr_t predStride,
+                     xg_pixel *recon, intptr_t reconStride, uint32_t width, uint32_t height);
+/* Xin266IDctAdd (h266_encoder/source/h266_idct_add.c:45-104): inverse
+ * transform then reconstruction, sizes up to 64x64.  The LMCS chroma
+ * residual scaling the reference applies when LMCS is enabled
+ * (:84-92) is not part of this entry. */
+int32_t Xg266IDctAdd(const xg_coeff *input, intptr_t inputStride, const xg_pixel *pred, intptr_t predStride,
+                     xg_pixel *output, intptr_t outputStride, uint32_t width, uint32_t height, uint32_t mtsIdx,
+                     int32_t isIntraLuma, int32_t enableMts);
+
 /* Mirror of the hot-path slots of xin_func_struct (h266_func_struct.h:719-799),
  * same names, same index conventions (index = log2 of the width). */
 typedef void (*XgComputeDist)(xg_pixel *, intptr_t, xg_pixel *, intptr_t, uint32_t, uint32_t, uint32_t *);
@@ -225,6 +279,12 @@ typedef void (*XgFor2dDct2)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, uint32_t
                             int32_t, int32_t);
 typedef void (*XgBlockSubForDct)(xg_pixel *, intptr_t, xg_pixel *, intptr_t, xg_coeff *, intptr_t,
                                  uint32_t, uint32_t);
+typedef void (*XgInv1dDct)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, int32_t, int32_t, int32_t);
+typedef void (*XgInv2dDct2)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, uint32_t, uint32_t, xg_coeff *,
+                            int32_t, int32_t);
+typedef void (*XgInv2dSkip)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, uint32_t, uint32_t);
+typedef void (*XgBlockAddForDct)(xg_coeff *, intptr_t, xg_pixel *, intptr_t, xg_pixel *, intptr_t, uint32_t,
+                                 uint32_t);
 typedef void (*XgComputeHada8x8)(xg_pixel *, intptr_t, uint32_t *);
 typedef void (*XgQuantInvQuant)(xg_coeff *, xg_coeff *, xg_coeff *, intptr_t, uint32_t, uint32_t, uint32_t, uint32_t,
                                 int32_t, int32_t, int32_t, int32_t, int32_t, int32_t, uint64_t *);
@@ -253,12 +313,16 @@ typedef struct xg_func_struct {
     XgFor1dDct           pfXinFor1dDct[XG_TRANS_TYPE][XG_BLOCK_NUM];
     XgBlockSubForDct     pfXinBlockSubForDct[XG_BLOCK_NUM];
     XgQuantInvQuant      pfXinQuantInvQuant[XG_BLOCK_NUM];
+    XgInv2dSkip          pfXinInv2dSkip[XG_BLOCK_NUM];
+    XgInv2dDct2          pfXinInv2dDct2[XG_BLOCK_NUM][XG_BLOCK_NUM];
+    XgInv1dDct           pfXinInv1dDct[XG_TRANS_TYPE][XG_BLOCK_NUM];
+    XgBlockAddForDct     pfXinBlockAddForDct[XG_BLOCK_NUM];
     XgComputeHada8x8     pfXinComputeHada8x8;
 } xg_func_struct;
 
 /* Fills every slot above with this library's functions (NULL where the
  * reference table holds NULL: SATD and averaged SAD/SATD widths 1-2,
- * quantisation width 128,
+ * quantisation width 128, reconstruction widths 1 and 128,
  * DST-VII/DCT-VIII outside 4..32, DCT-II 1 and 128), then binds
  * pfXinComputeDist / pfXinComputeAvgDist to SAD or SATD as Xin266FuncUpdate
  * does (h266_encoder_create.c:3332-3400).  Initialises the

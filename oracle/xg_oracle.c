@@ -858,6 +858,127 @@ void or_FDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
 }
 
 /* ---------------------------------------------------------------------------
+ * Inverse transforms.  The reference's 1-D kernels are closed
+ * (h26x_core/include/h26x_inverse_1d_trans.h:23-200); the 2-D dispatch,
+ * transform skip and reconstruction call order are source
+ * (h266_encoder/source/h266_inverse_trans.c:45-229, h266_idct_add.c:45-104).
+ * 1-D kernel restated from ITU-T H.266 clause 8.7.4.5 (transformation
+ * process): y[i] = sum_{k < nonZeroS} T[k][i] x[k], with nonZeroS the
+ * zero-out bound of clause 8.7.4.2 (32 for DCT-II-64, 16 for 32-point
+ * DST-VII/DCT-VIII); each pass rounds, shifts and clips to
+ * [-32768, 32767] (clause 8.7.4.2 clips the first pass to coeffMin/coeffMax;
+ * the output is COEFF-typed and saturates at the same bounds).
+ * ------------------------------------------------------------------------- */
+static int32_t clip16(int64_t v) { return (int32_t)(v < -32768 ? -32768 : v > 32767 ? 32767 : v); }
+
+/* XinInv1dDct (h266_func_struct.h:280-287), layout from the two calls at
+ * h266_inverse_trans.c:183-199: line j reads coefficient k at in[k*is + j]
+ * and writes sample i at out[j*os + i]; the last skipLine lines are written
+ * as zero without reading their input. */
+void or_IDct1d(int type, int n, const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+               int32_t line, int32_t skipLine, int32_t shift)
+{
+    int32_t T[64 * 64];
+    if (or_TrMatrix(type, n, T)) return;
+    const int nz = kept_outputs(type, n);
+    const int64_t rnd = shift > 0 ? ((int64_t)1 << (shift - 1)) : 0;
+    for (int32_t j = 0; j < line; j++)
+        for (int i = 0; i < n; i++) {
+            if (j >= line - skipLine) { out[j * os + i] = 0; continue; }
+            int64_t s = 0;
+            for (int k = 0; k < nz; k++) s += (int64_t)T[k * n + i] * in[k * is + j];
+            out[j * os + i] = clip16((s + rnd) >> shift);
+        }
+}
+
+/* Xin266IDct2WxH (h266_inverse_trans.c:51-89): vertical then horizontal
+ * DCT-II through tmp (height-strided), coefficient columns past 32 skipped. */
+void or_IDct2WxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+                 uint32_t w, uint32_t h, or_coeff *tmp, int32_t s1, int32_t s2)
+{
+    const int32_t skipW = w > 32 ? (int32_t)w - 32 : 0;
+    or_IDct1d(OR_DCT2, (int)h, in, is, tmp, h, (int32_t)w, skipW, s1);
+    or_IDct1d(OR_DCT2, (int)w, tmp, h, out, os, (int32_t)h, 0, s2);
+}
+
+/* Xin266ISkipWxH (h266_inverse_trans.c:91-113): arithmetic right shift by
+ * 15 - 8 - ((lgW + lgH) >> 1), truncated to int16 (the shift uses the 8-bit
+ * constant whatever the internal bit depth). */
+void or_ISkipWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os, uint32_t w, uint32_t h)
+{
+    int lgW = 0, lgH = 0;
+    while ((1u << lgW) < w) lgW++;
+    while ((1u << lgH) < h) lgH++;
+    const int shift = 15 - 8 - ((lgW + lgH) >> 1);
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) out[y * os + x] = (int16_t)(in[y * is + x] >> shift);
+}
+
+/* Xin266IDctWxH (h266_inverse_trans.c:115-229) for the two-sided cases:
+ * shifts 7 and 20 - bitDepth (:145-146), the MTS zero-out skip (:147), the
+ * implicit intra-luma DST-VII (:149-159), then 2-D DCT-II, skip or two 1-D
+ * passes.  Returns -1 for the one-sided branches (:201-228, whose result the
+ * reference leaves in its section scratch buffer) and unsupported kernels. */
+int or_IDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+               uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts, int bitDepth)
+{
+    int32_t hor, ver;
+    int lgW = 0, lgH = 0;
+    while ((1u << lgW) < w) lgW++;
+    while ((1u << lgH) < h) lgH++;
+    if (mtsIdx > 5) return -1;
+    or_MtsTypes(mtsIdx, &hor, &ver);
+    const int32_t s1 = 6 + 1, s2 = 6 + 15 - 1 - bitDepth;
+    const int32_t skipW = (hor != OR_DCT2 && w == 32) ? 16 : (w > 32 ? (int32_t)w - 32 : 0);
+    if (isIntraLuma && enableMts && mtsIdx == 0) {
+        if (w >= 4 && w <= 16) hor = OR_DST7;
+        if (h >= 4 && h <= 16) ver = OR_DST7;
+    }
+    if (hor == OR_SKIP) {
+        or_ISkipWxH(in, is, out, os, w, h);
+        return 0;
+    }
+    if (lgW == 0 || lgH == 0) return -1;
+    int32_t T[64 * 64];
+    if (or_TrMatrix(hor, (int)w, T) || or_TrMatrix(ver, (int)h, T)) return -1;
+    or_coeff *tmp = (or_coeff *)malloc(sizeof(or_coeff) * w * h);
+    if (hor == OR_DCT2 && ver == OR_DCT2) {
+        or_IDct2WxH(in, is, out, os, w, h, tmp, s1, s2);
+    } else {
+        or_IDct1d(ver, (int)h, in, is, tmp, h, (int32_t)w, skipW, s1);
+        or_IDct1d(hor, (int)w, tmp, h, out, os, (int32_t)h, 0, s2);
+    }
+    free(tmp);
+    return 0;
+}
+
+/* XinBlockAddForDct (h266_func_struct.h:530-538; Xin26xBlockRecon is
+ * closed): recon = Clip3(0, 2^bitDepth - 1, pred + residual). */
+void or_BlockRecon(const or_coeff *res, intptr_t rs, const or_pixel *pred, intptr_t ps,
+                   or_pixel *recon, intptr_t os, uint32_t w, uint32_t h, int bitDepth)
+{
+    const int32_t maxv = (1 << bitDepth) - 1;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            int32_t v = (int32_t)pred[y * ps + x] + res[y * rs + x];
+            recon[y * os + x] = (or_pixel)(v < 0 ? 0 : v > maxv ? maxv : v);
+        }
+}
+
+/* Xin266IDctAdd (h266_idct_add.c:45-104) without the LMCS chroma residual
+ * scaling: inverse transform into a 64-strided buffer, then the recon add. */
+int or_IDctAdd(const or_coeff *in, intptr_t is, const or_pixel *pred, intptr_t ps, or_pixel *out,
+               intptr_t os, uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts,
+               int bitDepth)
+{
+    or_coeff buf[64 * 64];
+    if (w > 64 || h > 64) return -1;
+    if (or_IDctWxH(in, is, buf, 64, w, h, mtsIdx, isIntraLuma, enableMts, bitDepth)) return -1;
+    or_BlockRecon(buf, 64, pred, ps, out, os, w, h, bitDepth);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
  * Frame-level drivers (checker for the batched kernels).
  * ------------------------------------------------------------------------- */
 

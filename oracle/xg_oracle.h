@@ -151,6 +151,20 @@ void or_FDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
                 uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts,
                 int bitDepth);
 void or_MtsTypes(uint32_t mtsIdx, int32_t *horTx, int32_t *verTx);
+
+/* ---- inverse transforms and reconstruction ---- */
+void or_IDct1d(int type, int n, const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+               int32_t line, int32_t skipLine, int32_t shift);
+void or_IDct2WxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+                 uint32_t w, uint32_t h, or_coeff *tmp, int32_t s1, int32_t s2);
+void or_ISkipWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os, uint32_t w, uint32_t h);
+int  or_IDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
+                uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts, int bitDepth);
+void or_BlockRecon(const or_coeff *res, intptr_t rs, const or_pixel *pred, intptr_t ps,
+                   or_pixel *recon, intptr_t os, uint32_t w, uint32_t h, int bitDepth);
+int  or_IDctAdd(const or_coeff *in, intptr_t is, const or_pixel *pred, intptr_t ps, or_pixel *out,
+                intptr_t os, uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts,
+                int bitDepth);
 /* Blocks [i0, i1) of a batch in the layout of the batched GPU entry
  * (int16 residuals at resid + offset[i], desc = lgW | lgH << 4 | mtsIdx << 8). */
 void or_FDctBatch(const int16_t *resid, const int64_t *offset, const uint32_t *desc, int32_t i0, int32_t i1,

@@ -68,6 +68,13 @@ def load_oracle():
         "or_FDct2WxH": (None, [P, SS, P, SS, U32, U32, P, I32, I32]),
         "or_FDctWxH": (None, [P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "or_FDctBatch": (None, [P, P, P, I32, I32, P]),
+        "or_IDct1d": (None, [ctypes.c_int, ctypes.c_int, P, SS, P, SS, I32, I32, I32]),
+        "or_IDct2WxH": (None, [P, SS, P, SS, U32, U32, P, I32, I32]),
+        "or_ISkipWxH": (None, [P, SS, P, SS, U32, U32]),
+        "or_IDctWxH": (ctypes.c_int, [P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "or_BlockRecon": (None, [P, SS, P, SS, P, SS, U32, U32, ctypes.c_int]),
+        "or_IDctAdd": (ctypes.c_int, [P, SS, P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]),
         "or_QuantInvQuant": (None, [P, P, P, SS, U32, U32, U32, U32, I32, I32, I32, I32, I32, I32, P]),
         "or_MePipeline": (None, [P, P, P, SS, SS, P, P, P, P, P, P, SS, SS, I32, I32, I32, I32, I32, U32, I32,
                                  ctypes.POINTER(OrRefinePrm)] + [P] * 15 + [SS, SS]),

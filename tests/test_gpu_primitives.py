@@ -128,6 +128,93 @@ def test_fdct2d_and_dispatcher(lib, oracle):
                     assert np.array_equal(got, exp), (w, h, mts, intra)
 
 
+IDCT = [(n.replace("FD", "ID"), t, k) for (n, t, k) in FDCT]
+
+
+def _coeffs(rng, h, w, amp):
+    """Dequantiser-range coefficients: a sparse low-frequency corner with a
+    few values at the int16 limits."""
+    c = np.zeros((h, w), np.int64)
+    c[:] = rng.integers(-amp, amp + 1, size=(h, w))
+    c[rng.random((h, w)) < 0.5] = 0
+    flat = c.reshape(-1)
+    flat[rng.integers(0, flat.size, max(1, flat.size // 32))] = 32767
+    flat[rng.integers(0, flat.size, max(1, flat.size // 32))] = -32768
+    return c.astype(np.int32)
+
+
+@pytest.mark.parametrize("name,typ,n", IDCT)
+def test_idct1d(lib, oracle, name, typ, n):
+    rng = np.random.default_rng(n * 11 + typ)
+    for line, skip, shift, amp in ((1, 0, 7, 300), (n, 0, 7, 32768), (37, 5, 10, 4000), (64, 32, 0, 200),
+                                   (8, 8, 7, 50)):
+        inp = _coeffs(rng, n, line + 3, amp)
+        got = np.full((line, n + 5), -7, np.int32)
+        exp = np.full((line, n + 5), -7, np.int32)
+        getattr(lib, name)(ptr(inp), line + 3, ptr(got), n + 5, line, skip, shift)
+        oracle.or_IDct1d(typ, n, ptr(inp), line + 3, ptr(exp), n + 5, line, skip, shift)
+        assert np.array_equal(got, exp), (name, line, skip, shift)
+
+
+def test_idct2d_skip_and_dispatcher(lib, oracle):
+    rng = np.random.default_rng(23)
+    for w in (2, 4, 8, 16, 32, 64):
+        for h in (2, 4, 8, 16, 32, 64):
+            c = _coeffs(rng, h, w + 2, 2000)
+            got = np.full((h, w + 1), -7, np.int32)
+            exp = np.full((h, w + 1), -7, np.int32)
+            tmp = np.zeros(w * h, np.int32)
+            lib.Xg266IDct2WxH(ptr(c), w + 2, ptr(got), w + 1, w, h, ptr(tmp), 7, 10)
+            oracle.or_IDct2WxH(ptr(c), w + 2, ptr(exp), w + 1, w, h, ptr(tmp), 7, 10)
+            assert np.array_equal(got, exp), (w, h)
+            for mts in (0, 1, 2, 3, 4, 5):
+                for intra, en in ((0, 0), (1, 1)):
+                    if mts > 1 and (w < 4 or h < 4 or w > 32 or h > 32):
+                        continue
+                    if intra and en and mts == 0 and (w < 4 or h < 4):
+                        continue
+                    got[:] = -7
+                    exp[:] = -7
+                    rc = lib.Xg266IDctWxH(ptr(c), w + 2, ptr(got), w + 1, w, h, mts, intra, en)
+                    assert rc == 0
+                    assert oracle.or_IDctWxH(ptr(c), w + 2, ptr(exp), w + 1, w, h, mts, intra, en, 10) == 0
+                    assert np.array_equal(got, exp), (w, h, mts, intra)
+    for w in (1, 2, 128):
+        for h in (1, 16, 128):
+            c = rng.integers(-2 ** 31, 2 ** 31, size=(h, w), dtype=np.int64).astype(np.int32)
+            got = np.zeros((h, w), np.int32)
+            exp = np.zeros((h, w), np.int32)
+            lib.Xg266ISkipWxH(ptr(c), w, ptr(got), w, w, h)
+            oracle.or_ISkipWxH(ptr(c), w, ptr(exp), w, w, h)
+            assert np.array_equal(got, exp), (w, h)
+    c = np.zeros((64, 64), np.int32)
+    for (w, h, mts) in ((1, 16, 0), (16, 1, 2), (64, 8, 2), (8, 8, 6), (128, 8, 0)):
+        assert lib.Xg266IDctWxH(ptr(c), 64, ptr(c), 64, w, h, mts, 0, 0) == capi.XG_ERR_ARG
+        lib.xg_clear_error()
+
+
+def test_block_recon_and_idct_add(lib, oracle):
+    rng = np.random.default_rng(24)
+    pred = rand_pixels(rng, (64, 70))
+    for w in (2, 4, 8, 16, 32, 64):
+        for h in (2, 4, 8, 16, 32, 64):
+            res = rng.integers(-1500, 1500, size=(h, w + 3)).astype(np.int32)
+            got = np.full((h, w + 1), 7, np.uint16)
+            exp = np.full((h, w + 1), 7, np.uint16)
+            lib.Xg26xBlockRecon(ptr(res), w + 3, ptr(pred), 70, ptr(got), w + 1, w, h)
+            oracle.or_BlockRecon(ptr(res), w + 3, ptr(pred), 70, ptr(exp), w + 1, w, h, 10)
+            assert np.array_equal(got, exp), (w, h)
+            c = _coeffs(rng, h, w, 3000)
+            for mts in (0, 1, 3):
+                if mts == 3 and (w < 4 or h < 4 or w > 32 or h > 32):
+                    continue
+                got[:] = 7
+                exp[:] = 7
+                assert lib.Xg266IDctAdd(ptr(c), w, ptr(pred), 70, ptr(got), w + 1, w, h, mts, 1, 1) == 0
+                assert oracle.or_IDctAdd(ptr(c), w, ptr(pred), 70, ptr(exp), w + 1, w, h, mts, 1, 1, 10) == 0
+                assert np.array_equal(got, exp), (w, h, mts)
+
+
 def test_block_sub(lib, oracle):
     rng = np.random.default_rng(22)
     a = rand_pixels(rng, (64, 70))
@@ -198,6 +285,13 @@ def test_func_table(lib):
     assert t.pfXinComputeAvgDist[4] == addr("Xg266ComputeAvgSatd")
     assert t.pfXinComputeAvgSadx8[0] == addr("Xg266ComputeAvgSadx8")
     assert t.pfXinQuantInvQuant[6] == addr("Xg266QuantInvQuant") and t.pfXinQuantInvQuant[7] is None
+    assert t.pfXinInv1dDct[0 * 8 + 6] == addr("Xg26xIDct2P64") and t.pfXinInv1dDct[0 * 8 + 0] is None
+    assert t.pfXinInv1dDct[1 * 8 + 5] == addr("Xg26xIDct8P32") and t.pfXinInv1dDct[1 * 8 + 6] is None
+    assert t.pfXinInv1dDct[2 * 8 + 2] == addr("Xg26xIDst7P4") and t.pfXinInv1dDct[2 * 8 + 1] is None
+    assert t.pfXinInv2dDct2[3 * 8 + 5] == addr("Xg266IDct2WxH") and t.pfXinInv2dDct2[7 * 8 + 3] is None
+    assert t.pfXinInv2dSkip[7] == addr("Xg266ISkipWxH") and t.pfXinInv2dSkip[0] == addr("Xg266ISkipWxH")
+    assert t.pfXinBlockAddForDct[1] == addr("Xg26xBlockRecon") and t.pfXinBlockAddForDct[6] is not None
+    assert t.pfXinBlockAddForDct[0] is None and t.pfXinBlockAddForDct[7] is None
     assert lib.Xg266FuncInit(ctypes.byref(t), 0) == 0
     assert t.pfXinComputeDist[3] == addr("Xg266ComputeSad")
     assert t.pfXinComputeAvgDist[4] == addr("Xg266ComputeAvgSad")

@@ -339,6 +339,153 @@ def test_dispatcher_types_and_implicit_mts(oracle):
     assert np.array_equal(out2, two_pass(r2, 2, 0))
 
 
+# ------------------------------------------------------- inverse transforms
+_MTS_TYPES = {0: (0, 0), 2: (2, 2), 3: (1, 2), 4: (2, 1), 5: (1, 1)}
+
+
+def _nonzero(typ, n):
+    return min(n, 16 if typ else 32)
+
+
+@pytest.mark.parametrize("typ,n", [(0, 2), (0, 4), (0, 8), (0, 16), (0, 32), (0, 64),
+                                   (2, 4), (2, 8), (2, 16), (2, 32), (1, 4), (1, 8), (1, 16), (1, 32)])
+def test_idct1d_impulse_is_float_basis_row(oracle, typ, n):
+    """A unit coefficient k on one line, shift 0, gives row k of the
+    transform read across the samples: within 1.6*A of the float basis
+    (pins orientation: a transposed matrix gives column k instead)."""
+    line, A = 3, 5
+    F = _float_basis(typ, n)
+    for k in range(_nonzero(typ, n)):
+        x = np.zeros((n, line), np.int32)
+        x[k, 1] = A
+        out = np.full((line, n), 77, np.int32)
+        oracle.or_IDct1d(typ, n, ptr(x), line, ptr(out), n, line, 0, 0)
+        assert np.max(np.abs(out[1] - A * F[k])) <= 1.6 * A, k
+        assert (out[0] == 0).all() and (out[2] == 0).all()
+
+
+def test_idct1d_round_clip_and_skip(oracle):
+    """Round half up, clip to int16 (H.266 8.7.4.2 Clip3(coeffMin, coeffMax)),
+    and skipped lines written as zero whatever their input."""
+    n, line = 8, 4
+    x = np.zeros((n, line), np.int32)
+    x[0, :] = (100, -100, 2000, 7)          # DC row: every sample = 64*x
+    out = np.zeros((line, n), np.int32)
+    oracle.or_IDct1d(0, n, ptr(x), line, ptr(out), n, line, 1, 7)
+    assert (out[0] == (64 * 100 + 64) >> 7).all() and (out[1] == (-6400 + 64) >> 7).all()
+    assert (out[2] == 1000).all() and (out[3] == 0).all()
+    oracle.or_IDct1d(0, n, ptr(x), line, ptr(out), n, line, 0, 0)
+    assert (out[2] == 32767).all()                          # 64*2000 saturates
+    x[0, 2] = -2000
+    oracle.or_IDct1d(0, n, ptr(x), line, ptr(out), n, line, 0, 0)
+    assert (out[2] == -32768).all()
+
+
+@pytest.mark.parametrize("mts", [0, 2, 3, 4, 5])
+def test_idct2d_against_float_inverse(oracle, mts):
+    """Every 2-D shape against the orthonormal float inverse scaled by
+    sqrt(WH)/32 (the product of the shifts 7 and 10 and the 64*sqrt(N)
+    matrix gains): RMS error within 5% -- a wrong shift is off by 2x, a
+    transposed or sign-flipped basis by far more."""
+    hor, ver = _MTS_TYPES[mts]
+    sizes = (2, 4, 8, 16, 32, 64) if mts == 0 else (4, 8, 16, 32)
+    for w in sizes:
+        for h in sizes:
+            C = np.zeros((h, w), np.int32)
+            C[:_nonzero(ver, h), :_nonzero(hor, w)] = RNG.integers(-300, 301, size=(_nonzero(ver, h), _nonzero(hor, w)))
+            y = np.zeros((h, w), np.int32)
+            assert oracle.or_IDctWxH(ptr(C), w, ptr(y), w, w, h, mts, 0, 0, 10) == 0
+            Fh = _float_basis(ver, h) / (64 * np.sqrt(h))
+            Fw = _float_basis(hor, w) / (64 * np.sqrt(w))
+            ideal = np.sqrt(w * h) / 32 * Fh.T @ C @ Fw
+            err = np.sqrt(np.mean((y - ideal) ** 2))
+            assert err <= 0.05 * np.sqrt(np.mean(ideal ** 2)) + 0.5, (w, h, err)
+
+
+def test_idct_inverts_fdct(oracle):
+    """Forward then inverse returns the 10-bit residual to within 1.5% RMS
+    for every shape without a zero-out (the integer bases are orthogonal to
+    about 2%)."""
+    for mts, (hor, ver) in _MTS_TYPES.items():
+        sizes = (2, 4, 8, 16, 32) if mts == 0 else (4, 8, 16)
+        for w in sizes:
+            for h in sizes:
+                r = RNG.integers(-1023, 1024, size=(h, w)).astype(np.int32)
+                c = np.zeros((h, w), np.int32)
+                b = np.zeros((h, w), np.int32)
+                oracle.or_FDctWxH(ptr(r), w, ptr(c), w, w, h, mts, 0, 0, 10)
+                assert oracle.or_IDctWxH(ptr(c), w, ptr(b), w, w, h, mts, 0, 0, 10) == 0
+                assert np.sqrt(np.mean((b - r) ** 2)) <= 0.015 * np.sqrt(np.mean(r.astype(float) ** 2)), (mts, w, h)
+
+
+def test_idct2d_dc_closed_form(oracle):
+    for (w, h) in ((2, 2), (4, 4), (8, 32), (64, 16), (64, 64)):
+        for c in (-32768, -1023, -7, 5, 1023, 32767):
+            C = np.zeros((h, w), np.int32)
+            C[0, 0] = c
+            y = np.zeros((h, w), np.int32)
+            assert oracle.or_IDctWxH(ptr(C), w, ptr(y), w, w, h, 0, 0, 0, 10) == 0
+            g = (64 * c + 64) >> 7
+            assert (y == (64 * g + 512) >> 10).all(), (w, h, c)
+
+
+def test_idct_zero_out_region_is_not_read(oracle):
+    """Coefficients past 32 (DCT-II-64) and past 16 (32-point DST-VII/DCT-VIII)
+    do not reach the output (H.266 8.7.4.2 nonZeroW/nonZeroH)."""
+    for (w, h, mts) in ((64, 64, 0), (64, 8, 0), (16, 64, 0), (32, 32, 2), (32, 8, 3), (8, 32, 4)):
+        hor, ver = _MTS_TYPES[mts]
+        C = np.zeros((h, w), np.int32)
+        C[:_nonzero(ver, h), :_nonzero(hor, w)] = RNG.integers(-500, 501, size=(_nonzero(ver, h), _nonzero(hor, w)))
+        G = RNG.integers(-500, 501, size=(h, w)).astype(np.int32)
+        G[:_nonzero(ver, h), :_nonzero(hor, w)] = C[:_nonzero(ver, h), :_nonzero(hor, w)]
+        a = np.zeros((h, w), np.int32)
+        b = np.zeros((h, w), np.int32)
+        oracle.or_IDctWxH(ptr(C), w, ptr(a), w, w, h, mts, 0, 0, 10)
+        oracle.or_IDctWxH(ptr(G), w, ptr(b), w, w, h, mts, 0, 0, 10)
+        assert np.array_equal(a, b), (w, h, mts)
+
+
+def test_inverse_dispatcher_types_and_implicit_mts(oracle):
+    """mtsIdx -> (hor, ver), implicit intra-luma DST-VII, bit-depth shift,
+    checked by composing the pinned 1-D passes; unsupported kernels and
+    one-sided blocks are refused."""
+    def two_pass(c, hor, ver, s2=10):
+        h, w = c.shape
+        tmp = np.zeros((w, h), np.int32)
+        out = np.zeros((h, w), np.int32)
+        oracle.or_IDct1d(ver, h, ptr(np.ascontiguousarray(c)), w, ptr(tmp), h, w, 0, 7)
+        oracle.or_IDct1d(hor, w, ptr(tmp), h, ptr(out), w, h, 0, s2)
+        return out
+    c = RNG.integers(-300, 301, size=(8, 16)).astype(np.int32)
+    for mts, (hor, ver) in _MTS_TYPES.items():
+        out = np.zeros((8, 16), np.int32)
+        assert oracle.or_IDctWxH(ptr(c), 16, ptr(out), 16, 16, 8, mts, 0, 1, 10) == 0
+        assert np.array_equal(out, two_pass(c, hor, ver)), mts
+    out = np.zeros((8, 16), np.int32)
+    oracle.or_IDctWxH(ptr(c), 16, ptr(out), 16, 16, 8, 0, 1, 1, 10)
+    assert np.array_equal(out, two_pass(c, 2, 2))
+    oracle.or_IDctWxH(ptr(c), 16, ptr(out), 16, 16, 8, 0, 0, 1, 8)
+    assert np.array_equal(out, two_pass(c, 0, 0, s2=12))
+    c2 = RNG.integers(-300, 301, size=(32, 8)).astype(np.int32)
+    out2 = np.zeros((32, 8), np.int32)
+    oracle.or_IDctWxH(ptr(c2), 8, ptr(out2), 8, 8, 32, 0, 1, 1, 10)
+    assert np.array_equal(out2, two_pass(c2, 2, 0))
+    bad = np.zeros((64, 64), np.int32)
+    assert oracle.or_IDctWxH(ptr(bad), 64, ptr(bad), 64, 64, 64, 2, 0, 0, 10) == -1
+    assert oracle.or_IDctWxH(ptr(bad), 64, ptr(bad), 64, 1, 16, 0, 0, 0, 10) == -1
+    assert oracle.or_IDctWxH(ptr(bad), 64, ptr(bad), 64, 8, 8, 6, 0, 0, 10) == -1
+
+
+def test_block_recon_clips_to_pixel_range(oracle):
+    """H.266 8.7.5: recon = Clip1(pred + res)."""
+    pred = rand_pixels(RNG, (16, 16)).astype(np.uint16)
+    res = RNG.integers(-1500, 1500, size=(16, 16)).astype(np.int32)
+    out = np.zeros((16, 16), np.uint16)
+    oracle.or_BlockRecon(ptr(res), 16, ptr(pred), 16, ptr(out), 16, 16, 16, 10)
+    assert np.array_equal(out, np.clip(pred.astype(np.int64) + res, 0, 1023))
+    assert (out == 0).any() and (out == 1023).any()
+
+
 def test_fdct_batch_matches_per_block_calls(oracle):
     """The batched helper (the transform CPU baseline) is per-block or_FDctWxH
     on the int16 layout of the batched GPU entry."""

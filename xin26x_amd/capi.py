@@ -83,12 +83,15 @@ class XgFuncStruct(ctypes.Structure):
                 ("pfXinLumaInterpS16", P * 32), ("pfXinChromaInterpS16", P * 32),
                 ("pfXinInterpAvg", P * 8), ("pfXinInterpWeight", P * 8),
                 ("pfXinFor2dDct2", P * 64), ("pfXinFor1dDct", P * 24),
-                ("pfXinBlockSubForDct", P * 8), ("pfXinQuantInvQuant", P * 8), ("pfXinComputeHada8x8", P)]
+                ("pfXinBlockSubForDct", P * 8), ("pfXinQuantInvQuant", P * 8),
+                ("pfXinInv2dSkip", P * 8), ("pfXinInv2dDct2", P * 64), ("pfXinInv1dDct", P * 24),
+                ("pfXinBlockAddForDct", P * 8), ("pfXinComputeHada8x8", P)]
 
 
 # Every symbol include/xg266.h declares, with its prototype (restype, argtypes).
 INTERP = (None, [P, SS, P, SS, I32, U32, U32, U32])
 FDCT1D = (None, [P, SS, P, SS, I32, I32])
+IDCT1D = (None, [P, SS, P, SS, I32, I32, I32])
 PROTOS = {
     "xg_last_error": (I32, []),
     "xg_clear_error": (None, []),
@@ -122,6 +125,15 @@ PROTOS = {
     "Xg26xFDct2WxH": (None, [P, SS, P, SS, U32, U32, P, I32, I32]),
     "Xg26xBlockSubForDct": (None, [P, SS, P, SS, P, SS, U32, U32]),
     "Xg266FDctWxH": (I32, [P, SS, P, SS, U32, U32, U32, I32, I32]),
+    "Xg26xIDct2P2": IDCT1D, "Xg26xIDct2P4": IDCT1D, "Xg26xIDct2P8": IDCT1D,
+    "Xg26xIDct2P16": IDCT1D, "Xg26xIDct2P32": IDCT1D, "Xg26xIDct2P64": IDCT1D,
+    "Xg26xIDst7P4": IDCT1D, "Xg26xIDst7P8": IDCT1D, "Xg26xIDst7P16": IDCT1D, "Xg26xIDst7P32": IDCT1D,
+    "Xg26xIDct8P4": IDCT1D, "Xg26xIDct8P8": IDCT1D, "Xg26xIDct8P16": IDCT1D, "Xg26xIDct8P32": IDCT1D,
+    "Xg266IDct2WxH": (None, [P, SS, P, SS, U32, U32, P, I32, I32]),
+    "Xg266ISkipWxH": (None, [P, SS, P, SS, U32, U32]),
+    "Xg266IDctWxH": (I32, [P, SS, P, SS, U32, U32, U32, I32, I32]),
+    "Xg26xBlockRecon": (None, [P, SS, P, SS, P, SS, U32, U32]),
+    "Xg266IDctAdd": (I32, [P, SS, P, SS, P, SS, U32, U32, U32, I32, I32]),
     "Xg266FuncInit": (I32, [ctypes.POINTER(XgFuncStruct), I32]),
     "Xg26xMotionEstFull": (None, [ctypes.POINTER(XgMeStruct)]),
     "Xg26xMotionEstDia": (None, [ctypes.POINTER(XgMeStruct)]),

@@ -230,6 +230,45 @@ __global__ void k_fdct1d(const int32_t *__restrict__ in, int type, int N, int li
     out[(size_t)k * line + j] = v;
 }
 
+// 1-D inverse transform (H.266 8.7.4.5 with the nonZeroS bound of 8.7.4.2):
+// in packed [N][line] (coefficient k of line j at in[k*line + j]), out packed
+// [line][N].  Lines j >= line - skip are written as zero; each output is
+// rounded, shifted and clipped to int16.
+__global__ void k_idct1d(const int32_t *__restrict__ in, int type, int N, int line, int skip, int shift,
+                         int32_t *__restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= line || i >= N) return;
+    int32_t v = 0;
+    if (j < line - skip) {
+        long long s = 0;
+        const int nz = tr_kept(type, N);
+        for (int k = 0; k < nz; k++) s += (long long)tr_entry(c_tr, type, N, k, i) * in[(size_t)k * line + j];
+        const long long rnd = shift > 0 ? (1LL << (shift - 1)) : 0;
+        const long long r = (s + rnd) >> shift;
+        v = (int32_t)(r < -32768 ? -32768 : r > 32767 ? 32767 : r);
+    }
+    out[(size_t)j * N + i] = v;
+}
+
+// Transform-skip inverse: arithmetic shift, truncated to int16.
+__global__ void k_iskip(const int32_t *__restrict__ in, int n, int shift, int32_t *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int16_t)(in[i] >> shift);
+}
+
+// Reconstruction: Clip3(0, 1023, pred + residual).
+__global__ void k_recon(const int32_t *__restrict__ res, const uint16_t *__restrict__ pred, int n,
+                        uint16_t *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = (int)pred[i] + res[i];
+    out[i] = (uint16_t)(v < 0 ? 0 : v > 1023 ? 1023 : v);
+}
+
 __global__ void k_blocksub(const uint16_t *__restrict__ a, const uint16_t *__restrict__ b, int n,
                            int32_t *__restrict__ d)
 {
@@ -467,6 +506,126 @@ static int32_t fdct2d_call(int hor, int ver, const int32_t *in, intptr_t is, int
     if (!finish(c)) return XG_ERR_HIP;
     unpack2d(out, os, c->host + blk, (int)w, (int)h, 4);
     return XG_OK;
+}
+
+// Packs the rows x cols corner of a strided source into a dstW-wide packed
+// buffer: the inverse passes read nothing else (coefficients past the
+// zero-out and skipped lines), so nothing else is read from the caller.
+static void pack_corner(char *dst, const int32_t *src, intptr_t stride, int rows, int cols, int dstW)
+{
+    for (int r = 0; r < rows; r++) memcpy(dst + (size_t)r * dstW * 4, src + (intptr_t)r * stride, (size_t)cols * 4);
+}
+
+// Device-side inverse pass: `in` packed [N][line] -> `out` packed [line][N].
+static bool idct1d_dev(CallCtx *c, const int32_t *din, int type, int N, int line, int skip, int shift,
+                       int32_t *dout)
+{
+    dim3 grid((line + 63) / 64, N), block(64);
+    k_idct1d<<<grid, block, 0, c->stream>>>(din, type, N, line, skip, shift, dout);
+    return XG_HIP(hipGetLastError());
+}
+
+static void idct1d_call(int type, int N, const int32_t *in, intptr_t is, int32_t *out, intptr_t os,
+                        int32_t line, int32_t skip, int32_t shift)
+{
+    if (line <= 0 || line > 128 || skip < 0 || skip > line || shift < 0 || shift > 30) {
+        set_error(XG_ERR_ARG, "IDct 1-D: bad line %d, skipLine %d or shift %d", line, skip, shift);
+        return;
+    }
+    const size_t inB = align_up((size_t)line * N * 4, 256), outB = (size_t)line * N * 4;
+    CallCtx *c = call_ctx(inB + outB, inB + outB);
+    if (!c) return;
+    pack_corner(c->host, in, is, tr_kept(type, N), line - skip, line);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, (size_t)line * N * 4, hipMemcpyHostToDevice, c->stream))) return;
+    if (!idct1d_dev(c, (int32_t *)c->dev, type, N, line, skip, shift, (int32_t *)(c->dev + inB))) return;
+    if (!XG_HIP(hipMemcpyAsync(c->host + inB, c->dev + inB, outB, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(out, os, c->host + inB, N, line, 4);
+}
+
+// Inverse of one TU on device: coefficients d0 packed [h][w] -> residual
+// packed [h][w] in d2 (d1 holds the transposed first pass).  hor == kSKIP
+// runs the transform-skip shift instead.
+static bool itx_dev(CallCtx *c, int hor, int ver, uint32_t w, uint32_t h, int skipW, int s1, int s2,
+                    const int32_t *d0, int32_t *d1, int32_t *d2)
+{
+    if (hor == kSKIP) {
+        const int n = (int)(w * h), shift = 15 - 8 - ((lg2(w) + lg2(h)) >> 1);
+        k_iskip<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(d0, n, shift, d2);
+        return XG_HIP(hipGetLastError());
+    }
+    return idct1d_dev(c, d0, ver, (int)h, (int)w, skipW, s1, d1) &&   // tmp[x*h + y]
+           idct1d_dev(c, d1, hor, (int)w, (int)h, 0, s2, d2);          // res[y*w + x]
+}
+
+// Rows and columns of a TU's coefficients that the inverse reads.
+static void itx_read_region(int hor, int ver, uint32_t w, uint32_t h, int skipW, int *rows, int *cols)
+{
+    if (hor == kSKIP) {
+        *rows = (int)h;
+        *cols = (int)w;
+        return;
+    }
+    *rows = tr_kept(ver, (int)h);
+    *cols = (int)w - skipW;
+}
+
+static int32_t itx_call(int hor, int ver, const int32_t *in, intptr_t is, int32_t *out, intptr_t os, uint32_t w,
+                        uint32_t h, int skipW, int32_t s1, int32_t s2)
+{
+    const size_t n = (size_t)w * h, blk = align_up(n * 4, 256);
+    CallCtx *c = call_ctx(3 * blk, 2 * blk);
+    if (!c) return XG_ERR_HIP;
+    int rows, cols;
+    itx_read_region(hor, ver, w, h, skipW, &rows, &cols);
+    pack_corner(c->host, in, is, rows, cols, (int)w);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, n * 4, hipMemcpyHostToDevice, c->stream))) return XG_ERR_HIP;
+    int32_t *d0 = (int32_t *)c->dev, *d1 = (int32_t *)(c->dev + blk), *d2 = (int32_t *)(c->dev + 2 * blk);
+    if (!itx_dev(c, hor, ver, w, h, skipW, s1, s2, d0, d1, d2)) return XG_ERR_HIP;
+    if (!XG_HIP(hipMemcpyAsync(c->host + blk, d2, n * 4, hipMemcpyDeviceToHost, c->stream))) return XG_ERR_HIP;
+    if (!finish(c)) return XG_ERR_HIP;
+    unpack2d(out, os, c->host + blk, (int)w, (int)h, 4);
+    return XG_OK;
+}
+
+static bool pow2_in(uint32_t v, uint32_t lo, uint32_t hi) { return v >= lo && v <= hi && !(v & (v - 1)); }
+
+// Transform types, zero-out skip and shifts of Xin266IDctWxH
+// (h266_encoder/source/h266_inverse_trans.c:115-199) for 10-bit samples.
+struct ItxPlan {
+    int hor, ver, skipW, s1, s2;
+};
+
+static bool itx_plan(uint32_t w, uint32_t h, uint32_t mtsIdx, int32_t isIntraLuma, int32_t enableMts,
+                     const char *fn, ItxPlan *p)
+{
+    static const int8_t tab[6][2] = {{kDCT2, kDCT2}, {kSKIP, kSKIP}, {kDST7, kDST7},
+                                     {kDCT8, kDST7}, {kDST7, kDCT8}, {kDCT8, kDCT8}};
+    if (mtsIdx > 5 || !pow2_in(w, 1, 128) || !pow2_in(h, 1, 128)) {
+        set_error(XG_ERR_ARG, "%s: unsupported %ux%u mts %u", fn, w, h, mtsIdx);
+        return false;
+    }
+    p->hor = tab[mtsIdx][0];
+    p->ver = tab[mtsIdx][1];
+    p->s1 = 6 + 1;                                                   // :145
+    p->s2 = 6 + 15 - 1 - 10;                                         // :146
+    p->skipW = (p->hor != kDCT2 && w == 32) ? 16 : (w > 32 ? (int)w - 32 : 0);   // :147
+    if (isIntraLuma && enableMts && mtsIdx == 0) {                    // :149-159
+        if (w >= 4 && w <= 16) p->hor = kDST7;
+        if (h >= 4 && h <= 16) p->ver = kDST7;
+    }
+    if (p->hor == kSKIP) return true;                                 // :173-182, any size to 128
+    if (w < 2 || h < 2 || w > 64 || h > 64) {
+        // One-sided blocks (:201-228) leave their result in the reference's
+        // section scratch buffer, so there is no output to reproduce.
+        set_error(XG_ERR_ARG, "%s: unsupported %ux%u for a transform", fn, w, h);
+        return false;
+    }
+    if ((p->hor != kDCT2 && (w < 4 || w > 32)) || (p->ver != kDCT2 && (h < 4 || h > 32))) {
+        set_error(XG_ERR_ARG, "%s: DST-VII/DCT-VIII need sizes 4..32", fn);
+        return false;
+    }
+    return true;
 }
 
 }  // namespace xg
@@ -722,6 +881,107 @@ int32_t Xg266FDctWxH(const xg_coeff *input, intptr_t inputStride, xg_coeff *outp
                        lw + 10 + 6 - 15, lh + 6);                         // :72-73
 }
 
+// ---- inverse transforms and reconstruction ----
+#define XG_IDCT(NAME, TYPE, N)                                                                          \
+    void NAME(xg_coeff *in, intptr_t is, xg_coeff *out, intptr_t os, int32_t line, int32_t skipLine,   \
+              int32_t shift)                                                                            \
+    {                                                                                                   \
+        idct1d_call(TYPE, N, in, is, out, os, line, skipLine, shift);                                   \
+    }
+XG_IDCT(Xg26xIDct2P2, kDCT2, 2)
+XG_IDCT(Xg26xIDct2P4, kDCT2, 4)
+XG_IDCT(Xg26xIDct2P8, kDCT2, 8)
+XG_IDCT(Xg26xIDct2P16, kDCT2, 16)
+XG_IDCT(Xg26xIDct2P32, kDCT2, 32)
+XG_IDCT(Xg26xIDct2P64, kDCT2, 64)
+XG_IDCT(Xg26xIDst7P4, kDST7, 4)
+XG_IDCT(Xg26xIDst7P8, kDST7, 8)
+XG_IDCT(Xg26xIDst7P16, kDST7, 16)
+XG_IDCT(Xg26xIDst7P32, kDST7, 32)
+XG_IDCT(Xg26xIDct8P4, kDCT8, 4)
+XG_IDCT(Xg26xIDct8P8, kDCT8, 8)
+XG_IDCT(Xg26xIDct8P16, kDCT8, 16)
+XG_IDCT(Xg26xIDct8P32, kDCT8, 32)
+
+void Xg266IDct2WxH(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                   uint32_t width, uint32_t height, xg_coeff *tempBuf, int32_t shift1st, int32_t shift2nd)
+{
+    (void)tempBuf;
+    if (!pow2_in(width, 2, 64) || !pow2_in(height, 2, 64) || shift1st < 0 || shift1st > 30 || shift2nd < 0 ||
+        shift2nd > 30) {
+        set_error(XG_ERR_ARG, "Xg266IDct2WxH: unsupported %ux%u or shifts %d/%d", width, height, shift1st,
+                  shift2nd);
+        return;
+    }
+    itx_call(kDCT2, kDCT2, input, inputStride, output, outputStride, width, height,
+             width > 32 ? (int)width - 32 : 0, shift1st, shift2nd);
+}
+
+void Xg266ISkipWxH(xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                   uint32_t width, uint32_t height)
+{
+    if (!pow2_in(width, 1, 128) || !pow2_in(height, 1, 128)) {
+        set_error(XG_ERR_ARG, "Xg266ISkipWxH: unsupported %ux%u", width, height);
+        return;
+    }
+    itx_call(kSKIP, kSKIP, input, inputStride, output, outputStride, width, height, 0, 0, 0);
+}
+
+int32_t Xg266IDctWxH(const xg_coeff *input, intptr_t inputStride, xg_coeff *output, intptr_t outputStride,
+                     uint32_t width, uint32_t height, uint32_t mtsIdx, int32_t isIntraLuma, int32_t enableMts)
+{
+    ItxPlan p;
+    if (!itx_plan(width, height, mtsIdx, isIntraLuma, enableMts, "Xg266IDctWxH", &p)) return XG_ERR_ARG;
+    return itx_call(p.hor, p.ver, input, inputStride, output, outputStride, width, height, p.skipW, p.s1, p.s2);
+}
+
+void Xg26xBlockRecon(xg_coeff *input, intptr_t inputStride, xg_pixel *pred, intptr_t predStride, xg_pixel *recon,
+                     intptr_t reconStride, uint32_t width, uint32_t height)
+{
+    if (!check_wh(width, height, 128, 128, "Xg26xBlockRecon")) return;
+    const size_t n = (size_t)width * height, blk = align_up(n * 4, 256);
+    CallCtx *c = call_ctx(3 * blk, 3 * blk);
+    if (!c) return;
+    pack2d(c->host, input, inputStride, (int)width, (int)height, 4);
+    pack2d(c->host + blk, pred, predStride, (int)width, (int)height, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, 2 * blk, hipMemcpyHostToDevice, c->stream))) return;
+    k_recon<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>((int32_t *)c->dev, (uint16_t *)(c->dev + blk),
+                                                              (int)n, (uint16_t *)(c->dev + 2 * blk));
+    if (!XG_HIP(hipGetLastError())) return;
+    if (!XG_HIP(hipMemcpyAsync(c->host + 2 * blk, c->dev + 2 * blk, n * 2, hipMemcpyDeviceToHost, c->stream))) return;
+    if (!finish(c)) return;
+    unpack2d(recon, reconStride, c->host + 2 * blk, (int)width, (int)height, 2);
+}
+
+int32_t Xg266IDctAdd(const xg_coeff *input, intptr_t inputStride, const xg_pixel *pred, intptr_t predStride,
+                     xg_pixel *output, intptr_t outputStride, uint32_t width, uint32_t height, uint32_t mtsIdx,
+                     int32_t isIntraLuma, int32_t enableMts)
+{
+    ItxPlan p;
+    if (!itx_plan(width, height, mtsIdx, isIntraLuma, enableMts, "Xg266IDctAdd", &p)) return XG_ERR_ARG;
+    if (width > 64 || height > 64) {                    // the reference's 64x64 residual buffer
+        set_error(XG_ERR_ARG, "Xg266IDctAdd: unsupported %ux%u", width, height);
+        return XG_ERR_ARG;
+    }
+    const size_t n = (size_t)width * height, blk = align_up(n * 4, 256);
+    CallCtx *c = call_ctx(5 * blk, 3 * blk);
+    if (!c) return XG_ERR_HIP;
+    int rows, cols;
+    itx_read_region(p.hor, p.ver, width, height, p.skipW, &rows, &cols);
+    pack_corner(c->host, input, inputStride, rows, cols, (int)width);
+    pack2d(c->host + blk, pred, predStride, (int)width, (int)height, 2);
+    if (!XG_HIP(hipMemcpyAsync(c->dev, c->host, 2 * blk, hipMemcpyHostToDevice, c->stream))) return XG_ERR_HIP;
+    int32_t *d0 = (int32_t *)c->dev, *d1 = (int32_t *)(c->dev + 2 * blk), *d2 = (int32_t *)(c->dev + 3 * blk);
+    uint16_t *dp = (uint16_t *)(c->dev + blk), *dr = (uint16_t *)(c->dev + 4 * blk);
+    if (!itx_dev(c, p.hor, p.ver, width, height, p.skipW, p.s1, p.s2, d0, d1, d2)) return XG_ERR_HIP;
+    k_recon<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(d2, dp, (int)n, dr);
+    if (!XG_HIP(hipGetLastError())) return XG_ERR_HIP;
+    if (!XG_HIP(hipMemcpyAsync(c->host + 2 * blk, dr, n * 2, hipMemcpyDeviceToHost, c->stream))) return XG_ERR_HIP;
+    if (!finish(c)) return XG_ERR_HIP;
+    unpack2d(output, outputStride, c->host + 2 * blk, (int)width, (int)height, 2);
+    return XG_OK;
+}
+
 }  // extern "C"
 
 extern "C" {
@@ -745,6 +1005,12 @@ int32_t Xg266FuncInit(xg_func_struct *f, int32_t satdMd)
                                                   Xg26xFDct8P16, Xg26xFDct8P32, nullptr, nullptr};
     static const XgFor1dDct dst7[XG_BLOCK_NUM] = {nullptr, nullptr, Xg26xFDst7P4, Xg26xFDst7P8,
                                                   Xg26xFDst7P16, Xg26xFDst7P32, nullptr, nullptr};
+    static const XgInv1dDct idct2[XG_BLOCK_NUM] = {nullptr, Xg26xIDct2P2, Xg26xIDct2P4, Xg26xIDct2P8,
+                                                   Xg26xIDct2P16, Xg26xIDct2P32, Xg26xIDct2P64, nullptr};
+    static const XgInv1dDct idct8[XG_BLOCK_NUM] = {nullptr, nullptr, Xg26xIDct8P4, Xg26xIDct8P8,
+                                                   Xg26xIDct8P16, Xg26xIDct8P32, nullptr, nullptr};
+    static const XgInv1dDct idst7[XG_BLOCK_NUM] = {nullptr, nullptr, Xg26xIDst7P4, Xg26xIDst7P8,
+                                                   Xg26xIDst7P16, Xg26xIDst7P32, nullptr, nullptr};
     for (int i = 0; i < XG_BLOCK_NUM; i++) {
         f->pfXinComputeSad[i] = Xg266ComputeSad;
         f->pfXinComputeSatd[i] = i >= 2 ? Xg266ComputeSatd : nullptr;
@@ -779,8 +1045,16 @@ int32_t Xg266FuncInit(xg_func_struct *f, int32_t satdMd)
         f->pfXinFor1dDct[1][i] = dct8[i];   // XIN_DCT8
         f->pfXinFor1dDct[2][i] = dst7[i];   // XIN_DST7
         f->pfXinBlockSubForDct[i] = Xg26xBlockSubForDct;
+        f->pfXinInv1dDct[0][i] = idct2[i];  // h266_func_init.c:376-401
+        f->pfXinInv1dDct[1][i] = idct8[i];
+        f->pfXinInv1dDct[2][i] = idst7[i];
+        f->pfXinInv2dSkip[i] = Xg266ISkipWxH;                              // :410-419
+        f->pfXinBlockAddForDct[i] = (i >= 1 && i <= 6) ? Xg26xBlockRecon : nullptr;   // :161-166
         for (int j = 0; j < XG_BLOCK_NUM - 1; j++)
-            if (i < XG_BLOCK_NUM - 1) f->pfXinFor2dDct2[i][j] = Xg26xFDct2WxH;
+            if (i < XG_BLOCK_NUM - 1) {
+                f->pfXinFor2dDct2[i][j] = Xg26xFDct2WxH;
+                f->pfXinInv2dDct2[i][j] = Xg266IDct2WxH;                  // :72-85
+            }
     }
     f->pfXinComputeHada8x8 = Xg266ComputeHada8x8;
     return XG_OK;
