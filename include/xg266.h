@@ -473,7 +473,7 @@ int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best
  * stage 0 the refinement kernel(s) of the eager modes, 1 a whole graph replay,
  * 2 the forward-transform kernel (xg_fdct_run), 3 the integer-search kernel
  * (xg_search_blocks, xg_full_search_frame, the full search of xg_me_pipeline),
- * 4 a whole xg_me_pipeline frame. */
+ * 4 a whole xg_me_pipeline frame, 5 the reconstruction kernel (xg_recon_run). */
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
 int32_t xg_session_reset_timers(xg_session *s);
 int32_t xg_session_set_timing(xg_session *s, int32_t enable);
@@ -599,6 +599,27 @@ int32_t xg_me_pipeline_read(xg_session *s, int16_t *fullMv, int16_t *predMv, int
  * coefficients past the VVC zero-out are written as 0. */
 int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t nblocks);
 int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff);
+
+/* Batched reconstruction: inverse transform of dequantised coefficients and
+ * the prediction add, for TUs whose inverse does not depend on each other
+ * (the batched form of Xin266IDctAdd, h266_encoder/source/h266_idct_add.c:45-104,
+ * as Xin266ReconCu calls it per TU at h266_itrans_recon.c:127-199; the LMCS
+ * chroma scaling is not applied).
+ * xg_recon_plan takes HOST arrays: block i's coefficients are at
+ * coeff + offset[i] (int16 -- the dequantiser's output range --, row-major
+ * [ver freq][hor freq], stride = width), its prediction at pred + offset[i]
+ * and its reconstruction goes to recon + offset[i] (10-bit samples, stride =
+ * width); offsets are multiples of min(8, width) samples.
+ * desc[i] = lgW | lgH << 4 | mtsIdx << 8 | implicitMts << 12: sides 2..64,
+ * mtsIdx 0..5 as h26x_trans_context.h:42-51 (1 = transform skip), and
+ * implicitMts = 1 for an intra luma TU with MTS enabled, which turns DCT-II
+ * sides of 4..16 into DST-VII when mtsIdx is 0 (h266_inverse_trans.c:149-159).
+ * Coefficients past the zero-out (32 for DCT-II-64, 16 for 32-point
+ * DST-VII/DCT-VIII) are not read.  Shifts 7 and 10 (10-bit samples).
+ * xg_recon_run enqueues every planned block on the session stream (device
+ * pointers); recon must not overlap coeff or pred. */
+int32_t xg_recon_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t nblocks);
+int32_t xg_recon_run(xg_session *s, const int16_t *coeff, const xg_pixel *pred, xg_pixel *recon);
 
 #ifdef __cplusplus
 }

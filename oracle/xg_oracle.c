@@ -1111,6 +1111,23 @@ void or_FDctBatch(const int16_t *resid, const int64_t *offset, const uint32_t *d
     }
 }
 
+/* Blocks [i0, i1) of a reconstruction batch in the layout of the batched GPU
+ * entry: int16 coefficients, prediction and recon at offset[i], desc = lgW |
+ * lgH << 4 | mtsIdx << 8 | implicitMts << 12; per block or_IDctAdd
+ * (h266_idct_add.c:45-104) with 10-bit samples. */
+void or_ReconBatch(const int16_t *coeff, const or_pixel *pred, const int64_t *offset, const uint32_t *desc,
+                   int32_t i0, int32_t i1, or_pixel *recon)
+{
+    or_coeff tmp[64 * 64];
+    for (int32_t i = i0; i < i1; i++) {
+        const uint32_t w = 1u << (desc[i] & 15), h = 1u << ((desc[i] >> 4) & 15), mts = (desc[i] >> 8) & 15;
+        const int implicit = (desc[i] >> 12) & 1;
+        const int16_t *c = coeff + offset[i];
+        for (uint32_t k = 0; k < w * h; k++) tmp[k] = c[k];
+        or_IDctAdd(tmp, w, pred + offset[i], w, recon + offset[i], w, w, h, mts, implicit, implicit, 10);
+    }
+}
+
 static int32_t median3(int32_t a, int32_t b, int32_t c)
 {
     int32_t lo = a < b ? a : b, hi = a < b ? b : a;

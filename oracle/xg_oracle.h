@@ -162,6 +162,8 @@ int  or_IDctWxH(const or_coeff *in, intptr_t is, or_coeff *out, intptr_t os,
                 uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts, int bitDepth);
 void or_BlockRecon(const or_coeff *res, intptr_t rs, const or_pixel *pred, intptr_t ps,
                    or_pixel *recon, intptr_t os, uint32_t w, uint32_t h, int bitDepth);
+void or_ReconBatch(const int16_t *coeff, const or_pixel *pred, const int64_t *offset, const uint32_t *desc,
+                   int32_t i0, int32_t i1, or_pixel *recon);
 int  or_IDctAdd(const or_coeff *in, intptr_t is, const or_pixel *pred, intptr_t ps, or_pixel *out,
                 intptr_t os, uint32_t w, uint32_t h, uint32_t mtsIdx, int isIntraLuma, int enableMts,
                 int bitDepth);

@@ -73,6 +73,7 @@ def load_oracle():
         "or_ISkipWxH": (None, [P, SS, P, SS, U32, U32]),
         "or_IDctWxH": (ctypes.c_int, [P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "or_BlockRecon": (None, [P, SS, P, SS, P, SS, U32, U32, ctypes.c_int]),
+        "or_ReconBatch": (None, [P, P, P, P, I32, I32, P]),
         "or_IDctAdd": (ctypes.c_int, [P, SS, P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]),
         "or_QuantInvQuant": (None, [P, P, P, SS, U32, U32, U32, U32, I32, I32, I32, I32, I32, I32, P]),

@@ -171,6 +171,8 @@ PROTOS = {
     "xg_me_pipeline_read": (I32, [P, P, P, P, P, P, P]),
     "xg_fdct_plan": (I32, [P, P, P, I32]),
     "xg_fdct_run": (I32, [P, P, P]),
+    "xg_recon_plan": (I32, [P, P, P, I32]),
+    "xg_recon_run": (I32, [P, P, P, P]),
 }
 
 _lib = None

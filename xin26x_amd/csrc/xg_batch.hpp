@@ -1,0 +1,85 @@
+// xg_batch.hpp -- helpers shared by the batched block kernels (forward
+// transform, reconstruction): group staging of same-shape blocks into LDS,
+// byte packing for int8 MFMA operands, launch sizing and device arrays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xg_runtime.hpp"
+
+namespace xg {
+
+// low bytes of four int32, in order, as one dword
+__device__ __forceinline__ uint32_t bytes4(int a, int b, int c, int d)
+{
+    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+// low bytes of the two 16-bit halves of four dwords, in order, as 8 bytes
+__device__ __forceinline__ long bytes8_lo16(const uint32_t (&m)[4])
+{
+    return (long)(((unsigned long long)__builtin_amdgcn_perm(m[3], m[2], 0x06040200u) << 32) |
+                  __builtin_amdgcn_perm(m[1], m[0], 0x06040200u));
+}
+
+__device__ __forceinline__ long load8(const int8_t *p)
+{
+    return *(const long *)p;      // 8-byte aligned: matrix rows are multiples of 32, k offsets of 8
+}
+
+// Chunks of a group of same-shape blocks: `cs` samples per lane and load, so
+// every load is aligned (block offsets are multiples of cs); chunk c of the
+// group is sample c * cs of the LDS copy, blocks back to back.  Lane b holds
+// block b's offset relative to the group base.  Lanes past the group re-read
+// its last chunk: every load is issued and none waits for another.
+template <typename V, int N>
+__device__ __forceinline__ void stage_group(V (&c)[N], const int16_t *base, int myRel, int lgCpb, int nch, int cs,
+                                            int lane)
+{
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int ch = min(lane + 64 * i, nch - 1);
+        const int o = __shfl(myRel, ch >> lgCpb);
+        c[i] = *reinterpret_cast<const V *>(base + o + (ch & ((1 << lgCpb) - 1)) * cs);
+    }
+}
+
+// every slot is written (past the group: copies of its last chunk), so the
+// loads stay in one straight run ahead of the stores
+template <typename V, int N>
+__device__ __forceinline__ void stage_store(const V (&c)[N], void *lds, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < N; i++) reinterpret_cast<V *>(lds)[lane + 64 * i] = c[i];
+}
+
+// workgroups of 64 threads of `kernel` that fit on the device at once
+inline int resident_slots(const void *kernel)
+{
+    int dev = 0, cus = 256, per = 8;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        cus = prop.multiProcessorCount;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0) != hipSuccess || per < 1) per = 8;
+    return cus * per;
+}
+
+// grows a device array to hold n elements (contents not kept)
+template <typename T>
+bool grow(T *&ptr, int &cap, int n, hipStream_t st)
+{
+    if (n <= cap) return true;
+    if (ptr) {
+        hipStreamSynchronize(st);
+        hipFree(ptr);
+    }
+    ptr = nullptr;
+    cap = 0;
+    if (!XG_HIP(hipMalloc((void **)&ptr, sizeof(T) * (size_t)n))) return false;
+    cap = n;
+    return true;
+}
+
+}  // namespace xg

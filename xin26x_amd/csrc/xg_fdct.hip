@@ -26,6 +26,7 @@
 #include <string.h>
 #include <vector>
 
+#include "xg_batch.hpp"
 #include "xg_device.hpp"
 #include "xg_runtime.hpp"
 #include "xg_trmatrix.hpp"
@@ -143,45 +144,6 @@ struct FdctArgs {
 };
 
 
-// low bytes of four int32, in order, as one dword
-__device__ __forceinline__ uint32_t bytes4(int a, int b, int c, int d)
-{
-    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
-    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x0c0c0400u);
-    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
-}
-
-__device__ __forceinline__ long load_a(const int8_t *p)
-{
-    return *(const long *)p;      // 8-byte aligned: rows are multiples of 32, k offsets of 8
-}
-
-// Residual chunks of a group: min(8, W) samples per lane and load, so every
-// load is aligned (block offsets are multiples of min(8, W)); chunk c of the
-// group is sample c * cs of the LDS copy, blocks back to back.  Lane b holds
-// block b's offset relative to the group base.  Lanes past the group re-read
-// its last chunk: every load is issued and none waits for another.
-template <typename V, int N>
-__device__ __forceinline__ void stage_group(V (&c)[N], const int16_t *base, int myRel, int lgCpb, int nch, int cs,
-                                            int lane)
-{
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const int ch = min(lane + 64 * i, nch - 1);
-        const int o = __shfl(myRel, ch >> lgCpb);
-        c[i] = *reinterpret_cast<const V *>(base + o + (ch & ((1 << lgCpb) - 1)) * cs);
-    }
-}
-
-// every slot is written (past the group: copies of its last chunk), so the
-// loads stay in one straight run ahead of the stores
-template <typename V, int N>
-__device__ __forceinline__ void stage_store(const V (&c)[N], void *lds, int lane)
-{
-#pragma unroll
-    for (int i = 0; i < N; i++) reinterpret_cast<V *>(lds)[lane + 64 * i] = c[i];
-}
-
 // Zeroes the coefficients outside the kept Hk x Wk corner of every block of a
 // group with V-sized stores (V = int4 for widths >= 4; int2 for 2-wide blocks,
 // whose offsets are only 8-byte aligned).  The zero rows Hk..H-1 of a block are
@@ -215,10 +177,8 @@ __device__ __forceinline__ void limbs8(uint4 v, long &blo, long &bhi)
         m[q] = w[q] & 0x007F007Fu;
         hh[q] = as_u32(as_v2s(w[q]) >> (v2s){7, 7});
     }
-    blo = (long)(((unsigned long long)__builtin_amdgcn_perm(m[3], m[2], 0x06040200u) << 32) |
-                 __builtin_amdgcn_perm(m[1], m[0], 0x06040200u));
-    bhi = (long)(((unsigned long long)__builtin_amdgcn_perm(hh[3], hh[2], 0x06040200u) << 32) |
-                 __builtin_amdgcn_perm(hh[1], hh[0], 0x06040200u));
+    blo = bytes8_lo16(m);
+    bhi = bytes8_lo16(hh);
 }
 
 // One workgroup = one wave; it walks groups of same-descriptor blocks.  A group
@@ -270,8 +230,8 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
 #pragma unroll
             for (int ki = 0; ki < 2; ki++) {
                 const bool u1 = 16 * mi < Wk && 32 * ki < W, u2 = 16 * mi < Hk && 32 * ki < H;
-                a1[mi][ki] = load_a(TW + (u1 ? (16 * mi + lr) * cw + 32 * ki : lr * cw) + 8 * lq);
-                a2[mi][ki] = load_a(TH + (u2 ? (16 * mi + lr) * ch + 32 * ki : lr * ch) + 8 * lq);
+                a1[mi][ki] = load8(TW + (u1 ? (16 * mi + lr) * cw + 32 * ki : lr * cw) + 8 * lq);
+                a2[mi][ki] = load8(TH + (u2 ? (16 * mi + lr) * ch + 32 * ki : lr * ch) + 8 * lq);
             }
         if (W >= 8) {
             const int lgCpb = lgW + lgH - 3, nch = G_n << lgCpb;
@@ -515,33 +475,7 @@ int32_t plan_fail(xg_session *s)
     return XG_ERR_ARG;
 }
 
-// grows a device array to hold n elements (contents not kept)
-template <typename T>
-bool grow(T *&ptr, int &cap, int n, hipStream_t st)
-{
-    if (n <= cap) return true;
-    if (ptr) {
-        hipStreamSynchronize(st);
-        hipFree(ptr);
-    }
-    ptr = nullptr;
-    cap = 0;
-    if (!XG_HIP(hipMalloc((void **)&ptr, sizeof(T) * (size_t)n))) return false;
-    cap = n;
-    return true;
-}
 }  // namespace
-
-// workgroups of 64 threads that fit on the device at once
-static int resident_slots(const void *kernel)
-{
-    int dev = 0, cus = 256, per = 8;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-        cus = prop.multiProcessorCount;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0) != hipSuccess || per < 1) per = 8;
-    return cus * per;
-}
 
 extern "C" {
 

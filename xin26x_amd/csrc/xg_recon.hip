@@ -1,0 +1,476 @@
+// xg_recon.hip -- batched reconstruction: inverse DCT-II / DST-VII / DCT-VIII
+// (or transform skip) of dequantised coefficients plus the prediction add, on
+// int8 MFMA.
+//
+// Per block (W x H, int16 coefficients) the inverse of
+// h266_encoder/source/h266_inverse_trans.c:115-199 followed by the add of
+// h266_idct_add.c:95-103, as H.266 clause 8.7.4 defines the arithmetic:
+//   stage 1  E[i][x] = Clip16((sum_{k<nzH} T_H[k][i] * C[k][x] + 64) >> 7),   x < nzW
+//   stage 2  R[i][o] = Clip16((sum_{x<nzW} T_W[x][o] * E[i][x] + 512) >> 10)
+//   recon    Clip3(0, 1023, pred + R)
+// nzW, nzH are the zero-out bounds (32 for DCT-II-64, 16 for 32-point
+// DST-VII/DCT-VIII): coefficients past them are not read, so both contractions
+// have K <= 32, one MFMA step.  Both data operands are split into int8 limbs,
+//   v = 16384*l2 + 128*l1 + l0, l0, l1 in [0,127], l2 in [-2,1]   (3 MFMAs)
+// |limb partial| <= 127*90*32 = 365,760 and |sum| <= 90*32768*32 < 2^31.
+//
+// Operand roles: stage 1 has the coefficient columns in A (M = columns (b, x),
+// K = k) and the basis in B (N = output row i), so each lane receives four
+// consecutive columns x of one row i and packs them straight into the limb
+// planes; stage 2 has the basis in A (M = output column o, K = x) and those
+// limb rows in B (N = rows (b, i)), so each lane receives four consecutive
+// output columns of one row: one 8-byte prediction load and recon store.
+// Both bases are read as R[o][k] = T[k][o] (the transposed matrix, zero for
+// k >= nz), so padding bytes past nz multiply zeros.
+//
+// Operand layout of v_mfma_i32_16x16x32_i8 (one wavefront): lane l supplies
+// A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16] as 8 packed bytes and
+// receives D[4*(l/16) + r][l%16], r = 0..3.
+#include <algorithm>
+#include <mutex>
+#include <string.h>
+#include <vector>
+
+#include "xg_batch.hpp"
+#include "xg_device.hpp"
+#include "xg_runtime.hpp"
+#include "xg_trmatrix.hpp"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+namespace xg {
+
+// Transposed, zero-padded int8 bases: for (type, lgN) max(16, N) rows o of 32
+// bytes, entry k = T[k][o] for k < nz(type, N), zero elsewhere.
+struct RecMats { int off[3][7]; int total; };
+
+static std::once_flag g_rmat_once;
+static int8_t *g_rmats = nullptr;
+static RecMats g_rm;
+static bool g_rmat_ok = false;
+
+static bool rec_mats_init()
+{
+    std::call_once(g_rmat_once, [] {
+        int o = 0;
+        for (int t = 0; t < 3; t++)
+            for (int lg = 1; lg <= 6; lg++) {
+                const int N = 1 << lg;
+                const bool ok = t == kDCT2 ? true : (N >= 4 && N <= 32);
+                g_rm.off[t][lg] = ok ? o : -1;
+                if (ok) o += std::max(16, N) * 32;
+            }
+        g_rm.total = o;
+        std::vector<int8_t> h((size_t)o, 0);
+        for (int t = 0; t < 3; t++)
+            for (int lg = 1; lg <= 6; lg++) {
+                if (g_rm.off[t][lg] < 0) continue;
+                const int N = 1 << lg, nz = tr_kept(t, N);
+                for (int r = 0; r < N; r++)
+                    for (int k = 0; k < nz; k++)
+                        h[(size_t)g_rm.off[t][lg] + (size_t)r * 32 + k] = (int8_t)tr_entry(kTr, t, N, k, r);
+            }
+        if (!XG_HIP(hipMalloc((void **)&g_rmats, h.size()))) return;
+        if (!XG_HIP(hipMemcpy(g_rmats, h.data(), h.size(), hipMemcpyHostToDevice))) return;
+        g_rmat_ok = true;
+    });
+    return g_rmat_ok;
+}
+
+constexpr int kRecGroupMax = 64;    // blocks per group: one relative offset per lane
+constexpr int kRecStage = 2048;     // int16 coefficients staged per group (rows k < nzH)
+constexpr int kRecPlane = 3072;     // bytes per limb plane of the stage-1 output
+constexpr int kRecSlack = 32;       // stage-2 fragments of the last row reach 24 bytes past it
+
+// limb-plane row stride: 8-byte aligned, long rows padded against bank conflicts
+__host__ __device__ inline int rec_stride(int nzW) { return nzW <= 8 ? 8 : nzW + 8; }
+
+struct RecGroup {
+    long long base;     // element offset the group's relative block offsets start from
+    uint32_t desc;      // lgW | lgH << 4 | horTx << 8 | verTx << 12
+    int32_t count;      // blocks, 1..kRecGroupMax, all of this descriptor
+    int32_t first;      // index of the group's first relative offset
+    int32_t pad;
+};
+
+inline int rec_group_cap(int W, int H, int hor, int ver)
+{
+    if (hor == kSKIP) return kRecGroupMax;
+    const int nzW = tr_kept(hor, W), nzH = tr_kept(ver, H);
+    int c = kRecStage / (nzH * W);
+    c = std::min(c, kRecPlane / (H * rec_stride(nzW)));
+    return std::min(c, kRecGroupMax);
+}
+
+struct RecArgs {
+    const int16_t *coeff;
+    const uint16_t *pred;
+    uint16_t *recon;
+    int n;
+    const int8_t *mats;
+    int matOff[3][7];
+};
+
+__device__ __forceinline__ int clip16(int v) { return min(max(v, -32768), 32767); }
+
+// three int8 limbs of 8 int16 values packed two per dword (value order)
+__device__ __forceinline__ void limbs3(const uint32_t (&w)[4], long &b0, long &b1, long &b2)
+{
+    uint32_t m0[4], m1[4], m2[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        m0[q] = w[q] & 0x007F007Fu;
+        m1[q] = (w[q] >> 7) & 0x007F007Fu;
+        m2[q] = as_u32(as_v2s(w[q]) >> (v2s){14, 14});
+    }
+    b0 = bytes8_lo16(m0);
+    b1 = bytes8_lo16(m1);
+    b2 = bytes8_lo16(m2);
+}
+
+// Transform skip (h266_inverse_trans.c:91-113 then the add): two samples per
+// lane and step, straight from global memory.
+__device__ __forceinline__ void recon_skip(const RecArgs &a, const RecGroup &G, int myRel, int lgW, int lgH,
+                                           int lane)
+{
+    const int lgA = lgW + lgH, total = G.count << lgA;
+    const int shift = 15 - 8 - (lgA >> 1);
+    const int16_t *Cg = a.coeff + G.base;
+    const uint16_t *Pg = a.pred + G.base;
+    uint16_t *Og = a.recon + G.base;
+    for (int e0 = 0; e0 < total; e0 += 128) {
+        const int e = e0 + 2 * lane;
+        const int b = min(e >> lgA, G.count - 1);
+        const int o = __shfl(myRel, b) + (e & ((1 << lgA) - 1));
+        if (e < total) {
+            const v2s c = as_v2s(*reinterpret_cast<const uint32_t *>(Cg + o));
+            const uint32_t p = *reinterpret_cast<const uint32_t *>(Pg + o);
+            const int r0 = (int)(p & 0xFFFF) + (int16_t)(c[0] >> shift);
+            const int r1 = (int)(p >> 16) + (int16_t)(c[1] >> shift);
+            *reinterpret_cast<uint32_t *>(Og + o) = pack16(min(max(r0, 0), 1023), min(max(r1, 0), 1023));
+        }
+    }
+}
+
+// One workgroup = one wave; it walks groups of same-descriptor blocks.  A group
+// issues its coefficient loads (rows below the zero-out only) and the basis
+// fragments before its first wait.
+__global__ void __launch_bounds__(64) k_recon_mfma(RecArgs a, const RecGroup *__restrict__ groups,
+                                                    const int32_t *__restrict__ rel)
+{
+    constexpr int NC = kRecStage / 512;             // 16-byte chunks per lane
+    constexpr int PL = kRecPlane + kRecSlack;
+    __shared__ uint4 stg[64 * NC];
+    __shared__ uint64_t limb[3 * PL / 8];
+    const int16_t *S = reinterpret_cast<const int16_t *>(stg);
+    uint8_t *P0 = reinterpret_cast<uint8_t *>(limb), *P1 = P0 + PL, *P2 = P1 + PL;
+    const int lane = threadIdx.x;
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int gi = blockIdx.x; gi < a.n; gi += gridDim.x) {
+        const RecGroup G = groups[gi];
+        const int G_n = G.count;
+        const int myRel = rel[G.first + min(lane, G_n - 1)];
+        const uint32_t desc = G.desc;
+        const int lgW = desc & 15, lgH = (desc >> 4) & 15;
+        const int hor = (desc >> 8) & 15, ver = (desc >> 12) & 15;
+        if (hor == kSKIP) {
+            recon_skip(a, G, myRel, lgW, lgH, lane);
+            continue;
+        }
+        const int W = 1 << lgW, H = 1 << lgH;
+        const int nzW = tr_kept(hor, W), nzH = tr_kept(ver, H);
+        const int lgNzW = 31 - __builtin_clz(nzW), lgNzH = 31 - __builtin_clz(nzH);
+        const int XS = rec_stride(nzW);
+        const int8_t *TH = a.mats + a.matOff[ver][lgH];
+        const int8_t *TW = a.mats + a.matOff[hor][lgW];
+
+        // ---- loads: basis fragments of both stages, then the coefficient rows ----
+        // fragments past the block's side re-read row 0: the loads are
+        // unconditional, so the waits below count them exactly
+        long b1[4], a2[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; mi++) {
+            b1[mi] = load8(TH + (16 * mi < H ? 16 * mi + lr : lr) * 32 + 8 * lq);
+            a2[mi] = load8(TW + (16 * mi < W ? 16 * mi + lr : lr) * 32 + 8 * lq);
+        }
+        const int16_t *Cg = a.coeff + G.base;
+        if (W >= 8) {
+            const int lgCpb = lgNzH + lgW - 3, nch = G_n << lgCpb;
+            uint4 c[NC];
+            stage_group(c, Cg, myRel, lgCpb, nch, 8, lane);
+            stage_store(c, stg, lane);
+        } else if (W == 4) {
+            const int lgCpb = lgNzH, nch = G_n << lgCpb;
+            uint2 c[2 * NC];
+            stage_group(c, Cg, myRel, lgCpb, nch, 4, lane);
+            stage_store(c, stg, lane);
+        } else {
+            const int lgCpb = lgNzH, nch = G_n << lgCpb;
+            uint32_t c[4 * NC];
+            stage_group(c, Cg, myRel, lgCpb, nch, 2, lane);
+            stage_store(c, stg, lane);
+        }
+        __syncthreads();
+
+        // ---- stage 1: M = columns (b, x < nzW), N = i < H, K = k < nzH ----
+        const int cols1 = G_n << lgNzW;
+        for (int n0 = 0; n0 < cols1; n0 += 16) {
+            // A fragment: column m = n0 + lr, rows k = 8 lq .. +7 (clamped reads;
+            // rows past nzH meet zero basis bytes)
+            const int m = min(n0 + lr, cols1 - 1);
+            const int16_t *col = S + ((m >> lgNzW) << (lgNzH + lgW)) + (m & (nzW - 1));
+            uint32_t w[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k0 = min(8 * lq + 2 * q, nzH - 1), k1 = min(8 * lq + 2 * q + 1, nzH - 1);
+                w[q] = pack16(col[k0 << lgW], col[k1 << lgW]);
+            }
+            long f0, f1, f2;
+            limbs3(w, f0, f1, f2);
+            // this lane's D: columns n0 + 4 lq + r (r = 0..3) of output row i
+            const int oc = n0 + 4 * lq, ob = oc >> lgNzW, ox = oc & (nzW - 1);
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++) {
+                if (16 * mi >= H) break;
+                v4i d0 = {64, 64, 64, 64}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};   // rounding rides in d0
+                d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f0, b1[mi], d0, 0, 0, 0);
+                d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f1, b1[mi], d1, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f2, b1[mi], d2, 0, 0, 0);
+                int e[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) e[r] = clip16((d0[r] + (d1[r] << 7) + (d2[r] << 14)) >> 7);
+                const uint32_t L0 = bytes4(e[0], e[1], e[2], e[3]) & 0x7F7F7F7Fu;
+                const uint32_t L1 = bytes4(e[0] >> 7, e[1] >> 7, e[2] >> 7, e[3] >> 7) & 0x7F7F7F7Fu;
+                const uint32_t L2 = bytes4(e[0] >> 14, e[1] >> 14, e[2] >> 14, e[3] >> 14);
+                const int i = 16 * mi + lr;
+                if (i < H && ob < G_n) {
+                    const int o = ((ob << lgH) + i) * XS + ox;
+                    if (nzW >= 4) {
+                        *reinterpret_cast<uint32_t *>(P0 + o) = L0;
+                        *reinterpret_cast<uint32_t *>(P1 + o) = L1;
+                        *reinterpret_cast<uint32_t *>(P2 + o) = L2;
+                    } else {
+                        // 2-wide blocks: columns r = 0, 1 belong to block ob, 2, 3 to block ob + 1
+                        *reinterpret_cast<uint16_t *>(P0 + o) = (uint16_t)L0;
+                        *reinterpret_cast<uint16_t *>(P1 + o) = (uint16_t)L1;
+                        *reinterpret_cast<uint16_t *>(P2 + o) = (uint16_t)L2;
+                        if (ob + 1 < G_n) {
+                            const int o2 = o + (H * XS);
+                            *reinterpret_cast<uint16_t *>(P0 + o2) = (uint16_t)(L0 >> 16);
+                            *reinterpret_cast<uint16_t *>(P1 + o2) = (uint16_t)(L1 >> 16);
+                            *reinterpret_cast<uint16_t *>(P2 + o2) = (uint16_t)(L2 >> 16);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- stage 2: M = output column o < W, N = rows (b, i), K = x < nzW ----
+        const int rows2 = G_n << lgH;
+        const uint16_t *Pg = a.pred + G.base;
+        uint16_t *Og = a.recon + G.base;
+        for (int n0 = 0; n0 < rows2; n0 += 16) {
+            const int row = min(n0 + lr, rows2 - 1);
+            const int lrow = row * XS + 8 * lq;                // bytes past nzW meet zero basis bytes
+            const long p0 = *reinterpret_cast<const long *>(P0 + lrow);
+            const long p1 = *reinterpret_cast<const long *>(P1 + lrow);
+            const long p2 = *reinterpret_cast<const long *>(P2 + lrow);
+            const int b = row >> lgH, i = row & (H - 1);
+            const int rowOff = __shfl(myRel, b) + (i << lgW);
+            const bool rowOk = n0 + lr < rows2;
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++) {
+                if (16 * mi >= W) break;
+                v4i d0 = {512, 512, 512, 512}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
+                d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p0, d0, 0, 0, 0);
+                d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p1, d1, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p2, d2, 0, 0, 0);
+                int v[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = clip16((d0[r] + (d1[r] << 7) + (d2[r] << 14)) >> 10);
+                const int o0 = 16 * mi + 4 * lq;                // this lane's output columns o0 .. o0 + 3
+                if (rowOk && o0 < W) {
+                    const int off = rowOff + o0;
+                    if (W >= 4) {
+                        const uint2 p = *reinterpret_cast<const uint2 *>(Pg + off);
+                        const int q0 = min(max((int)(p.x & 0xFFFF) + v[0], 0), 1023);
+                        const int q1 = min(max((int)(p.x >> 16) + v[1], 0), 1023);
+                        const int q2 = min(max((int)(p.y & 0xFFFF) + v[2], 0), 1023);
+                        const int q3 = min(max((int)(p.y >> 16) + v[3], 0), 1023);
+                        *reinterpret_cast<uint2 *>(Og + off) = make_uint2(pack16(q0, q1), pack16(q2, q3));
+                    } else {
+                        const uint32_t p = *reinterpret_cast<const uint32_t *>(Pg + off);
+                        const int q0 = min(max((int)(p & 0xFFFF) + v[0], 0), 1023);
+                        const int q1 = min(max((int)(p >> 16) + v[1], 0), 1023);
+                        *reinterpret_cast<uint32_t *>(Og + off) = pack16(q0, q1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace xg
+
+// ---------------------------------------------------------------------------
+// Plan / run
+// ---------------------------------------------------------------------------
+using namespace xg;
+
+namespace {
+struct RecPlan {
+    RecGroup *dGroups = nullptr;
+    int32_t *dRel = nullptr;
+    int n = 0, cap = 0, relCap = 0;
+    bool valid = false;
+};
+std::mutex g_rplan_mu;
+std::vector<std::pair<xg_session *, RecPlan>> g_rplans;
+
+RecPlan *rplan_for(xg_session *s, bool create)
+{
+    for (auto &p : g_rplans)
+        if (p.first == s) return &p.second;
+    if (!create) return nullptr;
+    g_rplans.push_back({s, RecPlan()});
+    return &g_rplans.back().second;
+}
+
+int32_t rplan_fail(xg_session *s)
+{
+    std::lock_guard<std::mutex> g(g_rplan_mu);
+    if (RecPlan *p = rplan_for(s, false)) p->valid = false;
+    return XG_ERR_ARG;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t xg_recon_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t n)
+{
+    if (!s || n < 0 || (n > 0 && (!offset || !desc))) {
+        set_error(XG_ERR_ARG, "xg_recon_plan: bad arguments");
+        return s ? rplan_fail(s) : XG_ERR_ARG;
+    }
+    if (!rec_mats_init()) return XG_ERR_HIP;
+    static const int8_t mts[6][2] = {{kDCT2, kDCT2}, {kSKIP, kSKIP}, {kDST7, kDST7},
+                                     {kDCT8, kDST7}, {kDST7, kDCT8}, {kDCT8, kDCT8}};
+    struct Item {
+        long long off;
+        uint32_t desc;
+    };
+    std::vector<Item> items((size_t)n);
+    for (int i = 0; i < n; i++) {
+        const uint32_t d = desc[i];
+        const int lgW = d & 15, lgH = (d >> 4) & 15, m = (d >> 8) & 15, implicit = (d >> 12) & 1;
+        if (lgW < 1 || lgW > 6 || lgH < 1 || lgH > 6 || m > 5 || (d >> 13) || offset[i] < 0) {
+            set_error(XG_ERR_ARG, "xg_recon_plan: block %d has descriptor 0x%x", i, d);
+            return rplan_fail(s);
+        }
+        int hor = mts[m][0], ver = mts[m][1];
+        const int W = 1 << lgW, H = 1 << lgH;
+        if (implicit && m == 0) {                      // h266_inverse_trans.c:149-159
+            if (W >= 4 && W <= 16) hor = kDST7;
+            if (H >= 4 && H <= 16) ver = kDST7;
+        }
+        if (hor != kSKIP && ((hor != kDCT2 && (W < 4 || W > 32)) || (ver != kDCT2 && (H < 4 || H > 32)))) {
+            set_error(XG_ERR_ARG, "xg_recon_plan: block %d: DST-VII/DCT-VIII need sides 4..32", i);
+            return rplan_fail(s);
+        }
+        if (offset[i] % std::min(8, W)) {
+            set_error(XG_ERR_ARG, "xg_recon_plan: block %d: offset must be a multiple of min(8, width) samples", i);
+            return rplan_fail(s);
+        }
+        items[i].off = offset[i];
+        items[i].desc = (uint32_t)(lgW | (lgH << 4) | (hor << 8) | (ver << 12));
+    }
+    // largest blocks first, blocks of one descriptor in memory order
+    std::sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+        const int ax = (x.desc & 15) + ((x.desc >> 4) & 15), ay = (y.desc & 15) + ((y.desc >> 4) & 15);
+        if (ax != ay) return ax > ay;
+        if (x.desc != y.desc) return x.desc < y.desc;
+        return x.off < y.off;
+    });
+    std::vector<RecGroup> groups;
+    std::vector<int32_t> rel((size_t)n);
+    for (int i = 0; i < n;) {
+        const uint32_t d = items[i].desc;
+        const int cap = rec_group_cap(1 << (d & 15), 1 << ((d >> 4) & 15), (d >> 8) & 15, (d >> 12) & 15);
+        RecGroup g;
+        memset(&g, 0, sizeof(g));
+        g.desc = d;
+        g.base = items[i].off;
+        g.first = i;
+        // relative offsets are int32: a group spans less than 2^30 samples
+        while (i < n && items[i].desc == d && g.count < cap && items[i].off - g.base < (1LL << 30)) {
+            rel[(size_t)i] = (int32_t)(items[i].off - g.base);
+            g.count++;
+            i++;
+        }
+        groups.push_back(g);
+    }
+    const int ng = (int)groups.size();
+    std::lock_guard<std::mutex> g(g_rplan_mu);
+    RecPlan *p = rplan_for(s, true);
+    p->valid = false;
+    hipStream_t st = (hipStream_t)xg_session_stream(s);
+    if (!grow(p->dGroups, p->cap, std::max(ng, 1), st) || !grow(p->dRel, p->relCap, std::max(n, 1), st))
+        return XG_ERR_NOMEM;
+    if (ng && !XG_HIP(hipMemcpyAsync(p->dGroups, groups.data(), sizeof(RecGroup) * (size_t)ng, hipMemcpyHostToDevice,
+                                     st)))
+        return XG_ERR_HIP;
+    if (n && !XG_HIP(hipMemcpyAsync(p->dRel, rel.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st)))
+        return XG_ERR_HIP;
+    if (!XG_HIP(hipStreamSynchronize(st))) return XG_ERR_HIP;
+    p->n = ng;
+    p->valid = true;
+    return XG_OK;
+}
+
+int32_t xg_recon_run(xg_session *s, const int16_t *coeff, const xg_pixel *pred, xg_pixel *recon)
+{
+    if (!s || !coeff || !pred || !recon) {
+        set_error(XG_ERR_ARG, "xg_recon_run: bad arguments");
+        return XG_ERR_ARG;
+    }
+    RecPlan *p;
+    {
+        std::lock_guard<std::mutex> g(g_rplan_mu);
+        p = rplan_for(s, false);
+    }
+    if (!p || !p->valid) {
+        set_error(XG_ERR_STATE, "xg_recon_run: no plan (call xg_recon_plan first; a failed plan leaves none)");
+        return XG_ERR_STATE;
+    }
+    if (p->n == 0) return XG_OK;
+    RecArgs a;
+    a.coeff = coeff;
+    a.pred = pred;
+    a.recon = recon;
+    a.n = p->n;
+    a.mats = g_rmats;
+    memcpy(a.matOff, g_rm.off, sizeof(a.matOff));
+    static const int slots = resident_slots((const void *)k_recon_mfma);
+    hipStream_t st = (hipStream_t)xg_session_stream(s);
+    int ev[2];
+    const bool timed = session_timer_begin(s, 5, ev);
+    k_recon_mfma<<<std::min(a.n, slots), 64, 0, st>>>(a, p->dGroups, p->dRel);
+    if (timed) session_timer_end(s, 5, ev);
+    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
+}
+
+}  // extern "C"
+
+void xg::recon_release(xg_session *s)
+{
+    std::lock_guard<std::mutex> g(g_rplan_mu);
+    for (size_t i = 0; i < g_rplans.size(); i++)
+        if (g_rplans[i].first == s) {
+            if (g_rplans[i].second.dGroups) hipFree(g_rplans[i].second.dGroups);
+            if (g_rplans[i].second.dRel) hipFree(g_rplans[i].second.dRel);
+            g_rplans.erase(g_rplans.begin() + (long)i);
+            return;
+        }
+}
