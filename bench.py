@@ -124,12 +124,23 @@ def session_time(lib, s, stage, fn, steps, warmup):
     return ms.value, wall
 
 
+_RESIDUALS = {}
+
+
+def residual_workload():
+    """The 200,000-block residual workload, generated once per process."""
+    from xin26x_amd import workloads as wl
+    if not _RESIDUALS:
+        _RESIDUALS["blocks"] = wl.residual_blocks(n=200000)
+    return _RESIDUALS["blocks"]
+
+
 def transforms_leg(lib, steps, warmup, cpu_seconds):
     """200,000 residual blocks, shapes 4..64 x 4..64, DCT-II or DST-VII/DCT-VIII:
     the batched int8-MFMA forward transform, against the HBM and MFMA rooflines."""
     import torch
-    from xin26x_amd import capi, workloads as wl
-    desc, offs, resid = wl.residual_blocks(n=200000)
+    from xin26x_amd import capi
+    desc, offs, resid = residual_workload()
     n = len(desc)
     w = 1 << (desc & 15).astype(np.int64)
     h = 1 << ((desc >> 4) & 15).astype(np.int64)
@@ -199,6 +210,100 @@ def transforms_cpu(desc, offs, resid, gpu_coeff, seconds):
     exact = all(np.array_equal(out[offs[i0]:offs[i1 - 1] + (1 << (desc[i1 - 1] & 15)) * (1 << ((desc[i1 - 1] >> 4) & 15))],
                                gpu_coeff[offs[i0]:offs[i1 - 1] + (1 << (desc[i1 - 1] & 15)) * (1 << ((desc[i1 - 1] >> 4) & 15))])
                 for i0, i1 in done)
+    return {"value": round(blocks / dt, 1), "unit": "blocks/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C, int64 matrix products, -O3) on the first {blocks} blocks in chunks of "
+                      f"{chunk}, {threads} host threads, {dt:.1f} s",
+            "bit_exact_vs_gpu_on_sample": bool(exact)}
+
+
+PMC_RECON = os.path.join(ROOT, "profiles", "k_recon_mfma_200k_blocks.md")
+
+
+def recon_leg(lib, steps, warmup, cpu_seconds):
+    """Reconstruction of the same 200,000 blocks: their forward coefficients
+    (computed on the GPU, int16) through the batched inverse transform plus
+    the add of a 10-bit prediction, against the HBM and MFMA rooflines."""
+    import torch
+    from xin26x_amd import capi, workloads as wl
+    desc, offs, resid = residual_workload()
+    n = len(desc)
+    w = 1 << (desc & 15).astype(np.int64)
+    h = 1 << ((desc >> 4) & 15).astype(np.int64)
+    mts = (desc >> 8) & 15
+    nz = lambda side: np.where(mts > 0, np.where(side == 32, 16, side), np.minimum(side, 32))
+    nzw, nzh = nz(w), nz(h)
+    samples = int((w * h).sum())
+    macs = int((nzw * h * nzh + h * w * nzw).sum())        # stage 1 kept columns, then every output sample
+    alg_bytes = int((2 * nzh * w + 4 * w * h).sum())        # int16 coefficient rows below the zero-out; pred in, recon out
+    pred = np.minimum((wl.uniform01(6, resid.size, stream=4) * 1024).astype(np.int64), 1023).astype(np.uint16)
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        dres = torch.from_numpy(resid).cuda()
+        dco = torch.empty(resid.size, dtype=torch.int32, device="cuda")
+        capi.check(lib.xg_fdct_plan(s, offs.ctypes.data, desc.ctypes.data, n), "fdct plan")
+        capi.check(lib.xg_fdct_run(s, dres.data_ptr(), dco.data_ptr()), "fdct run")
+        capi.check(lib.xg_session_sync(s), "sync")
+        coeff = dco.clamp(-32768, 32767).to(torch.int16)
+        del dres, dco
+        dpred = torch.from_numpy(pred.view(np.int16)).cuda()
+        drec = torch.empty(resid.size, dtype=torch.int16, device="cuda")
+        capi.check(lib.xg_recon_plan(s, offs.ctypes.data, desc.ctypes.data, n), "recon plan")
+        run = lambda: capi.check(lib.xg_recon_run(s, coeff.data_ptr(), dpred.data_ptr(), drec.data_ptr()),
+                                 "recon run")
+        k_ms, wall_ms = session_time(lib, s, 5, run, steps, warmup)
+        coeff_h = coeff.cpu().numpy()
+        recon = drec.cpu().numpy().view(np.uint16)
+    finally:
+        lib.xg_session_destroy(s)
+    k_s = k_ms / 1e3
+    traffic, src = profiled_traffic(PMC_RECON)
+    out = {"workload": "200,000 TUs of the transform workload: their forward coefficients (int16) through "
+                       "the inverse DCT-II/DST-VII/DCT-VIII (shifts 7 and 10, zero-out) plus a uniform 10-bit "
+                       "prediction, clipped",
+           "value": round(n / (wall_ms / 1e3), 1), "unit": "blocks/s",
+           "samples_per_s": round(samples / (wall_ms / 1e3), 1), "ms_per_batch": round(wall_ms, 4),
+           "kernel": "k_recon_mfma", "kernel_ms": round(k_ms, 4),
+           "roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
+                        "traffic": traffic, "traffic_source": src, "alg_bytes_per_launch": alg_bytes},
+           "mfma": {"achieved": round(2 * macs / k_s / 1e12, 3), "peak": MFMA_I8_PEAK_TOPS, "unit": "TOP/s",
+                    "frac": round(2 * macs / k_s / 1e12 / MFMA_I8_PEAK_TOPS, 6),
+                    "useful_macs_per_launch": macs}}
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = recon_cpu(desc, offs, coeff_h, pred, recon, cpu_seconds)
+    return out
+
+
+def recon_cpu(desc, offs, coeff, pred, gpu_recon, seconds):
+    """The oracle's reconstruction on the host, chunks of the same blocks;
+    the chunks it computes are also compared with the GPU's samples."""
+    from tests.oracle_lib import load_oracle
+    oracle = load_oracle()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = np.zeros(pred.size, np.uint16)
+    chunk, n = 2000, len(desc)
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    done = []
+
+    def work(i0):
+        i1 = min(n, i0 + chunk)
+        oracle.or_ReconBatch(P(coeff), P(pred), P(offs), P(desc), i0, i1, P(out))
+        return i0, i1
+
+    blocks = 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        pos = 0
+        while time.perf_counter() - t0 < seconds and pos < n:
+            starts = [pos + k * chunk for k in range(threads * 2) if pos + k * chunk < n]
+            pos += threads * 2 * chunk
+            for i0, i1 in ex.map(work, starts):
+                done.append((i0, i1))
+                blocks += i1 - i0
+    dt = time.perf_counter() - t0
+    end = lambda i: offs[i] + (1 << (desc[i] & 15)) * (1 << ((desc[i] >> 4) & 15))
+    exact = all(np.array_equal(out[offs[i0]:end(i1 - 1)], gpu_recon[offs[i0]:end(i1 - 1)]) for i0, i1 in done)
     return {"value": round(blocks / dt, 1), "unit": "blocks/s", "cores": threads, "kind": "port",
             "sample": f"oracle (scalar C, int64 matrix products, -O3) on the first {blocks} blocks in chunks of "
                       f"{chunk}, {threads} host threads, {dt:.1f} s",
@@ -503,12 +608,14 @@ def main():
     plain = frame_px / t_plain / 1e6
     plain_tuned = frame_px / t_plain_tuned / 1e6
 
-    transforms = search = pipeline = None
+    transforms = reconstruction = search = pipeline = None
     if not args.quick and not args.no_extra:
         log("full motion-search pipeline, 4K, two references")
         pipeline = pipeline_leg(lib, max(5, args.steps // 4), 2, 0 if args.no_cpu else 10.0)
         log("transforms: 200,000 residual blocks")
         transforms = transforms_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
+        log("reconstruction: the same 200,000 blocks")
+        reconstruction = recon_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
         log("plain 416x240 full search")
         search = search_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
 
@@ -567,6 +674,7 @@ def main():
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "transforms": transforms,
+        "reconstruction": reconstruction,
         "plain_search_416x240": search,
         "pipeline_4k": pipeline,
         "serving": serving,

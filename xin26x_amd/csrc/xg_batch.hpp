@@ -34,13 +34,15 @@ __device__ __forceinline__ long load8(const int8_t *p)
 // group is sample c * cs of the LDS copy, blocks back to back.  Lane b holds
 // block b's offset relative to the group base.  Lanes past the group re-read
 // its last chunk: every load is issued and none waits for another.
-template <typename V, int N>
+// T threads share the work (thread t issues chunks t, t + T, ...); myRel is
+// read from the calling wave's lanes.
+template <typename V, int N, int T = 64>
 __device__ __forceinline__ void stage_group(V (&c)[N], const int16_t *base, int myRel, int lgCpb, int nch, int cs,
-                                            int lane)
+                                            int tid)
 {
 #pragma unroll
     for (int i = 0; i < N; i++) {
-        const int ch = min(lane + 64 * i, nch - 1);
+        const int ch = min(tid + T * i, nch - 1);
         const int o = __shfl(myRel, ch >> lgCpb);
         c[i] = *reinterpret_cast<const V *>(base + o + (ch & ((1 << lgCpb) - 1)) * cs);
     }
@@ -48,21 +50,22 @@ __device__ __forceinline__ void stage_group(V (&c)[N], const int16_t *base, int 
 
 // every slot is written (past the group: copies of its last chunk), so the
 // loads stay in one straight run ahead of the stores
-template <typename V, int N>
-__device__ __forceinline__ void stage_store(const V (&c)[N], void *lds, int lane)
+template <typename V, int N, int T = 64>
+__device__ __forceinline__ void stage_store(const V (&c)[N], void *lds, int tid)
 {
 #pragma unroll
-    for (int i = 0; i < N; i++) reinterpret_cast<V *>(lds)[lane + 64 * i] = c[i];
+    for (int i = 0; i < N; i++) reinterpret_cast<V *>(lds)[tid + T * i] = c[i];
 }
 
-// workgroups of 64 threads of `kernel` that fit on the device at once
-inline int resident_slots(const void *kernel)
+// workgroups of `threads` threads of `kernel` that fit on the device at once
+inline int resident_slots(const void *kernel, int threads = 64)
 {
     int dev = 0, cus = 256, per = 8;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         cus = prop.multiProcessorCount;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0) != hipSuccess || per < 1) per = 8;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1)
+        per = threads <= 64 ? 8 : 2;
     return cus * per;
 }
 

@@ -78,8 +78,8 @@ static bool rec_mats_init()
 }
 
 constexpr int kRecGroupMax = 64;    // blocks per group: one relative offset per lane
-constexpr int kRecStage = 2048;     // int16 coefficients staged per group (rows k < nzH)
-constexpr int kRecPlane = 3072;     // bytes per limb plane of the stage-1 output
+constexpr int kRecStage = 4096;     // int16 coefficients staged per group (rows k < nzH)
+constexpr int kRecPlane = 6144;     // bytes per limb plane of the stage-1 output
 constexpr int kRecSlack = 32;       // stage-2 fragments of the last row reach 24 bytes past it
 
 // limb-plane row stride: 8-byte aligned, long rows padded against bank conflicts
@@ -128,18 +128,21 @@ __device__ __forceinline__ void limbs3(const uint32_t (&w)[4], long &b0, long &b
     b2 = bytes8_lo16(m2);
 }
 
+constexpr int kRecWaves = 4;                   // waves sharing one group
+constexpr int kRecThreads = 64 * kRecWaves;
+
 // Transform skip (h266_inverse_trans.c:91-113 then the add): two samples per
-// lane and step, straight from global memory.
+// thread and step, straight from global memory.
 __device__ __forceinline__ void recon_skip(const RecArgs &a, const RecGroup &G, int myRel, int lgW, int lgH,
-                                           int lane)
+                                           int tid)
 {
     const int lgA = lgW + lgH, total = G.count << lgA;
     const int shift = 15 - 8 - (lgA >> 1);
     const int16_t *Cg = a.coeff + G.base;
     const uint16_t *Pg = a.pred + G.base;
     uint16_t *Og = a.recon + G.base;
-    for (int e0 = 0; e0 < total; e0 += 128) {
-        const int e = e0 + 2 * lane;
+    for (int e0 = 0; e0 < total; e0 += 2 * kRecThreads) {
+        const int e = e0 + 2 * tid;
         const int b = min(e >> lgA, G.count - 1);
         const int o = __shfl(myRel, b) + (e & ((1 << lgA) - 1));
         if (e < total) {
@@ -152,19 +155,45 @@ __device__ __forceinline__ void recon_skip(const RecArgs &a, const RecGroup &G, 
     }
 }
 
-// One workgroup = one wave; it walks groups of same-descriptor blocks.  A group
-// issues its coefficient loads (rows below the zero-out only) and the basis
-// fragments before its first wait.
-__global__ void __launch_bounds__(64) k_recon_mfma(RecArgs a, const RecGroup *__restrict__ groups,
-                                                    const int32_t *__restrict__ rel)
+// How the waves of a group share a stage of `tiles` 16-wide tiles with up to
+// four 16-wide sub-tiles each (mi): whole tiles per wave when there are at
+// least as many tiles as waves (or the tiles do not divide the waves), else
+// the waves of one tile take every (waves / tiles)-th sub-tile.
+struct Split {
+    int t0, tstep, m0, mmask;
+};
+
+__device__ __forceinline__ Split work_split(int tiles, int wave)
 {
-    constexpr int NC = kRecStage / 512;             // 16-byte chunks per lane
+    Split s;
+    if (tiles >= kRecWaves || (kRecWaves % tiles) != 0) {
+        s.t0 = wave;
+        s.tstep = kRecWaves;
+        s.m0 = 0;
+        s.mmask = 0;
+    } else {
+        s.t0 = wave % tiles;
+        s.tstep = kRecWaves;                      // one tile per wave
+        s.m0 = wave / tiles;
+        s.mmask = kRecWaves / tiles - 1;          // sub-tiles m0, m0 + waves/tiles, ...
+    }
+    return s;
+}
+
+// One workgroup = four waves; they walk groups of same-descriptor blocks.  A
+// group issues its coefficient loads (rows below the zero-out only) and the
+// basis fragments before its first wait; each stage's tiles are shared by the
+// four waves.
+__global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const RecGroup *__restrict__ groups,
+                                                             const int32_t *__restrict__ rel)
+{
+    constexpr int NC = kRecStage / (8 * kRecThreads);     // 16-byte chunks per thread
     constexpr int PL = kRecPlane + kRecSlack;
-    __shared__ uint4 stg[64 * NC];
+    __shared__ uint4 stg[kRecThreads * NC];
     __shared__ uint64_t limb[3 * PL / 8];
     const int16_t *S = reinterpret_cast<const int16_t *>(stg);
     uint8_t *P0 = reinterpret_cast<uint8_t *>(limb), *P1 = P0 + PL, *P2 = P1 + PL;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     for (int gi = blockIdx.x; gi < a.n; gi += gridDim.x) {
         const RecGroup G = groups[gi];
@@ -174,7 +203,7 @@ __global__ void __launch_bounds__(64) k_recon_mfma(RecArgs a, const RecGroup *__
         const int lgW = desc & 15, lgH = (desc >> 4) & 15;
         const int hor = (desc >> 8) & 15, ver = (desc >> 12) & 15;
         if (hor == kSKIP) {
-            recon_skip(a, G, myRel, lgW, lgH, lane);
+            recon_skip(a, G, myRel, lgW, lgH, tid);
             continue;
         }
         const int W = 1 << lgW, H = 1 << lgH;
@@ -197,24 +226,26 @@ __global__ void __launch_bounds__(64) k_recon_mfma(RecArgs a, const RecGroup *__
         if (W >= 8) {
             const int lgCpb = lgNzH + lgW - 3, nch = G_n << lgCpb;
             uint4 c[NC];
-            stage_group(c, Cg, myRel, lgCpb, nch, 8, lane);
-            stage_store(c, stg, lane);
+            stage_group<uint4, NC, kRecThreads>(c, Cg, myRel, lgCpb, nch, 8, tid);
+            stage_store<uint4, NC, kRecThreads>(c, stg, tid);
         } else if (W == 4) {
             const int lgCpb = lgNzH, nch = G_n << lgCpb;
             uint2 c[2 * NC];
-            stage_group(c, Cg, myRel, lgCpb, nch, 4, lane);
-            stage_store(c, stg, lane);
+            stage_group<uint2, 2 * NC, kRecThreads>(c, Cg, myRel, lgCpb, nch, 4, tid);
+            stage_store<uint2, 2 * NC, kRecThreads>(c, stg, tid);
         } else {
             const int lgCpb = lgNzH, nch = G_n << lgCpb;
             uint32_t c[4 * NC];
-            stage_group(c, Cg, myRel, lgCpb, nch, 2, lane);
-            stage_store(c, stg, lane);
+            stage_group<uint32_t, 4 * NC, kRecThreads>(c, Cg, myRel, lgCpb, nch, 2, tid);
+            stage_store<uint32_t, 4 * NC, kRecThreads>(c, stg, tid);
         }
         __syncthreads();
 
         // ---- stage 1: M = columns (b, x < nzW), N = i < H, K = k < nzH ----
         const int cols1 = G_n << lgNzW;
-        for (int n0 = 0; n0 < cols1; n0 += 16) {
+        const Split s1 = work_split((cols1 + 15) >> 4, wave);
+        for (int t = s1.t0; 16 * t < cols1; t += s1.tstep) {
+            const int n0 = 16 * t;
             // A fragment: column m = n0 + lr, rows k = 8 lq .. +7 (clamped reads;
             // rows past nzH meet zero basis bytes)
             const int m = min(n0 + lr, cols1 - 1);
@@ -232,6 +263,7 @@ __global__ void __launch_bounds__(64) k_recon_mfma(RecArgs a, const RecGroup *__
 #pragma unroll
             for (int mi = 0; mi < 4; mi++) {
                 if (16 * mi >= H) break;
+                if (mi < s1.m0 || ((mi - s1.m0) & s1.mmask)) continue;
                 v4i d0 = {64, 64, 64, 64}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};   // rounding rides in d0
                 d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f0, b1[mi], d0, 0, 0, 0);
                 d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f1, b1[mi], d1, 0, 0, 0);
@@ -270,7 +302,9 @@ __global__ void __launch_bounds__(64) k_recon_mfma(RecArgs a, const RecGroup *__
         const int rows2 = G_n << lgH;
         const uint16_t *Pg = a.pred + G.base;
         uint16_t *Og = a.recon + G.base;
-        for (int n0 = 0; n0 < rows2; n0 += 16) {
+        const Split s2 = work_split((rows2 + 15) >> 4, wave);
+        for (int t = s2.t0; 16 * t < rows2; t += s2.tstep) {
+            const int n0 = 16 * t;
             const int row = min(n0 + lr, rows2 - 1);
             const int lrow = row * XS + 8 * lq;                // bytes past nzW meet zero basis bytes
             const long p0 = *reinterpret_cast<const long *>(P0 + lrow);
@@ -282,6 +316,7 @@ __global__ void __launch_bounds__(64) k_recon_mfma(RecArgs a, const RecGroup *__
 #pragma unroll
             for (int mi = 0; mi < 4; mi++) {
                 if (16 * mi >= W) break;
+                if (mi < s2.m0 || ((mi - s2.m0) & s2.mmask)) continue;
                 v4i d0 = {512, 512, 512, 512}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
                 d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p0, d0, 0, 0, 0);
                 d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p1, d1, 0, 0, 0);
@@ -452,11 +487,11 @@ int32_t xg_recon_run(xg_session *s, const int16_t *coeff, const xg_pixel *pred, 
     a.n = p->n;
     a.mats = g_rmats;
     memcpy(a.matOff, g_rm.off, sizeof(a.matOff));
-    static const int slots = resident_slots((const void *)k_recon_mfma);
+    static const int slots = resident_slots((const void *)k_recon_mfma, kRecThreads);
     hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
     const bool timed = session_timer_begin(s, 5, ev);
-    k_recon_mfma<<<std::min(a.n, slots), 64, 0, st>>>(a, p->dGroups, p->dRel);
+    k_recon_mfma<<<std::min(a.n, slots), kRecThreads, 0, st>>>(a, p->dGroups, p->dRel);
     if (timed) session_timer_end(s, 5, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
