@@ -10,9 +10,13 @@
 //   recon    Clip3(0, 1023, pred + R)
 // nzW, nzH are the zero-out bounds (32 for DCT-II-64, 16 for 32-point
 // DST-VII/DCT-VIII): coefficients past them are not read, so both contractions
-// have K <= 32, one MFMA step.  Both data operands are split into int8 limbs,
-//   v = 16384*l2 + 128*l1 + l0, l0, l1 in [0,127], l2 in [-2,1]   (3 MFMAs)
-// |limb partial| <= 127*90*32 = 365,760 and |sum| <= 90*32768*32 < 2^31.
+// have K <= 32, one MFMA step.  Both data operands (int16) are split into two
+// signed int8 limbs with an offset,
+//   v = 256*h + l + 128,  h = v >> 8 in [-128,127],  l = (v & 255) - 128 in [-128,127]
+// so sum_k T[k][o] v_k = 256 * (T.h) + (T.l) + 128 * sum_{k<nz} T[k][o]: two
+// MFMAs, and the last term, a constant per output index, starts in the
+// accumulator with the rounding offset.
+// |limb partial| <= 128*90*32 = 368,640 and |sum| <= 90*32768*32 < 2^31.
 //
 // Operand roles: stage 1 has the coefficient columns in A (M = columns (b, x),
 // K = k) and the basis in B (N = output row i), so each lane receives four
@@ -41,8 +45,9 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 namespace xg {
 
 // Transposed, zero-padded int8 bases: for (type, lgN) max(16, N) rows o of 32
-// bytes, entry k = T[k][o] for k < nz(type, N), zero elsewhere.
-struct RecMats { int off[3][7]; int total; };
+// bytes, entry k = T[k][o] for k < nz(type, N), zero elsewhere; then, per
+// (type, lgN), max(16, N) int32 limb-offset terms 128 * sum_{k<nz} T[k][o].
+struct RecMats { int off[3][7]; int coff[3][7]; int total; };
 
 static std::once_flag g_rmat_once;
 static int8_t *g_rmats = nullptr;
@@ -60,15 +65,26 @@ static bool rec_mats_init()
                 g_rm.off[t][lg] = ok ? o : -1;
                 if (ok) o += std::max(16, N) * 32;
             }
+        for (int t = 0; t < 3; t++)
+            for (int lg = 1; lg <= 6; lg++) {
+                const int N = 1 << lg;
+                g_rm.coff[t][lg] = g_rm.off[t][lg] < 0 ? -1 : o;
+                if (g_rm.off[t][lg] >= 0) o += 4 * std::max(16, N);
+            }
         g_rm.total = o;
         std::vector<int8_t> h((size_t)o, 0);
         for (int t = 0; t < 3; t++)
             for (int lg = 1; lg <= 6; lg++) {
                 if (g_rm.off[t][lg] < 0) continue;
                 const int N = 1 << lg, nz = tr_kept(t, N);
-                for (int r = 0; r < N; r++)
-                    for (int k = 0; k < nz; k++)
+                for (int r = 0; r < N; r++) {
+                    int32_t c = 0;
+                    for (int k = 0; k < nz; k++) {
                         h[(size_t)g_rm.off[t][lg] + (size_t)r * 32 + k] = (int8_t)tr_entry(kTr, t, N, k, r);
+                        c += 128 * tr_entry(kTr, t, N, k, r);
+                    }
+                    memcpy(&h[(size_t)g_rm.coff[t][lg] + 4 * (size_t)r], &c, 4);
+                }
             }
         if (!XG_HIP(hipMalloc((void **)&g_rmats, h.size()))) return;
         if (!XG_HIP(hipMemcpy(g_rmats, h.data(), h.size(), hipMemcpyHostToDevice))) return;
@@ -79,7 +95,7 @@ static bool rec_mats_init()
 
 constexpr int kRecGroupMax = 64;    // blocks per group: one relative offset per lane
 constexpr int kRecStage = 4096;     // int16 coefficients staged per group (rows k < nzH)
-constexpr int kRecPlane = 6144;     // bytes per limb plane of the stage-1 output
+constexpr int kRecPlane = 6144;     // bytes per limb plane of the stage-1 output (two planes)
 constexpr int kRecSlack = 32;       // stage-2 fragments of the last row reach 24 bytes past it
 
 // limb-plane row stride: 8-byte aligned, long rows padded against bank conflicts
@@ -109,23 +125,23 @@ struct RecArgs {
     int n;
     const int8_t *mats;
     int matOff[3][7];
+    int constOff[3][7];
 };
 
 __device__ __forceinline__ int clip16(int v) { return min(max(v, -32768), 32767); }
 
-// three int8 limbs of 8 int16 values packed two per dword (value order)
-__device__ __forceinline__ void limbs3(const uint32_t (&w)[4], long &b0, long &b1, long &b2)
+// offset limbs of 8 int16 values packed two per dword (value order):
+// l = (v & 255) - 128 (the low byte with its top bit flipped), h = v >> 8
+__device__ __forceinline__ void limbs2(const uint32_t (&w)[4], long &bl, long &bh)
 {
-    uint32_t m0[4], m1[4], m2[4];
+    uint32_t ml[4], mh[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        m0[q] = w[q] & 0x007F007Fu;
-        m1[q] = (w[q] >> 7) & 0x007F007Fu;
-        m2[q] = as_u32(as_v2s(w[q]) >> (v2s){14, 14});
+        ml[q] = w[q] ^ 0x00800080u;
+        mh[q] = as_u32(as_v2s(w[q]) >> (v2s){8, 8});
     }
-    b0 = bytes8_lo16(m0);
-    b1 = bytes8_lo16(m1);
-    b2 = bytes8_lo16(m2);
+    bl = bytes8_lo16(ml);
+    bh = bytes8_lo16(mh);
 }
 
 constexpr int kRecWaves = 4;                   // waves sharing one group
@@ -190,9 +206,11 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
     constexpr int NC = kRecStage / (8 * kRecThreads);     // 16-byte chunks per thread
     constexpr int PL = kRecPlane + kRecSlack;
     __shared__ uint4 stg[kRecThreads * NC];
-    __shared__ uint64_t limb[3 * PL / 8];
+    __shared__ uint64_t limb[2 * PL / 8];
+    __shared__ int4 cstv[32];                               // offset terms: [0, 64) stage 1, [64, 128) stage 2
+    int32_t *cst = reinterpret_cast<int32_t *>(cstv);
     const int16_t *S = reinterpret_cast<const int16_t *>(stg);
-    uint8_t *P0 = reinterpret_cast<uint8_t *>(limb), *P1 = P0 + PL, *P2 = P1 + PL;
+    uint8_t *PL0 = reinterpret_cast<uint8_t *>(limb), *PH = PL0 + PL;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     for (int gi = blockIdx.x; gi < a.n; gi += gridDim.x) {
@@ -221,6 +239,13 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
         for (int mi = 0; mi < 4; mi++) {
             b1[mi] = load8(TH + (16 * mi < H ? 16 * mi + lr : lr) * 32 + 8 * lq);
             a2[mi] = load8(TW + (16 * mi < W ? 16 * mi + lr : lr) * 32 + 8 * lq);
+        }
+        // limb-offset terms of both bases (max(16, N) entries each) into LDS
+        if (tid < 64) {
+            const int32_t *CH = reinterpret_cast<const int32_t *>(a.mats + a.constOff[ver][lgH]);
+            const int32_t *CW = reinterpret_cast<const int32_t *>(a.mats + a.constOff[hor][lgW]);
+            cst[tid] = CH[min(tid, max(16, H) - 1)];
+            cst[64 + tid] = CW[min(tid, max(16, W) - 1)];
         }
         const int16_t *Cg = a.coeff + G.base;
         if (W >= 8) {
@@ -256,41 +281,38 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
                 const int k0 = min(8 * lq + 2 * q, nzH - 1), k1 = min(8 * lq + 2 * q + 1, nzH - 1);
                 w[q] = pack16(col[k0 << lgW], col[k1 << lgW]);
             }
-            long f0, f1, f2;
-            limbs3(w, f0, f1, f2);
+            long fl, fh;
+            limbs2(w, fl, fh);
             // this lane's D: columns n0 + 4 lq + r (r = 0..3) of output row i
             const int oc = n0 + 4 * lq, ob = oc >> lgNzW, ox = oc & (nzW - 1);
 #pragma unroll
             for (int mi = 0; mi < 4; mi++) {
                 if (16 * mi >= H) break;
                 if (mi < s1.m0 || ((mi - s1.m0) & s1.mmask)) continue;
-                v4i d0 = {64, 64, 64, 64}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};   // rounding rides in d0
-                d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f0, b1[mi], d0, 0, 0, 0);
-                d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f1, b1[mi], d1, 0, 0, 0);
-                d2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(f2, b1[mi], d2, 0, 0, 0);
+                // offset term of output row i and the rounding ride in dl
+                const int c1 = cst[16 * mi + lr] + 64;                            // + stage-1 rounding
+                v4i dl = {c1, c1, c1, c1}, dh = {0, 0, 0, 0};
+                dl = __builtin_amdgcn_mfma_i32_16x16x32_i8(fl, b1[mi], dl, 0, 0, 0);
+                dh = __builtin_amdgcn_mfma_i32_16x16x32_i8(fh, b1[mi], dh, 0, 0, 0);
                 int e[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) e[r] = clip16((d0[r] + (d1[r] << 7) + (d2[r] << 14)) >> 7);
-                const uint32_t L0 = bytes4(e[0], e[1], e[2], e[3]) & 0x7F7F7F7Fu;
-                const uint32_t L1 = bytes4(e[0] >> 7, e[1] >> 7, e[2] >> 7, e[3] >> 7) & 0x7F7F7F7Fu;
-                const uint32_t L2 = bytes4(e[0] >> 14, e[1] >> 14, e[2] >> 14, e[3] >> 14);
+                for (int r = 0; r < 4; r++) e[r] = clip16((dl[r] + (dh[r] << 8)) >> 7);
+                const uint32_t LL = bytes4(e[0], e[1], e[2], e[3]) ^ 0x80808080u;
+                const uint32_t LH = bytes4(e[0] >> 8, e[1] >> 8, e[2] >> 8, e[3] >> 8);
                 const int i = 16 * mi + lr;
                 if (i < H && ob < G_n) {
                     const int o = ((ob << lgH) + i) * XS + ox;
                     if (nzW >= 4) {
-                        *reinterpret_cast<uint32_t *>(P0 + o) = L0;
-                        *reinterpret_cast<uint32_t *>(P1 + o) = L1;
-                        *reinterpret_cast<uint32_t *>(P2 + o) = L2;
+                        *reinterpret_cast<uint32_t *>(PL0 + o) = LL;
+                        *reinterpret_cast<uint32_t *>(PH + o) = LH;
                     } else {
                         // 2-wide blocks: columns r = 0, 1 belong to block ob, 2, 3 to block ob + 1
-                        *reinterpret_cast<uint16_t *>(P0 + o) = (uint16_t)L0;
-                        *reinterpret_cast<uint16_t *>(P1 + o) = (uint16_t)L1;
-                        *reinterpret_cast<uint16_t *>(P2 + o) = (uint16_t)L2;
+                        *reinterpret_cast<uint16_t *>(PL0 + o) = (uint16_t)LL;
+                        *reinterpret_cast<uint16_t *>(PH + o) = (uint16_t)LH;
                         if (ob + 1 < G_n) {
                             const int o2 = o + (H * XS);
-                            *reinterpret_cast<uint16_t *>(P0 + o2) = (uint16_t)(L0 >> 16);
-                            *reinterpret_cast<uint16_t *>(P1 + o2) = (uint16_t)(L1 >> 16);
-                            *reinterpret_cast<uint16_t *>(P2 + o2) = (uint16_t)(L2 >> 16);
+                            *reinterpret_cast<uint16_t *>(PL0 + o2) = (uint16_t)(LL >> 16);
+                            *reinterpret_cast<uint16_t *>(PH + o2) = (uint16_t)(LH >> 16);
                         }
                     }
                 }
@@ -307,9 +329,8 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
             const int n0 = 16 * t;
             const int row = min(n0 + lr, rows2 - 1);
             const int lrow = row * XS + 8 * lq;                // bytes past nzW meet zero basis bytes
-            const long p0 = *reinterpret_cast<const long *>(P0 + lrow);
-            const long p1 = *reinterpret_cast<const long *>(P1 + lrow);
-            const long p2 = *reinterpret_cast<const long *>(P2 + lrow);
+            const long pl = *reinterpret_cast<const long *>(PL0 + lrow);
+            const long ph = *reinterpret_cast<const long *>(PH + lrow);
             const int b = row >> lgH, i = row & (H - 1);
             const int rowOff = __shfl(myRel, b) + (i << lgW);
             const bool rowOk = n0 + lr < rows2;
@@ -317,13 +338,14 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
             for (int mi = 0; mi < 4; mi++) {
                 if (16 * mi >= W) break;
                 if (mi < s2.m0 || ((mi - s2.m0) & s2.mmask)) continue;
-                v4i d0 = {512, 512, 512, 512}, d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
-                d0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p0, d0, 0, 0, 0);
-                d1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p1, d1, 0, 0, 0);
-                d2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], p2, d2, 0, 0, 0);
+                // offset terms of output columns o0 .. o0 + 3 and the rounding ride in dl
+                const int4 c2 = cstv[16 + 4 * mi + lq];
+                v4i dl = {c2.x + 512, c2.y + 512, c2.z + 512, c2.w + 512}, dh = {0, 0, 0, 0};
+                dl = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], pl, dl, 0, 0, 0);
+                dh = __builtin_amdgcn_mfma_i32_16x16x32_i8(a2[mi], ph, dh, 0, 0, 0);
                 int v[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) v[r] = clip16((d0[r] + (d1[r] << 7) + (d2[r] << 14)) >> 10);
+                for (int r = 0; r < 4; r++) v[r] = clip16((dl[r] + (dh[r] << 8)) >> 10);
                 const int o0 = 16 * mi + 4 * lq;                // this lane's output columns o0 .. o0 + 3
                 if (rowOk && o0 < W) {
                     const int off = rowOff + o0;
@@ -487,6 +509,7 @@ int32_t xg_recon_run(xg_session *s, const int16_t *coeff, const xg_pixel *pred, 
     a.n = p->n;
     a.mats = g_rmats;
     memcpy(a.matOff, g_rm.off, sizeof(a.matOff));
+    memcpy(a.constOff, g_rm.coff, sizeof(a.constOff));
     static const int slots = resident_slots((const void *)k_recon_mfma, kRecThreads);
     hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
