@@ -57,6 +57,38 @@ __device__ __forceinline__ void stage_store(const V (&c)[N], void *lds, int tid)
     for (int i = 0; i < N; i++) reinterpret_cast<V *>(lds)[tid + T * i] = c[i];
 }
 
+// How the WAVES waves of a workgroup share a stage of `tiles` 16-wide tiles
+// with up to four 16-wide sub-tiles each (mi): whole tiles per wave when there
+// are at least as many tiles as waves (or the tiles do not divide the waves),
+// else the waves of one tile take every (WAVES / tiles)-th sub-tile.
+struct Split {
+    int t0, tstep, m0, mmask;
+};
+
+template <int WAVES>
+__device__ __forceinline__ Split work_split(int tiles, int wave)
+{
+    Split s;
+    if (tiles >= WAVES || (WAVES % tiles) != 0) {
+        s.t0 = wave;
+        s.tstep = WAVES;
+        s.m0 = 0;
+        s.mmask = 0;
+    } else {
+        s.t0 = wave % tiles;
+        s.tstep = WAVES;                          // one tile per wave
+        s.m0 = wave / tiles;
+        s.mmask = WAVES / tiles - 1;              // sub-tiles m0, m0 + WAVES/tiles, ...
+    }
+    return s;
+}
+
+// sub-tile mi is this wave's share of its tile
+__device__ __forceinline__ bool split_takes(const Split &s, int mi)
+{
+    return mi >= s.m0 && ((mi - s.m0) & s.mmask) == 0;
+}
+
 // workgroups of `threads` threads of `kernel` that fit on the device at once
 inline int resident_slots(const void *kernel, int threads = 64)
 {

@@ -171,31 +171,6 @@ __device__ __forceinline__ void recon_skip(const RecArgs &a, const RecGroup &G, 
     }
 }
 
-// How the waves of a group share a stage of `tiles` 16-wide tiles with up to
-// four 16-wide sub-tiles each (mi): whole tiles per wave when there are at
-// least as many tiles as waves (or the tiles do not divide the waves), else
-// the waves of one tile take every (waves / tiles)-th sub-tile.
-struct Split {
-    int t0, tstep, m0, mmask;
-};
-
-__device__ __forceinline__ Split work_split(int tiles, int wave)
-{
-    Split s;
-    if (tiles >= kRecWaves || (kRecWaves % tiles) != 0) {
-        s.t0 = wave;
-        s.tstep = kRecWaves;
-        s.m0 = 0;
-        s.mmask = 0;
-    } else {
-        s.t0 = wave % tiles;
-        s.tstep = kRecWaves;                      // one tile per wave
-        s.m0 = wave / tiles;
-        s.mmask = kRecWaves / tiles - 1;          // sub-tiles m0, m0 + waves/tiles, ...
-    }
-    return s;
-}
-
 // One workgroup = four waves; they walk groups of same-descriptor blocks.  A
 // group issues its coefficient loads (rows below the zero-out only) and the
 // basis fragments before its first wait; each stage's tiles are shared by the
@@ -268,7 +243,7 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
 
         // ---- stage 1: M = columns (b, x < nzW), N = i < H, K = k < nzH ----
         const int cols1 = G_n << lgNzW;
-        const Split s1 = work_split((cols1 + 15) >> 4, wave);
+        const Split s1 = work_split<kRecWaves>((cols1 + 15) >> 4, wave);
         for (int t = s1.t0; 16 * t < cols1; t += s1.tstep) {
             const int n0 = 16 * t;
             // A fragment: column m = n0 + lr, rows k = 8 lq .. +7 (clamped reads;
@@ -288,7 +263,7 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
 #pragma unroll
             for (int mi = 0; mi < 4; mi++) {
                 if (16 * mi >= H) break;
-                if (mi < s1.m0 || ((mi - s1.m0) & s1.mmask)) continue;
+                if (!split_takes(s1, mi)) continue;
                 // offset term of output row i and the rounding ride in dl
                 const int c1 = cst[16 * mi + lr] + 64;                            // + stage-1 rounding
                 v4i dl = {c1, c1, c1, c1}, dh = {0, 0, 0, 0};
@@ -324,7 +299,7 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
         const int rows2 = G_n << lgH;
         const uint16_t *Pg = a.pred + G.base;
         uint16_t *Og = a.recon + G.base;
-        const Split s2 = work_split((rows2 + 15) >> 4, wave);
+        const Split s2 = work_split<kRecWaves>((rows2 + 15) >> 4, wave);
         for (int t = s2.t0; 16 * t < rows2; t += s2.tstep) {
             const int n0 = 16 * t;
             const int row = min(n0 + lr, rows2 - 1);
@@ -337,7 +312,7 @@ __global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const Rec
 #pragma unroll
             for (int mi = 0; mi < 4; mi++) {
                 if (16 * mi >= W) break;
-                if (mi < s2.m0 || ((mi - s2.m0) & s2.mmask)) continue;
+                if (!split_takes(s2, mi)) continue;
                 // offset terms of output columns o0 .. o0 + 3 and the rounding ride in dl
                 const int4 c2 = cstv[16 + 4 * mi + lq];
                 v4i dl = {c2.x + 512, c2.y + 512, c2.z + 512, c2.w + 512}, dh = {0, 0, 0, 0};
