@@ -254,6 +254,40 @@ int32_t Xg266IDctAdd(const xg_coeff *input, intptr_t inputStride, const xg_pixel
                      xg_pixel *output, intptr_t outputStride, uint32_t width, uint32_t height, uint32_t mtsIdx,
                      int32_t isIntraLuma, int32_t enableMts);
 
+/* XinBiliInterp (h266_func_struct.h:150-157): pfXinBiliInterp[!!fracX][!!fracY][lgW]
+ * (h266_func_init.c:496-530), replacing Xin266BiliInterpCopy / Hor / Vet / HorVet
+ * (h266_core/source/h266_bilinear_filter.c:46-260): the DMVR bilinear
+ * prediction, width + 4 columns of int16 per row; frac = fracY << 4 | fracX.
+ * 10-bit: Copy keeps the samples, each filter pass is ((16-f)*a + f*b + 8) >> 4.
+ * Widths 1..128, heights 1..132. */
+void Xg266BiliInterpCopy(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride, int32_t frac,
+                         uint32_t width, uint32_t height);
+void Xg266BiliInterpHor(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride, int32_t frac,
+                        uint32_t width, uint32_t height);
+void Xg266BiliInterpVet(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride, int32_t frac,
+                        uint32_t width, uint32_t height);
+void Xg266BiliInterpHorVet(const xg_pixel *src, intptr_t srcStride, int16_t *dst, intptr_t dstStride, int32_t frac,
+                           uint32_t width, uint32_t height);
+/* XinComputeDistS16 (h266_func_struct.h:540-548): pfXinComputeSadS16[] (h266_func_init.c:118-125),
+ * replacing the closed Xin26xComputeSadS16 (h26x_core/include/h26x_compute_dist.h:942-950):
+ * SAD of rows 0, 2^subShift, ... scaled back by << subShift. */
+void Xg26xComputeSadS16(int16_t *input, intptr_t inputStride, int16_t *pred, intptr_t predStride, uint32_t width,
+                        uint32_t height, uint32_t subShift, uint32_t *sad);
+/* One bi-predicted block for decoder-side MV refinement: the fields
+ * Xin266DeriveMvFromDmvr (h266_encoder/source/h266_inter_prediction.c:976-1195)
+ * reads from its section, CU and PU. */
+typedef struct xg_dmvr {
+    const xg_pixel *ref0; intptr_t stride0;     /* sample (0, 0) of each padded reference plane (luma) */
+    const xg_pixel *ref1; intptr_t stride1;
+    int32_t  cuPelX, cuPelY;                    /* block position, luma samples */
+    uint32_t width, height;                     /* 8..128, powers of two */
+    int32_t  mv0x, mv0y, mv1x, mv1y;            /* list 0 / list 1 MVs, 1/16 pel */
+    int16_t  minMvX, minMvY, maxMvX, maxMvY;    /* integer-pel MV window (secSet->minMv / maxMv) */
+} xg_dmvr;
+/* Replaces Xin266DeriveMvFromDmvr: one (x, y) delta in 1/16 pel per subblock
+ * of min(16, side) in raster order (mvdL0SubPu), host pointers. */
+int32_t Xg266DeriveMvFromDmvr(const xg_dmvr *dmvr, int16_t *mvdL0SubPu);
+
 /* Mirror of the hot-path slots of xin_func_struct (h266_func_struct.h:719-799),
  * same names, same index conventions (index = log2 of the width). */
 typedef void (*XgComputeDist)(xg_pixel *, intptr_t, xg_pixel *, intptr_t, uint32_t, uint32_t, uint32_t *);
@@ -285,6 +319,9 @@ typedef void (*XgInv2dDct2)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, uint32_t
 typedef void (*XgInv2dSkip)(xg_coeff *, intptr_t, xg_coeff *, intptr_t, uint32_t, uint32_t);
 typedef void (*XgBlockAddForDct)(xg_coeff *, intptr_t, xg_pixel *, intptr_t, xg_pixel *, intptr_t, uint32_t,
                                  uint32_t);
+typedef void (*XgBiliInterp)(const xg_pixel *, intptr_t, int16_t *, intptr_t, int32_t, uint32_t, uint32_t);
+typedef void (*XgComputeDistS16)(int16_t *, intptr_t, int16_t *, intptr_t, uint32_t, uint32_t, uint32_t,
+                                 uint32_t *);
 typedef void (*XgComputeHada8x8)(xg_pixel *, intptr_t, uint32_t *);
 typedef void (*XgQuantInvQuant)(xg_coeff *, xg_coeff *, xg_coeff *, intptr_t, uint32_t, uint32_t, uint32_t, uint32_t,
                                 int32_t, int32_t, int32_t, int32_t, int32_t, int32_t, uint64_t *);
@@ -317,6 +354,8 @@ typedef struct xg_func_struct {
     XgInv2dDct2          pfXinInv2dDct2[XG_BLOCK_NUM][XG_BLOCK_NUM];
     XgInv1dDct           pfXinInv1dDct[XG_TRANS_TYPE][XG_BLOCK_NUM];
     XgBlockAddForDct     pfXinBlockAddForDct[XG_BLOCK_NUM];
+    XgBiliInterp         pfXinBiliInterp[2][2][XG_BLOCK_NUM];
+    XgComputeDistS16     pfXinComputeSadS16[XG_BLOCK_NUM];
     XgComputeHada8x8     pfXinComputeHada8x8;
 } xg_func_struct;
 
@@ -473,7 +512,8 @@ int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best
  * stage 0 the refinement kernel(s) of the eager modes, 1 a whole graph replay,
  * 2 the forward-transform kernel (xg_fdct_run), 3 the integer-search kernel
  * (xg_search_blocks, xg_full_search_frame, the full search of xg_me_pipeline),
- * 4 a whole xg_me_pipeline frame, 5 the reconstruction kernel (xg_recon_run). */
+ * 4 a whole xg_me_pipeline frame, 5 the reconstruction kernel (xg_recon_run),
+ * 6 the DMVR kernel (xg_dmvr_run). */
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
 int32_t xg_session_reset_timers(xg_session *s);
 int32_t xg_session_set_timing(xg_session *s, int32_t enable);
@@ -620,6 +660,22 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff);
  * pointers); recon must not overlap coeff or pred. */
 int32_t xg_recon_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t nblocks);
 int32_t xg_recon_run(xg_session *s, const int16_t *coeff, const xg_pixel *pred, xg_pixel *recon);
+
+/* Batched DMVR (the refinement of Xg266DeriveMvFromDmvr for many blocks).
+ * xg_dmvr_plan takes a HOST array of blocks and writes subOffset[0..n]
+ * (optional): block i's subblock deltas are mvd[2*subOffset[i] ..
+ * 2*subOffset[i+1]).  xg_dmvr_run enqueues the refinement of every planned
+ * block on the session stream: ref0 / ref1 are device luma planes (data at
+ * sample (0, 0), padded by `pad` samples), and every block's reads (its
+ * clipped MVs, -2 pels, w + 5 x h + 5 samples) must lie inside the padding,
+ * else XG_ERR_ARG.  mvd is a device array of int16 (x, y) pairs. */
+typedef struct xg_dmvr_block {
+    int32_t x, y, width, height;                 /* luma position, sides 8..128 */
+    int32_t mv0x, mv0y, mv1x, mv1y;              /* 1/16 pel */
+    int16_t minMvX, minMvY, maxMvX, maxMvY;      /* integer-pel MV window */
+} xg_dmvr_block;
+int32_t xg_dmvr_plan(xg_session *s, const xg_dmvr_block *blocks, int32_t nblocks, int32_t *subOffset);
+int32_t xg_dmvr_run(xg_session *s, const xg_plane *ref0, const xg_plane *ref1, int16_t *mvd);
 
 #ifdef __cplusplus
 }

@@ -979,6 +979,177 @@ int or_IDctAdd(const or_coeff *in, intptr_t is, const or_pixel *pred, intptr_t p
 }
 
 /* ---------------------------------------------------------------------------
+ * DMVR: bilinear prediction (h266_core/source/h266_bilinear_filter.c, source),
+ * the S16 SAD with row subsampling (closed Xin26xComputeSadS16,
+ * h26x_core/include/h26x_compute_dist.h:942-950) and the decoder-side MV
+ * refinement of Xin266DeriveMvFromDmvr (h266_encoder/source/
+ * h266_inter_prediction.c:864-1195, source).  10-bit: BILINEAR_PREC equals the
+ * internal bit depth (h266_inter_pred_context.h:42-44), so Copy keeps the
+ * samples and both filter passes shift by 4 with offset 8.
+ * ------------------------------------------------------------------------- */
+#define OR_BILI_STRIDE 256      /* BILI_BUF_STRIDE = XIN_MAX_CU_SIZE * 2 */
+
+/* Every entry writes width + 4 columns (the DMVR padding) of `height` rows. */
+void or_BiliInterpCopy(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                       uint32_t w, uint32_t h)
+{
+    (void)frac;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w + 4; x++) dst[y * ds + x] = (int16_t)src[y * ss + x];
+}
+
+/* two taps (16 - f, f) on samples x and x + 1, (sum + 8) >> 4 */
+static int16_t bili2(int32_t a, int32_t b, int32_t f) { return (int16_t)(((16 - f) * a + f * b + 8) >> 4); }
+
+void or_BiliInterpHor(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                      uint32_t w, uint32_t h)
+{
+    const int32_t f = frac & 15;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w + 4; x++) dst[y * ds + x] = bili2(src[y * ss + x], src[y * ss + x + 1], f);
+}
+
+void or_BiliInterpVet(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                      uint32_t w, uint32_t h)
+{
+    const int32_t f = frac >> 4;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w + 4; x++) dst[y * ds + x] = bili2(src[y * ss + x], src[(y + 1) * ss + x], f);
+}
+
+/* horizontal pass over height + 1 rows into int16, then the vertical pass on it */
+void or_BiliInterpHorVet(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                         uint32_t w, uint32_t h)
+{
+    const int32_t fx = frac & 15, fy = frac >> 4;
+    int16_t *t = (int16_t *)malloc(sizeof(int16_t) * (w + 4) * (h + 1));
+    for (uint32_t y = 0; y < h + 1; y++)
+        for (uint32_t x = 0; x < w + 4; x++) t[y * (w + 4) + x] = bili2(src[y * ss + x], src[y * ss + x + 1], fx);
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w + 4; x++)
+            dst[y * ds + x] = bili2(t[y * (w + 4) + x], t[(y + 1) * (w + 4) + x], fy);
+    free(t);
+}
+
+/* SAD over the rows y = 0, 2^subShift, 2*2^subShift, ..., scaled back by
+ * << subShift (the subsampled-SAD convention of HM/VTM's DistParam.subShift,
+ * which the reference's DMVR cost follows with its >> 1). */
+uint32_t or_ComputeSadS16(const int16_t *a, intptr_t as, const int16_t *b, intptr_t bs, uint32_t w, uint32_t h,
+                          uint32_t subShift)
+{
+    uint32_t sad = 0;
+    for (uint32_t y = 0; y < h; y += 1u << subShift)
+        for (uint32_t x = 0; x < w; x++) {
+            const int32_t d = (int32_t)a[y * as + x] - b[y * bs + x];
+            sad += (uint32_t)(d < 0 ? -d : d);
+        }
+    return sad << subShift;
+}
+
+/* Xin266DivForMaxQ7 (:864-912): three-bit quotient of N / D in 1/16 units, clamped to 8. */
+static int32_t div_q7(int64_t N, int64_t D)
+{
+    int sign = 0;
+    int32_t q = 0;
+    if (N < 0) { sign = 1; N = -N; }
+    D <<= 3;
+    if (N >= D) { N -= D; q++; }
+    q <<= 1;
+    D >>= 1;
+    if (N >= D) { N -= D; q++; }
+    q <<= 1;
+    if (N >= (D >> 1)) q++;
+    return sign ? -q : q;
+}
+
+/* Xin266SubPelRefineDmvr / Xin266SubPelErrorSrfc (:914-974): the parametric
+ * error surface around the integer minimum c = costs[0] with neighbours
+ * left, top, right, bottom = costs[1..4]. */
+void or_DmvrSubPel(int non0Cost, int16_t *dmv, const int64_t costs[5])
+{
+    if (!non0Cost || abs(dmv[0]) == 32 || abs(dmv[1]) == 32) return;
+    for (int hv = 0; hv < 2; hv++) {
+        const int64_t num = (costs[hv + 1] - costs[hv + 3]) * 16;
+        const int64_t den = costs[hv + 1] + costs[hv + 3] - 2 * costs[0];
+        if (den == 0) continue;
+        int32_t d;
+        if (costs[hv + 1] != costs[0] && costs[hv + 3] != costs[0]) d = div_q7(num, den);
+        else d = costs[hv + 1] == costs[0] ? -8 : 8;
+        dmv[hv] = (int16_t)(dmv[hv] + d);
+    }
+}
+
+/* The L0 or L1 bilinear prediction of a DMVR block: the clipped MV moved by
+ * -2 pels in both directions, (w + 4) x (h + 4) int16 samples at stride 256
+ * (:1033-1090). */
+static void dmvr_bili(const or_pixel *ref, intptr_t rs, int32_t cuX, int32_t cuY, int32_t mvx, int32_t mvy,
+                      const or_dmvr *d, int16_t *buf)
+{
+    mvx = mvx < d->minMvX * 16 ? d->minMvX * 16 : mvx > d->maxMvX * 16 ? d->maxMvX * 16 : mvx;
+    mvy = mvy < d->minMvY * 16 ? d->minMvY * 16 : mvy > d->maxMvY * 16 ? d->maxMvY * 16 : mvy;
+    const int32_t ox = cuX * 16 + mvx - 32, oy = cuY * 16 + mvy - 32;
+    const int32_t fx = ox & 15, fy = oy & 15, frac = (fy << 4) + fx;
+    const or_pixel *src = ref + (intptr_t)(oy >> 4) * rs + (ox >> 4);
+    if (fx && fy) or_BiliInterpHorVet(src, rs, buf, OR_BILI_STRIDE, frac, d->width, d->height + 4);
+    else if (fx) or_BiliInterpHor(src, rs, buf, OR_BILI_STRIDE, frac, d->width, d->height + 4);
+    else if (fy) or_BiliInterpVet(src, rs, buf, OR_BILI_STRIDE, frac, d->width, d->height + 4);
+    else or_BiliInterpCopy(src, rs, buf, OR_BILI_STRIDE, frac, d->width, d->height + 4);
+}
+
+/* Xin266DeriveMvFromDmvr (:976-1195): per subblock of min(16, side), an
+ * integer search over the 25 offsets of the 5x5 window (mirrored between the
+ * lists) with the early exit on a small centre cost, then the parametric
+ * sub-pel step.  mvd receives one (x, y) delta in 1/16 pel per subblock,
+ * raster order. */
+void or_DeriveMvFromDmvr(const or_dmvr *d, int16_t *mvd)
+{
+    static const int8_t kSearch[25][2] = {
+        {-2, -2}, {-1, -2}, {0, -2}, {1, -2}, {2, -2}, {-2, -1}, {-1, -1}, {0, -1}, {1, -1}, {2, -1},
+        {-2, 0},  {-1, 0},  {0, 0},  {1, 0},  {2, 0},  {-2, 1},  {-1, 1},  {0, 1},  {1, 1},  {2, 1},
+        {-2, 2},  {-1, 2},  {0, 2},  {1, 2},  {2, 2}};
+    const uint32_t subW = d->width < 16 ? d->width : 16, subH = d->height < 16 ? d->height : 16;
+    int16_t *p0 = (int16_t *)malloc(sizeof(int16_t) * OR_BILI_STRIDE * OR_BILI_STRIDE);
+    int16_t *p1 = (int16_t *)malloc(sizeof(int16_t) * OR_BILI_STRIDE * OR_BILI_STRIDE);
+    dmvr_bili(d->ref0, d->stride0, d->cuPelX, d->cuPelY, d->mv0x, d->mv0y, d, p0);
+    dmvr_bili(d->ref1, d->stride1, d->cuPelX, d->cuPelY, d->mv1x, d->mv1y, d, p1);
+    int n = 0;
+    for (uint32_t y0 = 0; y0 < d->height; y0 += subH)
+        for (uint32_t x0 = 0; x0 < d->width; x0 += subW) {
+            uint32_t cost[25];
+            for (int i = 0; i < 25; i++) cost[i] = UINT32_MAX;
+            const int16_t *c0 = p0 + (2 + y0) * OR_BILI_STRIDE + x0 + 2;
+            const int16_t *c1 = p1 + (2 + y0) * OR_BILI_STRIDE + x0 + 2;
+            int non0 = 1;
+            int best = 12;                          /* index of the minimum in the 5x5 grid */
+            uint32_t minCost = or_ComputeSadS16(c0, OR_BILI_STRIDE, c1, OR_BILI_STRIDE, subW, subH, 1) >> 1;
+            minCost -= minCost >> 2;
+            if (minCost < subH * subW) {
+                non0 = 0;
+            } else {
+                cost[12] = minCost;
+                for (int i = 0; i < 25; i++) {
+                    const int dx = kSearch[i][0], dy = kSearch[i][1];
+                    if (cost[i] == UINT32_MAX)
+                        cost[i] = or_ComputeSadS16(c0 + dy * OR_BILI_STRIDE + dx, OR_BILI_STRIDE,
+                                                   c1 - dy * OR_BILI_STRIDE - dx, OR_BILI_STRIDE, subW, subH, 1) >> 1;
+                    if (cost[i] < minCost) { minCost = cost[i]; best = i; }
+                }
+            }
+            int16_t dmv[2] = {(int16_t)(kSearch[best][0] * 16), (int16_t)(kSearch[best][1] * 16)};
+            if (non0) {
+                const int64_t cs[5] = {cost[best], cost[best - 1 < 0 ? 0 : best - 1], cost[best - 5 < 0 ? 0 : best - 5],
+                                       cost[best + 1 > 24 ? 24 : best + 1], cost[best + 5 > 24 ? 24 : best + 5]};
+                or_DmvrSubPel(non0, dmv, cs);
+            }
+            mvd[2 * n] = dmv[0];
+            mvd[2 * n + 1] = dmv[1];
+            n++;
+        }
+    free(p0);
+    free(p1);
+}
+
+/* ---------------------------------------------------------------------------
  * Frame-level drivers (checker for the batched kernels).
  * ------------------------------------------------------------------------- */
 

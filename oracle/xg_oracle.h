@@ -108,6 +108,29 @@ uint32_t or_ComputeAvgSatd(const or_pixel *in, intptr_t is, const or_pixel *a, c
 void or_ComputeAvgSadx8(const or_pixel *in, intptr_t is, const or_pixel *const a[8], const or_pixel *const b[8],
                         intptr_t rs, uint32_t w, uint32_t h, uint32_t sad[8]);
 
+/* ---- DMVR (h266_bilinear_filter.c; h266_inter_prediction.c:864-1195) ---- */
+void or_BiliInterpCopy(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                       uint32_t w, uint32_t h);
+void or_BiliInterpHor(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                      uint32_t w, uint32_t h);
+void or_BiliInterpVet(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                      uint32_t w, uint32_t h);
+void or_BiliInterpHorVet(const or_pixel *src, intptr_t ss, int16_t *dst, intptr_t ds, int32_t frac,
+                         uint32_t w, uint32_t h);
+uint32_t or_ComputeSadS16(const int16_t *a, intptr_t as, const int16_t *b, intptr_t bs, uint32_t w, uint32_t h,
+                          uint32_t subShift);
+typedef struct or_dmvr {
+    const or_pixel *ref0; intptr_t stride0;     /* sample (0, 0) of each padded reference plane */
+    const or_pixel *ref1; intptr_t stride1;
+    int32_t  cuPelX, cuPelY;                    /* block position, luma samples */
+    uint32_t width, height;
+    int32_t  mv0x, mv0y, mv1x, mv1y;            /* 1/16 pel */
+    int16_t  minMvX, minMvY, maxMvX, maxMvY;    /* integer-pel MV window */
+} or_dmvr;
+void or_DeriveMvFromDmvr(const or_dmvr *d, int16_t *mvd);
+/* the parametric sub-pel step: costs = centre, left, top, right, bottom; dmv in 1/16 pel */
+void or_DmvrSubPel(int non0Cost, int16_t *dmv, const int64_t costs[5]);
+
 /* ---- rate terms ---- */
 uint32_t or_ExpGolombBits(int32_t v);
 uint32_t or_MvdBits(int32_t dx, int32_t dy);

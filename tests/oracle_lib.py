@@ -40,6 +40,14 @@ class OrMe(ctypes.Structure):
                 ("ref", P), ("refStride", SS)]
 
 
+class OrDmvr(ctypes.Structure):
+    _fields_ = [("ref0", P), ("stride0", SS), ("ref1", P), ("stride1", SS),
+                ("cuPelX", I32), ("cuPelY", I32), ("width", U32), ("height", U32),
+                ("mv0x", I32), ("mv0y", I32), ("mv1x", I32), ("mv1y", I32),
+                ("minMvX", ctypes.c_int16), ("minMvY", ctypes.c_int16),
+                ("maxMvX", ctypes.c_int16), ("maxMvY", ctypes.c_int16)]
+
+
 def load_oracle():
     if not os.path.exists(ORACLE_SO):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
@@ -74,6 +82,13 @@ def load_oracle():
         "or_IDctWxH": (ctypes.c_int, [P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "or_BlockRecon": (None, [P, SS, P, SS, P, SS, U32, U32, ctypes.c_int]),
         "or_ReconBatch": (None, [P, P, P, P, I32, I32, P]),
+        "or_BiliInterpCopy": (None, [P, SS, P, SS, I32, U32, U32]),
+        "or_BiliInterpHor": (None, [P, SS, P, SS, I32, U32, U32]),
+        "or_BiliInterpVet": (None, [P, SS, P, SS, I32, U32, U32]),
+        "or_BiliInterpHorVet": (None, [P, SS, P, SS, I32, U32, U32]),
+        "or_ComputeSadS16": (U32, [P, SS, P, SS, U32, U32, U32]),
+        "or_DeriveMvFromDmvr": (None, [ctypes.POINTER(OrDmvr), P]),
+        "or_DmvrSubPel": (None, [ctypes.c_int, P, P]),
         "or_IDctAdd": (ctypes.c_int, [P, SS, P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]),
         "or_QuantInvQuant": (None, [P, P, P, SS, U32, U32, U32, U32, I32, I32, I32, I32, I32, I32, P]),

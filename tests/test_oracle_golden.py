@@ -11,6 +11,10 @@ tests/golden/quant_golden.npz holds the outputs of the reference's own build of
 h266_encoder/source/h266_quant_inv_quant.c (generator
 tests/golden/make_quant_golden.py): every coefficient-group shape the encoder
 passes, QPs 0..51 with the RDOQ / intra / inter rounding offsets, int16 extremes.
+
+tests/golden/bili_golden.npz holds the outputs of the reference's own build of
+h266_core/source/h266_bilinear_filter.c (generator tests/golden/make_bili_golden.py):
+every DMVR bilinear phase pair, widths 1..128, flat and checkerboard extremes.
 """
 import numpy as np
 
@@ -107,3 +111,15 @@ def test_oracle_quant_matches_reference_golden(oracle):
     # the fixture exercises both clips and empty groups
     assert (g["q"] == 32767).any() or (g["q"] == -32768).any() or (np.abs(g["r"]) == 32767).any()
     assert (g["maps"] == 0).any() and (g["maps"] != 0).any()
+
+
+def test_oracle_bilinear_matches_reference_golden(oracle):
+    import os
+    import bili_cases as bc
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "bili_golden.npz"))
+    cases = bc.build_cases()
+    assert np.array_equal(cases, g["cases"]) and np.array_equal(bc.make_plane(), g["plane"])
+    out, offs = bc.replay(bc.resolve(oracle, "or_"), cases, np.ascontiguousarray(g["plane"]))
+    assert np.array_equal(offs, g["offsets"])
+    bad = np.nonzero(out != g["out"])[0]
+    assert bad.size == 0, f"first mismatch in case {int(np.searchsorted(offs, bad[0], side='right') - 1)}"

@@ -67,6 +67,21 @@ class XgMv(ctypes.Structure):
     _fields_ = [("mvX", I16), ("mvY", I16)]
 
 
+class XgDmvr(ctypes.Structure):
+    _fields_ = [("ref0", P), ("stride0", SS), ("ref1", P), ("stride1", SS),
+                ("cuPelX", I32), ("cuPelY", I32), ("width", U32), ("height", U32),
+                ("mv0x", I32), ("mv0y", I32), ("mv1x", I32), ("mv1y", I32),
+                ("minMvX", ctypes.c_int16), ("minMvY", ctypes.c_int16),
+                ("maxMvX", ctypes.c_int16), ("maxMvY", ctypes.c_int16)]
+
+
+class XgDmvrBlock(ctypes.Structure):
+    _fields_ = [("x", I32), ("y", I32), ("width", I32), ("height", I32),
+                ("mv0x", I32), ("mv0y", I32), ("mv1x", I32), ("mv1y", I32),
+                ("minMvX", ctypes.c_int16), ("minMvY", ctypes.c_int16),
+                ("maxMvX", ctypes.c_int16), ("maxMvY", ctypes.c_int16)]
+
+
 class XgMeStruct(ctypes.Structure):
     _fields_ = [("width", U32), ("height", U32), ("minMv", XgMv), ("maxMv", XgMv),
                 ("meCand", XgMv * 12), ("candNum", U32), ("bestMv", XgMv), ("predMv", XgMv),
@@ -85,13 +100,15 @@ class XgFuncStruct(ctypes.Structure):
                 ("pfXinFor2dDct2", P * 64), ("pfXinFor1dDct", P * 24),
                 ("pfXinBlockSubForDct", P * 8), ("pfXinQuantInvQuant", P * 8),
                 ("pfXinInv2dSkip", P * 8), ("pfXinInv2dDct2", P * 64), ("pfXinInv1dDct", P * 24),
-                ("pfXinBlockAddForDct", P * 8), ("pfXinComputeHada8x8", P)]
+                ("pfXinBlockAddForDct", P * 8), ("pfXinBiliInterp", P * 32), ("pfXinComputeSadS16", P * 8),
+                ("pfXinComputeHada8x8", P)]
 
 
 # Every symbol include/xg266.h declares, with its prototype (restype, argtypes).
 INTERP = (None, [P, SS, P, SS, I32, U32, U32, U32])
 FDCT1D = (None, [P, SS, P, SS, I32, I32])
 IDCT1D = (None, [P, SS, P, SS, I32, I32, I32])
+BILI = (None, [P, SS, P, SS, I32, U32, U32])
 PROTOS = {
     "xg_last_error": (I32, []),
     "xg_clear_error": (None, []),
@@ -173,6 +190,12 @@ PROTOS = {
     "xg_fdct_run": (I32, [P, P, P]),
     "xg_recon_plan": (I32, [P, P, P, I32]),
     "xg_recon_run": (I32, [P, P, P, P]),
+    "xg_dmvr_plan": (I32, [P, P, I32, P]),
+    "xg_dmvr_run": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), P]),
+    "Xg266BiliInterpCopy": BILI, "Xg266BiliInterpHor": BILI, "Xg266BiliInterpVet": BILI,
+    "Xg266BiliInterpHorVet": BILI,
+    "Xg26xComputeSadS16": (None, [P, SS, P, SS, U32, U32, U32, P]),
+    "Xg266DeriveMvFromDmvr": (I32, [ctypes.POINTER(XgDmvr), P]),
 }
 
 _lib = None

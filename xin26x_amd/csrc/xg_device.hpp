@@ -176,6 +176,19 @@ __device__ __forceinline__ uint32_t align16(uint32_t hi, uint32_t lo, uint32_t s
     return __builtin_amdgcn_alignbit(hi, lo, sh);
 }
 
+// Block-reduce helper for one workgroup of 256 threads.
+__device__ __forceinline__ uint32_t block_sum256(uint32_t v)
+{
+    __shared__ uint32_t red[4];
+    v = (uint32_t)group_sum<64>((int)v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    uint32_t t = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return t;
+}
+
 // Exp-Golomb se(v) length: k = v > 0 ? 2v-1 : -2v; 2*floor(log2(k+1)) + 1.
 __device__ __forceinline__ uint32_t eg_bits(int v)
 {

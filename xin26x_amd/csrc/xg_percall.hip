@@ -101,19 +101,6 @@ __global__ void k_bipred(const int16_t *__restrict__ a, const int16_t *__restric
     dst[i] = (uint16_t)clip_pix(v);
 }
 
-// Block-reduce helper for one workgroup of 256 threads.
-__device__ __forceinline__ uint32_t block_sum256(uint32_t v)
-{
-    __shared__ uint32_t red[4];
-    v = (uint32_t)group_sum<64>((int)v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = v;
-    __syncthreads();
-    uint32_t t = red[0] + red[1] + red[2] + red[3];
-    __syncthreads();
-    return t;
-}
-
 // SAD of packed reference blocks (one every refPitch samples) against one
 // packed input block; one workgroup per reference block.
 __global__ void __launch_bounds__(256) k_sad(const uint16_t *__restrict__ in,
@@ -1050,6 +1037,11 @@ int32_t Xg266FuncInit(xg_func_struct *f, int32_t satdMd)
         f->pfXinInv1dDct[2][i] = idst7[i];
         f->pfXinInv2dSkip[i] = Xg266ISkipWxH;                              // :410-419
         f->pfXinBlockAddForDct[i] = (i >= 1 && i <= 6) ? Xg26xBlockRecon : nullptr;   // :161-166
+        f->pfXinBiliInterp[0][0][i] = Xg266BiliInterpCopy;                  // :496-530
+        f->pfXinBiliInterp[1][0][i] = Xg266BiliInterpHor;
+        f->pfXinBiliInterp[0][1][i] = Xg266BiliInterpVet;
+        f->pfXinBiliInterp[1][1][i] = Xg266BiliInterpHorVet;
+        f->pfXinComputeSadS16[i] = Xg26xComputeSadS16;                       // :118-125
         for (int j = 0; j < XG_BLOCK_NUM - 1; j++)
             if (i < XG_BLOCK_NUM - 1) {
                 f->pfXinFor2dDct2[i][j] = Xg26xFDct2WxH;

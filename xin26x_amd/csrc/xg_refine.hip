@@ -1188,6 +1188,7 @@ int32_t xg_session_destroy(xg_session *s)
     if (s->stream) hipStreamSynchronize(s->stream);
     fdct_release(s);
     recon_release(s);
+    dmvr_release(s);
     pipe_release(s);
     if (s->graphExec) hipGraphExecDestroy(s->graphExec);
     if (s->graph) hipGraphDestroy(s->graph);

@@ -44,6 +44,8 @@ void unpack2d(void *dst, intptr_t dstStride, const void *src, int w, int h, size
 void fdct_release(xg_session *s);
 // Releases the reconstruction plan a session owns (xg_recon.hip).
 void recon_release(xg_session *s);
+// Releases the DMVR plan a session owns (xg_dmvr.hip).
+void dmvr_release(xg_session *s);
 
 void pipe_release(xg_session *s);
 // Refinement of the session's current blocks, validated and uploaded but not
@@ -55,7 +57,7 @@ void *&session_pipe(xg_session *s);
 // Session kernel timers (HIP events on the session stream), by stage:
 // 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search,
 // 4 a whole motion-search pipeline frame.
-constexpr int kTimerStages = 6;
+constexpr int kTimerStages = 7;
 bool session_timer_begin(xg_session *s, int stage, int ev[2]);   // false when timing is off
 void session_timer_end(xg_session *s, int stage, const int ev[2]);
 
