@@ -310,6 +310,110 @@ def recon_cpu(desc, offs, coeff, pred, gpu_recon, seconds):
             "bit_exact_vs_gpu_on_sample": bool(exact)}
 
 
+PMC_DMVR = os.path.join(ROOT, "profiles", "k_dmvr_1080p.md")
+
+
+def dmvr_blocks(fp, seed=12):
+    """The bi-predicted DMVR candidates of the 1080p workload: its QTMT blocks
+    with both sides >= 8 and area >= 128; list-0 MV = the block's base MV plus
+    a seeded 1/16-pel offset, list-1 MV mirrored with its own offset; the MV
+    window of a CU in a 64-sample CTU, kept inside 80 samples of padding."""
+    from xin26x_amd import capi, workloads as wl
+    b = fp.blocks
+    sel = (b[:, 2] >= 8) & (b[:, 3] >= 8) & (b[:, 2] * b[:, 3] >= 128)
+    b = b[sel]
+    off = wl.randint(seed, 4 * len(b), -12, 12, stream=7).reshape(len(b), 4)
+    arr = (capi.XgDmvrBlock * len(b))()
+    for i, (x, y, w, h, mx, my) in enumerate(b[:, :6].tolist()):
+        arr[i] = capi.XgDmvrBlock(x, y, w, h, mx + int(off[i, 0]), my + int(off[i, 1]),
+                                  -mx + int(off[i, 2]), -my + int(off[i, 3]),
+                                  -x - 72, -y - 72, fp.width - x - w + 72, fp.height - y - h + 72)
+    return arr, b
+
+
+def dmvr_leg(lib, fp, cur, ref, steps, warmup, cpu_seconds):
+    """Decoder-side MV refinement of the 1080p workload's bi-predicted blocks
+    (list 0 = the reference picture, list 1 = the current picture as a second
+    reference): one workgroup per 16x16 subblock."""
+    import torch
+    from xin26x_amd import capi
+    arr, b = dmvr_blocks(fp)
+    n = len(b)
+    sub = np.zeros(n + 1, np.int32)
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        capi.check(lib.xg_dmvr_plan(s, arr, n, sub.ctypes.data), "dmvr plan")
+        nsub = int(sub[-1])
+        mvd = torch.empty(2 * nsub, dtype=torch.int16, device="cuda")
+        p0, p1 = ref.pic.plane[0], cur.pic.plane[0]
+        run = lambda: capi.check(lib.xg_dmvr_run(s, ctypes.byref(p0), ctypes.byref(p1), mvd.data_ptr()), "dmvr run")
+        k_ms, wall_ms = session_time(lib, s, 6, run, steps, warmup)
+        got = mvd.cpu().numpy()
+    finally:
+        lib.xg_session_destroy(s)
+    sw, sh = np.minimum(b[:, 2], 16), np.minimum(b[:, 3], 16)
+    per = (b[:, 2] // sw) * (b[:, 3] // sh)
+    alg_bytes = int((per * 2 * (sw + 5) * (sh + 5) * 2).sum()) + nsub * 4    # both lists' regions, one delta out
+    k_s = k_ms / 1e3
+    traffic, src = profiled_traffic(PMC_DMVR)
+    out = {"workload": f"1080p, {n} bi-predicted QTMT blocks (sides >= 8, area >= 128), {nsub} subblocks of up "
+                       "to 16x16; list 0 = base MV + seeded 1/16-pel offset into the reference picture, list 1 = "
+                       "mirrored MV into the current picture; bilinear prediction, 25 mirrored SADs, parametric "
+                       "sub-pel step",
+           "value": round(nsub / (wall_ms / 1e3), 1), "unit": "subblocks/s",
+           "blocks_per_s": round(n / (wall_ms / 1e3), 1), "ms_per_frame": round(wall_ms, 4),
+           "kernel": "k_dmvr", "kernel_ms": round(k_ms, 4),
+           "roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
+                        "traffic": traffic, "traffic_source": src, "alg_bytes_per_launch": alg_bytes}}
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = dmvr_cpu(fp, arr, b, sub, got, cpu_seconds)
+    return out
+
+
+def dmvr_cpu(fp, arr, b, sub, gpu_mvd, seconds):
+    """The oracle's refinement on the host, chunks of the same blocks; the
+    blocks it computes are also compared with the GPU's deltas."""
+    from tests.oracle_lib import OrDmvr, load_oracle
+    oracle = load_oracle()
+    pad = 80
+    r0 = np.ascontiguousarray(np.pad(fp.ref[0], pad, mode="edge"))
+    r1 = np.ascontiguousarray(np.pad(fp.cur[0], pad, mode="edge"))
+    st = r0.shape[1]
+    base0, base1 = r0.ctypes.data + 2 * (pad * st + pad), r1.ctypes.data + 2 * (pad * st + pad)
+    out = np.zeros(2 * int(sub[-1]), np.int16)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    chunk, n = 500, len(b)
+
+    def work(i0):
+        for i in range(i0, min(n, i0 + chunk)):
+            a = arr[i]
+            d = OrDmvr(ref0=base0, stride0=st, ref1=base1, stride1=st, cuPelX=a.x, cuPelY=a.y, width=a.width,
+                       height=a.height, mv0x=a.mv0x, mv0y=a.mv0y, mv1x=a.mv1x, mv1y=a.mv1y, minMvX=a.minMvX,
+                       minMvY=a.minMvY, maxMvX=a.maxMvX, maxMvY=a.maxMvY)
+            oracle.or_DeriveMvFromDmvr(ctypes.byref(d), ctypes.c_void_p(out.ctypes.data + 4 * int(sub[i])))
+        return i0, min(n, i0 + chunk)
+
+    done, blocks = [], 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        pos = 0
+        while time.perf_counter() - t0 < seconds and pos < n:
+            starts = [pos + k * chunk for k in range(threads * 2) if pos + k * chunk < n]
+            pos += threads * 2 * chunk
+            for i0, i1 in ex.map(work, starts):
+                done.append((i0, i1))
+                blocks += i1 - i0
+    dt = time.perf_counter() - t0
+    subs = sum(int(sub[i1] - sub[i0]) for i0, i1 in done)
+    exact = all(np.array_equal(out[2 * sub[i0]:2 * sub[i1]], gpu_mvd[2 * sub[i0]:2 * sub[i1]]) for i0, i1 in done)
+    return {"value": round(subs / dt, 1), "unit": "subblocks/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C) on the first {blocks} blocks ({subs} subblocks) in chunks of {chunk}, "
+                      f"{threads} host threads, {dt:.1f} s",
+            "bit_exact_vs_gpu_on_sample": bool(exact)}
+
+
 def search_leg(lib, steps, warmup, cpu_seconds):
     """The plain 416x240 integer full search: 390 16x16 blocks, +/-16, lambda 4."""
     import torch
@@ -633,6 +737,11 @@ def main():
                                "fractional refinement, MC; reference = previous frame",
                    "realtime_30fps": rt, "saturated": sat}
 
+    dmvr = None
+    if not args.quick and not args.no_extra:
+        log("DMVR: the 1080p bi-predicted blocks")
+        dmvr = dmvr_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
+
     traffic, traffic_src = profiled_traffic(PMC_REFINE)
 
     cpu = None
@@ -675,6 +784,7 @@ def main():
         "cpu_baseline": cpu,
         "transforms": transforms,
         "reconstruction": reconstruction,
+        "dmvr_1080p": dmvr,
         "plain_search_416x240": search,
         "pipeline_4k": pipeline,
         "serving": serving,

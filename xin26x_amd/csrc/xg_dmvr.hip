@@ -13,8 +13,8 @@
 //   * the best offset, if inside the 3x3 interior, gets the parametric
 //     sub-pel step (Xin266SubPelErrorSrfc, Xin266DivForMaxQ7).
 // One 64-thread workgroup per subblock: both lists' source regions (at most
-// 21 x 21 samples) and bilinear predictions (20 x 20) live in LDS, lane i < 25
-// costs offset i, lane 0 runs the scan.
+// 21 x 21 samples) and bilinear predictions (20 x 20) live in LDS, the 25
+// costs are summed row by row across the lanes, lane 0 runs the scan.
 //
 // The per-call table slots (bilinear filters, S16 SAD) and the per-call
 // refinement stage host data and run the same device code.
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
                                               const int2 *__restrict__ subs, int nSub, int16_t *__restrict__ mvd)
 {
     __shared__ uint16_t src[2][21 * 21];
-    __shared__ int16_t bil[2][20 * 20];
+    __shared__ __align__(16) int16_t bil[2][20 * 20];
     __shared__ uint32_t cost[25];
     const int lane = threadIdx.x;
     for (int si = blockIdx.x; si < nSub; si += gridDim.x) {
@@ -124,6 +124,7 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
                 src[l][r * 21 + c] = need ? s[(intptr_t)r * p.stride + c] : 0;
             }
         }
+        if (lane < 25) cost[lane] = 0;
         __syncthreads();
 #pragma unroll
         for (int l = 0; l < 2; l++) {
@@ -140,20 +141,31 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
             }
         }
         __syncthreads();
-        if (lane < 25) {
-            const int dx = c_dmvrSearch[lane][0], dy = c_dmvrSearch[lane][1];
-            const int16_t *a = &bil[0][(2 + dy) * 20 + 2 + dx];
-            const int16_t *b = &bil[1][(2 - dy) * 20 + 2 - dx];
-            uint32_t s = 0;
-            for (int r = 0; r < sh; r += 2)
-                for (int c = 0; c < sw; c++) s += (uint32_t)abs((int)a[r * 20 + c] - (int)b[r * 20 + c]);
-            if (lane == 12) s -= s >> 2;                 // the centre cost, less a quarter
-            cost[lane] = s;
+        // 25 offsets x sh/2 rows: each item is one row's SAD on packed sample
+        // pairs (v_sad_u16; both offsets' rows start at the same parity, so one
+        // realignment shift serves both), summed into its offset's cost
+        {
+            const uint32_t *B0 = reinterpret_cast<const uint32_t *>(bil[0]);
+            const uint32_t *B1 = reinterpret_cast<const uint32_t *>(bil[1]);
+            const int nr = sh >> 1;
+            for (int it = lane; it < 25 * nr; it += 64) {
+                const int i = it / nr, r = 2 * (it - i * nr);
+                const int dx = c_dmvrSearch[i][0], dy = c_dmvrSearch[i][1];
+                const int ia = (2 + dy + r) * 20 + 2 + dx, ib = (2 - dy + r) * 20 + 2 - dx;
+                const uint32_t sh16 = (ia & 1) * 16;
+                const uint32_t *pa = B0 + (ia >> 1), *pb = B1 + (ib >> 1);
+                uint32_t acc = 0;
+                for (int k = 0; k < (sw >> 1); k++)
+                    acc = __builtin_amdgcn_sad_u16(align16(pa[k + 1], pa[k], sh16), align16(pb[k + 1], pb[k], sh16),
+                                                   acc);
+                atomicAdd(&cost[i], acc);
+            }
         }
         __syncthreads();
         if (lane == 0) {
             int dx = 0, dy = 0;
-            uint32_t minCost = cost[12];
+            uint32_t minCost = cost[12] - (cost[12] >> 2);       // the centre cost, less a quarter
+            cost[12] = minCost;
             if (minCost >= (uint32_t)(sw * sh)) {
                 int best = 12;
                 for (int i = 0; i < 25; i++)
