@@ -310,6 +310,82 @@ def recon_cpu(desc, offs, coeff, pred, gpu_recon, seconds):
             "bit_exact_vs_gpu_on_sample": bool(exact)}
 
 
+def lookahead_leg(lib, fp, cur, ref, steps, warmup, cpu_seconds, unit=16, rng=32, lam=1024):
+    """Lookahead motion estimation of the 1080p pair: both luma planes downscaled
+    2x, every 16x16 unit of the 960x540 picture searched +/-32 (+/-64 at full
+    resolution)."""
+    import torch
+    from xin26x_amd import capi
+    hw, hh = fp.width // 2, fp.height // 2
+    nb = (hw // unit) * (hh // unit)
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        mv = torch.empty(2 * nb, dtype=torch.int16, device="cuda")
+        cost = torch.empty(nb, dtype=torch.int32, device="cuda")
+        sad = torch.empty(nb, dtype=torch.int32, device="cuda")
+        p0, p1 = cur.pic.plane[0], ref.pic.plane[0]
+        run = lambda: capi.check(lib.xg_lookahead_me(s, ctypes.byref(p0), ctypes.byref(p1), unit, rng, lam,
+                                                     mv.data_ptr(), cost.data_ptr(), sad.data_ptr()), "lookahead")
+        k_ms, wall_ms = session_time(lib, s, 3, run, steps, warmup)
+        got = (mv.cpu().numpy().reshape(nb, 2), cost.cpu().numpy().view(np.uint32), sad.cpu().numpy().view(np.uint32))
+    finally:
+        lib.xg_session_destroy(s)
+    evals = nb * (2 * rng + 1) ** 2 * unit * unit
+    out = {"workload": f"1080p pair downscaled 2x (960x540), {nb} units of {unit}x{unit}, integer full search "
+                       f"+/-{rng} (+/-{2 * rng} at full resolution), SAD + lambda * MV bits",
+           "value": round(fp.width * fp.height / (wall_ms / 1e3) / 1e6, 1), "unit": "Mpixel/s (full-resolution)",
+           "ms_per_frame": round(wall_ms, 4), "kernel": "k_down2x2 x 2 + k_full_search_frame_p<16>",
+           "search_kernel_ms": round(k_ms, 4), "abs_diffs_per_s": round(evals / (k_ms / 1e3), 1)}
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = lookahead_cpu(fp, unit, rng, lam, got, cpu_seconds)
+    return out
+
+
+def lookahead_cpu(fp, unit, rng, lam, got, seconds):
+    """The oracle's downscale and full search on the host over rows of units;
+    the rows it computes are compared with the GPU's MVs, costs and SADs."""
+    from tests.oracle_lib import load_oracle
+    oracle = load_oracle()
+    P = lambda a, o=0: ctypes.c_void_p(a.ctypes.data + 2 * o)
+    W, H, pad = fp.width, fp.height, 80
+    hw, hh, hp = W // 2, H // 2, pad // 2
+    t0 = time.perf_counter()
+    cur = np.ascontiguousarray(fp.cur[0])
+    refp = np.ascontiguousarray(np.pad(fp.ref[0], pad, mode="edge"))
+    hcur = np.zeros((hh, hw), np.uint16)
+    href = np.zeros((hh + 2 * hp, hw + 2 * hp), np.uint16)
+    oracle.or_Downscale2x2(P(cur), W, P(hcur), hw, hw, hh)
+    oracle.or_Downscale2x2(P(refp), W + 2 * pad, P(href), hw + 2 * hp, hw + 2 * hp, hh + 2 * hp)
+    nbx, nb = hw // unit, (hw // unit) * (hh // unit)
+    mx, my = np.zeros(nb, np.int16), np.zeros(nb, np.int16)
+    c, sd = np.zeros(nb, np.uint32), np.zeros(nb, np.uint32)
+    threads = max(1, min(16, os.cpu_count() or 1))
+
+    def work(r):
+        b0, b1 = r * nbx, (r + 1) * nbx
+        # or_FullSearchFrame writes blocks [b0, b1) at their own indices of the output arrays
+        oracle.or_FullSearchFrame(P(hcur), hw, P(href, hp * (hw + 2 * hp) + hp), hw + 2 * hp, hw, hh, unit, unit, rng,
+                                  lam, rng, b0, b1, P(mx), P(my), P(c), P(sd))
+        return r
+
+    rows = []
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        r = 0
+        while time.perf_counter() - t0 < seconds and r < hh // unit:
+            batch = list(range(r, min(hh // unit, r + threads)))
+            rows += list(ex.map(work, batch))
+            r += len(batch)
+    dt = time.perf_counter() - t0
+    sel = np.concatenate([np.arange(q * nbx, (q + 1) * nbx) for q in rows]) if rows else np.zeros(0, np.int64)
+    exact = bool(np.array_equal(got[0][sel, 0], mx[sel]) and np.array_equal(got[0][sel, 1], my[sel])
+                 and np.array_equal(got[1][sel], c[sel]) and np.array_equal(got[2][sel], sd[sel]))
+    px = len(rows) * unit * 2 * W
+    return {"value": round(px / dt / 1e6, 2), "unit": "Mpixel/s (full-resolution)", "cores": threads, "kind": "port",
+            "sample": f"oracle downscale + full search on {len(rows)} of {hh // unit} unit rows, {threads} host "
+                      f"threads, {dt:.1f} s", "bit_exact_vs_gpu_on_sample": exact}
+
+
 PMC_DMVR = os.path.join(ROOT, "profiles", "k_dmvr_1080p.md")
 
 
@@ -737,10 +813,12 @@ def main():
                                "fractional refinement, MC; reference = previous frame",
                    "realtime_30fps": rt, "saturated": sat}
 
-    dmvr = None
+    dmvr = lookahead = None
     if not args.quick and not args.no_extra:
         log("DMVR: the 1080p bi-predicted blocks")
         dmvr = dmvr_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
+        log("lookahead motion estimation at half resolution")
+        lookahead = lookahead_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
 
     traffic, traffic_src = profiled_traffic(PMC_REFINE)
 
@@ -785,6 +863,7 @@ def main():
         "transforms": transforms,
         "reconstruction": reconstruction,
         "dmvr_1080p": dmvr,
+        "lookahead_1080p": lookahead,
         "plain_search_416x240": search,
         "pipeline_4k": pipeline,
         "serving": serving,

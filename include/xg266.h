@@ -593,6 +593,20 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
                              int32_t ctuSize, int16_t *bestMv, uint32_t *bestCost,
                              uint32_t *bestSad);
 
+/* Lookahead motion estimation on half-resolution pictures: the role of the
+ * closed Xin26xLookaheadFrame (h26x_core/include/h26x_look_ahead.h:27), whose
+ * per-unit MVs laMv start the inter CU search as laMv << 1
+ * (h266_encoder/source/h266_analyze_inter_cu.c:247-261).  Both luma planes
+ * (device; ref padded by an even pad >= 2 * (range + 4)) are downscaled 2x
+ * into session buffers, each sample the rounded mean (a + b + c + d + 2) >> 2
+ * of its 2x2 block (the reference over its padding too); then every
+ * unitSize x unitSize unit (8 or 16) wholly inside the half-resolution
+ * picture gets the integer full search +/-range of xg_full_search_frame,
+ * predictor (0, 0), SAD + lambda * MV bits.  Outputs per unit, raster order:
+ * laMv in half-resolution quarter pel (int16 x, y), cost, SAD (device). */
+int32_t xg_lookahead_me(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t unitSize,
+                        int32_t range, uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad);
+
 /* Full motion-search pipeline of one frame with two reference pictures, on a
  * raster grid of 16x16 blocks (frame sides multiples of 16): per block and
  * reference, integer full search +/-range around (0,0) (as Xg26xMotionEstFull,

@@ -1149,6 +1149,18 @@ void or_DeriveMvFromDmvr(const or_dmvr *d, int16_t *mvd)
     free(p1);
 }
 
+/* Lookahead downscale (the role of the closed Xin26xDownscale2x2,
+ * h266_encoder/include/h266_func_struct.h:425-433): each output sample is the
+ * rounded mean of its 2x2 source block.  A design choice, not a restatement. */
+void or_Downscale2x2(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds, uint32_t w, uint32_t h)
+{
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            const or_pixel *p = src + (intptr_t)(2 * y) * ss + 2 * x;
+            dst[y * ds + x] = (or_pixel)((p[0] + p[1] + p[ss] + p[ss + 1] + 2) >> 2);
+        }
+}
+
 /* ---------------------------------------------------------------------------
  * Frame-level drivers (checker for the batched kernels).
  * ------------------------------------------------------------------------- */

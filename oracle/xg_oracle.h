@@ -131,6 +131,9 @@ void or_DeriveMvFromDmvr(const or_dmvr *d, int16_t *mvd);
 /* the parametric sub-pel step: costs = centre, left, top, right, bottom; dmv in 1/16 pel */
 void or_DmvrSubPel(int non0Cost, int16_t *dmv, const int64_t costs[5]);
 
+/* ---- lookahead: 2x2 rounded-mean downscale (a design choice; the reference's is closed) ---- */
+void or_Downscale2x2(const or_pixel *src, intptr_t ss, or_pixel *dst, intptr_t ds, uint32_t w, uint32_t h);
+
 /* ---- rate terms ---- */
 uint32_t or_ExpGolombBits(int32_t v);
 uint32_t or_MvdBits(int32_t dx, int32_t dy);

@@ -89,6 +89,7 @@ def load_oracle():
         "or_ComputeSadS16": (U32, [P, SS, P, SS, U32, U32, U32]),
         "or_DeriveMvFromDmvr": (None, [ctypes.POINTER(OrDmvr), P]),
         "or_DmvrSubPel": (None, [ctypes.c_int, P, P]),
+        "or_Downscale2x2": (None, [P, SS, P, SS, U32, U32]),
         "or_IDctAdd": (ctypes.c_int, [P, SS, P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]),
         "or_QuantInvQuant": (None, [P, P, P, SS, U32, U32, U32, U32, I32, I32, I32, I32, I32, I32, P]),

@@ -245,3 +245,47 @@ def test_search_blocks_every_width(oracle):
         m.refStride = rs
         oracle.or_MotionEstFull(ctypes.byref(m))
         assert (got[i, 0], got[i, 1]) == (4 * m.bestMvX, 4 * m.bestMvY), (i, blocks[i])
+
+
+@pytest.mark.parametrize("unit,rng_", [(16, 16), (8, 24)])
+def test_lookahead_me(oracle, unit, rng_):
+    """Half-resolution lookahead search: the 2x2-mean downscale of both
+    pictures (the reference over its padding) then the frame full search of
+    every whole unit, against the oracle's downscale and full search."""
+    import torch
+    lib = capi.load()
+    cur, ref = wl.frame_pair_416x240()
+    W, H, pad = 416, 240, 2 * (rng_ + 4) + 8
+    dc = DevicePicture([cur, cur[::2, ::2].copy(), cur[::2, ::2].copy()], pad=0)
+    dr = DevicePicture([ref, ref[::2, ::2].copy(), ref[::2, ::2].copy()], pad=pad)
+    hw, hh, hp = W // 2, H // 2, pad // 2
+    nb = (hw // unit) * (hh // unit)
+    s = ctypes.c_void_p()
+    assert lib.xg_session_create(ctypes.byref(s), 1024) == 0
+    mv = torch.zeros(2 * nb, dtype=torch.int16, device="cuda")
+    cost = torch.zeros(nb, dtype=torch.int32, device="cuda")
+    sad = torch.zeros(nb, dtype=torch.int32, device="cuda")
+    rc = lib.xg_lookahead_me(s, ctypes.byref(dc.pic.plane[0]), ctypes.byref(dr.pic.plane[0]), unit, rng_, 1024,
+                             mv.data_ptr(), cost.data_ptr(), sad.data_ptr())
+    assert rc == 0 and lib.xg_session_sync(s) == 0
+    hcur = np.zeros((hh, hw), np.uint16)
+    oracle.or_Downscale2x2(ptr(cur), W, ptr(hcur), hw, hw, hh)
+    refp = np.ascontiguousarray(np.pad(ref, pad, mode="edge"))
+    href = np.zeros((hh + 2 * hp, hw + 2 * hp), np.uint16)
+    oracle.or_Downscale2x2(ptr(refp), W + 2 * pad, ptr(href), hw + 2 * hp, hw + 2 * hp, hh + 2 * hp)
+    emx = np.zeros(nb, np.int16)
+    emy = np.zeros(nb, np.int16)
+    ec = np.zeros(nb, np.uint32)
+    es = np.zeros(nb, np.uint32)
+    oracle.or_FullSearchFrame(ptr(hcur), hw, ptr(href, hp * (hw + 2 * hp) + hp), hw + 2 * hp, hw, hh, unit, unit,
+                              rng_, 1024, rng_, 0, nb, ptr(emx), ptr(emy), ptr(ec), ptr(es))
+    g = mv.cpu().numpy().reshape(nb, 2)
+    assert np.array_equal(g[:, 0], emx) and np.array_equal(g[:, 1], emy)
+    assert np.array_equal(cost.cpu().numpy().view(np.uint32), ec)
+    assert np.array_equal(sad.cpu().numpy().view(np.uint32), es)
+    # an odd padding is refused
+    dr2 = DevicePicture([ref, ref[::2, ::2].copy(), ref[::2, ::2].copy()], pad=pad + 1)
+    assert lib.xg_lookahead_me(s, ctypes.byref(dc.pic.plane[0]), ctypes.byref(dr2.pic.plane[0]), unit, rng_, 1024,
+                               mv.data_ptr(), cost.data_ptr(), sad.data_ptr()) == capi.XG_ERR_ARG
+    lib.xg_clear_error()
+    lib.xg_session_destroy(s)

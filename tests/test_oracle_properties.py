@@ -612,6 +612,19 @@ def test_dmvr_composition_brute_force(oracle):
                 k += 1
 
 
+def test_downscale_is_the_rounded_2x2_mean(oracle):
+    src = RNG.integers(0, 1024, size=(34, 50)).astype(np.uint16)
+    dst = np.zeros((17, 25), np.uint16)
+    oracle.or_Downscale2x2(ptr(src), 50, ptr(dst), 25, 25, 17)
+    s = src.astype(np.int64)
+    exp = (s[0::2, 0::2] + s[0::2, 1::2] + s[1::2, 0::2] + s[1::2, 1::2] + 2) >> 2
+    assert np.array_equal(dst, exp)
+    flat = np.full((4, 4), 1023, np.uint16)
+    out = np.zeros((2, 2), np.uint16)
+    oracle.or_Downscale2x2(ptr(flat), 4, ptr(out), 2, 2, 2)
+    assert (out == 1023).all()
+
+
 def test_fdct_batch_matches_per_block_calls(oracle):
     """The batched helper (the transform CPU baseline) is per-block or_FDctWxH
     on the int16 layout of the batched GPU entry."""

@@ -1,0 +1,117 @@
+// xg_lookahead.hip -- lookahead motion estimation on half-resolution pictures.
+//
+// The reference's lookahead (Xin26xLookaheadFrame, h26x_core/include/h26x_look_ahead.h:27)
+// searches 2x-downscaled pictures per unit of laUnitSize samples (8 with B frames, 16
+// without, h266_encoder/source/h266_encoder_create.c:1745) and leaves one MV per unit in
+// laMv; the inter CU search takes laMv << 1 as a full-resolution start candidate
+// (h266_encoder/source/h266_analyze_inter_cu.c:247-261).  Its downscale
+// (Xin26xDownscale2x2) and its search are closed, so this entry is a design of its own
+// that fills the same role:
+//   * downscale: each half-resolution sample is the rounded mean of its 2x2 source
+//     block, (a + b + c + d + 2) >> 2.  The reference picture is downscaled over its
+//     padding too (as h266_enc_init.c:2804-2813 downscales the padded buffer), so the
+//     half-resolution reference carries half the padding;
+//   * search: the integer full search of xg_full_search_frame on every unit wholly
+//     inside the half-resolution picture, predictor (0, 0), SAD + lambda * MV bits.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "xg_device.hpp"
+#include "xg_runtime.hpp"
+
+namespace xg {
+
+// dst (w x h) = rounded 2x2 means of src; src points at the top-left source sample
+__global__ void k_down2x2(const uint16_t *__restrict__ src, intptr_t ss, uint16_t *__restrict__ dst, intptr_t ds,
+                          int w, int h)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const uint16_t *p = src + (intptr_t)(2 * y) * ss + 2 * x;
+    dst[(intptr_t)y * ds + x] = (uint16_t)(((int)p[0] + p[1] + p[ss] + p[ss + 1] + 2) >> 2);
+}
+
+}  // namespace xg
+
+using namespace xg;
+
+namespace {
+struct HalfPics {
+    uint16_t *cur = nullptr, *ref = nullptr;
+    size_t curCap = 0, refCap = 0;
+};
+std::mutex g_la_mu;
+std::vector<std::pair<xg_session *, HalfPics>> g_la;
+
+HalfPics *half_for(xg_session *s)
+{
+    for (auto &p : g_la)
+        if (p.first == s) return &p.second;
+    g_la.push_back({s, HalfPics()});
+    return &g_la.back().second;
+}
+
+bool ensure(uint16_t *&p, size_t &cap, size_t n, hipStream_t st)
+{
+    if (n <= cap) return true;
+    if (p) {
+        hipStreamSynchronize(st);
+        hipFree(p);
+    }
+    p = nullptr;
+    cap = 0;
+    if (!XG_HIP(hipMalloc((void **)&p, n * sizeof(uint16_t)))) return false;
+    cap = n;
+    return true;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t xg_lookahead_me(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t unitSize, int32_t range,
+                        uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad)
+{
+    if (!s || !cur || !ref || !cur->data || !ref->data || !laMv || !cost || !sad || (unitSize != 8 && unitSize != 16) ||
+        range < 0 || range > 64 || cur->width != ref->width || cur->height != ref->height || cur->width < 2 ||
+        cur->height < 2 || (ref->pad & 1) || ref->pad / 2 < range + 4) {
+        set_error(XG_ERR_ARG, "xg_lookahead_me: bad arguments (units 8 or 16, range <= 64, reference padding even "
+                              "and >= 2 * (range + 4))");
+        return XG_ERR_ARG;
+    }
+    const int hw = cur->width / 2, hh = cur->height / 2, hp = ref->pad / 2;
+    const int rw = hw + 2 * hp, rh = hh + 2 * hp;
+    hipStream_t st = (hipStream_t)xg_session_stream(s);
+    xg_plane hc, hr;
+    {
+        std::lock_guard<std::mutex> g(g_la_mu);
+        HalfPics *h = half_for(s);
+        // the current picture's stride is kept even, so the packed search kernel takes it
+        const int cs = (hw + 1) & ~1;
+        if (!ensure(h->cur, h->curCap, (size_t)cs * hh, st) || !ensure(h->ref, h->refCap, (size_t)rw * rh, st))
+            return XG_ERR_NOMEM;
+        hc = xg_plane{h->cur, cs, hw, hh, 0};
+        hr = xg_plane{h->ref + (size_t)hp * rw + hp, rw, hw, hh, hp};
+    }
+    k_down2x2<<<dim3((hw + 127) / 128, hh), 128, 0, st>>>(cur->data, cur->stride, hc.data, hc.stride, hw, hh);
+    k_down2x2<<<dim3((rw + 127) / 128, rh), 128, 0, st>>>(ref->data - (intptr_t)ref->pad * ref->stride - ref->pad,
+                                                           ref->stride, hr.data - (intptr_t)hp * rw - hp, rw, rw, rh);
+    if (!XG_HIP(hipGetLastError())) return XG_ERR_HIP;
+    // window of a CU with ctuSize = range: MVs up to range + 8 samples past the picture
+    // edge, so the search keeps its full +/-range everywhere
+    return xg_full_search_frame(s, &hc, &hr, unitSize, unitSize, range, sadLambda, range, laMv, cost, sad);
+}
+
+}  // extern "C"
+
+void xg::lookahead_release(xg_session *s)
+{
+    std::lock_guard<std::mutex> g(g_la_mu);
+    for (size_t i = 0; i < g_la.size(); i++)
+        if (g_la[i].first == s) {
+            if (g_la[i].second.cur) hipFree(g_la[i].second.cur);
+            if (g_la[i].second.ref) hipFree(g_la[i].second.ref);
+            g_la.erase(g_la.begin() + (long)i);
+            return;
+        }
+}

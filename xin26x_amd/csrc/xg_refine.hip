@@ -1189,6 +1189,7 @@ int32_t xg_session_destroy(xg_session *s)
     fdct_release(s);
     recon_release(s);
     dmvr_release(s);
+    lookahead_release(s);
     pipe_release(s);
     if (s->graphExec) hipGraphExecDestroy(s->graphExec);
     if (s->graph) hipGraphDestroy(s->graph);

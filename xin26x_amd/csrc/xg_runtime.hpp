@@ -46,6 +46,8 @@ void fdct_release(xg_session *s);
 void recon_release(xg_session *s);
 // Releases the DMVR plan a session owns (xg_dmvr.hip).
 void dmvr_release(xg_session *s);
+// Releases the half-resolution pictures of the lookahead search (xg_lookahead.hip).
+void lookahead_release(xg_session *s);
 
 void pipe_release(xg_session *s);
 // Refinement of the session's current blocks, validated and uploaded but not
