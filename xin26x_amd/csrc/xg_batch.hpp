@@ -107,8 +107,8 @@ bool grow(T *&ptr, int &cap, int n, hipStream_t st)
 {
     if (n <= cap) return true;
     if (ptr) {
-        hipStreamSynchronize(st);
-        hipFree(ptr);
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(ptr);
     }
     ptr = nullptr;
     cap = 0;

@@ -515,8 +515,8 @@ void xg::dmvr_release(xg_session *s)
     std::lock_guard<std::mutex> g(g_dplan_mu);
     for (size_t i = 0; i < g_dplans.size(); i++)
         if (g_dplans[i].first == s) {
-            if (g_dplans[i].second.dTasks) hipFree(g_dplans[i].second.dTasks);
-            if (g_dplans[i].second.dSubs) hipFree(g_dplans[i].second.dSubs);
+            if (g_dplans[i].second.dTasks) (void)hipFree(g_dplans[i].second.dTasks);
+            if (g_dplans[i].second.dSubs) (void)hipFree(g_dplans[i].second.dSubs);
             g_dplans.erase(g_dplans.begin() + (long)i);
             return;
         }

@@ -614,8 +614,8 @@ void xg::fdct_release(xg_session *s)
     std::lock_guard<std::mutex> g(g_plan_mu);
     for (size_t i = 0; i < g_plans.size(); i++)
         if (g_plans[i].first == s) {
-            if (g_plans[i].second.dGroups) hipFree(g_plans[i].second.dGroups);
-            if (g_plans[i].second.dRel) hipFree(g_plans[i].second.dRel);
+            if (g_plans[i].second.dGroups) (void)hipFree(g_plans[i].second.dGroups);
+            if (g_plans[i].second.dRel) (void)hipFree(g_plans[i].second.dRel);
             g_plans.erase(g_plans.begin() + (long)i);
             return;
         }

@@ -56,8 +56,8 @@ bool ensure(uint16_t *&p, size_t &cap, size_t n, hipStream_t st)
 {
     if (n <= cap) return true;
     if (p) {
-        hipStreamSynchronize(st);
-        hipFree(p);
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(p);
     }
     p = nullptr;
     cap = 0;
@@ -109,8 +109,8 @@ void xg::lookahead_release(xg_session *s)
     std::lock_guard<std::mutex> g(g_la_mu);
     for (size_t i = 0; i < g_la.size(); i++)
         if (g_la[i].first == s) {
-            if (g_la[i].second.cur) hipFree(g_la[i].second.cur);
-            if (g_la[i].second.ref) hipFree(g_la[i].second.ref);
+            if (g_la[i].second.cur) (void)hipFree(g_la[i].second.cur);
+            if (g_la[i].second.ref) (void)hipFree(g_la[i].second.ref);
             g_la.erase(g_la.begin() + (long)i);
             return;
         }

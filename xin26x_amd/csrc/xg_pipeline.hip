@@ -504,10 +504,10 @@ void pipe_release(xg_session *s)
 {
     PipeState *p = static_cast<PipeState *>(session_pipe(s));
     if (!p) return;
-    if (p->exec) hipGraphExecDestroy(p->exec);
-    if (p->graph) hipGraphDestroy(p->graph);
-    hipFree(p->dMv);
-    hipFree(p->dCost);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->dMv);
+    (void)hipFree(p->dCost);
     delete p;
     session_pipe(s) = nullptr;
 }
@@ -581,8 +581,8 @@ int32_t xg_me_pipeline(xg_session *s, const xg_pipeline_cfg *cfg, const xg_pictu
         if ((rc = xg_session_set_blocks(s, grid.data(), nb)) != XG_OK) return rc;
         if (nb > p->cap) {
             if (!XG_HIP(hipStreamSynchronize(v.stream))) return XG_ERR_HIP;
-            hipFree(p->dMv);
-            hipFree(p->dCost);
+            (void)hipFree(p->dMv);
+            (void)hipFree(p->dCost);
             p->dMv = nullptr;
             p->dCost = nullptr;
             p->cap = 0;
@@ -629,8 +629,8 @@ int32_t xg_me_pipeline(xg_session *s, const xg_pipeline_cfg *cfg, const xg_pictu
     for (size_t i = 0; i < sizeof(xg_refine_cfg); i++) key = hmix(key, rb[i]);
     key = hmix(key, (uint64_t)(uintptr_t)p->dMv);
     if (!p->exec || key != p->key) {
-        if (p->exec) hipGraphExecDestroy(p->exec);
-        if (p->graph) hipGraphDestroy(p->graph);
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
         p->exec = nullptr;
         p->graph = nullptr;
         if (!XG_HIP(hipStreamBeginCapture(v.stream, hipStreamCaptureModeThreadLocal))) return XG_ERR_HIP;

@@ -501,8 +501,8 @@ void xg::recon_release(xg_session *s)
     std::lock_guard<std::mutex> g(g_rplan_mu);
     for (size_t i = 0; i < g_rplans.size(); i++)
         if (g_rplans[i].first == s) {
-            if (g_rplans[i].second.dGroups) hipFree(g_rplans[i].second.dGroups);
-            if (g_rplans[i].second.dRel) hipFree(g_rplans[i].second.dRel);
+            if (g_rplans[i].second.dGroups) (void)hipFree(g_rplans[i].second.dGroups);
+            if (g_rplans[i].second.dRel) (void)hipFree(g_rplans[i].second.dRel);
             g_rplans.erase(g_rplans.begin() + (long)i);
             return;
         }

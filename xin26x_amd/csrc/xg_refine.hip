@@ -786,7 +786,7 @@ static bool stage_upload(xg_session *s, void *dst, const void *src, size_t bytes
     if (bytes > s->hStageCap) {
         if (s->hStage) {
             if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
-            hipHostFree(s->hStage);
+            (void)hipHostFree(s->hStage);
         }
         s->hStage = nullptr;
         s->hStageCap = 0;
@@ -906,9 +906,9 @@ static bool build_tasks(xg_session *s, int ncand)
     assign_tasks(s, s->tasks, G, tasksW, wgBegin);
     if (s->tasks.size() > s->taskCap || G + 1 > s->wgCap) {
         if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
-        hipFree(s->dTasks);
-        hipFree(s->dTasksW);
-        hipFree(s->dWgBegin);
+        (void)hipFree(s->dTasks);
+        (void)hipFree(s->dTasksW);
+        (void)hipFree(s->dWgBegin);
         s->dTasks = s->dTasksW = nullptr;
         s->dWgBegin = nullptr;
         s->taskCap = 0;
@@ -1105,10 +1105,10 @@ static bool enqueue_batched(xg_session *s, RefineArgs ra, bool timed)
         e1 = ev_get(s);
         if (e0 < 0 || e1 < 0) timed = false;
     }
-    if (timed) hipEventRecord(s->evPool[e0], s->stream);
+    if (timed) (void)hipEventRecord(s->evPool[e0], s->stream);
     k_refine_persistent<<<grid, 64, 0, s->stream>>>(ra);
     if (timed) {
-        hipEventRecord(s->evPool[e1], s->stream);
+        (void)hipEventRecord(s->evPool[e1], s->stream);
         s->evPending[0].push_back({e0, e1});
     }
     return XG_HIP(hipGetLastError());
@@ -1123,7 +1123,7 @@ static bool enqueue_plain(xg_session *s, RefineArgs ra, bool timed)
         e1 = ev_get(s);
         if (e0 < 0 || e1 < 0) timed = false;
     }
-    if (timed) hipEventRecord(s->evPool[e0], s->stream);
+    if (timed) (void)hipEventRecord(s->evPool[e0], s->stream);
     for (int oi = 0; oi < kNumShapes; oi++) {
         const int b0 = s->shapeTaskBegin[oi], b1 = s->shapeTaskBegin[oi + 1];
         if (b1 <= b0) continue;
@@ -1134,7 +1134,7 @@ static bool enqueue_plain(xg_session *s, RefineArgs ra, bool timed)
         k_refine_shape<<<b1 - b0, 64, 0, s->stream>>>(ra);
     }
     if (timed) {
-        hipEventRecord(s->evPool[e1], s->stream);
+        (void)hipEventRecord(s->evPool[e1], s->stream);
         s->evPending[0].push_back({e0, e1});
     }
     return XG_HIP(hipGetLastError());
@@ -1185,28 +1185,28 @@ int32_t xg_session_create(xg_session **out, int32_t maxBlocks)
 int32_t xg_session_destroy(xg_session *s)
 {
     if (!s) return XG_OK;
-    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     fdct_release(s);
     recon_release(s);
     dmvr_release(s);
     lookahead_release(s);
     pipe_release(s);
-    if (s->graphExec) hipGraphExecDestroy(s->graphExec);
-    if (s->graph) hipGraphDestroy(s->graph);
-    for (auto e : s->evPool) hipEventDestroy(e);
-    hipFree(s->dPics);
-    hipFree(s->dPred);
-    hipFree(s->dBlocks);
-    hipFree(s->dTasks);
-    hipFree(s->dTasksW);
-    hipFree(s->dWgBegin);
-    hipFree(s->dCandCost);
-    hipFree(s->dBest);
-    hipFree(s->dDone);
-    hipFree(s->dBestKey);
-    hipFree(s->dOff);
-    if (s->hStage) hipHostFree(s->hStage);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->graphExec) (void)hipGraphExecDestroy(s->graphExec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (auto e : s->evPool) (void)hipEventDestroy(e);
+    (void)hipFree(s->dPics);
+    (void)hipFree(s->dPred);
+    (void)hipFree(s->dBlocks);
+    (void)hipFree(s->dTasks);
+    (void)hipFree(s->dTasksW);
+    (void)hipFree(s->dWgBegin);
+    (void)hipFree(s->dCandCost);
+    (void)hipFree(s->dBest);
+    (void)hipFree(s->dDone);
+    (void)hipFree(s->dBestKey);
+    (void)hipFree(s->dOff);
+    if (s->hStage) (void)hipHostFree(s->hStage);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return XG_OK;
 }
@@ -1327,8 +1327,8 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
     key = mix(key, ((uint64_t)s->nregions << 16) | (uint64_t)s->persistFactor);
     for (int i = 0; i < cfg->ncand; i++) key = mix(key, ((uint64_t)(uint16_t)cfg->offx[i] << 16) | (uint16_t)cfg->offy[i]);
     if (!s->graphExec || key != s->graphKey) {
-        if (s->graphExec) hipGraphExecDestroy(s->graphExec);
-        if (s->graph) hipGraphDestroy(s->graph);
+        if (s->graphExec) (void)hipGraphExecDestroy(s->graphExec);
+        if (s->graph) (void)hipGraphDestroy(s->graph);
         s->graphExec = nullptr;
         s->graph = nullptr;
         if (!XG_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal))) return XG_ERR_HIP;
@@ -1347,10 +1347,10 @@ int32_t xg_refine_frac(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
         e1 = ev_get(s);
         if (e0 < 0 || e1 < 0) timed = false;
     }
-    if (timed) hipEventRecord(s->evPool[e0], s->stream);
+    if (timed) (void)hipEventRecord(s->evPool[e0], s->stream);
     if (!XG_HIP(hipGraphLaunch(s->graphExec, s->stream))) return XG_ERR_HIP;
     if (timed) {
-        hipEventRecord(s->evPool[e1], s->stream);
+        (void)hipEventRecord(s->evPool[e1], s->stream);
         s->evPending[1].push_back({e0, e1});
     }
     return XG_OK;
@@ -1456,7 +1456,7 @@ static float time_frame(xg_session *s, const xg_refine_cfg *cfg, int reps, hipEv
             if (!enqueue_batched(s, ra, false)) return 0.f;
         if (!XG_HIP(hipEventRecord(e1, s->stream)) || !XG_HIP(hipEventSynchronize(e1))) return 0.f;
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (!XG_HIP(hipEventElapsedTime(&ms, e0, e1))) return 0.f;
         if (trial > 0) best = std::min(best, ms);   // the first trial warms up
     }
     *ok = true;
@@ -1514,8 +1514,8 @@ int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t rep
         }
         s->tile[sh] = bestT;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc == XG_OK && path) {
         FILE *f = fopen(path, "w");
         if (!f) {

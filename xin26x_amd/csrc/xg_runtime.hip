@@ -75,7 +75,7 @@ CallCtx *call_ctx(size_t devBytes, size_t hostBytes)
     }
     CallCtx *c = t_ctx;
     if (devBytes > c->devCap) {
-        if (c->dev) hipFree(c->dev);
+        if (c->dev) (void)hipFree(c->dev);
         c->dev = nullptr;
         c->devCap = 0;
         size_t cap = align_up(devBytes < (1u << 20) ? (1u << 20) : devBytes, 4096);
@@ -86,7 +86,7 @@ CallCtx *call_ctx(size_t devBytes, size_t hostBytes)
         c->devCap = cap;
     }
     if (hostBytes > c->hostCap) {
-        if (c->host) hipHostFree(c->host);
+        if (c->host) (void)hipHostFree(c->host);
         c->host = nullptr;
         c->hostCap = 0;
         size_t cap = align_up(hostBytes < (1u << 20) ? (1u << 20) : hostBytes, 4096);

@@ -235,7 +235,7 @@ struct xg_server {
         if (rc == XG_OK && hipEventRecord(b.done, hs) != hipSuccess) rc = XG_ERR_HIP;
         if (rc != XG_OK) {
             log_msg(0, "serve: batch of %zu frames failed (%d)", b.reqs.size(), rc);
-            hipStreamSynchronize(hs);
+            (void)hipStreamSynchronize(hs);
             for (Request *r : b.reqs) {
                 streams[r->stream].inFlight = false;
                 finish_request(r, rc, (int)b.reqs.size(), b.startNs);
@@ -292,7 +292,7 @@ struct xg_server {
                 // nothing can launch until a batch completes
                 hipEvent_t e = batch[oldest].done;
                 lk.unlock();
-                hipEventSynchronize(e);
+                (void)hipEventSynchronize(e);
                 lk.lock();
             } else if (active > 0) {
                 cvWork.wait_for(lk, std::chrono::microseconds(100));
@@ -354,18 +354,18 @@ int32_t xg_server_destroy(xg_server *srv)
     for (int i = 0; i < kMaxStreams; i++) {
         Stream &st = srv->streams[i];
         for (Request *r : st.pending) {
-            if (r->pinned) hipHostFree(r->pinned);
+            if (r->pinned) (void)hipHostFree(r->pinned);
             delete r;
         }
         st.pending.clear();
-        if (st.mem) hipFree(st.mem);
+        if (st.mem) (void)hipFree(st.mem);
     }
     for (int i = 0; i < kSessions; i++) {
         if (srv->sess[i]) xg_session_destroy(srv->sess[i]);
-        if (srv->batch[i].done) hipEventDestroy(srv->batch[i].done);
-        if (srv->batch[i].pinnedResults) hipHostFree(srv->batch[i].pinnedResults);
+        if (srv->batch[i].done) (void)hipEventDestroy(srv->batch[i].done);
+        if (srv->batch[i].pinnedResults) (void)hipHostFree(srv->batch[i].pinnedResults);
     }
-    for (auto &kv : srv->pinnedPool) hipHostFree(kv.second);
+    for (auto &kv : srv->pinnedPool) (void)hipHostFree(kv.second);
     delete srv;
     return XG_OK;
 }
@@ -416,7 +416,7 @@ int32_t xg_server_open_stream(xg_server *srv, int32_t width, int32_t height, int
         for (int k = 0; k < 2; k++) {
             if (xg_session_set_picture(srv->sess[si], 2 * id + k, &st.pic[k]) != XG_OK ||
                 xg_session_set_pred(srv->sess[si], 2 * id + k, &st.pred) != XG_OK) {
-                hipFree(st.mem);
+                (void)hipFree(st.mem);
                 st.mem = nullptr;
                 return XG_ERR_HIP;
             }
@@ -450,7 +450,7 @@ int32_t xg_server_close_stream(xg_server *srv, int32_t streamId)
     }
     srv->cvDone.notify_all();
     srv->cvDone.wait(lk, [&] { return !st.inFlight; });
-    hipFree(st.mem);
+    (void)hipFree(st.mem);
     st.mem = nullptr;
     srv->bytesStreams -= st.bytes;
     st.open = st.closing = false;
