@@ -9,7 +9,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def ptr(a, off=0):
-    return ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
+    """Address of a[off] (flat); the pointer object keeps `a` alive, so a
+    temporary array passed inline outlives the call it is passed to."""
+    p = ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
+    p._keep = a
+    return p
 
 
 def header_functions():

@@ -123,3 +123,16 @@ def test_oracle_bilinear_matches_reference_golden(oracle):
     assert np.array_equal(offs, g["offsets"])
     bad = np.nonzero(out != g["out"])[0]
     assert bad.size == 0, f"first mismatch in case {int(np.searchsorted(offs, bad[0], side='right') - 1)}"
+
+
+def test_oracle_conventions_frozen(oracle):
+    """Build-defined conventions of the closed reference functions, frozen
+    (tests/golden/convention_cases.py): a change to any of them must be
+    deliberate and regenerate the fixture."""
+    import os
+    import convention_cases as cc
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "convention_golden.npz"))
+    got = cc.compute(oracle)
+    assert set(got) == set(g.files)
+    for k in g.files:
+        assert np.array_equal(got[k], g[k]), k
