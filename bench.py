@@ -386,6 +386,62 @@ def lookahead_cpu(fp, unit, rng, lam, got, seconds):
                       f"threads, {dt:.1f} s", "bit_exact_vs_gpu_on_sample": exact}
 
 
+def bipred_leg(lib, fp, cur, ref, pred, steps, warmup, cpu_seconds):
+    """B-frame bi-prediction SATD of every 1080p block: the input against the
+    rounded average of two predictions (the reference picture and the
+    refinement's MC output)."""
+    import torch
+    from xin26x_amd import capi
+    from xin26x_amd.device import blocks_to_struct
+    barr = blocks_to_struct(fp.blocks)
+    n = len(barr)
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        capi.check(lib.xg_bipred_plan(s, barr.ctypes.data, n), "bipred plan")
+        dist = torch.empty(n, dtype=torch.int32, device="cuda")
+        pi, p0, p1 = cur.pic.plane[0], ref.pic.plane[0], pred.pic.plane[0]
+        run = lambda: capi.check(lib.xg_bipred_run(s, ctypes.byref(pi), ctypes.byref(p0), ctypes.byref(p1), 1,
+                                                   dist.data_ptr()), "bipred run")
+        k_ms, wall_ms = session_time(lib, s, 7, run, steps, warmup)
+        got = dist.cpu().numpy().view(np.uint32)
+        pred_h = pred.tensors[0].cpu().numpy().view(np.uint16).reshape(fp.height, fp.width)
+    finally:
+        lib.xg_session_destroy(s)
+    px = int((fp.blocks[:, 2] * fp.blocks[:, 3]).sum())
+    alg_bytes = 3 * 2 * px
+    k_s = k_ms / 1e3
+    out = {"workload": f"1080p, {n} QTMT blocks: SATD of the input against (reference + MC prediction + 1) >> 1 "
+                       "(8x8 Hadamard tiles, 4x4 where a side is not a multiple of 8)",
+           "value": round(fp.width * fp.height / (wall_ms / 1e3) / 1e6, 1), "unit": "Mpixel/s",
+           "blocks_per_s": round(n / (wall_ms / 1e3), 1), "ms_per_frame": round(wall_ms, 4),
+           "kernel": "k_bipred_dist", "kernel_ms": round(k_ms, 4),
+           "roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
+                        "alg_bytes_per_launch": alg_bytes}}
+    if cpu_seconds > 0:
+        from tests.oracle_lib import load_oracle
+        oracle = load_oracle()
+        inp, r0 = np.ascontiguousarray(fp.cur[0]), np.ascontiguousarray(fp.ref[0])
+        p1h = np.ascontiguousarray(pred_h)
+        W = fp.width
+        P = lambda a, o: ctypes.c_void_p(a.ctypes.data + 2 * o)
+        t0 = time.perf_counter()
+        done = 0
+        exact = True
+        for i, (x, y, w, h) in enumerate(fp.blocks[:, :4].tolist()):
+            o = y * W + x
+            exact &= oracle.or_ComputeAvgSatd(P(inp, o), W, P(r0, o), P(p1h, o), W, w, h) == int(got[i])
+            done += 1
+            if time.perf_counter() - t0 > cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(done / dt, 1), "unit": "blocks/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle (scalar C) on the first {done} blocks, one thread, {dt:.1f} s",
+                               "bit_exact_vs_gpu_on_sample": bool(exact)}
+    return out
+
+
 PMC_DMVR = os.path.join(ROOT, "profiles", "k_dmvr_1080p.md")
 
 
@@ -813,12 +869,14 @@ def main():
                                "fractional refinement, MC; reference = previous frame",
                    "realtime_30fps": rt, "saturated": sat}
 
-    dmvr = lookahead = None
+    dmvr = lookahead = bipred = None
     if not args.quick and not args.no_extra:
         log("DMVR: the 1080p bi-predicted blocks")
         dmvr = dmvr_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
         log("lookahead motion estimation at half resolution")
         lookahead = lookahead_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
+        log("bi-prediction SATD of the 1080p blocks")
+        bipred = bipred_leg(lib, fp, cur, ref, pred, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
 
     traffic, traffic_src = profiled_traffic(PMC_REFINE)
 
@@ -864,6 +922,7 @@ def main():
         "reconstruction": reconstruction,
         "dmvr_1080p": dmvr,
         "lookahead_1080p": lookahead,
+        "bipred_1080p": bipred,
         "plain_search_416x240": search,
         "pipeline_4k": pipeline,
         "serving": serving,

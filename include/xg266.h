@@ -513,7 +513,8 @@ int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best
  * 2 the forward-transform kernel (xg_fdct_run), 3 the integer-search kernel
  * (xg_search_blocks, xg_full_search_frame, the full search of xg_me_pipeline),
  * 4 a whole xg_me_pipeline frame, 5 the reconstruction kernel (xg_recon_run),
- * 6 the DMVR kernel (xg_dmvr_run). */
+ * 6 the DMVR kernel (xg_dmvr_run), 7 the bi-prediction distortion kernel
+ * (xg_bipred_run). */
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
 int32_t xg_session_reset_timers(xg_session *s);
 int32_t xg_session_set_timing(xg_session *s, int32_t enable);
@@ -690,6 +691,18 @@ typedef struct xg_dmvr_block {
 } xg_dmvr_block;
 int32_t xg_dmvr_plan(xg_session *s, const xg_dmvr_block *blocks, int32_t nblocks, int32_t *subOffset);
 int32_t xg_dmvr_run(xg_session *s, const xg_plane *ref0, const xg_plane *ref1, int16_t *mvd);
+
+/* Bi-prediction distortion of a frame's blocks: the B-frame step of
+ * h266_encoder/source/h266_analyze_inter_cu.c:1212-1222, pfXinComputeAvgDist
+ * (input, pred0, pred1) for every block in one launch -- SATD (satd != 0, as
+ * Xg266ComputeAvgSatd: 8x8 Hadamard tiles when both sides are multiples of 8,
+ * else 4x4) or SAD of input - ((pred0 + pred1 + 1) >> 1).
+ * xg_bipred_plan takes a HOST array of blocks (x, y, w, h; sides multiples of
+ * 4 in 4..128); xg_bipred_run checks every block against the three device
+ * planes and writes dist[i] (device) for block i. */
+int32_t xg_bipred_plan(xg_session *s, const xg_block *blocks, int32_t nblocks);
+int32_t xg_bipred_run(xg_session *s, const xg_plane *input, const xg_plane *pred0, const xg_plane *pred1,
+                      int32_t satd, uint32_t *dist);
 
 #ifdef __cplusplus
 }

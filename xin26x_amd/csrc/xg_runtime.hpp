@@ -48,6 +48,8 @@ void recon_release(xg_session *s);
 void dmvr_release(xg_session *s);
 // Releases the half-resolution pictures of the lookahead search (xg_lookahead.hip).
 void lookahead_release(xg_session *s);
+// Releases the bi-prediction distortion plan a session owns (xg_bipred.hip).
+void bipred_release(xg_session *s);
 
 void pipe_release(xg_session *s);
 // Refinement of the session's current blocks, validated and uploaded but not
@@ -59,7 +61,7 @@ void *&session_pipe(xg_session *s);
 // Session kernel timers (HIP events on the session stream), by stage:
 // 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search,
 // 4 a whole motion-search pipeline frame.
-constexpr int kTimerStages = 7;
+constexpr int kTimerStages = 8;
 bool session_timer_begin(xg_session *s, int stage, int ev[2]);   // false when timing is off
 void session_timer_end(xg_session *s, int stage, const int ev[2]);
 
