@@ -310,6 +310,114 @@ def recon_cpu(desc, offs, coeff, pred, gpu_recon, seconds):
             "bit_exact_vs_gpu_on_sample": bool(exact)}
 
 
+PMC_QUANT = os.path.join(ROOT, "profiles", "k_quant_200k_blocks.md")
+RESIDUAL_QP = 22
+
+
+def quant_descs(desc):
+    """Quantiser descriptors of the transform workload's TUs: 4x4 groups, QP 22
+    (table index QP + 12; 96% of the TUs keep non-zero levels, 7% of the
+    coefficients), a P frame's dead-zone rounding."""
+    return ((desc & 0xFFF) | 2 << 12 | 2 << 15 | (RESIDUAL_QP + 12) << 18).astype(np.uint32)
+
+
+def residual_loop_leg(lib, steps, warmup, cpu_seconds):
+    """The residual loop of the 200,000 TUs on the GPU, per batch: forward
+    transform, quantisation + dequantisation (QP 32), reconstruction with a
+    prediction; the quantiser against the HBM roofline."""
+    import torch
+    from xin26x_amd import capi, workloads as wl
+    desc, offs, resid = residual_workload()
+    n = len(desc)
+    dq_desc = quant_descs(desc)
+    w = 1 << (desc & 15).astype(np.int64)
+    h = 1 << ((desc >> 4) & 15).astype(np.int64)
+    region = np.minimum(w, 32) * np.minimum(h, 32)
+    samples = int((w * h).sum())
+    # coefficients read over the region; levels and dequantised values (int16) written over the TU; the maps
+    alg_bytes = int((4 * region + 4 * w * h).sum()) + 8 * n
+    pred = np.minimum((wl.uniform01(6, resid.size, stream=4) * 1024).astype(np.int64), 1023).astype(np.uint16)
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        dres = torch.from_numpy(resid).cuda()
+        dco = torch.empty(resid.size, dtype=torch.int32, device="cuda")
+        dq = torch.empty(resid.size, dtype=torch.int16, device="cuda")
+        dr = torch.empty(resid.size, dtype=torch.int16, device="cuda")
+        dnz = torch.empty(n, dtype=torch.int64, device="cuda")
+        dpred = torch.from_numpy(pred.view(np.int16)).cuda()
+        drec = torch.empty(resid.size, dtype=torch.int16, device="cuda")
+        capi.check(lib.xg_fdct_plan(s, offs.ctypes.data, desc.ctypes.data, n), "fdct plan")
+        capi.check(lib.xg_quant_plan(s, offs.ctypes.data, dq_desc.ctypes.data, n), "quant plan")
+        capi.check(lib.xg_recon_plan(s, offs.ctypes.data, desc.ctypes.data, n), "recon plan")
+
+        def chain():
+            capi.check(lib.xg_fdct_run(s, dres.data_ptr(), dco.data_ptr()), "fdct run")
+            capi.check(lib.xg_quant_run(s, dco.data_ptr(), dq.data_ptr(), dr.data_ptr(), dnz.data_ptr()), "quant")
+            capi.check(lib.xg_recon_run(s, dr.data_ptr(), dpred.data_ptr(), drec.data_ptr()), "recon run")
+
+        q_ms, wall_ms = session_time(lib, s, 8, chain, steps, warmup)
+        coeff = dco.cpu().numpy()
+        got = (dq.cpu().numpy(), dr.cpu().numpy(), dnz.cpu().numpy().view(np.uint64))
+        nz_share = float((got[2] != 0).mean())
+    finally:
+        lib.xg_session_destroy(s)
+    k_s = q_ms / 1e3
+    traffic, src = profiled_traffic(PMC_QUANT)
+    out = {"workload": "200,000 TUs of the transform workload per batch: forward transform, quantisation + "
+                       f"dequantisation (QP {RESIDUAL_QP}, 4x4 groups, P-frame dead zone), reconstruction with a "
+                       "uniform 10-bit prediction",
+           "value": round(n / (wall_ms / 1e3), 1), "unit": "blocks/s",
+           "samples_per_s": round(samples / (wall_ms / 1e3), 1), "ms_per_batch": round(wall_ms, 4),
+           "kernels": "k_fdct_mfma + k_quant + k_recon_mfma", "quant_kernel_ms": round(q_ms, 4),
+           "tus_with_nonzero_levels": round(nz_share, 4),
+           "quant_roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
+                              "traffic": traffic, "traffic_source": src, "alg_bytes_per_launch": alg_bytes}}
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = quant_cpu(desc, dq_desc, offs, coeff, got, cpu_seconds)
+    return out
+
+
+def quant_cpu(desc, dq_desc, offs, coeff, got, seconds):
+    """The oracle's quantiser on the host over chunks of the same TUs (the GPU's
+    forward coefficients as input); the chunks it computes are compared with the
+    GPU's levels, dequantised values and maps."""
+    from tests.oracle_lib import load_oracle
+    oracle = load_oracle()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    q = np.zeros(coeff.size, np.int16)
+    r = np.zeros(coeff.size, np.int16)
+    nz = np.zeros(len(desc), np.uint64)
+    chunk, n = 4000, len(desc)
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    done = []
+
+    def work(i0):
+        i1 = min(n, i0 + chunk)
+        oracle.or_QuantBatch(P(coeff), P(offs), P(dq_desc), i0, i1, P(q), P(r), P(nz))
+        return i0, i1
+
+    blocks = passes = 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < seconds:      # whole passes over the batch until the time is up
+            for i0, i1 in ex.map(work, range(0, n, chunk)):
+                if passes == 0:
+                    done.append((i0, i1))
+                blocks += i1 - i0
+            passes += 1
+    dt = time.perf_counter() - t0
+    end = lambda i: offs[i] + (1 << (desc[i] & 15)) * (1 << ((desc[i] >> 4) & 15))
+    exact = all(np.array_equal(q[offs[i0]:end(i1 - 1)], got[0][offs[i0]:end(i1 - 1)]) and
+                np.array_equal(r[offs[i0]:end(i1 - 1)], got[1][offs[i0]:end(i1 - 1)]) and
+                np.array_equal(nz[i0:i1], got[2][i0:i1]) for i0, i1 in done)
+    return {"value": round(blocks / dt, 1), "unit": "blocks/s (quantisation only)", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C, -O3) quantiser, {passes} passes over the {n} TUs in chunks of {chunk}, "
+                      f"{threads} host threads, {dt:.1f} s",
+            "bit_exact_vs_gpu_on_sample": bool(exact)}
+
+
 def lookahead_leg(lib, fp, cur, ref, steps, warmup, cpu_seconds, unit=16, rng=32, lam=1024):
     """Lookahead motion estimation of the 1080p pair: both luma planes downscaled
     2x, every 16x16 unit of the 960x540 picture searched +/-32 (+/-64 at full
@@ -386,6 +494,9 @@ def lookahead_cpu(fp, unit, rng, lam, got, seconds):
                       f"threads, {dt:.1f} s", "bit_exact_vs_gpu_on_sample": exact}
 
 
+PMC_BIPRED = os.path.join(ROOT, "profiles", "k_bipred_tiles_1080p.md")
+
+
 def bipred_leg(lib, fp, cur, ref, pred, steps, warmup, cpu_seconds):
     """B-frame bi-prediction SATD of every 1080p block: the input against the
     rounded average of two predictions (the reference picture and the
@@ -415,9 +526,10 @@ def bipred_leg(lib, fp, cur, ref, pred, steps, warmup, cpu_seconds):
                        "(8x8 Hadamard tiles, 4x4 where a side is not a multiple of 8)",
            "value": round(fp.width * fp.height / (wall_ms / 1e3) / 1e6, 1), "unit": "Mpixel/s",
            "blocks_per_s": round(n / (wall_ms / 1e3), 1), "ms_per_frame": round(wall_ms, 4),
-           "kernel": "k_bipred_dist", "kernel_ms": round(k_ms, 4),
+           "kernel": "k_zero_u32 + k_bipred_tiles", "kernel_ms": round(k_ms, 4),
            "roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
+                        "traffic": profiled_traffic(PMC_BIPRED)[0], "traffic_source": profiled_traffic(PMC_BIPRED)[1],
                         "alg_bytes_per_launch": alg_bytes}}
     if cpu_seconds > 0:
         from tests.oracle_lib import load_oracle
@@ -844,7 +956,7 @@ def main():
     plain = frame_px / t_plain / 1e6
     plain_tuned = frame_px / t_plain_tuned / 1e6
 
-    transforms = reconstruction = search = pipeline = None
+    transforms = reconstruction = residual_loop = search = pipeline = None
     if not args.quick and not args.no_extra:
         log("full motion-search pipeline, 4K, two references")
         pipeline = pipeline_leg(lib, max(5, args.steps // 4), 2, 0 if args.no_cpu else 10.0)
@@ -852,6 +964,8 @@ def main():
         transforms = transforms_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
         log("reconstruction: the same 200,000 blocks")
         reconstruction = recon_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
+        log("residual loop: forward transform, quantisation, reconstruction of the same blocks")
+        residual_loop = residual_loop_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
         log("plain 416x240 full search")
         search = search_leg(lib, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
 
@@ -920,6 +1034,7 @@ def main():
         "cpu_baseline": cpu,
         "transforms": transforms,
         "reconstruction": reconstruction,
+        "residual_loop": residual_loop,
         "dmvr_1080p": dmvr,
         "lookahead_1080p": lookahead,
         "bipred_1080p": bipred,

@@ -514,7 +514,7 @@ int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best
  * (xg_search_blocks, xg_full_search_frame, the full search of xg_me_pipeline),
  * 4 a whole xg_me_pipeline frame, 5 the reconstruction kernel (xg_recon_run),
  * 6 the DMVR kernel (xg_dmvr_run), 7 the bi-prediction distortion kernel
- * (xg_bipred_run). */
+ * (xg_bipred_run), 8 the quantisation kernel (xg_quant_run). */
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);
 int32_t xg_session_reset_timers(xg_session *s);
 int32_t xg_session_set_timing(xg_session *s, int32_t enable);
@@ -675,6 +675,31 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff);
  * pointers); recon must not overlap coeff or pred. */
 int32_t xg_recon_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t nblocks);
 int32_t xg_recon_run(xg_session *s, const int16_t *coeff, const xg_pixel *pred, xg_pixel *recon);
+
+/* Batched quantisation + dequantisation: pfXinQuantInvQuant for every TU of a
+ * batch, as Xin266Transform calls it (h266_encoder/source/h266_trans_quant.c:
+ * 204-222) with the parameters it derives at :104-120 from the tables of
+ * h266_encoder/source/h266_encoder_create.c:94-104,1677-1705 (10-bit); the
+ * per-TU arithmetic of Xin266QuantInvQuant (h266_quant_inv_quant.c:80-143).
+ * xg_quant_plan takes HOST arrays: TU i's forward coefficients are at
+ * tCoeff + offset[i] (xg_coeff, row-major, stride = width; offset a multiple of
+ * min(4, width)) and its levels and dequantised coefficients go to the same
+ * offset of qCoeff / rCoeff (int16 -- both are clipped to that range --, the
+ * layout xg_recon_run reads).  desc[i] = lgW | lgH << 4 | mtsIdx << 8 |
+ * lgCGW << 12 | lgCGH << 15 | qp << 18 | rdoq << 24 | intraFrame << 25:
+ * sides 2..64; mtsIdx 1 = transform skip; coefficient groups of
+ * 2^lgCGW x 2^lgCGH inside the min(W, 32) x min(H, 32) region that is
+ * quantised (at most 64 groups there, 256 in the TU, 64 coefficients each);
+ * qp the quantParam index (QP + 12 for 10-bit, 0..63); rdoq selects the
+ * 1 << (qShift - 1) rounding of an RDOQ configuration, else the dead zone of
+ * 171 (intraFrame, frameType >= I) or 85 << (qShift - 9).  Past the region
+ * both outputs are zero.
+ * xg_quant_run enqueues every planned TU on the session stream (device
+ * pointers: tCoeff 16-byte aligned, the others 8-byte aligned) and writes
+ * nzMap[i]: bit k set when group k of TU i's region (raster order) holds a
+ * non-zero level, as *nzCGBitMapRs. */
+int32_t xg_quant_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t nblocks);
+int32_t xg_quant_run(xg_session *s, const xg_coeff *tCoeff, int16_t *qCoeff, int16_t *rCoeff, uint64_t *nzMap);
 
 /* Batched DMVR (the refinement of Xg266DeriveMvFromDmvr for many blocks).
  * xg_dmvr_plan takes a HOST array of blocks and writes subOffset[0..n]

@@ -1452,3 +1452,63 @@ void or_QuantInvQuant(int32_t *q, const int32_t *t, int32_t *r, intptr_t stride,
         }
     *nzMap = map;
 }
+
+/* ----------------------------------------------------------------------------
+ * Quantiser parameters of one TU: Xin266Transform
+ * (h266_encoder/source/h266_trans_quant.c:104-120) with the quantParam table
+ * of Xin266ConstructQuantParam (h266_encoder/source/h266_encoder_create.c:
+ * 1677-1705; scales :94-104, the H.265/H.266 level scales and their
+ * half-octave-shifted set for odd log2 areas), bit depth 10:
+ *   adj = (lgW + lgH odd) and not skip, lgSize = (lgW + lgH) / 2,
+ *   qShift = 14 + (15 - 10) + qp / 6 - lgSize - adj,
+ *   iqShift = 6 - (15 - 10) + lgSize + adj,
+ *   qAdd = rdoq and not skip ? 1 << (qShift - 1)
+ *                            : (I frame ? 171 : 85) << (qShift - 9),
+ *   iqAdd = not skip ? 1 << (iqShift - 1) : 1 << (iqShift - lgSize + 15 - 10 - 1).
+ * ------------------------------------------------------------------------- */
+static const int32_t kQScale[2][6] = {{26214, 23302, 20560, 18396, 16384, 14564},
+                                      {18396, 16384, 14564, 13107, 11651, 10280}};
+static const int32_t kIqScale[2][6] = {{40, 45, 51, 57, 64, 72}, {57, 64, 72, 80, 90, 102}};
+
+void or_QuantParams(uint32_t desc, int32_t out[6])
+{
+    const int lgW = desc & 15, lgH = (desc >> 4) & 15, skip = ((desc >> 8) & 15) == 1;
+    const int qp = (desc >> 18) & 63, rdoq = (desc >> 24) & 1, intra = (desc >> 25) & 1;
+    const int lgTu = lgW + lgH, adj = (lgTu & 1) && !skip, lgSize = lgTu >> 1;
+    const int per = qp / 6, rem = qp % 6;
+    const int qShift = 14 + (15 - 10) + per - lgSize - adj;
+    const int iqShift = 6 - (15 - 10) + lgSize + adj;
+    out[0] = kQScale[adj][rem];
+    out[1] = (rdoq && !skip) ? 1 << (qShift - 1) : (intra ? 171 : 85) << (qShift - 9);
+    out[2] = qShift;
+    out[3] = kIqScale[adj][rem] << per;
+    out[4] = !skip ? 1 << (iqShift - 1) : 1 << ((iqShift - lgSize) + (15 - 10) - 1);
+    out[5] = iqShift;
+}
+
+/* The batched quantiser (h266_trans_quant.c:204-222 per TU, the region being
+ * XIN_MIN(width, 32) x XIN_MIN(height, 32); the reference zeroes the
+ * dequantised coefficients past it at :163-188). */
+void or_QuantBatch(const int32_t *t, const int64_t *offset, const uint32_t *desc, int32_t i0, int32_t i1,
+                   int16_t *q, int16_t *r, uint64_t *nzMap)
+{
+    for (int32_t i = i0; i < i1; i++) {
+        const uint32_t d = desc[i];
+        const int w = 1 << (d & 15), h = 1 << ((d >> 4) & 15);
+        const int rw = w < 32 ? w : 32, rh = h < 32 ? h : 32;
+        int32_t p[6];
+        or_QuantParams(d, p);
+        int32_t *q32 = (int32_t *)malloc(sizeof(int32_t) * (size_t)w * h);
+        int32_t *r32 = (int32_t *)malloc(sizeof(int32_t) * (size_t)w * h);
+        memset(q32, 0, sizeof(int32_t) * (size_t)w * h);
+        memset(r32, 0, sizeof(int32_t) * (size_t)w * h);
+        or_QuantInvQuant(q32, t + offset[i], r32, w, (uint32_t)rw, (uint32_t)rh, 1u << ((d >> 12) & 7),
+                         1u << ((d >> 15) & 7), p[0], p[1], p[2], p[3], p[4], p[5], nzMap + i);
+        for (int k = 0; k < w * h; k++) {
+            q[offset[i] + k] = (int16_t)q32[k];
+            r[offset[i] + k] = (int16_t)r32[k];
+        }
+        free(q32);
+        free(r32);
+    }
+}

@@ -252,6 +252,20 @@ void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, i
 void or_QuantInvQuant(int32_t *q, const int32_t *t, int32_t *r, intptr_t stride, uint32_t w, uint32_t h,
                       uint32_t cgw, uint32_t cgh, int32_t qMult, int32_t qAdd, int32_t qShift, int32_t iqMult,
                       int32_t iqAdd, int32_t iqShift, uint64_t *nzMap);
+/* Per-TU quantiser parameters as Xin266Transform derives them
+ * (h266_encoder/source/h266_trans_quant.c:104-120, tables of
+ * h266_encoder/source/h266_encoder_create.c:94-104,1677-1705; 10-bit).
+ * desc = lgW | lgH << 4 | mtsIdx << 8 | lgCGW << 12 | lgCGH << 15 | qp << 18
+ * | rdoq << 24 | intraFrame << 25, qp the table index (QP + 12, 0..63),
+ * mtsIdx 1 = transform skip.  out = {qMult, qAdd, qShift, iqMult, iqAdd,
+ * iqShift}. */
+void or_QuantParams(uint32_t desc, int32_t out[6]);
+/* TUs [i0, i1) of a batch: TU i's coefficients at t + offset[i] (stride =
+ * width); its levels and dequantised values go to q / r + offset[i] as int16,
+ * over the min(w, 32) x min(h, 32) region (zeros past it), and its non-zero
+ * group map to nzMap[i]. */
+void or_QuantBatch(const int32_t *t, const int64_t *offset, const uint32_t *desc, int32_t i0, int32_t i1,
+                   int16_t *q, int16_t *r, uint64_t *nzMap);
 
 #ifdef __cplusplus
 }

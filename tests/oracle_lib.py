@@ -93,6 +93,8 @@ def load_oracle():
         "or_IDctAdd": (ctypes.c_int, [P, SS, P, SS, P, SS, U32, U32, U32, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]),
         "or_QuantInvQuant": (None, [P, P, P, SS, U32, U32, U32, U32, I32, I32, I32, I32, I32, I32, P]),
+        "or_QuantParams": (None, [U32, P]),
+        "or_QuantBatch": (None, [P, P, P, I32, I32, P, P, P]),
         "or_MePipeline": (None, [P, P, P, SS, SS, P, P, P, P, P, P, SS, SS, I32, I32, I32, I32, I32, U32, I32,
                                  ctypes.POINTER(OrRefinePrm)] + [P] * 15 + [SS, SS]),
         "or_BlockSubForDct": (None, [P, SS, P, SS, P, SS, U32, U32]),

@@ -1,0 +1,181 @@
+"""Batched quantisation + dequantisation against the CPU oracle: every TU
+shape 2..64 x 2..64 with its coefficient-group shapes (4x4, 2x8, 8x2, 2x2,
+16x1, 1x16), every QP index, transform skip, RDOQ and dead-zone rounding for
+I and P frames, coefficients up to the int32 limits (wrapping products),
+64-wide TUs (zeros past the 32x32 region), levels, dequantised values and
+non-zero group maps bit-exact; the fdct -> quant -> recon chain against the
+oracle's; argument errors."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.helpers import ptr
+from xin26x_amd import capi, workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    l = capi.load()
+    l.xg_clear_error()
+    yield l
+    assert l.xg_last_error() == 0
+
+
+def qdesc(lgw, lgh, qp, mts=0, rdoq=1, intra=0, lgcw=2, lgch=2):
+    return lgw | lgh << 4 | mts << 8 | lgcw << 12 | lgch << 15 | qp << 18 | rdoq << 24 | intra << 25
+
+
+def _cg_shapes(lgw, lgh):
+    """Coefficient-group shapes that tile the region: 4x4 when both sides allow,
+    the 2-wide / 2-high VVC shapes, and the 16x1 / 1x16 ones."""
+    rw, rh = min(lgw, 5), min(lgh, 5)
+    out = []
+    for cw, ch in ((2, 2), (1, 3), (3, 1), (1, 1), (4, 0), (0, 4)):
+        if cw <= rw and ch <= rh and (rw - cw) + (rh - ch) <= 6 and (lgw - cw) + (lgh - ch) <= 8:
+            out.append((cw, ch))
+    return out
+
+
+def _batch(rng, reps=2):
+    descs = []
+    for lgw in range(1, 7):
+        for lgh in range(1, 7):
+            for (cw, ch) in _cg_shapes(lgw, lgh):
+                for _ in range(reps):
+                    skip = int(rng.random() < 0.25 and lgw <= 5 and lgh <= 5)
+                    descs.append(qdesc(lgw, lgh, int(rng.integers(0, 64)), mts=1 if skip else 0,
+                                       rdoq=int(rng.integers(0, 2)), intra=int(rng.integers(0, 2)), lgcw=cw, lgch=ch))
+    order = rng.permutation(len(descs))
+    desc = np.array([descs[i] for i in order], np.uint32)
+    offs, o = [], 0
+    for d in desc.tolist():
+        w, h = 1 << (d & 15), 1 << ((d >> 4) & 15)
+        o = (o + 3) // 4 * 4 + 4 * int(rng.integers(0, 2))     # gaps between some TUs
+        offs.append(o)
+        o += w * h
+    total = (o + 7) // 8 * 8
+    t = rng.laplace(0, 300, total).astype(np.int64)
+    big = rng.random(total) < 0.02
+    t[big] = rng.integers(-(2 ** 31) + 1, 2 ** 31, int(big.sum()))
+    t[rng.random(total) < 0.3] = 0
+    return desc, np.array(offs, np.int64), np.clip(t, -(2 ** 31) + 1, 2 ** 31 - 1).astype(np.int32)
+
+
+def _oracle(oracle, desc, offs, t):
+    q = np.full(t.size, 0x5A5A, np.int16)
+    r = np.full(t.size, 0x5A5A, np.int16)
+    nz = np.zeros(len(desc), np.uint64)
+    oracle.or_QuantBatch(ptr(t), ptr(offs), ptr(desc), 0, len(desc), ptr(q), ptr(r), ptr(nz))
+    return q, r, nz
+
+
+def _gpu(lib, s, desc, offs, t):
+    import torch
+    assert lib.xg_quant_plan(s, ptr(offs), ptr(desc), len(desc)) == 0
+    dt = torch.from_numpy(t).cuda()
+    dq = torch.full((t.size,), 0x5A5A, dtype=torch.int16, device="cuda")
+    dr = torch.full((t.size,), 0x5A5A, dtype=torch.int16, device="cuda")
+    dnz = torch.zeros(len(desc), dtype=torch.int64, device="cuda")
+    assert lib.xg_quant_run(s, dt.data_ptr(), dq.data_ptr(), dr.data_ptr(), dnz.data_ptr()) == 0
+    assert lib.xg_session_sync(s) == 0
+    return dq.cpu().numpy(), dr.cpu().numpy(), dnz.cpu().numpy().view(np.uint64)
+
+
+def _session(lib):
+    s = ctypes.c_void_p()
+    assert lib.xg_session_create(ctypes.byref(s), 16) == 0
+    return s
+
+
+def test_quant_batch_matches_oracle(lib, oracle):
+    rng = np.random.default_rng(81)
+    s = _session(lib)
+    try:
+        for _ in range(2):                                  # a second plan on the same session
+            desc, offs, t = _batch(rng)
+            eq, er, enz = _oracle(oracle, desc, offs, t)
+            gq, gr, gnz = _gpu(lib, s, desc, offs, t)
+            bad = np.nonzero((gq != eq) | (gr != er))[0]
+            assert bad.size == 0, f"first mismatch at coefficient {bad[0]}"
+            badm = np.nonzero(gnz != enz)[0]
+            assert badm.size == 0, (int(badm[0]), hex(int(desc[badm[0]])), hex(int(gnz[badm[0]])),
+                                    hex(int(enz[badm[0]])))
+            assert (enz != 0).any() and (enz == 0).any()
+    finally:
+        lib.xg_session_destroy(s)
+
+
+def test_quant_chain_fdct_quant_recon(lib, oracle):
+    """The residual loop of the transform workload on the GPU -- forward
+    transform, quantisation at QP 22 (P frame, dead zone), reconstruction with a
+    prediction -- against the oracle's chain, on 4,000 blocks."""
+    import torch
+    desc_t, offs, resid = wl.residual_blocks(n=4000)
+    n = len(desc_t)
+    qp = 22 + 12
+    desc_q = np.array([(d & 0xFFF) | 2 << 12 | 2 << 15 | qp << 18 for d in desc_t.tolist()], np.uint32)
+    pred = (np.arange(resid.size) * 7 % 1024).astype(np.uint16)
+    coeff = np.zeros(resid.size, np.int32)
+    oracle.or_FDctBatch(ptr(resid), ptr(offs), ptr(desc_t), 0, n, ptr(coeff))
+    eq, er, enz = _oracle(oracle, desc_q, offs, coeff)
+    erec = np.zeros(resid.size, np.uint16)
+    oracle.or_ReconBatch(ptr(er), ptr(pred), ptr(offs), ptr(desc_t), 0, n, ptr(erec))
+    s = _session(lib)
+    try:
+        dres = torch.from_numpy(resid).cuda()
+        dco = torch.empty(resid.size, dtype=torch.int32, device="cuda")
+        dq = torch.empty(resid.size, dtype=torch.int16, device="cuda")
+        dr = torch.empty(resid.size, dtype=torch.int16, device="cuda")
+        dnz = torch.empty(n, dtype=torch.int64, device="cuda")
+        dpred = torch.from_numpy(pred.view(np.int16)).cuda()
+        drec = torch.empty(resid.size, dtype=torch.int16, device="cuda")
+        assert lib.xg_fdct_plan(s, ptr(offs), ptr(desc_t), n) == 0
+        assert lib.xg_quant_plan(s, ptr(offs), ptr(desc_q), n) == 0
+        assert lib.xg_recon_plan(s, ptr(offs), ptr(desc_t), n) == 0
+        assert lib.xg_fdct_run(s, dres.data_ptr(), dco.data_ptr()) == 0
+        assert lib.xg_quant_run(s, dco.data_ptr(), dq.data_ptr(), dr.data_ptr(), dnz.data_ptr()) == 0
+        assert lib.xg_recon_run(s, dr.data_ptr(), dpred.data_ptr(), drec.data_ptr()) == 0
+        assert lib.xg_session_sync(s) == 0
+        inside = np.zeros(resid.size, bool)
+        for d, o in zip(desc_t.tolist(), offs.tolist()):
+            inside[o:o + (1 << (d & 15)) * (1 << ((d >> 4) & 15))] = True
+        assert np.array_equal(dq.cpu().numpy()[inside], eq[inside])
+        assert np.array_equal(dr.cpu().numpy()[inside], er[inside])
+        assert np.array_equal(dnz.cpu().numpy().view(np.uint64), enz)
+        assert (enz != 0).mean() > 0.5 and (eq[inside] != 0).any()
+        assert np.array_equal(drec.cpu().numpy().view(np.uint16)[inside], erec[inside])
+    finally:
+        lib.xg_session_destroy(s)
+
+
+def test_quant_rejects_bad_inputs(lib):
+    import torch
+    s = _session(lib)
+    try:
+        offs = np.zeros(1, np.int64)
+        t = torch.zeros(4096, dtype=torch.int32, device="cuda")
+        q = torch.zeros(4096, dtype=torch.int16, device="cuda")
+        nz = torch.zeros(1, dtype=torch.int64, device="cuda")
+        assert lib.xg_quant_run(s, t.data_ptr(), q.data_ptr(), q.data_ptr(), nz.data_ptr()) == capi.XG_ERR_STATE
+        lib.xg_clear_error()
+        for d, o in ((qdesc(6, 6, 30, lgcw=6, lgch=0), 0),    # group wider than the 32-wide region
+                     (qdesc(0, 3, 30, lgcw=0, lgch=2), 0),    # 1-wide TU
+                     (qdesc(5, 5, 30, lgcw=0, lgch=0), 0),    # 1024 groups in the region
+                     (qdesc(4, 4, 30) | 1 << 26, 0),          # unknown descriptor bits
+                     (qdesc(4, 4, 30), 6)):                   # offset not a multiple of 4
+            offs[0] = o
+            assert lib.xg_quant_plan(s, ptr(offs), ptr(np.array([d], np.uint32)), 1) == capi.XG_ERR_ARG, hex(d)
+            lib.xg_clear_error()
+        assert lib.xg_quant_run(s, t.data_ptr(), q.data_ptr(), q.data_ptr(), nz.data_ptr()) == capi.XG_ERR_STATE
+        lib.xg_clear_error()
+        offs[0] = 0
+        assert lib.xg_quant_plan(s, ptr(offs), ptr(np.array([qdesc(4, 4, 30)], np.uint32)), 1) == 0
+        assert lib.xg_quant_run(s, t.data_ptr() + 4, q.data_ptr(), q.data_ptr(), nz.data_ptr()) == capi.XG_ERR_ARG
+        lib.xg_clear_error()
+        assert lib.xg_quant_plan(s, None, None, 0) == 0
+        assert lib.xg_quant_run(s, t.data_ptr(), q.data_ptr(), q.data_ptr(), nz.data_ptr()) == 0
+    finally:
+        lib.xg_session_destroy(s)

@@ -638,3 +638,97 @@ def test_fdct_batch_matches_per_block_calls(oracle):
         r32 = resid[o:o + w * h].astype(np.int32)
         oracle.or_FDctWxH(ptr(r32), w, ptr(exp, o), w, w, h, mts, 0, 0, 10)
     assert np.array_equal(got, exp)
+
+
+# ---- batched quantiser parameters (h266_trans_quant.c:104-120) ----
+
+def _qdesc(lgw, lgh, qp, skip=0, rdoq=1, intra=0, lgcgw=2, lgcgh=2):
+    return lgw | lgh << 4 | (1 if skip else 0) << 8 | lgcgw << 12 | lgcgh << 15 | qp << 18 | rdoq << 24 | intra << 25
+
+
+def _qparams(oracle, d):
+    out = np.zeros(6, np.int32)
+    oracle.or_QuantParams(d, ptr(out))
+    return [int(v) for v in out]
+
+
+def test_quant_scales_are_level_scales(oracle):
+    """The dequantiser scales are the H.265/H.266 levelScale {40, 45, 51, 57,
+    64, 72} times 2^(qp/6); each forward scale times its inverse is 2^20
+    within 0.005%; the set used for odd log2 areas is the same tables shifted
+    by half an octave (three entries)."""
+    base_q, base_iq, adj_q, adj_iq = [], [], [], []
+    for rem in range(6):
+        p0 = _qparams(oracle, _qdesc(2, 2, 12 + rem))            # 4x4: even area, per = 2
+        p1 = _qparams(oracle, _qdesc(2, 3, 12 + rem))            # 4x8: odd area
+        base_q.append(p0[0]); base_iq.append(p0[3] >> 2)
+        adj_q.append(p1[0]); adj_iq.append(p1[3] >> 2)
+    assert base_iq == [40, 45, 51, 57, 64, 72]
+    for q, iq in zip(base_q + adj_q, base_iq + adj_iq):
+        assert abs(q * iq - (1 << 20)) < 1 << 20 >> 14
+    for r in range(6):
+        assert adj_q[r] == (base_q[r + 3] if r < 3 else base_q[r - 3] // 2)
+        assert adj_iq[r] == (base_iq[r + 3] if r < 3 else 2 * base_iq[r - 3])
+    # qp / 6 doubles the dequantiser scale and adds one to the forward shift
+    a, b = _qparams(oracle, _qdesc(3, 3, 20)), _qparams(oracle, _qdesc(3, 3, 26))
+    assert b[3] == 2 * a[3] and b[2] == a[2] + 1 and b[0] == a[0]
+
+
+def _round_trip(oracle, d, t):
+    w, h = 1 << (d & 15), 1 << ((d >> 4) & 15)
+    tt = np.ascontiguousarray(t.reshape(h, w).astype(np.int32))
+    q = np.zeros(w * h, np.int16)
+    r = np.zeros(w * h, np.int16)
+    nz = np.zeros(1, np.uint64)
+    oracle.or_QuantBatch(ptr(tt), ptr(np.zeros(1, np.int64)), ptr(np.array([d], np.uint32)), 0, 1,
+                         ptr(q), ptr(r), ptr(nz))
+    return q.reshape(h, w), r.reshape(h, w), int(nz[0])
+
+
+def test_quant_round_trip_within_half_a_step(oracle):
+    """Quantise then dequantise: with RDOQ rounding the result is within half a
+    quantiser step (plus one for the integer rounding) of the input at every
+    TU size and QP -- a forward or inverse shift off by one, or a wrong
+    half-octave set, breaks it by a factor of two or sqrt(2).  Without RDOQ
+    the dead zone keeps it within one step; transform skip adds its fixed
+    dequantiser offset of 32 / 2^iqShift."""
+    rng = np.random.default_rng(71)
+    for lgw in range(1, 7):
+        for lgh in range(1, 7):
+            w, h = 1 << lgw, 1 << lgh
+            for qp in range(0, 64, 3):
+                for skip, rdoq, intra in ((0, 1, 0), (0, 0, 1), (0, 0, 0), (1, 1, 0)):
+                    if skip and (w > 32 or h > 32):
+                        continue
+                    lgcg = (min(lgw, 2), min(lgh, 2))
+                    d = _qdesc(lgw, lgh, qp, skip, rdoq, intra, *lgcg)
+                    p = _qparams(oracle, d)
+                    step = p[3] / (1 << p[5])               # dequantiser step in coefficient units
+                    lim = min(30000, int(200 * step) + 50)
+                    t = rng.integers(-lim, lim + 1, size=w * h)
+                    _, r, _ = _round_trip(oracle, d, t)
+                    reg = np.zeros((h, w), bool)
+                    reg[:min(h, 32), :min(w, 32)] = True
+                    err = np.abs(r.astype(np.int64) - t.reshape(h, w))[reg]
+                    bound = (step / 2 if rdoq and not skip else step) + 1
+                    if skip:
+                        bound += 32 / (1 << p[5])
+                    assert err.max() <= bound * 1.001 + 1e-9, (w, h, qp, skip, rdoq, intra, err.max(), step)
+                    assert (r[~reg] == 0).all()
+
+
+def test_quant_batch_region_and_group_map(oracle):
+    """A 64x64 TU quantises its top-left 32x32 (64 groups of 4x4, bit k = group
+    k in raster order); a group holding a single non-zero level sets only its
+    bit."""
+    d = _qdesc(6, 6, 30)
+    t = np.zeros(64 * 64, np.int64)
+    q, r, nz = _round_trip(oracle, d, t)
+    assert nz == 0 and not q.any() and not r.any()
+    for (y, x) in ((0, 0), (5, 17), (31, 31), (12, 3)):
+        t = np.zeros((64, 64), np.int64)
+        t[y, x] = 20000
+        t[40, 50] = 20000                                   # past the region: ignored
+        q, r, nz = _round_trip(oracle, d, t.ravel())
+        assert nz == 1 << ((y // 4) * 8 + x // 4), (y, x, hex(nz))
+        assert q[y, x] != 0 and q[40, 50] == 0 and r[40, 50] == 0

@@ -1191,6 +1191,7 @@ int32_t xg_session_destroy(xg_session *s)
     dmvr_release(s);
     lookahead_release(s);
     bipred_release(s);
+    quant_release(s);
     pipe_release(s);
     if (s->graphExec) (void)hipGraphExecDestroy(s->graphExec);
     if (s->graph) (void)hipGraphDestroy(s->graph);

@@ -50,6 +50,8 @@ void dmvr_release(xg_session *s);
 void lookahead_release(xg_session *s);
 // Releases the bi-prediction distortion plan a session owns (xg_bipred.hip).
 void bipred_release(xg_session *s);
+// Releases the quantiser plan a session owns (xg_quant.hip).
+void quant_release(xg_session *s);
 
 void pipe_release(xg_session *s);
 // Refinement of the session's current blocks, validated and uploaded but not
@@ -60,8 +62,9 @@ void *&session_pipe(xg_session *s);
 
 // Session kernel timers (HIP events on the session stream), by stage:
 // 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search,
-// 4 a whole motion-search pipeline frame.
-constexpr int kTimerStages = 8;
+// 4 a whole motion-search pipeline frame, 5 reconstruction, 6 DMVR,
+// 7 bi-prediction distortion, 8 quantisation.
+constexpr int kTimerStages = 9;
 bool session_timer_begin(xg_session *s, int stage, int ev[2]);   // false when timing is off
 void session_timer_end(xg_session *s, int stage, const int ev[2]);
 
