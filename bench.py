@@ -359,7 +359,22 @@ def residual_loop_leg(lib, steps, warmup, cpu_seconds):
         q_ms, wall_ms = session_time(lib, s, 8, chain, steps, warmup)
         coeff = dco.cpu().numpy()
         got = (dq.cpu().numpy(), dr.cpu().numpy(), dnz.cpu().numpy().view(np.uint64))
+        rec = drec.cpu().numpy()
         nz_share = float((got[2] != 0).mean())
+        # the same loop with the quantiser fused into the forward transform's last stage
+        capi.check(lib.xg_fdct_quant_plan(s, offs.ctypes.data, desc.ctypes.data, dq_desc.ctypes.data, n), "fq plan")
+
+        def fused():
+            capi.check(lib.xg_fdct_quant_run(s, dres.data_ptr(), dq.data_ptr(), dr.data_ptr(), dnz.data_ptr()), "fq")
+            capi.check(lib.xg_recon_run(s, dr.data_ptr(), dpred.data_ptr(), drec.data_ptr()), "recon run")
+
+        dq.zero_()
+        dr.zero_()
+        dnz.zero_()
+        fq_ms, fwall_ms = session_time(lib, s, 2, fused, steps, warmup)
+        fused_same = bool(np.array_equal(dq.cpu().numpy(), got[0]) and np.array_equal(dr.cpu().numpy(), got[1])
+                          and np.array_equal(dnz.cpu().numpy().view(np.uint64), got[2])
+                          and np.array_equal(drec.cpu().numpy(), rec))
     finally:
         lib.xg_session_destroy(s)
     k_s = q_ms / 1e3
@@ -371,6 +386,9 @@ def residual_loop_leg(lib, steps, warmup, cpu_seconds):
            "samples_per_s": round(samples / (wall_ms / 1e3), 1), "ms_per_batch": round(wall_ms, 4),
            "kernels": "k_fdct_mfma + k_quant + k_recon_mfma", "quant_kernel_ms": round(q_ms, 4),
            "tus_with_nonzero_levels": round(nz_share, 4),
+           "fused": {"kernels": "k_fdct_mfma<QUANT> (forward transform + quantiser) + k_recon_mfma",
+                     "value": round(n / (fwall_ms / 1e3), 1), "unit": "blocks/s", "ms_per_batch": round(fwall_ms, 4),
+                     "fdct_quant_kernel_ms": round(fq_ms, 4), "same_outputs_as_three_kernels": fused_same},
            "quant_roofline": {"bound": "hbm", "achieved": round(alg_bytes / k_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
                               "traffic": traffic, "traffic_source": src, "alg_bytes_per_launch": alg_bytes}}

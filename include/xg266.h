@@ -701,6 +701,21 @@ int32_t xg_recon_run(xg_session *s, const int16_t *coeff, const xg_pixel *pred, 
 int32_t xg_quant_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t nblocks);
 int32_t xg_quant_run(xg_session *s, const xg_coeff *tCoeff, int16_t *qCoeff, int16_t *rCoeff, uint64_t *nzMap);
 
+/* Forward transform with the quantiser fused into its last stage: the
+ * Xin266SubFDctWxH -> pfXinQuantInvQuant sequence of Xin266Transform
+ * (h266_encoder/source/h266_trans_quant.c:146-222) for a batch of TUs, with
+ * no int32 coefficient buffer in between.  xg_fdct_quant_plan takes the
+ * arguments of xg_fdct_plan plus qdesc[i], block i's xg_quant_plan descriptor:
+ * its sides and mtsIdx must repeat desc[i]'s and its coefficient groups must
+ * be 4x4 (sides >= 4).  xg_fdct_quant_run writes exactly what xg_fdct_run
+ * followed by xg_quant_run writes to qCoeff, rCoeff and nzMap (device
+ * pointers, 8-byte aligned); it counts as the forward-transform stage (2) of
+ * the session timers.  A plan made by xg_fdct_plan serves xg_fdct_run only;
+ * one made by xg_fdct_quant_plan serves both. */
+int32_t xg_fdct_quant_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, const uint32_t *qdesc,
+                           int32_t nblocks);
+int32_t xg_fdct_quant_run(xg_session *s, const int16_t *resid, int16_t *qCoeff, int16_t *rCoeff, uint64_t *nzMap);
+
 /* Batched DMVR (the refinement of Xg266DeriveMvFromDmvr for many blocks).
  * xg_dmvr_plan takes a HOST array of blocks and writes subOffset[0..n]
  * (optional): block i's subblock deltas are mvd[2*subOffset[i] ..

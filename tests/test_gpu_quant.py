@@ -179,3 +179,72 @@ def test_quant_rejects_bad_inputs(lib):
         assert lib.xg_quant_run(s, t.data_ptr(), q.data_ptr(), q.data_ptr(), nz.data_ptr()) == 0
     finally:
         lib.xg_session_destroy(s)
+
+
+def test_fused_fdct_quant_matches_the_two_kernels(lib, oracle):
+    """xg_fdct_quant_run writes what xg_fdct_run + xg_quant_run write: every
+    shape 4..64 x 4..64 of the transform workload, per-block QPs, RDOQ and
+    dead-zone rounding, I and P frames; then the same plan still serves the
+    plain forward transform."""
+    import torch
+    desc_t, offs, resid = wl.residual_blocks(n=6000)
+    n = len(desc_t)
+    rng = np.random.default_rng(83)
+    qp = rng.integers(0, 64, n)
+    qp[: n // 2] = rng.integers(22, 40, n // 2)               # half at QPs with realistic sparsity
+    desc_q = np.array([(int(d) & 0xFFF) | 2 << 12 | 2 << 15 | int(q) << 18 | int(rd) << 24 | int(ia) << 25
+                       for d, q, rd, ia in zip(desc_t, qp, rng.integers(0, 2, n), rng.integers(0, 2, n))], np.uint32)
+    coeff = np.zeros(resid.size, np.int32)
+    oracle.or_FDctBatch(ptr(resid), ptr(offs), ptr(desc_t), 0, n, ptr(coeff))
+    eq, er, enz = _oracle(oracle, desc_q, offs, coeff)
+    inside = np.zeros(resid.size, bool)
+    for d, o in zip(desc_t.tolist(), offs.tolist()):
+        inside[o:o + (1 << (d & 15)) * (1 << ((d >> 4) & 15))] = True
+    s = _session(lib)
+    try:
+        dres = torch.from_numpy(resid).cuda()
+        dq = torch.full((resid.size,), 0x5A5A, dtype=torch.int16, device="cuda")
+        dr = torch.full((resid.size,), 0x5A5A, dtype=torch.int16, device="cuda")
+        dnz = torch.zeros(n, dtype=torch.int64, device="cuda")
+        assert lib.xg_fdct_quant_plan(s, ptr(offs), ptr(desc_t), ptr(desc_q), n) == 0
+        for _ in range(2):                                      # a second run over the same outputs
+            assert lib.xg_fdct_quant_run(s, dres.data_ptr(), dq.data_ptr(), dr.data_ptr(), dnz.data_ptr()) == 0
+            assert lib.xg_session_sync(s) == 0
+            gq, gr = dq.cpu().numpy(), dr.cpu().numpy()
+            bad = np.nonzero(((gq != eq) | (gr != er)) & inside)[0]
+            assert bad.size == 0, f"first mismatch at coefficient {bad[0]}"
+            assert np.array_equal(dnz.cpu().numpy().view(np.uint64), enz)
+            assert (gq[~inside] == 0x5A5A).all()                # nothing written between the TUs
+        assert (enz != 0).mean() > 0.3 and (enz == 0).any()
+        dco = torch.empty(resid.size, dtype=torch.int32, device="cuda")
+        assert lib.xg_fdct_run(s, dres.data_ptr(), dco.data_ptr()) == 0
+        assert lib.xg_session_sync(s) == 0
+        assert np.array_equal(dco.cpu().numpy()[inside], coeff[inside])
+    finally:
+        lib.xg_session_destroy(s)
+
+
+def test_fused_fdct_quant_rejects_bad_inputs(lib):
+    import torch
+    s = _session(lib)
+    try:
+        t = torch.zeros(4096, dtype=torch.int16, device="cuda")
+        nz = torch.zeros(4, dtype=torch.int64, device="cuda")
+        offs = np.zeros(1, np.int64)
+        d = np.array([4 | 4 << 4], np.uint32)
+        assert lib.xg_fdct_plan(s, ptr(offs), ptr(d), 1) == 0     # a plain plan does not quantise
+        assert lib.xg_fdct_quant_run(s, t.data_ptr(), t.data_ptr(), t.data_ptr(), nz.data_ptr()) == capi.XG_ERR_STATE
+        lib.xg_clear_error()
+        for dd, qd in ((4 | 4 << 4, 5 | 4 << 4 | 2 << 12 | 2 << 15),     # sides differ
+                       (1 | 4 << 4, 1 | 4 << 4 | 1 << 12 | 2 << 15),     # 2-wide: no 4x4 groups
+                       (4 | 4 << 4, 4 | 4 << 4 | 1 << 12 | 3 << 15),     # 2x8 groups
+                       (4 | 4 << 4, 4 | 4 << 4 | 2 << 12 | 2 << 15 | 1 << 27)):
+            assert lib.xg_fdct_quant_plan(s, ptr(offs), ptr(np.array([dd], np.uint32)),
+                                          ptr(np.array([qd], np.uint32)), 1) == capi.XG_ERR_ARG, (hex(dd), hex(qd))
+            lib.xg_clear_error()
+        assert lib.xg_fdct_quant_run(s, t.data_ptr(), t.data_ptr(), t.data_ptr(), nz.data_ptr()) == capi.XG_ERR_STATE
+        lib.xg_clear_error()
+        assert lib.xg_fdct_quant_plan(s, None, None, None, 0) == 0
+        assert lib.xg_fdct_quant_run(s, t.data_ptr(), t.data_ptr(), t.data_ptr(), nz.data_ptr()) == 0
+    finally:
+        lib.xg_session_destroy(s)

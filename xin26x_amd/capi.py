@@ -192,6 +192,8 @@ PROTOS = {
     "xg_recon_run": (I32, [P, P, P, P]),
     "xg_quant_plan": (I32, [P, P, P, I32]),
     "xg_quant_run": (I32, [P, P, P, P, P]),
+    "xg_fdct_quant_plan": (I32, [P, P, P, P, I32]),
+    "xg_fdct_quant_run": (I32, [P, P, P, P, P]),
     "xg_lookahead_me": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, U32, P, P, P]),
     "xg_bipred_plan": (I32, [P, P, I32]),
     "xg_bipred_run": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, P]),

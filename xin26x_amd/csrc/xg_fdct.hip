@@ -28,6 +28,7 @@
 
 #include "xg_batch.hpp"
 #include "xg_device.hpp"
+#include "xg_quant.hpp"
 #include "xg_runtime.hpp"
 #include "xg_trmatrix.hpp"
 
@@ -136,6 +137,10 @@ inline int group_cap(int W, int H, int Wk, int stage, int plane)
 struct FdctArgs {
     const int16_t *resid;
     int32_t *coeff;
+    int16_t *q, *r;                // fused quantiser: levels and dequantised values (int16)
+    unsigned long long *nz;        // fused quantiser: group map per TU
+    const uint32_t *qd;            // fused quantiser: descriptor of each planned block (plan order)
+    const int32_t *tu;             // fused quantiser: caller's index of each planned block
     int n;
     const int8_t *mats;
     int matOff[3][7];
@@ -149,16 +154,16 @@ struct FdctArgs {
 // whose offsets are only 8-byte aligned).  The zero rows Hk..H-1 of a block are
 // one contiguous run; the columns right of Wk exist only for widths 32 and 64,
 // where W - Wk >= 16.  All counts are powers of two.
-template <typename V>
-__device__ __forceinline__ void zero_out(int32_t *Og, int myRel, int G_n, int W, int H, int Wk, int Hk, int lane)
+template <typename V, typename E>
+__device__ __forceinline__ void zero_out(E *Og, int myRel, int G_n, int W, int H, int Wk, int Hk, int lane)
 {
-    constexpr int g = sizeof(V) / 4;
+    constexpr int g = sizeof(V) / sizeof(E);
     const int tail = (W * (H - Hk)) / g, side = (W - Wk) / g;
     const int lgSide = side ? 31 - __builtin_clz(side) : 0;
     V z;
     memset(&z, 0, sizeof(z));
     for (int b = 0; b < G_n; b++) {
-        int32_t *O = Og + __builtin_amdgcn_readlane(myRel, b);
+        E *O = Og + __builtin_amdgcn_readlane(myRel, b);
         for (int e = lane; e < tail; e += 64) *reinterpret_cast<V *>(O + Hk * W + g * e) = z;
         for (int e = lane; e < (Hk << lgSide) && side; e += 64) {
             const int j = e >> lgSide;
@@ -188,7 +193,11 @@ __device__ __forceinline__ void limbs8(uint4 v, long &blo, long &bhi)
 // basis in B, so D comes out transposed: stage 1 leaves four consecutive rows
 // y of one column k per lane (packed straight into the limb planes), stage 2
 // four consecutive columns k of one output row j (one 16-byte store).
-template <int HMAX>
+// QUANT: the quantiser of each block is applied to those four coefficients
+// in registers (they are one row of a 4x4 coefficient group, whose other rows
+// are the lanes of the same quad), and the levels and dequantised values are
+// stored instead of the coefficients; the group maps collect in LDS.
+template <int HMAX, bool QUANT>
 __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *__restrict__ groups,
                                                    const int32_t *__restrict__ rel)
 {
@@ -198,12 +207,18 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
     __shared__ uint64_t limb[3 * PL / 8];
     const int16_t *S = reinterpret_cast<const int16_t *>(stg);
     uint8_t *P0 = reinterpret_cast<uint8_t *>(limb), *P1 = P0 + PL, *P2 = P1 + PL;
+    __shared__ unsigned long long nzl[QUANT ? kGroupMax : 1];
+    __shared__ QParams qpar[QUANT ? kGroupMax : 1];      // quantiser of each block of the group
     const int lane = threadIdx.x;
     const int lr = lane & 15, lq = lane >> 4;
     for (int gi = blockIdx.x; gi < a.n; gi += gridDim.x) {
         const FdctGroup G = groups[gi];
         const int G_n = G.count;
         const int myRel = rel[G.first + min(lane, G_n - 1)];
+        if constexpr (QUANT) {
+            qpar[lane] = quant_params(a.qd[G.first + min(lane, G_n - 1)]);
+            nzl[lane] = 0;
+        }
         const uint32_t desc = G.desc;
         const int lgW = desc & 15, lgH = (desc >> 4) & 15;
         const int hor = (desc >> 8) & 15, ver = (desc >> 12) & 15;
@@ -418,6 +433,27 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[r] = (d0[r] + (d1[r] << 7) + (d2[r] << 14)) >> s2;
                 const int j = dH ? lr & (H - 1) : 16 * mi + lr;
+                if constexpr (QUANT) {
+                    // Wk >= 4 and H >= 4 here (4x4 groups): v[] is one row of a group
+                    const QParams qp = qpar[obk < G_n ? obk : 0];
+                    int qv[4], rv[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) qv[r] = qiq(v[r], qp, rv[r]);
+                    const bool live = j < Hk && obk < G_n;
+                    if (live) {
+                        const int o = orel + j * W + ok0;
+                        *reinterpret_cast<uint2 *>(a.q + G.base + o) = make_uint2(pack16(qv[0], qv[1]),
+                                                                                  pack16(qv[2], qv[3]));
+                        *reinterpret_cast<uint2 *>(a.r + G.base + o) = make_uint2(pack16(rv[0], rv[1]),
+                                                                                  pack16(rv[2], rv[3]));
+                    }
+                    int f = live && (qv[0] | qv[1] | qv[2] | qv[3]) != 0;
+                    f |= shfl_xor<1>(f);                    // the quad holds the group's four rows
+                    f |= shfl_xor<2>(f);
+                    if (live && f && (lr & 3) == 0)
+                        atomicOr(&nzl[obk], 1ull << ((j >> 2) * (min(W, 32) >> 2) + (ok0 >> 2)));
+                    continue;
+                }
                 if (j < Hk && obk < G_n) {
                     if (Wk >= 4) {
                         *reinterpret_cast<int4 *>(Og + orel + j * W + ok0) = make_int4(v[0], v[1], v[2], v[3]);
@@ -430,7 +466,15 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
             }
         }
         // zero-out region: rows Hk..H-1 whole, then columns Wk..W-1 of the kept rows
-        if (Wk < W || Hk < H) {
+        if constexpr (QUANT) {
+            // levels and dequantised values of zero coefficients are zero
+            if (Wk < W || Hk < H) {
+                zero_out<uint2>(a.q + G.base, myRel, G_n, W, H, Wk, Hk, lane);
+                zero_out<uint2>(a.r + G.base, myRel, G_n, W, H, Wk, Hk, lane);
+            }
+            __syncthreads();
+            if (lane < G_n) a.nz[a.tu[G.first + lane]] = nzl[lane];
+        } else if (Wk < W || Hk < H) {
             if (W >= 4)
                 zero_out<int4>(Og, myRel, G_n, W, H, Wk, Hk, lane);
             else
@@ -451,9 +495,11 @@ namespace {
 struct FdctPlan {
     FdctGroup *dGroups = nullptr;     // groups of blocks up to 32 high first, then the 64-high ones
     int32_t *dRel = nullptr;          // block offsets relative to their group's base
+    uint32_t *dQd = nullptr;          // fused quantiser: block descriptors in plan order
+    int32_t *dTu = nullptr;           // fused quantiser: caller's index of each block in plan order
     int n = 0, nLow = 0, cap = 0;     // groups
-    int relCap = 0;
-    bool valid = false;
+    int relCap = 0, qdCap = 0, tuCap = 0;
+    bool valid = false, quant = false;
 };
 std::mutex g_plan_mu;
 std::vector<std::pair<xg_session *, FdctPlan>> g_plans;
@@ -477,12 +523,13 @@ int32_t plan_fail(xg_session *s)
 
 }  // namespace
 
-extern "C" {
-
-int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t n)
+// The forward-transform plan; with qdesc, also the quantiser descriptors of a
+// fused quantising run (4x4 coefficient groups, sides matching the transform's).
+static int32_t fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, const uint32_t *qdesc,
+                         int32_t n, const char *fn)
 {
     if (!s || n < 0 || (n > 0 && (!offset || !desc))) {
-        set_error(XG_ERR_ARG, "xg_fdct_plan: bad arguments");
+        set_error(XG_ERR_ARG, "%s: bad arguments", fn);
         return s ? plan_fail(s) : XG_ERR_ARG;
     }
     if (!mats_init()) return XG_ERR_HIP;
@@ -491,25 +538,36 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
     struct Item {
         long long off;
         uint32_t desc;
+        int32_t idx;
     };
     std::vector<Item> items((size_t)n);
     for (int i = 0; i < n; i++) {
         const int lgW = desc[i] & 15, lgH = (desc[i] >> 4) & 15, m = (desc[i] >> 8) & 15;
         if (lgW < 1 || lgW > 6 || lgH < 1 || lgH > 6 || m > 5 || m == 1) {
-            set_error(XG_ERR_ARG, "xg_fdct_plan: block %d has descriptor 0x%x", i, desc[i]);
+            set_error(XG_ERR_ARG, "%s: block %d has descriptor 0x%x", fn, i, desc[i]);
             return plan_fail(s);
+        }
+        if (qdesc) {
+            const uint32_t q = qdesc[i];
+            if ((q & 0xFFF) != (desc[i] & 0xFFF) || lgW < 2 || lgH < 2 || ((q >> 12) & 7) != 2 ||
+                ((q >> 15) & 7) != 2 || (q >> 26)) {
+                set_error(XG_ERR_ARG, "%s: block %d: quantiser descriptor 0x%x must repeat the sides and mtsIdx "
+                                      "of 0x%x, with 4x4 coefficient groups (sides >= 4)", fn, i, q, desc[i]);
+                return plan_fail(s);
+            }
         }
         const int hor = mts[m][0], ver = mts[m][1];
         if ((hor != kDCT2 && (lgW < 2 || lgW > 5)) || (ver != kDCT2 && (lgH < 2 || lgH > 5)) || offset[i] < 0) {
-            set_error(XG_ERR_ARG, "xg_fdct_plan: block %d: DST-VII/DCT-VIII need sides 4..32", i);
+            set_error(XG_ERR_ARG, "%s: block %d: DST-VII/DCT-VIII need sides 4..32", fn, i);
             return plan_fail(s);
         }
         if (offset[i] % std::min(8, 1 << lgW)) {
-            set_error(XG_ERR_ARG, "xg_fdct_plan: block %d: offset must be a multiple of min(8, width) samples", i);
+            set_error(XG_ERR_ARG, "%s: block %d: offset must be a multiple of min(8, width) samples", fn, i);
             return plan_fail(s);
         }
         items[i].off = offset[i];
         items[i].desc = (uint32_t)(lgW | (lgH << 4) | (hor << 8) | (ver << 12));
+        items[i].idx = i;
     }
     // two launches: blocks up to 32 high (small LDS tile), then 64-high blocks;
     // inside each, descriptors with the largest blocks first, blocks of one
@@ -520,7 +578,8 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
         const int ax = (x.desc & 15) + ((x.desc >> 4) & 15), ay = (y.desc & 15) + ((y.desc >> 4) & 15);
         if (ax != ay) return ax > ay;
         if (x.desc != y.desc) return x.desc < y.desc;
-        return x.off < y.off;
+        if (x.off != y.off) return x.off < y.off;
+        return x.idx < y.idx;
     });
     std::vector<FdctGroup> groups;
     std::vector<int32_t> rel((size_t)n);
@@ -557,11 +616,46 @@ int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc,
         return XG_ERR_HIP;
     if (n && !XG_HIP(hipMemcpyAsync(p->dRel, rel.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st)))
         return XG_ERR_HIP;
+    std::vector<uint32_t> qd;
+    std::vector<int32_t> tu;
+    if (qdesc) {
+        qd.resize((size_t)n);
+        tu.resize((size_t)n);
+        for (int i = 0; i < n; i++) {
+            qd[(size_t)i] = qdesc[items[i].idx];
+            tu[(size_t)i] = items[i].idx;
+        }
+        if (!grow(p->dQd, p->qdCap, std::max(n, 1), st) || !grow(p->dTu, p->tuCap, std::max(n, 1), st))
+            return XG_ERR_NOMEM;
+        if (n && (!XG_HIP(hipMemcpyAsync(p->dQd, qd.data(), sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice,
+                                         st)) ||
+                  !XG_HIP(hipMemcpyAsync(p->dTu, tu.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st))))
+            return XG_ERR_HIP;
+    }
     if (!XG_HIP(hipStreamSynchronize(st))) return XG_ERR_HIP;
     p->n = ng;
     p->nLow = nLow;
+    p->quant = qdesc != nullptr;
     p->valid = true;
     return XG_OK;
+}
+
+extern "C" {
+
+int32_t xg_fdct_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, int32_t n)
+{
+    return fdct_plan(s, offset, desc, nullptr, n, "xg_fdct_plan");
+}
+
+int32_t xg_fdct_quant_plan(xg_session *s, const int64_t *offset, const uint32_t *desc, const uint32_t *qdesc,
+                           int32_t n)
+{
+    if (n > 0 && !qdesc) {
+        set_error(XG_ERR_ARG, "xg_fdct_quant_plan: bad arguments");
+        return s ? plan_fail(s) : XG_ERR_ARG;
+    }
+    static const uint32_t kNone = 0;     // an empty plan is still a quantising one
+    return fdct_plan(s, offset, desc, qdesc ? qdesc : &kNone, n, "xg_fdct_quant_plan");
 }
 
 int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
@@ -581,6 +675,7 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
     }
     if (p->n == 0) return XG_OK;
     FdctArgs a;
+    memset(&a, 0, sizeof(a));
     a.resid = resid;
     a.coeff = coeff;
     a.mats = g_mats;
@@ -588,20 +683,68 @@ int32_t xg_fdct_run(xg_session *s, const int16_t *resid, xg_coeff *coeff)
     memcpy(a.matCols, g_mi.cols, sizeof(a.matCols));
     memcpy(a.diagOff, g_mi.diag, sizeof(a.diagOff));
     // one resident wave per workgroup slot the chip holds (registers and LDS bound it)
-    static const int slots[2] = {resident_slots((const void *)k_fdct_mfma<32>),
-                                 resident_slots((const void *)k_fdct_mfma<64>)};
+    static const int slots[2] = {resident_slots((const void *)k_fdct_mfma<32, false>),
+                                 resident_slots((const void *)k_fdct_mfma<64, false>)};
     hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
     const bool timed = session_timer_begin(s, 2, ev);
     if (p->nLow > 0) {
         FdctArgs lo = a;
         lo.n = p->nLow;
-        k_fdct_mfma<32><<<std::min(lo.n, slots[0]), 64, 0, st>>>(lo, p->dGroups, p->dRel);
+        k_fdct_mfma<32, false><<<std::min(lo.n, slots[0]), 64, 0, st>>>(lo, p->dGroups, p->dRel);
     }
     if (p->n > p->nLow) {
         FdctArgs hi = a;
         hi.n = p->n - p->nLow;
-        k_fdct_mfma<64><<<std::min(hi.n, slots[1]), 64, 0, st>>>(hi, p->dGroups + p->nLow, p->dRel);
+        k_fdct_mfma<64, false><<<std::min(hi.n, slots[1]), 64, 0, st>>>(hi, p->dGroups + p->nLow, p->dRel);
+    }
+    if (timed) session_timer_end(s, 2, ev);
+    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
+}
+
+int32_t xg_fdct_quant_run(xg_session *s, const int16_t *resid, int16_t *qCoeff, int16_t *rCoeff, uint64_t *nzMap)
+{
+    if (!s || !resid || !qCoeff || !rCoeff || !nzMap || ((uintptr_t)qCoeff & 7) || ((uintptr_t)rCoeff & 7) ||
+        ((uintptr_t)nzMap & 7)) {
+        set_error(XG_ERR_ARG, "xg_fdct_quant_run: bad arguments (outputs 8-byte aligned)");
+        return XG_ERR_ARG;
+    }
+    FdctPlan *p;
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        p = plan_for(s, false);
+    }
+    if (!p || !p->valid || !p->quant) {
+        set_error(XG_ERR_STATE, "xg_fdct_quant_run: no quantising plan (call xg_fdct_quant_plan first)");
+        return XG_ERR_STATE;
+    }
+    if (p->n == 0) return XG_OK;
+    FdctArgs a;
+    memset(&a, 0, sizeof(a));
+    a.resid = resid;
+    a.q = qCoeff;
+    a.r = rCoeff;
+    a.nz = reinterpret_cast<unsigned long long *>(nzMap);
+    a.qd = p->dQd;
+    a.tu = p->dTu;
+    a.mats = g_mats;
+    memcpy(a.matOff, g_mi.off, sizeof(a.matOff));
+    memcpy(a.matCols, g_mi.cols, sizeof(a.matCols));
+    memcpy(a.diagOff, g_mi.diag, sizeof(a.diagOff));
+    static const int slots[2] = {resident_slots((const void *)k_fdct_mfma<32, true>),
+                                 resident_slots((const void *)k_fdct_mfma<64, true>)};
+    hipStream_t st = (hipStream_t)xg_session_stream(s);
+    int ev[2];
+    const bool timed = session_timer_begin(s, 2, ev);
+    if (p->nLow > 0) {
+        FdctArgs lo = a;
+        lo.n = p->nLow;
+        k_fdct_mfma<32, true><<<std::min(lo.n, slots[0]), 64, 0, st>>>(lo, p->dGroups, p->dRel);
+    }
+    if (p->n > p->nLow) {
+        FdctArgs hi = a;
+        hi.n = p->n - p->nLow;
+        k_fdct_mfma<64, true><<<std::min(hi.n, slots[1]), 64, 0, st>>>(hi, p->dGroups + p->nLow, p->dRel);
     }
     if (timed) session_timer_end(s, 2, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
@@ -616,6 +759,8 @@ void xg::fdct_release(xg_session *s)
         if (g_plans[i].first == s) {
             if (g_plans[i].second.dGroups) (void)hipFree(g_plans[i].second.dGroups);
             if (g_plans[i].second.dRel) (void)hipFree(g_plans[i].second.dRel);
+            if (g_plans[i].second.dQd) (void)hipFree(g_plans[i].second.dQd);
+            if (g_plans[i].second.dTu) (void)hipFree(g_plans[i].second.dTu);
             g_plans.erase(g_plans.begin() + (long)i);
             return;
         }

@@ -27,45 +27,12 @@
 
 #include "xg_batch.hpp"
 #include "xg_device.hpp"
+#include "xg_quant.hpp"
 #include "xg_runtime.hpp"
 
 namespace xg {
 
-__constant__ int c_qScale[2][6] = {{26214, 23302, 20560, 18396, 16384, 14564},
-                                   {18396, 16384, 14564, 13107, 11651, 10280}};
-__constant__ int c_iqScale[2][6] = {{40, 45, 51, 57, 64, 72}, {57, 64, 72, 80, 90, 102}};
-
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
-
-struct QParams {
-    int qMult, qAdd, qShift, iqMult, iqAdd, iqShift;
-};
-
-// Xin266Transform's derivation for one TU (bit depth 10: 15 - 10 = 5).
-__device__ __forceinline__ QParams quant_params(uint32_t d)
-{
-    const int lgW = d & 15, lgH = (d >> 4) & 15, skip = ((d >> 8) & 15) == 1;
-    const int qp = (d >> 18) & 63, rdoq = (d >> 24) & 1, intra = (d >> 25) & 1;
-    const int lgTu = lgW + lgH, adj = (lgTu & 1) && !skip, lgSize = lgTu >> 1;
-    const int per = qp / 6, rem = qp - 6 * per;
-    QParams p;
-    p.qShift = 14 + 5 + per - lgSize - adj;
-    p.iqShift = 6 - 5 + lgSize + adj;
-    p.qMult = c_qScale[adj][rem];
-    p.iqMult = c_iqScale[adj][rem] << per;
-    p.qAdd = (rdoq && !skip) ? 1 << (p.qShift - 1) : (intra ? 171 : 85) << (p.qShift - 9);
-    p.iqAdd = !skip ? 1 << (p.iqShift - 1) : 1 << ((p.iqShift - lgSize) + 5 - 1);
-    return p;
-}
-
-__device__ __forceinline__ int qiq(int t, const QParams &p, int &r)
-{
-    const int a = t < 0 ? -t : t;
-    int q = (int)((unsigned)a * (unsigned)p.qMult + (unsigned)p.qAdd) >> p.qShift;
-    q = min(max(t < 0 ? -q : q, -32768), 32767);
-    r = min(max((int)((unsigned)q * (unsigned)p.iqMult + (unsigned)p.iqAdd) >> p.iqShift, -32768), 32767);
-    return q;
-}
 
 // One CG: cgh rows of cgw coefficients (stride w), V per vector access.
 template <int V>
