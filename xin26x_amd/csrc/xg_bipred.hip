@@ -176,10 +176,10 @@ int32_t xg_bipred_run(xg_session *s, const xg_plane *input, const xg_plane *pred
     }
     if (p->n == 0) return XG_OK;
     hipStream_t st = (hipStream_t)xg_session_stream(s);
-    int ev[2];
-    const bool timed = session_timer_begin(s, 7, ev);
     // the sums start at zero (a kernel, not a memset: the launch may be captured)
     k_zero_u32<<<(p->n + 255) / 256, 256, 0, st>>>(dist, p->n);
+    int ev[2];
+    const bool timed = session_timer_begin(s, 7, ev);      // the distortion kernel alone
     k_bipred_tiles<<<(p->nTiles + 255) / 256, 256, 0, st>>>(BiPic{input->data, input->stride},
                                                             BiPic{pred0->data, pred0->stride},
                                                             BiPic{pred1->data, pred1->stride}, p->dTiles, p->nTiles,
