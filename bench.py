@@ -36,6 +36,36 @@ PMC_REFINE = os.path.join(ROOT, "profiles", "k_refine_1080p.md")
 PMC_FDCT = os.path.join(ROOT, "profiles", "k_fdct_mfma_200k_blocks.md")
 
 
+def valu_roofline(path, kernel_ms):
+    """The bound that binds an integer-ALU kernel: its wave-level VALU
+    instructions per launch (SQ_INSTS_VALU of the committed profile of the same
+    workload) over the live kernel time, against the VALU issue peak."""
+    insts = profiled_counter(path, "SQ_INSTS_VALU")
+    if not insts or kernel_ms <= 0:
+        return None
+    achieved = insts / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "valu", "achieved": round(achieved, 1), "peak": VALU_PEAK_G, "unit": "G wave-instr/s",
+            "frac": round(achieved / VALU_PEAK_G, 4), "insts_per_launch": insts,
+            "insts_source": os.path.relpath(path, ROOT)}
+
+
+def profiled_counter(path, name):
+    """Per-launch value of an SQ counter from a committed rocprofv3 summary's
+    counter table (`| NAME | 1,234 |`), or None."""
+    try:
+        with open(path) as f:
+            for ln in f:
+                cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+                if len(cells) == 2 and cells[0] == name:
+                    return int(cells[1].replace(",", ""))
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+VALU_PEAK_G = 256 * 2.4        # G wave-level VALU instructions/s: 256 CUs x 2.4 GHz x one per CU per cycle
+
+
 def profiled_traffic(path):
     """HBM bytes per launch from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (calibrated against a copy kernel), or (None, None)."""
@@ -1045,10 +1075,11 @@ def main():
         "luma_cand_px_per_s": round(cand_px / t_graph, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
-                     "binding": "VALU issue (integer filters + Hadamard; SQ_INSTS_VALU in the profile gives the "
-                                "utilisation); HBM traffic is ~1.4x the algorithmic bytes, far from its ceiling",
+                     "binding": "VALU issue (integer filters + Hadamard): see valu_roofline; HBM traffic is ~1.4x "
+                                "the algorithmic bytes, far from its ceiling",
                      "kernel": "k_refine_persistent", "kernel_ms": round(ms.value, 4), "launches": cnt.value,
                      "alg_bytes_per_launch": alg_bytes},
+        "valu_roofline": valu_roofline(PMC_REFINE, ms.value),
         "cpu_baseline": cpu,
         "transforms": transforms,
         "reconstruction": reconstruction,
