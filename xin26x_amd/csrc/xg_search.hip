@@ -231,11 +231,15 @@ __device__ __forceinline__ unsigned long long block_min_u64_16(unsigned long lon
 // Window rows needed by the packed core for nx candidates per row.
 __host__ __device__ inline int packed_ws(int nx, int bw) { return (nx + bw + 7) & ~1; }
 
-// Threads for a packed search over an n x n candidate window: every item at once, 64..1024.
+// Threads for a packed search over an n x n candidate window: the items
+// (search row x candidate quad) spread evenly over the fewest passes a
+// workgroup of at most 1024 threads needs, so no pass runs nearly empty
+// (+/-32: 1105 items, two passes of 576 threads rather than 1024 + 81).
 static inline int packed_threads(int n)
 {
     const int items = n * ((n + 3) >> 2);
-    return std::min(1024, std::max(64, (items + 63) & ~63));
+    const int passes = (items + 1023) / 1024;
+    return std::min(1024, std::max(64, ((items + passes - 1) / passes + 63) & ~63));
 }
 
 // Packed frame form for block widths 8, 16 and 32; same cost, window and tie
