@@ -120,12 +120,12 @@ __global__ void __launch_bounds__(256) k_full_search_frame(FsFrameArgs a)
 // ---------------------------------------------------------------------------
 // Packed integer search core (block widths 4..64, powers of two)
 //
-// Each lane costs four horizontally adjacent candidates (4q .. 4q + 3) of one
-// search row dyi: a window row is read as BW/2 + 2 dwords D (sample pairs from
-// an even column), F = D realigned by one sample, and v_sad_u16 takes two
+// Each lane costs K horizontally adjacent candidates (Kq .. Kq + K - 1) of one
+// search row dyi: a window row is read as BW/2 + K/2 dwords D (sample pairs
+// from an even column), F = D realigned by one sample, and v_sad_u16 takes two
 // absolute differences per instruction against the packed current row C.
-// Window rows are ws samples (ws even, >= nx + BW + 6, so the reads of the
-// last quad stay inside the row); the block is bh rows of BW/2 dwords at a
+// Window rows are ws samples (ws even, >= nx + BW + K - 1, so the reads of the
+// last item stay inside the row); the block is bh rows of BW/2 dwords at a
 // 16-byte boundary.  Cost = (SAD << 8) + (EG(4 mx - pmx) + EG(4 my - pmy)) *
 // lambda; key = cost << 32 | raster index, as in full_search_core.
 // ---------------------------------------------------------------------------
@@ -155,23 +155,26 @@ __device__ __forceinline__ void stage_cur(uint32_t *curL, const uint16_t *cp, in
         curL[i] = *reinterpret_cast<const uint32_t *>(cp + (intptr_t)(i / nd) * cs + 2 * (i % nd));
 }
 
-template <int BW>
+template <int BW, int K = 4>
 __device__ __forceinline__ unsigned long long packed_items(const uint32_t *w32, int ws, const uint32_t *curL, int bh,
                                                            int x0, int x1, int y0, int nx, int ny, uint32_t lambda,
                                                            int pmx, int pmy)
 {
     constexpr int ND = BW / 2;                      // dword pairs per block row
-    const int nq = (nx + 3) >> 2, items = ny * nq;
+    constexpr int KD = K / 2;                       // dwords the lane's candidates span beyond the block row
+    const int nq = (nx + K - 1) / K, items = ny * nq;
     unsigned long long best = ~0ull;
     for (int it = threadIdx.x; it < items; it += blockDim.x) {
         const int dyi = it / nq, q = it - dyi * nq;
-        const uint32_t *rp = w32 + ((dyi * ws) >> 1) + 2 * q;
-        uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+        const uint32_t *rp = w32 + ((dyi * ws) >> 1) + KD * q;
+        uint32_t acc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = 0;
 #pragma unroll 2
         for (int i = 0; i < bh; i++) {
-            uint32_t D[ND + 2], F[ND + 1], C[ND];
+            uint32_t D[ND + KD], F[ND + KD - 1], C[ND];
 #pragma unroll
-            for (int j = 0; j < ND + 2; j++) D[j] = rp[i * (ws >> 1) + j];
+            for (int j = 0; j < ND + KD; j++) D[j] = rp[i * (ws >> 1) + j];
             if constexpr (ND >= 4) {
                 const uint4 *c4 = reinterpret_cast<const uint4 *>(curL + i * ND);
 #pragma unroll
@@ -187,24 +190,23 @@ __device__ __forceinline__ unsigned long long packed_items(const uint32_t *w32, 
                 for (int j = 0; j < ND; j++) C[j] = curL[i * ND + j];
             }
 #pragma unroll
-            for (int j = 0; j < ND + 1; j++) F[j] = align16(D[j + 1], D[j], 16);    // pairs from odd samples
+            for (int j = 0; j < ND + KD - 1; j++) F[j] = align16(D[j + 1], D[j], 16);    // pairs from odd samples
 #pragma unroll
-            for (int j = 0; j < ND; j++) {
-                acc0 = __builtin_amdgcn_sad_u16(D[j], C[j], acc0);
-                acc1 = __builtin_amdgcn_sad_u16(F[j], C[j], acc1);
-                acc2 = __builtin_amdgcn_sad_u16(D[j + 1], C[j], acc2);
-                acc3 = __builtin_amdgcn_sad_u16(F[j + 1], C[j], acc3);
-            }
+            for (int j = 0; j < ND; j++)
+#pragma unroll
+                for (int m = 0; m < KD; m++) {
+                    acc[2 * m] = __builtin_amdgcn_sad_u16(D[j + m], C[j], acc[2 * m]);          // candidate K q + 2m
+                    acc[2 * m + 1] = __builtin_amdgcn_sad_u16(F[j + m], C[j], acc[2 * m + 1]);  // K q + 2m + 1
+                }
         }
         const int my = y0 + dyi;
         const uint32_t by = eg_bits(4 * my - pmy);
-        const uint32_t acc[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int mx = x0 + 4 * q + k;
+        for (int k = 0; k < K; k++) {
+            const int mx = x0 + K * q + k;
             if (mx <= x1) {
                 const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * mx - pmx) + by) * lambda;
-                const unsigned long long key = ((unsigned long long)cost << 32) | (uint32_t)(dyi * nx + 4 * q + k);
+                const unsigned long long key = ((unsigned long long)cost << 32) | (uint32_t)(dyi * nx + K * q + k);
                 best = key < best ? key : best;
             }
         }
@@ -229,15 +231,20 @@ __device__ __forceinline__ unsigned long long block_min_u64_16(unsigned long lon
 }
 
 // Window rows needed by the packed core for nx candidates per row.
-__host__ __device__ inline int packed_ws(int nx, int bw) { return (nx + bw + 7) & ~1; }
+__host__ __device__ inline int packed_ws(int nx, int bw, int k = 4) { return (nx + bw + k + 3) & ~1; }
 
-// Threads for a packed search over an n x n candidate window: the items
-// (search row x candidate quad) spread evenly over the fewest passes a
-// workgroup of at most 1024 threads needs, so no pass runs nearly empty
-// (+/-32: 1105 items, two passes of 576 threads rather than 1024 + 81).
-static inline int packed_threads(int n)
+// Candidates per lane of the frame form.  Eight (each window-row read serving
+// twice the SADs) measured slightly slower than four at +/-16 and +/-32.
+constexpr int kFrameK = 4;
+
+// Threads for a packed search over an n x n candidate window, k candidates
+// per lane: the items (search row x run of k candidates) spread evenly over
+// the fewest passes a workgroup of at most 1024 threads needs, so no pass runs
+// nearly empty (+/-32, k = 4: 1105 items, two passes of 576 threads rather
+// than 1024 + 81).
+static inline int packed_threads(int n, int k = 4)
 {
-    const int items = n * ((n + 3) >> 2);
+    const int items = n * ((n + k - 1) / k);
     const int passes = (items + 1023) / 1024;
     return std::min(1024, std::max(64, ((items + passes - 1) / passes + 63) & ~63));
 }
@@ -262,7 +269,7 @@ __global__ void __launch_bounds__(1024) k_full_search_frame_p(FsFrameArgs a, int
     stage_rows(win, ws, a.ref + (intptr_t)(y + y0) * a.rs + (x + x0), a.rs, rows, nx + BW - 1);
     stage_cur(curL, a.cur + (intptr_t)y * a.cs + x, a.cs, BW, a.bh);
     __syncthreads();
-    unsigned long long best = packed_items<BW>(smem32, ws, curL, a.bh, x0, x1, y0, nx, ny, a.lambda, 0, 0);
+    unsigned long long best = packed_items<BW, kFrameK>(smem32, ws, curL, a.bh, x0, x1, y0, nx, ny, a.lambda, 0, 0);
     best = block_min_u64_16(best, red);
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
@@ -643,7 +650,7 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
         return XG_ERR_ARG;
     }
     // packed form: widths 8, 16, 32 with a dword-aligned current plane
-    const int nxm = 2 * range + 1, wsP = packed_ws(nxm, bw);
+    const int nxm = 2 * range + 1, wsP = packed_ws(nxm, bw, kFrameK);
     const size_t ldsP = ((size_t)(2 * range + bh) * wsP + (size_t)bw * bh) * 2 + 16;
     const bool packed = (bw == 8 || bw == 16 || bw == 32) && ((uintptr_t)cur->data & 3) == 0 && (cur->stride & 1) == 0 &&
                         ldsP <= 64 * 1024;
@@ -657,7 +664,7 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
     int ev[2];
     const bool timed = session_timer_begin(s, 3, ev);
     if (packed) {
-        const int threads = packed_threads(nxm);
+        const int threads = packed_threads(nxm, kFrameK);
         if (bw == 8)
             k_full_search_frame_p<8><<<nbx * nby, threads, ldsP, st>>>(a, wsP);
         else if (bw == 16)
