@@ -207,18 +207,14 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
     __shared__ uint64_t limb[3 * PL / 8];
     const int16_t *S = reinterpret_cast<const int16_t *>(stg);
     uint8_t *P0 = reinterpret_cast<uint8_t *>(limb), *P1 = P0 + PL, *P2 = P1 + PL;
-    __shared__ unsigned long long nzl[QUANT ? kGroupMax : 1];
-    __shared__ QParams qpar[QUANT ? kGroupMax : 1];      // quantiser of each block of the group
     const int lane = threadIdx.x;
     const int lr = lane & 15, lq = lane >> 4;
     for (int gi = blockIdx.x; gi < a.n; gi += gridDim.x) {
         const FdctGroup G = groups[gi];
         const int G_n = G.count;
         const int myRel = rel[G.first + min(lane, G_n - 1)];
-        if constexpr (QUANT) {
-            qpar[lane] = quant_params(a.qd[G.first + min(lane, G_n - 1)]);
-            nzl[lane] = 0;
-        }
+        uint32_t myQd = 0;                                   // quantiser descriptor of block `lane`
+        if constexpr (QUANT) myQd = a.qd[G.first + min(lane, G_n - 1)];
         const uint32_t desc = G.desc;
         const int lgW = desc & 15, lgH = (desc >> 4) & 15;
         const int hor = (desc >> 8) & 15, ver = (desc >> 12) & 15;
@@ -356,6 +352,14 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
         }
         __syncthreads();
 
+        // fused quantiser: the group maps live in the residual stage, which
+        // stage 1 has finished reading
+        unsigned long long *nzl = reinterpret_cast<unsigned long long *>(stg);
+        if constexpr (QUANT) {
+            nzl[lane] = 0;
+            __syncthreads();
+        }
+
         // ---- stage 2: M = columns (b, k) of the group, N = j < Hk, K = y < H ----
         const int cols2 = G_n * Wk;
         const int rnd2 = 1 << (s2 - 1);
@@ -435,10 +439,10 @@ __global__ void __launch_bounds__(64) k_fdct_mfma(FdctArgs a, const FdctGroup *_
                 const int j = dH ? lr & (H - 1) : 16 * mi + lr;
                 if constexpr (QUANT) {
                     // Wk >= 4 and H >= 4 here (4x4 groups): v[] is one row of a group
-                    const QParams qp = qpar[obk < G_n ? obk : 0];
+                    const QParams qp = quant_params((uint32_t)__shfl((int)myQd, obk < G_n ? obk : 0));
                     int qv[4], rv[4];
 #pragma unroll
-                    for (int r = 0; r < 4; r++) qv[r] = qiq(v[r], qp, rv[r]);
+                    for (int r = 0; r < 4; r++) qv[r] = qiq24(v[r], qp, rv[r]);
                     const bool live = j < Hk && obk < G_n;
                     if (live) {
                         const int o = orel + j * W + ok0;

@@ -66,4 +66,16 @@ __device__ __forceinline__ int qiq(int t, const QParams &p, int &r)
     return q;
 }
 
+// qiq for |t| < 2^24 (forward-transform outputs: below 2^22 for any int16
+// residual), with the full-rate 24-bit multiply: its low 32 bits are those of
+// the wrapping int32 product, so the result is qiq's.
+__device__ __forceinline__ int qiq24(int t, const QParams &p, int &r)
+{
+    const unsigned a = (unsigned)(t < 0 ? -t : t);
+    int q = (int)(__umul24(a, (unsigned)p.qMult) + (unsigned)p.qAdd) >> p.qShift;
+    q = min(max(t < 0 ? -q : q, -32768), 32767);
+    r = min(max((int)((unsigned)q * (unsigned)p.iqMult + (unsigned)p.iqAdd) >> p.iqShift, -32768), 32767);
+    return q;
+}
+
 }  // namespace xg
