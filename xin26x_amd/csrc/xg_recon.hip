@@ -175,7 +175,11 @@ __device__ __forceinline__ void recon_skip(const RecArgs &a, const RecGroup &G, 
 // group issues its coefficient loads (rows below the zero-out only) and the
 // basis fragments before its first wait; each stage's tiles are shared by the
 // four waves.
-__global__ void __launch_bounds__(kRecThreads) k_recon_mfma(RecArgs a, const RecGroup *__restrict__ groups,
+// Six waves per SIMD: without the hint the compiler keeps MFMA accumulators in
+// AGPRs and the unified register budget (75 + 8) allows five; asked for six it
+// fits in 79 VGPRs with no spills (0.0784 -> 0.0720 ms on the 200k-TU
+// workload; seven measured the same as six).
+__global__ void __launch_bounds__(kRecThreads) __attribute__((amdgpu_waves_per_eu(6))) k_recon_mfma(RecArgs a, const RecGroup *__restrict__ groups,
                                                              const int32_t *__restrict__ rel)
 {
     constexpr int NC = kRecStage / (8 * kRecThreads);     // 16-byte chunks per thread
