@@ -884,8 +884,8 @@ def main():
                     help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "The path would shard by streams or CTU-row bands; see DESIGN.md.", file=sys.stderr)
+        print(f"bench.py runs on one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}). Across GPUs "
+              "the work would shard by streams, or by CTU-row bands of one stream.", file=sys.stderr)
         sys.exit(2)
 
     import torch
