@@ -609,6 +609,26 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
 int32_t xg_lookahead_me(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t unitSize,
                         int32_t range, uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad);
 
+/* xg_lookahead_me without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, checks and outputs as xg_lookahead_me, but no session: the
+ * kernels are enqueued on `stream` (a hipStream_t; NULL is the null stream)
+ * and both half-resolution pictures live in a static device array of
+ * XG_LOOKAHEAD_STATIC_MAX_BYTES that every device the library runs on holds
+ * until the process ends.  The two pictures need
+ * 2 * (align128(even(w/2) * (h/2)) + (w/2 + pad) * (h/2 + pad)) bytes for a
+ * w x h current plane and a reference padded by `pad`; a 3840x2160 pair with
+ * the +/-64 minimum padding of 136 needs 9.1 MB.  Larger inputs return
+ * XG_ERR_ARG before anything is enqueued.  The call allocates nothing and
+ * makes no synchronous HIP call; the scratch is fully rewritten before it is
+ * read, so no state carries over between calls.  Calls whose work overlaps on
+ * one device share that device's array and race: serialise them (one stream,
+ * or events). */
+#define XG_LOOKAHEAD_STATIC_MAX_BYTES (12u << 20)
+int32_t xg_lookahead_me_static(void *stream, const xg_plane *cur, const xg_plane *ref, int32_t unitSize,
+                               int32_t range, uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad);
+
 /* Full motion-search pipeline of one frame with two reference pictures, on a
  * raster grid of 16x16 blocks (frame sides multiples of 16): per block and
  * reference, integer full search +/-range around (0,0) (as Xg26xMotionEstFull,

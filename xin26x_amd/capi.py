@@ -17,6 +17,7 @@ U32 = ctypes.c_uint32
 U8 = ctypes.c_uint8
 
 XG_MAX_CAND = 64
+XG_LOOKAHEAD_STATIC_MAX_BYTES = 12 << 20      # include/xg266.h
 XG_OK, XG_ERR_ARG, XG_ERR_HIP, XG_ERR_NOMEM, XG_ERR_BUDGET, XG_ERR_STATE = range(6)
 MODE_PLAIN, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
 
@@ -195,6 +196,7 @@ PROTOS = {
     "xg_fdct_quant_plan": (I32, [P, P, P, P, I32]),
     "xg_fdct_quant_run": (I32, [P, P, P, P, P]),
     "xg_lookahead_me": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, U32, P, P, P]),
+    "xg_lookahead_me_static": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, U32, P, P, P]),
     "xg_bipred_plan": (I32, [P, P, I32]),
     "xg_bipred_run": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, P]),
     "xg_dmvr_plan": (I32, [P, P, I32, P]),

@@ -65,6 +65,80 @@ bool ensure(uint16_t *&p, size_t &cap, size_t n, hipStream_t st)
     cap = n;
     return true;
 }
+
+// Enqueues the two downscales into the half-resolution planes `hc` (current) and
+// `hr` (reference, hp samples of padding) and the unit search on `st`.
+int32_t lookahead_launch(hipStream_t st, xg_session *timer, const xg_plane *cur, const xg_plane *ref,
+                         const xg_plane &hc, const xg_plane &hr, int32_t unitSize, int32_t range, uint32_t sadLambda,
+                         int16_t *laMv, uint32_t *cost, uint32_t *sad)
+{
+    const int hp = hr.pad, rw = hr.stride, rh = hr.height + 2 * hp;
+    k_down2x2<<<dim3((hc.width + 127) / 128, hc.height), 128, 0, st>>>(cur->data, cur->stride, hc.data, hc.stride,
+                                                                       hc.width, hc.height);
+    k_down2x2<<<dim3((rw + 127) / 128, rh), 128, 0, st>>>(ref->data - (intptr_t)ref->pad * ref->stride - ref->pad,
+                                                           ref->stride, hr.data - (intptr_t)hp * rw - hp, rw, rw, rh);
+    if (!XG_HIP(hipGetLastError())) return XG_ERR_HIP;
+    // window of a CU with ctuSize = range: MVs up to range + 8 samples past the picture
+    // edge, so the search keeps its full +/-range everywhere
+    return full_search_frame_on(st, timer, &hc, &hr, unitSize, unitSize, range, sadLambda, range, laMv, cost, sad);
+}
+
+bool lookahead_args_ok(const char *fn, const xg_plane *cur, const xg_plane *ref, int32_t unitSize, int32_t range,
+                       const int16_t *laMv, const uint32_t *cost, const uint32_t *sad)
+{
+    if (!cur || !ref || !cur->data || !ref->data || !laMv || !cost || !sad || (unitSize != 8 && unitSize != 16) ||
+        range < 0 || range > 64 || cur->width != ref->width || cur->height != ref->height || cur->width < 2 ||
+        cur->height < 2 || (ref->pad & 1) || ref->pad / 2 < range + 4) {
+        set_error(XG_ERR_ARG, "%s: bad arguments (units 8 or 16, range <= 64, reference padding even "
+                              "and >= 2 * (range + 4))", fn);
+        return false;
+    }
+    return true;
+}
+
+// Samples of the half-resolution current plane (even stride) and reference
+// plane (with its padding) for a cur->width x cur->height picture.
+void half_sizes(const xg_plane *cur, const xg_plane *ref, size_t *curN, size_t *refN)
+{
+    const int hw = cur->width / 2, hh = cur->height / 2, hp = ref->pad / 2;
+    *curN = (size_t)((hw + 1) & ~1) * hh;
+    *refN = (size_t)(hw + 2 * hp) * (hh + 2 * hp);
+}
+
+}  // namespace
+
+// ---- the no-allocation form ------------------------------------------------
+// Both half-resolution planes of xg_lookahead_me_static live in this array:
+// the current plane at sample 0, the reference plane at the next multiple of
+// 128 samples (256 bytes).  Every device the library runs on holds its own
+// copy for the life of the process.  Its host-side name is not a device
+// address: host code uses only the per-device pointer from hipGetSymbolAddress.
+static constexpr size_t kLaScratchSamples = (size_t)XG_LOOKAHEAD_STATIC_MAX_BYTES / sizeof(uint16_t);
+static __device__ __align__(256) uint16_t scratch[kLaScratchSamples];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint16_t *g_scratch[kMaxDevices] = {};
+
+// The calling thread's current device's copy of `scratch`, resolved once per
+// device with that device current; nullptr (error recorded) on failure.
+uint16_t *scratch_on_current_device()
+{
+    int dev = -1;
+    if (!XG_HIP(hipGetDevice(&dev))) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error(XG_ERR_ARG, "xg_lookahead_me_static: device %d beyond the %d the library tracks", dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (!XG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) || !p) return nullptr;
+        g_scratch[dev] = (uint16_t *)p;
+    }
+    return g_scratch[dev];
+}
 }  // namespace
 
 extern "C" {
@@ -72,15 +146,15 @@ extern "C" {
 int32_t xg_lookahead_me(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t unitSize, int32_t range,
                         uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad)
 {
-    if (!s || !cur || !ref || !cur->data || !ref->data || !laMv || !cost || !sad || (unitSize != 8 && unitSize != 16) ||
-        range < 0 || range > 64 || cur->width != ref->width || cur->height != ref->height || cur->width < 2 ||
-        cur->height < 2 || (ref->pad & 1) || ref->pad / 2 < range + 4) {
-        set_error(XG_ERR_ARG, "xg_lookahead_me: bad arguments (units 8 or 16, range <= 64, reference padding even "
-                              "and >= 2 * (range + 4))");
+    if (!s) {
+        set_error(XG_ERR_ARG, "xg_lookahead_me: bad arguments (no session)");
         return XG_ERR_ARG;
     }
+    if (!lookahead_args_ok("xg_lookahead_me", cur, ref, unitSize, range, laMv, cost, sad)) return XG_ERR_ARG;
     const int hw = cur->width / 2, hh = cur->height / 2, hp = ref->pad / 2;
-    const int rw = hw + 2 * hp, rh = hh + 2 * hp;
+    const int rw = hw + 2 * hp;
+    size_t curN, refN;
+    half_sizes(cur, ref, &curN, &refN);
     hipStream_t st = (hipStream_t)xg_session_stream(s);
     xg_plane hc, hr;
     {
@@ -88,18 +162,34 @@ int32_t xg_lookahead_me(xg_session *s, const xg_plane *cur, const xg_plane *ref,
         HalfPics *h = half_for(s);
         // the current picture's stride is kept even, so the packed search kernel takes it
         const int cs = (hw + 1) & ~1;
-        if (!ensure(h->cur, h->curCap, (size_t)cs * hh, st) || !ensure(h->ref, h->refCap, (size_t)rw * rh, st))
-            return XG_ERR_NOMEM;
+        if (!ensure(h->cur, h->curCap, curN, st) || !ensure(h->ref, h->refCap, refN, st)) return XG_ERR_NOMEM;
         hc = xg_plane{h->cur, cs, hw, hh, 0};
         hr = xg_plane{h->ref + (size_t)hp * rw + hp, rw, hw, hh, hp};
     }
-    k_down2x2<<<dim3((hw + 127) / 128, hh), 128, 0, st>>>(cur->data, cur->stride, hc.data, hc.stride, hw, hh);
-    k_down2x2<<<dim3((rw + 127) / 128, rh), 128, 0, st>>>(ref->data - (intptr_t)ref->pad * ref->stride - ref->pad,
-                                                           ref->stride, hr.data - (intptr_t)hp * rw - hp, rw, rw, rh);
-    if (!XG_HIP(hipGetLastError())) return XG_ERR_HIP;
-    // window of a CU with ctuSize = range: MVs up to range + 8 samples past the picture
-    // edge, so the search keeps its full +/-range everywhere
-    return xg_full_search_frame(s, &hc, &hr, unitSize, unitSize, range, sadLambda, range, laMv, cost, sad);
+    return lookahead_launch(st, s, cur, ref, hc, hr, unitSize, range, sadLambda, laMv, cost, sad);
+}
+
+int32_t xg_lookahead_me_static(void *stream, const xg_plane *cur, const xg_plane *ref, int32_t unitSize,
+                               int32_t range, uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad)
+{
+    if (!lookahead_args_ok("xg_lookahead_me_static", cur, ref, unitSize, range, laMv, cost, sad)) return XG_ERR_ARG;
+    size_t curN, refN;
+    half_sizes(cur, ref, &curN, &refN);
+    const size_t refOff = align_up(curN, 128);
+    if (refOff + refN > kLaScratchSamples) {
+        set_error(XG_ERR_ARG, "xg_lookahead_me_static: the half-resolution pictures need %zu bytes, more than the "
+                  "%zu of the static scratch (XG_LOOKAHEAD_STATIC_MAX_BYTES)", (refOff + refN) * sizeof(uint16_t),
+                  kLaScratchSamples * sizeof(uint16_t));
+        return XG_ERR_ARG;
+    }
+    uint16_t *base = scratch_on_current_device();
+    if (!base) return XG_ERR_HIP;
+    const int hw = cur->width / 2, hh = cur->height / 2, hp = ref->pad / 2;
+    const int rw = hw + 2 * hp;
+    const xg_plane hc{base, (hw + 1) & ~1, hw, hh, 0};
+    const xg_plane hr{base + refOff + (size_t)hp * rw + hp, rw, hw, hh, hp};
+    return lookahead_launch((hipStream_t)stream, nullptr, cur, ref, hc, hr, unitSize, range, sadLambda, laMv, cost,
+                            sad);
 }
 
 }  // extern "C"

@@ -53,6 +53,12 @@ void bipred_release(xg_session *s);
 // Releases the quantiser plan a session owns (xg_quant.hip).
 void quant_release(xg_session *s);
 
+// Frame full search of xg_full_search_frame enqueued on `st`; `timer`, when
+// not null, is the session whose stage-3 timer brackets the launch.
+int32_t full_search_frame_on(hipStream_t st, xg_session *timer, const xg_plane *cur, const xg_plane *ref,
+                             int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda, int32_t ctuSize,
+                             int16_t *bestMv, uint32_t *bestCost, uint32_t *bestSad);
+
 void pipe_release(xg_session *s);
 // Refinement of the session's current blocks, validated and uploaded but not
 // launched, for callers that enqueue (or capture) it among their own kernels.

@@ -636,7 +636,21 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
                              int32_t range, uint32_t sadLambda, int32_t ctuSize, int16_t *bestMv,
                              uint32_t *bestCost, uint32_t *bestSad)
 {
-    if (!s || !cur || !ref || !bestMv || !bestCost || !bestSad || bw < 4 || bh < 4 || bw > 64 || bh > 64 ||
+    if (!s) {
+        set_error(XG_ERR_ARG, "xg_full_search_frame: bad arguments");
+        return XG_ERR_ARG;
+    }
+    return xg::full_search_frame_on((hipStream_t)xg_session_stream(s), s, cur, ref, bw, bh, range, sadLambda,
+                                    ctuSize, bestMv, bestCost, bestSad);
+}
+
+}  // extern "C"
+
+int32_t xg::full_search_frame_on(hipStream_t st, xg_session *timer, const xg_plane *cur, const xg_plane *ref,
+                                 int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda, int32_t ctuSize,
+                                 int16_t *bestMv, uint32_t *bestCost, uint32_t *bestSad)
+{
+    if (!cur || !ref || !bestMv || !bestCost || !bestSad || bw < 4 || bh < 4 || bw > 64 || bh > 64 ||
         range < 0 || range > 64 || cur->width != ref->width || cur->height != ref->height ||
         ref->pad < range + 4) {
         set_error(XG_ERR_ARG, "xg_full_search_frame: bad arguments");
@@ -660,9 +674,8 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
     a.fw = cur->width; a.fh = cur->height; a.bw = bw; a.bh = bh; a.range = range; a.ctu = ctuSize; a.nbx = nbx;
     a.lambda = sadLambda;
     a.bestMv = bestMv; a.bestCost = bestCost; a.bestSad = bestSad;
-    hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
-    const bool timed = session_timer_begin(s, 3, ev);
+    const bool timed = timer && session_timer_begin(timer, 3, ev);
     if (packed) {
         const int threads = packed_threads(nxm, kFrameK);
         if (bw == 8)
@@ -674,8 +687,6 @@ int32_t xg_full_search_frame(xg_session *s, const xg_plane *cur, const xg_plane 
     } else {
         k_full_search_frame<<<nbx * nby, 256, lds, st>>>(a);
     }
-    if (timed) session_timer_end(s, 3, ev);
+    if (timed) session_timer_end(timer, 3, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
-
-}  // extern "C"
