@@ -655,30 +655,6 @@ void or_FullSubPelMe(or_me *me)
     free(buf);
 }
 
-/* 1/16-pel pass (build extension of the A7 pattern): re-cost the current
- * quarter-pel best in 1/16-pel MVD units, then the +/-1 (1/16) ring in the
- * order L, R, T, B, TL, TR, BL, BR, strict <.  bestCost/bestSad are updated. */
-void or_SixteenthPelMe(or_me *me, int16_t *bx16, int16_t *by16)
-{
-    or_pixel *buf = (or_pixel *)malloc(sizeof(or_pixel) * 128 * 128);
-    const int32_t px = me->predMvX * 4, py = me->predMvY * 4;
-    int32_t cx = me->bestMvX * 4, cy = me->bestMvY * 4;
-    uint32_t sad0 = sad_at_mv16(me, cx, cy, buf);
-    uint32_t best = (sad0 << 8) + or_MvdBits(cx - px, cy - py) * me->sadLambda, bestSad = sad0;
-    int32_t bx = cx, by = cy;
-    for (int k = 0; k < 8; k++) {
-        int32_t tx = cx + kRingX[k], ty = cy + kRingY[k];
-        uint32_t sad = sad_at_mv16(me, tx, ty, buf);
-        uint32_t c = (sad << 8) + or_MvdBits(tx - px, ty - py) * me->sadLambda;
-        if (c < best) { best = c; bestSad = sad; bx = tx; by = ty; }
-    }
-    *bx16 = (int16_t)bx;
-    *by16 = (int16_t)by;
-    me->bestCost = best;
-    me->bestSad = bestSad;
-    free(buf);
-}
-
 /* h266_analyze_inter_cu.c:133-293 (Xin266FillMeCand): meCand[0] = (0,0); the
  * predictor and the extra quarter-pel candidates are rounded to integer pel by
  * (v + 2) >> 2 (:271-272) and appended unless already present (:265-291). */
@@ -1317,6 +1293,33 @@ static int32_t median3(int32_t a, int32_t b, int32_t c)
     return c < lo ? lo : (c > hi ? hi : c);
 }
 
+/* Stage 2 inputs of the pipeline for block b and reference r, from the
+ * stage-1 full-search MVs fullMv[(b*2+r)*2 + {0,1}] (integer pel) of a raster
+ * grid nbx blocks wide: the predictor is the component-wise median of the
+ * left, top and top-right blocks' MVs in quarter pel (missing neighbours
+ * count as (0,0)); the start candidates are Xin266FillMeCand of that predictor
+ * and the own, left, top and top-right MVs in that order
+ * (h266_analyze_inter_cu.c:133-293: (0,0), predictor, then the parent and
+ * neighbour MVs, each rounded to integer pel and de-duplicated). */
+void or_PipelineCands(const int16_t *fullMv, int32_t nbx, int32_t b, int32_t r, int16_t *predX, int16_t *predY,
+                      int16_t *candX, int16_t *candY, uint32_t *candNum)
+{
+    const int32_t bx = b % nbx, by = b / nbx;
+    int16_t nq[4][2];                       /* own, left, top, top-right, quarter pel */
+    const int32_t nbi[4] = {b, bx > 0 ? b - 1 : -1, by > 0 ? b - nbx : -1,
+                            (by > 0 && bx + 1 < nbx) ? b - nbx + 1 : -1};
+    for (int k = 0; k < 4; k++)
+        for (int c = 0; c < 2; c++)
+            nq[k][c] = nbi[k] < 0 ? 0 : (int16_t)(fullMv[(nbi[k] * 2 + r) * 2 + c] * 4);
+    const int16_t px = (int16_t)median3(nq[1][0], nq[2][0], nq[3][0]);
+    const int16_t py = (int16_t)median3(nq[1][1], nq[2][1], nq[3][1]);
+    const int16_t ex[4] = {nq[0][0], nq[1][0], nq[2][0], nq[3][0]};
+    const int16_t ey[4] = {nq[0][1], nq[1][1], nq[2][1], nq[3][1]};
+    or_FillMeCand(px, py, ex, ey, 4, candX, candY, candNum);
+    *predX = px;
+    *predY = py;
+}
+
 /* Full motion-search pipeline over a raster grid of bw x bh blocks with two
  * reference pictures (a build composition of the reference's stages):
  *  1. full search +/-range around (0,0), predictor (0,0), per block and
@@ -1367,23 +1370,14 @@ void or_MePipeline(const or_pixel *curY, const or_pixel *curU, const or_pixel *c
     for (int32_t r = 0; r < 2; r++)
         for (int32_t b = 0; b < nb; b++) {
             const int32_t bx = b % nbx, by = b / nbx, x = bx * bw, y = by * bh;
-            int16_t nq[4][2];                       /* own, left, top, top-right, quarter pel */
-            const int32_t nbi[4] = {b, bx > 0 ? b - 1 : -1, by > 0 ? b - nbx : -1,
-                                    (by > 0 && bx + 1 < nbx) ? b - nbx + 1 : -1};
-            for (int k = 0; k < 4; k++)
-                for (int c = 0; c < 2; c++)
-                    nq[k][c] = nbi[k] < 0 ? 0 : (int16_t)(fullMv[(nbi[k] * 2 + r) * 2 + c] * 4);
-            const int16_t px = (int16_t)median3(nq[1][0], nq[2][0], nq[3][0]);
-            const int16_t py = (int16_t)median3(nq[1][1], nq[2][1], nq[3][1]);
-            predMv[(b * 2 + r) * 2] = px;
-            predMv[(b * 2 + r) * 2 + 1] = py;
             int32_t mnx, mny, mxx, mxy;
             cu_mv_window(x, y, bw, bh, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
             or_me me;
             memset(&me, 0, sizeof(me));
-            const int16_t ex[4] = {nq[0][0], nq[1][0], nq[2][0], nq[3][0]};
-            const int16_t ey[4] = {nq[0][1], nq[1][1], nq[2][1], nq[3][1]};
-            or_FillMeCand(px, py, ex, ey, 4, me.candX, me.candY, &me.candNum);
+            int16_t px, py;
+            or_PipelineCands(fullMv, nbx, b, r, &px, &py, me.candX, me.candY, &me.candNum);
+            predMv[(b * 2 + r) * 2] = px;
+            predMv[(b * 2 + r) * 2 + 1] = py;
             me.width = (uint32_t)bw; me.height = (uint32_t)bh;
             me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny; me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
             me.predMvX = px; me.predMvY = py;

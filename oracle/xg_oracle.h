@@ -160,7 +160,6 @@ typedef struct or_me {
 void or_MotionEstFull(or_me *me);
 void or_MotionEstDia(or_me *me);
 void or_FullSubPelMe(or_me *me);
-void or_SixteenthPelMe(or_me *me, int16_t *bestMv16X, int16_t *bestMv16Y);
 void or_FillMeCand(int16_t predX, int16_t predY, const int16_t *qx, const int16_t *qy,
                    uint32_t n, int16_t *outX, int16_t *outY, uint32_t *outNum);
 
@@ -230,6 +229,10 @@ void or_RefineFracBlocks(const or_pixel *curY, const or_pixel *curU, const or_pi
 /* Full motion-search pipeline (two references, grid of bw x bh blocks):
  * full search, median predictor + FillMeCand + diamond, half/quarter-pel
  * ring, reference choice, 1/16-pel refinement + MC.  Layout in xg_oracle.c. */
+/* Stage-2 predictor and start candidates of the pipeline for block b and
+ * reference r (see or_MePipeline). */
+void or_PipelineCands(const int16_t *fullMv, int32_t nbx, int32_t b, int32_t r, int16_t *predX, int16_t *predY,
+                      int16_t *candX, int16_t *candY, uint32_t *candNum);
 void or_MePipeline(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV, intptr_t csy, intptr_t csc,
                    const or_pixel *r0Y, const or_pixel *r0U, const or_pixel *r0V,
                    const or_pixel *r1Y, const or_pixel *r1U, const or_pixel *r1V, intptr_t rsy, intptr_t rsc,

@@ -48,10 +48,12 @@ class OrDmvr(ctypes.Structure):
                 ("maxMvX", ctypes.c_int16), ("maxMvY", ctypes.c_int16)]
 
 
-def load_oracle():
-    if not os.path.exists(ORACLE_SO):
+def load_oracle(path=None):
+    """The oracle library with every prototype set; `path` loads another
+    build of the same source (the mutation tests build planted variants)."""
+    if path is None and not os.path.exists(ORACLE_SO):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
-    lib = ctypes.CDLL(ORACLE_SO)
+    lib = ctypes.CDLL(path or ORACLE_SO)
     sig = {
         "or_ComputeSad": (U32, [P, SS, P, SS, U32, U32]),
         "or_ComputeSatd": (U32, [P, SS, P, SS, U32, U32]),
@@ -69,7 +71,7 @@ def load_oracle():
         "or_MotionEstFull": (None, [ctypes.POINTER(OrMe)]),
         "or_MotionEstDia": (None, [ctypes.POINTER(OrMe)]),
         "or_FullSubPelMe": (None, [ctypes.POINTER(OrMe)]),
-        "or_SixteenthPelMe": (None, [ctypes.POINTER(OrMe), P, P]),
+        "or_PipelineCands": (None, [P, I32, I32, I32, P, P, P, P, P]),
         "or_FillMeCand": (None, [ctypes.c_int16, ctypes.c_int16, P, P, U32, P, P, P]),
         "or_TrMatrix": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P]),
         "or_FDct1d": (None, [ctypes.c_int, ctypes.c_int, P, SS, P, SS, I32, I32]),

@@ -1,0 +1,422 @@
+"""Pins the oracle's frame-level drivers -- or_RefineFracBlocks (the checker
+of the batched refinement, the pipeline's last stage and serving),
+or_FullSubPelMe and or_MePipeline -- against the independent numpy rebuilds
+of tests/np_rebuild.py, whose own parts are pinned here too (interpolation
+against the reference's golden vectors).  The mutation tests plant plausible
+mistakes in copies of the oracle's source (a swapped chroma fraction, a CU
+window off by one, <= for <, a ring or diamond order, another median
+neighbour, a candidate order, the reference tie) and require the rebuild
+comparison to catch each one.  CPU only."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import golden_cases as gc
+from tests import np_rebuild as nr
+from tests.helpers import ptr
+from tests.oracle_lib import OrMe, OrRefinePrm, load_oracle
+from tests.pipeline_common import oracle_pipeline
+from xin26x_amd import workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PAD = 80
+U32MAX = 0xFFFFFFFF
+
+
+# -------------------------------------------------------- the numpy parts ---
+def test_np_interp_matches_reference_golden(golden):
+    """The numpy table dispatch reproduces the reference C's own outputs for
+    every luma and chroma Copy / Hor / Vet / HorVet case (filterIndex 0)."""
+    cases, offs, u = golden["cases"], golden["offsets"], golden["src_u16"]
+    keys = {"LumaCopy": (False, 0), "LumaHor": (False, 0x0F), "LumaVet": (False, 0xF0), "LumaHorVet": (False, 0xFF),
+            "ChromaHor": (True, 0x1F), "ChromaVet": (True, 0x3E0), "ChromaHorVet": (True, 0x3FF)}
+    n = 0
+    for i, c in enumerate(cases):
+        key = gc.KEYS[int(c[0])]
+        frac, w, h, fi, ox, oy = (int(v) for v in c[1:7])
+        if key not in keys or fi != 0:
+            continue
+        chroma, mask = keys[key]
+        got = nr.interp(u, ox, oy, w, h, frac & mask, chroma)
+        exp = golden["out_u16"][offs[i]:offs[i + 1]].reshape(h, w)
+        assert np.array_equal(got, exp), (key, frac, w, h)
+        n += 1
+    assert n > 1500
+
+
+def test_np_satd_and_bits_closed_forms():
+    a = np.full((8, 8), 700)
+    b = a - 9
+    assert nr.satd(a, b) == (64 * 9 + 2) >> 2
+    assert nr.satd(a[:4, :4], b[:4, :4]) == (16 * 9 + 1) >> 1
+    assert nr.satd(a[:2, :2], b[:2, :2]) == 4 * 9
+    assert [nr.eg_bits(v) for v in (0, 1, -1, 2, -2, 3, 7, -8)] == [1, 3, 3, 5, 5, 5, 7, 9]
+    # the CU window of h266_enc_init.c:1462-1465 for a 16x8 CU at (32, 24) of a 1920x1080 frame, 64x64 CTUs
+    assert nr.cu_window(32, 24, 16, 8, 1920, 1080, 64) == (-104, -96, 1944, 1120)
+    # chroma address of luma (8, 4) with MV (-3, 21) in 1/16 pel: refOff (125, 85) -> int (3, 2), frac (21, 29)
+    assert nr.mc_split(8, 4, -3, 21, True) == (3, 2, (21 << 5) | 29)
+
+
+# ------------------------------------------------------------ refinement ---
+def _refine_inputs(width=320, height=192, seed=11):
+    """The 1080p workload's generator at width x height; about a third of the
+    blocks get base MVs up to +/-4000 (1/16 pel), so their candidates clip to
+    the CU window, and every block a predictor off its base MV."""
+    fp = wl.frame_pair_1080p(width=width, height=height)
+    b = fp.blocks.copy()
+    rng = np.random.default_rng(seed)
+    far = rng.random(len(b)) < 0.35
+    b[far, 4] = rng.integers(-4000, 4001, far.sum())
+    b[far, 5] = rng.integers(-4000, 4001, far.sum())
+    b[:, 6] = b[:, 4] + rng.integers(-40, 41, len(b))
+    b[:, 7] = b[:, 5] + rng.integers(-40, 41, len(b))
+    return fp, b
+
+
+def oracle_refine(lib, fp, blocks, offs, lam, use_chroma):
+    W, H = fp.width, fp.height
+    refp = [np.ascontiguousarray(np.pad(p, PAD if i == 0 else PAD // 2, mode="edge")) for i, p in enumerate(fp.ref)]
+    blk = np.ascontiguousarray(np.asarray(blocks, np.int32))
+    n, nc = len(blocks), len(offs)
+    offx = np.ascontiguousarray(np.array([o[0] for o in offs], np.int16))
+    offy = np.ascontiguousarray(np.array([o[1] for o in offs], np.int16))
+    prm = OrRefinePrm(nc, offx.ctypes.data, offy.ctypes.data, lam, 64, int(use_chroma))
+    cost = np.zeros(n * nc, np.uint32)
+    bi, bc, bx, by = np.zeros(n, np.int32), np.zeros(n, np.uint32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    pred = [np.zeros((H, W), np.uint16), np.zeros((H // 2, W // 2), np.uint16), np.zeros((H // 2, W // 2), np.uint16)]
+    rsy, rsc = W + 2 * PAD, W // 2 + PAD
+    lib.or_RefineFracBlocks(ptr(fp.cur[0]), ptr(fp.cur[1]), ptr(fp.cur[2]), W, W // 2,
+                            ptr(refp[0], PAD * rsy + PAD), ptr(refp[1], (PAD // 2) * rsc + PAD // 2),
+                            ptr(refp[2], (PAD // 2) * rsc + PAD // 2), rsy, rsc, W, H, ptr(blk), n,
+                            ctypes.byref(prm), ptr(cost), ptr(bi), ptr(bc), ptr(bx), ptr(by),
+                            ptr(pred[0]), ptr(pred[1]), ptr(pred[2]), W, W // 2)
+    return cost.reshape(n, nc), bi, bc, bx, by, pred
+
+
+def check_refine(lib, nblocks=None, use_chroma=True):
+    fp, blocks = _refine_inputs()
+    if nblocks:
+        blocks = blocks[:nblocks]
+    offs = [tuple(int(v) for v in o) for o in wl.CAND32]
+    cost, bi, bc, bx, by, pred = oracle_refine(lib, fp, blocks, offs, wl.LAMBDA_REFINE, use_chroma)
+    npred = [np.zeros_like(p, dtype=np.int64) for p in fp.cur]
+    ncost, nbi, nbc, nbx, nby = nr.refine_blocks(nr.Pic(fp.cur, 0), nr.Pic(fp.ref, PAD), blocks, offs,
+                                                 wl.LAMBDA_REFINE, 64, use_chroma, npred if use_chroma else None)
+    assert np.array_equal(cost, ncost), "candidate costs"
+    assert np.array_equal(bi, nbi) and np.array_equal(bc, nbc), "best candidate / cost"
+    assert np.array_equal(bx, nbx) and np.array_equal(by, nby), "best MV"
+    if use_chroma:
+        for p in range(3):
+            assert np.array_equal(pred[p], npred[p]), f"prediction plane {p}"
+    return len(blocks), blocks
+
+
+def test_refine_frac_blocks_rebuild(oracle):
+    n, blocks = check_refine(oracle)
+    assert n >= 300
+    fw, fh = 320, 192
+    # the set covers frame-edge blocks and candidates clipped to the CU window
+    assert np.any(blocks[:, 0] == 0) and np.any(blocks[:, 0] + blocks[:, 2] == fw)
+    assert np.any(blocks[:, 1] + blocks[:, 3] == fh)
+    clipped = 0
+    for (x, y, w, h, mx, my, _a, _b) in blocks.tolist():
+        mnx, mny, mxx, mxy = nr.cu_window(x, y, w, h, fw, fh, 64)
+        clipped += not (16 * mnx <= mx - 12 and mx + 12 <= 16 * mxx and 16 * mny <= my - 12 and my + 12 <= 16 * mxy)
+    assert clipped > 60
+
+
+def test_refine_frac_blocks_rebuild_luma_only(oracle):
+    check_refine(oracle, nblocks=120, use_chroma=False)
+
+
+# --------------------------------------------------------------- sub-pel ---
+def _subpel_cases():
+    """(cur plane, ref plane, block list) for three kinds of content: smooth
+    texture with fractional motion, a flat picture (every SAD is zero, so MV
+    bits alone decide and ties between symmetric neighbours are common), and
+    two-level tiles (many equal SADs)."""
+    rng = np.random.default_rng(5)
+    H = W = 96
+    smooth = wl.smooth_field(H, W, 21).astype(np.uint16)
+    shifted = np.clip(np.roll(smooth, (1, -2), axis=(0, 1)).astype(np.int64) + rng.integers(-3, 4, (H, W)),
+                      0, 1023).astype(np.uint16)
+    flat = np.full((H, W), 517, np.uint16)
+    tiles = (np.kron(rng.integers(0, 2, (H // 4, W // 4)), np.ones((4, 4), np.int64)) * 600 + 200).astype(np.uint16)
+    tiles_cur = np.roll(tiles, (0, 1), axis=(0, 1))
+    out = []
+    # alternating columns over a smooth vertical profile: the half-pel filter
+    # cancels (-1)^x exactly, so the TL and TR predictions are identical and,
+    # with the predictor straight above or below the centre, so are their
+    # costs; the current block is that prediction, so TL/TR win the half ring
+    # on a tie that only the ring order decides
+    prof = (400 + 200 * np.sin(np.arange(H + 2 * PAD) / 7.0)).astype(np.int64)
+    alt = np.clip(prof[PAD:PAD + H, None] + 60 * (1 - 2 * (np.arange(W)[None, :] & 1)), 0, 1023).astype(np.uint16)
+    apic = nr.Pic([alt, alt[::2, ::2], alt[::2, ::2]], PAD)
+    for k in range(24):
+        w, h = [(16, 16), (8, 8), (16, 8), (8, 4)][k % 4]
+        x, y = 16 + 4 * (k % 8), 16 + 4 * (k // 8)
+        mv = (4 * int(rng.integers(-3, 4)), 4 * int(rng.integers(-3, 4)))
+        sgn = -1 if k % 2 == 0 else 1          # TL/TR tie (-2 rows) or BL/BR tie (+2 rows)
+        cur = alt.copy()
+        tx, ty = mv[0] - 2, mv[1] + 2 * sgn
+        cur[y:y + h, x:x + w] = apic.pred(0, x + ((4 * tx) >> 4), y + ((4 * ty) >> 4), w, h,
+                                          (((4 * ty) & 15) << 4) | ((4 * tx) & 15))
+        out.append((cur, alt, x, y, w, h, mv, (mv[0], mv[1] + sgn * int(rng.integers(0, 3))), 4, U32MAX))
+    for cur, ref in ((shifted, smooth), (flat, flat), (tiles_cur, tiles)):
+        for k in range(40):
+            w, h = [(16, 16), (8, 8), (8, 16), (16, 4), (32, 8), (4, 4)][k % 6]
+            x, y = int(rng.integers(0, (W - w) // 4 + 1)) * 4, int(rng.integers(0, (H - h) // 4 + 1)) * 4
+            mv = (int(rng.integers(-12, 13)) * 4, int(rng.integers(-12, 13)) * 4)
+            mvp = (mv[0] + int(rng.integers(-6, 7)), mv[1] + int(rng.integers(-6, 7)))
+            lam = int(rng.choice([0, 4, 1024, 4096]))
+            start = U32MAX if k % 3 == 0 else int(rng.integers(0, 1 << 20))
+            out.append((cur, ref, x, y, w, h, mv, mvp, lam, start))
+    return out
+
+
+def check_subpel(lib):
+    pics = {}
+    for (cur, ref, x, y, w, h, mv, mvp, lam, start) in _subpel_cases():
+        key = id(ref)
+        if key not in pics:
+            pics[key] = (np.ascontiguousarray(np.pad(ref, PAD, mode="edge")), nr.Pic([ref, ref[::2, ::2], ref[::2, ::2]],
+                                                                                     PAD))
+        refp, rpic = pics[key]
+        H, W = cur.shape
+        win = nr.cu_window(x, y, w, h, W, H, 16)
+        m = OrMe()
+        m.width, m.height = w, h
+        m.minMvX, m.minMvY, m.maxMvX, m.maxMvY = win
+        m.predMvX, m.predMvY = mvp
+        m.bestMvX, m.bestMvY = mv
+        m.bestCost = start
+        m.sadLambda = lam
+        m.input = cur.ctypes.data + 2 * (y * W + x)
+        m.inputStride = W
+        m.ref = refp.ctypes.data + 2 * ((y + PAD) * (W + 2 * PAD) + x + PAD)
+        m.refStride = W + 2 * PAD
+        lib.or_FullSubPelMe(ctypes.byref(m))
+        (ex, ey), ec = nr.full_sub_pel(cur[y:y + h, x:x + w], rpic, x, y, mv, start, mvp, lam, win)
+        assert (m.bestMvX, m.bestMvY, m.bestCost) == (ex, ey, ec), (x, y, w, h, mv, mvp, lam, start)
+
+
+def test_full_sub_pel_rebuild(oracle):
+    check_subpel(oracle)
+
+
+def test_full_sub_pel_outside_window_is_a_no_op(oracle):
+    cur = np.full((32, 32), 300, np.uint16)
+    refp = np.ascontiguousarray(np.pad(cur, PAD, mode="edge"))
+    m = OrMe()
+    m.width = m.height = 8
+    m.minMvX, m.minMvY, m.maxMvX, m.maxMvY = -4, -4, 4, 4
+    m.bestMvX, m.bestMvY = 4 * 5, 0           # integer part 5 > maxMvX
+    m.bestCost = 12345
+    m.sadLambda = 4
+    m.input, m.inputStride = cur.ctypes.data, 32
+    m.ref, m.refStride = refp.ctypes.data + 2 * (PAD * (32 + 2 * PAD) + PAD), 32 + 2 * PAD
+    oracle.or_FullSubPelMe(ctypes.byref(m))
+    assert (m.bestMvX, m.bestMvY, m.bestCost) == (20, 0, 12345)
+
+
+# ------------------------------------------------------- diamond search ---
+def check_diamond(lib):
+    """or_MotionEstDia against the rebuild on period-2 content: the reference
+    picture repeats every two columns, so the L and R neighbours of any point
+    have equal SADs and, with the predictor on the point's column, equal bits;
+    the current block matches the odd columns, so a step from an even column
+    takes that tie.  Random smooth content and random candidate lists too."""
+    rng = np.random.default_rng(8)
+    H = W = 80
+    prof = (300 + 250 * np.cos(np.arange(H) / 5.0)).astype(np.int64)
+    per = np.clip(prof[:, None] + 90 * (np.arange(W)[None, :] & 1), 0, 1023).astype(np.uint16)
+    smooth = wl.smooth_field(H, W, 33).astype(np.uint16)
+    cases = []
+    for k in range(60):
+        w, h = [(16, 16), (8, 8), (16, 8)][k % 3]
+        x, y = 8 + 4 * int(rng.integers(0, 8)), 8 + 4 * int(rng.integers(0, 8))
+        if k < 40:
+            ref = per
+            cur = np.roll(per, (0, 1), axis=(0, 1))          # matches at odd horizontal offsets
+            bx, by = 2 * int(rng.integers(-3, 4)), int(rng.integers(-2, 3))
+            mvp = (4 * bx, 4 * by)
+            cands = [(bx, by)] + [(int(rng.integers(-5, 6)) * 2, int(rng.integers(-3, 4))) for _ in range(3)]
+            lam = int(rng.choice([4, 64]))
+        else:
+            ref = smooth
+            cur = np.clip(np.roll(smooth, (2, -3), axis=(0, 1)).astype(np.int64) + rng.integers(-4, 5, (H, W)), 0,
+                          1023).astype(np.uint16)
+            mvp = (int(rng.integers(-20, 21)), int(rng.integers(-20, 21)))
+            cands = [(int(rng.integers(-6, 7)), int(rng.integers(-6, 7))) for _ in range(int(rng.integers(1, 6)))]
+            lam = int(rng.choice([0, 4, 1024]))
+        cases.append((cur, ref, x, y, w, h, mvp, cands, lam, int(rng.integers(2, 9))))
+    for (cur, ref, x, y, w, h, mvp, cands, lam, sw) in cases:
+        refp = np.ascontiguousarray(np.pad(ref, PAD, mode="edge"))
+        win = nr.cu_window(x, y, w, h, W, H, 16)
+        m = OrMe()
+        m.width, m.height = w, h
+        m.minMvX, m.minMvY, m.maxMvX, m.maxMvY = win
+        m.predMvX, m.predMvY = mvp
+        m.searchWidth = m.searchHeight = sw
+        m.sadLambda = lam
+        m.candNum = len(cands)
+        for i, (a, b) in enumerate(cands):
+            m.candX[i], m.candY[i] = a, b
+        m.input, m.inputStride = cur.ctypes.data + 2 * (y * W + x), W
+        m.ref, m.refStride = refp.ctypes.data + 2 * ((y + PAD) * (W + 2 * PAD) + x + PAD), W + 2 * PAD
+        lib.or_MotionEstDia(ctypes.byref(m))
+        (ex, ey), ec, es = nr.diamond(cur[y:y + h, x:x + w], nr.Pic([ref, ref[::2, ::2], ref[::2, ::2]], PAD), x, y,
+                                      win, mvp, sw, lam, cands)
+        assert (m.bestMvX, m.bestMvY, m.bestCost, m.bestSad) == (4 * ex, 4 * ey, ec, es), (x, y, w, h, mvp, cands)
+
+
+def test_diamond_rebuild(oracle):
+    check_diamond(oracle)
+
+
+def check_pipeline_cands(lib):
+    """The pipeline's predictor and start list for every block of random
+    stage-1 MV fields (small values, so neighbours often coincide and the
+    de-duplication and order decide the list)."""
+    rng = np.random.default_rng(4)
+    for trial in range(40):
+        nbx, nby = int(rng.integers(1, 7)), int(rng.integers(1, 5))
+        full = rng.integers(-3, 4, (nbx * nby, 2, 2)) * int(rng.choice([1, 1, 5]))
+        fm = np.ascontiguousarray(full.astype(np.int16).reshape(-1))
+        for b in range(nbx * nby):
+            for r in range(2):
+                px, py = ctypes.c_int16(), ctypes.c_int16()
+                cx, cy = np.zeros(12, np.int16), np.zeros(12, np.int16)
+                n = ctypes.c_uint32()
+                lib.or_PipelineCands(ptr(fm), nbx, b, r, ctypes.byref(px), ctypes.byref(py), ptr(cx), ptr(cy),
+                                     ctypes.byref(n))
+                p, cands = nr.pipeline_cands(full, nbx, b, r)
+                assert (px.value, py.value) == p, (trial, b, r)
+                assert list(zip(cx[:n.value].tolist(), cy[:n.value].tolist())) == cands, (trial, b, r)
+
+
+def test_pipeline_candidate_lists_rebuild(oracle):
+    check_pipeline_cands(oracle)
+
+
+# -------------------------------------------------------------- pipeline ---
+def _pipeline_scenes():
+    """Three small two-reference scenes: a layered scene with independent
+    object motion; a periodic two-level pattern (many exact SAD ties between
+    positions a period apart, so candidate and diamond order matter); and a
+    scene whose two references are the same picture (every reference choice
+    is a tie)."""
+    fr = wl.layered_frames(128, 64, seed=9, nobj=3, max_mv=10, sides=(16, 48))
+    rng = np.random.default_rng(3)
+    tile = rng.integers(0, 2, (4, 4)) * 500 + 250
+    per = np.tile(tile, (64 // 4 + 4, 128 // 4 + 4)).astype(np.uint16)
+    perc = [per[:64, :128], per[:32, :64], per[:32, :64]]
+    cur = [np.ascontiguousarray(per[1:65, 2:130]), np.ascontiguousarray(per[1:33, 2:66]),
+           np.ascontiguousarray(per[1:33, 2:66])]
+    cur[0] = cur[0].copy()
+    cur[0][20:36, 40:72] = np.clip(cur[0][20:36, 40:72].astype(np.int64) + rng.integers(-40, 41, (16, 32)), 0,
+                                   1023).astype(np.uint16)
+    same = wl.layered_frames(128, 64, seed=13, nobj=2, max_mv=6, sides=(16, 40))
+    return [(fr[2], fr[1], fr[0]), (tuple(cur), tuple(np.ascontiguousarray(p) for p in perc),
+                                    tuple(np.ascontiguousarray(np.roll(p, 1, axis=1)) for p in perc)),
+            (same[2], same[1], same[1])]
+
+
+def check_pipeline(lib, rng_=12, lam=1024):
+    offs = [tuple(int(v) for v in o) for o in wl.CAND17_SIXTEENTH]
+    for si, (cur, r0, r1) in enumerate(_pipeline_scenes()):
+        out = oracle_pipeline(lib, cur, r0, r1, rng=rng_, lam=lam)
+        exp = nr.me_pipeline(nr.Pic(list(cur), 0), [nr.Pic(list(r0), PAD), nr.Pic(list(r1), PAD)], 16, 16, rng_, lam,
+                             64, offs, wl.LAMBDA_REFINE)
+        nb = len(exp["ref"])
+        tag = f"scene {si}"
+        assert np.array_equal(out["full"].reshape(nb, 2, 2), exp["full"]), f"{tag}: full-search MVs"
+        assert np.array_equal(out["pred"].reshape(nb, 2, 2), exp["pred"]), f"{tag}: median predictors"
+        assert np.array_equal(out["int"].reshape(nb, 2, 2), exp["int"]), f"{tag}: diamond MVs"
+        assert np.array_equal(out["intCost"].reshape(nb, 2), exp["icost"]), f"{tag}: diamond costs"
+        assert np.array_equal(out["q"].reshape(nb, 2, 2), exp["q"]), f"{tag}: quarter-pel MVs"
+        assert np.array_equal(out["qCost"].reshape(nb, 2), exp["qcost"]), f"{tag}: quarter-pel costs"
+        assert np.array_equal(out["ref"], exp["ref"]), f"{tag}: reference choice"
+        assert np.array_equal(out["cand"].reshape(nb, -1), exp["cand"]), f"{tag}: 1/16 candidate costs"
+        assert np.array_equal(np.stack([out["bi"], out["bc"], out["bx"], out["by"]], 1), exp["best"]), f"{tag}: best"
+        for p in range(3):
+            assert np.array_equal(out["pred_pic"][p], exp["pred_planes"][p]), f"{tag}: prediction plane {p}"
+
+
+def test_me_pipeline_rebuild(oracle):
+    check_pipeline(oracle)
+
+
+# ------------------------------------------------------------- mutations ---
+MUTANTS = {
+    # or_RefineFracBlocks
+    "refine: chroma fraction nibbles swapped": ("int32_t fracC = ((ry & 31) << 5) + (rx & 31);",
+                                                "int32_t fracC = ((rx & 31) << 5) + (ry & 31);", "refine"),
+    "refine: CU window one sample narrower": ("*mnx = (int16_t)(-x - ctu - (16 - 8));",
+                                              "*mnx = (int16_t)(-x - ctu - (16 - 9));", "refine"),
+    "refine: <= keeps the last of equal costs": ("if (cost < best) { best = cost; bi = c;",
+                                                 "if (cost <= best) { best = cost; bi = c;", "refine"),
+    "refine: luma fraction from the chroma split": ("int32_t fracY = ((ry & 15) << 4) + (rx & 15);",
+                                                    "int32_t fracY = ((ry & 15) << 4) + (ry & 15);", "refine"),
+    # or_FullSubPelMe
+    "sub-pel: TL and TR swapped in the ring": ("static const int8_t kRingX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};",
+                                               "static const int8_t kRingX[8] = {-1, 1, 0, 0, 1, -1, -1, 1};",
+                                               "subpel"),
+    "sub-pel: quarter ring around the integer best": ("const int32_t cx = mx, cy = my;",
+                                                      "const int32_t cx = me->bestMvX, cy = me->bestMvY;", "subpel"),
+    "sub-pel: <= in the rings": ("if (c < best) { best = c; mx = tx; my = ty; }",
+                                 "if (c <= best) { best = c; mx = tx; my = ty; }", "subpel"),
+    # or_MePipeline
+    "pipeline: top-left instead of top-right in the median": (
+        "(by > 0 && bx + 1 < nbx) ? b - nbx + 1 : -1}", "(by > 0 && bx > 0) ? b - nbx - 1 : -1}", "pipeline"),
+    "pipeline: left and top swapped in the candidate list": (
+        "const int16_t ex[4] = {nq[0][0], nq[1][0], nq[2][0], nq[3][0]};\n"
+        "    const int16_t ey[4] = {nq[0][1], nq[1][1], nq[2][1], nq[3][1]};",
+        "const int16_t ex[4] = {nq[0][0], nq[2][0], nq[1][0], nq[3][0]};\n"
+        "    const int16_t ey[4] = {nq[0][1], nq[2][1], nq[1][1], nq[3][1]};", "cands"),
+    "pipeline: predictor from the mean of the extremes": (
+        "static int32_t median3(int32_t a, int32_t b, int32_t c)\n{",
+        "static int32_t median3(int32_t a, int32_t b, int32_t c)\n{\n"
+        "    if (1) { int32_t l = a < b ? a : b, h = a < b ? b : a; l = c < l ? c : l; h = c > h ? c : h; "
+        "return (l + h) / 2; }", "cands"),
+    "diamond: R before L": ("static const int8_t dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};",
+                            "static const int8_t dxs[4] = {1, -1, 0, 0}, dys[4] = {0, 0, -1, 1};", "diamond"),
+    "diamond: candidates evaluated last to first": ("    for (uint32_t i = 0; i < me->candNum; i++) {\n"
+                                                    "        int32_t mx = me->candX[i], my = me->candY[i];",
+                                                    "    for (uint32_t j = me->candNum; j-- > 0;) {\n"
+                                                    "        const uint32_t i = j;\n"
+                                                    "        int32_t mx = me->candX[i], my = me->candY[i];",
+                                                    "diamond"),
+    "pipeline: reference ties go to reference 1": ("const int32_t r = qCost[b * 2 + 1] < qCost[b * 2] ? 1 : 0;",
+                                                   "const int32_t r = qCost[b * 2 + 1] <= qCost[b * 2] ? 1 : 0;",
+                                                   "pipeline"),
+}
+CHECKS = {"refine": lambda lib: check_refine(lib, nblocks=160), "subpel": check_subpel, "pipeline": check_pipeline,
+          "diamond": check_diamond, "cands": check_pipeline_cands}
+
+
+@pytest.fixture(scope="module")
+def mutant_dir(tmp_path_factory):
+    if not shutil.which("gcc"):
+        pytest.skip("gcc is needed to build the planted variants")
+    return tmp_path_factory.mktemp("oracle_mutants")
+
+
+@pytest.mark.parametrize("name", sorted(MUTANTS))
+def test_rebuild_catches_planted_mistake(name, mutant_dir):
+    old, new, check = MUTANTS[name]
+    src = open(os.path.join(ROOT, "oracle", "xg_oracle.c")).read()
+    assert src.count(old) == 1, f"mutation site not found once: {old!r}"
+    d = mutant_dir / name.split(":")[0].replace(" ", "_") / str(sorted(MUTANTS).index(name))
+    d.mkdir(parents=True)
+    shutil.copy(os.path.join(ROOT, "oracle", "xg_oracle.h"), d / "xg_oracle.h")
+    (d / "xg_oracle.c").write_text(src.replace(old, new))
+    so = d / "liboracle_mutant.so"
+    subprocess.check_call(["gcc", "-O1", "-fPIC", "-shared", "-std=c11", "-o", str(so), str(d / "xg_oracle.c")])
+    lib = load_oracle(str(so))
+    with pytest.raises(AssertionError):
+        CHECKS[check](lib)
