@@ -811,6 +811,10 @@ def pipeline_leg(lib, steps, warmup, cpu_seconds):
         graph_ms = (time.perf_counter() - t0) / steps * 1e3
         q = np.zeros(4 * nb, np.int16)
         capi.check(lib.xg_me_pipeline_read(s, None, None, None, None, q.ctypes.data, None), "read")
+        best = np.zeros(nb * 4, np.int32)
+        cand = np.zeros(nb * len(REFINE_OFFS), np.uint32)
+        capi.check(lib.xg_session_read(s, cand.ctypes.data, best.ctypes.data), "read")
+        gpu = {"q": q, "best": best.reshape(nb, 4), "cand": cand, "pred": pred.planes()}
     finally:
         lib.xg_session_destroy(s)
     k_s = k_ms / 1e3
@@ -828,40 +832,54 @@ def pipeline_leg(lib, steps, warmup, cpu_seconds):
            "median_quarter_mv_ref0": [int(np.median(q.reshape(-1, 2, 2)[:, 0, 0])),
                                       int(np.median(q.reshape(-1, 2, 2)[:, 0, 1]))]}
     if cpu_seconds > 0:
-        out["cpu_baseline"] = pipeline_cpu(cur, ref0, ref1, cpu_seconds)
+        out["cpu_baseline"] = pipeline_cpu(cur, ref0, ref1, gpu, cpu_seconds)
+        out["bit_exact_on_cpu_sample"] = out["cpu_baseline"]["bit_exact_vs_gpu_on_sample"]
     return out
 
 
-def pipeline_cpu(cur, ref0, ref1, seconds):
-    """The oracle's pipeline on 128x128 crops of the same frames (edge-padded
-    crops, so the search windows match the interior of the frame), one crop per
-    host thread."""
+def pipeline_cpu(cur, ref0, ref1, gpu, seconds):
+    """The oracle's pipeline on bands of whole 16x16 block rows of the same 4K
+    frame (or_MePipelineSample: each band computes its rows' stages plus the
+    stage-1 searches of the row above), one band per host thread, rounds until
+    about `seconds` have passed; every block it finished is compared with the
+    GPU's quarter-pel MVs, 1/16 candidate costs, winners and prediction."""
     from tests.oracle_lib import load_oracle
-    from tests.pipeline_common import oracle_pipeline
+    from tests.pipeline_common import oracle_pipeline_sample
     oracle = load_oracle()
     threads = max(1, min(16, os.cpu_count() or 1))
-    cw = 128
     H, W = cur[0].shape
-    crops = []
-    for k in range(64):
-        x = (k * 977) % (W - 2 * cw) // 16 * 16 + cw
-        y = (k * 571) % (H - 2 * cw) // 16 * 16 + cw
-        crop = lambda f: tuple(np.ascontiguousarray(p[(y >> (i > 0)):(y + cw) >> (i > 0), (x >> (i > 0)):(x + cw) >> (i > 0)])
-                               for i, p in enumerate(f))
-        crops.append((crop(cur), crop(ref0), crop(ref1)))
-    px = 0
+    nbx, nby = W // 16, H // 16
+    k = 2                                       # block rows per band (and per thread)
+    starts = list(range(0, nby - k + 1, k))
+    starts = starts[::3] + starts[1::3] + starts[2::3]       # spread each round over the frame
+    done, exact, pos = [], True, 0
     t0 = time.perf_counter()
-    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
-        pos = 0
-        while time.perf_counter() - t0 < seconds:
-            batch = [crops[(pos + k) % len(crops)] for k in range(threads)]
-            pos += threads
-            list(ex.map(lambda c: oracle_pipeline(oracle, *c), batch))
-            px += threads * cw * cw
+    while time.perf_counter() - t0 < seconds and pos < len(starts):
+        rows = [r for r0 in starts[pos:pos + threads] for r in range(r0, r0 + k)]
+        nthr = len(starts[pos:pos + threads])
+        pos += threads
+        sample = np.array([r * nbx + c for r in rows for c in range(nbx)], np.int32)
+        e = oracle_pipeline_sample(oracle, cur, ref0, ref1, sample, threads=nthr)
+        n = len(sample)
+        exact &= bool(np.array_equal(gpu["q"].reshape(-1, 4)[sample], e["q"].reshape(n, 4)))
+        exact &= bool(np.array_equal(gpu["cand"].reshape(-1, len(e["cand"]) // n)[sample], e["cand"].reshape(n, -1)))
+        b = gpu["best"][sample]
+        exact &= bool(np.array_equal(b[:, 0], e["bx"]) and np.array_equal(b[:, 1], e["by"]) and
+                      np.array_equal(b[:, 3], e["bi"]))
+        for r in rows:
+            for p in range(3):
+                sh = 1 if p == 0 else 2
+                exact &= bool(np.array_equal(gpu["pred"][p][r * 16 // sh:(r + 1) * 16 // sh],
+                                             e["pred_pic"][p][r * 16 // sh:(r + 1) * 16 // sh]))
+        done += rows
     dt = time.perf_counter() - t0
+    px = len(done) * nbx * 256
     return {"value": round(px / dt / 1e6, 4), "unit": "Mpixel/s", "cores": threads, "kind": "port",
-            "sample": f"oracle pipeline (scalar C, -O3) on {px // (cw * cw)} 128x128 crops of the same frames, "
-                      f"one per thread, {dt:.1f} s"}
+            "bit_exact_vs_gpu_on_sample": exact,
+            "sample": f"oracle pipeline (scalar C, -O3) on {len(done)} of {nby} block rows ({len(done) * nbx} blocks) "
+                      f"of the same 4K frame, bands of {k} rows, one band per thread (each also runs the stage-1 "
+                      f"searches of the row above it), {dt:.1f} s; the quarter-pel MVs, 1/16 candidate costs, "
+                      f"winners and prediction of those blocks compared with the GPU's"}
 
 
 def main():

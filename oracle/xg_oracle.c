@@ -1338,6 +1338,107 @@ void or_PipelineCands(const int16_t *fullMv, int32_t nbx, int32_t b, int32_t r, 
  * Per block b and reference r: fullMv/intMv/qMv[(b*2+r)*2 + {0,1}] (int pel,
  * int pel, quarter pel), predMv likewise (quarter pel), intCost/qCost[b*2+r];
  * refIdx[b]; candCost[b*ncand + c], bestIdx/bestCost/bestMvx/bestMvy[b]. */
+/* The pictures and parameters every stage of the pipeline reads. */
+typedef struct {
+    const or_pixel *cur[3];
+    const or_pixel *ref[2][3];
+    intptr_t csy, csc, rsy, rsc;
+    int32_t fw, fh, bw, bh, nbx, range, ctu;
+    uint32_t lambda;
+    const or_refine_prm *prm;
+} pipe_ctx;
+
+/* Stage 1 of block b against reference r: full search +/-range around (0,0),
+ * predictor (0,0); out = integer-pel MV. */
+static void pipe_full(const pipe_ctx *c, int32_t b, int32_t r, int16_t out[2])
+{
+    const int32_t x = (b % c->nbx) * c->bw, y = (b / c->nbx) * c->bh;
+    int32_t mnx, mny, mxx, mxy;
+    cu_mv_window(x, y, c->bw, c->bh, c->fw, c->fh, c->ctu, &mnx, &mny, &mxx, &mxy);
+    or_me me;
+    memset(&me, 0, sizeof(me));
+    me.width = (uint32_t)c->bw; me.height = (uint32_t)c->bh;
+    me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny; me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
+    me.searchWidth = me.searchHeight = c->range;
+    me.sadLambda = c->lambda;
+    me.input = c->cur[0] + y * c->csy + x; me.inputStride = c->csy;
+    me.ref = c->ref[r][0] + y * c->rsy + x; me.refStride = c->rsy;
+    or_MotionEstFull(&me);
+    out[0] = (int16_t)(me.bestMvX >> 2);
+    out[1] = (int16_t)(me.bestMvY >> 2);
+}
+
+/* Stages 2-3 of block b against reference r, given the stage-1 MVs of b and
+ * its left, top and top-right neighbours in the raster array fullMv:
+ * predictor and start list (or_PipelineCands), diamond search, half- and
+ * quarter-pel rings.  Outputs for this (b, r) at pred[2], imv[2], *icost,
+ * qmv[2], *qcost. */
+static void pipe_stage2(const pipe_ctx *c, const int16_t *fullMv, int32_t b, int32_t r, int16_t pred[2],
+                        int16_t imv[2], uint32_t *icost, int16_t qmv[2], uint32_t *qcost)
+{
+    const int32_t x = (b % c->nbx) * c->bw, y = (b / c->nbx) * c->bh;
+    int32_t mnx, mny, mxx, mxy;
+    cu_mv_window(x, y, c->bw, c->bh, c->fw, c->fh, c->ctu, &mnx, &mny, &mxx, &mxy);
+    or_me me;
+    memset(&me, 0, sizeof(me));
+    int16_t px, py;
+    or_PipelineCands(fullMv, c->nbx, b, r, &px, &py, me.candX, me.candY, &me.candNum);
+    pred[0] = px;
+    pred[1] = py;
+    me.width = (uint32_t)c->bw; me.height = (uint32_t)c->bh;
+    me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny; me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
+    me.predMvX = px; me.predMvY = py;
+    me.searchWidth = me.searchHeight = c->range;
+    me.sadLambda = c->lambda;
+    me.input = c->cur[0] + y * c->csy + x; me.inputStride = c->csy;
+    me.ref = c->ref[r][0] + y * c->rsy + x; me.refStride = c->rsy;
+    or_MotionEstDia(&me);
+    imv[0] = (int16_t)(me.bestMvX >> 2);
+    imv[1] = (int16_t)(me.bestMvY >> 2);
+    *icost = me.bestCost;
+    or_FullSubPelMe(&me);
+    qmv[0] = me.bestMvX;
+    qmv[1] = me.bestMvY;
+    *qcost = me.bestCost;
+}
+
+/* Stages 4-5 of block b: the reference with the lower quarter-pel cost (ties:
+ * reference 0), then its 1/16-pel refinement + MC around 4x the quarter-pel MV
+ * with predictor 4x that reference's median predictor.  pred2/qmv2/qcost2 hold
+ * both references' stage-2/3 results ([r*2 + c] and [r]). */
+static int32_t pipe_final(const pipe_ctx *c, int32_t b, const int16_t *pred2, const int16_t *qmv2,
+                          const uint32_t *qcost2, uint32_t *candCost, int32_t *bestIdx, uint32_t *bestCost,
+                          int32_t *bestMvx, int32_t *bestMvy, or_pixel *predY, or_pixel *predU, or_pixel *predV,
+                          intptr_t psy, intptr_t psc)
+{
+    const int32_t r = qcost2[1] < qcost2[0] ? 1 : 0;
+    or_block blk;
+    blk.x = (b % c->nbx) * c->bw; blk.y = (b / c->nbx) * c->bh; blk.w = c->bw; blk.h = c->bh;
+    blk.mvx = qmv2[r * 2] * 4; blk.mvy = qmv2[r * 2 + 1] * 4;
+    blk.mvpx = pred2[r * 2] * 4; blk.mvpy = pred2[r * 2 + 1] * 4;
+    or_RefineFracBlocks(c->cur[0], c->cur[1], c->cur[2], c->csy, c->csc, c->ref[r][0], c->ref[r][1], c->ref[r][2],
+                        c->rsy, c->rsc, c->fw, c->fh, &blk, 1, c->prm, candCost, bestIdx, bestCost, bestMvx, bestMvy,
+                        predY, predU, predV, psy, psc);
+    return r;
+}
+
+static pipe_ctx pipe_ctx_make(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV, intptr_t csy,
+                              intptr_t csc, const or_pixel *r0Y, const or_pixel *r0U, const or_pixel *r0V,
+                              const or_pixel *r1Y, const or_pixel *r1U, const or_pixel *r1V, intptr_t rsy,
+                              intptr_t rsc, int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range,
+                              uint32_t sadLambda, int32_t ctuSize, const or_refine_prm *prm)
+{
+    pipe_ctx c;
+    c.cur[0] = curY; c.cur[1] = curU; c.cur[2] = curV;
+    c.ref[0][0] = r0Y; c.ref[0][1] = r0U; c.ref[0][2] = r0V;
+    c.ref[1][0] = r1Y; c.ref[1][1] = r1U; c.ref[1][2] = r1V;
+    c.csy = csy; c.csc = csc; c.rsy = rsy; c.rsc = rsc;
+    c.fw = fw; c.fh = fh; c.bw = bw; c.bh = bh; c.nbx = fw / bw; c.range = range; c.ctu = ctuSize;
+    c.lambda = sadLambda;
+    c.prm = prm;
+    return c;
+}
+
 void or_MePipeline(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV, intptr_t csy, intptr_t csc,
                    const or_pixel *r0Y, const or_pixel *r0U, const or_pixel *r0V,
                    const or_pixel *r1Y, const or_pixel *r1U, const or_pixel *r1V, intptr_t rsy, intptr_t rsc,
@@ -1348,63 +1449,65 @@ void or_MePipeline(const or_pixel *curY, const or_pixel *curU, const or_pixel *c
                    int32_t *bestMvx, int32_t *bestMvy, or_pixel *predY, or_pixel *predU, or_pixel *predV,
                    intptr_t psy, intptr_t psc)
 {
-    const int32_t nbx = fw / bw, nby = fh / bh, nb = nbx * nby;
-    const or_pixel *refs[2] = {r0Y, r1Y};
+    const pipe_ctx c = pipe_ctx_make(curY, curU, curV, csy, csc, r0Y, r0U, r0V, r1Y, r1U, r1V, rsy, rsc, fw, fh,
+                                     bw, bh, range, sadLambda, ctuSize, prm);
+    const int32_t nb = c.nbx * (fh / bh);
     for (int32_t r = 0; r < 2; r++)
-        for (int32_t b = 0; b < nb; b++) {
-            const int32_t x = (b % nbx) * bw, y = (b / nbx) * bh;
-            int32_t mnx, mny, mxx, mxy;
-            cu_mv_window(x, y, bw, bh, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
-            or_me me;
-            memset(&me, 0, sizeof(me));
-            me.width = (uint32_t)bw; me.height = (uint32_t)bh;
-            me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny; me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
-            me.searchWidth = me.searchHeight = range;
-            me.sadLambda = sadLambda;
-            me.input = curY + y * csy + x; me.inputStride = csy;
-            me.ref = refs[r] + y * rsy + x; me.refStride = rsy;
-            or_MotionEstFull(&me);
-            fullMv[(b * 2 + r) * 2] = (int16_t)(me.bestMvX >> 2);
-            fullMv[(b * 2 + r) * 2 + 1] = (int16_t)(me.bestMvY >> 2);
-        }
+        for (int32_t b = 0; b < nb; b++) pipe_full(&c, b, r, fullMv + (b * 2 + r) * 2);
     for (int32_t r = 0; r < 2; r++)
-        for (int32_t b = 0; b < nb; b++) {
-            const int32_t bx = b % nbx, by = b / nbx, x = bx * bw, y = by * bh;
-            int32_t mnx, mny, mxx, mxy;
-            cu_mv_window(x, y, bw, bh, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
-            or_me me;
-            memset(&me, 0, sizeof(me));
-            int16_t px, py;
-            or_PipelineCands(fullMv, nbx, b, r, &px, &py, me.candX, me.candY, &me.candNum);
-            predMv[(b * 2 + r) * 2] = px;
-            predMv[(b * 2 + r) * 2 + 1] = py;
-            me.width = (uint32_t)bw; me.height = (uint32_t)bh;
-            me.minMvX = (int16_t)mnx; me.minMvY = (int16_t)mny; me.maxMvX = (int16_t)mxx; me.maxMvY = (int16_t)mxy;
-            me.predMvX = px; me.predMvY = py;
-            me.searchWidth = me.searchHeight = range;
-            me.sadLambda = sadLambda;
-            me.input = curY + y * csy + x; me.inputStride = csy;
-            me.ref = refs[r] + y * rsy + x; me.refStride = rsy;
-            or_MotionEstDia(&me);
-            intMv[(b * 2 + r) * 2] = (int16_t)(me.bestMvX >> 2);
-            intMv[(b * 2 + r) * 2 + 1] = (int16_t)(me.bestMvY >> 2);
-            intCost[b * 2 + r] = me.bestCost;
-            or_FullSubPelMe(&me);
-            qMv[(b * 2 + r) * 2] = me.bestMvX;
-            qMv[(b * 2 + r) * 2 + 1] = me.bestMvY;
-            qCost[b * 2 + r] = me.bestCost;
+        for (int32_t b = 0; b < nb; b++)
+            pipe_stage2(&c, fullMv, b, r, predMv + (b * 2 + r) * 2, intMv + (b * 2 + r) * 2, intCost + b * 2 + r,
+                        qMv + (b * 2 + r) * 2, qCost + b * 2 + r);
+    for (int32_t b = 0; b < nb; b++)
+        refIdx[b] = pipe_final(&c, b, predMv + b * 4, qMv + b * 4, qCost + b * 2,
+                               candCost + (size_t)b * prm->ncand, bestIdx + b, bestCost + b, bestMvx + b,
+                               bestMvy + b, predY, predU, predV, psy, psc);
+}
+
+/* The pipeline for the blocks sample[0..n) only: each needs the stage-1 MVs
+ * of itself and its left, top and top-right neighbours, and nothing else, so
+ * every output equals or_MePipeline's for that block.  Outputs are indexed by
+ * sample position i (fullMv/predMv/intMv/qMv[(i*2+r)*2 + c], intCost/qCost
+ * [i*2+r], refIdx/bestIdx/bestCost/bestMvx/bestMvy[i], candCost[i*ncand+k]);
+ * the prediction of each sampled block is written into the planes at its
+ * position.  Stage-1 results are computed once per block within a call, so a
+ * sample of whole block rows costs one extra row of full searches.  Samples
+ * may be split over threads (no shared state). */
+void or_MePipelineSample(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV, intptr_t csy,
+                         intptr_t csc, const or_pixel *r0Y, const or_pixel *r0U, const or_pixel *r0V,
+                         const or_pixel *r1Y, const or_pixel *r1U, const or_pixel *r1V, intptr_t rsy, intptr_t rsc,
+                         int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda,
+                         int32_t ctuSize, const or_refine_prm *prm, const int32_t *sample, int32_t n,
+                         int16_t *fullMv, int16_t *predMv, int16_t *intMv, uint32_t *intCost, int16_t *qMv,
+                         uint32_t *qCost, int32_t *refIdx, uint32_t *candCost, int32_t *bestIdx, uint32_t *bestCost,
+                         int32_t *bestMvx, int32_t *bestMvy, or_pixel *predY, or_pixel *predU, or_pixel *predV,
+                         intptr_t psy, intptr_t psc)
+{
+    const pipe_ctx c = pipe_ctx_make(curY, curU, curV, csy, csc, r0Y, r0U, r0V, r1Y, r1U, r1V, rsy, rsc, fw, fh,
+                                     bw, bh, range, sadLambda, ctuSize, prm);
+    const int32_t nbx = c.nbx, nb = nbx * (fh / bh);
+    int16_t *raster = (int16_t *)calloc((size_t)nb * 4, sizeof(int16_t));
+    uint8_t *have = (uint8_t *)calloc((size_t)nb, 1);     /* stage 1 done for this block, this call */
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t b = sample[i], bx = b % nbx, by = b / nbx;
+        const int32_t nbi[4] = {b, bx > 0 ? b - 1 : -1, by > 0 ? b - nbx : -1,
+                                (by > 0 && bx + 1 < nbx) ? b - nbx + 1 : -1};
+        for (int k = 0; k < 4; k++)
+            if (nbi[k] >= 0 && !have[nbi[k]]) {
+                for (int32_t r = 0; r < 2; r++) pipe_full(&c, nbi[k], r, raster + (nbi[k] * 2 + r) * 2);
+                have[nbi[k]] = 1;
+            }
+        for (int32_t r = 0; r < 2; r++) {
+            fullMv[(i * 2 + r) * 2] = raster[(b * 2 + r) * 2];
+            fullMv[(i * 2 + r) * 2 + 1] = raster[(b * 2 + r) * 2 + 1];
+            pipe_stage2(&c, raster, b, r, predMv + (i * 2 + r) * 2, intMv + (i * 2 + r) * 2, intCost + i * 2 + r,
+                        qMv + (i * 2 + r) * 2, qCost + i * 2 + r);
         }
-    for (int32_t b = 0; b < nb; b++) {
-        const int32_t r = qCost[b * 2 + 1] < qCost[b * 2] ? 1 : 0;
-        refIdx[b] = r;
-        or_block blk;
-        blk.x = (b % nbx) * bw; blk.y = (b / nbx) * bh; blk.w = bw; blk.h = bh;
-        blk.mvx = qMv[(b * 2 + r) * 2] * 4; blk.mvy = qMv[(b * 2 + r) * 2 + 1] * 4;
-        blk.mvpx = predMv[(b * 2 + r) * 2] * 4; blk.mvpy = predMv[(b * 2 + r) * 2 + 1] * 4;
-        or_RefineFracBlocks(curY, curU, curV, csy, csc, r ? r1Y : r0Y, r ? r1U : r0U, r ? r1V : r0V, rsy, rsc,
-                            fw, fh, &blk, 1, prm, candCost + (size_t)b * prm->ncand, bestIdx + b, bestCost + b,
-                            bestMvx + b, bestMvy + b, predY, predU, predV, psy, psc);
+        refIdx[i] = pipe_final(&c, b, predMv + i * 4, qMv + i * 4, qCost + i * 2, candCost + (size_t)i * prm->ncand,
+                               bestIdx + i, bestCost + i, bestMvx + i, bestMvy + i, predY, predU, predV, psy, psc);
     }
+    free(have);
+    free(raster);
 }
 
 /* ---------------------------------------------------------------------------

@@ -242,6 +242,17 @@ void or_MePipeline(const or_pixel *curY, const or_pixel *curU, const or_pixel *c
                    uint32_t *qCost, int32_t *refIdx, uint32_t *candCost, int32_t *bestIdx, uint32_t *bestCost,
                    int32_t *bestMvx, int32_t *bestMvy, or_pixel *predY, or_pixel *predU, or_pixel *predV,
                    intptr_t psy, intptr_t psc);
+/* or_MePipeline for the blocks sample[0..n) only, outputs indexed by sample
+ * position (see xg_oracle.c); thread-safe across disjoint samples. */
+void or_MePipelineSample(const or_pixel *curY, const or_pixel *curU, const or_pixel *curV, intptr_t csy,
+                         intptr_t csc, const or_pixel *r0Y, const or_pixel *r0U, const or_pixel *r0V,
+                         const or_pixel *r1Y, const or_pixel *r1U, const or_pixel *r1V, intptr_t rsy, intptr_t rsc,
+                         int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range, uint32_t sadLambda,
+                         int32_t ctuSize, const or_refine_prm *prm, const int32_t *sample, int32_t n,
+                         int16_t *fullMv, int16_t *predMv, int16_t *intMv, uint32_t *intCost, int16_t *qMv,
+                         uint32_t *qCost, int32_t *refIdx, uint32_t *candCost, int32_t *bestIdx, uint32_t *bestCost,
+                         int32_t *bestMvx, int32_t *bestMvy, or_pixel *predY, or_pixel *predU, or_pixel *predV,
+                         intptr_t psy, intptr_t psc);
 
 /* Config-1 stage: integer full search for every block of a frame. */
 void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs,

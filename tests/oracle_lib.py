@@ -99,6 +99,8 @@ def load_oracle(path=None):
         "or_QuantBatch": (None, [P, P, P, I32, I32, P, P, P]),
         "or_MePipeline": (None, [P, P, P, SS, SS, P, P, P, P, P, P, SS, SS, I32, I32, I32, I32, I32, U32, I32,
                                  ctypes.POINTER(OrRefinePrm)] + [P] * 15 + [SS, SS]),
+        "or_MePipelineSample": (None, [P, P, P, SS, SS, P, P, P, P, P, P, SS, SS, I32, I32, I32, I32, I32, U32, I32,
+                                       ctypes.POINTER(OrRefinePrm), P, I32] + [P] * 15 + [SS, SS]),
         "or_BlockSubForDct": (None, [P, SS, P, SS, P, SS, U32, U32]),
         "or_MtsTypes": (None, [U32, P, P]),
         "or_LumaInterpolation": (None, [P, SS, P, SS, I32, U32, U32, U32]),

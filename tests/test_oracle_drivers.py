@@ -372,7 +372,8 @@ MUTANTS = {
                                  "if (c <= best) { best = c; mx = tx; my = ty; }", "subpel"),
     # or_MePipeline
     "pipeline: top-left instead of top-right in the median": (
-        "(by > 0 && bx + 1 < nbx) ? b - nbx + 1 : -1}", "(by > 0 && bx > 0) ? b - nbx - 1 : -1}", "pipeline"),
+        "\n                            (by > 0 && bx + 1 < nbx) ? b - nbx + 1 : -1};",
+        "\n                            (by > 0 && bx > 0) ? b - nbx - 1 : -1};", "pipeline"),
     "pipeline: left and top swapped in the candidate list": (
         "const int16_t ex[4] = {nq[0][0], nq[1][0], nq[2][0], nq[3][0]};\n"
         "    const int16_t ey[4] = {nq[0][1], nq[1][1], nq[2][1], nq[3][1]};",
@@ -391,9 +392,8 @@ MUTANTS = {
                                                     "        const uint32_t i = j;\n"
                                                     "        int32_t mx = me->candX[i], my = me->candY[i];",
                                                     "diamond"),
-    "pipeline: reference ties go to reference 1": ("const int32_t r = qCost[b * 2 + 1] < qCost[b * 2] ? 1 : 0;",
-                                                   "const int32_t r = qCost[b * 2 + 1] <= qCost[b * 2] ? 1 : 0;",
-                                                   "pipeline"),
+    "pipeline: reference ties go to reference 1": ("const int32_t r = qcost2[1] < qcost2[0] ? 1 : 0;",
+                                                   "const int32_t r = qcost2[1] <= qcost2[0] ? 1 : 0;", "pipeline"),
 }
 CHECKS = {"refine": lambda lib: check_refine(lib, nblocks=160), "subpel": check_subpel, "pipeline": check_pipeline,
           "diamond": check_diamond, "cands": check_pipeline_cands}

@@ -284,7 +284,7 @@ static void interp_u16(int taps, bool outS16, Dir dir, const uint16_t *src, intp
                        intptr_t ds, int fx, int fy, int fi, uint32_t w, uint32_t h, const char *fn)
 {
     if (!check_wh(w, h, 128, 128, fn)) return;
-    if (fi > 2) fi = 0;
+    if (fi < 0 || fi > 2) fi = 0;   // the reference treats any other index as the default filter
     const int hb = taps / 2 - 1, rw = (int)w + taps - 1, rh = (int)h + taps - 1;
     const size_t inB = align_up((size_t)rw * rh * 2, 256), outB = (size_t)w * h * 2;
     CallCtx *c = call_ctx(inB + outB, inB + outB);
@@ -318,7 +318,7 @@ static void interp_s16(int taps, bool outS16, const int16_t *src, intptr_t ss, v
                        int fy, int fi, uint32_t w, uint32_t h, const char *fn)
 {
     if (!check_wh(w, h, 128, 128, fn)) return;
-    if (fi > 2) fi = 0;
+    if (fi < 0 || fi > 2) fi = 0;   // the reference treats any other index as the default filter
     const int hb = taps / 2 - 1, rh = (int)h + taps - 1;
     const size_t inB = align_up((size_t)w * rh * 2, 256), outB = (size_t)w * h * 2;
     CallCtx *c = call_ctx(inB + outB, inB + outB);

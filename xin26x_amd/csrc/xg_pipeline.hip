@@ -492,6 +492,7 @@ __global__ void k_pipe_select(PipeArgs a)
 // ---------------------------------------------------------------------------
 struct PipeState {
     int fw = 0, fh = 0, nb = 0, cap = 0;
+    uint64_t gridVersion = 0;          // session blocksVersion right after the grid was installed
     int16_t *dMv = nullptr;            // fullMv | predMv | intMv | qMv, 4 x nb x 4 int16
     uint32_t *dCost = nullptr;         // intCost | qCost, 2 x nb x 2
     hipGraph_t graph = nullptr;
@@ -566,7 +567,8 @@ int32_t xg_me_pipeline(xg_session *s, const xg_pipeline_cfg *cfg, const xg_pictu
     if ((rc = xg_session_set_picture(s, 0, cur)) != XG_OK || (rc = xg_session_set_picture(s, 1, ref0)) != XG_OK ||
         (rc = xg_session_set_picture(s, 2, ref1)) != XG_OK)
         return rc;
-    if (p->nb != nb || p->fw != fw || p->fh != fh) {
+    // (re)install the grid when the frame changed or the caller replaced the session's blocks
+    if (p->nb != nb || p->fw != fw || p->fh != fh || p->gridVersion != v.blocksVersion) {
         std::vector<xg_block> grid(nb);
         for (int b = 0; b < nb; b++) {
             xg_block &B = grid[b];
@@ -595,6 +597,7 @@ int32_t xg_me_pipeline(xg_session *s, const xg_pipeline_cfg *cfg, const xg_pictu
         p->fw = fw;
         p->fh = fh;
         session_view(s, &v);
+        p->gridVersion = v.blocksVersion;
     }
     if ((rc = refine_prepare(s, &cfg->refine, pred)) != XG_OK) return rc;
     PipeArgs &A = p->args;
@@ -628,6 +631,9 @@ int32_t xg_me_pipeline(xg_session *s, const xg_pipeline_cfg *cfg, const xg_pictu
     const uint8_t *rb = reinterpret_cast<const uint8_t *>(&cfg->refine);
     for (size_t i = 0; i < sizeof(xg_refine_cfg); i++) key = hmix(key, rb[i]);
     key = hmix(key, (uint64_t)(uintptr_t)p->dMv);
+    key = hmix(key, (uint64_t)(uintptr_t)p->dCost);
+    // the captured refinement launch: grid, task lists, tile table, per-block state
+    key = hmix(key, refine_prepared_key(s));
     if (!p->exec || key != p->key) {
         if (p->exec) (void)hipGraphExecDestroy(p->exec);
         if (p->graph) (void)hipGraphDestroy(p->graph);

@@ -770,6 +770,7 @@ bool session_view(xg_session *s, SessionView *v)
     v->hPics = s->pics;
     v->picSet = s->picSet;
     v->numCUs = s->numCUs;
+    v->blocksVersion = s->blocksVersion;
     return true;
 }
 
@@ -1377,6 +1378,22 @@ int32_t refine_prepare(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
 bool refine_enqueue_prepared(xg_session *s)
 {
     return s->preparedValid && enqueue_batched(s, s->prepared, false);
+}
+
+uint64_t refine_prepared_key(xg_session *s)
+{
+    const RefineArgs &a = s->prepared;
+    uint64_t k = mix(s->tasksKey, ((uint64_t)s->gridW << 32) | (uint64_t)s->tasks.size());
+    for (const void *ptr : {(const void *)a.pics, (const void *)a.blocks, (const void *)s->dTasksW,
+                            (const void *)s->dWgBegin, (const void *)a.candCost, (const void *)a.best,
+                            (const void *)a.bestKey, (const void *)a.done, (const void *)a.off,
+                            (const void *)a.predTab})
+        k = mix(k, (uint64_t)(uintptr_t)ptr);
+    k = mix(k, ((uint64_t)(uint32_t)a.ncand << 32) | (uint32_t)a.ctuSize);
+    k = mix(k, ((uint64_t)a.lambda << 32) | ((uint64_t)(uint32_t)a.useChroma << 1) | (uint64_t)(a.doMc != 0));
+    k = mix(k, ((uint64_t)(uint16_t)a.offMinX << 48) | ((uint64_t)(uint16_t)a.offMaxX << 32) |
+                   ((uint64_t)(uint16_t)a.offMinY << 16) | (uint64_t)(uint16_t)a.offMaxY);
+    return mix(k, (uint64_t)s->preparedValid);
 }
 
 void *&session_pipe(xg_session *s) { return s->pipe; }

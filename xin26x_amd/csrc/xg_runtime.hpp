@@ -64,6 +64,10 @@ void pipe_release(xg_session *s);
 // launched, for callers that enqueue (or capture) it among their own kernels.
 int32_t refine_prepare(xg_session *s, const xg_refine_cfg *cfg, const xg_picture *pred);
 bool refine_enqueue_prepared(xg_session *s);
+// Hash of everything the prepared refinement launch takes by value (grid,
+// task lists, per-block state and output pointers, configuration): a graph
+// that captured refine_enqueue_prepared() is valid only under the same key.
+uint64_t refine_prepared_key(xg_session *s);
 void *&session_pipe(xg_session *s);
 
 // Session kernel timers (HIP events on the session stream), by stage:
@@ -84,6 +88,7 @@ struct SessionView {
     const xg_picture *hPics;       // host picture table
     const bool *picSet;
     int numCUs;
+    uint64_t blocksVersion;        // bumped by every xg_session_set_blocks
 };
 bool session_view(xg_session *s, SessionView *v);
 constexpr int kSessionPicSlots = 64;
