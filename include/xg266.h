@@ -524,34 +524,49 @@ int32_t xg_session_set_timing(xg_session *s, int32_t enable);
  * 3. Serving: continuous batching of frames from many concurrent streams
  * ---------------------------------------------------------------------- */
 
-/* A server keeps each admitted stream's reference picture resident in HBM and
- * a dispatcher thread that, whenever a device session is free, batches the
- * oldest pending frame of every stream without a frame in flight into one
- * integer-search launch and one refinement+MC launch.  Frames that arrive
- * while a batch is on the GPU join the next batch; two sessions alternate so
- * consecutive batches overlap on the device.  A stream's first frame uses
- * itself as reference; afterwards each processed frame becomes the reference
- * of the next. */
+/* A server keeps each admitted stream's state resident in HBM: its reference
+ * and current pictures, its prediction picture, its block partition with the
+ * persistent kernel's task assignment, and an MV field of 4x4 units holding
+ * the final MVs of its last frame.  A dispatcher thread, whenever one of two
+ * device sessions is free, batches the oldest pending frame of every stream
+ * without a frame in flight into shared launches: one padding launch, one
+ * integer-search launch and one refinement+MC launch over all their blocks.
+ * Frames that arrive while a batch is on the GPU join the next batch; the two
+ * sessions alternate so consecutive batches overlap on the device.  A
+ * stream's first frame uses itself as reference; afterwards each processed
+ * frame becomes the reference of the next.
+ *
+ * Predictors are resident too: the predictor of every block (the centre of its
+ * integer search and the MVD origin of every cost) is the MV, in 1/16 pel, of
+ * the 4x4 unit at the block's centre ((x + w/2) >> 2, (y + h/2) >> 2) in the
+ * stream's MV field; every processed frame writes each block's final MV into
+ * all its units.  Before a stream's first frame the field is zero.  The MV and
+ * predictor fields of submitted blocks are ignored. */
 typedef struct xg_server xg_server;
 
 typedef struct xg_server_cfg {
     uint64_t      hbmBudget;       /* bytes of stream state the server may hold      */
     int32_t       maxStreams;      /* <= 32                                            */
-    int32_t       maxBatchBlocks;  /* blocks per batch (sizes the device sessions)     */
+    int32_t       maxBatchBlocks;  /* blocks per batch                                 */
     int32_t       searchRange;     /* integer search range in pel (<= 32)              */
     uint32_t      searchLambda;    /* sadLambda of the integer search                  */
     int32_t       maxDelayUs;      /* a free session waits up to this long after the
                                       oldest waiting frame arrived, to gather more
                                       frames into its batch (0 = launch at once)      */
     xg_refine_cfg refine;          /* fractional refinement + MC                       */
+    int32_t       minBatchFrames;  /* a free session holds its batch until this many
+                                      streams (at most the open ones) have a frame to
+                                      launch; bounded by maxDelayUs when that is set
+                                      (0 = no minimum)                                 */
 } xg_server_cfg;
 
 typedef struct xg_frame_result {
     int32_t  stream;               /* stream id                                        */
-    int32_t  status;               /* XG_OK, or XG_ERR_STATE if dropped by close       */
+    int32_t  status;               /* XG_OK; XG_ERR_STATE if dropped by close;
+                                      XG_ERR_HIP if its batch failed on the device    */
     int32_t  nblocks;
     int32_t  batchFrames;          /* frames in the batch this frame ran in            */
-    uint64_t tag;                  /* caller's tag from xg_server_submit               */
+    uint64_t tag;                  /* caller's tag from the submit call                */
     int64_t  submitNs, startNs, doneNs;   /* steady-clock times                       */
 } xg_frame_result;
 
@@ -561,19 +576,50 @@ typedef struct xg_server_stats {
     int32_t  streamsOpen;
 } xg_server_stats;
 
+/* Where the memory behind the pointers of a submitted frame lives. */
+typedef enum xg_mem_kind {
+    XG_MEM_HOST = 0,         /* pageable host memory: planes are copied before the
+                                submit call returns; outputs are written when the
+                                frame completes                                       */
+    XG_MEM_HOST_PINNED = 1,  /* page-locked host memory (hipHostMalloc): read and
+                                written by DMA; must stay valid and unchanged until
+                                the frame is reported                                  */
+    XG_MEM_DEVICE = 2        /* device memory on the server's GPU: same lifetime rule */
+} xg_mem_kind;
+
+typedef struct xg_server_frame {
+    int32_t          planesKind;   /* xg_mem_kind of planes[]                          */
+    const xg_pixel  *planes[3];    /* Y, Cb, Cr: unpadded 10-bit samples               */
+    intptr_t         strides[3];   /* in samples                                       */
+    const xg_block  *blocks;       /* host partition (x, y, w, h used), or NULL to reuse
+                                      the stream's last one                           */
+    int32_t          nblocks;
+    int32_t          outputsKind;  /* xg_mem_kind of results and pred[]                */
+    xg_block_result *results;      /* nblocks entries                                  */
+    xg_pixel        *pred[3];      /* prediction planes, or pred[0] NULL for none      */
+    intptr_t         predStride[3];/* in samples (0: the plane width)                  */
+    uint64_t         tag;
+} xg_server_frame;
+
 int32_t xg_server_create(xg_server **out, const xg_server_cfg *cfg);
 int32_t xg_server_destroy(xg_server *srv);
 /* Admission by HBM budget: XG_ERR_BUDGET if the stream's state (two padded
- * pictures, one prediction picture, block buffers) does not fit. */
+ * pictures, one prediction picture, MV field, and block, task and output
+ * buffers sized for a partition of 4x4 blocks) does not fit. */
 int32_t xg_server_open_stream(xg_server *srv, int32_t width, int32_t height, int32_t *streamId);
 /* Waits for the stream's frame in flight, drops its pending frames (reported
  * with status XG_ERR_STATE) and releases its memory. */
 int32_t xg_server_close_stream(xg_server *srv, int32_t streamId);
-/* Queues one frame (host planes Y, Cb, Cr with strides in samples, copied
- * before returning) and its blocks (host; pic/ref fields are ignored).  On
- * completion the per-block results are written to `results` (host, nblocks
- * entries, caller-owned until the frame is reported) and, when `pred` is not
- * NULL, the prediction planes to pred[0..2] (host, strides predStride[]). */
+/* Queues one frame.  Nothing is copied when planes are pinned or on the
+ * device, and a partition equal to the stream's last is neither uploaded nor
+ * re-assigned.  A new partition is validated (sides 4..64, powers of two, on
+ * the 4-sample grid, inside the picture) and assigned to the persistent
+ * workgroups on the calling thread. */
+int32_t xg_server_submit_frame(xg_server *srv, int32_t streamId, const xg_server_frame *frame);
+/* xg_server_submit_frame with pageable planes (copied before returning) and
+ * pageable outputs: per-block results to `results` (nblocks entries,
+ * caller-owned until the frame is reported) and, when `pred` is not NULL, the
+ * prediction planes to pred[0..2] (strides predStride[]). */
 int32_t xg_server_submit(xg_server *srv, int32_t streamId, const xg_pixel *const planes[3],
                          const intptr_t strides[3], const xg_block *blocks, int32_t nblocks,
                          xg_block_result *results, xg_pixel *const pred[3], const intptr_t predStride[3],

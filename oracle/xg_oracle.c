@@ -1222,6 +1222,40 @@ void or_RefineFracBlocks(const or_pixel *curY, const or_pixel *curU, const or_pi
     free(bufV);
 }
 
+/* The integer search of the serving path for blocks [i0, i1) of blk: per block
+ * the window +/-range around the predictor (1/16 pel, taken to quarter pel by
+ * >> 2 and rounded to integer pel as Xin266FillMeCand rounds,
+ * h266_analyze_inter_cu.c:271-272) with that centre first clipped into the CU
+ * MV window (h266_enc_init.c:1462-1465), then the window intersected with it;
+ * raster scan, cost (SAD << 8) + bits(4 mv - mvp / 4) * sadLambda, strict <
+ * keeps the earliest (the rule of or_MotionEstFull).  The winner goes to the
+ * block's mvx/mvy in 1/16 pel.  cur/ref point at sample (0,0) of their planes
+ * (ref padded). */
+void or_SearchBlocks(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t fw, int32_t fh,
+                     or_block *blk, int32_t i0, int32_t i1, int32_t range, uint32_t sadLambda, int32_t ctuSize)
+{
+    for (int32_t i = i0; i < i1; i++) {
+        or_block *B = &blk[i];
+        int32_t mnx, mny, mxx, mxy;
+        cu_mv_window(B->x, B->y, B->w, B->h, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
+        const int32_t pqx = B->mvpx >> 2, pqy = B->mvpy >> 2;
+        const int32_t cx = clip3((pqx + 2) >> 2, mnx, mxx), cy = clip3((pqy + 2) >> 2, mny, mxy);
+        const int32_t x0 = cx - range > mnx ? cx - range : mnx, x1 = cx + range < mxx ? cx + range : mxx;
+        const int32_t y0 = cy - range > mny ? cy - range : mny, y1 = cy + range < mxy ? cy + range : mxy;
+        uint32_t best = UINT32_MAX;
+        int32_t bx = cx, by = cy;
+        for (int32_t my = y0; my <= y1; my++)
+            for (int32_t mx = x0; mx <= x1; mx++) {
+                const uint32_t sad = or_ComputeSad(cur + B->y * cs + B->x, cs, ref + (B->y + my) * rs + B->x + mx,
+                                                   rs, (uint32_t)B->w, (uint32_t)B->h);
+                const uint32_t c = (sad << 8) + or_MvdBits(4 * mx - pqx, 4 * my - pqy) * sadLambda;
+                if (c < best) { best = c; bx = mx; by = my; }
+            }
+        B->mvx = 16 * bx;
+        B->mvy = 16 * by;
+    }
+}
+
 void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs,
                         int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range,
                         uint32_t sadLambda, int32_t ctuSize, int32_t blk0, int32_t blk1,

@@ -255,6 +255,10 @@ void or_MePipelineSample(const or_pixel *curY, const or_pixel *curU, const or_pi
                          intptr_t psy, intptr_t psc);
 
 /* Config-1 stage: integer full search for every block of a frame. */
+/* Integer search of the serving path (predictor-centred, centre clipped into
+ * the CU window) for blocks [i0, i1); writes each block's mvx/mvy. */
+void or_SearchBlocks(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t fw, int32_t fh,
+                     or_block *blk, int32_t i0, int32_t i1, int32_t range, uint32_t sadLambda, int32_t ctuSize);
 void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs,
                         int32_t frameW, int32_t frameH, int32_t bw, int32_t bh,
                         int32_t range, uint32_t sadLambda, int32_t ctuSize,

@@ -207,13 +207,16 @@ def _int_cost(cur_blk, ref, x, y, mx, my, mvp, lam):
     return ((s << 8) + (eg_bits(4 * mx - mvp[0]) + eg_bits(4 * my - mvp[1])) * lam) & U32, s
 
 
-def full_search(cur_blk, ref, x, y, win, mvp, rng, lam):
+def full_search(cur_blk, ref, x, y, win, mvp, rng, lam, clip_centre=False):
     """Raster full search (dy outer, dx inner, ascending) over +/-rng around
-    the rounded predictor ((v + 2) >> 2) inside the window; strict <.
-    Returns ((mx, my) integer pel, cost, sad)."""
+    the rounded predictor ((v + 2) >> 2, quarter pel in) inside the window;
+    strict <.  clip_centre: the rounded centre is first clipped into the
+    window (the serving search).  Returns ((mx, my) integer pel, cost, sad)."""
     h, w = cur_blk.shape
     mnx, mny, mxx, mxy = win
     cx, cy = (mvp[0] + 2) >> 2, (mvp[1] + 2) >> 2
+    if clip_centre:
+        cx, cy = min(max(cx, mnx), mxx), min(max(cy, mny), mxy)
     x0, x1 = max(cx - rng, mnx), min(cx + rng, mxx)
     y0, y1 = max(cy - rng, mny), min(cy + rng, mxy)
     pp = ref.pads[0]

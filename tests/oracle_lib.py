@@ -109,6 +109,7 @@ def load_oracle(path=None):
         "or_ChromaInterpolationS16": (None, [P, SS, P, SS, I32, U32, U32]),
         "or_RefineFracBlocks": (None, [P, P, P, SS, SS, P, P, P, SS, SS, I32, I32, P, I32,
                                        ctypes.POINTER(OrRefinePrm), P, P, P, P, P, P, P, P, SS, SS]),
+        "or_SearchBlocks": (None, [P, SS, P, SS, I32, I32, P, I32, I32, I32, U32, I32]),
         "or_FullSearchFrame": (None, [P, SS, P, SS, I32, I32, I32, I32, I32, U32, I32, I32, I32,
                                       P, P, P, P]),
     }

@@ -1,0 +1,77 @@
+"""The oracle's view of one served frame: the stream's predictors from its MV
+field, the serving integer search (or_SearchBlocks), the refinement + MC
+(or_RefineFracBlocks), and the field the frame leaves behind."""
+import ctypes
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from tests.helpers import ptr
+from tests.oracle_lib import OrRefinePrm
+from xin26x_amd import workloads as wl
+
+PAD = 80            # ctuSize + 16, the server's picture padding
+RANGE, LAMBDA = 16, 1024
+
+
+def block_array(src):
+    b = src.blocks
+    out = np.zeros((len(b), 8), np.int32)
+    for i, k in enumerate(("x", "y", "w", "h")):
+        out[:, i] = b[k]
+    return out
+
+
+def predictors(blocks, field):
+    """The MV of the 4x4 unit at each block's centre ((x + w/2) >> 2,
+    (y + h/2) >> 2) in field[uy, ux] (1/16 pel), zero when field is None."""
+    if field is None:
+        return np.zeros((len(blocks), 2), np.int32)
+    ux = (blocks[:, 0] + blocks[:, 2] // 2) >> 2
+    uy = (blocks[:, 1] + blocks[:, 3] // 2) >> 2
+    return field[uy, ux]
+
+
+def next_field(blocks, mv, field, w, h):
+    """Every block's final MV written to all of its 4x4 units."""
+    f = np.zeros((h // 4, w // 4, 2), np.int32) if field is None else field.copy()
+    for (x, y, bw, bh), (mx, my) in zip(blocks[:, :4].tolist(), mv.tolist()):
+        f[y // 4:(y + bh) // 4, x // 4:(x + bw) // 4] = (mx, my)
+    return f
+
+
+def oracle_frame(oracle, src, cur, ref, field, threads=16):
+    """(results [n, 4] as mvx, mvy, cost, cand; prediction planes; new field)
+    for one frame of `src` with current planes `cur`, reference planes `ref`
+    (the previous frame, or cur for the first) and the previous MV field."""
+    W, H = src.w, src.h
+    blocks = block_array(src)
+    blocks[:, 6:8] = predictors(blocks, field)
+    n = len(blocks)
+    refp = [np.ascontiguousarray(np.pad(p, PAD if i == 0 else PAD // 2, mode="edge")) for i, p in enumerate(ref)]
+    rsy, rsc = W + 2 * PAD, W // 2 + PAD
+    blk = np.ascontiguousarray(blocks)
+    step = max(1, (n + threads - 1) // threads)
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda i0: oracle.or_SearchBlocks(ptr(cur[0]), W, ptr(refp[0], PAD * rsy + PAD), rsy, W, H,
+                                                      ptr(blk), i0, min(n, i0 + step), RANGE, LAMBDA, 64),
+                    range(0, n, step)))
+    offx = np.ascontiguousarray(wl.CAND32[:, 0])
+    offy = np.ascontiguousarray(wl.CAND32[:, 1])
+    prm = OrRefinePrm(32, offx.ctypes.data, offy.ctypes.data, wl.LAMBDA_REFINE, 64, 1)
+    cost = np.zeros(n * 32, np.uint32)
+    bi, bc, bx, by = (np.zeros(n, t_) for t_ in (np.int32, np.uint32, np.int32, np.int32))
+    pred = [np.zeros((H, W), np.uint16), np.zeros((H // 2, W // 2), np.uint16), np.zeros((H // 2, W // 2), np.uint16)]
+
+    def part(i0):
+        i1 = min(n, i0 + step)
+        oracle.or_RefineFracBlocks(ptr(cur[0]), ptr(cur[1]), ptr(cur[2]), W, W // 2, ptr(refp[0], PAD * rsy + PAD),
+                                   ptr(refp[1], (PAD // 2) * rsc + PAD // 2), ptr(refp[2], (PAD // 2) * rsc + PAD // 2),
+                                   rsy, rsc, W, H, ptr(blk, 8 * i0), i1 - i0, ctypes.byref(prm), ptr(cost, 32 * i0),
+                                   ptr(bi, i0), ptr(bc, i0), ptr(bx, i0), ptr(by, i0), ptr(pred[0]), ptr(pred[1]),
+                                   ptr(pred[2]), W, W // 2)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(part, range(0, n, step)))
+    res = np.stack([bx, by, bc.view(np.int32), bi], axis=1)
+    return res, pred, next_field(blocks, res[:, :2], field, W, H)

@@ -1,26 +1,35 @@
-"""The serving loop: frames of several streams batched together give exactly
-the results each stream gets alone, and those equal the oracle's integer
-search + fractional refinement + MC on the same pictures (reference = the
-stream's previous frame, edge-padded).  Admission by HBM budget, malformed
-block rejection and a stream closed mid-sequence are exercised too."""
-import ctypes
+"""The serving loop at its workload: the 16-stream mix (6 x 640x360, 6 x
+1280x720, 4 x 1920x1080) for 8 distinct frames each.  Frames batched across
+all streams give exactly the results and predictions each stream gets alone;
+sampled frames of every resolution equal the oracle's predictor derivation
+(the stream's MV field), serving search, refinement and MC on the same
+pictures (reference = the stream's previous frame, edge-padded), and one
+stream is followed by the oracle through its whole sequence.  Device-resident
+and page-locked submissions give the pageable path's results.  Admission by
+HBM budget, malformed block rejection and a stream closed mid-sequence are
+exercised too."""
+import hashlib
 
 import numpy as np
 import pytest
 
-from tests.helpers import ptr
-from tests.oracle_lib import OrMe, OrRefinePrm
-from xin26x_amd import capi, workloads as wl
-from xin26x_amd.serve import Server, StreamSource, server_cfg
+from tests.serve_common import block_array, oracle_frame, predictors
+from xin26x_amd import capi
+from xin26x_amd.serve import MIX_16, DeviceFrames, Server, StreamSource, mix_sources, server_cfg
 
 pytestmark = pytest.mark.gpu
-SIZES = [(256, 128), (192, 128), (128, 64)]
-FRAMES = 4
+FRAMES = 8
+SAMPLED = {0: (2, 6), 6: (0, 4), 12: (3, 7)}     # stream -> frames checked against the oracle
 
 
 @pytest.fixture(scope="module")
-def sources():
-    return [StreamSource(w, h, 900 + i, nvar=3) for i, (w, h) in enumerate(SIZES)]
+def mix():
+    return mix_sources(seed=9)
+
+
+@pytest.fixture(scope="module")
+def small():
+    return [StreamSource(w, h, 900 + i) for i, (w, h) in enumerate([(256, 128), (192, 128), (128, 64)])]
 
 
 def _bufs(src):
@@ -30,109 +39,156 @@ def _bufs(src):
     return res, pred
 
 
+def _digest(pred):
+    h = hashlib.sha256()
+    for p in pred:
+        h.update(p.tobytes())
+    return h.hexdigest()
+
+
 def _drain(srv, n):
     got = []
-    for _ in range(400):
-        got += srv.poll(64, 50)
+    for _ in range(1200):
+        got += srv.poll(256, 50)
         if len(got) >= n:
             break
     assert len(got) == n, f"{len(got)} of {n} frames reported"
     return got
 
 
-def _run_alone(srv, src):
+def _run_alone(srv, src, frames=FRAMES, keep_pred=False):
     rc, sid = srv.open(src.w, src.h)
     assert rc == 0
     out = []
-    for t in range(FRAMES):
+    for t in range(frames):
         res, pred = _bufs(src)
         assert srv.submit(sid, src.frame(t), src.blocks, res, pred, tag=t) == 0
         r = _drain(srv, 1)[0]
-        assert r.status == 0 and r.tag == t
-        out.append((res.copy(), [p.copy() for p in pred]))
+        assert r.status == 0 and r.tag == t and r.batchFrames == 1
+        out.append((res.reshape(-1, 4).copy(), [p.copy() for p in pred] if keep_pred else None, _digest(pred)))
     assert srv.close_stream(sid) == 0
     return out
 
 
-def _oracle_frame(oracle, src, t):
-    cur = src.frame(t)
-    ref = src.frame(t - 1) if t > 0 else cur
-    W, H = src.w, src.h
-    pad = 80
-    refp = [np.ascontiguousarray(np.pad(p, pad if i == 0 else pad // 2, mode="edge")) for i, p in enumerate(ref)]
-    rsy, rsc = W + 2 * pad, W // 2 + pad
-    blocks = src.blocks
-    n = len(blocks)
-    b8 = np.zeros((n, 8), np.int32)
-    for i in range(n):
-        x, y, w, h = int(blocks["x"][i]), int(blocks["y"][i]), int(blocks["w"][i]), int(blocks["h"][i])
-        m = OrMe()
-        m.width, m.height = w, h
-        m.minMvX, m.minMvY = -x - 72, -y - 72
-        m.maxMvX, m.maxMvY = W + 72 - w - x, H + 72 - h - y
-        m.searchWidth = m.searchHeight = 16
-        m.sadLambda = 1024
-        m.input = cur[0].ctypes.data + 2 * (y * W + x)
-        m.inputStride = W
-        m.ref = refp[0].ctypes.data + 2 * ((y + pad) * rsy + x + pad)
-        m.refStride = rsy
-        oracle.or_MotionEstFull(ctypes.byref(m))
-        b8[i] = (x, y, w, h, 4 * m.bestMvX, 4 * m.bestMvY, 0, 0)
-    offx = np.ascontiguousarray(wl.CAND32[:, 0])
-    offy = np.ascontiguousarray(wl.CAND32[:, 1])
-    prm = OrRefinePrm(32, offx.ctypes.data, offy.ctypes.data, wl.LAMBDA_REFINE, 64, 1)
-    cost = np.zeros(n * 32, np.uint32)
-    bi, bc, bx, by = (np.zeros(n, t_) for t_ in (np.int32, np.uint32, np.int32, np.int32))
-    pred = [np.zeros((H, W), np.uint16), np.zeros((H // 2, W // 2), np.uint16), np.zeros((H // 2, W // 2), np.uint16)]
-    oracle.or_RefineFracBlocks(ptr(cur[0]), ptr(cur[1]), ptr(cur[2]), W, W // 2, ptr(refp[0], pad * rsy + pad),
-                               ptr(refp[1], (pad // 2) * rsc + pad // 2), ptr(refp[2], (pad // 2) * rsc + pad // 2),
-                               rsy, rsc, W, H, ptr(b8), n, ctypes.byref(prm), ptr(cost), ptr(bi), ptr(bc), ptr(bx),
-                               ptr(by), ptr(pred[0]), ptr(pred[1]), ptr(pred[2]), W, W // 2)
-    res = np.stack([bx, by, bc.view(np.int32), bi], axis=1).reshape(-1)
-    return res, pred
-
-
-def test_alone_equals_mix_equals_oracle(oracle, sources):
-    # a 20 ms gathering window: frames submitted together share batches
-    srv = Server(server_cfg(sum(len(s.blocks) for s in sources), max_delay_us=20000))
+def test_16_stream_mix_alone_equals_mix_equals_oracle(oracle, mix):
+    assert [(s.w, s.h) for s in mix] == MIX_16
+    n_all = sum(len(s.blocks) for s in mix)
+    print(f"serving mix: {len(mix)} streams, {n_all} blocks per tick, {FRAMES} frames", flush=True)
+    # every batch waits for a frame of every stream: deterministic batching
+    srv = Server(server_cfg(n_all, min_batch_frames=len(mix)))
     try:
-        alone = [_run_alone(srv, s) for s in sources]
+        alone = [_run_alone(srv, s, keep_pred=k in SAMPLED) for k, s in enumerate(mix)]
+        print("alone runs done", flush=True)
         sids = []
-        for s in sources:
+        for s in mix:
             rc, sid = srv.open(s.w, s.h)
             assert rc == 0
             sids.append(sid)
         bufs = {}
-        for t in range(FRAMES):            # interleaved, without waiting: frames batch across streams
-            for k, s in enumerate(sources):
+        for t in range(FRAMES):            # every frame of every stream offered at once
+            for k, s in enumerate(mix):
                 res, pred = _bufs(s)
                 bufs[(k, t)] = (res, pred)
                 assert srv.submit(sids[k], s.frame(t), s.blocks, res, pred, tag=t) == 0
-        got = _drain(srv, FRAMES * len(sources))
+        got = _drain(srv, FRAMES * len(mix))
         assert all(r.status == 0 for r in got)
-        assert max(r.batchFrames for r in got) > 1, "frames of different streams never shared a batch"
-        for k, s in enumerate(sources):
+        assert all(r.batchFrames == len(mix) for r in got), sorted({r.batchFrames for r in got})
+        for k, s in enumerate(mix):
             for t in range(FRAMES):
                 res, pred = bufs[(k, t)]
-                ares, apred = alone[k][t]
-                assert np.array_equal(res, ares), (k, t)
-                for p in range(3):
-                    assert np.array_equal(pred[p], apred[p]), (k, t, p)
-        for t in range(FRAMES):
-            eres, epred = _oracle_frame(oracle, sources[0], t)
-            ares, apred = alone[0][t]
-            assert np.array_equal(ares, eres), t
-            for p in range(3):
-                assert np.array_equal(apred[p], epred[p]), (t, p)
+                assert np.array_equal(res.reshape(-1, 4), alone[k][t][0]), (k, t)
+                assert _digest(pred) == alone[k][t][2], (k, t)
         st = srv.stats()
-        assert st.framesDone == FRAMES * len(sources) * 2 and st.framesDropped == 0
+        assert st.framesDone == 2 * FRAMES * len(mix) and st.framesDropped == 0
+        assert st.batches == len(mix) * FRAMES + FRAMES
+    finally:
+        srv.destroy()
+    print("mix == alone", flush=True)
+    # sampled frames of every resolution: the oracle from the GPU's previous frame (its MV field)
+    for k, frames in SAMPLED.items():
+        s = mix[k]
+        blocks = block_array(s)
+        for t in frames:
+            field = None
+            if t > 0:
+                from tests.serve_common import next_field
+                field = None
+                for u in range(t):       # the field after frame t-1 is written by every earlier frame in order
+                    field = next_field(blocks, alone[k][u][0][:, :2], field, s.w, s.h)
+            ref = s.frame(t - 1) if t > 0 else s.frame(t)
+            eres, epred, _ = oracle_frame(oracle, s, s.frame(t), ref, field)
+            assert np.array_equal(alone[k][t][0], eres), (k, t)
+            for p in range(3):
+                assert np.array_equal(alone[k][t][1][p], epred[p]), (k, t, p)
+            print(f"oracle: stream {k} ({s.w}x{s.h}) frame {t} bit-exact", flush=True)
+    # the 360p stream followed by the oracle alone through its whole sequence
+    s = mix[0]
+    field = None
+    for t in range(FRAMES):
+        eres, _, field = oracle_frame(oracle, s, s.frame(t), s.frame(t - 1) if t else s.frame(t), field)
+        assert np.array_equal(alone[0][t][0], eres), t
+    # predictors really come from the previous frame: frame 1's are frame 0's MVs at the block centres
+    assert np.any(predictors(block_array(s), field) != 0)
+
+
+def test_device_and_pinned_submissions_match_pageable(small):
+    import torch
+    src = small[0]
+    srv = Server(server_cfg(len(src.blocks)))
+    try:
+        want = _run_alone(srv, src, frames=4, keep_pred=True)
+        dev = DeviceFrames(src, 4)
+        for kind in ("device", "pinned"):
+            rc, sid = srv.open(src.w, src.h)
+            assert rc == 0
+            for t in range(4):
+                if kind == "device":
+                    res = torch.zeros(len(src.blocks) * 4, dtype=torch.int32, device="cuda")
+                    pred = [torch.zeros((src.h >> (p > 0), src.w >> (p > 0)), dtype=torch.int16, device="cuda")
+                            for p in range(3)]
+                    rc = srv.submit_frame(sid, dev.planes(t), dev.strides(), capi.XG_MEM_DEVICE, res.data_ptr(),
+                                          capi.XG_MEM_DEVICE, src.blocks if t == 0 else None,
+                                          [p.data_ptr() for p in pred], tag=t)
+                else:
+                    planes = [torch.from_numpy(p.view(np.int16)).pin_memory() for p in src.frame(t)]
+                    res = torch.zeros(len(src.blocks) * 4, dtype=torch.int32, pin_memory=True)
+                    pred = [torch.zeros((src.h >> (p > 0), src.w >> (p > 0)), dtype=torch.int16, pin_memory=True)
+                            for p in range(3)]
+                    rc = srv.submit_frame(sid, [p.data_ptr() for p in planes], [src.w, src.w // 2, src.w // 2],
+                                          capi.XG_MEM_HOST_PINNED, res.data_ptr(), capi.XG_MEM_HOST_PINNED,
+                                          src.blocks if t == 0 else None, [p.data_ptr() for p in pred], tag=t)
+                assert rc == 0
+                r = _drain(srv, 1)[0]
+                assert r.status == 0 and r.tag == t
+                assert np.array_equal(res.cpu().numpy().reshape(-1, 4), want[t][0]), (kind, t)
+                for p in range(3):
+                    assert np.array_equal(pred[p].cpu().numpy().view(np.uint16), want[t][1][p]), (kind, t, p)
+            assert srv.close_stream(sid) == 0
     finally:
         srv.destroy()
 
 
-def test_admission_by_budget(sources):
+def test_reused_partition_needs_a_first_one(small):
     lib = capi.load()
-    s0 = sources[0]
+    src = small[2]
+    srv = Server(server_cfg(4096))
+    try:
+        rc, sid = srv.open(src.w, src.h)
+        assert rc == 0
+        res, _ = _bufs(src)
+        f = src.frame(0)
+        rc = srv.submit_frame(sid, [p.ctypes.data for p in f], [src.w, src.w // 2, src.w // 2], capi.XG_MEM_HOST,
+                              res.ctypes.data, capi.XG_MEM_HOST, None)
+        assert rc == capi.XG_ERR_STATE
+        assert srv.stats().framesDone == 0
+    finally:
+        lib.xg_clear_error()
+        srv.destroy()
+
+
+def test_admission_by_budget(small):
+    lib = capi.load()
+    s0 = small[0]
     probe = Server(server_cfg(4096))
     base = probe.stats().bytesInUse
     rc, sid = probe.open(s0.w, s0.h)
@@ -156,9 +212,9 @@ def test_admission_by_budget(sources):
         srv.destroy()
 
 
-def test_malformed_blocks_rejected(sources):
+def test_malformed_blocks_rejected(small):
     lib = capi.load()
-    s0 = sources[0]
+    s0 = small[0]
     srv = Server(server_cfg(4096))
     try:
         rc, sid = srv.open(s0.w, s0.h)
@@ -178,29 +234,29 @@ def test_malformed_blocks_rejected(sources):
         srv.destroy()
 
 
-def test_close_mid_sequence_leaves_others_intact(sources):
+def test_close_mid_sequence_leaves_others_intact(small):
     lib = capi.load()
-    a_src, b_src = sources[0], sources[1]
-    srv = Server(server_cfg(sum(len(s.blocks) for s in sources)))
+    a_src, b_src = small[0], small[1]
+    srv = Server(server_cfg(sum(len(s.blocks) for s in small)))
     try:
-        b_alone = _run_alone(srv, b_src)
+        b_alone = _run_alone(srv, b_src, frames=4)
         rc, a = srv.open(a_src.w, a_src.h)
         rc2, b = srv.open(b_src.w, b_src.h)
         assert rc == 0 and rc2 == 0
         keep = []
-        for t in range(FRAMES):
+        for t in range(4):
             ra, _ = _bufs(a_src)
             rb, pb = _bufs(b_src)
             keep.append((ra, rb, pb))
             assert srv.submit(a, a_src.frame(t), a_src.blocks, ra, tag=100 + t) == 0
             assert srv.submit(b, b_src.frame(t), b_src.blocks, rb, pb, tag=t) == 0
         assert srv.close_stream(a) == 0                # drops what has not started
-        got = _drain(srv, 2 * FRAMES)
+        got = _drain(srv, 8)
         for r in got:
             if r.stream == b:
                 assert r.status == 0
-                res_b, pred_b = keep[r.tag][1], keep[r.tag][2]
-                assert np.array_equal(res_b, b_alone[r.tag][0])
+                assert np.array_equal(keep[r.tag][1].reshape(-1, 4), b_alone[r.tag][0])
+                assert _digest(keep[r.tag][2]) == b_alone[r.tag][2]
             else:
                 assert r.status in (0, capi.XG_ERR_STATE)
         assert srv.submit(a, a_src.frame(0), a_src.blocks, keep[0][0]) == capi.XG_ERR_STATE

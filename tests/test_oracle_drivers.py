@@ -278,6 +278,44 @@ def test_diamond_rebuild(oracle):
     check_diamond(oracle)
 
 
+def check_search_blocks(lib):
+    """The serving search (or_SearchBlocks) against the numpy full search with
+    the centre clipped into the CU window: QTMT blocks of a smooth frame,
+    predictors near the motion, far outside the window (the clip decides the
+    window) and on 1/16-pel values whose rounding to integer pel matters."""
+    rng = np.random.default_rng(12)
+    W, H = 192, 128
+    ref = wl.smooth_field(H, W, 44).astype(np.uint16)
+    cur = np.clip(np.roll(ref, (3, -5), axis=(0, 1)).astype(np.int64) + rng.integers(-3, 4, (H, W)), 0,
+                  1023).astype(np.uint16)
+    parts = []
+    for cy in range(0, H, 64):
+        for cx in range(0, W, 64):
+            parts += wl.qtmt_partition(cx, cy, W, H, 77 + cy * 3 + cx)
+    b = np.zeros((len(parts), 8), np.int32)
+    b[:, :4] = np.array(parts)
+    n = len(b)
+    kind = np.arange(n) % 3
+    b[:, 6] = np.where(kind == 0, 16 * 5 + rng.integers(-40, 41, n), np.where(kind == 1, rng.choice([-9000, 9000], n),
+                                                                          rng.integers(-300, 301, n)))
+    b[:, 7] = np.where(kind == 0, -16 * 3 + rng.integers(-40, 41, n), rng.integers(-2000, 2001, n))
+    blk = np.ascontiguousarray(b)
+    refp = np.ascontiguousarray(np.pad(ref, PAD, mode="edge"))
+    rs = W + 2 * PAD
+    lib.or_SearchBlocks(ptr(cur), W, ptr(refp, PAD * rs + PAD), rs, W, H, ptr(blk), 0, n, 6, 1024, 64)
+    rpic = nr.Pic([ref, ref[::2, ::2], ref[::2, ::2]], PAD)
+    for i in range(n):
+        x, y, w, h, _mx, _my, px, py = (int(v) for v in b[i])
+        win = nr.cu_window(x, y, w, h, W, H, 64)
+        (mx, my), _c, _s = nr.full_search(cur[y:y + h, x:x + w], rpic, x, y, win, (px >> 2, py >> 2), 6, 1024,
+                                          clip_centre=True)
+        assert (int(blk[i, 4]), int(blk[i, 5])) == (16 * mx, 16 * my), (i, b[i])
+
+
+def test_search_blocks_rebuild(oracle):
+    check_search_blocks(oracle)
+
+
 def check_pipeline_cands(lib):
     """The pipeline's predictor and start list for every block of random
     stage-1 MV fields (small values, so neighbours often coincide and the
@@ -384,6 +422,12 @@ MUTANTS = {
         "static int32_t median3(int32_t a, int32_t b, int32_t c)\n{\n"
         "    if (1) { int32_t l = a < b ? a : b, h = a < b ? b : a; l = c < l ? c : l; h = c > h ? c : h; "
         "return (l + h) / 2; }", "cands"),
+    "serving search: centre not clipped into the window": (
+        "const int32_t cx = clip3((pqx + 2) >> 2, mnx, mxx), cy = clip3((pqy + 2) >> 2, mny, mxy);",
+        "const int32_t cx = (pqx + 2) >> 2, cy = (pqy + 2) >> 2;", "search"),
+    "serving search: predictor taken as integer pel without rounding": (
+        "const int32_t pqx = B->mvpx >> 2, pqy = B->mvpy >> 2;",
+        "const int32_t pqx = (B->mvpx >> 4) << 2, pqy = (B->mvpy >> 4) << 2;", "search"),
     "diamond: R before L": ("static const int8_t dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};",
                             "static const int8_t dxs[4] = {1, -1, 0, 0}, dys[4] = {0, 0, -1, 1};", "diamond"),
     "diamond: candidates evaluated last to first": ("    for (uint32_t i = 0; i < me->candNum; i++) {\n"
@@ -396,7 +440,7 @@ MUTANTS = {
                                                    "const int32_t r = qcost2[1] <= qcost2[0] ? 1 : 0;", "pipeline"),
 }
 CHECKS = {"refine": lambda lib: check_refine(lib, nblocks=160), "subpel": check_subpel, "pipeline": check_pipeline,
-          "diamond": check_diamond, "cands": check_pipeline_cands}
+          "diamond": check_diamond, "cands": check_pipeline_cands, "search": check_search_blocks}
 
 
 @pytest.fixture(scope="module")

@@ -46,7 +46,17 @@ class XgBlockResult(ctypes.Structure):
 
 class XgServerCfg(ctypes.Structure):
     _fields_ = [("hbmBudget", ctypes.c_uint64), ("maxStreams", I32), ("maxBatchBlocks", I32),
-                ("searchRange", I32), ("searchLambda", U32), ("maxDelayUs", I32), ("refine", XgRefineCfg)]
+                ("searchRange", I32), ("searchLambda", U32), ("maxDelayUs", I32), ("refine", XgRefineCfg),
+                ("minBatchFrames", I32)]
+
+
+XG_MEM_HOST, XG_MEM_HOST_PINNED, XG_MEM_DEVICE = range(3)
+
+
+class XgServerFrame(ctypes.Structure):
+    _fields_ = [("planesKind", I32), ("planes", ctypes.c_void_p * 3), ("strides", ctypes.c_ssize_t * 3),
+                ("blocks", ctypes.c_void_p), ("nblocks", I32), ("outputsKind", I32), ("results", ctypes.c_void_p),
+                ("pred", ctypes.c_void_p * 3), ("predStride", ctypes.c_ssize_t * 3), ("tag", ctypes.c_uint64)]
 
 
 class XgPipelineCfg(ctypes.Structure):
@@ -182,6 +192,7 @@ PROTOS = {
     "xg_server_open_stream": (I32, [P, I32, I32, ctypes.POINTER(I32)]),
     "xg_server_close_stream": (I32, [P, I32]),
     "xg_server_submit": (I32, [P, I32, P, P, P, I32, P, P, P, ctypes.c_uint64]),
+    "xg_server_submit_frame": (I32, [P, I32, ctypes.POINTER(XgServerFrame)]),
     "xg_server_poll": (I32, [P, ctypes.POINTER(XgFrameResult), I32, I32]),
     "xg_server_stats_get": (I32, [P, ctypes.POINTER(XgServerStats)]),
     "xg_me_pipeline": (I32, [P, ctypes.POINTER(XgPipelineCfg), ctypes.POINTER(XgPicture), ctypes.POINTER(XgPicture),

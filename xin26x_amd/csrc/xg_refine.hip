@@ -78,6 +78,8 @@ struct RefineArgs {
     const int16_t *off;            // device: offx[XG_MAX_CAND] then offy[XG_MAX_CAND]
     int doMc;                      // 0: costs only (tile timing), no block completion or MC
     const xg_picture *predTab;     // per picture slot: prediction planes (unpadded) the MC writes
+    int2 *mvField;                 // when set: the winner's MV is also written to every 4x4 unit
+    int mvStride;                  //   of the block in this field (units per row)
 };
 
 // CU MV window in 1/16 pel (h266_enc_init.c:1462-1465, int16 as stored there).
@@ -550,6 +552,11 @@ __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_bl
         r.cand = c;
         a.best[b] = r;
     }
+    if (a.mvField) {
+        const int uw = B.w >> 2, nu = uw * (B.h >> 2);
+        int2 *f = a.mvField + (intptr_t)(B.y >> 2) * a.mvStride + (B.x >> 2);
+        for (int u = lane; u < nu; u += 64) f[(intptr_t)(u / uw) * a.mvStride + u % uw] = make_int2(mvx, mvy);
+    }
     const int sw = min(B.w, kSub), sh = min(B.h, kSub);
     if (sw == B.w && sh == B.h) {
         mc_sub(a, B, mvx, mvy, B.x, B.y, sw, sh, last, L);
@@ -669,6 +676,35 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     for (int t = a.wgBegin[blockIdx.x]; t < e; t++) {
         const int2 tk = a.tasks[t];
         refine_one(a, tk, a.blocks[tk.x], L);
+    }
+}
+
+// Multi-list form (serving): one persistent grid over the frames of several
+// streams.  Each list carries its own blocks, workgroup-major task assignment
+// and outputs; workgroup g runs, list after list, the tasks of virtual
+// workgroup (g + rot) mod grid, so small lists are spread over the grid.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_refine_multi(RefineArgs base, const RefineList *__restrict__ lists, int nLists, int grid)
+{
+    __shared__ RefineLds L;
+    lds_init(L, base);
+    for (int l = 0; l < nLists; l++) {
+        const RefineList R = lists[l];
+        RefineArgs a = base;
+        a.blocks = R.blocks;
+        a.candCost = R.candCost;
+        a.best = R.best;
+        a.bestKey = R.bestKey;
+        a.done = R.done;
+        a.mvField = R.mvField;
+        a.mvStride = R.mvStride;
+        int v = (int)blockIdx.x + R.rot;
+        if (v >= grid) v -= grid;
+        const int e = R.wgBegin[v + 1];
+        for (int t = R.wgBegin[v]; t < e; t++) {
+            const int2 tk = R.tasks[t];
+            refine_one(a, tk, R.blocks[tk.x], L);
+        }
     }
 }
 
@@ -823,15 +859,15 @@ static long task_cost(const xg_block &B, int nc)
 // the workgroups g with g % nregions == r (workgroup g runs on XCD g % 8, so a
 // band's reference rows stay in one XCD's L2), and inside a band each task,
 // heaviest first, goes to the least loaded workgroup.
-static void assign_tasks(const xg_session *s, const std::vector<int2> &tasks, int G, std::vector<int2> &out,
-                         std::vector<int> &begin)
+static void assign_tasks(const xg_block *blocks, int nregions, const std::vector<int2> &tasks, int G,
+                         std::vector<int2> &out, std::vector<int> &begin)
 {
-    const int R = std::max(1, std::min(s->nregions, G));
+    const int R = std::max(1, std::min(nregions, G));
     const size_t n = tasks.size();
     std::vector<long> cost(n);
     long total = 0;
     for (size_t i = 0; i < n; i++) {
-        cost[i] = task_cost(s->blocks[tasks[i].x], tasks[i].y >> 16);
+        cost[i] = task_cost(blocks[tasks[i].x], tasks[i].y >> 16);
         total += cost[i];
     }
     // bands: tasks ordered by picture position, cut at equal cost
@@ -840,7 +876,7 @@ static void assign_tasks(const xg_session *s, const std::vector<int2> &tasks, in
         std::vector<size_t> pos(n);
         for (size_t i = 0; i < n; i++) pos[i] = i;
         std::stable_sort(pos.begin(), pos.end(), [&](size_t x, size_t y) {
-            const xg_block &A = s->blocks[tasks[x].x], &B = s->blocks[tasks[y].x];
+            const xg_block &A = blocks[tasks[x].x], &B = blocks[tasks[y].x];
             return A.pic != B.pic ? A.pic < B.pic : A.y != B.y ? A.y < B.y : A.x < B.x;
         });
         long acc = 0;
@@ -872,6 +908,32 @@ static void assign_tasks(const xg_session *s, const std::vector<int2> &tasks, in
     begin[G] = (int)out.size();
 }
 
+// Tasks of `n` blocks (per-shape candidates per task from the tile table),
+// shapes heaviest first: shape-major list and each shape's first task.
+static void make_tasks(const int *tile, const xg_block *blocks, int n, int ncand, std::vector<int2> &tasks,
+                       int st[kNumShapes + 1], std::vector<int> &order)
+{
+    std::vector<std::vector<int>> per(kNumShapes);
+    for (int b = 0; b < n; b++) per[shape_of(blocks[b].w, blocks[b].h)].push_back(b);
+    order.resize(kNumShapes);
+    for (int i = 0; i < kNumShapes; i++) order[i] = i;
+    auto weight = [&](int sh) {
+        const int w = 4 << (sh / 5), h = 4 << (sh % 5);
+        return (long)w * h * std::min(tile[sh], ncand);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return weight(x) > weight(y); });
+    tasks.clear();
+    for (int oi = 0; oi < kNumShapes; oi++) {
+        const int sh = order[oi];
+        st[oi] = (int)tasks.size();
+        const int cpt = std::max(1, std::min(tile[sh], ncand));
+        for (int b : per[sh])
+            for (int c0 = 0; c0 < ncand; c0 += cpt)
+                tasks.push_back(make_int2(b, c0 | (std::min(cpt, ncand - c0) << 16)));
+    }
+    st[kNumShapes] = (int)tasks.size();
+}
+
 static bool build_tasks(xg_session *s, int ncand)
 {
     const int G = std::max(1, s->numCUs * s->persistFactor);
@@ -879,32 +941,12 @@ static bool build_tasks(xg_session *s, int ncand)
     for (int i = 0; i < kNumShapes; i++) key = mix(key, (uint64_t)s->tile[i]);
     key = mix(key, ((uint64_t)s->nregions << 32) | (uint64_t)G);
     if (key == s->tasksKey) return true;
-    std::vector<std::vector<int>> per(kNumShapes);
-    for (int b = 0; b < s->nblocks; b++) per[shape_of(s->blocks[b].w, s->blocks[b].h)].push_back(b);
-    // order shapes by task weight (samples per task), heaviest first
-    std::vector<int> order(kNumShapes);
-    for (int i = 0; i < kNumShapes; i++) order[i] = i;
-    auto weight = [&](int sh) {
-        const int w = 4 << (sh / 5), h = 4 << (sh % 5);
-        return (long)w * h * std::min(s->tile[sh], ncand);
-    };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return weight(x) > weight(y); });
-    s->tasks.clear();
     int st[kNumShapes + 1];
-    for (int oi = 0; oi < kNumShapes; oi++) {
-        const int sh = order[oi];
-        st[oi] = (int)s->tasks.size();
-        const int cpt = std::max(1, std::min(s->tile[sh], ncand));
-        for (int b : per[sh])
-            for (int c0 = 0; c0 < ncand; c0 += cpt)
-                s->tasks.push_back(make_int2(b, c0 | (std::min(cpt, ncand - c0) << 16)));
-    }
-    st[kNumShapes] = (int)s->tasks.size();
+    make_tasks(s->tile, s->blocks.data(), s->nblocks, ncand, s->tasks, st, s->shapeOrder);
     memcpy(s->shapeTaskBegin, st, sizeof(st));
-    s->shapeOrder = order;
     std::vector<int2> tasksW;
     std::vector<int> wgBegin;
-    assign_tasks(s, s->tasks, G, tasksW, wgBegin);
+    assign_tasks(s->blocks.data(), s->nregions, s->tasks, G, tasksW, wgBegin);
     if (s->tasks.size() > s->taskCap || G + 1 > s->wgCap) {
         if (!XG_HIP(hipStreamSynchronize(s->stream))) return false;
         (void)hipFree(s->dTasks);
@@ -1378,6 +1420,31 @@ int32_t refine_prepare(xg_session *s, const xg_refine_cfg *cfg, const xg_picture
 bool refine_enqueue_prepared(xg_session *s)
 {
     return s->preparedValid && enqueue_batched(s, s->prepared, false);
+}
+
+int refine_grid(xg_session *s) { return std::max(1, s->numCUs * s->persistFactor); }
+
+void refine_assign(xg_session *s, const xg_block *blocks, int n, int ncand, std::vector<int2> &tasksW,
+                   std::vector<int> &wgBegin)
+{
+    std::vector<int2> tasks;
+    std::vector<int> order;
+    int st[kNumShapes + 1];
+    make_tasks(s->tile, blocks, n, ncand, tasks, st, order);
+    assign_tasks(blocks, s->nregions, tasks, refine_grid(s), tasksW, wgBegin);
+}
+
+int32_t refine_multi_enqueue(xg_session *s, const xg_refine_cfg *cfg, const RefineList *dLists, int nLists)
+{
+    if (!validate_cfg(cfg)) return XG_ERR_ARG;
+    if (nLists <= 0) return XG_OK;
+    RefineArgs base;
+    if (!fill_args(s, cfg, base, true)) return XG_ERR_HIP;
+    int ev[2];
+    const bool timed = session_timer_begin(s, 0, ev);
+    k_refine_multi<<<refine_grid(s), 64, 0, s->stream>>>(base, dLists, nLists, refine_grid(s));
+    if (timed) session_timer_end(s, 0, ev);
+    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
 uint64_t refine_prepared_key(xg_session *s)

@@ -7,6 +7,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/xg266.h"
 
 namespace xg {
@@ -69,6 +71,46 @@ bool refine_enqueue_prepared(xg_session *s);
 // that captured refine_enqueue_prepared() is valid only under the same key.
 uint64_t refine_prepared_key(xg_session *s);
 void *&session_pipe(xg_session *s);
+
+// One frame's blocks in a multi-list refinement launch (serving): several
+// frames of several streams share one persistent grid, each with its own
+// resident block list, task assignment and outputs.
+struct RefineList {
+    const xg_block *blocks;            // device block list (slots, base MV, predictor)
+    const int2 *tasks;                 // workgroup-major tasks for the session's grid
+    const int *wgBegin;                // grid + 1 entries
+    uint32_t *candCost;                // [n * ncand]
+    xg_block_result *best;             // [n]
+    unsigned long long *bestKey;       // [n], all-ones when idle
+    int *done;                         // [n], zero when idle
+    int2 *mvField;                     // 4x4-unit MV field the winners are written to (or null)
+    int mvStride;                      // units per field row
+    int rot;                           // workgroup g runs virtual workgroup (g + rot) mod grid
+};
+// One frame's blocks in a multi-list integer-search launch (serving).
+struct SearchList {
+    xg_block *blocks;                  // device; slots, predictor and base MV are written here
+    const int2 *mvField;               // the stream's 4x4-unit MV field (predictor source)
+    int mvStride;                      // units per field row
+    int begin;                         // first workgroup of this list in the launch
+    uint8_t curSlot, refSlot;          // picture slots of the frame and its reference
+};
+// Enqueues k_search_multi over lists[0..nLists) (device) covering nBlocks
+// blocks in all on `st`: each block's predictor from its stream's MV field,
+// then the search +/-range around it.  maxW/maxH: the largest block sides.
+int32_t search_multi_enqueue(hipStream_t st, xg_session *timer, const xg_picture *dPics, const SearchList *dLists,
+                             int nLists, int nBlocks, int maxW, int maxH, int range, uint32_t lambda, int ctu);
+
+// Persistent workgroups of the session's batched form (CUs x workgroups per CU).
+int refine_grid(xg_session *s);
+// Workgroup-major tasks of `n` blocks for the session's grid, tile table and
+// bands, as the batched form builds them for its own block list.
+void refine_assign(xg_session *s, const xg_block *blocks, int n, int ncand, std::vector<int2> &tasksW,
+                   std::vector<int> &wgBegin);
+// Enqueues one k_refine_multi launch over device lists[0..nLists) on the
+// session stream, with the configuration's offsets and the session's picture
+// and prediction tables.
+int32_t refine_multi_enqueue(xg_session *s, const xg_refine_cfg *cfg, const RefineList *dLists, int nLists);
 
 // Session kernel timers (HIP events on the session stream), by stage:
 // 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search,

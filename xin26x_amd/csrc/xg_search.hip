@@ -415,18 +415,15 @@ __global__ void __launch_bounds__(256) k_subpel_one(const uint16_t *reg, int rst
 
 static inline int clampi_h(int v, int lo, int hi) { return std::min(std::max(v, lo), hi); }
 
-// Integer search of every block of a session (one workgroup per block): the
-// window +/-range around the rounded predictor, clipped to the CU MV window,
-// staged in LDS with the block; result written back as the block's base MV.
-// Integer search of every session block around its predictor (packed core;
-// ws: window row pitch for the widest block).
-__global__ void __launch_bounds__(1024) k_search_blocks(const xg_picture *pics, xg_block *blocks, int range,
-                                                        uint32_t lambda, int ctu, int ws)
+// Integer search of one block (its workgroup): the window +/-range around the
+// rounded predictor, clipped to the CU MV window, staged in LDS with the block
+// (packed core; ws: window row pitch for the widest block).  Every thread
+// returns the winner as the block's base MV (1/16 pel) in B.mvx / B.mvy.
+__device__ __forceinline__ void search_block(const xg_picture *pics, xg_block &B, int range, uint32_t lambda, int ctu,
+                                             int ws)
 {
     extern __shared__ uint32_t smem32[];
     __shared__ unsigned long long red[16];
-    const int b = blockIdx.x;
-    xg_block B = blocks[b];
     const xg_picture &P = pics[B.pic], &R = pics[B.ref];
     const int fw = P.plane[0].width, fh = P.plane[0].height;
     const int mnx = (int16_t)(-B.x - ctu - 8), mny = (int16_t)(-B.y - ctu - 8);
@@ -453,13 +450,46 @@ __global__ void __launch_bounds__(1024) k_search_blocks(const xg_picture *pics, 
     default: k = packed_items<64>(smem32, ws, curL, B.h, x0, x1, y0, nx, ny, lambda, pmx, pmy); break;
     }
     k = block_min_u64_16(k, red);
+    const int i = (int)(uint32_t)k;
+    B.mvx = 16 * (x0 + i % nx);
+    B.mvy = 16 * (y0 + i / nx);
+}
+
+// Integer search of every session block around its predictor; the result is
+// written back as the block's base MV.
+__global__ void __launch_bounds__(1024) k_search_blocks(const xg_picture *pics, xg_block *blocks, int range,
+                                                        uint32_t lambda, int ctu, int ws)
+{
+    const int b = blockIdx.x;
+    xg_block B = blocks[b];
+    search_block(pics, B, range, lambda, ctu, ws);
     if (threadIdx.x == 0) {
-        const int i = (int)(uint32_t)k;
-        B.mvx = 16 * (x0 + i % nx);
-        B.mvy = 16 * (y0 + i / nx);
         blocks[b].mvx = B.mvx;
         blocks[b].mvy = B.mvy;
     }
+}
+
+// Multi-list form (serving): the blocks of several streams' frames in one
+// launch.  Workgroup w belongs to the last list whose `begin` is <= w.  Each
+// block first takes its frame's picture slots and, as predictor, the MV of
+// the 4x4 unit at its centre in the stream's MV field (the final MVs of the
+// stream's previous frame; zero before the first); the whole updated block
+// (slots, predictor, base MV) is written back for the refinement.
+__global__ void __launch_bounds__(1024) k_search_multi(const xg_picture *pics, const SearchList *__restrict__ lists,
+                                                       int nLists, int range, uint32_t lambda, int ctu, int ws)
+{
+    int l = 0;
+    while (l + 1 < nLists && (int)blockIdx.x >= lists[l + 1].begin) l++;
+    const SearchList S = lists[l];
+    const int b = (int)blockIdx.x - S.begin;
+    xg_block B = S.blocks[b];
+    B.pic = S.curSlot;
+    B.ref = S.refSlot;
+    const int2 p = S.mvField[(intptr_t)((B.y + (B.h >> 1)) >> 2) * S.mvStride + ((B.x + (B.w >> 1)) >> 2)];
+    B.mvpx = p.x;
+    B.mvpy = p.y;
+    search_block(pics, B, range, lambda, ctu, ws);
+    if (threadIdx.x == 0) S.blocks[b] = B;
 }
 
 }  // namespace xg
@@ -687,6 +717,23 @@ int32_t xg::full_search_frame_on(hipStream_t st, xg_session *timer, const xg_pla
     } else {
         k_full_search_frame<<<nbx * nby, 256, lds, st>>>(a);
     }
+    if (timed) session_timer_end(timer, 3, ev);
+    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
+}
+
+int32_t xg::search_multi_enqueue(hipStream_t st, xg_session *timer, const xg_picture *dPics, const SearchList *dLists,
+                                 int nLists, int nBlocks, int maxW, int maxH, int range, uint32_t lambda, int ctu)
+{
+    if (nLists <= 0 || nBlocks <= 0) return XG_OK;
+    const int ws = packed_ws(2 * range + 1, maxW);
+    const size_t lds = ((size_t)(2 * range + maxH) * ws + (size_t)maxW * maxH) * 2 + 16;
+    if (lds > 64 * 1024) {
+        set_error(XG_ERR_ARG, "search_multi_enqueue: window exceeds the LDS budget");
+        return XG_ERR_ARG;
+    }
+    int ev[2];
+    const bool timed = timer && session_timer_begin(timer, 3, ev);
+    k_search_multi<<<nBlocks, packed_threads(2 * range + 1), lds, st>>>(dPics, dLists, nLists, range, lambda, ctu, ws);
     if (timed) session_timer_end(timer, 3, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }

@@ -1,16 +1,25 @@
 // xg_serve.hip -- serving: continuous batching of frames from many streams.
 //
-// Each admitted stream owns two padded pictures in HBM (the current frame and
-// its reference, which swap roles after every frame) and one prediction
-// picture.  Submitted frames are copied to pinned host memory and queued per
-// stream.  A dispatcher thread keeps up to two batches in flight, one per
-// device session (each with its own HIP stream): whenever a session is free it
-// takes the oldest pending frame of every stream that has no frame in flight,
-// uploads the frames, pads them on the GPU, and enqueues one integer-search
-// launch and one refinement+MC launch over all their blocks.  Frames arriving
-// while a batch runs join the next one; with maxDelayUs > 0 a free session
-// also waits that long after the oldest waiting frame arrived.  Results are copied back to the
-// caller's buffers when the batch's event completes.
+// Every admitted stream owns one HBM allocation that stays resident for its
+// life: two padded pictures (the current frame and its reference, which swap
+// roles after every frame), the prediction picture, a 4x4-unit MV field (the
+// final MVs of its last frame, the source of the next frame's predictors),
+// its block partition with the persistent kernel's task assignment, and the
+// per-block outputs.  A partition is uploaded and assigned to workgroups only
+// when a frame brings a new one.
+//
+// Frames are submitted without staging when the caller's planes are in
+// page-locked host memory or on the device (xg_server_submit_frame): the
+// dispatcher reads them by DMA.  A dispatcher thread keeps up to two batches
+// in flight, one per device session (each with its own HIP stream).  When a
+// session is free it takes the oldest pending frame of every stream that has
+// no frame in flight and enqueues, for all of them together: the plane copies,
+// one padding launch, one integer-search launch (k_search_multi: each block's
+// predictor read from its stream's MV field, then +/-range around it) and one
+// refinement + MC launch (k_refine_multi, which also writes the winners into
+// the MV fields), then the result copies.  The launches walk a per-batch
+// table of per-stream descriptors, so a batch costs the host a few hundred
+// bytes of descriptors, not a merged block list.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +29,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <stddef.h>
 #include <string.h>
 #include <thread>
 #include <vector>
@@ -34,15 +44,32 @@ static int64_t now_ns()
         .count();
 }
 
-// Edge replication of the padding ring of one plane (data points at sample (0,0)).
-__global__ void k_pad_plane(uint16_t *data, intptr_t stride, int w, int h, int pad)
+// One plane whose padding ring is filled by edge replication.
+struct PadPlane {
+    uint16_t *data;                    // sample (0, 0)
+    intptr_t stride;
+    int w, h, pad;
+};
+
+// One workgroup per padded row of one plane (blockIdx.y = plane): rows above
+// and below the picture are filled across their whole width, rows inside it
+// only in their left and right padding.
+__global__ void __launch_bounds__(256) k_pad_multi(const PadPlane *__restrict__ planes)
 {
-    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x) - pad;
-    const int y = (int)blockIdx.y - pad;
-    if (x >= w + pad) return;
-    if (x >= 0 && x < w && y >= 0 && y < h) return;
-    const int sx = min(max(x, 0), w - 1), sy = min(max(y, 0), h - 1);
-    data[(intptr_t)y * stride + x] = data[(intptr_t)sy * stride + sx];
+    const PadPlane P = planes[blockIdx.y];
+    const int y = (int)blockIdx.x - P.pad;
+    if (y >= P.h + P.pad) return;
+    const int sy = min(max(y, 0), P.h - 1);
+    uint16_t *row = P.data + (intptr_t)y * P.stride;
+    const uint16_t *src = P.data + (intptr_t)sy * P.stride;
+    if (y < 0 || y >= P.h) {
+        for (int x = (int)threadIdx.x - P.pad; x < P.w + P.pad; x += 256) row[x] = src[min(max(x, 0), P.w - 1)];
+    } else {
+        for (int i = threadIdx.x; i < 2 * P.pad; i += 256) {
+            const int x = i < P.pad ? i - P.pad : P.w + (i - P.pad);
+            row[x] = src[i < P.pad ? 0 : P.w - 1];
+        }
+    }
 }
 
 namespace {
@@ -54,12 +81,37 @@ struct Request {
     int stream = 0;
     uint64_t tag = 0;
     int64_t submitNs = 0;
+    int planesKind = XG_MEM_HOST;
+    const uint16_t *planes[3] = {nullptr, nullptr, nullptr};   // caller's (pinned or device)
+    intptr_t strides[3] = {0, 0, 0};
+    uint16_t *staged = nullptr;        // XG_MEM_HOST planes: Y | Cb | Cr packed at submit
+    size_t stagedBytes = 0;
+    uint16_t *predStage = nullptr;     // pageable prediction outputs: read back here first
+    size_t predStageBytes = 0;
+    bool newPartition = false;         // this frame brings a partition (uploaded at launch)
     std::vector<xg_block> blocks;
-    uint16_t *pinned = nullptr;        // Y | Cb | Cr packed; reused for the prediction read-back
-    size_t pinnedBytes = 0;
+    std::vector<int2> tasksW;
+    std::vector<int> wgBegin;
+    int nblocks = 0;
+    int outputsKind = XG_MEM_HOST;
     xg_block_result *results = nullptr;
     xg_pixel *pred[3] = {nullptr, nullptr, nullptr};
     intptr_t predStride[3] = {0, 0, 0};
+};
+
+// Device state of a stream, carved from one allocation.
+struct StreamDev {
+    xg_picture pic[2];                 // padded current / reference pictures
+    xg_picture pred;                   // unpadded prediction picture
+    int2 *field = nullptr;             // 4x4-unit MV field, fieldStride units per row
+    int fieldStride = 0;
+    xg_block *blocks = nullptr;        // partition (slots, predictor and base MV rewritten per frame)
+    int2 *tasks = nullptr;             // workgroup-major task assignment
+    int *wgBegin = nullptr;
+    uint32_t *candCost = nullptr;
+    xg_block_result *best = nullptr;
+    unsigned long long *bestKey = nullptr;
+    int *done = nullptr;
 };
 
 struct Stream {
@@ -67,19 +119,37 @@ struct Stream {
     int w = 0, h = 0, cur = 0;
     void *mem = nullptr;
     size_t bytes = 0;
-    xg_picture pic[2];
-    xg_picture pred;
+    StreamDev d;
+    int maxBlocks = 0, taskCap = 0;
+    xg_block_result *hRes = nullptr;   // page-locked staging for pageable result buffers
+    // partition last accepted at submit (host copy)
+    std::vector<xg_block> lastBlocks;
+    // partition currently on the device (set when a frame carrying it launches)
+    int devBlocks = 0, devTasks = 0, devMaxW = 0, devMaxH = 0;
     std::deque<Request *> pending;
     int64_t frames = 0;
 };
 
+// Per-session descriptor tables of one batch: host (page-locked) and device.
+struct Descs {
+    SearchList search[kMaxStreams];
+    RefineList refine[kMaxStreams];
+    PadPlane pad[3 * kMaxStreams];
+};
+
+template <typename T>
+T *dev_member(void *base, size_t offset)
+{
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + offset);
+}
+
 struct Batch {
     bool active = false;
     std::vector<Request *> reqs;
-    std::vector<int> off;
     hipEvent_t done = nullptr;
     int64_t startNs = 0;
-    xg_block_result *pinnedResults = nullptr;
+    Descs *hDesc = nullptr;
+    Descs *dDesc = nullptr;
 };
 
 }  // namespace
@@ -100,11 +170,12 @@ struct xg_server {
     std::multimap<size_t, uint16_t *> pinnedPool;
     uint64_t bytesSessions = 0, bytesStreams = 0;
     int64_t framesDone = 0, framesDropped = 0, batches = 0;
+    int grid = 1;
 
     uint16_t *pinned_get(size_t bytes)
     {
-        auto it = pinnedPool.find(bytes);
-        if (it != pinnedPool.end()) {
+        auto it = pinnedPool.lower_bound(bytes);
+        if (it != pinnedPool.end() && it->first == bytes) {
             uint16_t *p = it->second;
             pinnedPool.erase(it);
             return p;
@@ -113,14 +184,17 @@ struct xg_server {
         if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
         return (uint16_t *)p;
     }
-    void pinned_put(uint16_t *p, size_t bytes) { pinnedPool.emplace(bytes, p); }
+    void pinned_put(uint16_t *p, size_t bytes)
+    {
+        if (p) pinnedPool.emplace(bytes, p);
+    }
 
     void finish_request(Request *r, int32_t status, int batchFrames, int64_t startNs)
     {
         xg_frame_result fr;
         fr.stream = r->stream;
         fr.status = status;
-        fr.nblocks = (int32_t)r->blocks.size();
+        fr.nblocks = r->nblocks;
         fr.batchFrames = batchFrames;
         fr.tag = r->tag;
         fr.submitNs = r->submitNs;
@@ -129,24 +203,27 @@ struct xg_server {
         completed.push_back(fr);
         if (status == XG_OK) framesDone++;
         else framesDropped++;
-        if (r->pinned) pinned_put(r->pinned, r->pinnedBytes);
+        pinned_put(r->staged, r->stagedBytes);
+        pinned_put(r->predStage, r->predStageBytes);
         delete r;
     }
 
-    // Finalises a completed batch: results to the callers, reference swap.
+    // Finalises a completed batch: pageable outputs to the callers, reference swap.
     void finalize(int si)
     {
         Batch &b = batch[si];
         const int nf = (int)b.reqs.size();
-        for (int i = 0; i < nf; i++) {
-            Request *r = b.reqs[i];
+        for (Request *r : b.reqs) {
             Stream &st = streams[r->stream];
-            memcpy(r->results, b.pinnedResults + b.off[i], sizeof(xg_block_result) * r->blocks.size());
-            if (r->pred[0]) {
-                const int cw = st.w / 2, ch = st.h / 2;
-                unpack2d(r->pred[0], r->predStride[0], r->pinned, st.w, st.h, 2);
-                unpack2d(r->pred[1], r->predStride[1], r->pinned + (size_t)st.w * st.h, cw, ch, 2);
-                unpack2d(r->pred[2], r->predStride[2], r->pinned + (size_t)st.w * st.h + (size_t)cw * ch, cw, ch, 2);
+            if (r->outputsKind == XG_MEM_HOST) {
+                memcpy(r->results, st.hRes, sizeof(xg_block_result) * (size_t)r->nblocks);
+                if (r->pred[0]) {
+                    const int cw = st.w / 2, ch = st.h / 2;
+                    unpack2d(r->pred[0], r->predStride[0], r->predStage, st.w, st.h, 2);
+                    unpack2d(r->pred[1], r->predStride[1], r->predStage + (size_t)st.w * st.h, cw, ch, 2);
+                    unpack2d(r->pred[2], r->predStride[2], r->predStage + (size_t)st.w * st.h + (size_t)cw * ch, cw,
+                             ch, 2);
+                }
             }
             st.cur ^= 1;           // this frame's picture becomes the next frame's reference
             st.hasRef = true;
@@ -155,9 +232,89 @@ struct xg_server {
             finish_request(r, XG_OK, nf, b.startNs);
         }
         b.reqs.clear();
-        b.off.clear();
         b.active = false;
         cvDone.notify_all();
+    }
+
+    // A batch whose device work failed: every frame is reported with XG_ERR_HIP.
+    void fail_batch(int si, int32_t rc)
+    {
+        Batch &b = batch[si];
+        log_msg(0, "serve: batch of %zu frames failed (%d)", b.reqs.size(), rc);
+        for (Request *r : b.reqs) {
+            streams[r->stream].inFlight = false;
+            finish_request(r, rc, (int)b.reqs.size(), b.startNs);
+        }
+        b.reqs.clear();
+        b.active = false;
+        cvDone.notify_all();
+    }
+
+    static hipMemcpyKind kind_of(int memKind, bool toDevice)
+    {
+        if (memKind == XG_MEM_DEVICE) return hipMemcpyDeviceToDevice;
+        return toDevice ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    }
+
+    // Enqueues frame r's inputs: a new partition, the planes into the current
+    // picture (padding is one launch for the whole batch, afterwards).
+    int32_t stage_inputs(Request *r, hipStream_t hs)
+    {
+        Stream &st = streams[r->stream];
+        if (r->newPartition) {
+            const size_t nb = r->blocks.size(), nt = r->tasksW.size();
+            if (hipMemcpyAsync(st.d.blocks, r->blocks.data(), sizeof(xg_block) * nb, hipMemcpyHostToDevice, hs) !=
+                    hipSuccess ||
+                (nt && hipMemcpyAsync(st.d.tasks, r->tasksW.data(), sizeof(int2) * nt, hipMemcpyHostToDevice, hs) !=
+                           hipSuccess) ||
+                hipMemcpyAsync(st.d.wgBegin, r->wgBegin.data(), sizeof(int) * r->wgBegin.size(),
+                               hipMemcpyHostToDevice, hs) != hipSuccess)
+                return XG_ERR_HIP;
+            int mw = 0, mh = 0;
+            for (const xg_block &B : r->blocks) {
+                mw = std::max(mw, (int)B.w);
+                mh = std::max(mh, (int)B.h);
+            }
+            st.devBlocks = (int)nb;
+            st.devTasks = (int)nt;
+            st.devMaxW = mw;
+            st.devMaxH = mh;
+        }
+        const xg_picture &P = st.d.pic[st.cur];
+        const uint16_t *src = r->staged;
+        for (int p = 0; p < 3; p++) {
+            const xg_plane &pl = P.plane[p];
+            const uint16_t *s = r->staged ? src : r->planes[p];
+            const size_t spitch = r->staged ? (size_t)pl.width * 2 : (size_t)r->strides[p] * 2;
+            const hipMemcpyKind k = r->staged ? hipMemcpyHostToDevice : kind_of(r->planesKind, true);
+            if (hipMemcpy2DAsync(pl.data, pl.stride * 2, s, spitch, (size_t)pl.width * 2, pl.height, k, hs) !=
+                hipSuccess)
+                return XG_ERR_HIP;
+            src += (size_t)pl.width * pl.height;
+        }
+        return XG_OK;
+    }
+
+    // Enqueues frame r's outputs: results and, when asked for, the prediction.
+    int32_t read_outputs(Request *r, hipStream_t hs)
+    {
+        Stream &st = streams[r->stream];
+        xg_block_result *dst = r->outputsKind == XG_MEM_HOST ? st.hRes : r->results;
+        if (hipMemcpyAsync(dst, st.d.best, sizeof(xg_block_result) * (size_t)r->nblocks,
+                           kind_of(r->outputsKind, false), hs) != hipSuccess)
+            return XG_ERR_HIP;
+        if (!r->pred[0]) return XG_OK;
+        uint16_t *stage = r->predStage;
+        for (int p = 0; p < 3; p++) {
+            const xg_plane &pl = st.d.pred.plane[p];
+            uint16_t *d = r->outputsKind == XG_MEM_HOST ? stage : r->pred[p];
+            const size_t dpitch = r->outputsKind == XG_MEM_HOST ? (size_t)pl.width * 2 : (size_t)r->predStride[p] * 2;
+            if (hipMemcpy2DAsync(d, dpitch, pl.data, pl.stride * 2, (size_t)pl.width * 2, pl.height,
+                                 kind_of(r->outputsKind, false), hs) != hipSuccess)
+                return XG_ERR_HIP;
+            stage += (size_t)pl.width * pl.height;
+        }
+        return XG_OK;
     }
 
     // Assembles and enqueues one batch on session si; returns false if nothing to do.
@@ -172,76 +329,67 @@ struct xg_server {
         std::sort(ids.begin(), ids.end(), [&](int x, int y) {
             return streams[x].pending.front()->submitNs < streams[y].pending.front()->submitNs;
         });
-        std::vector<xg_block> blocks;
         b.reqs.clear();
-        b.off.clear();
+        int nblk = 0;
         for (int id : ids) {
             Request *r = streams[id].pending.front();
-            if (!b.reqs.empty() && (int)(blocks.size() + r->blocks.size()) > cfg.maxBatchBlocks) break;
+            if (!b.reqs.empty() && nblk + r->nblocks > cfg.maxBatchBlocks) break;
             streams[id].pending.pop_front();
-            Stream &st = streams[id];
-            const int curSlot = 2 * id + st.cur, refSlot = st.hasRef ? 2 * id + (st.cur ^ 1) : curSlot;
-            b.off.push_back((int)blocks.size());
-            for (xg_block B : r->blocks) {
-                B.pic = (uint8_t)curSlot;
-                B.ref = (uint8_t)refSlot;
-                B.mvx = B.mvy = 0;
-                blocks.push_back(B);
-            }
+            streams[id].inFlight = true;
+            nblk += r->nblocks;
             b.reqs.push_back(r);
-            st.inFlight = true;
         }
         xg_session *s = sess[si];
         hipStream_t hs = (hipStream_t)xg_session_stream(s);
         b.startNs = now_ns();
-        int32_t rc = xg_session_set_blocks(s, blocks.data(), (int32_t)blocks.size());
-        for (size_t i = 0; i < b.reqs.size() && rc == XG_OK; i++) {
-            Request *r = b.reqs[i];
-            Stream &st = streams[r->stream];
-            const xg_picture &P = st.pic[st.cur];
-            const uint16_t *src = r->pinned;
-            for (int p = 0; p < 3 && rc == XG_OK; p++) {
-                const xg_plane &pl = P.plane[p];
-                if (hipMemcpy2DAsync(pl.data, pl.stride * 2, src, (size_t)pl.width * 2, (size_t)pl.width * 2,
-                                     pl.height, hipMemcpyHostToDevice, hs) != hipSuccess)
-                    rc = XG_ERR_HIP;
-                dim3 grid((pl.width + 2 * pl.pad + 255) / 256, pl.height + 2 * pl.pad);
-                k_pad_plane<<<grid, 256, 0, hs>>>(pl.data, pl.stride, pl.width, pl.height, pl.pad);
-                src += (size_t)pl.width * pl.height;
-            }
-        }
-        if (rc == XG_OK) rc = xg_search_blocks(s, cfg.searchRange, cfg.searchLambda, cfg.refine.ctuSize);
-        if (rc == XG_OK) rc = xg_refine_frac(s, &cfg.refine, nullptr, XG_MODE_BATCHED);
-        if (rc == XG_OK) {
-            xg_block_result *dBest = nullptr;
-            xg_session_outputs(s, nullptr, &dBest);
-            if (hipMemcpyAsync(b.pinnedResults, dBest, sizeof(xg_block_result) * blocks.size(),
-                               hipMemcpyDeviceToHost, hs) != hipSuccess)
-                rc = XG_ERR_HIP;
-        }
-        for (size_t i = 0; i < b.reqs.size() && rc == XG_OK; i++) {
-            Request *r = b.reqs[i];
-            if (!r->pred[0]) continue;
-            Stream &st = streams[r->stream];
-            uint16_t *dst = r->pinned;
+        int32_t rc = XG_OK;
+        for (Request *r : b.reqs)
+            if (rc == XG_OK) rc = stage_inputs(r, hs);
+        // descriptor tables of the batch
+        Descs &D = *b.hDesc;
+        const int nf = (int)b.reqs.size();
+        int begin = 0, taskAcc = 0, maxW = 4, maxH = 4;
+        for (int i = 0; i < nf && rc == XG_OK; i++) {
+            Stream &st = streams[b.reqs[i]->stream];
+            const int id = b.reqs[i]->stream;
+            const uint8_t curSlot = (uint8_t)(2 * id + st.cur);
+            const uint8_t refSlot = (uint8_t)(st.hasRef ? 2 * id + (st.cur ^ 1) : curSlot);
+            D.search[i] = SearchList{st.d.blocks, st.d.field, st.d.fieldStride, begin, curSlot, refSlot};
+            begin += st.devBlocks;
+            maxW = std::max(maxW, st.devMaxW);
+            maxH = std::max(maxH, st.devMaxH);
+            // spread small partitions over the grid, in steps of whole XCD bands
+            const int rot = (taskAcc / 8 * 8) % grid;
+            taskAcc += st.devTasks;
+            D.refine[i] = RefineList{st.d.blocks, st.d.tasks, st.d.wgBegin, st.d.candCost, st.d.best,
+                                     st.d.bestKey, st.d.done, st.d.field, st.d.fieldStride, rot};
             for (int p = 0; p < 3; p++) {
-                const xg_plane &pl = st.pred.plane[p];
-                if (hipMemcpy2DAsync(dst, (size_t)pl.width * 2, pl.data, pl.stride * 2, (size_t)pl.width * 2, pl.height,
-                                     hipMemcpyDeviceToHost, hs) != hipSuccess)
-                    rc = XG_ERR_HIP;
-                dst += (size_t)pl.width * pl.height;
+                const xg_plane &pl = st.d.pic[st.cur].plane[p];
+                D.pad[3 * i + p] = PadPlane{pl.data, pl.stride, pl.width, pl.height, pl.pad};
             }
         }
+        if (rc == XG_OK && hipMemcpyAsync(b.dDesc, b.hDesc, sizeof(Descs), hipMemcpyHostToDevice, hs) != hipSuccess)
+            rc = XG_ERR_HIP;
+        if (rc == XG_OK) {
+            int rows = 0;
+            for (int i = 0; i < 3 * nf; i++) rows = std::max(rows, D.pad[i].h + 2 * D.pad[i].pad);
+            k_pad_multi<<<dim3(rows, 3 * nf), 256, 0, hs>>>(dev_member<PadPlane>(b.dDesc, offsetof(Descs, pad)));
+            if (hipGetLastError() != hipSuccess) rc = XG_ERR_HIP;
+        }
+        if (rc == XG_OK) {
+            SessionView v;
+            session_view(s, &v);
+            rc = search_multi_enqueue(hs, s, v.dPics, dev_member<SearchList>(b.dDesc, offsetof(Descs, search)), nf,
+                                      begin, maxW, maxH, cfg.searchRange, cfg.searchLambda, cfg.refine.ctuSize);
+        }
+        if (rc == XG_OK)
+            rc = refine_multi_enqueue(s, &cfg.refine, dev_member<RefineList>(b.dDesc, offsetof(Descs, refine)), nf);
+        for (Request *r : b.reqs)
+            if (rc == XG_OK) rc = read_outputs(r, hs);
         if (rc == XG_OK && hipEventRecord(b.done, hs) != hipSuccess) rc = XG_ERR_HIP;
         if (rc != XG_OK) {
-            log_msg(0, "serve: batch of %zu frames failed (%d)", b.reqs.size(), rc);
             (void)hipStreamSynchronize(hs);
-            for (Request *r : b.reqs) {
-                streams[r->stream].inFlight = false;
-                finish_request(r, rc, (int)b.reqs.size(), b.startNs);
-            }
-            b.reqs.clear();
-            cvDone.notify_all();
+            fail_batch(si, rc);
             return true;
         }
         b.active = true;
@@ -249,32 +397,53 @@ struct xg_server {
         return true;
     }
 
-    // Nanoseconds a free session should still wait to gather frames: > 0 while
-    // the oldest launchable frame is younger than maxDelayUs and the launchable
-    // frames do not fill a batch.
+    // Streams that may still bring a frame (open, not closing).
+    int live_streams() const
+    {
+        int n = 0;
+        for (int i = 0; i < kMaxStreams; i++) n += streams[i].open && !streams[i].closing;
+        return n;
+    }
+
+    // Nanoseconds a free session should still wait to gather frames (INT64_MAX:
+    // until more frames arrive).  Waits while fewer than minBatchFrames (capped
+    // at the live streams) are launchable, and while the oldest launchable
+    // frame is younger than maxDelayUs and the launchable frames do not fill a
+    // batch; maxDelayUs > 0 also bounds the first wait.
     int64_t gather_wait_ns() const
     {
-        if (cfg.maxDelayUs <= 0) return 0;
         int64_t oldest = INT64_MAX, nblk = 0;
+        int n = 0;
         for (int i = 0; i < kMaxStreams; i++) {
             const Stream &st = streams[i];
             if (!st.open || st.closing || st.inFlight || st.pending.empty()) continue;
             oldest = std::min(oldest, st.pending.front()->submitNs);
-            nblk += (int64_t)st.pending.front()->blocks.size();
+            nblk += st.pending.front()->nblocks;
+            n++;
         }
-        if (oldest == INT64_MAX || nblk >= cfg.maxBatchBlocks) return 0;
-        return std::max<int64_t>(0, oldest + (int64_t)cfg.maxDelayUs * 1000 - now_ns());
+        if (n == 0 || nblk >= cfg.maxBatchBlocks) return 0;
+        const int64_t left = cfg.maxDelayUs > 0 ? oldest + (int64_t)cfg.maxDelayUs * 1000 - now_ns() : 0;
+        if (cfg.minBatchFrames > 0 && n < std::min(cfg.minBatchFrames, live_streams()))
+            return cfg.maxDelayUs > 0 ? std::max<int64_t>(0, left) : INT64_MAX;
+        return std::max<int64_t>(0, left);
     }
 
     void loop()
     {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            for (int i = 0; i < kSessions; i++)
-                if (batch[i].active && hipEventQuery(batch[i].done) == hipSuccess) finalize(i);
+            for (int i = 0; i < kSessions; i++) {
+                if (!batch[i].active) continue;
+                const hipError_t q = hipEventQuery(batch[i].done);
+                if (q == hipSuccess) finalize(i);
+                else if (q != hipErrorNotReady) fail_batch(i, XG_ERR_HIP);
+            }
             const int64_t gather = stop ? 0 : gather_wait_ns();
+            bool anyActive = batch[0].active || batch[1].active;
             if (gather > 0) {
-                cvWork.wait_for(lk, std::chrono::nanoseconds(std::min<int64_t>(gather, 100000)));
+                // wake for new frames, or to poll the batches in flight
+                const int64_t w = anyActive ? std::min<int64_t>(gather, 100000) : std::min<int64_t>(gather, 50000000);
+                cvWork.wait_for(lk, std::chrono::nanoseconds(w));
                 continue;
             }
             bool launched = false;
@@ -292,8 +461,9 @@ struct xg_server {
                 // nothing can launch until a batch completes
                 hipEvent_t e = batch[oldest].done;
                 lk.unlock();
-                (void)hipEventSynchronize(e);
+                const hipError_t se = hipEventSynchronize(e);
                 lk.lock();
+                if (se != hipSuccess && batch[oldest].active) fail_batch(oldest, XG_ERR_HIP);
             } else if (active > 0) {
                 cvWork.wait_for(lk, std::chrono::microseconds(100));
             } else {
@@ -301,14 +471,107 @@ struct xg_server {
             }
         }
     }
+
+    // Validates a partition (cheap; under the lock).  Returns whether it
+    // differs from the stream's last one, i.e. needs a task assignment.
+    int32_t check_partition(int id, const xg_block *blocks, int n, bool *changed)
+    {
+        Stream &st = streams[id];
+        for (int i = 0; i < n; i++) {   // malformed descriptors are rejected before anything is queued
+            const xg_block &B = blocks[i];
+            const bool okw = B.w >= 4 && B.w <= 64 && !(B.w & (B.w - 1)),
+                       okh = B.h >= 4 && B.h <= 64 && !(B.h & (B.h - 1));
+            if (!okw || !okh || B.x < 0 || B.y < 0 || (B.x & 3) || (B.y & 3) || B.x + B.w > st.w ||
+                B.y + B.h > st.h) {
+                set_error(XG_ERR_ARG, "xg_server_submit: block %d (%d,%d %dx%d) is malformed", i, B.x, B.y, B.w, B.h);
+                return XG_ERR_ARG;
+            }
+        }
+        if (n > st.maxBlocks) {
+            set_error(XG_ERR_ARG, "xg_server_submit: %d blocks, more than the stream's %d", n, st.maxBlocks);
+            return XG_ERR_ARG;
+        }
+        *changed = !(n == (int)st.lastBlocks.size() && memcmp(blocks, st.lastBlocks.data(), sizeof(xg_block) * n) == 0);
+        return XG_OK;
+    }
+
+    // Queues r; a frame bringing a partition makes it the stream's last one.
+    int32_t enqueue(int id, Request *r, const xg_block *blocks)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        Stream &st = streams[id];
+        if (!st.open || st.closing) {
+            set_error(XG_ERR_STATE, "xg_server_submit: stream %d closed", id);
+            return XG_ERR_STATE;
+        }
+        if (r->newPartition) st.lastBlocks.assign(blocks, blocks + r->nblocks);
+        r->submitNs = now_ns();
+        st.pending.push_back(r);
+        cvWork.notify_one();
+        return XG_OK;
+    }
 };
+
+namespace {
+
+// Per-stream device layout: offsets of every part of the one allocation.
+struct StreamLayout {
+    size_t pic[2], pred, field, blocks, tasks, wgBegin, candCost, best, bestKey, done, total;
+    int pad, ys, cs, maxBlocks, taskCap, fieldStride;
+    size_t ypic, cpic;
+};
+
+StreamLayout stream_layout(const xg_server_cfg &cfg, int width, int height, int grid, const int *taskPerBlock)
+{
+    StreamLayout L;
+    L.pad = cfg.refine.ctuSize + 16;
+    const int cpad = L.pad / 2;
+    L.ys = width + 2 * L.pad;
+    L.cs = width / 2 + 2 * cpad;
+    L.ypic = (size_t)L.ys * (height + 2 * L.pad);
+    L.cpic = (size_t)L.cs * (height / 2 + 2 * cpad);
+    const size_t picBytes = align_up((L.ypic + 2 * L.cpic) * 2, 4096);
+    L.maxBlocks = (width / 4) * (height / 4);
+    L.taskCap = L.maxBlocks * *taskPerBlock;
+    L.fieldStride = width / 4;
+    size_t o = 0;
+    L.pic[0] = o; o += picBytes;
+    L.pic[1] = o; o += picBytes;
+    L.pred = o; o += align_up((size_t)width * height * 3, 4096);
+    L.field = o; o += align_up((size_t)L.maxBlocks * sizeof(int2), 256);
+    L.blocks = o; o += align_up((size_t)L.maxBlocks * sizeof(xg_block), 256);
+    L.tasks = o; o += align_up((size_t)L.taskCap * sizeof(int2), 256);
+    L.wgBegin = o; o += align_up((size_t)(grid + 1) * sizeof(int), 256);
+    L.candCost = o; o += align_up((size_t)L.maxBlocks * cfg.refine.ncand * sizeof(uint32_t), 256);
+    L.best = o; o += align_up((size_t)L.maxBlocks * sizeof(xg_block_result), 256);
+    L.bestKey = o; o += align_up((size_t)L.maxBlocks * sizeof(unsigned long long), 256);
+    L.done = o; o += align_up((size_t)L.maxBlocks * sizeof(int), 256);
+    L.total = o;
+    return L;
+}
+
+// Tasks per block at most, over every block shape, for the session's tiles.
+int max_tasks_per_block(xg_session *s, int ncand)
+{
+    int m = 1;
+    for (int lw = 2; lw <= 6; lw++)
+        for (int lh = 2; lh <= 6; lh++) {
+            const int t = std::max(1, std::min(ncand, (int)xg_session_get_tile(s, lw, lh)));
+            m = std::max(m, (ncand + t - 1) / t);
+        }
+    return m;
+}
+
+}  // namespace
 
 extern "C" {
 
 int32_t xg_server_create(xg_server **out, const xg_server_cfg *cfg)
 {
     if (!out || !cfg || cfg->maxStreams < 1 || cfg->maxStreams > kMaxStreams || cfg->maxBatchBlocks < 1 ||
-        cfg->searchRange < 0 || cfg->searchRange > 32 || cfg->maxDelayUs < 0 || cfg->maxDelayUs > 1000000 || cfg->refine.ncand < 1 || cfg->refine.ncand > XG_MAX_CAND) {
+        cfg->searchRange < 0 || cfg->searchRange > 32 || cfg->maxDelayUs < 0 || cfg->maxDelayUs > 1000000 ||
+        cfg->minBatchFrames < 0 || cfg->minBatchFrames > kMaxStreams || cfg->refine.ncand < 1 ||
+        cfg->refine.ncand > XG_MAX_CAND || cfg->refine.ctuSize < 16 || cfg->refine.ctuSize > 128) {
         set_error(XG_ERR_ARG, "xg_server_create: bad configuration");
         return XG_ERR_ARG;
     }
@@ -316,9 +579,7 @@ int32_t xg_server_create(xg_server **out, const xg_server_cfg *cfg)
     if (!device_init()) return XG_ERR_HIP;
     xg_server *srv = new xg_server();
     srv->cfg = *cfg;
-    // per session: blocks, tasks, costs, results and completion words per block
-    srv->bytesSessions = (uint64_t)kSessions * (uint64_t)cfg->maxBatchBlocks *
-                         (sizeof(xg_block) + 2 * 8 + XG_MAX_CAND * 4 + sizeof(xg_block_result) + 4 + 8);
+    srv->bytesSessions = (uint64_t)kSessions * (sizeof(Descs) + 4096);
     if (srv->bytesSessions > cfg->hbmBudget) {
         set_error(XG_ERR_BUDGET, "xg_server_create: sessions alone need %llu bytes",
                   (unsigned long long)srv->bytesSessions);
@@ -326,15 +587,16 @@ int32_t xg_server_create(xg_server **out, const xg_server_cfg *cfg)
         return XG_ERR_BUDGET;
     }
     for (int i = 0; i < kSessions; i++) {
-        if (xg_session_create(&srv->sess[i], cfg->maxBatchBlocks) != XG_OK ||
+        if (xg_session_create(&srv->sess[i], 1) != XG_OK || xg_session_set_sched(srv->sess[i], 8, 12) != XG_OK ||
             hipEventCreateWithFlags(&srv->batch[i].done, hipEventDisableTiming) != hipSuccess ||
-            hipHostMalloc((void **)&srv->batch[i].pinnedResults, sizeof(xg_block_result) * cfg->maxBatchBlocks,
-                          hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void **)&srv->batch[i].hDesc, sizeof(Descs), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc((void **)&srv->batch[i].dDesc, sizeof(Descs)) != hipSuccess) {
             set_error(XG_ERR_NOMEM, "xg_server_create: session allocation failed");
             xg_server_destroy(srv);
             return XG_ERR_NOMEM;
         }
     }
+    srv->grid = refine_grid(srv->sess[0]);
     srv->thread = std::thread([srv] { srv->loop(); });
     *out = srv;
     return XG_OK;
@@ -354,16 +616,19 @@ int32_t xg_server_destroy(xg_server *srv)
     for (int i = 0; i < kMaxStreams; i++) {
         Stream &st = srv->streams[i];
         for (Request *r : st.pending) {
-            if (r->pinned) (void)hipHostFree(r->pinned);
+            if (r->staged) (void)hipHostFree(r->staged);
+            if (r->predStage) (void)hipHostFree(r->predStage);
             delete r;
         }
         st.pending.clear();
         if (st.mem) (void)hipFree(st.mem);
+        if (st.hRes) (void)hipHostFree(st.hRes);
     }
     for (int i = 0; i < kSessions; i++) {
         if (srv->sess[i]) xg_session_destroy(srv->sess[i]);
         if (srv->batch[i].done) (void)hipEventDestroy(srv->batch[i].done);
-        if (srv->batch[i].pinnedResults) (void)hipHostFree(srv->batch[i].pinnedResults);
+        if (srv->batch[i].hDesc) (void)hipHostFree(srv->batch[i].hDesc);
+        if (srv->batch[i].dDesc) (void)hipFree(srv->batch[i].dDesc);
     }
     for (auto &kv : srv->pinnedPool) (void)hipHostFree(kv.second);
     delete srv;
@@ -381,54 +646,77 @@ int32_t xg_server_open_stream(xg_server *srv, int32_t width, int32_t height, int
     int id = -1, open = 0;
     for (int i = 0; i < kMaxStreams; i++) {
         if (srv->streams[i].open) open++;
-        else if (id < 0) id = i;
+        else if (id < 0 && !srv->streams[i].mem) id = i;
     }
     if (id < 0 || open >= srv->cfg.maxStreams) {
         set_error(XG_ERR_BUDGET, "xg_server_open_stream: all %d stream slots in use", srv->cfg.maxStreams);
         return XG_ERR_BUDGET;
     }
-    const int pad = srv->cfg.refine.ctuSize + 16, cpad = pad / 2;
-    const intptr_t ys = width + 2 * pad, cs = width / 2 + 2 * cpad;
-    const size_t ypic = (size_t)ys * (height + 2 * pad), cpic = (size_t)cs * (height / 2 + 2 * cpad);
-    const size_t picBytes = align_up((ypic + 2 * cpic) * 2, 4096);
-    const size_t predBytes = align_up((size_t)width * height * 3, 4096);     // 1.5 samples x 2 bytes
-    const size_t bytes = 2 * picBytes + predBytes;
-    if (srv->bytesSessions + srv->bytesStreams + bytes > srv->cfg.hbmBudget) {
-        set_error(XG_ERR_BUDGET, "xg_server_open_stream: %zu bytes do not fit the HBM budget", bytes);
+    const int tpb = max_tasks_per_block(srv->sess[0], srv->cfg.refine.ncand);
+    const StreamLayout L = stream_layout(srv->cfg, width, height, srv->grid, &tpb);
+    const size_t resBytes = sizeof(xg_block_result) * (size_t)L.maxBlocks;
+    if (srv->bytesSessions + srv->bytesStreams + L.total > srv->cfg.hbmBudget) {
+        set_error(XG_ERR_BUDGET, "xg_server_open_stream: %zu bytes do not fit the HBM budget", L.total);
         return XG_ERR_BUDGET;
     }
     Stream &st = srv->streams[id];
-    if (!XG_HIP(hipMalloc(&st.mem, bytes))) return XG_ERR_NOMEM;
-    char *base = (char *)st.mem;
-    for (int k = 0; k < 2; k++) {
-        uint16_t *p = (uint16_t *)(base + k * picBytes);
-        st.pic[k].plane[0] = xg_plane{p + pad * ys + pad, ys, width, height, pad};
-        uint16_t *c1 = p + ypic, *c2 = c1 + cpic;
-        st.pic[k].plane[1] = xg_plane{c1 + cpad * cs + cpad, cs, width / 2, height / 2, cpad};
-        st.pic[k].plane[2] = xg_plane{c2 + cpad * cs + cpad, cs, width / 2, height / 2, cpad};
+    if (!XG_HIP(hipMalloc(&st.mem, L.total))) return XG_ERR_NOMEM;
+    if (!XG_HIP(hipHostMalloc((void **)&st.hRes, resBytes, hipHostMallocDefault))) {
+        (void)hipFree(st.mem);
+        st.mem = nullptr;
+        return XG_ERR_NOMEM;
     }
-    uint16_t *pr = (uint16_t *)(base + 2 * picBytes);
-    st.pred.plane[0] = xg_plane{pr, width, width, height, 0};
-    st.pred.plane[1] = xg_plane{pr + (size_t)width * height, width / 2, width / 2, height / 2, 0};
-    st.pred.plane[2] = xg_plane{pr + (size_t)width * height * 5 / 4, width / 2, width / 2, height / 2, 0};
+    char *base = (char *)st.mem;
+    const int pad = L.pad, cpad = pad / 2;
+    for (int k = 0; k < 2; k++) {
+        uint16_t *p = (uint16_t *)(base + L.pic[k]);
+        st.d.pic[k].plane[0] = xg_plane{p + pad * L.ys + pad, L.ys, width, height, pad};
+        uint16_t *c1 = p + L.ypic, *c2 = c1 + L.cpic;
+        st.d.pic[k].plane[1] = xg_plane{c1 + cpad * L.cs + cpad, L.cs, width / 2, height / 2, cpad};
+        st.d.pic[k].plane[2] = xg_plane{c2 + cpad * L.cs + cpad, L.cs, width / 2, height / 2, cpad};
+    }
+    uint16_t *pr = (uint16_t *)(base + L.pred);
+    st.d.pred.plane[0] = xg_plane{pr, width, width, height, 0};
+    st.d.pred.plane[1] = xg_plane{pr + (size_t)width * height, width / 2, width / 2, height / 2, 0};
+    st.d.pred.plane[2] = xg_plane{pr + (size_t)width * height * 5 / 4, width / 2, width / 2, height / 2, 0};
+    st.d.field = (int2 *)(base + L.field);
+    st.d.fieldStride = L.fieldStride;
+    st.d.blocks = (xg_block *)(base + L.blocks);
+    st.d.tasks = (int2 *)(base + L.tasks);
+    st.d.wgBegin = (int *)(base + L.wgBegin);
+    st.d.candCost = (uint32_t *)(base + L.candCost);
+    st.d.best = (xg_block_result *)(base + L.best);
+    st.d.bestKey = (unsigned long long *)(base + L.bestKey);
+    st.d.done = (int *)(base + L.done);
+    // MV field starts at zero; per-block completion words idle (all-ones / zero)
+    bool ok = XG_HIP(hipMemset(st.d.field, 0, (size_t)L.maxBlocks * sizeof(int2))) &&
+              XG_HIP(hipMemset(st.d.bestKey, 0xFF, (size_t)L.maxBlocks * sizeof(unsigned long long))) &&
+              XG_HIP(hipMemset(st.d.done, 0, (size_t)L.maxBlocks * sizeof(int))) &&
+              XG_HIP(hipDeviceSynchronize());
     // the dispatcher holds no session call while we hold the lock
-    for (int si = 0; si < kSessions; si++)
-        for (int k = 0; k < 2; k++) {
-            if (xg_session_set_picture(srv->sess[si], 2 * id + k, &st.pic[k]) != XG_OK ||
-                xg_session_set_pred(srv->sess[si], 2 * id + k, &st.pred) != XG_OK) {
-                (void)hipFree(st.mem);
-                st.mem = nullptr;
-                return XG_ERR_HIP;
-            }
-        }
+    for (int si = 0; si < kSessions && ok; si++)
+        for (int k = 0; k < 2 && ok; k++)
+            ok = xg_session_set_picture(srv->sess[si], 2 * id + k, &st.d.pic[k]) == XG_OK &&
+                 xg_session_set_pred(srv->sess[si], 2 * id + k, &st.d.pred) == XG_OK;
+    if (!ok) {
+        (void)hipFree(st.mem);
+        (void)hipHostFree(st.hRes);
+        st.mem = nullptr;
+        st.hRes = nullptr;
+        return XG_ERR_HIP;
+    }
     st.open = true;
     st.closing = st.inFlight = st.hasRef = false;
     st.w = width;
     st.h = height;
     st.cur = 0;
-    st.bytes = bytes;
+    st.bytes = L.total;
+    st.maxBlocks = L.maxBlocks;
+    st.taskCap = L.taskCap;
+    st.lastBlocks.clear();
+    st.devBlocks = st.devTasks = 0;
     st.frames = 0;
-    srv->bytesStreams += bytes;
+    srv->bytesStreams += L.total;
     *streamId = id;
     return XG_OK;
 }
@@ -449,79 +737,146 @@ int32_t xg_server_close_stream(xg_server *srv, int32_t streamId)
         srv->finish_request(r, XG_ERR_STATE, 0, 0);
     }
     srv->cvDone.notify_all();
+    srv->cvWork.notify_all();          // a batch held for more streams may go now
     srv->cvDone.wait(lk, [&] { return !st.inFlight; });
     (void)hipFree(st.mem);
+    (void)hipHostFree(st.hRes);
     st.mem = nullptr;
+    st.hRes = nullptr;
     srv->bytesStreams -= st.bytes;
     st.open = st.closing = false;
+    st.lastBlocks.clear();
     return XG_OK;
+}
+
+int32_t xg_server_submit_frame(xg_server *srv, int32_t streamId, const xg_server_frame *f)
+{
+    if (!srv || !f || streamId < 0 || streamId >= kMaxStreams || !f->results ||
+        f->planesKind < XG_MEM_HOST || f->planesKind > XG_MEM_DEVICE || f->outputsKind < XG_MEM_HOST ||
+        f->outputsKind > XG_MEM_DEVICE || !f->planes[0] || !f->planes[1] || !f->planes[2] ||
+        (f->blocks && f->nblocks < 1)) {
+        set_error(XG_ERR_ARG, "xg_server_submit_frame: bad arguments");
+        return XG_ERR_ARG;
+    }
+    Request *r = new Request();
+    int w, h, taskCap = 0;
+    {
+        std::lock_guard<std::mutex> g(srv->mu);
+        Stream &st = srv->streams[streamId];
+        if (!st.open || st.closing) {
+            delete r;
+            set_error(XG_ERR_STATE, "xg_server_submit: stream %d is not open", streamId);
+            return XG_ERR_STATE;
+        }
+        w = st.w;
+        h = st.h;
+        for (int p = 0; p < 3; p++)
+            if (f->strides[p] < (p ? w / 2 : w)) {
+                delete r;
+                set_error(XG_ERR_ARG, "xg_server_submit: plane %d stride %ld below its width", p, (long)f->strides[p]);
+                return XG_ERR_ARG;
+            }
+        int32_t rc;
+        if (f->blocks) {
+            rc = srv->check_partition(streamId, f->blocks, f->nblocks, &r->newPartition);
+            r->nblocks = f->nblocks;
+            taskCap = st.taskCap;
+        } else if (st.lastBlocks.empty()) {
+            set_error(XG_ERR_STATE, "xg_server_submit: stream %d has no partition yet", streamId);
+            rc = XG_ERR_STATE;
+        } else {
+            r->nblocks = (int)st.lastBlocks.size();
+            rc = XG_OK;
+        }
+        if (rc != XG_OK) {
+            delete r;
+            return rc;
+        }
+        if (f->outputsKind == XG_MEM_HOST && f->pred[0]) {
+            r->predStageBytes = (size_t)w * h * 3;
+            r->predStage = srv->pinned_get(r->predStageBytes);
+            if (!r->predStage) {
+                delete r;
+                set_error(XG_ERR_NOMEM, "xg_server_submit: pinned staging allocation failed");
+                return XG_ERR_NOMEM;
+            }
+        }
+        if (f->planesKind == XG_MEM_HOST) {
+            r->stagedBytes = (size_t)w * h * 3;
+            r->staged = srv->pinned_get(r->stagedBytes);
+            if (!r->staged) {
+                srv->pinned_put(r->predStage, r->predStageBytes);
+                delete r;
+                set_error(XG_ERR_NOMEM, "xg_server_submit: pinned staging allocation failed");
+                return XG_ERR_NOMEM;
+            }
+        }
+    }
+    r->stream = streamId;
+    r->tag = f->tag;
+    r->planesKind = f->planesKind;
+    for (int p = 0; p < 3; p++) {
+        r->planes[p] = f->planes[p];
+        r->strides[p] = f->strides[p];
+    }
+    r->outputsKind = f->outputsKind;
+    r->results = f->results;
+    if (f->pred[0])
+        for (int p = 0; p < 3; p++) {
+            r->pred[p] = f->pred[p];
+            r->predStride[p] = f->predStride[p] ? f->predStride[p] : (p ? w / 2 : w);
+        }
+    auto drop = [&](int32_t code) {
+        std::lock_guard<std::mutex> g(srv->mu);
+        srv->pinned_put(r->staged, r->stagedBytes);
+        srv->pinned_put(r->predStage, r->predStageBytes);
+        delete r;
+        return code;
+    };
+    if (r->newPartition) {   // the task assignment runs outside the server lock
+        r->blocks.assign(f->blocks, f->blocks + f->nblocks);
+        for (xg_block &B : r->blocks) B.mvx = B.mvy = B.mvpx = B.mvpy = 0;
+        refine_assign(srv->sess[0], r->blocks.data(), f->nblocks, srv->cfg.refine.ncand, r->tasksW, r->wgBegin);
+        if ((int)r->tasksW.size() > taskCap) {
+            set_error(XG_ERR_ARG, "xg_server_submit: partition needs %zu tasks, more than %d", r->tasksW.size(),
+                      taskCap);
+            return drop(XG_ERR_ARG);
+        }
+    }
+    if (r->staged) {      // pageable planes: copied before returning
+        pack2d(r->staged, f->planes[0], f->strides[0], w, h, 2);
+        pack2d(r->staged + (size_t)w * h, f->planes[1], f->strides[1], w / 2, h / 2, 2);
+        pack2d(r->staged + (size_t)w * h * 5 / 4, f->planes[2], f->strides[2], w / 2, h / 2, 2);
+    }
+    const int32_t rc = srv->enqueue(streamId, r, f->blocks);
+    return rc == XG_OK ? rc : drop(rc);
 }
 
 int32_t xg_server_submit(xg_server *srv, int32_t streamId, const xg_pixel *const planes[3], const intptr_t strides[3],
                          const xg_block *blocks, int32_t nblocks, xg_block_result *results, xg_pixel *const pred[3],
                          const intptr_t predStride[3], uint64_t tag)
 {
-    if (!srv || streamId < 0 || streamId >= kMaxStreams || !planes || !strides || !blocks || nblocks < 1 ||
-        nblocks > srv->cfg.maxBatchBlocks || !results) {
+    if (!planes || !strides || !blocks || nblocks < 1) {
         set_error(XG_ERR_ARG, "xg_server_submit: bad arguments");
         return XG_ERR_ARG;
     }
-    int w, h;
-    {
-        std::lock_guard<std::mutex> g(srv->mu);
-        const Stream &st = srv->streams[streamId];
-        if (!st.open || st.closing) {
-            set_error(XG_ERR_STATE, "xg_server_submit: stream %d is not open", streamId);
-            return XG_ERR_STATE;
-        }
-        w = st.w;
-        h = st.h;
-    }
-    for (int i = 0; i < nblocks; i++) {   // malformed descriptors are rejected before anything is queued
-        const xg_block &B = blocks[i];
-        const bool okw = B.w >= 4 && B.w <= 64 && !(B.w & (B.w - 1)), okh = B.h >= 4 && B.h <= 64 && !(B.h & (B.h - 1));
-        if (!okw || !okh || B.x < 0 || B.y < 0 || (B.x & 3) || (B.y & 3) || B.x + B.w > w || B.y + B.h > h) {
-            set_error(XG_ERR_ARG, "xg_server_submit: block %d (%d,%d %dx%d) is malformed", i, B.x, B.y, B.w, B.h);
-            return XG_ERR_ARG;
+    xg_server_frame f;
+    memset(&f, 0, sizeof(f));
+    f.planesKind = XG_MEM_HOST;
+    f.outputsKind = XG_MEM_HOST;
+    for (int p = 0; p < 3; p++) {
+        f.planes[p] = planes[p];
+        f.strides[p] = strides[p];
+        if (pred) {
+            f.pred[p] = pred[p];
+            f.predStride[p] = predStride ? predStride[p] : 0;
         }
     }
-    Request *r = new Request();
-    r->stream = streamId;
-    r->tag = tag;
-    r->blocks.assign(blocks, blocks + nblocks);
-    r->results = results;
-    if (pred)
-        for (int p = 0; p < 3; p++) {
-            r->pred[p] = pred[p];
-            r->predStride[p] = predStride ? predStride[p] : (p ? w / 2 : w);
-        }
-    r->pinnedBytes = (size_t)w * h * 3;
-    {
-        std::lock_guard<std::mutex> g(srv->mu);
-        r->pinned = srv->pinned_get(r->pinnedBytes);
-    }
-    if (!r->pinned) {
-        delete r;
-        set_error(XG_ERR_NOMEM, "xg_server_submit: pinned staging allocation failed");
-        return XG_ERR_NOMEM;
-    }
-    pack2d(r->pinned, planes[0], strides[0], w, h, 2);
-    pack2d(r->pinned + (size_t)w * h, planes[1], strides[1], w / 2, h / 2, 2);
-    pack2d(r->pinned + (size_t)w * h * 5 / 4, planes[2], strides[2], w / 2, h / 2, 2);
-    {
-        std::lock_guard<std::mutex> g(srv->mu);
-        Stream &st = srv->streams[streamId];
-        if (!st.open || st.closing) {
-            srv->pinned_put(r->pinned, r->pinnedBytes);
-            delete r;
-            set_error(XG_ERR_STATE, "xg_server_submit: stream %d closed", streamId);
-            return XG_ERR_STATE;
-        }
-        r->submitNs = now_ns();
-        st.pending.push_back(r);
-    }
-    srv->cvWork.notify_one();
-    return XG_OK;
+    f.blocks = blocks;
+    f.nblocks = nblocks;
+    f.results = results;
+    f.tag = tag;
+    return xg_server_submit_frame(srv, streamId, &f);
 }
 
 int32_t xg_server_poll(xg_server *srv, xg_frame_result *out, int32_t cap, int32_t timeoutMs)
