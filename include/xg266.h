@@ -451,12 +451,12 @@ typedef struct xg_session xg_session;
 
 /* A session owns a HIP stream, the per-shape task lists and the result
  * buffers for up to maxBlocks blocks.  Pictures are passed per call through a
- * picture set of up to 16 slots (xg_block.pic / .ref index it). */
+ * picture set of up to 128 slots (xg_block.pic / .ref index it). */
 int32_t xg_session_create(xg_session **out, int32_t maxBlocks);
 int32_t xg_session_destroy(xg_session *s);
 /* HIP stream of the session (hipStream_t) so callers can order their work. */
 void   *xg_session_stream(xg_session *s);
-/* Installs a picture in slot `slot` (0..15).  Device pointers. */
+/* Installs a picture in slot `slot` (0..127).  Device pointers. */
 int32_t xg_session_set_picture(xg_session *s, int32_t slot, const xg_picture *pic);
 /* Uploads the block list (host memory) and builds per-shape task lists:
  * tasks sorted by shape, largest first, each covering `candPerTask`

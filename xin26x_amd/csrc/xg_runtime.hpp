@@ -94,7 +94,17 @@ struct SearchList {
     int mvStride;                      // units per field row
     int begin;                         // first workgroup of this list in the launch
     uint8_t curSlot, refSlot;          // picture slots of the frame and its reference
+    const int *gidx;                   // k_search_groups: block indices, group by group
+    const int2 *groups;                // k_search_groups: (first entry of gidx, count) per group
 };
+// Groups of a partition for k_search_groups: blocks of one 64x64 CTU, at most
+// kSearchGroupMax per group, whose current samples span at most 64x64.
+constexpr int kSearchGroupMax = 64;
+void search_groups(const xg_block *blocks, int n, std::vector<int> &gidx, std::vector<int2> &groups);
+// Enqueues k_search_groups over lists[0..nLists) (device) covering nGroups
+// groups in all on `st` (same results as k_search_multi).
+int32_t search_groups_enqueue(hipStream_t st, xg_session *timer, const xg_picture *dPics, const SearchList *dLists,
+                              int nLists, int nGroups, int range, uint32_t lambda, int ctu);
 // Enqueues k_search_multi over lists[0..nLists) (device) covering nBlocks
 // blocks in all on `st`: each block's predictor from its stream's MV field,
 // then the search +/-range around it.  maxW/maxH: the largest block sides.
@@ -133,7 +143,7 @@ struct SessionView {
     uint64_t blocksVersion;        // bumped by every xg_session_set_blocks
 };
 bool session_view(xg_session *s, SessionView *v);
-constexpr int kSessionPicSlots = 64;
+constexpr int kSessionPicSlots = 128;
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 

@@ -737,3 +737,289 @@ int32_t xg::search_multi_enqueue(hipStream_t st, xg_session *timer, const xg_pic
     if (timed) session_timer_end(timer, 3, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
+
+// ---------------------------------------------------------------------------
+// Serving search by CTU group
+//
+// k_search_multi spends a workgroup and a window staging per block; at 1080p
+// most blocks are 4..16 wide, so most of its time is setup.  Here a workgroup
+// of four waves takes a group (up to 64 blocks of one CTU): it derives every
+// block's predictor and window, stages the union of the windows once in LDS
+// (with the CTU's current samples), and spreads (block, 64 candidate-rows)
+// units over its waves; each block's minimum (cost << 32 | raster index in
+// its own window) is kept by an LDS 64-bit atomic min, so the winner and the
+// tie order are those of search_block.  The LDS window starts at an even
+// column; a block whose window starts at an odd one reads from one column to
+// the left and skips its first candidate (s = 1).  A group whose union does
+// not fit stages one block's window at a time instead.
+// ---------------------------------------------------------------------------
+namespace xg {
+
+constexpr int kGroupWaves = 4;
+constexpr int kGroupCurDw = 64 * 32;             // the group's current samples: 64 rows x 32 dwords
+constexpr int kGroupK = 4;                       // candidates per lane
+
+struct GroupBlk {
+    int b;                                       // block index in the list
+    int x0, y0, nx, ny;                          // window in integer pel
+    int ox, oy;                                  // window origin in the LDS window (ox even)
+    int s;                                       // 1 when the origin is one column left of x0
+    int nq, items, w, h;
+    int pmx, pmy;                                // predictor, quarter pel
+    int cur;                                     // dword offset of the block in the current samples
+};
+
+// K SADs (candidates K q .. K q + K - 1 of origin-relative column) of one
+// item for a block BW wide, h rows; window rows ws2 dwords apart, current rows
+// 32 dwords apart.
+template <int BW>
+__device__ __forceinline__ void group_item(const uint32_t *rp, int ws2, const uint32_t *cp, int h, uint32_t acc[kGroupK])
+{
+    constexpr int ND = BW / 2, KD = kGroupK / 2;
+#pragma unroll
+    for (int k = 0; k < kGroupK; k++) acc[k] = 0;
+#pragma unroll 2
+    for (int i = 0; i < h; i++) {
+        uint32_t D[ND + KD], F[ND + KD - 1], C[ND];
+#pragma unroll
+        for (int j = 0; j < ND + KD; j++) D[j] = rp[i * ws2 + j];
+#pragma unroll
+        for (int j = 0; j < ND; j++) C[j] = cp[i * 32 + j];
+#pragma unroll
+        for (int j = 0; j < ND + KD - 1; j++) F[j] = align16(D[j + 1], D[j], 16);
+#pragma unroll
+        for (int j = 0; j < ND; j++)
+#pragma unroll
+            for (int m = 0; m < KD; m++) {
+                acc[2 * m] = __builtin_amdgcn_sad_u16(D[j + m], C[j], acc[2 * m]);
+                acc[2 * m + 1] = __builtin_amdgcn_sad_u16(F[j + m], C[j], acc[2 * m + 1]);
+            }
+    }
+}
+
+// Runs units [u0, u1) (stride kGroupWaves from the wave's first) of blocks
+// [j0, j1): unit u of block j covers items 64 (u - unit[j]) .. + 63.
+__device__ __forceinline__ void group_units(const uint32_t *win, int ws, const uint32_t *cur, const GroupBlk *gb,
+                                            const int *unit, int j0, int j1, unsigned long long *gkey,
+                                            uint32_t lambda)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int total = unit[j1] - unit[j0];
+    for (int u = unit[j0] + wave; u < unit[j0] + total; u += kGroupWaves) {
+        int lo = j0, hi = j1 - 1;                 // the block whose units hold u
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (unit[mid] <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        const GroupBlk &G = gb[lo];
+        const int it = (u - unit[lo]) * 64 + lane;
+        unsigned long long key = ~0ull;
+        if (it < G.items) {
+            const int dyi = it / G.nq, q = it - dyi * G.nq;
+            const uint32_t *rp = win + (((G.oy + dyi) * ws + G.ox) >> 1) + (kGroupK / 2) * q;
+            const uint32_t *cp = cur + G.cur;
+            uint32_t acc[kGroupK];
+            switch (G.w) {
+            case 4: group_item<4>(rp, ws >> 1, cp, G.h, acc); break;
+            case 8: group_item<8>(rp, ws >> 1, cp, G.h, acc); break;
+            case 16: group_item<16>(rp, ws >> 1, cp, G.h, acc); break;
+            case 32: group_item<32>(rp, ws >> 1, cp, G.h, acc); break;
+            default: group_item<64>(rp, ws >> 1, cp, G.h, acc); break;
+            }
+            const int my = G.y0 + dyi;
+            const uint32_t by = eg_bits(4 * my - G.pmy);
+#pragma unroll
+            for (int k = 0; k < kGroupK; k++) {
+                const int dx = kGroupK * q + k - G.s;
+                if (dx >= 0 && dx < G.nx) {
+                    const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * (G.x0 + dx) - G.pmx) + by) * lambda;
+                    const unsigned long long kk = ((unsigned long long)cost << 32) | (uint32_t)(dyi * G.nx + dx);
+                    key = kk < key ? kk : key;
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const unsigned long long o = (unsigned long long)__shfl_xor((long long)key, m, 64);
+            key = o < key ? o : key;
+        }
+        if (lane == 0) atomicMin(&gkey[lo], key);
+    }
+}
+
+// rows x (ws/2) dwords of the reference luma plane from picture column cx
+// (even) and row cy into win.
+__device__ __forceinline__ void stage_win_dw(uint32_t *win, int ws, const xg_plane &pl, int cx, int cy, int rows)
+{
+    const int nd = ws >> 1, n = rows * nd;
+    const uint16_t *base = pl.data + (intptr_t)cy * pl.stride + cx;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int r = i / nd, c = i - r * nd;
+        win[i] = *reinterpret_cast<const uint32_t *>(base + (intptr_t)r * pl.stride + 2 * c);
+    }
+}
+
+__global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_picture *pics,
+                                                                    const SearchList *__restrict__ lists, int nLists,
+                                                                    int range, uint32_t lambda, int ctu, int winDw)
+{
+    extern __shared__ uint32_t win[];                        // winDw dwords
+    __shared__ uint32_t cur[kGroupCurDw];
+    __shared__ GroupBlk gb[kSearchGroupMax];
+    __shared__ unsigned long long gkey[kSearchGroupMax];
+    __shared__ int unit[kSearchGroupMax + 1];
+    __shared__ int box[6];                                   // union x0, y0, x1, y1; current x0, y0
+    int l = 0;
+    while (l + 1 < nLists && (int)blockIdx.x >= lists[l + 1].begin) l++;
+    const SearchList &L = lists[l];
+    const int2 G = L.groups[(int)blockIdx.x - L.begin];
+    const int count = G.y, t = threadIdx.x;
+    const xg_picture &P = pics[L.curSlot], &R = pics[L.refSlot];
+    const int fw = P.plane[0].width, fh = P.plane[0].height;
+    if (t == 0) {
+        box[0] = box[1] = INT_MAX;
+        box[2] = box[3] = INT_MIN;
+        box[4] = box[5] = INT_MAX;
+    }
+    __syncthreads();
+    xg_block B;
+    int x0 = 0, y0 = 0, nx = 0, ny = 0;
+    if (t < count) {
+        const int b = L.gidx[G.x + t];
+        B = L.blocks[b];
+        B.pic = L.curSlot;
+        B.ref = L.refSlot;
+        const int2 p = L.mvField[(intptr_t)((B.y + (B.h >> 1)) >> 2) * L.mvStride + ((B.x + (B.w >> 1)) >> 2)];
+        B.mvpx = p.x;
+        B.mvpy = p.y;
+        const int mnx = (int16_t)(-B.x - ctu - 8), mny = (int16_t)(-B.y - ctu - 8);
+        const int mxx = (int16_t)(fw + (ctu - B.w + 8) - B.x), mxy = (int16_t)(fh + (ctu - B.h + 8) - B.y);
+        const int cx = min(max(((B.mvpx >> 2) + 2) >> 2, mnx), mxx), cy = min(max(((B.mvpy >> 2) + 2) >> 2, mny), mxy);
+        x0 = max(cx - range, mnx);
+        y0 = max(cy - range, mny);
+        nx = min(cx + range, mxx) - x0 + 1;
+        ny = min(cy + range, mxy) - y0 + 1;
+        const int sx = (B.x + x0) & 1;                       // origin one column left when odd
+        const int nq = (nx + sx + kGroupK - 1) / kGroupK;
+        GroupBlk g;
+        g.b = b;
+        g.x0 = x0; g.y0 = y0; g.nx = nx; g.ny = ny;
+        g.s = sx;
+        g.nq = nq;
+        g.items = ny * nq;
+        g.w = B.w; g.h = B.h;
+        g.pmx = B.mvpx >> 2; g.pmy = B.mvpy >> 2;
+        gb[t] = g;
+        gkey[t] = ~0ull;
+        atomicMin(&box[0], B.x + x0 - sx);
+        atomicMin(&box[1], B.y + y0);
+        atomicMax(&box[2], B.x + x0 - sx + kGroupK * nq + B.w);   // columns the last item reads, exclusive
+        atomicMax(&box[3], B.y + y0 + ny - 1 + B.h);
+        atomicMin(&box[4], (int)B.x);
+        atomicMin(&box[5], (int)B.y);
+    }
+    __syncthreads();
+    const int ux0 = box[0], uy0 = box[1], ws = ((box[2] - box[0]) + 1) & ~1, rows = box[3] - box[1];
+    const int cx0 = box[4], cy0 = box[5];
+    if (t < count) {
+        gb[t].ox = B.x + x0 - gb[t].s - ux0;
+        gb[t].oy = B.y + y0 - uy0;
+        gb[t].cur = (B.y - cy0) * 32 + ((B.x - cx0) >> 1);
+    }
+    if (t == 0) {
+        unit[0] = 0;
+        for (int j = 0; j < count; j++) unit[j + 1] = unit[j] + (gb[j].items + 63) / 64;
+    }
+    // the group's current samples (within 64x64 of (cx0, cy0): checked on the host)
+    for (int i = t; i < kGroupCurDw; i += blockDim.x) {
+        const int r = i >> 5, c = i & 31;
+        const bool in = cy0 + r < fh && cx0 + 2 * c + 1 < fw + 1 && cx0 + 2 * c < fw;
+        cur[i] = in ? *reinterpret_cast<const uint32_t *>(P.plane[0].data + (intptr_t)(cy0 + r) * P.plane[0].stride +
+                                                           cx0 + 2 * c)
+                    : 0u;
+    }
+    if ((long)rows * (ws >> 1) <= winDw) {
+        stage_win_dw(win, ws, R.plane[0], ux0, uy0, rows);
+        __syncthreads();
+        group_units(win, ws, cur, gb, unit, 0, count, gkey, lambda);
+    } else {
+        // one block's window at a time
+        for (int j = 0; j < count; j++) {
+            __syncthreads();                                 // the previous block's readers are done
+            const GroupBlk g = gb[j];
+            const int bx = ux0 + g.ox, byy = uy0 + g.oy;     // the block's window origin in the picture
+            const int wsj = (kGroupK * g.nq + g.w + 1) & ~1, rj = g.ny + g.h - 1;
+            stage_win_dw(win, wsj, R.plane[0], bx, byy, rj);
+            if (t == 0) {
+                gb[j].ox = 0;
+                gb[j].oy = 0;
+            }
+            __syncthreads();
+            group_units(win, wsj, cur, gb, unit, j, j + 1, gkey, lambda);
+        }
+    }
+    __syncthreads();
+    if (t < count) {
+        const unsigned long long k = gkey[t];
+        const int i = (int)(uint32_t)k;
+        B.mvx = 16 * (x0 + i % nx);
+        B.mvy = 16 * (y0 + i / nx);
+        L.blocks[gb[t].b] = B;
+    }
+}
+
+}  // namespace xg
+
+void xg::search_groups(const xg_block *blocks, int n, std::vector<int> &gidx, std::vector<int2> &groups)
+{
+    // blocks of each 64x64 CTU (by top-left sample), in list order
+    std::vector<std::pair<long, int>> key(n);
+    for (int i = 0; i < n; i++) key[i] = {((long)(blocks[i].y >> 6) << 20) | (blocks[i].x >> 6), i};
+    std::stable_sort(key.begin(), key.end(), [](const std::pair<long, int> &a, const std::pair<long, int> &b) {
+        return a.first < b.first;
+    });
+    gidx.clear();
+    groups.clear();
+    for (int a = 0; a < n;) {
+        int e = a;
+        while (e < n && key[e].first == key[a].first) e++;
+        int x0 = INT_MAX, y0 = INT_MAX, x1 = INT_MIN, y1 = INT_MIN;
+        for (int k = a; k < e; k++) {
+            const xg_block &B = blocks[key[k].second];
+            x0 = std::min(x0, (int)B.x);
+            y0 = std::min(y0, (int)B.y);
+            x1 = std::max(x1, B.x + B.w);
+            y1 = std::max(y1, B.y + B.h);
+        }
+        const bool fits = x1 - x0 <= 64 && y1 - y0 <= 64;
+        for (int k = a; k < e;) {
+            const int c = fits ? std::min(kSearchGroupMax, e - k) : 1;   // blocks past 64x64: one per group
+            groups.push_back(make_int2((int)gidx.size(), c));
+            for (int m = 0; m < c; m++) gidx.push_back(key[k + m].second);
+            k += c;
+        }
+        a = e;
+    }
+}
+
+int32_t xg::search_groups_enqueue(hipStream_t st, xg_session *timer, const xg_picture *dPics, const SearchList *dLists,
+                                  int nLists, int nGroups, int range, uint32_t lambda, int ctu)
+{
+    if (nLists <= 0 || nGroups <= 0) return XG_OK;
+    // room for one 64x64 block's window at this range (the fallback), and for
+    // the union of a CTU's windows when their predictors agree
+    const int nqMax = (2 * range + 2 + kGroupK - 1) / kGroupK;
+    const int one = (2 * range + 64) * (((kGroupK * nqMax + 64) + 1) / 2);
+    const int winDw = std::max(one, (2 * range + 64 + 4) * ((2 * range + 64 + kGroupK + 8) / 2));
+    if (winDw * 4 + (int)(kGroupCurDw * 4 + kSearchGroupMax * (sizeof(GroupBlk) + 12)) > 64 * 1024) {
+        set_error(XG_ERR_ARG, "search_groups_enqueue: window exceeds the LDS budget");
+        return XG_ERR_ARG;
+    }
+    int ev[2];
+    const bool timed = timer && session_timer_begin(timer, 3, ev);
+    k_search_groups<<<nGroups, 64 * kGroupWaves, (size_t)winDw * 4, st>>>(dPics, dLists, nLists, range, lambda, ctu,
+                                                                          winDw);
+    if (timed) session_timer_end(timer, 3, ev);
+    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
+}

@@ -1,8 +1,9 @@
 // xg_serve.hip -- serving: continuous batching of frames from many streams.
 //
 // Every admitted stream owns one HBM allocation that stays resident for its
-// life: two padded pictures (the current frame and its reference, which swap
-// roles after every frame), the prediction picture, a 4x4-unit MV field (the
+// life: three padded pictures used in rotation (frame k is read into picture
+// k mod 3 and refers to picture (k - 1) mod 3, so the next frame's upload
+// never touches a picture in use), the prediction picture, a 4x4-unit MV field (the
 // final MVs of its last frame, the source of the next frame's predictors),
 // its block partition with the persistent kernel's task assignment, and the
 // per-block outputs.  A partition is uploaded and assigned to workgroups only
@@ -11,15 +12,20 @@
 // Frames are submitted without staging when the caller's planes are in
 // page-locked host memory or on the device (xg_server_submit_frame): the
 // dispatcher reads them by DMA.  A dispatcher thread keeps up to two batches
-// in flight, one per device session (each with its own HIP stream).  When a
-// session is free it takes the oldest pending frame of every stream that has
-// no frame in flight and enqueues, for all of them together: the plane copies,
-// one padding launch, one integer-search launch (k_search_multi: each block's
-// predictor read from its stream's MV field, then +/-range around it) and one
-// refinement + MC launch (k_refine_multi, which also writes the winners into
-// the MV fields), then the result copies.  The launches walk a per-batch
-// table of per-stream descriptors, so a batch costs the host a few hundred
-// bytes of descriptors, not a merged block list.
+// in flight, one per device session (each with its own HIP stream), and each
+// stream up to two frames.  When a session is free it takes the oldest pending
+// frame of every stream with fewer than two frames in flight and enqueues, for
+// all of them together: the plane copies and one padding launch; then, on the
+// device, a wait for the other session's refinement (which wrote the MV
+// fields, references and partitions this batch reads); then one
+// integer-search launch (k_search_groups, a workgroup per group of a CTU's
+// blocks: each block's predictor read from its stream's MV field, then
+// +/-range around it) and one refinement + MC launch (k_refine_multi, which
+// also writes the winners into the MV fields), then the result copies.
+// Results and predictions rotate with the pictures, so those copies never
+// hold up the next batch.  The launches walk a per-batch table of per-stream
+// descriptors, so a batch costs the host a few hundred bytes of descriptors,
+// not a merged block list.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,7 +80,10 @@ __global__ void __launch_bounds__(256) k_pad_multi(const PadPlane *__restrict__ 
 
 namespace {
 
-constexpr int kMaxStreams = kSessionPicSlots / 2;
+constexpr int kPics = 3;                          // pictures per stream, in rotation
+constexpr int kMaxStreams = 32;
+static_assert(kMaxStreams * kPics <= kSessionPicSlots, "picture slots");
+constexpr int kMaxInFlight = 2;                    // frames per stream on the device at once
 constexpr int kSessions = 2;
 
 struct Request {
@@ -88,11 +97,16 @@ struct Request {
     size_t stagedBytes = 0;
     uint16_t *predStage = nullptr;     // pageable prediction outputs: read back here first
     size_t predStageBytes = 0;
+    xg_block_result *resStage = nullptr;   // pageable results: read back here first
+    size_t resStageBytes = 0;
     bool newPartition = false;         // this frame brings a partition (uploaded at launch)
     std::vector<xg_block> blocks;
     std::vector<int2> tasksW;
     std::vector<int> wgBegin;
+    std::vector<int> gidx;             // the partition's search groups (k_search_groups)
+    std::vector<int2> groups;
     int nblocks = 0;
+    int curPic = 0, refPic = 0;        // the stream's pictures this frame reads into / refers to
     int outputsKind = XG_MEM_HOST;
     xg_block_result *results = nullptr;
     xg_pixel *pred[3] = {nullptr, nullptr, nullptr};
@@ -101,31 +115,34 @@ struct Request {
 
 // Device state of a stream, carved from one allocation.
 struct StreamDev {
-    xg_picture pic[2];                 // padded current / reference pictures
-    xg_picture pred;                   // unpadded prediction picture
+    xg_picture pic[kPics];             // padded pictures in rotation
+    xg_picture pred[kPics];            // unpadded prediction of the frame read into picture k
     int2 *field = nullptr;             // 4x4-unit MV field, fieldStride units per row
     int fieldStride = 0;
     xg_block *blocks = nullptr;        // partition (slots, predictor and base MV rewritten per frame)
     int2 *tasks = nullptr;             // workgroup-major task assignment
     int *wgBegin = nullptr;
+    int *gidx = nullptr;               // search groups: block indices, group by group
+    int2 *groups = nullptr;            //   and (first entry, count) per group
     uint32_t *candCost = nullptr;
-    xg_block_result *best = nullptr;
+    xg_block_result *best[kPics] = {}; // results of the frame read into picture k
     unsigned long long *bestKey = nullptr;
     int *done = nullptr;
 };
 
 struct Stream {
-    bool open = false, closing = false, inFlight = false, hasRef = false;
-    int w = 0, h = 0, cur = 0;
+    bool open = false, closing = false;
+    int inFlight = 0;                  // frames launched and not yet finalised
+    int64_t launched = 0;              // frames launched so far: the next one reads picture launched % kPics
+    int w = 0, h = 0;
     void *mem = nullptr;
     size_t bytes = 0;
     StreamDev d;
     int maxBlocks = 0, taskCap = 0;
-    xg_block_result *hRes = nullptr;   // page-locked staging for pageable result buffers
     // partition last accepted at submit (host copy)
     std::vector<xg_block> lastBlocks;
     // partition currently on the device (set when a frame carrying it launches)
-    int devBlocks = 0, devTasks = 0, devMaxW = 0, devMaxH = 0;
+    int devBlocks = 0, devTasks = 0, devGroups = 0;
     std::deque<Request *> pending;
     int64_t frames = 0;
 };
@@ -146,7 +163,8 @@ T *dev_member(void *base, size_t offset)
 struct Batch {
     bool active = false;
     std::vector<Request *> reqs;
-    hipEvent_t done = nullptr;
+    hipEvent_t done = nullptr;         // after the batch's output copies
+    hipEvent_t computed = nullptr;     // after its refinement: what the next batch's kernels depend on
     int64_t startNs = 0;
     Descs *hDesc = nullptr;
     Descs *dDesc = nullptr;
@@ -205,6 +223,7 @@ struct xg_server {
         else framesDropped++;
         pinned_put(r->staged, r->stagedBytes);
         pinned_put(r->predStage, r->predStageBytes);
+        pinned_put(reinterpret_cast<uint16_t *>(r->resStage), r->resStageBytes);
         delete r;
     }
 
@@ -216,7 +235,7 @@ struct xg_server {
         for (Request *r : b.reqs) {
             Stream &st = streams[r->stream];
             if (r->outputsKind == XG_MEM_HOST) {
-                memcpy(r->results, st.hRes, sizeof(xg_block_result) * (size_t)r->nblocks);
+                memcpy(r->results, r->resStage, sizeof(xg_block_result) * (size_t)r->nblocks);
                 if (r->pred[0]) {
                     const int cw = st.w / 2, ch = st.h / 2;
                     unpack2d(r->pred[0], r->predStride[0], r->predStage, st.w, st.h, 2);
@@ -225,9 +244,7 @@ struct xg_server {
                              ch, 2);
                 }
             }
-            st.cur ^= 1;           // this frame's picture becomes the next frame's reference
-            st.hasRef = true;
-            st.inFlight = false;
+            st.inFlight--;
             st.frames++;
             finish_request(r, XG_OK, nf, b.startNs);
         }
@@ -242,7 +259,7 @@ struct xg_server {
         Batch &b = batch[si];
         log_msg(0, "serve: batch of %zu frames failed (%d)", b.reqs.size(), rc);
         for (Request *r : b.reqs) {
-            streams[r->stream].inFlight = false;
+            streams[r->stream].inFlight--;
             finish_request(r, rc, (int)b.reqs.size(), b.startNs);
         }
         b.reqs.clear();
@@ -256,38 +273,19 @@ struct xg_server {
         return toDevice ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
     }
 
-    // Enqueues frame r's inputs: a new partition, the planes into the current
-    // picture (padding is one launch for the whole batch, afterwards).
-    int32_t stage_inputs(Request *r, hipStream_t hs)
+    // Enqueues frame r's planes into its picture (padding is one launch for
+    // the whole batch, afterwards).  The picture is one no frame in flight uses.
+    int32_t stage_planes(Request *r, hipStream_t hs)
     {
         Stream &st = streams[r->stream];
-        if (r->newPartition) {
-            const size_t nb = r->blocks.size(), nt = r->tasksW.size();
-            if (hipMemcpyAsync(st.d.blocks, r->blocks.data(), sizeof(xg_block) * nb, hipMemcpyHostToDevice, hs) !=
-                    hipSuccess ||
-                (nt && hipMemcpyAsync(st.d.tasks, r->tasksW.data(), sizeof(int2) * nt, hipMemcpyHostToDevice, hs) !=
-                           hipSuccess) ||
-                hipMemcpyAsync(st.d.wgBegin, r->wgBegin.data(), sizeof(int) * r->wgBegin.size(),
-                               hipMemcpyHostToDevice, hs) != hipSuccess)
-                return XG_ERR_HIP;
-            int mw = 0, mh = 0;
-            for (const xg_block &B : r->blocks) {
-                mw = std::max(mw, (int)B.w);
-                mh = std::max(mh, (int)B.h);
-            }
-            st.devBlocks = (int)nb;
-            st.devTasks = (int)nt;
-            st.devMaxW = mw;
-            st.devMaxH = mh;
-        }
-        const xg_picture &P = st.d.pic[st.cur];
+        const xg_picture &P = st.d.pic[r->curPic];
         const uint16_t *src = r->staged;
         for (int p = 0; p < 3; p++) {
             const xg_plane &pl = P.plane[p];
-            const uint16_t *s = r->staged ? src : r->planes[p];
+            const uint16_t *sp = r->staged ? src : r->planes[p];
             const size_t spitch = r->staged ? (size_t)pl.width * 2 : (size_t)r->strides[p] * 2;
             const hipMemcpyKind k = r->staged ? hipMemcpyHostToDevice : kind_of(r->planesKind, true);
-            if (hipMemcpy2DAsync(pl.data, pl.stride * 2, s, spitch, (size_t)pl.width * 2, pl.height, k, hs) !=
+            if (hipMemcpy2DAsync(pl.data, pl.stride * 2, sp, spitch, (size_t)pl.width * 2, pl.height, k, hs) !=
                 hipSuccess)
                 return XG_ERR_HIP;
             src += (size_t)pl.width * pl.height;
@@ -295,18 +293,39 @@ struct xg_server {
         return XG_OK;
     }
 
+    // Enqueues frame r's new partition, if it brings one (after the wait for
+    // the frames that still use the old one).
+    int32_t stage_partition(Request *r, hipStream_t hs)
+    {
+        if (!r->newPartition) return XG_OK;
+        Stream &st = streams[r->stream];
+        const size_t nb = r->blocks.size(), nt = r->tasksW.size();
+        if (hipMemcpyAsync(st.d.blocks, r->blocks.data(), sizeof(xg_block) * nb, hipMemcpyHostToDevice, hs) !=
+                hipSuccess ||
+            (nt && hipMemcpyAsync(st.d.tasks, r->tasksW.data(), sizeof(int2) * nt, hipMemcpyHostToDevice, hs) !=
+                       hipSuccess) ||
+            hipMemcpyAsync(st.d.wgBegin, r->wgBegin.data(), sizeof(int) * r->wgBegin.size(), hipMemcpyHostToDevice,
+                           hs) != hipSuccess ||
+            hipMemcpyAsync(st.d.gidx, r->gidx.data(), sizeof(int) * r->gidx.size(), hipMemcpyHostToDevice, hs) !=
+                hipSuccess ||
+            hipMemcpyAsync(st.d.groups, r->groups.data(), sizeof(int2) * r->groups.size(), hipMemcpyHostToDevice,
+                           hs) != hipSuccess)
+            return XG_ERR_HIP;
+        return XG_OK;
+    }
+
     // Enqueues frame r's outputs: results and, when asked for, the prediction.
     int32_t read_outputs(Request *r, hipStream_t hs)
     {
         Stream &st = streams[r->stream];
-        xg_block_result *dst = r->outputsKind == XG_MEM_HOST ? st.hRes : r->results;
-        if (hipMemcpyAsync(dst, st.d.best, sizeof(xg_block_result) * (size_t)r->nblocks,
+        xg_block_result *dst = r->outputsKind == XG_MEM_HOST ? r->resStage : r->results;
+        if (hipMemcpyAsync(dst, st.d.best[r->curPic], sizeof(xg_block_result) * (size_t)r->nblocks,
                            kind_of(r->outputsKind, false), hs) != hipSuccess)
             return XG_ERR_HIP;
         if (!r->pred[0]) return XG_OK;
         uint16_t *stage = r->predStage;
         for (int p = 0; p < 3; p++) {
-            const xg_plane &pl = st.d.pred.plane[p];
+            const xg_plane &pl = st.d.pred[r->curPic].plane[p];
             uint16_t *d = r->outputsKind == XG_MEM_HOST ? stage : r->pred[p];
             const size_t dpitch = r->outputsKind == XG_MEM_HOST ? (size_t)pl.width * 2 : (size_t)r->predStride[p] * 2;
             if (hipMemcpy2DAsync(d, dpitch, pl.data, pl.stride * 2, (size_t)pl.width * 2, pl.height,
@@ -317,14 +336,18 @@ struct xg_server {
         return XG_OK;
     }
 
+    bool launchable(const Stream &st) const
+    {
+        return st.open && !st.closing && st.inFlight < kMaxInFlight && !st.pending.empty();
+    }
+
     // Assembles and enqueues one batch on session si; returns false if nothing to do.
     bool launch(int si)
     {
         Batch &b = batch[si];
         std::vector<int> ids;
         for (int i = 0; i < kMaxStreams; i++)
-            if (streams[i].open && !streams[i].closing && !streams[i].inFlight && !streams[i].pending.empty())
-                ids.push_back(i);
+            if (launchable(streams[i])) ids.push_back(i);
         if (ids.empty()) return false;
         std::sort(ids.begin(), ids.end(), [&](int x, int y) {
             return streams[x].pending.front()->submitNs < streams[y].pending.front()->submitNs;
@@ -332,10 +355,14 @@ struct xg_server {
         b.reqs.clear();
         int nblk = 0;
         for (int id : ids) {
-            Request *r = streams[id].pending.front();
+            Stream &st = streams[id];
+            Request *r = st.pending.front();
             if (!b.reqs.empty() && nblk + r->nblocks > cfg.maxBatchBlocks) break;
-            streams[id].pending.pop_front();
-            streams[id].inFlight = true;
+            st.pending.pop_front();
+            r->curPic = (int)(st.launched % kPics);
+            r->refPic = st.launched ? (int)((st.launched + kPics - 1) % kPics) : r->curPic;
+            st.launched++;
+            st.inFlight++;
             nblk += r->nblocks;
             b.reqs.push_back(r);
         }
@@ -344,27 +371,31 @@ struct xg_server {
         b.startNs = now_ns();
         int32_t rc = XG_OK;
         for (Request *r : b.reqs)
-            if (rc == XG_OK) rc = stage_inputs(r, hs);
+            if (rc == XG_OK) rc = stage_planes(r, hs);
         // descriptor tables of the batch
         Descs &D = *b.hDesc;
         const int nf = (int)b.reqs.size();
-        int begin = 0, taskAcc = 0, maxW = 4, maxH = 4;
+        int begin = 0, taskAcc = 0;
         for (int i = 0; i < nf && rc == XG_OK; i++) {
-            Stream &st = streams[b.reqs[i]->stream];
-            const int id = b.reqs[i]->stream;
-            const uint8_t curSlot = (uint8_t)(2 * id + st.cur);
-            const uint8_t refSlot = (uint8_t)(st.hasRef ? 2 * id + (st.cur ^ 1) : curSlot);
-            D.search[i] = SearchList{st.d.blocks, st.d.field, st.d.fieldStride, begin, curSlot, refSlot};
-            begin += st.devBlocks;
-            maxW = std::max(maxW, st.devMaxW);
-            maxH = std::max(maxH, st.devMaxH);
+            Request *r = b.reqs[i];
+            Stream &st = streams[r->stream];
+            const int id = r->stream;
+            if (r->newPartition) {        // the device partition changes with this frame
+                st.devBlocks = (int)r->blocks.size();
+                st.devTasks = (int)r->tasksW.size();
+                st.devGroups = (int)r->groups.size();
+            }
+            D.search[i] = SearchList{st.d.blocks, st.d.field, st.d.fieldStride, begin,
+                                     (uint8_t)(kPics * id + r->curPic), (uint8_t)(kPics * id + r->refPic),
+                                     st.d.gidx, st.d.groups};
+            begin += st.devGroups;
             // spread small partitions over the grid, in steps of whole XCD bands
             const int rot = (taskAcc / 8 * 8) % grid;
             taskAcc += st.devTasks;
-            D.refine[i] = RefineList{st.d.blocks, st.d.tasks, st.d.wgBegin, st.d.candCost, st.d.best,
+            D.refine[i] = RefineList{st.d.blocks, st.d.tasks, st.d.wgBegin, st.d.candCost, st.d.best[r->curPic],
                                      st.d.bestKey, st.d.done, st.d.field, st.d.fieldStride, rot};
             for (int p = 0; p < 3; p++) {
-                const xg_plane &pl = st.d.pic[st.cur].plane[p];
+                const xg_plane &pl = st.d.pic[r->curPic].plane[p];
                 D.pad[3 * i + p] = PadPlane{pl.data, pl.stride, pl.width, pl.height, pl.pad};
             }
         }
@@ -376,14 +407,22 @@ struct xg_server {
             k_pad_multi<<<dim3(rows, 3 * nf), 256, 0, hs>>>(dev_member<PadPlane>(b.dDesc, offsetof(Descs, pad)));
             if (hipGetLastError() != hipSuccess) rc = XG_ERR_HIP;
         }
+        // the other session's batch holds the previous frames of these streams:
+        // their MV fields, references and partitions (outputs rotate with the
+        // pictures, so its output copies need not be waited for)
+        const Batch &o = batch[si ^ 1];
+        if (rc == XG_OK && o.active && hipStreamWaitEvent(hs, o.computed, 0) != hipSuccess) rc = XG_ERR_HIP;
+        for (Request *r : b.reqs)
+            if (rc == XG_OK) rc = stage_partition(r, hs);
         if (rc == XG_OK) {
             SessionView v;
             session_view(s, &v);
-            rc = search_multi_enqueue(hs, s, v.dPics, dev_member<SearchList>(b.dDesc, offsetof(Descs, search)), nf,
-                                      begin, maxW, maxH, cfg.searchRange, cfg.searchLambda, cfg.refine.ctuSize);
+            rc = search_groups_enqueue(hs, s, v.dPics, dev_member<SearchList>(b.dDesc, offsetof(Descs, search)), nf,
+                                       begin, cfg.searchRange, cfg.searchLambda, cfg.refine.ctuSize);
         }
         if (rc == XG_OK)
             rc = refine_multi_enqueue(s, &cfg.refine, dev_member<RefineList>(b.dDesc, offsetof(Descs, refine)), nf);
+        if (rc == XG_OK && hipEventRecord(b.computed, hs) != hipSuccess) rc = XG_ERR_HIP;
         for (Request *r : b.reqs)
             if (rc == XG_OK) rc = read_outputs(r, hs);
         if (rc == XG_OK && hipEventRecord(b.done, hs) != hipSuccess) rc = XG_ERR_HIP;
@@ -416,7 +455,7 @@ struct xg_server {
         int n = 0;
         for (int i = 0; i < kMaxStreams; i++) {
             const Stream &st = streams[i];
-            if (!st.open || st.closing || st.inFlight || st.pending.empty()) continue;
+            if (!launchable(st)) continue;
             oldest = std::min(oldest, st.pending.front()->submitNs);
             nblk += st.pending.front()->nblocks;
             n++;
@@ -516,7 +555,8 @@ namespace {
 
 // Per-stream device layout: offsets of every part of the one allocation.
 struct StreamLayout {
-    size_t pic[2], pred, field, blocks, tasks, wgBegin, candCost, best, bestKey, done, total;
+    size_t pic[kPics], pred[kPics], field, blocks, tasks, wgBegin, gidx, groups, candCost, best[kPics], bestKey,
+        done, total;
     int pad, ys, cs, maxBlocks, taskCap, fieldStride;
     size_t ypic, cpic;
 };
@@ -535,15 +575,25 @@ StreamLayout stream_layout(const xg_server_cfg &cfg, int width, int height, int 
     L.taskCap = L.maxBlocks * *taskPerBlock;
     L.fieldStride = width / 4;
     size_t o = 0;
-    L.pic[0] = o; o += picBytes;
-    L.pic[1] = o; o += picBytes;
-    L.pred = o; o += align_up((size_t)width * height * 3, 4096);
+    for (int k = 0; k < kPics; k++) {
+        L.pic[k] = o;
+        o += picBytes;
+    }
+    for (int k = 0; k < kPics; k++) {
+        L.pred[k] = o;
+        o += align_up((size_t)width * height * 3, 4096);
+    }
     L.field = o; o += align_up((size_t)L.maxBlocks * sizeof(int2), 256);
     L.blocks = o; o += align_up((size_t)L.maxBlocks * sizeof(xg_block), 256);
     L.tasks = o; o += align_up((size_t)L.taskCap * sizeof(int2), 256);
     L.wgBegin = o; o += align_up((size_t)(grid + 1) * sizeof(int), 256);
+    L.gidx = o; o += align_up((size_t)L.maxBlocks * sizeof(int), 256);
+    L.groups = o; o += align_up((size_t)L.maxBlocks * sizeof(int2), 256);
     L.candCost = o; o += align_up((size_t)L.maxBlocks * cfg.refine.ncand * sizeof(uint32_t), 256);
-    L.best = o; o += align_up((size_t)L.maxBlocks * sizeof(xg_block_result), 256);
+    for (int k = 0; k < kPics; k++) {
+        L.best[k] = o;
+        o += align_up((size_t)L.maxBlocks * sizeof(xg_block_result), 256);
+    }
     L.bestKey = o; o += align_up((size_t)L.maxBlocks * sizeof(unsigned long long), 256);
     L.done = o; o += align_up((size_t)L.maxBlocks * sizeof(int), 256);
     L.total = o;
@@ -589,6 +639,7 @@ int32_t xg_server_create(xg_server **out, const xg_server_cfg *cfg)
     for (int i = 0; i < kSessions; i++) {
         if (xg_session_create(&srv->sess[i], 1) != XG_OK || xg_session_set_sched(srv->sess[i], 8, 12) != XG_OK ||
             hipEventCreateWithFlags(&srv->batch[i].done, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&srv->batch[i].computed, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc((void **)&srv->batch[i].hDesc, sizeof(Descs), hipHostMallocDefault) != hipSuccess ||
             hipMalloc((void **)&srv->batch[i].dDesc, sizeof(Descs)) != hipSuccess) {
             set_error(XG_ERR_NOMEM, "xg_server_create: session allocation failed");
@@ -618,15 +669,16 @@ int32_t xg_server_destroy(xg_server *srv)
         for (Request *r : st.pending) {
             if (r->staged) (void)hipHostFree(r->staged);
             if (r->predStage) (void)hipHostFree(r->predStage);
+            if (r->resStage) (void)hipHostFree(r->resStage);
             delete r;
         }
         st.pending.clear();
         if (st.mem) (void)hipFree(st.mem);
-        if (st.hRes) (void)hipHostFree(st.hRes);
     }
     for (int i = 0; i < kSessions; i++) {
         if (srv->sess[i]) xg_session_destroy(srv->sess[i]);
         if (srv->batch[i].done) (void)hipEventDestroy(srv->batch[i].done);
+        if (srv->batch[i].computed) (void)hipEventDestroy(srv->batch[i].computed);
         if (srv->batch[i].hDesc) (void)hipHostFree(srv->batch[i].hDesc);
         if (srv->batch[i].dDesc) (void)hipFree(srv->batch[i].dDesc);
     }
@@ -654,38 +706,36 @@ int32_t xg_server_open_stream(xg_server *srv, int32_t width, int32_t height, int
     }
     const int tpb = max_tasks_per_block(srv->sess[0], srv->cfg.refine.ncand);
     const StreamLayout L = stream_layout(srv->cfg, width, height, srv->grid, &tpb);
-    const size_t resBytes = sizeof(xg_block_result) * (size_t)L.maxBlocks;
     if (srv->bytesSessions + srv->bytesStreams + L.total > srv->cfg.hbmBudget) {
         set_error(XG_ERR_BUDGET, "xg_server_open_stream: %zu bytes do not fit the HBM budget", L.total);
         return XG_ERR_BUDGET;
     }
     Stream &st = srv->streams[id];
     if (!XG_HIP(hipMalloc(&st.mem, L.total))) return XG_ERR_NOMEM;
-    if (!XG_HIP(hipHostMalloc((void **)&st.hRes, resBytes, hipHostMallocDefault))) {
-        (void)hipFree(st.mem);
-        st.mem = nullptr;
-        return XG_ERR_NOMEM;
-    }
     char *base = (char *)st.mem;
     const int pad = L.pad, cpad = pad / 2;
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < kPics; k++) {
         uint16_t *p = (uint16_t *)(base + L.pic[k]);
         st.d.pic[k].plane[0] = xg_plane{p + pad * L.ys + pad, L.ys, width, height, pad};
         uint16_t *c1 = p + L.ypic, *c2 = c1 + L.cpic;
         st.d.pic[k].plane[1] = xg_plane{c1 + cpad * L.cs + cpad, L.cs, width / 2, height / 2, cpad};
         st.d.pic[k].plane[2] = xg_plane{c2 + cpad * L.cs + cpad, L.cs, width / 2, height / 2, cpad};
     }
-    uint16_t *pr = (uint16_t *)(base + L.pred);
-    st.d.pred.plane[0] = xg_plane{pr, width, width, height, 0};
-    st.d.pred.plane[1] = xg_plane{pr + (size_t)width * height, width / 2, width / 2, height / 2, 0};
-    st.d.pred.plane[2] = xg_plane{pr + (size_t)width * height * 5 / 4, width / 2, width / 2, height / 2, 0};
+    for (int k = 0; k < kPics; k++) {
+        uint16_t *pr = (uint16_t *)(base + L.pred[k]);
+        st.d.pred[k].plane[0] = xg_plane{pr, width, width, height, 0};
+        st.d.pred[k].plane[1] = xg_plane{pr + (size_t)width * height, width / 2, width / 2, height / 2, 0};
+        st.d.pred[k].plane[2] = xg_plane{pr + (size_t)width * height * 5 / 4, width / 2, width / 2, height / 2, 0};
+        st.d.best[k] = (xg_block_result *)(base + L.best[k]);
+    }
     st.d.field = (int2 *)(base + L.field);
     st.d.fieldStride = L.fieldStride;
     st.d.blocks = (xg_block *)(base + L.blocks);
     st.d.tasks = (int2 *)(base + L.tasks);
     st.d.wgBegin = (int *)(base + L.wgBegin);
+    st.d.gidx = (int *)(base + L.gidx);
+    st.d.groups = (int2 *)(base + L.groups);
     st.d.candCost = (uint32_t *)(base + L.candCost);
-    st.d.best = (xg_block_result *)(base + L.best);
     st.d.bestKey = (unsigned long long *)(base + L.bestKey);
     st.d.done = (int *)(base + L.done);
     // MV field starts at zero; per-block completion words idle (all-ones / zero)
@@ -695,26 +745,25 @@ int32_t xg_server_open_stream(xg_server *srv, int32_t width, int32_t height, int
               XG_HIP(hipDeviceSynchronize());
     // the dispatcher holds no session call while we hold the lock
     for (int si = 0; si < kSessions && ok; si++)
-        for (int k = 0; k < 2 && ok; k++)
-            ok = xg_session_set_picture(srv->sess[si], 2 * id + k, &st.d.pic[k]) == XG_OK &&
-                 xg_session_set_pred(srv->sess[si], 2 * id + k, &st.d.pred) == XG_OK;
+        for (int k = 0; k < kPics && ok; k++)
+            ok = xg_session_set_picture(srv->sess[si], kPics * id + k, &st.d.pic[k]) == XG_OK &&
+                 xg_session_set_pred(srv->sess[si], kPics * id + k, &st.d.pred[k]) == XG_OK;
     if (!ok) {
         (void)hipFree(st.mem);
-        (void)hipHostFree(st.hRes);
         st.mem = nullptr;
-        st.hRes = nullptr;
         return XG_ERR_HIP;
     }
     st.open = true;
-    st.closing = st.inFlight = st.hasRef = false;
+    st.closing = false;
+    st.inFlight = 0;
+    st.launched = 0;
     st.w = width;
     st.h = height;
-    st.cur = 0;
     st.bytes = L.total;
     st.maxBlocks = L.maxBlocks;
     st.taskCap = L.taskCap;
     st.lastBlocks.clear();
-    st.devBlocks = st.devTasks = 0;
+    st.devBlocks = st.devTasks = st.devGroups = 0;
     st.frames = 0;
     srv->bytesStreams += L.total;
     *streamId = id;
@@ -738,11 +787,9 @@ int32_t xg_server_close_stream(xg_server *srv, int32_t streamId)
     }
     srv->cvDone.notify_all();
     srv->cvWork.notify_all();          // a batch held for more streams may go now
-    srv->cvDone.wait(lk, [&] { return !st.inFlight; });
+    srv->cvDone.wait(lk, [&] { return st.inFlight == 0; });
     (void)hipFree(st.mem);
-    (void)hipHostFree(st.hRes);
     st.mem = nullptr;
-    st.hRes = nullptr;
     srv->bytesStreams -= st.bytes;
     st.open = st.closing = false;
     st.lastBlocks.clear();
@@ -792,10 +839,16 @@ int32_t xg_server_submit_frame(xg_server *srv, int32_t streamId, const xg_server
             delete r;
             return rc;
         }
-        if (f->outputsKind == XG_MEM_HOST && f->pred[0]) {
-            r->predStageBytes = (size_t)w * h * 3;
-            r->predStage = srv->pinned_get(r->predStageBytes);
-            if (!r->predStage) {
+        if (f->outputsKind == XG_MEM_HOST) {
+            r->resStageBytes = sizeof(xg_block_result) * (size_t)r->nblocks;
+            r->resStage = reinterpret_cast<xg_block_result *>(srv->pinned_get(r->resStageBytes));
+            if (f->pred[0]) {
+                r->predStageBytes = (size_t)w * h * 3;
+                r->predStage = srv->pinned_get(r->predStageBytes);
+            }
+            if (!r->resStage || (f->pred[0] && !r->predStage)) {
+                srv->pinned_put(reinterpret_cast<uint16_t *>(r->resStage), r->resStageBytes);
+                srv->pinned_put(r->predStage, r->predStageBytes);
                 delete r;
                 set_error(XG_ERR_NOMEM, "xg_server_submit: pinned staging allocation failed");
                 return XG_ERR_NOMEM;
@@ -805,6 +858,7 @@ int32_t xg_server_submit_frame(xg_server *srv, int32_t streamId, const xg_server
             r->stagedBytes = (size_t)w * h * 3;
             r->staged = srv->pinned_get(r->stagedBytes);
             if (!r->staged) {
+                srv->pinned_put(reinterpret_cast<uint16_t *>(r->resStage), r->resStageBytes);
                 srv->pinned_put(r->predStage, r->predStageBytes);
                 delete r;
                 set_error(XG_ERR_NOMEM, "xg_server_submit: pinned staging allocation failed");
@@ -830,6 +884,7 @@ int32_t xg_server_submit_frame(xg_server *srv, int32_t streamId, const xg_server
         std::lock_guard<std::mutex> g(srv->mu);
         srv->pinned_put(r->staged, r->stagedBytes);
         srv->pinned_put(r->predStage, r->predStageBytes);
+        srv->pinned_put(reinterpret_cast<uint16_t *>(r->resStage), r->resStageBytes);
         delete r;
         return code;
     };
@@ -837,6 +892,7 @@ int32_t xg_server_submit_frame(xg_server *srv, int32_t streamId, const xg_server
         r->blocks.assign(f->blocks, f->blocks + f->nblocks);
         for (xg_block &B : r->blocks) B.mvx = B.mvy = B.mvpx = B.mvpy = 0;
         refine_assign(srv->sess[0], r->blocks.data(), f->nblocks, srv->cfg.refine.ncand, r->tasksW, r->wgBegin);
+        search_groups(r->blocks.data(), f->nblocks, r->gidx, r->groups);
         if ((int)r->tasksW.size() > taskCap) {
             set_error(XG_ERR_ARG, "xg_server_submit: partition needs %zu tasks, more than %d", r->tasksW.size(),
                       taskCap);
