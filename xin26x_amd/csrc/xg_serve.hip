@@ -78,6 +78,33 @@ __global__ void __launch_bounds__(256) k_pad_multi(const PadPlane *__restrict__ 
     }
 }
 
+// One device plane copied into a padded picture plane.
+struct UploadPlane {
+    const uint16_t *src;               // sample (0, 0), dword aligned, even stride
+    intptr_t sstride;
+    PadPlane dst;
+};
+
+// Copy and edge padding in one pass (device-resident frames): one workgroup
+// per padded destination row (blockIdx.y = plane), each thread moving sample
+// pairs; a pair left or right of the picture takes the clamped edge sample.
+__global__ void __launch_bounds__(256) k_upload_pad(const UploadPlane *__restrict__ planes)
+{
+    const UploadPlane U = planes[blockIdx.y];
+    const PadPlane &P = U.dst;
+    const int y = (int)blockIdx.x - P.pad;
+    if (y >= P.h + P.pad) return;
+    const int sy = min(max(y, 0), P.h - 1);
+    const uint16_t *src = U.src + (intptr_t)sy * U.sstride;
+    uint32_t *row = reinterpret_cast<uint32_t *>(P.data + (intptr_t)y * P.stride - P.pad);
+    const int pairs = (P.w + 2 * P.pad) >> 1;
+    const uint32_t lo = src[0] * 0x10001u, hi = src[P.w - 1] * 0x10001u;
+    for (int i = threadIdx.x; i < pairs; i += 256) {
+        const int x = 2 * i - P.pad;
+        row[i] = x < 0 ? lo : x >= P.w ? hi : *reinterpret_cast<const uint32_t *>(src + x);
+    }
+}
+
 namespace {
 
 constexpr int kPics = 3;                          // pictures per stream, in rotation
@@ -151,7 +178,8 @@ struct Stream {
 struct Descs {
     SearchList search[kMaxStreams];
     RefineList refine[kMaxStreams];
-    PadPlane pad[3 * kMaxStreams];
+    PadPlane pad[3 * kMaxStreams];     // planes copied by DMA, padded afterwards
+    UploadPlane up[3 * kMaxStreams];   // device planes copied and padded in one pass
 };
 
 template <typename T>
@@ -275,8 +303,18 @@ struct xg_server {
 
     // Enqueues frame r's planes into its picture (padding is one launch for
     // the whole batch, afterwards).  The picture is one no frame in flight uses.
+    // Device planes, dword aligned with even strides, take k_upload_pad.
+    static bool fused_upload(const Request *r)
+    {
+        if (r->staged || r->planesKind != XG_MEM_DEVICE) return false;
+        for (int p = 0; p < 3; p++)
+            if (((uintptr_t)r->planes[p] & 3) || (r->strides[p] & 1)) return false;
+        return true;
+    }
+
     int32_t stage_planes(Request *r, hipStream_t hs)
     {
+        if (fused_upload(r)) return XG_OK;
         Stream &st = streams[r->stream];
         const xg_picture &P = st.d.pic[r->curPic];
         const uint16_t *src = r->staged;
@@ -375,7 +413,7 @@ struct xg_server {
         // descriptor tables of the batch
         Descs &D = *b.hDesc;
         const int nf = (int)b.reqs.size();
-        int begin = 0, taskAcc = 0;
+        int begin = 0, taskAcc = 0, nPad = 0, nUp = 0;
         for (int i = 0; i < nf && rc == XG_OK; i++) {
             Request *r = b.reqs[i];
             Stream &st = streams[r->stream];
@@ -396,15 +434,23 @@ struct xg_server {
                                      st.d.bestKey, st.d.done, st.d.field, st.d.fieldStride, rot};
             for (int p = 0; p < 3; p++) {
                 const xg_plane &pl = st.d.pic[r->curPic].plane[p];
-                D.pad[3 * i + p] = PadPlane{pl.data, pl.stride, pl.width, pl.height, pl.pad};
+                const PadPlane pp{pl.data, pl.stride, pl.width, pl.height, pl.pad};
+                if (fused_upload(r)) D.up[nUp++] = UploadPlane{r->planes[p], r->strides[p], pp};
+                else D.pad[nPad++] = pp;
             }
         }
         if (rc == XG_OK && hipMemcpyAsync(b.dDesc, b.hDesc, sizeof(Descs), hipMemcpyHostToDevice, hs) != hipSuccess)
             rc = XG_ERR_HIP;
-        if (rc == XG_OK) {
+        if (rc == XG_OK && nPad) {
             int rows = 0;
-            for (int i = 0; i < 3 * nf; i++) rows = std::max(rows, D.pad[i].h + 2 * D.pad[i].pad);
-            k_pad_multi<<<dim3(rows, 3 * nf), 256, 0, hs>>>(dev_member<PadPlane>(b.dDesc, offsetof(Descs, pad)));
+            for (int i = 0; i < nPad; i++) rows = std::max(rows, D.pad[i].h + 2 * D.pad[i].pad);
+            k_pad_multi<<<dim3(rows, nPad), 256, 0, hs>>>(dev_member<PadPlane>(b.dDesc, offsetof(Descs, pad)));
+            if (hipGetLastError() != hipSuccess) rc = XG_ERR_HIP;
+        }
+        if (rc == XG_OK && nUp) {
+            int rows = 0;
+            for (int i = 0; i < nUp; i++) rows = std::max(rows, D.up[i].dst.h + 2 * D.up[i].dst.pad);
+            k_upload_pad<<<dim3(rows, nUp), 256, 0, hs>>>(dev_member<UploadPlane>(b.dDesc, offsetof(Descs, up)));
             if (hipGetLastError() != hipSuccess) rc = XG_ERR_HIP;
         }
         // the other session's batch holds the previous frames of these streams:
