@@ -190,46 +190,78 @@ struct RefineLds {
     uint32_t luma[kLumaLH * kLumaLW / 2];
     uint32_t cb[kChromaLH * kChromaLW / 2];
     uint32_t cr[kChromaLH * kChromaLW / 2];
-    uint32_t tapL[16 * 4];         // luma filter (index 0), packed tap pairs per phase
-    uint32_t tapC[32 * 2];         // chroma filter, packed tap pairs per phase
+    uint32_t tapL[16 * 3 * 5];     // luma filter (index 0): tap rows A, B, A' per phase (tap_row)
+    uint32_t tapC[32 * 3 * 3];     // chroma filter: the same per phase
     uint32_t csum[64];
     int16_t off[2 * XG_MAX_CAND];
     unsigned long long key;
     int last;
 };
 
+// Packed tap pairs of one phase of a TAPS-tap filter c, as three rows of
+// TP = TAPS / 2 + 1 dwords:
+//   A  = (c0, c1) (c2, c3) ... (c[T-2], c[T-1]) (0, 0)
+//   B  = (0, c0) (c1, c2) ... (c[T-1], 0)
+//   A' = (0, 0) (c0, c1) ... (c[T-2], c[T-1])
+// A window row held as dwords D[0..TP) from the even column at or left of
+// the first tap gives a lane's two outputs (columns x, x + 1) with TP dot2
+// each and no realignment: first tap even, s0 = D.A and s1 = D.B; first tap
+// odd (D[0] holds the sample left of it), s0 = D.B and s1 = D.A'.  The
+// vertical pass uses the first TAPS / 2 pairs of A.
+template <int TAPS>
+__device__ __forceinline__ uint32_t tap_row(const int8_t *c, int row, int j)
+{
+    constexpr int H = TAPS / 2;
+    if (row == 0) return j < H ? pack16(c[2 * j], c[2 * j + 1]) : 0u;
+    if (row == 1) return pack16(j == 0 ? 0 : c[2 * j - 1], j == H ? 0 : c[2 * j]);
+    return j == 0 ? 0u : pack16(c[2 * j - 2], c[2 * j - 1]);
+}
+
 __device__ __forceinline__ void lds_init(RefineLds &L, const RefineArgs &a)
 {
     const int lane = threadIdx.x;
-    L.tapL[lane] = pack16(c_luma[0][lane >> 2][2 * (lane & 3)], c_luma[0][lane >> 2][2 * (lane & 3) + 1]);
-    L.tapC[lane] = pack16(c_chroma[lane >> 1][2 * (lane & 1)], c_chroma[lane >> 1][2 * (lane & 1) + 1]);
+    for (int i = lane; i < 16 * 15; i += 64) L.tapL[i] = tap_row<8>(c_luma[0][i / 15], (i % 15) / 5, i % 5);
+    for (int i = lane; i < 32 * 9; i += 64) L.tapC[i] = tap_row<4>(c_chroma[i / 9], (i % 9) / 3, i % 3);
     for (int i = lane; i < 2 * XG_MAX_CAND; i += 64) L.off[i] = a.off[i];
     __syncthreads();
 }
 
+// The horizontal taps of a lane, chosen by the parity of its first tap
+// (tap_row): X gives column x, Y column x + 1.
+template <int TAPS>
+struct HTaps {
+    static constexpr int TP = TAPS / 2 + 1;
+    uint32_t X[TP], Y[TP];
+    __device__ __forceinline__ void load(const uint32_t *rows, bool odd)
+    {
+        const uint32_t *x = rows + (odd ? TP : 0), *y = rows + (odd ? 2 * TP : TP);
+#pragma unroll
+        for (int j = 0; j < TP; j++) {
+            X[j] = x[j];
+            Y[j] = y[j];
+        }
+    }
+};
+
 // First pass of one window row for the lane's two columns, split into the
-// LDS fetch (the dwords covering TAPS + 1 samples from the dword holding the
-// first tap) and the arithmetic, so a row can be fetched while the previous
-// one is filtered.  sh = 16 when the first tap is the high half of its dword.
+// LDS fetch (TAPS / 2 + 1 dwords from the one holding the first tap) and the
+// arithmetic, so a row can be fetched while the previous one is filtered.
 template <int TAPS>
 struct HRow {
-    static constexpr int ND = TAPS / 2 + 2;
+    static constexpr int ND = TAPS / 2 + 1;
     uint32_t D[ND];
     __device__ __forceinline__ void fetch(const uint32_t *q)
     {
 #pragma unroll
         for (int j = 0; j < ND; j++) D[j] = q[j];
     }
-    __device__ __forceinline__ void filter(uint32_t sh, const uint32_t *hp, int &t0, int &t1) const
+    __device__ __forceinline__ void filter(const HTaps<TAPS> &T, int &t0, int &t1) const
     {
-        uint32_t E[ND - 1];
-#pragma unroll
-        for (int j = 0; j < ND - 1; j++) E[j] = align16(D[j + 1], D[j], sh);  // (p[2j], p[2j+1])
         int s0 = kHOffset, s1 = kHOffset;
 #pragma unroll
-        for (int j = 0; j < TAPS / 2; j++) {
-            s0 = dot2(E[j], hp[j], s0);
-            s1 = dot2(align16(E[j + 1], E[j], 16), hp[j], s1);              // (p[2j+1], p[2j+2])
+        for (int j = 0; j < ND; j++) {
+            s0 = dot2(D[j], T.X[j], s0);
+            s1 = dot2(D[j], T.Y[j], s1);
         }
         t0 = s0 >> kHShift;
         t1 = s1 >> kHShift;
@@ -237,31 +269,35 @@ struct HRow {
 };
 
 template <int TAPS>
-__device__ __forceinline__ void hrow2(const uint32_t *win, int idx, const uint32_t *hp, int &t0, int &t1)
+__device__ __forceinline__ void hrow2(const uint32_t *win, int idx, const HTaps<TAPS> &T, int &t0, int &t1)
 {
     HRow<TAPS> h;
     h.fetch(win + (idx >> 1));
-    h.filter((uint32_t)(idx & 1) << 4, hp, t0, t1);
+    h.filter(T, t0, t1);
 }
 
 // Interpolation + SATD of one unit (a candidate's plane) of W x H samples for
 // the lane's columns 2m, 2m+1:
-//   win, ls: the plane's LDS window and its row stride in samples (even)
+//   win, LS: the plane's LDS window and its row stride in samples (even)
 //   wx, wy:  window position of the unit's first tap
-//   hp, vp:  packed tap pairs of the horizontal and vertical phases (LDS)
+//   hp, vp:  tap rows of the horizontal and vertical phases (tap_row, LDS)
 //   cur, cs: the current block (global, dword aligned) and its stride
 // Window rows are fetched one row ahead of the row being filtered, current
 // rows one tile ahead.
-template <int TAPS, int W, int TS>
-__device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int ls, int wx, int wy, const uint32_t *hp,
+template <int TAPS, int W, int TS, int LS>
+__device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int wx, int wy, const uint32_t *hp,
                                                 const uint32_t *vp, const uint16_t *cur, intptr_t cs, int H, int m,
                                                 int lane)
 {
     constexpr int NP = TS + TAPS - 2;               // packed row pairs per column in a tile
-    const int lw = ls >> 1;                         // row stride in dwords
+    constexpr int ls = LS, lw = LS >> 1;            // window row stride in samples and dwords (immediate offsets)
     const int base = wy * ls + wx + 2 * m;
-    const uint32_t sh = (uint32_t)(base & 1) << 4;
     const uint32_t *q = win + (base >> 1);          // dword of the first tap of the current row
+    HTaps<TAPS> ht;
+    ht.load(hp, base & 1);
+    uint32_t vt[TAPS / 2];
+#pragma unroll
+    for (int j = 0; j < TAPS / 2; j++) vt[j] = vp[j];
     uint32_t P0[NP], P1[NP];
     int tl0, tl1;
     HRow<TAPS> cur_row, next_row;
@@ -269,13 +305,13 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int ls, int
     {
         int a0, a1;
         next_row.fetch(q + lw);
-        cur_row.filter(sh, hp, a0, a1);
+        cur_row.filter(ht, a0, a1);
         cur_row = next_row;
 #pragma unroll
         for (int r = 1; r < TAPS - 1; r++) {
             int b0, b1;
             next_row.fetch(q + (r + 1) * lw);
-            cur_row.filter(sh, hp, b0, b1);
+            cur_row.filter(ht, b0, b1);
             cur_row = next_row;
             P0[r - 1] = pack16(a0, b0);
             P1[r - 1] = pack16(a1, b1);
@@ -301,7 +337,7 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int ls, int
         for (int k = 0; k < TS; k++) {
             int b0, b1;
             next_row.fetch(q + (k + 1) * lw);       // one row past the window at the very end: unused
-            cur_row.filter(sh, hp, b0, b1);
+            cur_row.filter(ht, b0, b1);
             cur_row = next_row;
             P0[TAPS - 2 + k] = pack16(tl0, b0);
             P1[TAPS - 2 + k] = pack16(tl1, b1);
@@ -315,8 +351,8 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int ls, int
             int s0 = kVOffset, s1 = kVOffset;
 #pragma unroll
             for (int j = 0; j < TAPS / 2; j++) {
-                s0 = dot2(P0[k + 2 * j], vp[j], s0);
-                s1 = dot2(P1[k + 2 * j], vp[j], s1);
+                s0 = dot2(P0[k + 2 * j], vt[j], s0);
+                s1 = dot2(P1[k + 2 * j], vt[j], s1);
             }
             d[k] = as_v2s(cw[k]) - as_v2s(pack16(clip_pix(s0 >> kVShift), clip_pix(s1 >> kVShift)));
         }
@@ -348,8 +384,8 @@ __device__ __forceinline__ void luma_task(const xg_block &B, const MvWin &mw, co
         const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
         const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
         const int rx = ((B.x + ox) << 4) + mvx, ry = ((B.y + oy) << 4) + mvy;
-        int acc = interp_satd_pass<8, W, TS>(L.luma, kLumaLW, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0,
-                                             L.tapL + 4 * (rx & 15), L.tapL + 4 * (ry & 15), cur, cs, H, m, lane);
+        int acc = interp_satd_pass<8, W, TS, kLumaLW>(L.luma, (rx >> 4) - 3 - wx0, (ry >> 4) - 3 - wy0,
+                                             L.tapL + 15 * (rx & 15), L.tapL + 15 * (ry & 15), cur, cs, H, m, lane);
         acc = group_sum<LN>(acc);
         if (m == 0 && ci < nc) csum[ci] += (uint32_t)acc;
     }
@@ -371,8 +407,8 @@ __device__ __forceinline__ void chroma_task(const xg_block &B, const MvWin &mw, 
         const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
         const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
         const int rx = ((B.x + ox) << 4) + mvx, ry = ((B.y + oy) << 4) + mvy;
-        int acc = interp_satd_pass<4, WC, TC>(pl ? L.cr : L.cb, kChromaLW, (rx >> 5) - 1 - cx0, (ry >> 5) - 1 - cy0,
-                                              L.tapC + 2 * (rx & 31), L.tapC + 2 * (ry & 31), pl ? curV : curU, cs,
+        int acc = interp_satd_pass<4, WC, TC, kChromaLW>(pl ? L.cr : L.cb, (rx >> 5) - 1 - cx0, (ry >> 5) - 1 - cy0,
+                                              L.tapC + 9 * (rx & 31), L.tapC + 9 * (ry & 31), pl ? curV : curU, cs,
                                               HC, m, lane);
         acc = group_sum<2 * LN>(acc);
         if (m == 0 && pl == 0 && ci < nc) csum[ci] += (uint32_t)acc;
@@ -459,11 +495,13 @@ __device__ __forceinline__ void mc_pass(const uint32_t *win, int ls, int wx, int
     for (int j = 0; j < TAPS; j++) vc[j] = cv[j];
     int t0[TAPS], t1[TAPS];
     const int base = (wy + y0) * ls + wx + 2 * m;
+    HTaps<TAPS> ht;
+    ht.load(hp, base & 1);
 #pragma unroll
-    for (int k = 0; k < TAPS - 1; k++) hrow2<TAPS>(win, base + k * ls, hp, t0[k], t1[k]);
+    for (int k = 0; k < TAPS - 1; k++) hrow2<TAPS>(win, base + k * ls, ht, t0[k], t1[k]);
     uint16_t *o = dst + (intptr_t)y0 * ds + 2 * m;
     for (int y = 0; y < rows; y++) {
-        hrow2<TAPS>(win, base + (y + TAPS - 1) * ls, hp, t0[TAPS - 1], t1[TAPS - 1]);
+        hrow2<TAPS>(win, base + (y + TAPS - 1) * ls, ht, t0[TAPS - 1], t1[TAPS - 1]);
         int s0 = kVOffset, s1 = kVOffset;
 #pragma unroll
         for (int j = 0; j < TAPS; j++) {
@@ -524,10 +562,10 @@ __device__ __forceinline__ void mc_sub(const RefineArgs &a, const xg_block &B, i
     const xg_picture &PT = a.predTab[B.pic];
     const xg_plane &P0 = PT.plane[0], &P1 = PT.plane[1], &P2 = PT.plane[2];
     const int rx = (sx << 4) + mvx, ry = (sy << 4) + mvy;   // the sub-block's reference position
-    mc_pass<8>(L.luma, kLumaLW, (rx >> 4) - 3 - w.wx0, (ry >> 4) - 3 - w.wy0, L.tapL + 4 * (rx & 15), ry & 15, sw, sh,
+    mc_pass<8>(L.luma, kLumaLW, (rx >> 4) - 3 - w.wx0, (ry >> 4) - 3 - w.wy0, L.tapL + 15 * (rx & 15), ry & 15, sw, sh,
                P0.data + (intptr_t)sy * P0.stride + sx, P0.stride, 0, 64);
     const int wc = sw >> 1, hc = sh >> 1, ox = (rx >> 5) - 1 - w.cx0, oy = (ry >> 5) - 1 - w.cy0;
-    const uint32_t *hc4 = L.tapC + 2 * (rx & 31);
+    const uint32_t *hc4 = L.tapC + 9 * (rx & 31);
     mc_pass<4>(L.cb, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
                P1.data + (intptr_t)(sy >> 1) * P1.stride + (sx >> 1), P1.stride, 0, 64);
     mc_pass<4>(L.cr, kChromaLW, ox, oy, hc4, ry & 31, wc, hc,
