@@ -171,6 +171,38 @@ __device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c)
 {
     return __builtin_amdgcn_sdot2(as_v2s(a), as_v2s(b), c, false);
 }
+
+// Offset-free form of the two passes, for kernels whose first-pass values are
+// only intermediates; the results are the same.  u = (sum + 32) >> 2 is the
+// standard first-pass value plus 8192 + 8 exactly ((sum - (8192 << 2)) >> 2 =
+// (sum >> 2) - 8192, and 32 is a multiple of 4).  Every vertical filter's
+// taps sum to 64, so the second pass over u already holds
+// 64 * (8192 + 8) = (8192 << 6) + 512 in its sum and is sum >> 10 alone.  For
+// 10-bit input u lies in [-6138, 22514] (luma; chroma inside that), so it
+// still packs as int16.  Both accumulators start from inline constants (32, 0).
+constexpr int kHRound = 32;
+// With the vertical taps scaled by 64 (|tap| <= 4096) the second-pass sum is
+// 64 times as large (|sum| < 2^28): clamped to [0, (1023 << 16) | 0xFFFF] its
+// upper half is clip(sum >> 10, 0, 1023), and two such halves pack with one
+// byte permute.
+constexpr int kClipHi16 = (kPixMax << 16) | 0xFFFF;
+__device__ __forceinline__ int clip_hi16(int s) { return min(max(s, 0), kClipHi16); }
+__device__ __forceinline__ uint32_t pack_hi16(int lo, int hi)
+{
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x07060302u);
+}
+// First product of a dot2 chain with an inline-constant start.  The clamp
+// bit (saturation of the i32 sum, which these sums never reach) keeps the
+// instruction in its three-operand form, so the constant needs no register.
+__device__ __forceinline__ int dot2_first(uint32_t a, uint32_t b, int c)
+{
+    return __builtin_amdgcn_sdot2(as_v2s(a), as_v2s(b), c, true);
+}
+// Both int16 taps of a packed pair times 64.
+__device__ __forceinline__ uint32_t taps_x64(uint32_t t)
+{
+    return __builtin_bit_cast(uint32_t, as_v2s(t) << (v2s){6, 6});
+}
 __device__ __forceinline__ uint32_t align16(uint32_t hi, uint32_t lo, uint32_t sh)
 {
     return __builtin_amdgcn_alignbit(hi, lo, sh);

@@ -96,15 +96,35 @@ __device__ __forceinline__ MvWin mv_window(const xg_block &B, int fw, int fh, in
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Task, block, list and picture descriptors are the same for every lane, but
+// the kernel also stores to global memory, so they are loaded into vector
+// registers.  Read back through readfirstlane they live in scalar registers,
+// and so does every address and loop bound derived from them.
+template <class T>
+__device__ __forceinline__ T uniform_load(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "descriptor of whole dwords");
+    constexpr int N = sizeof(T) / 4;
+    uint32_t w[N];
+    __builtin_memcpy(w, p, sizeof(T));
+#pragma unroll
+    for (int k = 0; k < N; k++) w[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)w[k]);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // ---------------------------------------------------------------------------
 // Interpolation + SATD of one plane for a pass of candidates, two adjacent
 // columns per lane.
 //
 // Each lane owns columns 2m and 2m+1 of one unit (a candidate, or for chroma
 // a candidate's Cb or Cr plane).  A window row is fetched as dword-aligned
-// LDS words and realigned with v_alignbit, so every 16-bit sample pair
-// (p[i], p[i+1]) the filters need is one register; the first pass is then
-// TAPS/2 v_dot2_i32_i16 per column.  First-pass values are kept as packed
+// LDS words (TAPS/2 + 1 of them) and filtered with taps packed to match the
+// parity of the lane's first sample (tap_row), so each column of the first
+// pass is TAPS/2 + 1 v_dot2_i32_i16 with no realignment.  First-pass values are kept as packed
 // consecutive-row pairs (t[r], t[r+1]), so the second pass is TAPS/2 more
 // dot2.  Differences, the vertical Walsh-Hadamard stages and the first
 // horizontal stages run on packed int16 (the largest magnitude before the
@@ -257,9 +277,10 @@ struct HRow {
     }
     __device__ __forceinline__ void filter(const HTaps<TAPS> &T, int &t0, int &t1) const
     {
-        int s0 = kHOffset, s1 = kHOffset;
+        int s0 = dot2_first(D[0], T.X[0], kHRound);  // offset-free first pass (kHRound)
+        int s1 = dot2_first(D[0], T.Y[0], kHRound);
 #pragma unroll
-        for (int j = 0; j < ND; j++) {
+        for (int j = 1; j < ND; j++) {
             s0 = dot2(D[j], T.X[j], s0);
             s1 = dot2(D[j], T.Y[j], s1);
         }
@@ -295,9 +316,9 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int wx, int
     const uint32_t *q = win + (base >> 1);          // dword of the first tap of the current row
     HTaps<TAPS> ht;
     ht.load(hp, base & 1);
-    uint32_t vt[TAPS / 2];
+    uint32_t vt[TAPS / 2];                          // vertical taps x 64 (clip_hi16)
 #pragma unroll
-    for (int j = 0; j < TAPS / 2; j++) vt[j] = vp[j];
+    for (int j = 0; j < TAPS / 2; j++) vt[j] = taps_x64(vp[j]);
     uint32_t P0[NP], P1[NP];
     int tl0, tl1;
     HRow<TAPS> cur_row, next_row;
@@ -348,13 +369,13 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int wx, int
         v2s d[TS];
 #pragma unroll
         for (int k = 0; k < TS; k++) {
-            int s0 = kVOffset, s1 = kVOffset;
+            int s0 = dot2_first(P0[k], vt[0], 0), s1 = dot2_first(P1[k], vt[0], 0);
 #pragma unroll
-            for (int j = 0; j < TAPS / 2; j++) {
+            for (int j = 1; j < TAPS / 2; j++) {
                 s0 = dot2(P0[k + 2 * j], vt[j], s0);
                 s1 = dot2(P1[k + 2 * j], vt[j], s1);
             }
-            d[k] = as_v2s(cw[k]) - as_v2s(pack16(clip_pix(s0 >> kVShift), clip_pix(s1 >> kVShift)));
+            d[k] = as_v2s(cw[k]) - as_v2s(pack_hi16(clip_hi16(s0), clip_hi16(s1)));
         }
         const int ts = tile_satd2<TS>(d, lane);
         if ((m & (TS / 2 - 1)) == 0) acc += ts;
@@ -502,7 +523,7 @@ __device__ __forceinline__ void mc_pass(const uint32_t *win, int ls, int wx, int
     uint16_t *o = dst + (intptr_t)y0 * ds + 2 * m;
     for (int y = 0; y < rows; y++) {
         hrow2<TAPS>(win, base + (y + TAPS - 1) * ls, ht, t0[TAPS - 1], t1[TAPS - 1]);
-        int s0 = kVOffset, s1 = kVOffset;
+        int s0 = 0, s1 = 0;                         // offset-free second pass (kHRound)
 #pragma unroll
         for (int j = 0; j < TAPS; j++) {
             s0 += vc[j] * t0[j];
@@ -559,7 +580,7 @@ __device__ __forceinline__ SubWin stage_sub(const RefineArgs &a, const xg_block 
 __device__ __forceinline__ void mc_sub(const RefineArgs &a, const xg_block &B, int mvx, int mvy, int sx, int sy,
                                        int sw, int sh, const SubWin &w, const RefineLds &L)
 {
-    const xg_picture &PT = a.predTab[B.pic];
+    const xg_picture PT = uniform_load(&a.predTab[B.pic]);
     const xg_plane &P0 = PT.plane[0], &P1 = PT.plane[1], &P2 = PT.plane[2];
     const int rx = (sx << 4) + mvx, ry = (sy << 4) + mvy;   // the sub-block's reference position
     mc_pass<8>(L.luma, kLumaLW, (rx >> 4) - 3 - w.wx0, (ry >> 4) - 3 - w.wy0, L.tapL + 15 * (rx & 15), ry & 15, sw, sh,
@@ -579,7 +600,7 @@ __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_bl
                                          unsigned long long key, RefineLds &L, const SubWin &last)
 {
     const int lane = threadIdx.x;
-    const int c = (int)(uint32_t)key;
+    const int c = uniform((int)(uint32_t)key);
     const int mvx = clampi(B.mvx + L.off[c], mw.mnx, mw.mxx);
     const int mvy = clampi(B.mvy + L.off[XG_MAX_CAND + c], mw.mny, mw.mxy);
     if (lane == 0) {
@@ -600,7 +621,7 @@ __device__ __forceinline__ void mc_block(const RefineArgs &a, int b, const xg_bl
         mc_sub(a, B, mvx, mvy, B.x, B.y, sw, sh, last, L);
         return;
     }
-    const xg_picture &R = a.pics[B.ref];
+    const xg_picture R = uniform_load(&a.pics[B.ref]);
     for (int sy = B.y; sy < B.y + B.h; sy += sh)
         for (int sx = B.x; sx < B.x + B.w; sx += sw) {
             __syncthreads();
@@ -614,8 +635,8 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int2 tk, const x
 {
     const int lane = threadIdx.x;
     const int b = tk.x, c0 = tk.y & 0xFFFF, nc = tk.y >> 16;
-    const xg_picture &P = a.pics[B.pic];
-    const xg_picture &R = a.pics[B.ref];
+    const xg_picture P = uniform_load(&a.pics[B.pic]);
+    const xg_picture R = uniform_load(&a.pics[B.ref]);
     const MvWin mw = mv_window(B, P.plane[0].width, P.plane[0].height, a.ctuSize);
 
     const intptr_t cs = P.plane[0].stride, ccs = P.plane[1].stride;
@@ -699,7 +720,7 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int2 tk, const x
         L.key = key;
     }
     __syncthreads();
-    if (L.last) mc_block(a, b, B, mw, L.key, L, w);
+    if (uniform(L.last)) mc_block(a, b, B, mw, L.key, L, w);
     __syncthreads();
 }
 
@@ -710,10 +731,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 {
     __shared__ RefineLds L;
     lds_init(L, a);
-    const int e = a.wgBegin[blockIdx.x + 1];
-    for (int t = a.wgBegin[blockIdx.x]; t < e; t++) {
-        const int2 tk = a.tasks[t];
-        refine_one(a, tk, a.blocks[tk.x], L);
+    const int e = uniform(a.wgBegin[blockIdx.x + 1]);
+    for (int t = uniform(a.wgBegin[blockIdx.x]); t < e; t++) {
+        const int2 tk = uniform_load(&a.tasks[t]);
+        refine_one(a, tk, uniform_load(&a.blocks[tk.x]), L);
     }
 }
 
@@ -727,7 +748,7 @@ k_refine_multi(RefineArgs base, const RefineList *__restrict__ lists, int nLists
     __shared__ RefineLds L;
     lds_init(L, base);
     for (int l = 0; l < nLists; l++) {
-        const RefineList R = lists[l];
+        const RefineList R = uniform_load(&lists[l]);
         RefineArgs a = base;
         a.blocks = R.blocks;
         a.candCost = R.candCost;
@@ -738,10 +759,10 @@ k_refine_multi(RefineArgs base, const RefineList *__restrict__ lists, int nLists
         a.mvStride = R.mvStride;
         int v = (int)blockIdx.x + R.rot;
         if (v >= grid) v -= grid;
-        const int e = R.wgBegin[v + 1];
-        for (int t = R.wgBegin[v]; t < e; t++) {
-            const int2 tk = R.tasks[t];
-            refine_one(a, tk, R.blocks[tk.x], L);
+        const int e = uniform(R.wgBegin[v + 1]);
+        for (int t = uniform(R.wgBegin[v]); t < e; t++) {
+            const int2 tk = uniform_load(&R.tasks[t]);
+            refine_one(a, tk, uniform_load(&R.blocks[tk.x]), L);
         }
     }
 }
@@ -753,8 +774,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     lds_init(L, a);
     const int t = a.taskBegin + (int)blockIdx.x;
     if (t < a.taskEnd) {
-        const int2 tk = a.tasks[t];
-        refine_one(a, tk, a.blocks[tk.x], L);
+        const int2 tk = uniform_load(&a.tasks[t]);
+        refine_one(a, tk, uniform_load(&a.blocks[tk.x]), L);
     }
 }
 
