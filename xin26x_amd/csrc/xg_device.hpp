@@ -198,10 +198,14 @@ __device__ __forceinline__ int dot2_first(uint32_t a, uint32_t b, int c)
 {
     return __builtin_amdgcn_sdot2(as_v2s(a), as_v2s(b), c, true);
 }
-// Both int16 taps of a packed pair times 64.
-__device__ __forceinline__ uint32_t taps_x64(uint32_t t)
+// First pass with its taps scaled by 64 as well (|tap| <= 4096, |sum| < 2^23):
+// u is bits 8..23 of the scaled sum (64 * (sum + 32)) >> 8, so the u of two
+// rows pack into one dword with one byte permute (bytes 1 and 2 of each) and
+// no shifts.
+constexpr int kHRound64 = kHRound << 6;
+__device__ __forceinline__ uint32_t pack_mid16(int lo, int hi)
 {
-    return __builtin_bit_cast(uint32_t, as_v2s(t) << (v2s){6, 6});
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x06050201u);
 }
 __device__ __forceinline__ uint32_t align16(uint32_t hi, uint32_t lo, uint32_t sh)
 {

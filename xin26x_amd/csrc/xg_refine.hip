@@ -218,8 +218,8 @@ struct RefineLds {
     int last;
 };
 
-// Packed tap pairs of one phase of a TAPS-tap filter c, as three rows of
-// TP = TAPS / 2 + 1 dwords:
+// Packed tap pairs of one phase of a TAPS-tap filter c, each tap times 64
+// (pack_mid16, clip_hi16), as three rows of TP = TAPS / 2 + 1 dwords:
 //   A  = (c0, c1) (c2, c3) ... (c[T-2], c[T-1]) (0, 0)
 //   B  = (0, c0) (c1, c2) ... (c[T-1], 0)
 //   A' = (0, 0) (c0, c1) ... (c[T-2], c[T-1])
@@ -232,9 +232,9 @@ template <int TAPS>
 __device__ __forceinline__ uint32_t tap_row(const int8_t *c, int row, int j)
 {
     constexpr int H = TAPS / 2;
-    if (row == 0) return j < H ? pack16(c[2 * j], c[2 * j + 1]) : 0u;
-    if (row == 1) return pack16(j == 0 ? 0 : c[2 * j - 1], j == H ? 0 : c[2 * j]);
-    return j == 0 ? 0u : pack16(c[2 * j - 2], c[2 * j - 1]);
+    if (row == 0) return j < H ? pack16(64 * c[2 * j], 64 * c[2 * j + 1]) : 0u;
+    if (row == 1) return pack16(j == 0 ? 0 : 64 * c[2 * j - 1], j == H ? 0 : 64 * c[2 * j]);
+    return j == 0 ? 0u : pack16(64 * c[2 * j - 2], 64 * c[2 * j - 1]);
 }
 
 __device__ __forceinline__ void lds_init(RefineLds &L, const RefineArgs &a)
@@ -275,17 +275,23 @@ struct HRow {
 #pragma unroll
         for (int j = 0; j < ND; j++) D[j] = q[j];
     }
-    __device__ __forceinline__ void filter(const HTaps<TAPS> &T, int &t0, int &t1) const
+    // Scaled sums 64 * (sum + 32) of the two columns (offset-free first pass,
+    // taps x 64): their bits 8..23 are the first-pass values (pack_mid16).
+    __device__ __forceinline__ void sums(const HTaps<TAPS> &T, int &s0, int &s1) const
     {
-        int s0 = dot2_first(D[0], T.X[0], kHRound);  // offset-free first pass (kHRound)
-        int s1 = dot2_first(D[0], T.Y[0], kHRound);
+        s0 = dot2_first(D[0], T.X[0], kHRound64);
+        s1 = dot2_first(D[0], T.Y[0], kHRound64);
 #pragma unroll
         for (int j = 1; j < ND; j++) {
             s0 = dot2(D[j], T.X[j], s0);
             s1 = dot2(D[j], T.Y[j], s1);
         }
-        t0 = s0 >> kHShift;
-        t1 = s1 >> kHShift;
+    }
+    __device__ __forceinline__ void filter(const HTaps<TAPS> &T, int &t0, int &t1) const
+    {
+        sums(T, t0, t1);
+        t0 >>= 8;
+        t1 >>= 8;
     }
 };
 
@@ -318,24 +324,24 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int wx, int
     ht.load(hp, base & 1);
     uint32_t vt[TAPS / 2];                          // vertical taps x 64 (clip_hi16)
 #pragma unroll
-    for (int j = 0; j < TAPS / 2; j++) vt[j] = taps_x64(vp[j]);
+    for (int j = 0; j < TAPS / 2; j++) vt[j] = vp[j];
     uint32_t P0[NP], P1[NP];
     int tl0, tl1;
     HRow<TAPS> cur_row, next_row;
     cur_row.fetch(q);
     {
-        int a0, a1;
+        int a0, a1;                                 // scaled first-pass sums (HRow::sums)
         next_row.fetch(q + lw);
-        cur_row.filter(ht, a0, a1);
+        cur_row.sums(ht, a0, a1);
         cur_row = next_row;
 #pragma unroll
         for (int r = 1; r < TAPS - 1; r++) {
             int b0, b1;
             next_row.fetch(q + (r + 1) * lw);
-            cur_row.filter(ht, b0, b1);
+            cur_row.sums(ht, b0, b1);
             cur_row = next_row;
-            P0[r - 1] = pack16(a0, b0);
-            P1[r - 1] = pack16(a1, b1);
+            P0[r - 1] = pack_mid16(a0, b0);
+            P1[r - 1] = pack_mid16(a1, b1);
             a0 = b0;
             a1 = b1;
         }
@@ -358,10 +364,10 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int wx, int
         for (int k = 0; k < TS; k++) {
             int b0, b1;
             next_row.fetch(q + (k + 1) * lw);       // one row past the window at the very end: unused
-            cur_row.filter(ht, b0, b1);
+            cur_row.sums(ht, b0, b1);
             cur_row = next_row;
-            P0[TAPS - 2 + k] = pack16(tl0, b0);
-            P1[TAPS - 2 + k] = pack16(tl1, b1);
+            P0[TAPS - 2 + k] = pack_mid16(tl0, b0);
+            P1[TAPS - 2 + k] = pack_mid16(tl1, b1);
             tl0 = b0;
             tl1 = b1;
         }
