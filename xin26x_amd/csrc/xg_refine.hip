@@ -733,7 +733,7 @@ __device__ __forceinline__ void refine_one(const RefineArgs &a, int2 tk, const x
 // Batched form: a persistent grid; workgroup g runs the tasks the host
 // assigned to it (balanced by estimated cost, heaviest first), so the launch
 // takes no atomics to schedule and leaves no state behind.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_refine_persistent(RefineArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_refine_persistent(RefineArgs a)
 {
     __shared__ RefineLds L;
     lds_init(L, a);
@@ -748,7 +748,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 // streams.  Each list carries its own blocks, workgroup-major task assignment
 // and outputs; workgroup g runs, list after list, the tasks of virtual
 // workgroup (g + rot) mod grid, so small lists are spread over the grid.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_refine_multi(RefineArgs base, const RefineList *__restrict__ lists, int nLists, int grid)
 {
     __shared__ RefineLds L;
@@ -774,7 +774,7 @@ k_refine_multi(RefineArgs base, const RefineList *__restrict__ lists, int nLists
 }
 
 // Plain form: one launch per block shape, one workgroup per task.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_refine_shape(RefineArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_refine_shape(RefineArgs a)
 {
     __shared__ RefineLds L;
     lds_init(L, a);
@@ -796,7 +796,7 @@ struct xg_session {
     hipStream_t stream = nullptr;
     int maxBlocks = 0;
     int numCUs = 256;
-    int persistFactor = 12;            // persistent workgroups per CU (registers allow 3 waves per SIMD)
+    int persistFactor = 16;            // persistent workgroups per CU (registers allow 4 waves per SIMD)
     int nregions = 1;                  // spatial partitions of the batched form (XCD-aligned)
 
     xg_picture pics[kPicSlots];
@@ -1631,7 +1631,7 @@ int32_t xg_session_autotune(xg_session *s, const xg_refine_cfg *cfg, int32_t rep
         // schedule with more bands wins unless it is over 1% slower
         int bestR = s->nregions, bestF = s->persistFactor;
         for (int r : {8, 4, 2, 1})
-            for (int f : {12, 10, 8}) {
+            for (int f : {16, 14, 12, 10, 8}) {
                 s->nregions = r;
                 s->persistFactor = f;
                 const float ms = time_frame(s, cfg, reps, e0, e1, &ok);

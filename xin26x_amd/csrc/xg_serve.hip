@@ -683,7 +683,7 @@ int32_t xg_server_create(xg_server **out, const xg_server_cfg *cfg)
         return XG_ERR_BUDGET;
     }
     for (int i = 0; i < kSessions; i++) {
-        if (xg_session_create(&srv->sess[i], 1) != XG_OK || xg_session_set_sched(srv->sess[i], 8, 12) != XG_OK ||
+        if (xg_session_create(&srv->sess[i], 1) != XG_OK || xg_session_set_sched(srv->sess[i], 8, 16) != XG_OK ||
             hipEventCreateWithFlags(&srv->batch[i].done, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&srv->batch[i].computed, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc((void **)&srv->batch[i].hDesc, sizeof(Descs), hipHostMallocDefault) != hipSuccess ||
