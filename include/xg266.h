@@ -536,12 +536,14 @@ int32_t xg_session_set_timing(xg_session *s, int32_t enable);
  * stream's first frame uses itself as reference; afterwards each processed
  * frame becomes the reference of the next.
  *
- * Predictors are resident too: the predictor of every block (the centre of its
- * integer search and the MVD origin of every cost) is the MV, in 1/16 pel, of
- * the 4x4 unit at the block's centre ((x + w/2) >> 2, (y + h/2) >> 2) in the
- * stream's MV field; every processed frame writes each block's final MV into
- * all its units.  Before a stream's first frame the field is zero.  The MV and
- * predictor fields of submitted blocks are ignored. */
+ * Predictors are resident too: every processed frame writes each block's
+ * final MV (1/16 pel) into all its 4x4 units of the stream's MV field (zero
+ * before the first frame).  The next frame's blocks take their AMVP candidates
+ * from that field as xg_amvp_from_field does, and the predictor (the centre of
+ * the integer search and the MVD origin of every cost) is the choice of
+ * xg_amvp_select among them with quarter-pel AMVR (imvIdx 0): the prediction
+ * SAD of each candidate against the block in the frame's reference.  The MV
+ * and predictor fields of submitted blocks are ignored. */
 typedef struct xg_server xg_server;
 
 typedef struct xg_server_cfg {
@@ -674,6 +676,54 @@ int32_t xg_lookahead_me(xg_session *s, const xg_plane *cur, const xg_plane *ref,
 #define XG_LOOKAHEAD_STATIC_MAX_BYTES (12u << 20)
 int32_t xg_lookahead_me_static(void *stream, const xg_plane *cur, const xg_plane *ref, int32_t unitSize,
                                int32_t range, uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad);
+
+/* AMVP predictor choice, batched: the static Xin266SelectMvp
+ * (h266_encoder/source/h266_analyze_inter_cu.c:60-131, called at :1003) for
+ * many blocks at once, with the rounding Xin266FillAmvpCand ends with
+ * (h266_get_neighbour_mv.c:2549-2558, Xin266ChangeMv32Prec :212-243) and the
+ * ME-precision predictor of Xin266ChangeMv2Me (:293-320).  Per block, each
+ * candidate (1/16 pel) is rounded to the AMVR precision of imvIdx (0 quarter,
+ * 1 integer, 2 4-pel, 3 half: amvrPrecision, h266_entropy_write.c:120-123)
+ * and back, its luma prediction formed as Xin266LumaInterPred does
+ * (h266_inter_prediction.c:302-354: MV clipped to the CU window of
+ * h266_enc_init.c:1462-1465, 8-tap filterIndex 0, the alternative half-pel
+ * filter when imvIdx is 3) and compared with the block by SAD
+ * (pfXinComputeDist); the first strictly lowest wins. */
+typedef struct xg_amvp_block {
+    int16_t x, y;          /* luma position                                  */
+    uint8_t w, h;          /* 4..64, multiples of 4                          */
+    uint8_t pic, ref;      /* session picture slots: current / reference     */
+    int32_t ncand;         /* 1 or 2 candidates                              */
+    int32_t cand[2][2];    /* candidate MVs, 1/16 pel, before AMVR rounding  */
+} xg_amvp_block;
+
+typedef struct xg_amvp_result {
+    int32_t  mvpIdx;       /* chosen candidate; -1: the block was rejected   */
+    uint32_t sad;          /* its prediction SAD (mvpSad)                    */
+    int32_t  mv[2];        /* the chosen candidate after rounding, 1/16 pel  */
+    int16_t  meMv[2];      /* the same in ME precision (quarter pel)         */
+} xg_amvp_result;
+
+/* Blocks and results are device arrays; the work is enqueued on the session's
+ * stream (xg_session_sync waits for it).  A block outside its picture, with a
+ * side outside 4..64 or not a multiple of 4, an unset slot or a reference
+ * padded by less than ctuSize + 12 gets mvpIdx -1 and sad 0xFFFFFFFF; the
+ * others are unaffected. */
+int32_t xg_amvp_select(xg_session *s, const xg_amvp_block *blocks, int32_t n, int32_t imvIdx, int32_t ctuSize,
+                       xg_amvp_result *out);
+
+/* Candidate lists of xg_amvp_select from an MV field of 4x4 units (device,
+ * int32 (x, y) pairs in 1/16 pel, fieldStride units per row, covering a
+ * width x height picture): the rule of Xin266FillAmvpCand
+ * (h266_get_neighbour_mv.c:2451-2560) with every unit inside the picture an
+ * available inter neighbour of the same reference and no temporal or history
+ * candidates: the below-left (A0) else left (A1) unit, the above-right (B0)
+ * else above (B1) else above-left (B2) unit, a duplicate pair reduced to one,
+ * a zero MV added when the list is empty or its one entry is non-zero, two
+ * candidates only when they differ.  blocks (device) give positions, sizes
+ * and slots; out (device) receives the lists. */
+int32_t xg_amvp_from_field(xg_session *s, const int32_t *field, int32_t fieldStride, int32_t width, int32_t height,
+                           const xg_block *blocks, int32_t n, xg_amvp_block *out);
 
 /* Full motion-search pipeline of one frame with two reference pictures, on a
  * raster grid of 16x16 blocks (frame sides multiples of 16): per block and

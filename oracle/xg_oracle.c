@@ -1256,6 +1256,92 @@ void or_SearchBlocks(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intp
     }
 }
 
+/* ---- AMVP predictor selection ---------------------------------------- */
+/* h266_get_neighbour_mv.c:212-243: left shift when the precision rises,
+ * rounding right shift (half toward zero) when it drops. */
+int32_t or_ChangeMvPrec(int32_t v, int32_t inPrec, int32_t outPrec)
+{
+    return or_MvRoundShift(v, inPrec - outPrec);
+}
+
+/* amvrPrecision[] of h266_entropy_write.c:120-123, indexed by imvIdx. */
+static const int32_t kAmvrPrec[4] = {4, 2, 0, 3};
+
+static int field_mv(const int32_t *field, intptr_t stride, int32_t fw, int32_t fh, int32_t px, int32_t py,
+                    int32_t out[2])
+{
+    if (px < 0 || py < 0 || px >= fw || py >= fh) return 0;
+    const int32_t *u = field + ((intptr_t)(py >> 2) * stride + (px >> 2)) * 2;
+    out[0] = u[0];
+    out[1] = u[1];
+    return 1;
+}
+
+/* h266_get_neighbour_mv.c:2451-2560: the first available of the left group
+ * (below-left A0, then left A1), the first of the top group (above-right B0,
+ * above B1, above-left B2), a duplicate pair reduced to one, a zero MV added
+ * when the list is empty or its one entry is non-zero, two candidates kept
+ * only when they differ. */
+int32_t or_AmvpFromField(const int32_t *field, intptr_t stride, int32_t fw, int32_t fh, int32_t x, int32_t y,
+                         int32_t w, int32_t h, int32_t cand[4])
+{
+    int32_t list[3][2];
+    int n = 0;
+    const int32_t lx[2] = {x - 1, x - 1}, ly[2] = {y + h, y + h - 1};
+    for (int k = 0; k < 2; k++)
+        if (field_mv(field, stride, fw, fh, lx[k], ly[k], list[n])) { n++; break; }
+    const int32_t tx[3] = {x + w, x + w - 1, x - 1}, ty[3] = {y - 1, y - 1, y - 1};
+    for (int k = 0; k < 3; k++)
+        if (field_mv(field, stride, fw, fh, tx[k], ty[k], list[n])) { n++; break; }
+    if (n == 2 && list[0][0] == list[1][0] && list[0][1] == list[1][1]) n = 1;
+    if (n < 1 || (n == 1 && (list[0][0] != 0 || list[0][1] != 0))) {
+        list[n][0] = list[n][1] = 0;
+        n++;
+    }
+    const int two = n >= 2 && (list[0][0] != list[1][0] || list[0][1] != list[1][1]);
+    cand[0] = list[0][0];
+    cand[1] = list[0][1];
+    cand[2] = two ? list[1][0] : 0;
+    cand[3] = two ? list[1][1] : 0;
+    return two ? 2 : 1;
+}
+
+/* h266_analyze_inter_cu.c:60-131 with the rounding at the end of
+ * Xin266FillAmvpCand (h266_get_neighbour_mv.c:2549-2558) and the luma
+ * prediction of Xin266LumaInterPred (h266_inter_prediction.c:302-354). */
+int32_t or_SelectMvp(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t fw, int32_t fh,
+                     int32_t x, int32_t y, int32_t w, int32_t h, int32_t ncand, const int32_t *cand,
+                     int32_t imvIdx, int32_t ctuSize, uint32_t *sad, int32_t mv[2], int16_t meMv[2])
+{
+    int32_t mnx, mny, mxx, mxy;
+    cu_mv_window(x, y, w, h, fw, fh, ctuSize, &mnx, &mny, &mxx, &mxy);
+    const int32_t prec = kAmvrPrec[imvIdx & 3];
+    or_pixel *pred = (or_pixel *)malloc(sizeof(or_pixel) * (size_t)w * (size_t)h);
+    uint32_t best = UINT32_MAX;
+    int32_t idx = 0, rounded[2][2] = {{0, 0}, {0, 0}};
+    for (int32_t c = 0; c < ncand && c < 2; c++) {
+        for (int k = 0; k < 2; k++)
+            rounded[c][k] = or_ChangeMvPrec(or_ChangeMvPrec(cand[2 * c + k], 6, prec), prec, 6);
+        const int32_t mx = clip3(rounded[c][0], mnx * 16, mxx * 16), my = clip3(rounded[c][1], mny * 16, mxy * 16);
+        const int32_t ox = x * 16 + mx, oy = y * 16 + my;
+        const int32_t frac = ((oy & 15) << 4) + (ox & 15);
+        or_LumaInterpolation(ref + (intptr_t)(oy >> 4) * rs + (ox >> 4), rs, pred, w, frac, (uint32_t)w, (uint32_t)h,
+                             imvIdx == 3 ? 1u : 0u);
+        const uint32_t s = or_ComputeSad(cur + (intptr_t)y * cs + x, cs, pred, w, (uint32_t)w, (uint32_t)h);
+        if (s < best) {
+            best = s;
+            idx = c;
+        }
+    }
+    free(pred);
+    *sad = best;
+    mv[0] = rounded[idx][0];
+    mv[1] = rounded[idx][1];
+    meMv[0] = (int16_t)or_ChangeMvPrec(rounded[idx][0], 6, 4);
+    meMv[1] = (int16_t)or_ChangeMvPrec(rounded[idx][1], 6, 4);
+    return idx;
+}
+
 void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs,
                         int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range,
                         uint32_t sadLambda, int32_t ctuSize, int32_t blk0, int32_t blk1,

@@ -266,6 +266,31 @@ void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, i
                         int16_t *bestMvX, int16_t *bestMvY, uint32_t *bestCost,
                         uint32_t *bestSad);
 
+/* ---- AMVP predictor selection ------------------------------------------ */
+/* One MV component from precision inPrec to outPrec (log2 of the units per
+ * 4 pel: 0 4-pel, 2 integer, 3 half, 4 quarter, 6 sixteenth), as
+ * Xin266ChangeMv32Prec (h266_get_neighbour_mv.c:212-243) does it. */
+int32_t or_ChangeMvPrec(int32_t v, int32_t inPrec, int32_t outPrec);
+/* AMVP list of the block (x, y, w, h) from an MV field of 4x4 units
+ * (field[(uy * stride + ux) * 2 + {0,1}], 1/16 pel, every unit inside the
+ * fw x fh picture an available inter neighbour): the rule of
+ * Xin266FillAmvpCand (h266_get_neighbour_mv.c:2451-2560) without temporal
+ * and history candidates.  Returns the number of candidates (1 or 2), written
+ * to cand[0..1] / cand[2..3], before AMVR rounding. */
+int32_t or_AmvpFromField(const int32_t *field, intptr_t stride, int32_t fw, int32_t fh, int32_t x, int32_t y,
+                         int32_t w, int32_t h, int32_t cand[4]);
+/* Xin266SelectMvp (h266_analyze_inter_cu.c:60-131) for one block: each of the
+ * ncand candidates (1/16 pel) rounded to the AMVR precision of imvIdx
+ * (0 quarter, 1 integer, 2 4-pel, 3 half; h266_entropy_write.c:120-123) and
+ * back, its luma prediction (MV clipped to the CU window; the alternative
+ * half-pel filter when imvIdx is 3) compared with the block by SAD; the
+ * first strictly lowest wins.  Returns its index; sad, the rounded MV (1/16
+ * pel) and the same in ME precision (quarter pel, Xin266ChangeMv2Me,
+ * h266_get_neighbour_mv.c:293-320).  cur/ref point at sample (0,0). */
+int32_t or_SelectMvp(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t fw, int32_t fh,
+                     int32_t x, int32_t y, int32_t w, int32_t h, int32_t ncand, const int32_t *cand,
+                     int32_t imvIdx, int32_t ctuSize, uint32_t *sad, int32_t mv[2], int16_t meMv[2]);
+
 /* ---- quantisation (h266_encoder/source/h266_quant_inv_quant.c:27-143) ---- */
 void or_QuantInvQuant(int32_t *q, const int32_t *t, int32_t *r, intptr_t stride, uint32_t w, uint32_t h,
                       uint32_t cgw, uint32_t cgh, int32_t qMult, int32_t qAdd, int32_t qShift, int32_t iqMult,

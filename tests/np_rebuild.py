@@ -47,10 +47,18 @@ def eg_bits(v):
     return _EG[v]
 
 
-def interp(plane, x, y, w, h, frac, chroma):
+# The alternative half-pel luma filter (filterIndex 1 at phase 8; pinned by the
+# reference's golden vectors with filterIndex 1).
+LUMA_ALT_HPEL = (0, 3, 9, 20, 20, 9, 3, 0)
+
+
+def interp(plane, x, y, w, h, frac, chroma, fi=0):
     """w x h prediction whose integer position is plane[y, x] (the reference's
-    src pointer), 10-bit, filterIndex 0.  `plane` must hold the taps' reach."""
+    src pointer), 10-bit, filterIndex 0 (or 1: the alternative half-pel luma
+    filter at phase 8).  `plane` must hold the taps' reach."""
     taps, shift, tab = (4, 5, CHROMA_FILTER) if chroma else (8, 4, LUMA_FILTER)
+    if fi == 1 and not chroma:
+        tab = [LUMA_ALT_HPEL if k == 8 else tab[k] for k in range(16)]
     mask = (1 << shift) - 1
     fx, fy = frac & mask, frac >> shift
     o = taps // 2 - 1                       # 3 for luma, 1 for chroma
@@ -348,3 +356,77 @@ def me_pipeline(cur, refs, bw, bh, rng, lam, ctu, offs, lam_refine):
         best[b] = (bi[0], bc[0], bmx[0], bmy[0])
     return dict(full=full, pred=pmv, int=imv, icost=icost, q=qmv, qcost=qcost, ref=ref_idx, cand=cand, best=best,
                 pred_planes=pred)
+
+
+# ------------------------------------------------------------------- AMVP ---
+AMVR_PREC = (4, 2, 0, 3)      # amvrPrecision by imvIdx, h266_entropy_write.c:120-123
+
+
+def change_prec(v, in_prec, out_prec):
+    """One MV component from precision in_prec to out_prec (log2 units per 4
+    pel): exact when the precision rises, else the nearest value with ties
+    toward zero (the rounding of Xin266ChangeMv32Prec,
+    h266_get_neighbour_mv.c:212-243), written with exact fractions."""
+    from fractions import Fraction
+    s = out_prec - in_prec
+    if s >= 0:
+        return v * (1 << s)
+    q = Fraction(v, 1 << -s)
+    lo = q.numerator // q.denominator
+    hi = lo + (q != lo)
+    if q - lo < hi - q:
+        return lo
+    if q - lo > hi - q:
+        return hi
+    return lo if abs(lo) < abs(hi) else hi
+
+
+def amvp_list(field, fw, fh, x, y, w, h):
+    """AMVP candidates of a block from an MV field [uy, ux, 2] of 4x4 units
+    (every unit inside the picture an available neighbour), restated from
+    Xin266FillAmvpCand (h266_get_neighbour_mv.c:2451-2560): below-left else
+    left; above-right else above else above-left; a duplicate pair kept once;
+    a zero MV appended to an empty list or to a single non-zero entry; the
+    second entry kept only when it differs from the first."""
+    def at(px, py):
+        if 0 <= px < fw and 0 <= py < fh:
+            return (int(field[py >> 2, px >> 2, 0]), int(field[py >> 2, px >> 2, 1]))
+        return None
+    out = []
+    for group in (((x - 1, y + h), (x - 1, y + h - 1)),
+                  ((x + w, y - 1), (x + w - 1, y - 1), (x - 1, y - 1))):
+        for px, py in group:
+            mv = at(px, py)
+            if mv is not None:
+                out.append(mv)
+                break
+    if len(out) == 2 and out[0] == out[1]:
+        out = out[:1]
+    if not out or (len(out) == 1 and out[0] != (0, 0)):
+        out.append((0, 0))
+    return out if len(out) >= 2 and out[0] != out[1] else out[:1]
+
+
+def select_mvp(cur, ref, pad, x, y, w, h, cands, imv, ctu):
+    """Xin266SelectMvp (h266_analyze_inter_cu.c:60-131) of one block: each
+    candidate rounded to the AMVR precision and back, its luma prediction
+    (MV clipped to the CU window, h266_inter_prediction.c:332-342; the
+    alternative half-pel filter for imvIdx 3) compared by SAD, the first
+    strictly lowest kept.  ref: the reference plane padded by `pad`.
+    Returns (index, sad, rounded MV, MV in quarter pel)."""
+    fh, fw = cur.shape
+    mnx, mny, mxx, mxy = cu_window(x, y, w, h, fw, fh, ctu)
+    prec = AMVR_PREC[imv]
+    best = None
+    for i, (cx, cy) in enumerate(cands):
+        rx = change_prec(change_prec(cx, 6, prec), prec, 6)
+        ry = change_prec(change_prec(cy, 6, prec), prec, 6)
+        ox = 16 * x + min(max(rx, 16 * mnx), 16 * mxx)
+        oy = 16 * y + min(max(ry, 16 * mny), 16 * mxy)
+        p = interp(ref, (ox >> 4) + pad, (oy >> 4) + pad, w, h, ((oy & 15) << 4) | (ox & 15), False,
+                   1 if imv == 3 else 0)
+        s = sad(cur[y:y + h, x:x + w], p)
+        if best is None or s < best[1]:
+            best = (i, s, (rx, ry))
+    i, s, (rx, ry) = best
+    return i, s, (rx, ry), (change_prec(rx, 6, 4), change_prec(ry, 6, 4))

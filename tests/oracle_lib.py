@@ -68,6 +68,9 @@ def load_oracle(path=None):
         "or_MvdBits": (U32, [I32, I32]),
         "or_SadLambda": (U32, [I32, I32]),
         "or_MvRoundShift": (I32, [I32, I32]),
+        "or_ChangeMvPrec": (I32, [I32, I32, I32]),
+        "or_AmvpFromField": (I32, [P, SS, I32, I32, I32, I32, I32, I32, P]),
+        "or_SelectMvp": (I32, [P, SS, P, SS, I32, I32, I32, I32, I32, I32, I32, P, I32, I32, P, P, P]),
         "or_MotionEstFull": (None, [ctypes.POINTER(OrMe)]),
         "or_MotionEstDia": (None, [ctypes.POINTER(OrMe)]),
         "or_FullSubPelMe": (None, [ctypes.POINTER(OrMe)]),

@@ -1,5 +1,6 @@
-"""The oracle's view of one served frame: the stream's predictors from its MV
-field, the serving integer search (or_SearchBlocks), the refinement + MC
+"""The oracle's view of one served frame: the stream's predictors (the AMVP
+candidates of its MV field, or_AmvpFromField, chosen by or_SelectMvp), the
+serving integer search (or_SearchBlocks), the refinement + MC
 (or_RefineFracBlocks), and the field the frame leaves behind."""
 import ctypes
 from concurrent.futures import ThreadPoolExecutor
@@ -22,14 +23,21 @@ def block_array(src):
     return out
 
 
-def predictors(blocks, field):
-    """The MV of the 4x4 unit at each block's centre ((x + w/2) >> 2,
-    (y + h/2) >> 2) in field[uy, ux] (1/16 pel), zero when field is None."""
-    if field is None:
-        return np.zeros((len(blocks), 2), np.int32)
-    ux = (blocks[:, 0] + blocks[:, 2] // 2) >> 2
-    uy = (blocks[:, 1] + blocks[:, 3] // 2) >> 2
-    return field[uy, ux]
+def predictors(oracle, blocks, field, cur, refp, W, H):
+    """Each block's predictor (1/16 pel) as the server derives it: the AMVP
+    candidates of the stream's MV field (or_AmvpFromField; the zero field
+    before the first frame), the quarter-pel AMVR choice by prediction SAD in
+    the reference (or_SelectMvp).  refp: the reference luma padded by PAD."""
+    f = np.zeros((H // 4, W // 4, 2), np.int32) if field is None else np.ascontiguousarray(field, dtype=np.int32)
+    out = np.zeros((len(blocks), 2), np.int32)
+    cand, mv, me, sad = np.zeros(4, np.int32), np.zeros(2, np.int32), np.zeros(2, np.int16), ctypes.c_uint32()
+    rs = W + 2 * PAD
+    for i, (x, y, w, h) in enumerate(blocks[:, :4].tolist()):
+        nc = oracle.or_AmvpFromField(ptr(f), W // 4, W, H, x, y, w, h, ptr(cand))
+        oracle.or_SelectMvp(ptr(cur), W, ptr(refp, PAD * rs + PAD), rs, W, H, x, y, w, h, nc, ptr(cand), 0, 64,
+                            ctypes.byref(sad), ptr(mv), ptr(me))
+        out[i] = mv
+    return out
 
 
 def next_field(blocks, mv, field, w, h):
@@ -46,9 +54,9 @@ def oracle_frame(oracle, src, cur, ref, field, threads=16):
     (the previous frame, or cur for the first) and the previous MV field."""
     W, H = src.w, src.h
     blocks = block_array(src)
-    blocks[:, 6:8] = predictors(blocks, field)
     n = len(blocks)
     refp = [np.ascontiguousarray(np.pad(p, PAD if i == 0 else PAD // 2, mode="edge")) for i, p in enumerate(ref)]
+    blocks[:, 6:8] = predictors(oracle, blocks, field, cur[0], refp[0], W, H)
     rsy, rsc = W + 2 * PAD, W // 2 + PAD
     blk = np.ascontiguousarray(blocks)
     step = max(1, (n + threads - 1) // threads)

@@ -1,0 +1,165 @@
+"""AMVP predictor choice on the GPU (xg_amvp_from_field, xg_amvp_select)
+against the oracle (or_AmvpFromField, or_SelectMvp), bit-exact: candidate
+lists from an MV field at every picture edge and block size, the four AMVR
+precisions (with the alternative half-pel filter), CU-window clipping of far
+candidates, first-of-equal ties, and rejected blocks."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.helpers import ptr
+from xin26x_amd import capi
+from xin26x_amd.device import DevicePicture, blocks_to_struct
+
+pytestmark = pytest.mark.gpu
+PAD = 80
+W, H = 320, 192
+AMVP_DTYPE = np.dtype([("x", "<i2"), ("y", "<i2"), ("w", "u1"), ("h", "u1"), ("pic", "u1"), ("ref", "u1"),
+                       ("ncand", "<i4"), ("cand", "<i4", (4,))])
+RES_DTYPE = np.dtype([("mvpIdx", "<i4"), ("sad", "<u4"), ("mv", "<i4", (2,)), ("meMv", "<i2", (2,))])
+
+
+def _scene(seed=21):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H + 8, 0:W + 8]
+    base = 512 + 300 * np.sin(xx / 6.0) * np.cos(yy / 8.0) + rng.normal(0, 30, xx.shape)
+    ref = np.clip(base[:H, :W], 0, 1023).astype(np.uint16)
+    cur = np.clip(base[3:H + 3, 5:W + 5] + rng.normal(0, 6, (H, W)), 0, 1023).astype(np.uint16)
+    chroma = [rng.integers(0, 1024, (H // 2, W // 2)).astype(np.uint16) for _ in range(2)]
+    field = rng.integers(-400, 401, size=(H // 4, W // 4, 2)).astype(np.int32)
+    field[rng.random((H // 4, W // 4)) < 0.15] = 0
+    blocks = []
+    for _ in range(60):
+        uy, ux = int(rng.integers(0, H // 4 - 3)), int(rng.integers(0, W // 4 - 3))
+        hh, ww = (int(v) for v in rng.integers(3, 11, 2))
+        field[uy:uy + hh, ux:ux + ww] = rng.integers(-400, 401, 2)
+        blocks.append((4 * ux + 4, 4 * uy + 4, 4, 4))
+    for w in (4, 8, 16, 32, 64):
+        for h in (4, 8, 16, 32, 64):
+            for x, y in ((0, 0), (W - w, H - h), (0, H - h), (W - w, 0)):
+                blocks.append((x, y, w, h))
+            for _ in range(12):
+                blocks.append((4 * int(rng.integers(0, (W - w) // 4 + 1)), 4 * int(rng.integers(0, (H - h) // 4 + 1)),
+                               w, h))
+    return rng, [cur] + chroma, [ref] + chroma, field, blocks
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    lib = capi.load()
+    rng, cur, ref, field, blocks = _scene()
+    s = ctypes.c_void_p()
+    assert lib.xg_session_create(ctypes.byref(s), 1) == 0
+    dc, dr = DevicePicture(cur, pad=PAD), DevicePicture(ref, pad=PAD)
+    assert lib.xg_session_set_picture(s, 0, dc.ref()) == 0
+    assert lib.xg_session_set_picture(s, 1, dr.ref()) == 0
+    yield lib, s, torch, rng, cur[0], ref[0], field, blocks
+    lib.xg_session_destroy(s)
+
+
+def _dev(torch, arr):
+    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).reshape(-1)).cuda()
+
+
+def _select(lib, s, torch, amvp, imv):
+    d_in = _dev(torch, amvp)
+    d_out = torch.zeros(len(amvp) * RES_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    assert lib.xg_amvp_select(s, ctypes.c_void_p(d_in.data_ptr()), len(amvp), imv, 64,
+                              ctypes.c_void_p(d_out.data_ptr())) == 0
+    assert lib.xg_session_sync(s) == 0
+    return d_out.cpu().numpy().view(RES_DTYPE)
+
+
+def _oracle_select(oracle, cur, refp, b, imv):
+    sad, mv, me = ctypes.c_uint32(), np.zeros(2, np.int32), np.zeros(2, np.int16)
+    c = np.ascontiguousarray(b["cand"].astype(np.int32))
+    rs = W + 2 * PAD
+    idx = oracle.or_SelectMvp(ptr(cur), W, ptr(refp, PAD * rs + PAD), rs, W, H, int(b["x"]), int(b["y"]), int(b["w"]),
+                              int(b["h"]), int(b["ncand"]), ptr(c), imv, 64, ctypes.byref(sad), ptr(mv), ptr(me))
+    return idx, sad.value, (int(mv[0]), int(mv[1])), (int(me[0]), int(me[1]))
+
+
+def test_amvp_lists_from_field(oracle, gpu):
+    lib, s, torch, rng, cur, ref, field, blocks = gpu
+    barr = blocks_to_struct([(x, y, w, h, 0, 0, 0, 0) for x, y, w, h in blocks])
+    d_b, d_f = _dev(torch, barr), _dev(torch, field)
+    d_out = torch.zeros(len(blocks) * AMVP_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    assert lib.xg_amvp_from_field(s, ctypes.c_void_p(d_f.data_ptr()), W // 4, W, H, ctypes.c_void_p(d_b.data_ptr()),
+                                  len(blocks), ctypes.c_void_p(d_out.data_ptr())) == 0
+    assert lib.xg_session_sync(s) == 0
+    got = d_out.cpu().numpy().view(AMVP_DTYPE)
+    fc = np.ascontiguousarray(field)
+    cand = np.zeros(4, np.int32)
+    n2 = 0
+    for i, (x, y, w, h) in enumerate(blocks):
+        nc = oracle.or_AmvpFromField(ptr(fc), W // 4, W, H, x, y, w, h, ptr(cand))
+        g = got[i]
+        assert (int(g["x"]), int(g["y"]), int(g["w"]), int(g["h"]), int(g["pic"]), int(g["ref"])) == (x, y, w, h, 0, 1)
+        assert int(g["ncand"]) == nc, (x, y, w, h)
+        assert list(g["cand"][:2 * nc]) == list(cand[:2 * nc]), (x, y, w, h)
+        n2 += nc == 2
+    assert n2 > 100
+    print(f"{len(blocks)} lists, {n2} with two candidates", flush=True)
+
+
+def test_amvp_select_matches_oracle(oracle, gpu):
+    lib, s, torch, rng, cur, ref, field, blocks = gpu
+    refp = np.ascontiguousarray(np.pad(ref, PAD, mode="edge"))
+    fc = np.ascontiguousarray(field)
+    cand = np.zeros(4, np.int32)
+    kinds = {}
+    a = np.zeros(len(blocks), AMVP_DTYPE)
+    for i, (x, y, w, h) in enumerate(blocks):
+        a[i] = (x, y, w, h, 0, 1, 0, (0, 0, 0, 0))
+    # the field's lists
+    lists = a.copy()
+    for i, (x, y, w, h) in enumerate(blocks):
+        lists[i]["ncand"] = oracle.or_AmvpFromField(ptr(fc), W // 4, W, H, x, y, w, h, ptr(cand))
+        lists[i]["cand"] = cand
+    kinds["field"] = lists
+    # far candidates: the CU window clips the prediction's MV
+    far = a.copy()
+    far["ncand"] = 2
+    far["cand"] = rng.integers(-2200, 2201, (len(blocks), 4))
+    kinds["far"] = far
+    # two candidates equal after rounding at every coarser precision: the first wins
+    tie = a.copy()
+    tie["ncand"] = 2
+    base = 16 * rng.integers(-40, 41, (len(blocks), 2))
+    tie["cand"][:, 0:2] = base
+    tie["cand"][:, 2] = base[:, 0] + 1
+    tie["cand"][:, 3] = base[:, 1] - 1
+    kinds["tie"] = tie
+    n_second = 0
+    for kind, arr in kinds.items():
+        for imv in range(4):
+            got = _select(lib, s, torch, arr, imv)
+            for i in range(len(arr)):
+                exp = _oracle_select(oracle, cur, refp, arr[i], imv)
+                g = got[i]
+                assert (int(g["mvpIdx"]), int(g["sad"]), (int(g["mv"][0]), int(g["mv"][1])),
+                        (int(g["meMv"][0]), int(g["meMv"][1]))) == exp, (kind, imv, arr[i])
+                n_second += exp[0] == 1
+            print(f"amvp select: {kind} lists, imvIdx {imv}: {len(arr)} blocks bit-exact", flush=True)
+    assert n_second > 200          # both outcomes of the choice occur
+
+
+def test_amvp_rejected_blocks(gpu):
+    lib, s, torch, rng, cur, ref, field, blocks = gpu
+    a = np.zeros(6, AMVP_DTYPE)
+    a[0] = (8, 8, 16, 16, 0, 1, 1, (0, 0, 0, 0))          # fine
+    a[1] = (8, 8, 6, 16, 0, 1, 1, (0, 0, 0, 0))           # side not a multiple of 4
+    a[2] = (W - 8, 8, 16, 16, 0, 1, 1, (0, 0, 0, 0))      # outside the picture
+    a[3] = (8, 8, 16, 16, 5, 1, 1, (0, 0, 0, 0))          # unset slot
+    a[4] = (8, 8, 16, 16, 0, 200, 1, (0, 0, 0, 0))        # slot beyond the table
+    a[5] = (8, 8, 128, 16, 0, 1, 1, (0, 0, 0, 0))         # side above 64
+    got = _select(lib, s, torch, a, 0)
+    assert int(got[0]["mvpIdx"]) == 0 and int(got[0]["sad"]) != 0xFFFFFFFF
+    for i in range(1, 6):
+        assert int(got[i]["mvpIdx"]) == -1 and int(got[i]["sad"]) == 0xFFFFFFFF, i
+    lib.xg_clear_error()
+    assert lib.xg_amvp_select(s, None, 4, 0, 64, None) == capi.XG_ERR_ARG
+    assert lib.xg_amvp_select(s, None, 0, 4, 64, None) == capi.XG_ERR_ARG
+    lib.xg_clear_error()

@@ -2,7 +2,7 @@
 1280x720, 4 x 1920x1080) for 8 distinct frames each.  Frames batched across
 all streams give exactly the results and predictions each stream gets alone;
 sampled frames of every resolution equal the oracle's predictor derivation
-(the stream's MV field), serving search, refinement and MC on the same
+(AMVP from the stream's MV field), serving search, refinement and MC on the same
 pictures (reference = the stream's previous frame, edge-padded), and one
 stream is followed by the oracle through its whole sequence.  Device-resident
 and page-locked submissions give the pageable path's results.  Admission by
@@ -13,7 +13,7 @@ import hashlib
 import numpy as np
 import pytest
 
-from tests.serve_common import block_array, oracle_frame, predictors
+from tests.serve_common import PAD, block_array, next_field, oracle_frame, predictors
 from xin26x_amd import capi
 from xin26x_amd.serve import MIX_16, DeviceFrames, Server, StreamSource, mix_sources, server_cfg
 
@@ -111,7 +111,6 @@ def test_16_stream_mix_alone_equals_mix_equals_oracle(oracle, mix):
         for t in frames:
             field = None
             if t > 0:
-                from tests.serve_common import next_field
                 field = None
                 for u in range(t):       # the field after frame t-1 is written by every earlier frame in order
                     field = next_field(blocks, alone[k][u][0][:, :2], field, s.w, s.h)
@@ -127,8 +126,13 @@ def test_16_stream_mix_alone_equals_mix_equals_oracle(oracle, mix):
     for t in range(FRAMES):
         eres, _, field = oracle_frame(oracle, s, s.frame(t), s.frame(t - 1) if t else s.frame(t), field)
         assert np.array_equal(alone[0][t][0], eres), t
-    # predictors really come from the previous frame: frame 1's are frame 0's MVs at the block centres
-    assert np.any(predictors(block_array(s), field) != 0)
+    # predictors really come from the previous frames' field: frame 3's are not all zero (frame 0 refers
+    # to itself, so its sub-quarter MVs round to zero at quarter-pel AMVR; later frames carry the pan)
+    f3 = None
+    for u in range(3):
+        f3 = next_field(block_array(s), alone[0][u][0][:, :2], f3, s.w, s.h)
+    refp = np.ascontiguousarray(np.pad(s.frame(2)[0], PAD, mode="edge"))
+    assert np.any(predictors(oracle, block_array(s), f3, s.frame(3)[0], refp, s.w, s.h) != 0)
 
 
 def test_device_and_pinned_submissions_match_pageable(small):

@@ -38,14 +38,17 @@ def test_np_interp_matches_reference_golden(golden):
     for i, c in enumerate(cases):
         key = gc.KEYS[int(c[0])]
         frac, w, h, fi, ox, oy = (int(v) for v in c[1:7])
-        if key not in keys or fi != 0:
+        if key not in keys or fi not in (0, 1) or (fi == 1 and key.startswith("Chroma")):
             continue
         chroma, mask = keys[key]
-        got = nr.interp(u, ox, oy, w, h, frac & mask, chroma)
+        got = nr.interp(u, ox, oy, w, h, frac & mask, chroma, fi)
         exp = golden["out_u16"][offs[i]:offs[i + 1]].reshape(h, w)
         assert np.array_equal(got, exp), (key, frac, w, h)
         n += 1
     assert n > 1500
+    nalt = sum(1 for c in cases if int(c[4]) == 1 and gc.KEYS[int(c[0])].startswith("Luma")
+               and gc.KEYS[int(c[0])] in keys)
+    assert nalt > 300                      # the alternative half-pel filter is covered too
 
 
 def test_np_satd_and_bits_closed_forms():
@@ -390,6 +393,96 @@ def test_me_pipeline_rebuild(oracle):
 
 
 # ------------------------------------------------------------- mutations ---
+# ------------------------------------------------------------------ AMVP ---
+def test_change_prec_closed_form(oracle):
+    """or_ChangeMvPrec against nearest-with-ties-toward-zero written with exact
+    fractions, for every precision change the AMVR and ME paths make."""
+    vals = list(range(-700, 701)) + [-40000, -32769, -32768, 32767, 32768, 40001]
+    for inp, outp in [(6, 4), (6, 2), (6, 0), (6, 3), (4, 6), (2, 6), (0, 6), (3, 6), (6, 6)]:
+        for v in vals:
+            assert oracle.or_ChangeMvPrec(v, inp, outp) == nr.change_prec(v, inp, outp), (v, inp, outp)
+
+
+APAD = 256          # reference padding of the AMVP checks: far candidates stay inside it unclipped
+
+
+def _amvp_scene(seed=5, W=192, H=128):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H + 2 * APAD, 0:W + 2 * APAD]
+    base = 512 + 300 * np.sin(xx / 7.0) * np.cos(yy / 9.0) + rng.normal(0, 40, xx.shape)
+    refp = np.clip(base, 0, 1023).astype(np.uint16)
+    cur = np.ascontiguousarray(np.clip(np.roll(base, (3, -5), axis=(0, 1))[APAD:APAD + H, APAD:APAD + W]
+                                       + rng.normal(0, 8, (H, W)), 0, 1023).astype(np.uint16))
+    field = rng.integers(-300, 301, size=(H // 4, W // 4, 2)).astype(np.int32)
+    field[rng.random((H // 4, W // 4)) < 0.15] = 0
+    blocks = []
+    for _ in range(60):                   # uniform patches: equal neighbours exercise the duplicate rule
+        uy, ux = int(rng.integers(0, H // 4 - 3)), int(rng.integers(0, W // 4 - 3))
+        hh, ww = (int(v) for v in rng.integers(3, 11, 2))
+        field[uy:uy + hh, ux:ux + ww] = rng.integers(-300, 301, 2)
+        blocks.append((4 * ux + 4, 4 * uy + 4, 4, 4))    # all its neighbours inside the patch
+    for w in (4, 8, 16, 32, 64):
+        for h in (4, 8, 16, 32, 64):
+            for x, y in ((0, 0), (W - w, H - h), (0, H - h), (W - w, 0)):
+                blocks.append((x, y, w, h))
+            for _ in range(14):
+                blocks.append((4 * int(rng.integers(0, (W - w) // 4 + 1)), 4 * int(rng.integers(0, (H - h) // 4 + 1)),
+                               w, h))
+    return cur, refp, field, blocks, rng
+
+
+def _both_groups_equal(field, fw, fh, x, y, w, h):
+    """Whether the left and top groups both yield a candidate and it is the same MV."""
+    def first(pts):
+        for px, py in pts:
+            if 0 <= px < fw and 0 <= py < fh:
+                return tuple(int(v) for v in field[py >> 2, px >> 2])
+        return None
+    a = first(((x - 1, y + h), (x - 1, y + h - 1)))
+    b = first(((x + w, y - 1), (x + w - 1, y - 1), (x - 1, y - 1)))
+    return a is not None and a == b
+
+
+def check_amvp(lib):
+    cur, refp, field, blocks, rng = _amvp_scene()
+    H, W = cur.shape
+    fc = np.ascontiguousarray(field)
+    n_two = n_edge = n_dup = 0
+    cand = np.zeros(4, np.int32)
+    sad, mv, me = ctypes.c_uint32(), np.zeros(2, np.int32), np.zeros(2, np.int16)
+    rs = W + 2 * APAD
+    for i, (x, y, w, h) in enumerate(blocks):
+        exp = nr.amvp_list(field, W, H, x, y, w, h)
+        nc = lib.or_AmvpFromField(ptr(fc), W // 4, W, H, x, y, w, h, ptr(cand))
+        assert [(int(cand[2 * k]), int(cand[2 * k + 1])) for k in range(nc)] == exp, (x, y, w, h)
+        n_two += nc == 2
+        n_dup += exp[0] != (0, 0) and _both_groups_equal(field, W, H, x, y, w, h)
+        # costed lists: the field's, a far pair (CU window clipping) and a pair equal after rounding (ties)
+        far = [(int(v) * 16 + int(rng.integers(-15, 16)), int(u) * 16) for v, u in rng.integers(-140, 141, (2, 2))]
+        base = (int(rng.integers(-60, 61)) * 16, int(rng.integers(-60, 61)) * 16)
+        tie = [base, (base[0] + 1, base[1] - 1)]
+        for lst in (exp, far, tie):
+            for imv in range(4):
+                if lst is tie and imv == 0:
+                    continue
+                c = np.ascontiguousarray(np.array(lst, np.int32).reshape(-1))
+                got = lib.or_SelectMvp(ptr(cur), W, ptr(refp, APAD * rs + APAD), rs, W, H, x, y, w, h, len(lst),
+                                       ptr(c), imv, 64, ctypes.byref(sad), ptr(mv), ptr(me))
+                e = nr.select_mvp(cur, refp, APAD, x, y, w, h, lst, imv, 64)
+                assert (got, sad.value, tuple(int(v) for v in mv), tuple(int(v) for v in me)) == \
+                    (e[0], e[1], e[2], e[3]), (x, y, w, h, lst, imv)
+        n_edge += x == 0 or y == 0
+    assert n_two > 100 and n_edge > 50 and n_dup >= 25
+
+
+def test_amvp_rebuild(oracle):
+    """or_AmvpFromField and or_SelectMvp against the numpy restatements: field
+    lists at every edge and size, AMVR rounding at all four precisions, the
+    alternative half-pel filter, CU-window clipping of far candidates and the
+    first-of-equal rule."""
+    check_amvp(oracle)
+
+
 MUTANTS = {
     # or_RefineFracBlocks
     "refine: chroma fraction nibbles swapped": ("int32_t fracC = ((ry & 31) << 5) + (rx & 31);",
@@ -438,9 +531,23 @@ MUTANTS = {
                                                     "diamond"),
     "pipeline: reference ties go to reference 1": ("const int32_t r = qcost2[1] < qcost2[0] ? 1 : 0;",
                                                    "const int32_t r = qcost2[1] <= qcost2[0] ? 1 : 0;", "pipeline"),
+    # or_AmvpFromField / or_SelectMvp
+    "amvp: left before below-left": ("const int32_t lx[2] = {x - 1, x - 1}, ly[2] = {y + h, y + h - 1};",
+                                     "const int32_t lx[2] = {x - 1, x - 1}, ly[2] = {y + h - 1, y + h};", "amvp"),
+    "amvp: duplicate pair kept": ("if (n == 2 && list[0][0] == list[1][0] && list[0][1] == list[1][1]) n = 1;",
+                                  "if (0) n = 1;", "amvp"),
+    "amvp: <= keeps the last of equal SADs": ("        if (s < best) {\n            best = s;\n            idx = c;",
+                                               "        if (s <= best) {\n            best = s;\n            idx = c;",
+                                               "amvp"),
+    "amvp: no AMVR rounding": ("rounded[c][k] = or_ChangeMvPrec(or_ChangeMvPrec(cand[2 * c + k], 6, prec), prec, 6);",
+                               "rounded[c][k] = cand[2 * c + k];", "amvp"),
+    "amvp: default filter at half-pel AMVR": ("imvIdx == 3 ? 1u : 0u);", "0u);", "amvp"),
+    "amvp: prediction MV not clipped to the CU window": (
+        "const int32_t mx = clip3(rounded[c][0], mnx * 16, mxx * 16), my = clip3(rounded[c][1], mny * 16, mxy * 16);",
+        "const int32_t mx = rounded[c][0], my = rounded[c][1];", "amvp"),
 }
 CHECKS = {"refine": lambda lib: check_refine(lib, nblocks=160), "subpel": check_subpel, "pipeline": check_pipeline,
-          "diamond": check_diamond, "cands": check_pipeline_cands, "search": check_search_blocks}
+          "diamond": check_diamond, "cands": check_pipeline_cands, "search": check_search_blocks, "amvp": check_amvp}
 
 
 @pytest.fixture(scope="module")

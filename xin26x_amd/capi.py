@@ -44,6 +44,15 @@ class XgBlockResult(ctypes.Structure):
     _fields_ = [("mvx", I32), ("mvy", I32), ("cost", U32), ("cand", I32)]
 
 
+class XgAmvpBlock(ctypes.Structure):
+    _fields_ = [("x", I16), ("y", I16), ("w", U8), ("h", U8), ("pic", U8), ("ref", U8),
+                ("ncand", I32), ("cand", I32 * 4)]
+
+
+class XgAmvpResult(ctypes.Structure):
+    _fields_ = [("mvpIdx", I32), ("sad", U32), ("mv", I32 * 2), ("meMv", I16 * 2)]
+
+
 class XgServerCfg(ctypes.Structure):
     _fields_ = [("hbmBudget", ctypes.c_uint64), ("maxStreams", I32), ("maxBatchBlocks", I32),
                 ("searchRange", I32), ("searchLambda", U32), ("maxDelayUs", I32), ("refine", XgRefineCfg),
@@ -207,6 +216,8 @@ PROTOS = {
     "xg_fdct_quant_plan": (I32, [P, P, P, P, I32]),
     "xg_fdct_quant_run": (I32, [P, P, P, P, P]),
     "xg_lookahead_me": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, U32, P, P, P]),
+    "xg_amvp_select": (I32, [P, P, I32, I32, I32, P]),
+    "xg_amvp_from_field": (I32, [P, P, I32, I32, I32, P, I32, P]),
     "xg_lookahead_me_static": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, U32, P, P, P]),
     "xg_bipred_plan": (I32, [P, P, I32]),
     "xg_bipred_run": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, P]),

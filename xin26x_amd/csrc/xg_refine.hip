@@ -1274,6 +1274,8 @@ int32_t xg_session_create(xg_session **out, int32_t maxBlocks)
     bool ok = XG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) &&
               XG_HIP(hipMalloc((void **)&s->dPics, sizeof(xg_picture) * kPicSlots)) &&
               XG_HIP(hipMalloc((void **)&s->dPred, sizeof(xg_picture) * kPicSlots)) &&
+              XG_HIP(hipMemset(s->dPics, 0, sizeof(xg_picture) * kPicSlots)) &&      // unset slots: null planes
+              XG_HIP(hipMemset(s->dPred, 0, sizeof(xg_picture) * kPicSlots)) &&
               XG_HIP(hipMalloc((void **)&s->dBlocks, sizeof(xg_block) * (size_t)maxBlocks)) &&
               XG_HIP(hipMalloc((void **)&s->dCandCost, sizeof(uint32_t) * (size_t)maxBlocks * XG_MAX_CAND)) &&
               XG_HIP(hipMalloc((void **)&s->dBest, sizeof(xg_block_result) * (size_t)maxBlocks)) &&

@@ -90,26 +90,30 @@ struct RefineList {
 // One frame's blocks in a multi-list integer-search launch (serving).
 struct SearchList {
     xg_block *blocks;                  // device; slots, predictor and base MV are written here
-    const int2 *mvField;               // the stream's 4x4-unit MV field (predictor source)
+    const int2 *mvField;               // the stream's 4x4-unit MV field (AMVP candidate source)
     int mvStride;                      // units per field row
     int begin;                         // first workgroup of this list in the launch
     uint8_t curSlot, refSlot;          // picture slots of the frame and its reference
     const int *gidx;                   // k_search_groups: block indices, group by group
     const int2 *groups;                // k_search_groups: (first entry of gidx, count) per group
+    int blockBegin;                    // first workgroup of this list in the k_amvp_multi launch
 };
 // Groups of a partition for k_search_groups: blocks of one 64x64 CTU, at most
 // kSearchGroupMax per group, whose current samples span at most 64x64.
 constexpr int kSearchGroupMax = 64;
 void search_groups(const xg_block *blocks, int n, std::vector<int> &gidx, std::vector<int2> &groups);
 // Enqueues k_search_groups over lists[0..nLists) (device) covering nGroups
-// groups in all on `st` (same results as k_search_multi).
+// groups in all on `st`: every block searched +/-range around the predictor
+// its list's blocks carry (mvpx, mvpy), with the results of search_block.
 int32_t search_groups_enqueue(hipStream_t st, xg_session *timer, const xg_picture *dPics, const SearchList *dLists,
                               int nLists, int nGroups, int range, uint32_t lambda, int ctu);
-// Enqueues k_search_multi over lists[0..nLists) (device) covering nBlocks
-// blocks in all on `st`: each block's predictor from its stream's MV field,
-// then the search +/-range around it.  maxW/maxH: the largest block sides.
-int32_t search_multi_enqueue(hipStream_t st, xg_session *timer, const xg_picture *dPics, const SearchList *dLists,
-                             int nLists, int nBlocks, int maxW, int maxH, int range, uint32_t lambda, int ctu);
+// Enqueues k_amvp_multi over lists[0..nLists) (device) covering nBlocks blocks
+// in all on `st`: each block takes its frame's picture slots and, as
+// predictor, the AMVP choice (Xin266SelectMvp, quarter-pel AMVR) among the
+// candidates its stream's MV field gives it (xg_amvp_from_field), written to
+// the block's mvpx / mvpy in 1/16 pel.
+int32_t amvp_multi_enqueue(hipStream_t st, const xg_picture *dPics, const SearchList *dLists, int nLists,
+                           int nBlocks, int ctu);
 
 // Persistent workgroups of the session's batched form (CUs x workgroups per CU).
 int refine_grid(xg_session *s);

@@ -469,29 +469,6 @@ __global__ void __launch_bounds__(1024) k_search_blocks(const xg_picture *pics, 
     }
 }
 
-// Multi-list form (serving): the blocks of several streams' frames in one
-// launch.  Workgroup w belongs to the last list whose `begin` is <= w.  Each
-// block first takes its frame's picture slots and, as predictor, the MV of
-// the 4x4 unit at its centre in the stream's MV field (the final MVs of the
-// stream's previous frame; zero before the first); the whole updated block
-// (slots, predictor, base MV) is written back for the refinement.
-__global__ void __launch_bounds__(1024) k_search_multi(const xg_picture *pics, const SearchList *__restrict__ lists,
-                                                       int nLists, int range, uint32_t lambda, int ctu, int ws)
-{
-    int l = 0;
-    while (l + 1 < nLists && (int)blockIdx.x >= lists[l + 1].begin) l++;
-    const SearchList S = lists[l];
-    const int b = (int)blockIdx.x - S.begin;
-    xg_block B = S.blocks[b];
-    B.pic = S.curSlot;
-    B.ref = S.refSlot;
-    const int2 p = S.mvField[(intptr_t)((B.y + (B.h >> 1)) >> 2) * S.mvStride + ((B.x + (B.w >> 1)) >> 2)];
-    B.mvpx = p.x;
-    B.mvpy = p.y;
-    search_block(pics, B, range, lambda, ctu, ws);
-    if (threadIdx.x == 0) S.blocks[b] = B;
-}
-
 }  // namespace xg
 
 using namespace xg;
@@ -721,30 +698,14 @@ int32_t xg::full_search_frame_on(hipStream_t st, xg_session *timer, const xg_pla
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
-int32_t xg::search_multi_enqueue(hipStream_t st, xg_session *timer, const xg_picture *dPics, const SearchList *dLists,
-                                 int nLists, int nBlocks, int maxW, int maxH, int range, uint32_t lambda, int ctu)
-{
-    if (nLists <= 0 || nBlocks <= 0) return XG_OK;
-    const int ws = packed_ws(2 * range + 1, maxW);
-    const size_t lds = ((size_t)(2 * range + maxH) * ws + (size_t)maxW * maxH) * 2 + 16;
-    if (lds > 64 * 1024) {
-        set_error(XG_ERR_ARG, "search_multi_enqueue: window exceeds the LDS budget");
-        return XG_ERR_ARG;
-    }
-    int ev[2];
-    const bool timed = timer && session_timer_begin(timer, 3, ev);
-    k_search_multi<<<nBlocks, packed_threads(2 * range + 1), lds, st>>>(dPics, dLists, nLists, range, lambda, ctu, ws);
-    if (timed) session_timer_end(timer, 3, ev);
-    return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
-}
-
 // ---------------------------------------------------------------------------
 // Serving search by CTU group
 //
-// k_search_multi spends a workgroup and a window staging per block; at 1080p
-// most blocks are 4..16 wide, so most of its time is setup.  Here a workgroup
-// of four waves takes a group (up to 64 blocks of one CTU): it derives every
-// block's predictor and window, stages the union of the windows once in LDS
+// A workgroup and a window staging per block would spend most of the time on
+// setup at 1080p, where most blocks are 4..16 wide.  Here a workgroup of four
+// waves takes a group (up to 64 blocks of one CTU): it takes every block's
+// predictor (written by the AMVP stage, k_amvp_multi) and window, stages the
+// union of the windows once in LDS
 // (with the CTU's current samples), and spreads (block, 64 candidate-rows)
 // units over its waves; each block's minimum (cost << 32 | raster index in
 // its own window) is kept by an LDS 64-bit atomic min, so the winner and the
@@ -890,9 +851,6 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
         B = L.blocks[b];
         B.pic = L.curSlot;
         B.ref = L.refSlot;
-        const int2 p = L.mvField[(intptr_t)((B.y + (B.h >> 1)) >> 2) * L.mvStride + ((B.x + (B.w >> 1)) >> 2)];
-        B.mvpx = p.x;
-        B.mvpy = p.y;
         const int mnx = (int16_t)(-B.x - ctu - 8), mny = (int16_t)(-B.y - ctu - 8);
         const int mxx = (int16_t)(fw + (ctu - B.w + 8) - B.x), mxy = (int16_t)(fh + (ctu - B.h + 8) - B.y);
         const int cx = min(max(((B.mvpx >> 2) + 2) >> 2, mnx), mxx), cy = min(max(((B.mvpy >> 2) + 2) >> 2, mny), mxy);

@@ -17,11 +17,12 @@
 // frame of every stream with fewer than two frames in flight and enqueues, for
 // all of them together: the plane copies and one padding launch; then, on the
 // device, a wait for the other session's refinement (which wrote the MV
-// fields, references and partitions this batch reads); then one
-// integer-search launch (k_search_groups, a workgroup per group of a CTU's
-// blocks: each block's predictor read from its stream's MV field, then
-// +/-range around it) and one refinement + MC launch (k_refine_multi, which
-// also writes the winners into the MV fields), then the result copies.
+// fields, references and partitions this batch reads); then one AMVP launch
+// (k_amvp_multi: each block's predictor chosen by prediction SAD among the
+// AMVP candidates its stream's MV field gives it), one integer-search launch
+// (k_search_groups, a workgroup per group of a CTU's blocks, +/-range around
+// each block's predictor) and one refinement + MC launch (k_refine_multi,
+// which also writes the winners into the MV fields), then the result copies.
 // Results and predictions rotate with the pictures, so those copies never
 // hold up the next batch.  The launches walk a per-batch table of per-stream
 // descriptors, so a batch costs the host a few hundred bytes of descriptors,
@@ -413,7 +414,7 @@ struct xg_server {
         // descriptor tables of the batch
         Descs &D = *b.hDesc;
         const int nf = (int)b.reqs.size();
-        int begin = 0, taskAcc = 0, nPad = 0, nUp = 0;
+        int begin = 0, blockBegin = 0, taskAcc = 0, nPad = 0, nUp = 0;
         for (int i = 0; i < nf && rc == XG_OK; i++) {
             Request *r = b.reqs[i];
             Stream &st = streams[r->stream];
@@ -425,8 +426,9 @@ struct xg_server {
             }
             D.search[i] = SearchList{st.d.blocks, st.d.field, st.d.fieldStride, begin,
                                      (uint8_t)(kPics * id + r->curPic), (uint8_t)(kPics * id + r->refPic),
-                                     st.d.gidx, st.d.groups};
+                                     st.d.gidx, st.d.groups, blockBegin};
             begin += st.devGroups;
+            blockBegin += st.devBlocks;
             // spread small partitions over the grid, in steps of whole XCD bands
             const int rot = (taskAcc / 8 * 8) % grid;
             taskAcc += st.devTasks;
@@ -461,10 +463,14 @@ struct xg_server {
         for (Request *r : b.reqs)
             if (rc == XG_OK) rc = stage_partition(r, hs);
         if (rc == XG_OK) {
+            // AMVP predictors from the streams' fields, then the search around them
             SessionView v;
             session_view(s, &v);
-            rc = search_groups_enqueue(hs, s, v.dPics, dev_member<SearchList>(b.dDesc, offsetof(Descs, search)), nf,
-                                       begin, cfg.searchRange, cfg.searchLambda, cfg.refine.ctuSize);
+            const SearchList *dl = dev_member<SearchList>(b.dDesc, offsetof(Descs, search));
+            rc = amvp_multi_enqueue(hs, v.dPics, dl, nf, blockBegin, cfg.refine.ctuSize);
+            if (rc == XG_OK)
+                rc = search_groups_enqueue(hs, s, v.dPics, dl, nf, begin, cfg.searchRange, cfg.searchLambda,
+                                           cfg.refine.ctuSize);
         }
         if (rc == XG_OK)
             rc = refine_multi_enqueue(s, &cfg.refine, dev_member<RefineList>(b.dDesc, offsetof(Descs, refine)), nf);
