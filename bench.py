@@ -23,6 +23,7 @@ import concurrent.futures
 import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -86,7 +87,45 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(fp, seconds, offs, lam):
+def host_threads():
+    """Host threads a CPU leg may use: the process's CPU affinity, at most 16
+    (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def one_and_all_cores(run, seconds):
+    """A CPU leg at one host thread (a third of its time budget) and at every
+    thread it may use (the rest): `value` is the all-thread figure,
+    `value_1core` the one-thread one; bit-exactness flags hold for both runs."""
+    n = host_threads()
+    one = run(seconds / 3, 1)
+    out = dict(run(seconds * 2 / 3, n)) if n > 1 else dict(one)
+    out["cores"] = n
+    out["value_1core"] = one["value"]
+    out["sample_1core"] = one["sample"]
+    out["cpu_model"] = cpu_model()
+    for k in ("bit_exact_vs_gpu_on_sample", "bit_exact_vs_gpu"):
+        if k in one:
+            out[k] = bool(out.get(k, True)) and bool(one[k])
+    return out
+
+
+def cpu_baseline(fp, seconds, offs, lam, threads=None):
     """The oracle on a bounded sample of the same workload, host threads."""
     from tests.oracle_lib import OrRefinePrm, load_oracle
     oracle = load_oracle()
@@ -100,7 +139,7 @@ def cpu_baseline(fp, seconds, offs, lam):
     prm = OrRefinePrm(nc, offx.ctypes.data, offy.ctypes.data, lam, 64, 1)
     rsy, rsc = W + 2 * pad, W // 2 + pad
     P = lambda a, off=0: ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or host_threads()
     chunk = 256
     n = len(blk)
 
@@ -165,6 +204,70 @@ def residual_workload():
     return _RESIDUALS["blocks"]
 
 
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libxin_ref.so")
+
+
+def ref_interp_cpu(fp, offs, seconds, threads=None):
+    """The reference's own code on the host: the interpolation share of the
+    same refinement (every block's candidates through the reference's 10-bit
+    h266_luma_mc.c / h266_chroma_mc.c, built from its sources into
+    oracle/_ref by oracle/Makefile, driven by oracle/ref_interp.c) in chunks
+    of blocks on host threads.  Its last luma prediction of each chunk is
+    compared with the oracle's.  None when that library was not built."""
+    if not os.path.exists(REF_LIB):
+        return None
+    from tests.oracle_lib import load_oracle
+    ref = ctypes.CDLL(REF_LIB)
+    ref.ref_RefineInterpShare.restype = ctypes.c_int64
+    oracle = load_oracle()
+    threads = threads or host_threads()
+    pad = 80
+    W, H = fp.width, fp.height
+    refp = [np.ascontiguousarray(np.pad(p, pad if i == 0 else pad // 2, mode="edge")) for i, p in enumerate(fp.ref)]
+    rsy, rsc = W + 2 * pad, W // 2 + pad
+    blk = np.ascontiguousarray(fp.blocks.astype(np.int32))
+    offx = np.ascontiguousarray(np.array([o[0] for o in offs], np.int16))
+    offy = np.ascontiguousarray(np.array([o[1] for o in offs], np.int16))
+    P = lambda a, off=0: ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
+    chunk, n, nc = 256, len(blk), len(offs)
+
+    def work(start):
+        cnt = min(chunk, n - start)
+        oy, ou, ov = np.zeros(64 * 64, np.uint16), np.zeros(32 * 32, np.uint16), np.zeros(32 * 32, np.uint16)
+        px = ref.ref_RefineInterpShare(P(refp[0], pad * rsy + pad), P(refp[1], (pad // 2) * rsc + pad // 2),
+                                       P(refp[2], (pad // 2) * rsc + pad // 2), ctypes.c_ssize_t(rsy),
+                                       ctypes.c_ssize_t(rsc), W, H, P(blk, start * 8), cnt, nc, P(offx), P(offy), 64,
+                                       P(oy), P(ou), P(ov))
+        x, y, w, h, mx, my = (int(v) for v in blk[start + cnt - 1, :6])
+        mnx, mny = np.int16(-x - 72), np.int16(-y - 72)
+        mxx, mxy = np.int16(W + (64 - w + 8) - x), np.int16(H + (64 - h + 8) - y)
+        mvx = min(max(mx + int(offx[-1]), 16 * int(mnx)), 16 * int(mxx))
+        mvy = min(max(my + int(offy[-1]), 16 * int(mny)), 16 * int(mxy))
+        rx, ry = 16 * x + mvx, 16 * y + mvy
+        e = np.zeros(64 * 64, np.uint16)
+        oracle.or_LumaInterpolation(P(refp[0], (pad + (ry >> 4)) * rsy + pad + (rx >> 4)), rsy, P(e), 64,
+                                    ((ry & 15) << 4) + (rx & 15), w, h, 0)
+        same = np.array_equal(oy.reshape(64, 64)[:h, :w], e.reshape(64, 64)[:h, :w])
+        return px, same
+
+    px, exact, pos = 0, True, 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < seconds:
+            starts = [((pos + k) * chunk) % n for k in range(threads * 2)]
+            pos += threads * 2
+            for p_, ok in ex.map(work, starts):
+                px += p_
+                exact &= bool(ok)
+    dt = time.perf_counter() - t0
+    return {"value": round(px / dt / 1e6, 3), "unit": "Mpixel/s", "cores": threads, "kind": "reference",
+            "sample": f"the reference's 10-bit h266_luma_mc.c + h266_chroma_mc.c (gcc -O2, built from its sources) "
+                      f"on the interpolation share only (luma 8-tap + Cb/Cr 4-tap prediction of all {nc} "
+                      f"candidates, no SATD, cost or MC) of the workload's blocks in chunks of {chunk}, {threads} "
+                      f"threads, {dt:.1f} s, {px / 1e6:.1f} Mpixel of blocks",
+            "bit_exact_vs_oracle_on_sample": bool(exact)}
+
+
 def transforms_leg(lib, steps, warmup, cpu_seconds):
     """200,000 residual blocks, shapes 4..64 x 4..64, DCT-II or DST-VII/DCT-VIII:
     the batched int8-MFMA forward transform, against the HBM and MFMA rooflines."""
@@ -206,16 +309,17 @@ def transforms_leg(lib, steps, warmup, cpu_seconds):
                     "frac": round(2 * macs / k_s / 1e12 / MFMA_I8_PEAK_TOPS, 6),
                     "useful_macs_per_launch": macs}}
     if cpu_seconds > 0:
-        out["cpu_baseline"] = transforms_cpu(desc, offs, resid, coeff, cpu_seconds)
+        out["cpu_baseline"] = one_and_all_cores(
+            lambda sec, t: transforms_cpu(desc, offs, resid, coeff, sec, threads=t), cpu_seconds)
     return out
 
 
-def transforms_cpu(desc, offs, resid, gpu_coeff, seconds):
+def transforms_cpu(desc, offs, resid, gpu_coeff, seconds, threads=None):
     """The oracle's integer transform on the host, chunks of the same blocks;
     the chunks it computes are also compared with the GPU's coefficients."""
     from tests.oracle_lib import load_oracle
     oracle = load_oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or host_threads()
     out = np.zeros(resid.size, np.int32)
     chunk, n = 2000, len(desc)
     P = lambda a: ctypes.c_void_p(a.ctypes.data)
@@ -301,16 +405,17 @@ def recon_leg(lib, steps, warmup, cpu_seconds):
                     "frac": round(2 * macs / k_s / 1e12 / MFMA_I8_PEAK_TOPS, 6),
                     "useful_macs_per_launch": macs}}
     if cpu_seconds > 0:
-        out["cpu_baseline"] = recon_cpu(desc, offs, coeff_h, pred, recon, cpu_seconds)
+        out["cpu_baseline"] = one_and_all_cores(
+            lambda sec, t: recon_cpu(desc, offs, coeff_h, pred, recon, sec, threads=t), cpu_seconds)
     return out
 
 
-def recon_cpu(desc, offs, coeff, pred, gpu_recon, seconds):
+def recon_cpu(desc, offs, coeff, pred, gpu_recon, seconds, threads=None):
     """The oracle's reconstruction on the host, chunks of the same blocks;
     the chunks it computes are also compared with the GPU's samples."""
     from tests.oracle_lib import load_oracle
     oracle = load_oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or host_threads()
     out = np.zeros(pred.size, np.uint16)
     chunk, n = 2000, len(desc)
     P = lambda a: ctypes.c_void_p(a.ctypes.data)
@@ -423,17 +528,18 @@ def residual_loop_leg(lib, steps, warmup, cpu_seconds):
                               "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
                               "traffic": traffic, "traffic_source": src, "alg_bytes_per_launch": alg_bytes}}
     if cpu_seconds > 0:
-        out["cpu_baseline"] = quant_cpu(desc, dq_desc, offs, coeff, got, cpu_seconds)
+        out["cpu_baseline"] = one_and_all_cores(
+            lambda sec, t: quant_cpu(desc, dq_desc, offs, coeff, got, sec, threads=t), cpu_seconds)
     return out
 
 
-def quant_cpu(desc, dq_desc, offs, coeff, got, seconds):
+def quant_cpu(desc, dq_desc, offs, coeff, got, seconds, threads=None):
     """The oracle's quantiser on the host over chunks of the same TUs (the GPU's
     forward coefficients as input); the chunks it computes are compared with the
     GPU's levels, dequantised values and maps."""
     from tests.oracle_lib import load_oracle
     oracle = load_oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or host_threads()
     q = np.zeros(coeff.size, np.int16)
     r = np.zeros(coeff.size, np.int16)
     nz = np.zeros(len(desc), np.uint64)
@@ -494,11 +600,12 @@ def lookahead_leg(lib, fp, cur, ref, steps, warmup, cpu_seconds, unit=16, rng=32
            "ms_per_frame": round(wall_ms, 4), "kernel": "k_down2x2 x 2 + k_full_search_frame_p<16>",
            "search_kernel_ms": round(k_ms, 4), "abs_diffs_per_s": round(evals / (k_ms / 1e3), 1)}
     if cpu_seconds > 0:
-        out["cpu_baseline"] = lookahead_cpu(fp, unit, rng, lam, got, cpu_seconds)
+        out["cpu_baseline"] = one_and_all_cores(
+            lambda sec, t: lookahead_cpu(fp, unit, rng, lam, got, sec, threads=t), cpu_seconds)
     return out
 
 
-def lookahead_cpu(fp, unit, rng, lam, got, seconds):
+def lookahead_cpu(fp, unit, rng, lam, got, seconds, threads=None):
     """The oracle's downscale and full search on the host over rows of units;
     the rows it computes are compared with the GPU's MVs, costs and SADs."""
     from tests.oracle_lib import load_oracle
@@ -516,7 +623,7 @@ def lookahead_cpu(fp, unit, rng, lam, got, seconds):
     nbx, nb = hw // unit, (hw // unit) * (hh // unit)
     mx, my = np.zeros(nb, np.int16), np.zeros(nb, np.int16)
     c, sd = np.zeros(nb, np.uint32), np.zeros(nb, np.uint32)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or host_threads()
 
     def work(r):
         b0, b1 = r * nbx, (r + 1) * nbx
@@ -580,26 +687,44 @@ def bipred_leg(lib, fp, cur, ref, pred, steps, warmup, cpu_seconds):
                         "traffic": profiled_traffic(PMC_BIPRED)[0], "traffic_source": profiled_traffic(PMC_BIPRED)[1],
                         "alg_bytes_per_launch": alg_bytes}}
     if cpu_seconds > 0:
-        from tests.oracle_lib import load_oracle
-        oracle = load_oracle()
-        inp, r0 = np.ascontiguousarray(fp.cur[0]), np.ascontiguousarray(fp.ref[0])
-        p1h = np.ascontiguousarray(pred_h)
-        W = fp.width
-        P = lambda a, o: ctypes.c_void_p(a.ctypes.data + 2 * o)
-        t0 = time.perf_counter()
-        done = 0
-        exact = True
-        for i, (x, y, w, h) in enumerate(fp.blocks[:, :4].tolist()):
-            o = y * W + x
-            exact &= oracle.or_ComputeAvgSatd(P(inp, o), W, P(r0, o), P(p1h, o), W, w, h) == int(got[i])
-            done += 1
-            if time.perf_counter() - t0 > cpu_seconds:
-                break
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(done / dt, 1), "unit": "blocks/s", "cores": 1, "kind": "port",
-                               "sample": f"oracle (scalar C) on the first {done} blocks, one thread, {dt:.1f} s",
-                               "bit_exact_vs_gpu_on_sample": bool(exact)}
+        out["cpu_baseline"] = one_and_all_cores(lambda sec, t: bipred_cpu(fp, pred_h, got, sec, threads=t),
+                                                cpu_seconds)
     return out
+
+
+def bipred_cpu(fp, pred_h, got, seconds, threads=None):
+    """or_ComputeAvgSatd over chunks of the same blocks on host threads; every
+    block it costs is compared with the GPU's SATD."""
+    from tests.oracle_lib import load_oracle
+    oracle = load_oracle()
+    threads = threads or host_threads()
+    inp, r0 = np.ascontiguousarray(fp.cur[0]), np.ascontiguousarray(fp.ref[0])
+    p1h = np.ascontiguousarray(pred_h)
+    W, blocks = fp.width, fp.blocks[:, :4].tolist()
+    P = lambda a, o: ctypes.c_void_p(a.ctypes.data + 2 * o)
+    chunk, n = 512, len(blocks)
+
+    def work(i0):
+        ok = True
+        for i in range(i0, min(n, i0 + chunk)):
+            x, y, w, h = blocks[i]
+            o = y * W + x
+            ok &= oracle.or_ComputeAvgSatd(P(inp, o), W, P(r0, o), P(p1h, o), W, w, h) == int(got[i])
+        return min(n, i0 + chunk) - i0, ok
+
+    done, exact, pos = 0, True, 0
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < seconds:
+            starts = [((pos + k) * chunk) % n for k in range(threads)]
+            pos += threads
+            for cnt, ok in ex.map(work, starts):
+                done += cnt
+                exact &= ok
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt, 1), "unit": "blocks/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C) on {done} blocks in chunks of {chunk}, {threads} threads, {dt:.1f} s",
+            "bit_exact_vs_gpu_on_sample": bool(exact)}
 
 
 PMC_DMVR = os.path.join(ROOT, "profiles", "k_dmvr_1080p.md")
@@ -660,11 +785,12 @@ def dmvr_leg(lib, fp, cur, ref, steps, warmup, cpu_seconds):
                         "unit": "GB/s", "frac": round(alg_bytes / k_s / 1e9 / HBM_PEAK_GBS, 5),
                         "traffic": traffic, "traffic_source": src, "alg_bytes_per_launch": alg_bytes}}
     if cpu_seconds > 0:
-        out["cpu_baseline"] = dmvr_cpu(fp, arr, b, sub, got, cpu_seconds)
+        out["cpu_baseline"] = one_and_all_cores(
+            lambda sec, t: dmvr_cpu(fp, arr, b, sub, got, sec, threads=t), cpu_seconds)
     return out
 
 
-def dmvr_cpu(fp, arr, b, sub, gpu_mvd, seconds):
+def dmvr_cpu(fp, arr, b, sub, gpu_mvd, seconds, threads=None):
     """The oracle's refinement on the host, chunks of the same blocks; the
     blocks it computes are also compared with the GPU's deltas."""
     from tests.oracle_lib import OrDmvr, load_oracle
@@ -675,7 +801,7 @@ def dmvr_cpu(fp, arr, b, sub, gpu_mvd, seconds):
     st = r0.shape[1]
     base0, base1 = r0.ctypes.data + 2 * (pad * st + pad), r1.ctypes.data + 2 * (pad * st + pad)
     out = np.zeros(2 * int(sub[-1]), np.int16)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or host_threads()
     chunk, n = 500, len(b)
 
     def work(i0):
@@ -741,25 +867,42 @@ def search_leg(lib, steps, warmup, cpu_seconds):
                             "note": "390 workgroups on 256 CUs: one wave of workgroups, so the frame is "
                                     "latency-bound (window staging, then 297 items of 16 rows each)"}}
     if cpu_seconds > 0:
-        from tests.oracle_lib import load_oracle
-        oracle = load_oracle()
-        refp = np.ascontiguousarray(np.pad(ref, 80, mode="edge"))
-        rs = W + 160
-        emx, emy = np.zeros(nb, np.int16), np.zeros(nb, np.int16)
-        ec, es = np.zeros(nb, np.uint32), np.zeros(nb, np.uint32)
-        P = lambda a, off=0: ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
-        frames, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < cpu_seconds:
-            oracle.or_FullSearchFrame(P(cur), W, P(refp, 80 * rs + 80), rs, W, H, 16, 16, 16, 1024, 64, 0, nb,
-                                      P(emx), P(emy), P(ec), P(es))
-            frames += 1
-        dt = time.perf_counter() - t0
-        exact = bool(np.array_equal(gmv.reshape(nb, 2)[:, 0], emx) and np.array_equal(gmv.reshape(nb, 2)[:, 1], emy))
-        out["cpu_baseline"] = {"value": round(W * H * frames / dt / 1e6, 3), "unit": "Mpixel/s", "cores": 1,
-                               "kind": "port", "sample": f"oracle (scalar C, -O3) full frames, 1 thread, {frames} "
-                                                         f"frames in {dt:.1f} s",
-                               "bit_exact_vs_gpu": exact}
+        out["cpu_baseline"] = one_and_all_cores(lambda sec, t: search_cpu(cur, ref, gmv, sec, threads=t),
+                                                cpu_seconds)
     return out
+
+
+def search_cpu(cur, ref, gmv, seconds, threads=None):
+    """or_FullSearchFrame on whole frames, the frame's blocks split into one
+    range per host thread; every frame is compared with the GPU's MVs."""
+    from tests.oracle_lib import load_oracle
+    oracle = load_oracle()
+    threads = threads or host_threads()
+    H, W = cur.shape
+    nb = (W // 16) * (H // 16)
+    refp = np.ascontiguousarray(np.pad(ref, 80, mode="edge"))
+    rs = W + 160
+    emx, emy = np.zeros(nb, np.int16), np.zeros(nb, np.int16)
+    ec, es = np.zeros(nb, np.uint32), np.zeros(nb, np.uint32)
+    P = lambda a, off=0: ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
+    cuts = [nb * k // threads for k in range(threads + 1)]
+
+    def work(k):
+        oracle.or_FullSearchFrame(P(cur), W, P(refp, 80 * rs + 80), rs, W, H, 16, 16, 16, 1024, 64, cuts[k],
+                                  cuts[k + 1], P(emx), P(emy), P(ec), P(es))
+
+    frames, exact = 0, True
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < seconds:
+            list(ex.map(work, range(threads)))
+            frames += 1
+            exact &= bool(np.array_equal(gmv.reshape(nb, 2)[:, 0], emx) and
+                          np.array_equal(gmv.reshape(nb, 2)[:, 1], emy))
+    dt = time.perf_counter() - t0
+    return {"value": round(W * H * frames / dt / 1e6, 3), "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C, -O3) full frames, the blocks split over {threads} threads, {frames} "
+                      f"frames in {dt:.1f} s", "bit_exact_vs_gpu": exact}
 
 
 SAD_PAIR_PEAK = 256 * 64 * 2.4e9 * 2 / 1e12   # abs-diffs/s: one v_sad_u16 (two samples) per lane per cycle
@@ -832,12 +975,13 @@ def pipeline_leg(lib, steps, warmup, cpu_seconds):
            "median_quarter_mv_ref0": [int(np.median(q.reshape(-1, 2, 2)[:, 0, 0])),
                                       int(np.median(q.reshape(-1, 2, 2)[:, 0, 1]))]}
     if cpu_seconds > 0:
-        out["cpu_baseline"] = pipeline_cpu(cur, ref0, ref1, gpu, cpu_seconds)
+        out["cpu_baseline"] = one_and_all_cores(
+            lambda sec, t: pipeline_cpu(cur, ref0, ref1, gpu, sec, threads=t), cpu_seconds)
         out["bit_exact_on_cpu_sample"] = out["cpu_baseline"]["bit_exact_vs_gpu_on_sample"]
     return out
 
 
-def pipeline_cpu(cur, ref0, ref1, gpu, seconds):
+def pipeline_cpu(cur, ref0, ref1, gpu, seconds, threads=None):
     """The oracle's pipeline on bands of whole 16x16 block rows of the same 4K
     frame (or_MePipelineSample: each band computes its rows' stages plus the
     stage-1 searches of the row above), one band per host thread, rounds until
@@ -846,7 +990,7 @@ def pipeline_cpu(cur, ref0, ref1, gpu, seconds):
     from tests.oracle_lib import load_oracle
     from tests.pipeline_common import oracle_pipeline_sample
     oracle = load_oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or host_threads()
     H, W = cur[0].shape
     nbx, nby = W // 16, H // 16
     k = 2                                       # block rows per band (and per thread)
@@ -1075,7 +1219,12 @@ def main():
     cpu = None
     if not args.no_cpu and not args.quick:
         log("CPU oracle baseline")
-        cpu = cpu_baseline(fp, args.cpu_seconds, offs, wl.LAMBDA_REFINE)
+        cpu = one_and_all_cores(lambda sec, t: cpu_baseline(fp, sec, offs, wl.LAMBDA_REFINE, threads=t),
+                                args.cpu_seconds)
+        if os.path.exists(REF_LIB):
+            log("cpu baseline: the reference's own interpolation")
+            cpu["reference_interp"] = one_and_all_cores(lambda sec, t: ref_interp_cpu(fp, offs, sec, threads=t),
+                                                        args.cpu_seconds / 2)
 
     line = {
         "metric": METRIC,
