@@ -677,6 +677,44 @@ int32_t xg_lookahead_me(xg_session *s, const xg_plane *cur, const xg_plane *ref,
 int32_t xg_lookahead_me_static(void *stream, const xg_plane *cur, const xg_plane *ref, int32_t unitSize,
                                int32_t range, uint32_t sadLambda, int16_t *laMv, uint32_t *cost, uint32_t *sad);
 
+/* Motion estimation of the motion-compensated temporal filter: the role of
+ * the closed Xin26xMctfMotionEstFrame (h26x_core/include/h26x_mctf.h:27, per
+ * reference, search level and block size) and Xin26xMctfSubSample (:35),
+ * filling MV fields as l0Mv / l1Mv / l2Mv / mv of h26x_mctf_struct.h:117-127.
+ * A design of this library, hierarchical as the published temporal filter of
+ * VVC's reference software: 2x2-mean subsampling; per level and block, the
+ * 3x3 lower-level MVs around (bx / 2, by / 2) scaled by the resolution factor
+ * and the zero MV, then the integer positions +/-range around the best, then
+ * at the finest level 7x7 grids of quarter-pel and 1/16-pel steps around the
+ * best, the first strictly lowest SSD winning each comparison (rows before
+ * columns); fractional predictions by the standard's 8-tap luma filter.  A
+ * position whose 8-tap reach [pos - 3, pos + blockSize + 4] leaves the
+ * reference's padding is not costed.  Planes are device planes; all work is
+ * enqueued on the session's stream. */
+
+/* dst (src / 2 in both sides, dst->pad samples of edge replication filled
+ * too) = rounded 2x2 means (a + b + c + d + 2) >> 2 of src. */
+int32_t xg_mctf_subsample(xg_session *s, const xg_plane *src, const xg_plane *dst);
+
+/* One level over every blockSize x blockSize block (8 or 16) wholly inside
+ * cur: range 0..8; lower (device int32 (x, y) pairs, lowerBw x lowerBh blocks,
+ * 1/16 of the lower level's pel) may be NULL (no candidates: the search is
+ * centred on zero); factor scales the lower MVs to this level; frac adds the
+ * fractional rounds.  Outputs (device, raster order of the block grid): mv
+ * as int32 (x, y) pairs in 1/16 of this level's pel, err the SSD. */
+int32_t xg_mctf_me_level(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t blockSize, int32_t range,
+                         const int32_t *lower, int32_t lowerBw, int32_t lowerBh, int32_t factor, int32_t frac,
+                         int32_t *mv, uint32_t *err);
+
+/* The whole search of one frame pair (sides >= 64): the 4x level (16x16
+ * blocks, +/-8), the 2x level (16x16, +/-5, lower MVs x 2), full resolution
+ * (16x16, +/-5, x 2) and 8x8 blocks at full resolution (range 0, x 1,
+ * fractional rounds).  The subsampled pictures (padded by 32 and 16) and the
+ * coarser fields live in session buffers.  With the full-resolution reference
+ * padded by at least 56 no position the hierarchy can reach is skipped.
+ * Outputs: the 8x8 grid's mv (int32 pairs, 1/16 pel) and err (device). */
+int32_t xg_mctf_me(xg_session *s, const xg_plane *cur, const xg_plane *ref, int32_t *mv, uint32_t *err);
+
 /* AMVP predictor choice, batched: the static Xin266SelectMvp
  * (h266_encoder/source/h266_analyze_inter_cu.c:60-131, called at :1003) for
  * many blocks at once, with the rounding Xin266FillAmvpCand ends with

@@ -1342,6 +1342,92 @@ int32_t or_SelectMvp(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intp
     return idx;
 }
 
+/* ---- MCTF motion estimation ----------------------------------------------
+ * The reference's MCTF motion search (Xin26xMctfMotionEstFrame,
+ * h26x_core/include/h26x_mctf.h:27, per level and block size; its subsampling
+ * Xin26xMctfSubSample, :35) is closed.  This is the library's own design in
+ * that role, hierarchical as the published temporal filter of VVC's
+ * reference software is: 2x2-mean subsampling, candidates from the lower
+ * level, an integer search, fractional rounds at the finest level; the
+ * fractional prediction is the standard's 8-tap luma filter. */
+void or_MctfSubsample(const or_pixel *src, intptr_t ss, int32_t w, int32_t h, or_pixel *dst, intptr_t ds,
+                      int32_t pad)
+{
+    const int32_t hw = w / 2, hh = h / 2;
+    for (int32_t y = -pad; y < hh + pad; y++)
+        for (int32_t x = -pad; x < hw + pad; x++) {
+            const int32_t cx = clip3(x, 0, hw - 1), cy = clip3(y, 0, hh - 1);
+            const or_pixel *p = src + (intptr_t)(2 * cy) * ss + 2 * cx;
+            dst[(intptr_t)y * ds + x] = (or_pixel)((p[0] + p[1] + p[ss] + p[ss + 1] + 2) >> 2);
+        }
+}
+
+/* SSD of the bs x bs block at (x, y) against the reference at MV (mx, my)
+ * (1/16 pel); UINT32_MAX when the 8-tap reach leaves the padding. */
+static uint32_t mctf_ssd(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t w, int32_t h,
+                         int32_t pad, int32_t bs, int32_t x, int32_t y, int32_t mx, int32_t my, or_pixel *buf)
+{
+    const int32_t px = x + (mx >> 4), py = y + (my >> 4);
+    if (px - 3 < -pad || py - 3 < -pad || px + bs + 4 > w - 1 + pad || py + bs + 4 > h - 1 + pad) return UINT32_MAX;
+    or_LumaInterpolation(ref + (intptr_t)py * rs + px, rs, buf, bs, ((my & 15) << 4) + (mx & 15), (uint32_t)bs,
+                         (uint32_t)bs, 0);
+    uint32_t e = 0;
+    for (int32_t r = 0; r < bs; r++)
+        for (int32_t c = 0; c < bs; c++) {
+            const int32_t d = (int32_t)cur[(intptr_t)(y + r) * cs + x + c] - (int32_t)buf[r * bs + c];
+            e += (uint32_t)(d * d);
+        }
+    return e;
+}
+
+void or_MctfMeLevel(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t w, int32_t h,
+                    int32_t pad, int32_t bs, int32_t range, const int32_t *lower, int32_t lowerBw, int32_t lowerBh,
+                    int32_t factor, int32_t frac, int32_t *mv, uint32_t *err)
+{
+    const int32_t nbx = w / bs, nby = h / bs;
+    or_pixel *buf = (or_pixel *)malloc(sizeof(or_pixel) * (size_t)bs * (size_t)bs);
+    for (int32_t by = 0; by < nby; by++)
+        for (int32_t bx = 0; bx < nbx; bx++) {
+            const int32_t x = bx * bs, y = by * bs;
+            uint32_t best = UINT32_MAX;
+            int32_t bmx = 0, bmy = 0;
+            if (lower) {
+                for (int32_t py = -1; py <= 1; py++)
+                    for (int32_t px = -1; px <= 1; px++) {
+                        const int32_t tx = bx / 2 + px, ty = by / 2 + py;
+                        if (tx < 0 || ty < 0 || tx >= lowerBw || ty >= lowerBh) continue;
+                        const int32_t cx = lower[2 * ((intptr_t)ty * lowerBw + tx)] * factor;
+                        const int32_t cy = lower[2 * ((intptr_t)ty * lowerBw + tx) + 1] * factor;
+                        const uint32_t e = mctf_ssd(cur, cs, ref, rs, w, h, pad, bs, x, y, cx, cy, buf);
+                        if (e < best) { best = e; bmx = cx; bmy = cy; }
+                    }
+                const uint32_t e = mctf_ssd(cur, cs, ref, rs, w, h, pad, bs, x, y, 0, 0, buf);
+                if (e < best) { best = e; bmx = 0; bmy = 0; }
+            }
+            const int32_t ix = bmx / 16, iy = bmy / 16;
+            for (int32_t y2 = iy - range; y2 <= iy + range; y2++)
+                for (int32_t x2 = ix - range; x2 <= ix + range; x2++) {
+                    const uint32_t e = mctf_ssd(cur, cs, ref, rs, w, h, pad, bs, x, y, 16 * x2, 16 * y2, buf);
+                    if (e < best) { best = e; bmx = 16 * x2; bmy = 16 * y2; }
+                }
+            if (frac) {
+                for (int32_t step = 4; step >= 1; step -= 3) {
+                    const int32_t ox = bmx, oy = bmy;
+                    for (int32_t y2 = -3 * step; y2 <= 3 * step; y2 += step)
+                        for (int32_t x2 = -3 * step; x2 <= 3 * step; x2 += step) {
+                            const uint32_t e = mctf_ssd(cur, cs, ref, rs, w, h, pad, bs, x, y, ox + x2, oy + y2, buf);
+                            if (e < best) { best = e; bmx = ox + x2; bmy = oy + y2; }
+                        }
+                }
+            }
+            const intptr_t b = (intptr_t)by * nbx + bx;
+            mv[2 * b] = bmx;
+            mv[2 * b + 1] = bmy;
+            err[b] = best;
+        }
+    free(buf);
+}
+
 void or_FullSearchFrame(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs,
                         int32_t fw, int32_t fh, int32_t bw, int32_t bh, int32_t range,
                         uint32_t sadLambda, int32_t ctuSize, int32_t blk0, int32_t blk1,

@@ -291,6 +291,27 @@ int32_t or_SelectMvp(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intp
                      int32_t x, int32_t y, int32_t w, int32_t h, int32_t ncand, const int32_t *cand,
                      int32_t imvIdx, int32_t ctuSize, uint32_t *sad, int32_t mv[2], int16_t meMv[2]);
 
+/* ---- MCTF motion estimation (design of the library; see or_MctfMeLevel) -- */
+/* dst (w/2 x h/2 plus `pad` samples of edge replication on every side) =
+ * rounded 2x2 means (a + b + c + d + 2) >> 2 of src (w x h); dst points at
+ * its sample (0, 0). */
+void or_MctfSubsample(const or_pixel *src, intptr_t ss, int32_t w, int32_t h, or_pixel *dst, intptr_t ds,
+                      int32_t pad);
+/* One level of the MCTF motion estimation: for every bs x bs block wholly
+ * inside the w x h picture (raster grid), the MV (1/16 of this level's pel)
+ * and SSD of the best of: the MVs of the 3x3 lower-level blocks around
+ * (bx / 2, by / 2) (lowerBw x lowerBh grid, times `factor`; lower NULL: none)
+ * and zero, scanned rows then columns, then the integer positions +/-range
+ * around that best (rows then columns), then, when frac is set, a 7x7 grid
+ * of quarter-pel steps and a 7x7 grid of 1/16-pel steps around the best
+ * (the standard's 8-tap luma prediction); the first strictly lowest SSD wins
+ * each comparison.  A position whose 8-tap reach [pos - 3, pos + bs + 4]
+ * leaves the reference's `pad` samples of padding is not evaluated.  mv and
+ * err: nbx * nby entries (x, y pairs for mv). */
+void or_MctfMeLevel(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t w, int32_t h,
+                    int32_t pad, int32_t bs, int32_t range, const int32_t *lower, int32_t lowerBw, int32_t lowerBh,
+                    int32_t factor, int32_t frac, int32_t *mv, uint32_t *err);
+
 /* ---- quantisation (h266_encoder/source/h266_quant_inv_quant.c:27-143) ---- */
 void or_QuantInvQuant(int32_t *q, const int32_t *t, int32_t *r, intptr_t stride, uint32_t w, uint32_t h,
                       uint32_t cgw, uint32_t cgh, int32_t qMult, int32_t qAdd, int32_t qShift, int32_t iqMult,

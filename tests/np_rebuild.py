@@ -430,3 +430,69 @@ def select_mvp(cur, ref, pad, x, y, w, h, cands, imv, ctu):
             best = (i, s, (rx, ry))
     i, s, (rx, ry) = best
     return i, s, (rx, ry), (change_prec(rx, 6, 4), change_prec(ry, 6, 4))
+
+
+# ------------------------------------------------------------------- MCTF ---
+def mctf_subsample(src, pad):
+    """Rounded 2x2 means of src (h/2 x w/2), edge-padded by `pad` samples."""
+    h, w = src.shape
+    s = src[:h // 2 * 2, :w // 2 * 2].astype(np.int64)
+    m = (s[0::2, 0::2] + s[0::2, 1::2] + s[1::2, 0::2] + s[1::2, 1::2] + 2) >> 2
+    return np.pad(m, pad, mode="edge").astype(np.uint16)
+
+
+def _mctf_ssd(cur, refp, pad, bs, x, y, mx, my):
+    """SSD of the block at (x, y) against the 8-tap prediction at (mx, my) in
+    1/16 pel, or None when the 8-tap reach leaves the padding."""
+    h, w = cur.shape
+    px, py = x + (mx >> 4), y + (my >> 4)
+    if px - 3 < -pad or py - 3 < -pad or px + bs + 4 > w - 1 + pad or py + bs + 4 > h - 1 + pad:
+        return None
+    p = interp(refp, px + pad, py + pad, bs, bs, ((my & 15) << 4) | (mx & 15), False)
+    d = cur[y:y + bs, x:x + bs].astype(np.int64) - p
+    return int((d * d).sum())
+
+
+def mctf_level(cur, refp, pad, bs, rng, lower, factor, frac):
+    """The library's MCTF search at one level, restated: per block, the 3x3
+    lower-level MVs around (bx / 2, by / 2) times factor and zero, then the
+    integer positions +/-rng around the best, then (frac) 7x7 grids at steps
+    of 4 and 1 (1/16 pel) around the best; first strictly lowest SSD.
+    Returns (mv [nby, nbx, 2], err [nby, nbx])."""
+    h, w = cur.shape
+    nbx, nby = w // bs, h // bs
+    mv = np.zeros((nby, nbx, 2), np.int64)
+    err = np.zeros((nby, nbx), np.int64)
+
+    for by in range(nby):
+        for bx in range(nbx):
+            x, y = bx * bs, by * bs
+            best, bm = None, (0, 0)
+
+            def test(mx, my):
+                nonlocal best, bm
+                e = _mctf_ssd(cur, refp, pad, bs, x, y, mx, my)
+                if e is not None and (best is None or e < best):
+                    best, bm = e, (mx, my)
+
+            if lower is not None:
+                lh, lw = lower.shape[:2]
+                for py in (-1, 0, 1):
+                    for px in (-1, 0, 1):
+                        tx, ty = bx // 2 + px, by // 2 + py
+                        if 0 <= tx < lw and 0 <= ty < lh:
+                            test(int(lower[ty, tx, 0]) * factor, int(lower[ty, tx, 1]) * factor)
+                test(0, 0)
+            cx, cy = int(bm[0] / 16), int(bm[1] / 16)          # C division: toward zero
+            for y2 in range(cy - rng, cy + rng + 1):
+                for x2 in range(cx - rng, cx + rng + 1):
+                    test(16 * x2, 16 * y2)
+            if frac:
+                for step in (4, 1):
+                    ox, oy = bm
+                    for y2 in range(-3 * step, 3 * step + 1, step):
+                        for x2 in range(-3 * step, 3 * step + 1, step):
+                            test(ox + x2, oy + y2)
+            mv[by, bx] = bm
+            err[by, bx] = 0xFFFFFFFF if best is None else best
+    return mv, err

@@ -66,6 +66,7 @@ def _dev(torch, arr):
 def _select(lib, s, torch, amvp, imv):
     d_in = _dev(torch, amvp)
     d_out = torch.zeros(len(amvp) * RES_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()               # torch's fills and copies are done before the session stream reads
     assert lib.xg_amvp_select(s, ctypes.c_void_p(d_in.data_ptr()), len(amvp), imv, 64,
                               ctypes.c_void_p(d_out.data_ptr())) == 0
     assert lib.xg_session_sync(s) == 0
@@ -86,6 +87,7 @@ def test_amvp_lists_from_field(oracle, gpu):
     barr = blocks_to_struct([(x, y, w, h, 0, 0, 0, 0) for x, y, w, h in blocks])
     d_b, d_f = _dev(torch, barr), _dev(torch, field)
     d_out = torch.zeros(len(blocks) * AMVP_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
     assert lib.xg_amvp_from_field(s, ctypes.c_void_p(d_f.data_ptr()), W // 4, W, H, ctypes.c_void_p(d_b.data_ptr()),
                                   len(blocks), ctypes.c_void_p(d_out.data_ptr())) == 0
     assert lib.xg_session_sync(s) == 0

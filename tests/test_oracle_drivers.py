@@ -483,6 +483,119 @@ def test_amvp_rebuild(oracle):
     check_amvp(oracle)
 
 
+# ------------------------------------------------------------------ MCTF ---
+MSLACK = 8          # allocated past the declared padding: planted mistakes that reach beyond it stay in memory
+
+
+def mctf_scene(seed=3, W=160, H=96):
+    """A textured reference, and a current picture that is the reference moved
+    by (6, -3) with a still region and a fractional-shifted one, plus noise,
+    and a flat area in both (ties between positions)."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H + 16, 0:W + 16]
+    tex = 512 + 240 * np.sin(xx / 5.0 + yy / 11.0) + 130 * np.cos(yy / 4.0 - xx / 13.0) + rng.normal(0, 30, xx.shape)
+    ref = np.clip(tex[8:8 + H, 8:8 + W], 0, 1023)
+    cur = np.clip(tex[8 + 3:8 + 3 + H, 8 - 6:8 - 6 + W], 0, 1023)
+    cur[40:72, 96:144] = ref[40:72, 96:144]                     # still region
+    sub = 0.5 * (tex[8 + 1:8 + 1 + H, 8 + 2:8 + 2 + W] + tex[8 + 1:8 + 1 + H, 8 + 3:8 + 3 + W])
+    cur[8:40, 8:56] = np.clip(sub, 0, 1023)[8:40, 8:56]          # half-pel motion
+    cur = np.clip(cur + rng.normal(0, 4, cur.shape), 0, 1023)
+    ref[32:80, 88:152] = 300                                     # flat: equal SSDs, so the tie rule decides
+    cur[40:72, 96:144] = 300
+    cur[64:96, 0:64] = ref[64:96, 0:64]                          # still and textured: the zero MV wins
+    return np.ascontiguousarray(cur.astype(np.uint16)), np.ascontiguousarray(ref.astype(np.uint16))
+
+
+def oracle_mctf_levels(lib, cur, ref, pads=(12, 24, 56)):
+    """The oracle's hierarchy (the frame search's four levels) with the given
+    reference paddings at 4x, 2x and full resolution; returns each level's
+    inputs and (mv [nby, nbx, 2], err) as numpy arrays."""
+    H, W = cur.shape
+
+    def sub(src, pad):
+        h, w = src.shape
+        ps = pad + MSLACK
+        out = np.zeros((h // 2 + 2 * ps, w // 2 + 2 * ps), np.uint16)
+        lib.or_MctfSubsample(ptr(src), w, w, h, ptr(out, ps * out.shape[1] + ps), out.shape[1], ps)
+        return out, out[ps:ps + h // 2, ps:ps + w // 2].copy()
+
+    def level(c, rp, pad, bs, rng, lower, factor, frac):
+        h, w = c.shape
+        ps = (rp.shape[0] - h) // 2
+        nbx, nby = w // bs, h // bs
+        mv, err = np.zeros(nbx * nby * 2, np.int32), np.zeros(nbx * nby, np.uint32)
+        low = None if lower is None else np.ascontiguousarray(lower.astype(np.int32))
+        lib.or_MctfMeLevel(ptr(c), w, ptr(rp, ps * rp.shape[1] + ps), rp.shape[1], w, h, pad, bs, rng,
+                           None if low is None else ptr(low), 0 if low is None else low.shape[1],
+                           0 if low is None else low.shape[0], factor, frac, ptr(mv), ptr(err))
+        return mv.reshape(nby, nbx, 2), err.reshape(nby, nbx)
+
+    r1 = np.ascontiguousarray(np.pad(ref, pads[2] + MSLACK, mode="edge"))
+    r2full, _ = sub(ref, pads[1])
+    _, c2 = sub(cur, 0)
+    r2 = r2full
+    r4full, _ = sub(r2[pads[1] + MSLACK:pads[1] + MSLACK + H // 2, pads[1] + MSLACK:pads[1] + MSLACK + W // 2].copy(),
+                    pads[0])
+    _, c4 = sub(c2, 0)
+    out = {}
+    out["l4"] = (c4, r4full, pads[0], 16, 8, None, 1, 0)
+    out["l4"] += level(*out["l4"])
+    out["l2"] = (c2, r2, pads[1], 16, 5, out["l4"][8], 2, 0)
+    out["l2"] += level(*out["l2"])
+    out["l1"] = (cur, r1, pads[2], 16, 5, out["l2"][8], 2, 0)
+    out["l1"] += level(*out["l1"])
+    out["l0"] = (cur, r1, pads[2], 8, 0, out["l1"][8], 1, 1)
+    out["l0"] += level(*out["l0"])
+    return out
+
+
+def check_mctf(lib):
+    cur, ref = mctf_scene()
+    H, W = cur.shape
+    # subsampling against the numpy means
+    ps = 24 + MSLACK
+    out = np.zeros((H // 2 + 2 * ps, W // 2 + 2 * ps), np.uint16)
+    lib.or_MctfSubsample(ptr(ref), W, W, H, ptr(out, ps * out.shape[1] + ps), out.shape[1], ps)
+    assert np.array_equal(out, nr.mctf_subsample(ref, ps))
+    lv = oracle_mctf_levels(lib, cur, ref)
+    n_frac = n_zero_pick = n_skipped = 0
+    for name in ("l4", "l2", "l1", "l0"):
+        c, rp, pad, bs, rng, lower, factor, frac, mv, err = lv[name]
+        ps = (rp.shape[0] - c.shape[0]) // 2
+        emv, eerr = nr.mctf_level(c, rp[ps - pad:rp.shape[0] - ps + pad, ps - pad:rp.shape[1] - ps + pad], pad, bs,
+                                  rng, lower, factor, frac)
+        assert np.array_equal(mv, emv), name
+        assert np.array_equal(err, eerr), name
+        if frac:
+            n_frac += int(np.count_nonzero((mv & 15) != 0))
+        n_zero_pick += int(np.count_nonzero((mv == 0).all(axis=2)))
+    assert n_frac > 20 and n_zero_pick > 5
+    # the padding rule: texture (not replication) in the padding, the current
+    # picture moved 6 samples right, so the left blocks' true match reads 9
+    # samples into a padding of 8 and must be skipped
+    rng = np.random.default_rng(8)
+    pad, w, h = 8, 64, 32
+    buf = rng.integers(0, 1024, (h + 2 * (pad + MSLACK), w + 2 * (pad + MSLACK))).astype(np.uint16)
+    o = pad + MSLACK
+    c = np.ascontiguousarray(buf[o:o + h, o - 6:o - 6 + w])
+    mv, err = np.zeros(2 * (w // 16) * (h // 16), np.int32), np.zeros((w // 16) * (h // 16), np.uint32)
+    lib.or_MctfMeLevel(ptr(c), w, ptr(buf, o * buf.shape[1] + o), buf.shape[1], w, h, pad, 16, 8, None, 0, 0, 1, 0,
+                       ptr(mv), ptr(err))
+    view = buf[MSLACK:buf.shape[0] - MSLACK, MSLACK:buf.shape[1] - MSLACK]
+    emv, eerr = nr.mctf_level(c, view, pad, 16, 8, None, 1, 0)
+    assert np.array_equal(mv.reshape(emv.shape), emv) and np.array_equal(err.reshape(eerr.shape), eerr)
+    free, _ = nr.mctf_level(c, buf, pad + MSLACK, 16, 8, None, 1, 0)
+    assert tuple(free[0, 0]) == (-96, 0) and tuple(emv[0, 0]) != (-96, 0)      # the rule decided it
+
+
+def test_mctf_rebuild(oracle):
+    """or_MctfSubsample and the four levels of or_MctfMeLevel against the numpy
+    restatements: candidates from the lower level, the integer search, the
+    quarter- and 1/16-pel rounds with the 8-tap filter, positions outside the
+    padding skipped."""
+    check_mctf(oracle)
+
+
 MUTANTS = {
     # or_RefineFracBlocks
     "refine: chroma fraction nibbles swapped": ("int32_t fracC = ((ry & 31) << 5) + (rx & 31);",
@@ -542,12 +655,28 @@ MUTANTS = {
     "amvp: no AMVR rounding": ("rounded[c][k] = or_ChangeMvPrec(or_ChangeMvPrec(cand[2 * c + k], 6, prec), prec, 6);",
                                "rounded[c][k] = cand[2 * c + k];", "amvp"),
     "amvp: default filter at half-pel AMVR": ("imvIdx == 3 ? 1u : 0u);", "0u);", "amvp"),
+    # or_MctfMeLevel
+    "mctf: <= in the integer search": ("if (e < best) { best = e; bmx = 16 * x2; bmy = 16 * y2; }",
+                                       "if (e <= best) { best = e; bmx = 16 * x2; bmy = 16 * y2; }", "mctf"),
+    "mctf: lower neighbours around (bx + 1) / 2": ("const int32_t tx = bx / 2 + px, ty = by / 2 + py;",
+                                                   "const int32_t tx = (bx + 1) / 2 + px, ty = (by + 1) / 2 + py;",
+                                                   "mctf"),
+    "mctf: lower MVs not scaled": ("const int32_t cx = lower[2 * ((intptr_t)ty * lowerBw + tx)] * factor;",
+                                   "const int32_t cx = lower[2 * ((intptr_t)ty * lowerBw + tx)];", "mctf"),
+    "mctf: second fractional round at half steps": ("for (int32_t step = 4; step >= 1; step -= 3) {",
+                                                    "for (int32_t step = 4; step >= 1; step -= 2) {", "mctf"),
+    "mctf: no zero candidate": ("const uint32_t e = mctf_ssd(cur, cs, ref, rs, w, h, pad, bs, x, y, 0, 0, buf);\n"
+                                "                if (e < best) { best = e; bmx = 0; bmy = 0; }",
+                                "", "mctf"),
+    "mctf: padding reach without the filter taps": ("if (px - 3 < -pad || py - 3 < -pad ||",
+                                                    "if (px < -pad || py < -pad ||", "mctf"),
     "amvp: prediction MV not clipped to the CU window": (
         "const int32_t mx = clip3(rounded[c][0], mnx * 16, mxx * 16), my = clip3(rounded[c][1], mny * 16, mxy * 16);",
         "const int32_t mx = rounded[c][0], my = rounded[c][1];", "amvp"),
 }
 CHECKS = {"refine": lambda lib: check_refine(lib, nblocks=160), "subpel": check_subpel, "pipeline": check_pipeline,
-          "diamond": check_diamond, "cands": check_pipeline_cands, "search": check_search_blocks, "amvp": check_amvp}
+          "diamond": check_diamond, "cands": check_pipeline_cands, "search": check_search_blocks, "amvp": check_amvp,
+          "mctf": check_mctf}
 
 
 @pytest.fixture(scope="module")

@@ -217,6 +217,10 @@ PROTOS = {
     "xg_fdct_quant_run": (I32, [P, P, P, P, P]),
     "xg_lookahead_me": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, U32, P, P, P]),
     "xg_amvp_select": (I32, [P, P, I32, I32, I32, P]),
+    "xg_mctf_subsample": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane)]),
+    "xg_mctf_me_level": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, P, I32, I32, I32, I32,
+                               P, P]),
+    "xg_mctf_me": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), P, P]),
     "xg_amvp_from_field": (I32, [P, P, I32, I32, I32, P, I32, P]),
     "xg_lookahead_me_static": (I32, [P, ctypes.POINTER(XgPlane), ctypes.POINTER(XgPlane), I32, I32, U32, P, P, P]),
     "xg_bipred_plan": (I32, [P, P, I32]),

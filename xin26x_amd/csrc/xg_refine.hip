@@ -1300,6 +1300,7 @@ int32_t xg_session_destroy(xg_session *s)
     recon_release(s);
     dmvr_release(s);
     lookahead_release(s);
+    mctf_release(s);
     bipred_release(s);
     quant_release(s);
     pipe_release(s);

@@ -50,6 +50,8 @@ void recon_release(xg_session *s);
 void dmvr_release(xg_session *s);
 // Releases the half-resolution pictures of the lookahead search (xg_lookahead.hip).
 void lookahead_release(xg_session *s);
+// Releases the level pictures and fields of the MCTF search (xg_mctf.hip).
+void mctf_release(xg_session *s);
 // Releases the bi-prediction distortion plan a session owns (xg_bipred.hip).
 void bipred_release(xg_session *s);
 // Releases the quantiser plan a session owns (xg_quant.hip).
