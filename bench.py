@@ -572,6 +572,99 @@ def quant_cpu(desc, dq_desc, offs, coeff, got, seconds, threads=None):
             "bit_exact_vs_gpu_on_sample": bool(exact)}
 
 
+def mctf_work(W, H):
+    """Squared differences one xg_mctf_me frame costs at most (every candidate
+    and position inside its padding): 4x level +/-8 on 16x16 blocks, 2x and
+    full resolution 10 candidates + +/-5 on 16x16, 8x8 blocks with 10
+    candidates, the centre and 2 x 49 fractional positions."""
+    w2, h2, w4, h4 = W // 2, H // 2, W // 4, H // 4
+    return ((w4 // 16) * (h4 // 16) * 289 * 256 + (w2 // 16) * (h2 // 16) * 131 * 256 +
+            (W // 16) * (H // 16) * 131 * 256 + (W // 8) * (H // 8) * 109 * 64)
+
+
+def mctf_leg(lib, fp, cur, ref, steps, warmup, cpu_seconds):
+    """MCTF motion estimation of the 1080p pair: four levels (4x, 2x, full on
+    16x16 blocks, 8x8 blocks with fractional rounds)."""
+    import torch
+    from xin26x_amd import capi
+    W, H = fp.width, fp.height
+    nb = (W // 8) * (H // 8)
+    s = ctypes.c_void_p()
+    capi.check(lib.xg_session_create(ctypes.byref(s), 16), "session")
+    try:
+        mv = torch.empty(2 * nb, dtype=torch.int32, device="cuda")
+        err = torch.empty(nb, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        p0, p1 = cur.pic.plane[0], ref.pic.plane[0]
+        run = lambda: capi.check(lib.xg_mctf_me(s, ctypes.byref(p0), ctypes.byref(p1), mv.data_ptr(),
+                                                err.data_ptr()), "mctf")
+        k_ms, wall_ms = session_time(lib, s, 10, run, steps, warmup)
+        frame_ms, _ = session_time(lib, s, 9, run, steps, 0)
+        got = mv.cpu().numpy().reshape(H // 8, W // 8, 2)
+    finally:
+        lib.xg_session_destroy(s)
+    work = mctf_work(W, H)
+    out = {"workload": f"1080p pair, MCTF motion estimation: 4x level (16x16, +/-8), 2x and full resolution (16x16, "
+                       f"+/-5 around 3x3 lower candidates), {nb} 8x8 blocks with quarter- and 1/16-pel rounds "
+                       f"(8-tap), SSD",
+           "value": round(W * H / (wall_ms / 1e3) / 1e6, 1), "unit": "Mpixel/s", "ms_per_frame": round(wall_ms, 4),
+           "device_ms_per_frame": round(frame_ms, 4), "kernel": "k_mctf_level (finest level)",
+           "kernel_ms": round(k_ms, 4), "sq_diffs_per_frame_max": work,
+           "sq_diffs_per_s": round(work / (frame_ms / 1e3), 1)}
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = one_and_all_cores(lambda sec, t: mctf_cpu(fp, got, sec, threads=t), cpu_seconds)
+    return out
+
+
+def mctf_cpu(fp, got, seconds, threads=None):
+    """The oracle's MCTF hierarchy on whole frames of the same pair, every
+    level's block rows split over host threads, frames until about `seconds`
+    have passed (at least one); the finest MVs are compared with the GPU's."""
+    from tests.oracle_lib import load_oracle
+    oracle = load_oracle()
+    threads = threads or host_threads()
+    cur, ref = np.ascontiguousarray(fp.cur[0]), np.ascontiguousarray(fp.ref[0])
+    H, W = cur.shape
+    P = lambda a, o=0: ctypes.c_void_p(a.ctypes.data + o * a.itemsize)
+
+    def sub(src, pad):
+        h, w = src.shape
+        o = np.zeros((h // 2 + 2 * pad, w // 2 + 2 * pad), np.uint16)
+        oracle.or_MctfSubsample(P(src), w, w, h, P(o, pad * o.shape[1] + pad), o.shape[1], pad)
+        return o, np.ascontiguousarray(o[pad:pad + h // 2, pad:pad + w // 2])
+
+    def level(ex, c, rp, pad, bs, rng, lower, factor, frac):
+        h, w = c.shape
+        nbx, nby = w // bs, h // bs
+        mv, err = np.zeros(2 * nbx * nby, np.int32), np.zeros(nbx * nby, np.uint32)
+        cuts = [nby * k // threads for k in range(threads + 1)]
+        lb, lh = (0, 0) if lower is None else (lower.shape[1], lower.shape[0])
+        list(ex.map(lambda k: oracle.or_MctfMeLevel(P(c), w, P(rp, pad * rp.shape[1] + pad), rp.shape[1], w, h, pad,
+                                                    bs, rng, None if lower is None else P(lower), lb, lh, factor,
+                                                    frac, cuts[k], cuts[k + 1], P(mv), P(err)), range(threads)))
+        return mv.reshape(nby, nbx, 2)
+
+    r1 = np.ascontiguousarray(np.pad(ref, 80, mode="edge"))
+    frames, exact = 0, True
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        while frames == 0 or time.perf_counter() - t0 < seconds:
+            r2, r2i = sub(ref, 32)
+            r4, _ = sub(r2i, 16)
+            _, c2 = sub(cur, 0)
+            _, c4 = sub(c2, 0)
+            m4 = np.ascontiguousarray(level(ex, c4, r4, 16, 16, 8, None, 1, 0))
+            m2 = np.ascontiguousarray(level(ex, c2, r2, 32, 16, 5, m4, 2, 0))
+            m1 = np.ascontiguousarray(level(ex, cur, r1, 80, 16, 5, m2, 2, 0))
+            m0 = level(ex, cur, r1, 80, 8, 0, m1, 1, 1)
+            exact &= bool(np.array_equal(m0, got))
+            frames += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(W * H * frames / dt / 1e6, 3), "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C, -O3) MCTF hierarchy on {frames} whole frame(s), each level's block rows "
+                      f"split over {threads} threads, {dt:.1f} s", "bit_exact_vs_gpu": exact}
+
+
 def lookahead_leg(lib, fp, cur, ref, steps, warmup, cpu_seconds, unit=16, rng=32, lam=1024):
     """Lookahead motion estimation of the 1080p pair: both luma planes downscaled
     2x, every 16x16 unit of the 960x540 picture searched +/-32 (+/-64 at full
@@ -1205,12 +1298,14 @@ def main():
                    "note": "device mode: frames resident in HBM before the timed region, results to page-locked "
                            "host memory; pinned mode adds the host-to-device upload of every frame"}
 
-    dmvr = lookahead = bipred = None
+    dmvr = lookahead = bipred = mctf = None
     if not args.quick and not args.no_extra:
         log("DMVR: the 1080p bi-predicted blocks")
         dmvr = dmvr_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
         log("lookahead motion estimation at half resolution")
         lookahead = lookahead_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 5.0)
+        log("MCTF motion estimation of the 1080p pair")
+        mctf = mctf_leg(lib, fp, cur, ref, args.steps, args.warmup, 0 if args.no_cpu else 6.0)
         log("bi-prediction SATD of the 1080p blocks")
         bipred = bipred_leg(lib, fp, cur, ref, pred, args.steps, args.warmup, 0 if args.no_cpu else 3.0)
 
@@ -1265,6 +1360,7 @@ def main():
         "residual_loop": residual_loop,
         "dmvr_1080p": dmvr,
         "lookahead_1080p": lookahead,
+        "mctf_1080p": mctf,
         "bipred_1080p": bipred,
         "plain_search_416x240": search,
         "pipeline_4k": pipeline,

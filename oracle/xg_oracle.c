@@ -1382,11 +1382,11 @@ static uint32_t mctf_ssd(const or_pixel *cur, intptr_t cs, const or_pixel *ref, 
 
 void or_MctfMeLevel(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t w, int32_t h,
                     int32_t pad, int32_t bs, int32_t range, const int32_t *lower, int32_t lowerBw, int32_t lowerBh,
-                    int32_t factor, int32_t frac, int32_t *mv, uint32_t *err)
+                    int32_t factor, int32_t frac, int32_t by0, int32_t by1, int32_t *mv, uint32_t *err)
 {
     const int32_t nbx = w / bs, nby = h / bs;
     or_pixel *buf = (or_pixel *)malloc(sizeof(or_pixel) * (size_t)bs * (size_t)bs);
-    for (int32_t by = 0; by < nby; by++)
+    for (int32_t by = by0 < 0 ? 0 : by0; by < by1 && by < nby; by++)
         for (int32_t bx = 0; bx < nbx; bx++) {
             const int32_t x = bx * bs, y = by * bs;
             uint32_t best = UINT32_MAX;

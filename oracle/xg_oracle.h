@@ -306,11 +306,12 @@ void or_MctfSubsample(const or_pixel *src, intptr_t ss, int32_t w, int32_t h, or
  * of quarter-pel steps and a 7x7 grid of 1/16-pel steps around the best
  * (the standard's 8-tap luma prediction); the first strictly lowest SSD wins
  * each comparison.  A position whose 8-tap reach [pos - 3, pos + bs + 4]
- * leaves the reference's `pad` samples of padding is not evaluated.  mv and
- * err: nbx * nby entries (x, y pairs for mv). */
+ * leaves the reference's `pad` samples of padding is not evaluated.  Block
+ * rows [by0, by1) are computed; mv and err hold nbx * nby entries (x, y pairs
+ * for mv). */
 void or_MctfMeLevel(const or_pixel *cur, intptr_t cs, const or_pixel *ref, intptr_t rs, int32_t w, int32_t h,
                     int32_t pad, int32_t bs, int32_t range, const int32_t *lower, int32_t lowerBw, int32_t lowerBh,
-                    int32_t factor, int32_t frac, int32_t *mv, uint32_t *err);
+                    int32_t factor, int32_t frac, int32_t by0, int32_t by1, int32_t *mv, uint32_t *err);
 
 /* ---- quantisation (h266_encoder/source/h266_quant_inv_quant.c:27-143) ---- */
 void or_QuantInvQuant(int32_t *q, const int32_t *t, int32_t *r, intptr_t stride, uint32_t w, uint32_t h,

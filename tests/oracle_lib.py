@@ -70,7 +70,7 @@ def load_oracle(path=None):
         "or_MvRoundShift": (I32, [I32, I32]),
         "or_ChangeMvPrec": (I32, [I32, I32, I32]),
         "or_MctfSubsample": (None, [P, SS, I32, I32, P, SS, I32]),
-        "or_MctfMeLevel": (None, [P, SS, P, SS, I32, I32, I32, I32, I32, P, I32, I32, I32, I32, P, P]),
+        "or_MctfMeLevel": (None, [P, SS, P, SS, I32, I32, I32, I32, I32, P, I32, I32, I32, I32, I32, I32, P, P]),
         "or_AmvpFromField": (I32, [P, SS, I32, I32, I32, I32, I32, I32, P]),
         "or_SelectMvp": (I32, [P, SS, P, SS, I32, I32, I32, I32, I32, I32, I32, P, I32, I32, P, P, P]),
         "or_MotionEstFull": (None, [ctypes.POINTER(OrMe)]),

@@ -140,7 +140,7 @@ def test_mctf_padding_rule(oracle, env):
     c = np.ascontiguousarray(buf[o:o + h, o - 6:o - 6 + w])
     emv, eerr = np.zeros(2 * 8, np.int32), np.zeros(8, np.uint32)
     oracle.or_MctfMeLevel(ptr(c), w, ptr(buf, o * buf.shape[1] + o), buf.shape[1], w, h, pad, 16, 8, None, 0, 0, 1, 0,
-                          ptr(emv), ptr(eerr))
+                          0, h // 16, ptr(emv), ptr(eerr))
     gmv, gerr = _level(lib, s, torch, c, buf, pad, 16, 8, None, 1, 0)
     assert np.array_equal(gmv.reshape(-1), emv) and np.array_equal(gerr.reshape(-1), eerr)
     assert tuple(gmv[0, 0]) != (-96, 0)

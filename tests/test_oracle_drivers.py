@@ -527,7 +527,7 @@ def oracle_mctf_levels(lib, cur, ref, pads=(12, 24, 56)):
         low = None if lower is None else np.ascontiguousarray(lower.astype(np.int32))
         lib.or_MctfMeLevel(ptr(c), w, ptr(rp, ps * rp.shape[1] + ps), rp.shape[1], w, h, pad, bs, rng,
                            None if low is None else ptr(low), 0 if low is None else low.shape[1],
-                           0 if low is None else low.shape[0], factor, frac, ptr(mv), ptr(err))
+                           0 if low is None else low.shape[0], factor, frac, 0, nby, ptr(mv), ptr(err))
         return mv.reshape(nby, nbx, 2), err.reshape(nby, nbx)
 
     r1 = np.ascontiguousarray(np.pad(ref, pads[2] + MSLACK, mode="edge"))
@@ -580,7 +580,7 @@ def check_mctf(lib):
     c = np.ascontiguousarray(buf[o:o + h, o - 6:o - 6 + w])
     mv, err = np.zeros(2 * (w // 16) * (h // 16), np.int32), np.zeros((w // 16) * (h // 16), np.uint32)
     lib.or_MctfMeLevel(ptr(c), w, ptr(buf, o * buf.shape[1] + o), buf.shape[1], w, h, pad, 16, 8, None, 0, 0, 1, 0,
-                       ptr(mv), ptr(err))
+                       0, h // 16, ptr(mv), ptr(err))
     view = buf[MSLACK:buf.shape[0] - MSLACK, MSLACK:buf.shape[1] - MSLACK]
     emv, eerr = nr.mctf_level(c, view, pad, 16, 8, None, 1, 0)
     assert np.array_equal(mv.reshape(emv.shape), emv) and np.array_equal(err.reshape(eerr.shape), eerr)

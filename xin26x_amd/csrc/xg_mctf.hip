@@ -59,7 +59,6 @@ struct MctfLds {
     uint16_t win[(2 * kMctfMaxRange + kMctfMaxBs) * (2 * kMctfMaxRange + kMctfMaxBs)];  // 32 x 32
     int16_t tmp[(kMctfMaxBs + 7) * kMctfMaxBs];
     uint16_t blk[kMctfMaxBs * kMctfMaxBs];
-    unsigned long long key[1];
 };
 
 // Whether the 8-tap reach of a prediction at MV (mx, my) for the block at
@@ -406,6 +405,8 @@ int32_t xg_mctf_me(xg_session *s, const xg_plane *cur, const xg_plane *ref, int3
     int32_t *m4 = b->mv, *m2 = m4 + 2 * (size_t)g4x * g4y, *m1 = m2 + 2 * (size_t)g2x * g2y;
     uint32_t *e4 = b->err, *e2 = e4 + (size_t)g4x * g4y, *e1 = e2 + (size_t)g2x * g2y;
     int32_t rc = XG_OK;
+    int evFrame[2], evLast[2];
+    const bool tFrame = session_timer_begin(s, 9, evFrame);
     if (rc == XG_OK) rc = xg_mctf_subsample(s, cur, &C2);
     if (rc == XG_OK) rc = xg_mctf_subsample(s, &C2, &C4);
     if (rc == XG_OK) rc = xg_mctf_subsample(s, ref, &R2);
@@ -413,7 +414,10 @@ int32_t xg_mctf_me(xg_session *s, const xg_plane *cur, const xg_plane *ref, int3
     if (rc == XG_OK) rc = level_launch(v.stream, &C4, &R4, 16, 8, nullptr, 0, 0, 1, 0, m4, e4);
     if (rc == XG_OK) rc = level_launch(v.stream, &C2, &R2, 16, 5, m4, g4x, g4y, 2, 0, m2, e2);
     if (rc == XG_OK) rc = level_launch(v.stream, cur, ref, 16, 5, m2, g2x, g2y, 2, 0, m1, e1);
+    const bool tLast = session_timer_begin(s, 10, evLast);
     if (rc == XG_OK) rc = level_launch(v.stream, cur, ref, 8, 0, m1, g1x, g1y, 1, 1, mv, err);
+    if (tLast) session_timer_end(s, 10, evLast);
+    if (tFrame) session_timer_end(s, 9, evFrame);
     return rc;
 }
 

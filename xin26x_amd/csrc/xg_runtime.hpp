@@ -132,7 +132,7 @@ int32_t refine_multi_enqueue(xg_session *s, const xg_refine_cfg *cfg, const Refi
 // 0 refinement (eager), 1 graph replay, 2 forward transform, 3 integer search,
 // 4 a whole motion-search pipeline frame, 5 reconstruction, 6 DMVR,
 // 7 bi-prediction distortion, 8 quantisation.
-constexpr int kTimerStages = 9;
+constexpr int kTimerStages = 11;
 bool session_timer_begin(xg_session *s, int stage, int ev[2]);   // false when timing is off
 void session_timer_end(xg_session *s, int stage, const int ev[2]);
 
