@@ -12,9 +12,11 @@
 //     and, below width x height, ends the search early;
 //   * the best offset, if inside the 3x3 interior, gets the parametric
 //     sub-pel step (Xin266SubPelErrorSrfc, Xin266DivForMaxQ7).
-// One 64-thread workgroup per subblock: both lists' source regions (at most
+// One 64-thread workgroup per subblock (each XCD's workgroups take one
+// contiguous eighth of the subblocks): both lists' source regions (at most
 // 21 x 21 samples) and bilinear predictions (20 x 20) live in LDS, the 25
-// costs are summed row by row across the lanes, lane 0 runs the scan.
+// costs are summed row by row across the lanes and chosen by a wave minimum
+// over packed keys; lane 0 runs the sub-pel step.
 //
 // The per-call table slots (bilinear filters, S16 SAD) and the per-call
 // refinement stage host data and run the same device code.
@@ -99,7 +101,17 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
     __shared__ __align__(16) int16_t bil[2][20 * 20];
     __shared__ uint32_t cost[25];
     const int lane = threadIdx.x;
-    for (int si = blockIdx.x; si < nSub; si += gridDim.x) {
+    // XCD-aware split: workgroup g runs on XCD g % 8, so each XCD takes one
+    // contiguous eighth of the subblocks (neighbouring blocks, the same
+    // reference rows) and its L2 fetches those rows once
+    int first = blockIdx.x, end = nSub, step = gridDim.x;
+    if (gridDim.x >= 8 && gridDim.x % 8 == 0) {
+        const int xcd = (int)blockIdx.x % 8, per = (int)gridDim.x / 8;
+        first = (int)((long long)nSub * xcd / 8) + (int)blockIdx.x / 8;
+        end = (int)((long long)nSub * (xcd + 1) / 8);
+        step = per;
+    }
+    for (int si = first; si < end; si += step) {
         const int2 m = subs[si];
         const DmvrTask t = tasks[m.x];
         const int W = 1 << t.lgW, H = 1 << t.lgH;
@@ -117,28 +129,31 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
             fxy[l][1] = fy;
             const DmvrPic p = l ? p1 : p0;
             const uint16_t *s = p.data + (intptr_t)(iy + sy) * p.stride + (ix + sx);
-            for (int e = lane; e < rw * rh; e += 64) {
-                const int r = e / rw, c = e - r * rw;
-                // the last row and column are read only by the filters that need them
-                const bool need = (c < bw || fx) && (r < bh || fy);
-                src[l][r * 21 + c] = need ? s[(intptr_t)r * p.stride + c] : 0;
-            }
+            // lane = column (rw <= 21 < 32) + 32 * row parity: no divisions
+            const int c = lane & 31;
+            if (c < rw)
+                for (int r = lane >> 5; r < rh; r += 2) {
+                    // the last row and column are read only by the filters that need them
+                    const bool need = (c < bw || fx) && (r < bh || fy);
+                    src[l][r * 21 + c] = need ? s[(intptr_t)r * p.stride + c] : 0;
+                }
         }
         if (lane < 25) cost[lane] = 0;
         __syncthreads();
 #pragma unroll
         for (int l = 0; l < 2; l++) {
             const int fx = fxy[l][0], fy = fxy[l][1];
-            for (int e = lane; e < bw * bh; e += 64) {
-                const int r = e / bw, c = e - r * bw;
-                const uint16_t *q = &src[l][r * 21 + c];
-                int v;
-                if (fx && fy) v = bili2(bili2(q[0], q[1], fx), bili2(q[21], q[22], fx), fy);
-                else if (fx) v = bili2(q[0], q[1], fx);
-                else if (fy) v = bili2(q[0], q[21], fy);
-                else v = q[0];
-                bil[l][r * 20 + c] = (int16_t)v;
-            }
+            const int c = lane & 31;                      // bw <= 20 < 32
+            if (c < bw)
+                for (int r = lane >> 5; r < bh; r += 2) {
+                    const uint16_t *q = &src[l][r * 21 + c];
+                    int v;
+                    if (fx && fy) v = bili2(bili2(q[0], q[1], fx), bili2(q[21], q[22], fx), fy);
+                    else if (fx) v = bili2(q[0], q[1], fx);
+                    else if (fy) v = bili2(q[0], q[21], fy);
+                    else v = q[0];
+                    bil[l][r * 20 + c] = (int16_t)v;
+                }
         }
         __syncthreads();
         // 25 offsets x sh/2 rows: each item is one row's SAD on packed sample
@@ -147,9 +162,9 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
         {
             const uint32_t *B0 = reinterpret_cast<const uint32_t *>(bil[0]);
             const uint32_t *B1 = reinterpret_cast<const uint32_t *>(bil[1]);
-            const int nr = sh >> 1;
+            const int nr = sh >> 1, lnr = __builtin_ctz(nr);  // sh is a power of two
             for (int it = lane; it < 25 * nr; it += 64) {
-                const int i = it / nr, r = 2 * (it - i * nr);
+                const int i = it >> lnr, r = 2 * (it & (nr - 1));
                 const int dx = c_dmvrSearch[i][0], dy = c_dmvrSearch[i][1];
                 const int ia = (2 + dy + r) * 20 + 2 + dx, ib = (2 - dy + r) * 20 + 2 - dx;
                 const uint32_t sh16 = (ia & 1) * 16;
@@ -162,17 +177,20 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
             }
         }
         __syncthreads();
+        // the scan of h266_inter_prediction.c:1125-1140 as one wave minimum:
+        // key (cost << 5 | code), the centre (cost less a quarter) code 0 so
+        // it keeps ties, offset i code i + 1 so the earliest of equal costs
+        // wins (costs < 2^17: 8 rows of 16 differences of 10-bit samples)
+        const uint32_t adj = cost[12] - (cost[12] >> 2);
+        uint32_t key = 0xFFFFFFFFu;
+        if (lane < 25) key = lane == 12 ? adj << 5 : (cost[lane] << 5) | (uint32_t)(lane + 1);
+#pragma unroll
+        for (int m = 1; m < 32; m <<= 1) key = min(key, (uint32_t)__shfl_xor((int)key, m, 64));
         if (lane == 0) {
             int dx = 0, dy = 0;
-            uint32_t minCost = cost[12] - (cost[12] >> 2);       // the centre cost, less a quarter
-            cost[12] = minCost;
-            if (minCost >= (uint32_t)(sw * sh)) {
-                int best = 12;
-                for (int i = 0; i < 25; i++)
-                    if (cost[i] < minCost) {
-                        minCost = cost[i];
-                        best = i;
-                    }
+            cost[12] = adj;
+            if (adj >= (uint32_t)(sw * sh)) {
+                const int best = (key & 31) ? (int)(key & 31) - 1 : 12;
                 dx = 16 * c_dmvrSearch[best][0];
                 dy = 16 * c_dmvrSearch[best][1];
                 if (abs(dx) < 32 && abs(dy) < 32) {
@@ -369,7 +387,9 @@ int32_t xg_dmvr_run(xg_session *s, const xg_plane *ref0, const xg_plane *ref1, i
     hipStream_t st = (hipStream_t)xg_session_stream(s);
     int ev[2];
     const bool timed = session_timer_begin(s, 6, ev);
-    k_dmvr<<<std::min(p->nSub, slots), 64, 0, st>>>(a, b, p->dTasks, p->dSubs, p->nSub, mvd);
+    int grid = std::min(p->nSub, slots);
+    if (grid >= 8) grid -= grid % 8;                // whole XCD rounds (k_dmvr's split)
+    k_dmvr<<<grid, 64, 0, st>>>(a, b, p->dTasks, p->dSubs, p->nSub, mvd);
     if (timed) session_timer_end(s, 6, ev);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
