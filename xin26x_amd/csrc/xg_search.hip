@@ -718,7 +718,7 @@ namespace xg {
 
 constexpr int kGroupWaves = 4;
 constexpr int kGroupCurDw = 64 * 32;             // the group's current samples: 64 rows x 32 dwords
-constexpr int kGroupK = 4;                       // candidates per lane
+constexpr int kGroupK = 8;                       // candidates per lane
 
 struct GroupBlk {
     int b;                                       // block index in the list
@@ -878,7 +878,9 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
         atomicMin(&box[5], (int)B.y);
     }
     __syncthreads();
-    const int ux0 = box[0], uy0 = box[1], ws = ((box[2] - box[0]) + 1) & ~1, rows = box[3] - box[1];
+    // row pitch: an odd number of dwords, so the lanes' rows spread over the LDS banks
+    const int ux0 = box[0], uy0 = box[1], rows = box[3] - box[1];
+    const int ws = (((box[2] - box[0]) + 1) & ~1) | 2;
     const int cx0 = box[4], cy0 = box[5];
     if (t < count) {
         gb[t].ox = B.x + x0 - gb[t].s - ux0;
@@ -907,7 +909,7 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
             __syncthreads();                                 // the previous block's readers are done
             const GroupBlk g = gb[j];
             const int bx = ux0 + g.ox, byy = uy0 + g.oy;     // the block's window origin in the picture
-            const int wsj = (kGroupK * g.nq + g.w + 1) & ~1, rj = g.ny + g.h - 1;
+            const int wsj = ((kGroupK * g.nq + g.w + 1) & ~1) | 2, rj = g.ny + g.h - 1;   // odd dword pitch
             stage_win_dw(win, wsj, R.plane[0], bx, byy, rj);
             if (t == 0) {
                 gb[j].ox = 0;
@@ -968,8 +970,8 @@ int32_t xg::search_groups_enqueue(hipStream_t st, xg_session *timer, const xg_pi
     // room for one 64x64 block's window at this range (the fallback), and for
     // the union of a CTU's windows when their predictors agree
     const int nqMax = (2 * range + 2 + kGroupK - 1) / kGroupK;
-    const int one = (2 * range + 64) * (((kGroupK * nqMax + 64) + 1) / 2);
-    const int winDw = std::max(one, (2 * range + 64 + 4) * ((2 * range + 64 + kGroupK + 8) / 2));
+    const int one = (2 * range + 64) * (((kGroupK * nqMax + 64) + 1) / 2 + 1);
+    const int winDw = std::max(one, (2 * range + 64 + 4) * ((2 * range + 64 + kGroupK + 8) / 2 + 1));
     if (winDw * 4 + (int)(kGroupCurDw * 4 + kSearchGroupMax * (sizeof(GroupBlk) + 12)) > 64 * 1024) {
         set_error(XG_ERR_ARG, "search_groups_enqueue: window exceeds the LDS budget");
         return XG_ERR_ARG;
