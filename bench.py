@@ -1289,10 +1289,11 @@ def main():
         log("serving: saturated from page-locked host frames (PCIe-inclusive)")
         pin = serve.run_mix(frames=min(60, nfr), fps=0, srcs=srcs, mode="pinned")
         serving = {"workload": "16 streams (6x 640x360, 6x 1280x720, 4x 1920x1080) 10-bit 4:2:0, "
-                               f"{nfr} distinct frames each; per frame: integer search +/-16 around each block's "
-                               "predictor (the stream's resident MV field: its previous frame's final MVs), "
-                               "32-candidate fractional refinement + MC; reference = previous frame; the blocks "
-                               "of all streams' frames in shared launches",
+                               f"{nfr} distinct frames each; per frame: AMVP predictor choice (spatial candidates "
+                               "from the stream's resident MV field, its previous frame's final MVs; prediction "
+                               "SAD at quarter pel), integer search +/-16 around the predictor, 32-candidate "
+                               "fractional refinement + MC; reference = previous frame; the blocks of all "
+                               "streams' frames in shared launches",
                    "realtime_30fps": rt, "saturated": sat, "saturated_pinned_host": pin,
                    "served_vs_plain": round(sat["mpixel_per_s"] / plain, 3) if plain else None,
                    "note": "device mode: frames resident in HBM before the timed region, results to page-locked "

@@ -288,3 +288,27 @@ def run_mix(frames=300, fps=30.0, jitter_ms=3.0, srcs=None, seed=5, max_delay_us
                 "hbm_bytes_in_use": int(st.bytesInUse)}
     finally:
         srv.destroy()
+
+
+def main(argv=None):
+    """Runs the 16-stream mix once and prints its report as one JSON line:
+    python -m xin26x_amd.serve [--frames N] [--fps F] [--mode device|pinned|host]."""
+    import argparse
+    import json
+
+    import torch
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--frames", type=int, default=40, help="distinct frames per stream")
+    ap.add_argument("--fps", type=float, default=0.0, help="arrival rate per stream (0: every frame at once)")
+    ap.add_argument("--mode", choices=("device", "pinned", "host"), default="device")
+    args = ap.parse_args(argv)
+    torch.cuda.set_device(0)
+    srcs = mix_sources()
+    dev = [DeviceFrames(s, args.frames) for s in srcs] if args.mode == "device" else None
+    print(f"serving: {len(srcs)} streams, {args.frames} frames each, mode {args.mode}", flush=True)
+    rep = run_mix(frames=args.frames, fps=args.fps, srcs=srcs, mode=args.mode, dev_frames=dev)
+    print(json.dumps(rep), flush=True)
+
+
+if __name__ == "__main__":
+    main()
