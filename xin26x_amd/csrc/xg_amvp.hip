@@ -7,16 +7,18 @@
 // dispatch of the 8-tap filter) and compared with the block by SAD; the
 // first strictly lowest wins (h266_analyze_inter_cu.c:97-126).
 //
-// Mapping: one 64-lane workgroup per block, both candidates at once, in
-// sub-blocks of at most 16 x 16.  A candidate's reference window, (H + 7)
-// rows of W + 7 samples from three rows / columns above-left of the clipped
-// integer position, is staged in LDS; the first filter pass (int16, the
-// standard's intermediate) goes to LDS as well, and the second pass, the
-// clip and the absolute difference with the current block are one loop whose
-// sums reduce across the wave.  The separable two-pass form with the
-// phase-0 tap (64 at the centre) gives exactly the copy, horizontal-only and
-// vertical-only results the reference's table dispatch produces, so every
-// candidate runs the same code.
+// Mapping: eight lanes per block, no LDS and no barriers.  Lane q of a
+// block's eight costs candidate q >> 2 on the 4-column strips q & 3,
+// (q & 3) + 4, ... of the block.  A strip walks down its H + 7 reference
+// rows: each row gives the four first-pass values of the strip (8 taps as
+// int16 pairs on v_dot2), which pair with the previous row's into the
+// register ring the vertical pass reads, and from the eighth row on each row
+// completes one output row of four predicted samples, compared with the
+// current block at once.  The strip SADs add over the four lanes of a
+// candidate by quad permutes, and the two candidates' lanes exchange theirs.
+// The separable two-pass form with the phase-0 tap (64 at the centre) gives
+// exactly the copy, horizontal-only and vertical-only results the reference's
+// table dispatch produces, so every candidate runs the same code.
 //
 // The candidate lists either come from the caller (xg_amvp_select) or are
 // derived from an MV field of 4x4 units (xg_amvp_from_field, and the serving
@@ -29,14 +31,10 @@
 namespace xg {
 
 constexpr int kAmvpMaxSide = 64;
-constexpr int kAmvpSub = 16;                           // blocks are costed in sub-blocks of at most 16 x 16
-constexpr int kAmvpPitch = kAmvpSub + 8;               // samples per staged row (W + 7 at most)
-constexpr int kAmvpRows = kAmvpSub + 7;
-
-struct AmvpLds {
-    uint16_t win[2][kAmvpRows * kAmvpPitch];          // both candidates' windows
-    int16_t tmp[2][kAmvpRows * kAmvpSub];             // both candidates' first passes
-};
+constexpr int kAmvpLanes = 8;                          // lanes per block
+constexpr int kAmvpWg = 256;
+constexpr int kAmvpPerWg = kAmvpWg / kAmvpLanes;       // blocks per workgroup
+constexpr int kAmvpMaxLists = 32;                      // lists of one k_amvp_multi launch
 
 // amvrPrecision (h266_entropy_write.c:120-123): log2 units per 4 pel by imvIdx.
 __device__ __forceinline__ int amvr_prec(int imvIdx)
@@ -54,56 +52,106 @@ __device__ __forceinline__ int change_prec(int v, int inPrec, int outPrec)
     return v >= 0 ? (v + ofs - 1) >> s : (v + ofs) >> s;
 }
 
-// Prediction SADs of nc (1 or 2) candidate MVs (1/16 pel, already rounded)
-// for the block at (x, y) of w x h: both candidates are staged, filtered and
-// compared in the same loops, sub-block by sub-block.  Every lane returns
-// the totals in sad[0..nc).
-__device__ void candidate_sads(const xg_plane &cur, const xg_plane &ref, int x, int y, int w, int h, int nc,
-                               const int *mv, int fi, int ctu, AmvpLds &L, uint32_t *sad)
+// The 8 taps of a filter row, times 64, as four int16 pairs.
+__device__ __forceinline__ void taps_x64(const int8_t *t, uint32_t *p)
 {
-    const int lane = threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < 4; m++) p[m] = pack16(64 * (int)t[2 * m], 64 * (int)t[2 * m + 1]);
+}
+
+// Prediction SAD of one candidate (mvx, mvy in 1/16 pel, before the CU-window
+// clip) over the strips g, g + 4, ... of the w x h block at (x, y).
+//
+// Row r of a strip at column cs (= the clipped integer x - 3 + 4 * strip)
+// needs samples cs .. cs + 10.  They are read as the dwords from cs rounded
+// down to an even sample (five dword loads) plus either the dword ending at
+// cs + 10 (odd start) or the sample cs + 10 alone (even start), so no load
+// leaves [cs - 1, cs + 10].  E_k is the sample pair (cs + 2k, cs + 2k + 1), O_k
+// (cs + 2k + 1, cs + 2k + 2); output column j of the strip filters E_{j/2} ..
+// for even j and O_{(j-1)/2} .. for odd j.
+//
+// Both passes use the taps times 64 (pack_mid16: the first-pass value of two
+// rows in one dword; clip_hi16: the clipped second pass in the upper half);
+// the first pass is the offset-free form (sum + 32) >> 2, whose vertical sum
+// is >> 10 alone (the taps of each filter add to 64).
+__device__ uint32_t strip_sad(const xg_plane &cur, const xg_plane &ref, int x, int y, int w, int h, int mvx,
+                              int mvy, int fi, int ctu, int g)
+{
     const int fw = cur.width, fh = cur.height;
     // CU MV window (h266_enc_init.c:1462-1465, int16 as stored there), 1/16 pel
     const int mnx = (int)(int16_t)(-x - ctu - 8) * 16, mny = (int)(int16_t)(-y - ctu - 8) * 16;
     const int mxx = (int)(int16_t)(fw + (ctu - w + 8) - x) * 16, mxy = (int)(int16_t)(fh + (ctu - h + 8) - y) * 16;
-    const uint16_t *src[2];
-    const int8_t *hf[2], *vf[2];
-    for (int c = 0; c < 2; c++) {
-        const int k = c < nc ? c : 0;
-        const int ox = x * 16 + min(max(mv[2 * k], mnx), mxx), oy = y * 16 + min(max(mv[2 * k + 1], mny), mxy);
-        hf[c] = taps_for<8>(ox & 15, fi);
-        vf[c] = taps_for<8>(oy & 15, fi);
-        src[c] = ref.data + (intptr_t)((oy >> 4) - 3) * ref.stride + ((ox >> 4) - 3);
-    }
-    const int sw = min(w, kAmvpSub), sh = min(h, kAmvpSub), ww = sw + 7, hh = sh + 7;
-    int acc0 = 0, acc1 = 0;
-    for (int sy = 0; sy < h; sy += sh)
-        for (int sx = 0; sx < w; sx += sw) {
-            __syncthreads();                           // the previous sub-block's readers are done
-            for (int i = lane; i < nc * hh * ww; i += 64) {
-                const int c = i >= hh * ww, j = i - c * hh * ww, r = j / ww, col = j - r * ww;
-                L.win[c][r * kAmvpPitch + col] = src[c][(intptr_t)(sy + r) * ref.stride + sx + col];
-            }
-            __syncthreads();
-            for (int i = lane; i < nc * hh * sw; i += 64) {
-                const int c = i >= hh * sw, j = i - c * hh * sw, r = j / sw, col = j - r * sw;
-                L.tmp[c][r * kAmvpSub + col] = (int16_t)hpass<8>(&L.win[c][r * kAmvpPitch + col], c ? hf[1] : hf[0]);
-            }
-            __syncthreads();
-            const uint16_t *cb = cur.data + (intptr_t)(y + sy) * cur.stride + x + sx;
-            for (int i = lane; i < nc * sh * sw; i += 64) {
-                const int c = i >= sh * sw, j = i - c * sh * sw, r = j / sw, col = j - r * sw;
-                const int8_t *v = c ? vf[1] : vf[0];
-                int s = 0;
+    const int ox = x * 16 + min(max(mvx, mnx), mxx), oy = y * 16 + min(max(mvy, mny), mxy);
+    uint32_t ht[4], vt[4];
+    taps_x64(taps_for<8>(ox & 15, fi), ht);
+    taps_x64(taps_for<8>(oy & 15, fi), vt);
+    const intptr_t rs = ref.stride, cst = cur.stride;
+    const uint16_t *rbase = ref.data + (intptr_t)((oy >> 4) - 3) * rs + ((ox >> 4) - 3);
+    const uint16_t *cbase = cur.data + (intptr_t)y * cst + x;
+    const int nrows = h + 7;
+    uint32_t sad = 0;
+    for (int s = g; s < (w >> 2); s += 4) {
+        const uint16_t *rp = rbase + 4 * s, *cp = cbase + 4 * s;
+        int prev[4] = {0, 0, 0, 0};
+        uint32_t ring[8][4];                           // ring[p & 7][j]: rows (p, p + 1) of column j
+        for (int r0 = 0; r0 < nrows; r0 += 8) {
+            uint32_t A[4][6];
+            uint16_t cv[4][4];
 #pragma unroll
-                for (int k = 0; k < 8; k++) s += (int)v[k] * (int)L.tmp[c][(r + k) * kAmvpSub + col];
-                const int d = abs((int)cb[(intptr_t)r * cur.stride + col] - vfinal_pix(s));
-                if (c) acc1 += d;
-                else acc0 += d;
+            for (int k = 0; k < 8; k++) {
+                // loads of four rows at a time (rows past the strip re-read its last row)
+                if ((k & 3) == 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int r = min(r0 + k + i, nrows - 1);
+                        const uintptr_t a = (uintptr_t)(rp + (intptr_t)r * rs);
+                        const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
+#pragma unroll
+                        for (int m = 0; m < 5; m++) A[i][m] = q[m];
+                        const uint32_t tail = *(const uint32_t *)((a + 18) & ~(uintptr_t)3);
+                        const uint32_t last = ((const uint16_t *)a)[10];
+                        A[i][5] = (a & 2) ? tail : last;
+                        const int o = min(max(r0 + k + i - 7, 0), h - 1);
+#pragma unroll
+                        for (int j = 0; j < 4; j++) cv[i][j] = cp[(intptr_t)o * cst + j];
+                    }
+                }
+                const int r = r0 + k, i = k & 3;
+                if (r >= nrows) break;
+                const bool od = ((uintptr_t)(rp + (intptr_t)r * rs) & 2) != 0;
+                uint32_t E[5], O[5];
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    const uint32_t sh = align16(A[i][m + 1], A[i][m], 16);
+                    E[m] = od ? sh : A[i][m];
+                    O[m] = od ? A[i][m + 1] : sh;
+                }
+                int hs[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t *P = (j & 1) ? O + (j >> 1) : E + (j >> 1);
+                    hs[j] = dot2(P[3], ht[3], dot2(P[2], ht[2], dot2(P[1], ht[1], dot2_first(P[0], ht[0], kHRound64))));
+                }
+                if (r >= 1) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) ring[(k + 7) & 7][j] = pack_mid16(prev[j], hs[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) prev[j] = hs[j];
+                if (r >= 7) {                          // output row r - 7: pairs from rows r - 7, r - 5, r - 3, r - 1
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int v = dot2(ring[(k + 7) & 7][j], vt[3],
+                                           dot2(ring[(k + 5) & 7][j], vt[2],
+                                                dot2(ring[(k + 3) & 7][j], vt[1],
+                                                     dot2_first(ring[(k + 1) & 7][j], vt[0], 0))));
+                        sad += (uint32_t)abs((int)cv[i][j] - (clip_hi16(v) >> 16));
+                    }
+                }
             }
         }
-    sad[0] = (uint32_t)group_sum<64>(acc0);
-    if (nc > 1) sad[1] = (uint32_t)group_sum<64>(acc1);
+    }
+    return sad;
 }
 
 // Whether a block can be costed: inside its picture, sides 4..64 and
@@ -116,28 +164,43 @@ __device__ __forceinline__ bool amvp_block_ok(const xg_plane &cur, const xg_plan
            ref.width == cur.width && ref.height == cur.height;
 }
 
-// Xin266SelectMvp for one block; returns the winner index (-1: rejected).
-// needSad false: a single candidate is taken without costing it.
-__device__ int select_mvp(const xg_picture *pics, int pic, int refSlot, int x, int y, int w, int h, int ncand,
-                          const int *cand, int imvIdx, int ctu, bool needSad, AmvpLds &L, uint32_t *sadOut, int *mvOut)
+// Xin266SelectMvp for one block, by the eight lanes of its group (every lane
+// of the wave calls it: the sums cross lanes).  Returns the winner index
+// (-1: rejected) with its SAD and rounded MV in every lane of the group.
+// cost false: a single candidate is taken without costing it.
+__device__ int select_mvp(const xg_picture *pics, bool live, int pic, int refSlot, int x, int y, int w, int h,
+                          int ncand, int c0x, int c0y, int c1x, int c1y, int imvIdx, int ctu, bool cost,
+                          uint32_t &sadOut, int &mvx, int &mvy)
 {
-    const bool slots = pic >= 0 && pic < kSessionPicSlots && refSlot >= 0 && refSlot < kSessionPicSlots;
-    const xg_plane cur = slots ? pics[pic].plane[0] : xg_plane{}, ref = slots ? pics[refSlot].plane[0] : xg_plane{};
-    if (!slots || !amvp_block_ok(cur, ref, x, y, w, h, ctu)) {
-        *sadOut = 0xFFFFFFFFu;
-        mvOut[0] = mvOut[1] = 0;
+    const bool slots = live && pic >= 0 && pic < kSessionPicSlots && refSlot >= 0 && refSlot < kSessionPicSlots;
+    xg_plane cur{}, ref{};
+    if (slots) {
+        cur = pics[pic].plane[0];
+        ref = pics[refSlot].plane[0];
+    }
+    const bool ok = slots && amvp_block_ok(cur, ref, x, y, w, h, ctu);
+    const int prec = amvr_prec(imvIdx), fi = imvIdx == 3 ? 1 : 0;
+    const int nc = min(max(ncand, 1), 2);
+    const int m0x = change_prec(change_prec(c0x, 6, prec), prec, 6);
+    const int m0y = change_prec(change_prec(c0y, 6, prec), prec, 6);
+    const int m1x = change_prec(change_prec(c1x, 6, prec), prec, 6);
+    const int m1y = change_prec(change_prec(c1y, 6, prec), prec, 6);
+    const int q = threadIdx.x & (kAmvpLanes - 1), c = q >> 2;
+    uint32_t part = 0;
+    if (ok && (cost || nc > 1) && c < nc)
+        part = strip_sad(cur, ref, x, y, w, h, c ? m1x : m0x, c ? m1y : m0y, fi, ctu, q & 3);
+    part = (uint32_t)group_sum<4>((int)part);          // over the candidate's four strip lanes
+    const uint32_t other = (uint32_t)shfl_xor<4>((int)part);
+    const uint32_t s0 = c ? other : part, s1 = c ? part : other;
+    const int idx = nc > 1 && s1 < s0 ? 1 : 0;         // the first strictly lowest
+    if (!ok) {
+        sadOut = 0xFFFFFFFFu;
+        mvx = mvy = 0;
         return -1;
     }
-    const int prec = amvr_prec(imvIdx), fi = imvIdx == 3 ? 1 : 0;
-    ncand = min(max(ncand, 1), 2);
-    int mv[4];
-    for (int k = 0; k < 2 * ncand; k++) mv[k] = change_prec(change_prec(cand[k], 6, prec), prec, 6);
-    uint32_t sad[2] = {0, 0xFFFFFFFFu};
-    if (needSad || ncand > 1) candidate_sads(cur, ref, x, y, w, h, ncand, mv, fi, ctu, L, sad);
-    const int idx = ncand > 1 && sad[1] < sad[0] ? 1 : 0;  // the first strictly lowest
-    *sadOut = sad[idx];
-    mvOut[0] = mv[2 * idx];
-    mvOut[1] = mv[2 * idx + 1];
+    sadOut = idx ? s1 : s0;
+    mvx = idx ? m1x : m0x;
+    mvy = idx ? m1y : m0y;
     return idx;
 }
 
@@ -152,44 +215,47 @@ __device__ __forceinline__ bool field_unit(const int2 *field, int stride, int fw
 
 __device__ int amvp_list(const int2 *field, int stride, int fw, int fh, int x, int y, int w, int h, int *cand)
 {
-    int2 l[3];
-    int n = 0;
-    if (field_unit(field, stride, fw, fh, x - 1, y + h, l[n]) ||        // A0 below-left
-        field_unit(field, stride, fw, fh, x - 1, y + h - 1, l[n]))      // A1 left
-        n++;
-    if (field_unit(field, stride, fw, fh, x + w, y - 1, l[n]) ||        // B0 above-right
-        field_unit(field, stride, fw, fh, x + w - 1, y - 1, l[n]) ||    // B1 above
-        field_unit(field, stride, fw, fh, x - 1, y - 1, l[n]))          // B2 above-left
-        n++;
-    if (n == 2 && l[0].x == l[1].x && l[0].y == l[1].y) n = 1;
-    if (n < 1 || (n == 1 && (l[0].x != 0 || l[0].y != 0))) l[n++] = make_int2(0, 0);
-    const bool two = n >= 2 && (l[0].x != l[1].x || l[0].y != l[1].y);
-    cand[0] = l[0].x;
-    cand[1] = l[0].y;
-    cand[2] = two ? l[1].x : 0;
-    cand[3] = two ? l[1].y : 0;
+    int2 a = make_int2(0, 0), b = make_int2(0, 0);
+    const bool hasA = field_unit(field, stride, fw, fh, x - 1, y + h, a) ||       // A0 below-left
+                      field_unit(field, stride, fw, fh, x - 1, y + h - 1, a);     // A1 left
+    const bool hasB = field_unit(field, stride, fw, fh, x + w, y - 1, b) ||       // B0 above-right
+                      field_unit(field, stride, fw, fh, x + w - 1, y - 1, b) ||   // B1 above
+                      field_unit(field, stride, fw, fh, x - 1, y - 1, b);         // B2 above-left
+    // the list so far: A then B, a duplicate pair reduced to one
+    int2 l0 = hasA ? a : b, l1 = b;
+    int n = (int)hasA + (int)hasB;
+    if (n == 2 && a.x == b.x && a.y == b.y) n = 1;
+    // a zero MV joins an empty list or one whose single entry is non-zero
+    if (n == 0) l0 = make_int2(0, 0), n = 1;
+    else if (n == 1 && (l0.x != 0 || l0.y != 0)) l1 = make_int2(0, 0), n = 2;
+    const bool two = n == 2 && (l0.x != l1.x || l0.y != l1.y);
+    cand[0] = l0.x;
+    cand[1] = l0.y;
+    cand[2] = two ? l1.x : 0;
+    cand[3] = two ? l1.y : 0;
     return two ? 2 : 1;
 }
 
-__global__ void __launch_bounds__(64) k_amvp_select(const xg_picture *pics, const xg_amvp_block *__restrict__ blocks,
-                                                    int n, int imvIdx, int ctu, xg_amvp_result *__restrict__ out)
+__global__ void __launch_bounds__(kAmvpWg) k_amvp_select(const xg_picture *pics,
+                                                         const xg_amvp_block *__restrict__ blocks, int n, int imvIdx,
+                                                         int ctu, xg_amvp_result *__restrict__ out)
 {
-    __shared__ AmvpLds L;
-    const int b = blockIdx.x;
-    if (b >= n) return;
-    const xg_amvp_block B = blocks[b];
-    const int cand[4] = {B.cand[0][0], B.cand[0][1], B.cand[1][0], B.cand[1][1]};
+    const int b = (int)blockIdx.x * kAmvpPerWg + (int)(threadIdx.x / kAmvpLanes);
+    const bool live = b < n;
+    xg_amvp_block B{};
+    if (live) B = blocks[b];
     uint32_t sad;
-    int mv[2];
-    const int idx = select_mvp(pics, B.pic, B.ref, B.x, B.y, B.w, B.h, B.ncand, cand, imvIdx, ctu, true, L, &sad, mv);
-    if (threadIdx.x == 0) {
+    int mvx, mvy;
+    const int idx = select_mvp(pics, live, B.pic, B.ref, B.x, B.y, B.w, B.h, B.ncand, B.cand[0][0], B.cand[0][1],
+                               B.cand[1][0], B.cand[1][1], imvIdx, ctu, true, sad, mvx, mvy);
+    if (live && (threadIdx.x & (kAmvpLanes - 1)) == 0) {
         xg_amvp_result r;
         r.mvpIdx = idx;
         r.sad = sad;
-        r.mv[0] = mv[0];
-        r.mv[1] = mv[1];
-        r.meMv[0] = (int16_t)change_prec(mv[0], 6, 4);     // Xin266ChangeMv2Me: 1/16 -> quarter pel
-        r.meMv[1] = (int16_t)change_prec(mv[1], 6, 4);
+        r.mv[0] = mvx;
+        r.mv[1] = mvy;
+        r.meMv[0] = (int16_t)change_prec(mvx, 6, 4);      // Xin266ChangeMv2Me: 1/16 -> quarter pel
+        r.meMv[1] = (int16_t)change_prec(mvy, 6, 4);
         out[b] = r;
     }
 }
@@ -216,31 +282,39 @@ __global__ void k_amvp_from_field(const int2 *field, int stride, int fw, int fh,
     out[b] = o;
 }
 
-// Serving form: the blocks of several streams' frames in one launch (workgroup
-// w belongs to the last list whose `blockBegin` is <= w).  Each block takes its
+// Serving form: the blocks of several streams' frames in one launch (block i
+// belongs to the last list whose `blockBegin` is <= i).  Each block takes its
 // frame's slots and, as predictor, the quarter-pel AMVR choice among the
 // candidates of its stream's MV field (the final MVs of the stream's previous
 // frame; zero before the first); slots and predictor are written back.
-__global__ void __launch_bounds__(64) k_amvp_multi(const xg_picture *pics, const SearchList *__restrict__ lists,
-                                                   int nLists, int ctu)
+__global__ void __launch_bounds__(kAmvpWg) k_amvp_multi(const xg_picture *pics, const SearchList *__restrict__ lists,
+                                                        int nLists, int nBlocks, int ctu)
 {
-    __shared__ AmvpLds L;
+    __shared__ int begins[kAmvpMaxLists];
+    if ((int)threadIdx.x < nLists) begins[threadIdx.x] = lists[threadIdx.x].blockBegin;
+    __syncthreads();
+    const int gb = (int)blockIdx.x * kAmvpPerWg + (int)(threadIdx.x / kAmvpLanes);
+    const bool live = gb < nBlocks;
     int l = 0;
-    while (l + 1 < nLists && (int)blockIdx.x >= lists[l + 1].blockBegin) l++;
-    const SearchList S = lists[l];
-    const int b = (int)blockIdx.x - S.blockBegin;
-    xg_block B = S.blocks[b];
-    const xg_plane &cur = pics[S.curSlot].plane[0];
-    int cand[4];
-    const int nc = amvp_list(S.mvField, S.mvStride, cur.width, cur.height, B.x, B.y, B.w, B.h, cand);
+    for (int i = 1; i < nLists; i++) l += begins[i] <= gb;
+    const SearchList &S = lists[l];
+    const int b = gb - begins[l];
+    xg_block B{};
+    int cand[4] = {0, 0, 0, 0}, nc = 1;
+    if (live) {
+        B = S.blocks[b];
+        const xg_plane &cur = pics[S.curSlot].plane[0];
+        nc = amvp_list(S.mvField, S.mvStride, cur.width, cur.height, B.x, B.y, B.w, B.h, cand);
+    }
     uint32_t sad;
-    int mv[2];
-    const int idx = select_mvp(pics, S.curSlot, S.refSlot, B.x, B.y, B.w, B.h, nc, cand, 0, ctu, false, L, &sad, mv);
-    if (threadIdx.x == 0) {
+    int mvx, mvy;
+    const int idx = select_mvp(pics, live, S.curSlot, S.refSlot, B.x, B.y, B.w, B.h, nc, cand[0], cand[1], cand[2],
+                               cand[3], 0, ctu, false, sad, mvx, mvy);
+    if (live && (threadIdx.x & (kAmvpLanes - 1)) == 0) {
         B.pic = S.curSlot;
         B.ref = S.refSlot;
-        B.mvpx = idx >= 0 ? mv[0] : 0;
-        B.mvpy = idx >= 0 ? mv[1] : 0;
+        B.mvpx = idx >= 0 ? mvx : 0;
+        B.mvpy = idx >= 0 ? mvy : 0;
         S.blocks[b] = B;
     }
 }
@@ -249,7 +323,8 @@ int32_t amvp_multi_enqueue(hipStream_t st, const xg_picture *dPics, const Search
                            int ctu)
 {
     if (nLists <= 0 || nBlocks <= 0) return XG_OK;
-    k_amvp_multi<<<nBlocks, 64, 0, st>>>(dPics, dLists, nLists, ctu);
+    if (nLists > kAmvpMaxLists) return XG_ERR_ARG;
+    k_amvp_multi<<<(nBlocks + kAmvpPerWg - 1) / kAmvpPerWg, kAmvpWg, 0, st>>>(dPics, dLists, nLists, nBlocks, ctu);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
@@ -269,7 +344,8 @@ int32_t xg_amvp_select(xg_session *s, const xg_amvp_block *blocks, int32_t n, in
         return XG_ERR_ARG;
     }
     if (n == 0) return XG_OK;
-    k_amvp_select<<<n, 64, 0, v.stream>>>(v.dPics, blocks, n, imvIdx, ctuSize, out);
+    k_amvp_select<<<(n + kAmvpPerWg - 1) / kAmvpPerWg, kAmvpWg, 0, v.stream>>>(v.dPics, blocks, n, imvIdx, ctuSize,
+                                                                               out);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
