@@ -705,11 +705,13 @@ int32_t xg::full_search_frame_on(hipStream_t st, xg_session *timer, const xg_pla
 // setup at 1080p, where most blocks are 4..16 wide.  Here a workgroup of four
 // waves takes a group (up to 64 blocks of one CTU): it takes every block's
 // predictor (written by the AMVP stage, k_amvp_multi) and window, stages the
-// union of the windows once in LDS
-// (with the CTU's current samples), and spreads (block, 64 candidate-rows)
-// units over its waves; each block's minimum (cost << 32 | raster index in
-// its own window) is kept by an LDS 64-bit atomic min, so the winner and the
-// tie order are those of search_block.  The LDS window starts at an even
+// union of the windows once in LDS (with the CTU's current samples), and
+// spreads the blocks' items over its waves, 64 consecutive items of the
+// group's list per wave step.  An item is K = 8 candidate columns of two
+// candidate rows (blocks up to 16 wide: each window row read serves both) or
+// of one row (wider blocks).  Each block's minimum (cost << 32 | raster index
+// in its own window) is kept by an LDS 64-bit atomic min, so the winner and
+// the tie order are those of search_block.  The LDS window starts at an even
 // column; a block whose window starts at an odd one reads from one column to
 // the left and skips its first candidate (s = 1).  A group whose union does
 // not fit stages one block's window at a time instead.
@@ -718,7 +720,7 @@ namespace xg {
 
 constexpr int kGroupWaves = 4;
 constexpr int kGroupCurDw = 64 * 32;             // the group's current samples: 64 rows x 32 dwords
-constexpr int kGroupK = 8;                       // candidates per lane
+constexpr int kGroupK = 10;                      // candidates per lane (an odd number of dwords apart: LDS banks)
 
 struct GroupBlk {
     int b;                                       // block index in the list
@@ -726,20 +728,21 @@ struct GroupBlk {
     int ox, oy;                                  // window origin in the LDS window (ox even)
     int s;                                       // 1 when the origin is one column left of x0
     int nq, items, w, h;
+    int rpi;                                     // candidate rows per item: 2 up to 16 wide, else 1
     int pmx, pmy;                                // predictor, quarter pel
     int cur;                                     // dword offset of the block in the current samples
 };
 
 // K SADs (candidates K q .. K q + K - 1 of origin-relative column) of one
-// item for a block BW wide, h rows; window rows ws2 dwords apart, current rows
-// 32 dwords apart.
+// candidate row for a block BW wide, h rows; window rows ws2 dwords apart,
+// current rows 32 dwords apart.
 template <int BW>
 __device__ __forceinline__ void group_item(const uint32_t *rp, int ws2, const uint32_t *cp, int h, uint32_t acc[kGroupK])
 {
     constexpr int ND = BW / 2, KD = kGroupK / 2;
 #pragma unroll
     for (int k = 0; k < kGroupK; k++) acc[k] = 0;
-#pragma unroll 2
+#pragma unroll(BW >= 32 ? 1 : 2)
     for (int i = 0; i < h; i++) {
         uint32_t D[ND + KD], F[ND + KD - 1], C[ND];
 #pragma unroll
@@ -758,54 +761,125 @@ __device__ __forceinline__ void group_item(const uint32_t *rp, int ws2, const ui
     }
 }
 
-// Runs units [u0, u1) (stride kGroupWaves from the wave's first) of blocks
-// [j0, j1): unit u of block j covers items 64 (u - unit[j]) .. + 63.
+// SADs of one window row against one current row, added to acc.
+template <int ND>
+__device__ __forceinline__ void row_sads(const uint32_t *D, const uint32_t *F, const uint32_t *C, uint32_t *acc)
+{
+    constexpr int KD = kGroupK / 2;
+#pragma unroll
+    for (int j = 0; j < ND; j++)
+#pragma unroll
+        for (int m = 0; m < KD; m++) {
+            acc[2 * m] = __builtin_amdgcn_sad_u16(D[j + m], C[j], acc[2 * m]);
+            acc[2 * m + 1] = __builtin_amdgcn_sad_u16(F[j + m], C[j], acc[2 * m + 1]);
+        }
+}
+
+// The K SADs of two candidate rows at once (acc0: the row at rp, acc1: the
+// next one): window row i serves current row i of the first and current row
+// i - 1 of the second, so each window row is read and realigned once for
+// both.  Reads window rows 0 .. h.
+template <int BW>
+__device__ __forceinline__ void group_item2(const uint32_t *rp, int ws2, const uint32_t *cp, int h,
+                                            uint32_t acc0[kGroupK], uint32_t acc1[kGroupK])
+{
+    constexpr int ND = BW / 2, KD = kGroupK / 2;
+#pragma unroll
+    for (int k = 0; k < kGroupK; k++) acc0[k] = acc1[k] = 0;
+    uint32_t D[ND + KD], F[ND + KD - 1], C[ND], Cp[ND];
+    auto load_row = [&](int i) {
+#pragma unroll
+        for (int j = 0; j < ND + KD; j++) D[j] = rp[i * ws2 + j];
+#pragma unroll
+        for (int j = 0; j < ND + KD - 1; j++) F[j] = align16(D[j + 1], D[j], 16);
+    };
+    load_row(0);
+#pragma unroll
+    for (int j = 0; j < ND; j++) C[j] = cp[j];
+    row_sads<ND>(D, F, C, acc0);
+#pragma unroll 2
+    for (int i = 1; i < h; i++) {
+#pragma unroll
+        for (int j = 0; j < ND; j++) Cp[j] = C[j];
+        load_row(i);
+#pragma unroll
+        for (int j = 0; j < ND; j++) C[j] = cp[i * 32 + j];
+        row_sads<ND>(D, F, C, acc0);
+        row_sads<ND>(D, F, Cp, acc1);
+    }
+    load_row(h);
+    row_sads<ND>(D, F, C, acc1);
+}
+
+// Runs the items of blocks [j0, j1), flattened: block j holds items
+// ibeg[j] .. ibeg[j + 1] - 1 of the group, and each wave takes 64
+// consecutive items at a time (stride kGroupWaves waves), so a wave may span
+// the end of one block and the start of the next (the host orders a group's
+// blocks by width, so such a wave mostly runs one width).
 __device__ __forceinline__ void group_units(const uint32_t *win, int ws, const uint32_t *cur, const GroupBlk *gb,
-                                            const int *unit, int j0, int j1, unsigned long long *gkey,
+                                            const int *ibeg, int j0, int j1, unsigned long long *gkey,
                                             uint32_t lambda)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int total = unit[j1] - unit[j0];
-    for (int u = unit[j0] + wave; u < unit[j0] + total; u += kGroupWaves) {
-        int lo = j0, hi = j1 - 1;                 // the block whose units hold u
+    const int first = ibeg[j0], total = ibeg[j1] - first;
+    for (int u0 = 64 * wave; u0 < total; u0 += 64 * kGroupWaves) {
+        const int ig = first + u0 + lane;
+        int lo = j0, hi = j1 - 1;                 // the block whose items hold ig
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (unit[mid] <= u) lo = mid;
+            if (ibeg[mid] <= ig) lo = mid;
             else hi = mid - 1;
         }
         const GroupBlk &G = gb[lo];
-        const int it = (u - unit[lo]) * 64 + lane;
+        const int it = ig - ibeg[lo];
         unsigned long long key = ~0ull;
-        if (it < G.items) {
-            const int dyi = it / G.nq, q = it - dyi * G.nq;
+        if (ig < first + total) {
+            // item: candidate rows rpi p .. rpi p + rpi - 1, columns K q .. K q + K - 1
+            const int p = it / G.nq, q = it - p * G.nq, dyi = G.rpi * p;
             const uint32_t *rp = win + (((G.oy + dyi) * ws + G.ox) >> 1) + (kGroupK / 2) * q;
             const uint32_t *cp = cur + G.cur;
-            uint32_t acc[kGroupK];
+            const int ws2 = ws >> 1;
+            uint32_t acc[2][kGroupK];
             switch (G.w) {
-            case 4: group_item<4>(rp, ws >> 1, cp, G.h, acc); break;
-            case 8: group_item<8>(rp, ws >> 1, cp, G.h, acc); break;
-            case 16: group_item<16>(rp, ws >> 1, cp, G.h, acc); break;
-            case 32: group_item<32>(rp, ws >> 1, cp, G.h, acc); break;
-            default: group_item<64>(rp, ws >> 1, cp, G.h, acc); break;
-            }
-            const int my = G.y0 + dyi;
-            const uint32_t by = eg_bits(4 * my - G.pmy);
+            case 4: group_item2<4>(rp, ws2, cp, G.h, acc[0], acc[1]); break;
+            case 8: group_item2<8>(rp, ws2, cp, G.h, acc[0], acc[1]); break;
+            case 16: group_item2<16>(rp, ws2, cp, G.h, acc[0], acc[1]); break;
+            case 32: group_item<32>(rp, ws2, cp, G.h, acc[0]); break;
+            default: {                                   // 64 wide: two 32-wide halves
+                uint32_t t[kGroupK];
+                group_item<32>(rp, ws2, cp, G.h, acc[0]);
+                group_item<32>(rp + 16, ws2, cp + 16, G.h, t);
 #pragma unroll
-            for (int k = 0; k < kGroupK; k++) {
-                const int dx = kGroupK * q + k - G.s;
-                if (dx >= 0 && dx < G.nx) {
-                    const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * (G.x0 + dx) - G.pmx) + by) * lambda;
-                    const unsigned long long kk = ((unsigned long long)cost << 32) | (uint32_t)(dyi * G.nx + dx);
-                    key = kk < key ? kk : key;
+                for (int k = 0; k < kGroupK; k++) acc[0][k] += t[k];
+                break;
+            }
+            }
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int dy = dyi + r;
+                const uint32_t by = eg_bits(4 * (G.y0 + dy) - G.pmy);
+#pragma unroll
+                for (int k = 0; k < kGroupK; k++) {
+                    const int dx = kGroupK * q + k - G.s;
+                    if (r < G.rpi && dy < G.ny && dx >= 0 && dx < G.nx) {
+                        const uint32_t cost = (acc[r][k] << 8) + (eg_bits(4 * (G.x0 + dx) - G.pmx) + by) * lambda;
+                        const unsigned long long kk = ((unsigned long long)cost << 32) | (uint32_t)(dy * G.nx + dx);
+                        key = kk < key ? kk : key;
+                    }
                 }
             }
         }
+        if (__builtin_amdgcn_readfirstlane(lo) == __builtin_amdgcn_readlane(lo, 63)) {
+            // one block in the wave (blocks are ordered, so lanes 0 and 63 bound it)
 #pragma unroll
-        for (int m = 1; m < 64; m <<= 1) {
-            const unsigned long long o = (unsigned long long)__shfl_xor((long long)key, m, 64);
-            key = o < key ? o : key;
+            for (int m = 1; m < 64; m <<= 1) {
+                const unsigned long long o = (unsigned long long)__shfl_xor((long long)key, m, 64);
+                key = o < key ? o : key;
+            }
+            if (lane == 0) atomicMin(&gkey[lo], key);
+        } else if (key != ~0ull) {
+            atomicMin(&gkey[lo], key);
         }
-        if (lane == 0) atomicMin(&gkey[lo], key);
     }
 }
 
@@ -821,7 +895,7 @@ __device__ __forceinline__ void stage_win_dw(uint32_t *win, int ws, const xg_pla
     }
 }
 
-__global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_picture *pics,
+__global__ void __launch_bounds__(64 * kGroupWaves) __attribute__((amdgpu_waves_per_eu(4))) k_search_groups(const xg_picture *pics,
                                                                     const SearchList *__restrict__ lists, int nLists,
                                                                     int range, uint32_t lambda, int ctu, int winDw)
 {
@@ -829,7 +903,7 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
     __shared__ uint32_t cur[kGroupCurDw];
     __shared__ GroupBlk gb[kSearchGroupMax];
     __shared__ unsigned long long gkey[kSearchGroupMax];
-    __shared__ int unit[kSearchGroupMax + 1];
+    __shared__ int ibeg[kSearchGroupMax + 1];
     __shared__ int box[6];                                   // union x0, y0, x1, y1; current x0, y0
     int l = 0;
     while (l + 1 < nLists && (int)blockIdx.x >= lists[l + 1].begin) l++;
@@ -865,7 +939,8 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
         g.x0 = x0; g.y0 = y0; g.nx = nx; g.ny = ny;
         g.s = sx;
         g.nq = nq;
-        g.items = ny * nq;
+        g.rpi = B.w <= 16 ? 2 : 1;
+        g.items = ((ny + g.rpi - 1) / g.rpi) * nq;
         g.w = B.w; g.h = B.h;
         g.pmx = B.mvpx >> 2; g.pmy = B.mvpy >> 2;
         gb[t] = g;
@@ -888,8 +963,8 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
         gb[t].cur = (B.y - cy0) * 32 + ((B.x - cx0) >> 1);
     }
     if (t == 0) {
-        unit[0] = 0;
-        for (int j = 0; j < count; j++) unit[j + 1] = unit[j] + (gb[j].items + 63) / 64;
+        ibeg[0] = 0;
+        for (int j = 0; j < count; j++) ibeg[j + 1] = ibeg[j] + gb[j].items;
     }
     // the group's current samples (within 64x64 of (cx0, cy0): checked on the host)
     for (int i = t; i < kGroupCurDw; i += blockDim.x) {
@@ -899,10 +974,10 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
                                                            cx0 + 2 * c)
                     : 0u;
     }
-    if ((long)rows * (ws >> 1) <= winDw) {
+    if ((long)(rows + 1) * (ws >> 1) <= winDw) {             // an item may read one row past the windows
         stage_win_dw(win, ws, R.plane[0], ux0, uy0, rows);
         __syncthreads();
-        group_units(win, ws, cur, gb, unit, 0, count, gkey, lambda);
+        group_units(win, ws, cur, gb, ibeg, 0, count, gkey, lambda);
     } else {
         // one block's window at a time
         for (int j = 0; j < count; j++) {
@@ -916,7 +991,7 @@ __global__ void __launch_bounds__(64 * kGroupWaves) k_search_groups(const xg_pic
                 gb[j].oy = 0;
             }
             __syncthreads();
-            group_units(win, wsj, cur, gb, unit, j, j + 1, gkey, lambda);
+            group_units(win, wsj, cur, gb, ibeg, j, j + 1, gkey, lambda);
         }
     }
     __syncthreads();
@@ -953,6 +1028,10 @@ void xg::search_groups(const xg_block *blocks, int n, std::vector<int> &gidx, st
             y1 = std::max(y1, B.y + B.h);
         }
         const bool fits = x1 - x0 <= 64 && y1 - y0 <= 64;
+        // narrowest first: the waves of k_search_groups run consecutive items
+        std::stable_sort(key.begin() + a, key.begin() + e, [&](const std::pair<long, int> &p, const std::pair<long, int> &q) {
+            return blocks[p.second].w < blocks[q.second].w;
+        });
         for (int k = a; k < e;) {
             const int c = fits ? std::min(kSearchGroupMax, e - k) : 1;   // blocks past 64x64: one per group
             groups.push_back(make_int2((int)gidx.size(), c));
@@ -970,7 +1049,7 @@ int32_t xg::search_groups_enqueue(hipStream_t st, xg_session *timer, const xg_pi
     // room for one 64x64 block's window at this range (the fallback), and for
     // the union of a CTU's windows when their predictors agree
     const int nqMax = (2 * range + 2 + kGroupK - 1) / kGroupK;
-    const int one = (2 * range + 64) * (((kGroupK * nqMax + 64) + 1) / 2 + 1);
+    const int one = (2 * range + 65) * (((kGroupK * nqMax + 64) + 1) / 2 + 1);
     const int winDw = std::max(one, (2 * range + 64 + 4) * ((2 * range + 64 + kGroupK + 8) / 2 + 1));
     if (winDw * 4 + (int)(kGroupCurDw * 4 + kSearchGroupMax * (sizeof(GroupBlk) + 12)) > 64 * 1024) {
         set_error(XG_ERR_ARG, "search_groups_enqueue: window exceeds the LDS budget");
