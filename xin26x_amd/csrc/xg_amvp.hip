@@ -8,9 +8,9 @@
 // first strictly lowest wins (h266_analyze_inter_cu.c:97-126).
 //
 // Mapping: eight lanes per block, no LDS and no barriers.  Lane q of a
-// block's eight costs candidate q >> 2 on the 4-column strips q & 3,
-// (q & 3) + 4, ... of the block.  A strip walks down its H + 7 reference
-// rows: each row gives the four first-pass values of the strip (8 taps as
+// block's eight costs candidate q >> 2 on its share of the block's 4-column
+// strips (blocks 4 or 8 wide are also split into row segments, so the four
+// lanes of a candidate stay busy).  A strip walks down its reference rows: each row gives the four first-pass values of the strip (8 taps as
 // int16 pairs on v_dot2), which pair with the previous row's into the
 // register ring the vertical pass reads, and from the eighth row on each row
 // completes one output row of four predicted samples, compared with the
@@ -59,23 +59,118 @@ __device__ __forceinline__ void taps_x64(const int8_t *t, uint32_t *p)
     for (int m = 0; m < 4; m++) p[m] = pack16(64 * (int)t[2 * m], 64 * (int)t[2 * m + 1]);
 }
 
-// Prediction SAD of one candidate (mvx, mvy in 1/16 pel, before the CU-window
-// clip) over the strips g, g + 4, ... of the w x h block at (x, y).
+typedef const __attribute__((address_space(1))) uint16_t gword;
+
+// The current samples (c, c + 1) and (c + 2, c + 3) of a row as two dwords.
+__device__ __forceinline__ void cur_pairs(const gword *c, bool aligned, uint32_t &p01, uint32_t &p23)
+{
+    if (aligned) {
+        p01 = *(gdword *)c;
+        p23 = *(gdword *)(c + 2);
+    } else {
+        p01 = pack16(c[0], c[1]);
+        p23 = pack16(c[2], c[3]);
+    }
+}
+
+// Prediction SAD of one candidate over the 4-column strips s0, s0 + ds, ...
+// (< ns) of rows [0, n) of a block: rbase is the reference sample three rows
+// and three columns above-left of the candidate's first predicted sample
+// (the clipped integer position), cbase the block's first current sample.
 //
-// Row r of a strip at column cs (= the clipped integer x - 3 + 4 * strip)
-// needs samples cs .. cs + 10.  They are read as the dwords from cs rounded
-// down to an even sample (five dword loads) plus either the dword ending at
-// cs + 10 (odd start) or the sample cs + 10 alone (even start), so no load
-// leaves [cs - 1, cs + 10].  E_k is the sample pair (cs + 2k, cs + 2k + 1), O_k
-// (cs + 2k + 1, cs + 2k + 2); output column j of the strip filters E_{j/2} ..
-// for even j and O_{(j-1)/2} .. for odd j.
+// Row r of a strip at column cs needs reference samples cs .. cs + 10.  They
+// are read as the five dwords from cs rounded down to an even sample plus
+// either the dword ending at cs + 10 (odd start) or the sample cs + 10 alone
+// (even start), so no load leaves [cs - 1, cs + 10].  With od the start's
+// parity, B_k = alignbit(A_{k+1}, A_k, 16 od) is the sample pair (cs + 2k,
+// cs + 2k + 1) and O_k = alignbit(B_{k+1}, B_k, 16) the pair (cs + 2k + 1,
+// cs + 2k + 2); output column j of the strip filters B_{j/2} .. for even j
+// and O_{(j-1)/2} .. for odd j.
 //
 // Both passes use the taps times 64 (pack_mid16: the first-pass value of two
 // rows in one dword; clip_hi16: the clipped second pass in the upper half);
 // the first pass is the offset-free form (sum + 32) >> 2, whose vertical sum
-// is >> 10 alone (the taps of each filter add to 64).
-__device__ uint32_t strip_sad(const xg_plane &cur, const xg_plane &ref, int x, int y, int w, int h, int mvx,
-                              int mvy, int fi, int ctu, int g)
+// is >> 10 alone (the taps of each filter add to 64).  Rows are walked with
+// one ring of eight row pairs per column and loaded four rows ahead; rows
+// past the strip re-read its last row.
+__device__ uint32_t strip_sad(const gword *rbase, intptr_t rs, const gword *cbase, intptr_t cst, bool curAligned,
+                              int n, int s0, int ds, int ns, const uint32_t *ht, const uint32_t *vt)
+{
+    const int nrows = n + 7;
+    uint32_t sad = 0;
+    for (int s = s0; s < ns; s += ds) {
+        const gword *lp = rbase + 4 * s;               // next reference row to load
+        const gword *lc = cbase + 4 * s;               // next current row to load
+        int prev[4] = {0, 0, 0, 0};
+        uint32_t ring[8][4];                           // ring[p & 7][j]: rows (p, p + 1) of column j
+        for (int r0 = 0; r0 < nrows; r0 += 8) {
+            uint32_t A[4][6], sh[4], C01[4], C23[4];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if ((k & 3) == 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int r = r0 + k + i;
+                        const uintptr_t a = (uintptr_t)lp;
+                        const gdword *q = (const gdword *)(a & ~(uintptr_t)3);
+#pragma unroll
+                        for (int m = 0; m < 5; m++) A[i][m] = q[m];
+                        const uint32_t tail = *(const gdword *)((a + 18) & ~(uintptr_t)3);
+                        const uint32_t last = lp[10];
+                        A[i][5] = (a & 2) ? tail : last;
+                        sh[i] = (uint32_t)(a & 2) << 3;        // 16 for an odd start
+                        if (r + 1 < nrows) lp += rs;
+                        cur_pairs(lc, curAligned, C01[i], C23[i]);
+                        if (r >= 7 && r + 1 < nrows) lc += cst;
+                    }
+                }
+                const int r = r0 + k, i = k & 3;
+                if (r >= nrows) break;
+                uint32_t B[6], O[5];
+#pragma unroll
+                for (int m = 0; m < 5; m++) B[m] = align16(A[i][m + 1], A[i][m], sh[i]);
+                B[5] = align16(A[i][5], A[i][5], sh[i]);
+#pragma unroll
+                for (int m = 0; m < 5; m++) O[m] = align16(B[m + 1], B[m], 16);
+                // first pass of the four columns, the chains interleaved
+                int hs[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) hs[j] = dot2_first((j & 1) ? O[j >> 1] : B[j >> 1], ht[0], kHRound64);
+#pragma unroll
+                for (int m = 1; m < 4; m++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) hs[j] = dot2((j & 1) ? O[(j >> 1) + m] : B[(j >> 1) + m], ht[m], hs[j]);
+                if (r >= 1) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) ring[(k + 7) & 7][j] = pack_mid16(prev[j], hs[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) prev[j] = hs[j];
+                if (r >= 7) {                          // output row r - 7: pairs from rows r - 7, r - 5, r - 3, r - 1
+                    int v[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) v[j] = dot2_first(ring[(k + 1) & 7][j], vt[0], 0);
+#pragma unroll
+                    for (int m = 1; m < 4; m++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) v[j] = dot2(ring[(k + 1 + 2 * m) & 7][j], vt[m], v[j]);
+                    sad = __builtin_amdgcn_sad_u16(pack_hi16(clip_hi16(v[0]), clip_hi16(v[1])), C01[i], sad);
+                    sad = __builtin_amdgcn_sad_u16(pack_hi16(clip_hi16(v[2]), clip_hi16(v[3])), C23[i], sad);
+                }
+            }
+        }
+    }
+    return sad;
+}
+
+// The lane's share of one candidate's prediction SAD for the w x h block at
+// (x, y) (mvx, mvy in 1/16 pel, before the CU-window clip): lane g of the
+// candidate's four.  A block of ns 4-column strips is split into
+// 4 / min(ns, 4) row segments so narrow blocks keep all four lanes busy: each
+// lane takes the strips g % L, g % L + L, ... (L = 4 / segments) of segment
+// g / L.
+__device__ uint32_t candidate_sad(const xg_plane &cur, const xg_plane &ref, int x, int y, int w, int h, int mvx,
+                                  int mvy, int fi, int ctu, int g)
 {
     const int fw = cur.width, fh = cur.height;
     // CU MV window (h266_enc_init.c:1462-1465, int16 as stored there), 1/16 pel
@@ -85,73 +180,13 @@ __device__ uint32_t strip_sad(const xg_plane &cur, const xg_plane &ref, int x, i
     uint32_t ht[4], vt[4];
     taps_x64(taps_for<8>(ox & 15, fi), ht);
     taps_x64(taps_for<8>(oy & 15, fi), vt);
+    const int ns = w >> 2, segs = ns == 1 ? 4 : ns == 2 ? 2 : 1, L = 4 / segs;
+    const int n = h / segs, seg = g / L;
     const intptr_t rs = ref.stride, cst = cur.stride;
-    const uint16_t *rbase = ref.data + (intptr_t)((oy >> 4) - 3) * rs + ((ox >> 4) - 3);
-    const uint16_t *cbase = cur.data + (intptr_t)y * cst + x;
-    const int nrows = h + 7;
-    uint32_t sad = 0;
-    for (int s = g; s < (w >> 2); s += 4) {
-        const uint16_t *rp = rbase + 4 * s, *cp = cbase + 4 * s;
-        int prev[4] = {0, 0, 0, 0};
-        uint32_t ring[8][4];                           // ring[p & 7][j]: rows (p, p + 1) of column j
-        for (int r0 = 0; r0 < nrows; r0 += 8) {
-            uint32_t A[4][6];
-            uint16_t cv[4][4];
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                // loads of four rows at a time (rows past the strip re-read its last row)
-                if ((k & 3) == 0) {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int r = min(r0 + k + i, nrows - 1);
-                        const uintptr_t a = (uintptr_t)(rp + (intptr_t)r * rs);
-                        const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
-#pragma unroll
-                        for (int m = 0; m < 5; m++) A[i][m] = q[m];
-                        const uint32_t tail = *(const uint32_t *)((a + 18) & ~(uintptr_t)3);
-                        const uint32_t last = ((const uint16_t *)a)[10];
-                        A[i][5] = (a & 2) ? tail : last;
-                        const int o = min(max(r0 + k + i - 7, 0), h - 1);
-#pragma unroll
-                        for (int j = 0; j < 4; j++) cv[i][j] = cp[(intptr_t)o * cst + j];
-                    }
-                }
-                const int r = r0 + k, i = k & 3;
-                if (r >= nrows) break;
-                const bool od = ((uintptr_t)(rp + (intptr_t)r * rs) & 2) != 0;
-                uint32_t E[5], O[5];
-#pragma unroll
-                for (int m = 0; m < 5; m++) {
-                    const uint32_t sh = align16(A[i][m + 1], A[i][m], 16);
-                    E[m] = od ? sh : A[i][m];
-                    O[m] = od ? A[i][m + 1] : sh;
-                }
-                int hs[4];
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const uint32_t *P = (j & 1) ? O + (j >> 1) : E + (j >> 1);
-                    hs[j] = dot2(P[3], ht[3], dot2(P[2], ht[2], dot2(P[1], ht[1], dot2_first(P[0], ht[0], kHRound64))));
-                }
-                if (r >= 1) {
-#pragma unroll
-                    for (int j = 0; j < 4; j++) ring[(k + 7) & 7][j] = pack_mid16(prev[j], hs[j]);
-                }
-#pragma unroll
-                for (int j = 0; j < 4; j++) prev[j] = hs[j];
-                if (r >= 7) {                          // output row r - 7: pairs from rows r - 7, r - 5, r - 3, r - 1
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int v = dot2(ring[(k + 7) & 7][j], vt[3],
-                                           dot2(ring[(k + 5) & 7][j], vt[2],
-                                                dot2(ring[(k + 3) & 7][j], vt[1],
-                                                     dot2_first(ring[(k + 1) & 7][j], vt[0], 0))));
-                        sad += (uint32_t)abs((int)cv[i][j] - (clip_hi16(v) >> 16));
-                    }
-                }
-            }
-        }
-    }
-    return sad;
+    const gword *rbase = (const gword *)ref.data + (intptr_t)((oy >> 4) - 3 + seg * n) * rs + ((ox >> 4) - 3);
+    const gword *cbase = (const gword *)cur.data + (intptr_t)(y + seg * n) * cst + x;
+    const bool aligned = (((uintptr_t)cbase & 3) | (cst & 1)) == 0;
+    return strip_sad(rbase, rs, cbase, cst, aligned, n, g % L, L, ns, ht, vt);
 }
 
 // Whether a block can be costed: inside its picture, sides 4..64 and
@@ -188,7 +223,7 @@ __device__ int select_mvp(const xg_picture *pics, bool live, int pic, int refSlo
     const int q = threadIdx.x & (kAmvpLanes - 1), c = q >> 2;
     uint32_t part = 0;
     if (ok && (cost || nc > 1) && c < nc)
-        part = strip_sad(cur, ref, x, y, w, h, c ? m1x : m0x, c ? m1y : m0y, fi, ctu, q & 3);
+        part = candidate_sad(cur, ref, x, y, w, h, c ? m1x : m0x, c ? m1y : m0y, fi, ctu, q & 3);
     part = (uint32_t)group_sum<4>((int)part);          // over the candidate's four strip lanes
     const uint32_t other = (uint32_t)shfl_xor<4>((int)part);
     const uint32_t s0 = c ? other : part, s1 = c ? part : other;
