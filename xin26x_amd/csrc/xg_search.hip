@@ -727,26 +727,33 @@ struct GroupBlk {
     int x0, y0, nx, ny;                          // window in integer pel
     int ox, oy;                                  // window origin in the LDS window (ox even)
     int s;                                       // 1 when the origin is one column left of x0
-    int nq, items, w, h;
+    int nq, np, items, w, h;                     // column groups, row groups (of rpi rows), nq * np
     int rpi;                                     // candidate rows per item: 2 up to 16 wide, else 1
     int pmx, pmy;                                // predictor, quarter pel
     int cur;                                     // dword offset of the block in the current samples
 };
 
+// Dword offset of row R of an LDS window whose rows are ws2 dwords apart plus
+// one dword after every second row: two rows apart is then an odd number of
+// dwords, so lanes two candidate rows apart use different banks.
+__device__ __forceinline__ int win_row(int R, int ws2) { return R * ws2 + (R >> 1); }
+
 // K SADs (candidates K q .. K q + K - 1 of origin-relative column) of one
-// candidate row for a block BW wide, h rows; window rows ws2 dwords apart,
-// current rows 32 dwords apart.
+// candidate row for a block BW wide, h rows: window rows R .. R + h - 1 from
+// dword column col (win_row layout, pitch ws2), current rows 32 dwords apart.
 template <int BW>
-__device__ __forceinline__ void group_item(const uint32_t *rp, int ws2, const uint32_t *cp, int h, uint32_t acc[kGroupK])
+__device__ __forceinline__ void group_item(const uint32_t *win, int col, int R, int ws2, const uint32_t *cp, int h,
+                                           uint32_t acc[kGroupK])
 {
     constexpr int ND = BW / 2, KD = kGroupK / 2;
 #pragma unroll
     for (int k = 0; k < kGroupK; k++) acc[k] = 0;
+    int off = win_row(R, ws2) + col;
 #pragma unroll(BW >= 32 ? 1 : 2)
     for (int i = 0; i < h; i++) {
         uint32_t D[ND + KD], F[ND + KD - 1], C[ND];
 #pragma unroll
-        for (int j = 0; j < ND + KD; j++) D[j] = rp[i * ws2 + j];
+        for (int j = 0; j < ND + KD; j++) D[j] = win[off + j];
 #pragma unroll
         for (int j = 0; j < ND; j++) C[j] = cp[i * 32 + j];
 #pragma unroll
@@ -758,6 +765,7 @@ __device__ __forceinline__ void group_item(const uint32_t *rp, int ws2, const ui
                 acc[2 * m] = __builtin_amdgcn_sad_u16(D[j + m], C[j], acc[2 * m]);
                 acc[2 * m + 1] = __builtin_amdgcn_sad_u16(F[j + m], C[j], acc[2 * m + 1]);
             }
+        off += ws2 + ((R + i) & 1);
     }
 }
 
@@ -775,23 +783,25 @@ __device__ __forceinline__ void row_sads(const uint32_t *D, const uint32_t *F, c
         }
 }
 
-// The K SADs of two candidate rows at once (acc0: the row at rp, acc1: the
-// next one): window row i serves current row i of the first and current row
-// i - 1 of the second, so each window row is read and realigned once for
-// both.  Reads window rows 0 .. h.
+// The K SADs of two candidate rows at once (acc0: window rows R .., acc1: the
+// next one): window row R + i serves current row i of the first and current
+// row i - 1 of the second, so each window row is read and realigned once for
+// both.  Reads window rows R .. R + h.
 template <int BW>
-__device__ __forceinline__ void group_item2(const uint32_t *rp, int ws2, const uint32_t *cp, int h,
+__device__ __forceinline__ void group_item2(const uint32_t *win, int col, int R, int ws2, const uint32_t *cp, int h,
                                             uint32_t acc0[kGroupK], uint32_t acc1[kGroupK])
 {
     constexpr int ND = BW / 2, KD = kGroupK / 2;
 #pragma unroll
     for (int k = 0; k < kGroupK; k++) acc0[k] = acc1[k] = 0;
     uint32_t D[ND + KD], F[ND + KD - 1], C[ND], Cp[ND];
+    int off = win_row(R, ws2) + col;
     auto load_row = [&](int i) {
 #pragma unroll
-        for (int j = 0; j < ND + KD; j++) D[j] = rp[i * ws2 + j];
+        for (int j = 0; j < ND + KD; j++) D[j] = win[off + j];
 #pragma unroll
         for (int j = 0; j < ND + KD - 1; j++) F[j] = align16(D[j + 1], D[j], 16);
+        off += ws2 + ((R + i) & 1);
     };
     load_row(0);
 #pragma unroll
@@ -835,39 +845,54 @@ __device__ __forceinline__ void group_units(const uint32_t *win, int ws, const u
         unsigned long long key = ~0ull;
         if (ig < first + total) {
             // item: candidate rows rpi p .. rpi p + rpi - 1, columns K q .. K q + K - 1
-            const int p = it / G.nq, q = it - p * G.nq, dyi = G.rpi * p;
-            const uint32_t *rp = win + (((G.oy + dyi) * ws + G.ox) >> 1) + (kGroupK / 2) * q;
+            // row group fastest: adjacent lanes read window rows rpi apart (win_row: distinct banks)
+            const int q = it / G.np, p = it - q * G.np, dyi = G.rpi * p;
+            const int R = G.oy + dyi, col = (G.ox >> 1) + (kGroupK / 2) * q;
             const uint32_t *cp = cur + G.cur;
             const int ws2 = ws >> 1;
             uint32_t acc[2][kGroupK];
             switch (G.w) {
-            case 4: group_item2<4>(rp, ws2, cp, G.h, acc[0], acc[1]); break;
-            case 8: group_item2<8>(rp, ws2, cp, G.h, acc[0], acc[1]); break;
-            case 16: group_item2<16>(rp, ws2, cp, G.h, acc[0], acc[1]); break;
-            case 32: group_item<32>(rp, ws2, cp, G.h, acc[0]); break;
+            case 4: group_item2<4>(win, col, R, ws2, cp, G.h, acc[0], acc[1]); break;
+            case 8: group_item2<8>(win, col, R, ws2, cp, G.h, acc[0], acc[1]); break;
+            case 16: group_item2<16>(win, col, R, ws2, cp, G.h, acc[0], acc[1]); break;
+            case 32: group_item<32>(win, col, R, ws2, cp, G.h, acc[0]); break;
             default: {                                   // 64 wide: two 32-wide halves
                 uint32_t t[kGroupK];
-                group_item<32>(rp, ws2, cp, G.h, acc[0]);
-                group_item<32>(rp + 16, ws2, cp + 16, G.h, t);
+                group_item<32>(win, col, R, ws2, cp, G.h, acc[0]);
+                group_item<32>(win, col + 16, R, ws2, cp + 16, G.h, t);
 #pragma unroll
                 for (int k = 0; k < kGroupK; k++) acc[0][k] += t[k];
                 break;
             }
             }
+            // the lane's best: its candidates come in raster order (rows, then columns), so a
+            // strict < keeps the lowest index among equal costs, as the 64-bit key order does
+            uint32_t bc = 0xFFFFFFFFu, bi = 0;
+            uint32_t bxl[kGroupK];
+            bool vx[kGroupK];
+#pragma unroll
+            for (int k = 0; k < kGroupK; k++) {
+                const int dx = kGroupK * q + k - G.s;
+                vx[k] = dx >= 0 && dx < G.nx;
+                bxl[k] = eg_bits(4 * (G.x0 + dx) - G.pmx);
+            }
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int dy = dyi + r;
-                const uint32_t by = eg_bits(4 * (G.y0 + dy) - G.pmy);
+                if (r < G.rpi && dy < G.ny) {
+                    const uint32_t by = eg_bits(4 * (G.y0 + dy) - G.pmy);
+                    const int row0 = dy * G.nx + kGroupK * q - G.s;
 #pragma unroll
-                for (int k = 0; k < kGroupK; k++) {
-                    const int dx = kGroupK * q + k - G.s;
-                    if (r < G.rpi && dy < G.ny && dx >= 0 && dx < G.nx) {
-                        const uint32_t cost = (acc[r][k] << 8) + (eg_bits(4 * (G.x0 + dx) - G.pmx) + by) * lambda;
-                        const unsigned long long kk = ((unsigned long long)cost << 32) | (uint32_t)(dy * G.nx + dx);
-                        key = kk < key ? kk : key;
+                    for (int k = 0; k < kGroupK; k++) {
+                        const uint32_t cost = (acc[r][k] << 8) + (bxl[k] + by) * lambda;
+                        if (vx[k] && cost < bc) {
+                            bc = cost;
+                            bi = (uint32_t)(row0 + k);
+                        }
                     }
                 }
             }
+            if (bc != 0xFFFFFFFFu) key = ((unsigned long long)bc << 32) | bi;
         }
         if (__builtin_amdgcn_readfirstlane(lo) == __builtin_amdgcn_readlane(lo, 63)) {
             // one block in the wave (blocks are ordered, so lanes 0 and 63 bound it)
@@ -884,14 +909,14 @@ __device__ __forceinline__ void group_units(const uint32_t *win, int ws, const u
 }
 
 // rows x (ws/2) dwords of the reference luma plane from picture column cx
-// (even) and row cy into win.
+// (even) and row cy into win, in the win_row layout.
 __device__ __forceinline__ void stage_win_dw(uint32_t *win, int ws, const xg_plane &pl, int cx, int cy, int rows)
 {
     const int nd = ws >> 1, n = rows * nd;
     const uint16_t *base = pl.data + (intptr_t)cy * pl.stride + cx;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int r = i / nd, c = i - r * nd;
-        win[i] = *reinterpret_cast<const uint32_t *>(base + (intptr_t)r * pl.stride + 2 * c);
+        win[win_row(r, nd) + c] = *reinterpret_cast<const uint32_t *>(base + (intptr_t)r * pl.stride + 2 * c);
     }
 }
 
@@ -940,7 +965,8 @@ __global__ void __launch_bounds__(64 * kGroupWaves) __attribute__((amdgpu_waves_
         g.s = sx;
         g.nq = nq;
         g.rpi = B.w <= 16 ? 2 : 1;
-        g.items = ((ny + g.rpi - 1) / g.rpi) * nq;
+        g.np = (ny + g.rpi - 1) / g.rpi;
+        g.items = g.np * nq;
         g.w = B.w; g.h = B.h;
         g.pmx = B.mvpx >> 2; g.pmy = B.mvpy >> 2;
         gb[t] = g;
@@ -974,7 +1000,7 @@ __global__ void __launch_bounds__(64 * kGroupWaves) __attribute__((amdgpu_waves_
                                                            cx0 + 2 * c)
                     : 0u;
     }
-    if ((long)(rows + 1) * (ws >> 1) <= winDw) {             // an item may read one row past the windows
+    if (win_row(rows + 1, ws >> 1) <= winDw) {               // an item may read one row past the windows
         stage_win_dw(win, ws, R.plane[0], ux0, uy0, rows);
         __syncthreads();
         group_units(win, ws, cur, gb, ibeg, 0, count, gkey, lambda);
@@ -1049,8 +1075,11 @@ int32_t xg::search_groups_enqueue(hipStream_t st, xg_session *timer, const xg_pi
     // room for one 64x64 block's window at this range (the fallback), and for
     // the union of a CTU's windows when their predictors agree
     const int nqMax = (2 * range + 2 + kGroupK - 1) / kGroupK;
-    const int one = (2 * range + 65) * (((kGroupK * nqMax + 64) + 1) / 2 + 1);
-    const int winDw = std::max(one, (2 * range + 64 + 4) * ((2 * range + 64 + kGroupK + 8) / 2 + 1));
+    // row pitches plus one dword per two rows (win_row)
+    const int oneRows = 2 * range + 65, oneNd = ((kGroupK * nqMax + 64) + 1) / 2 + 1;
+    const int one = oneRows * oneNd + oneRows / 2;
+    const int uRows = 2 * range + 64 + 4, uNd = (2 * range + 64 + kGroupK + 8) / 2 + 1;
+    const int winDw = std::max(one, uRows * uNd + uRows / 2);
     if (winDw * 4 + (int)(kGroupCurDw * 4 + kSearchGroupMax * (sizeof(GroupBlk) + 12)) > 64 * 1024) {
         set_error(XG_ERR_ARG, "search_groups_enqueue: window exceeds the LDS budget");
         return XG_ERR_ARG;
