@@ -372,17 +372,24 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int wx, int
             tl1 = b1;
         }
         q += TS * lw;
+        // second pass of the tile's rows, the 2 TS dot2 chains interleaved tap by tap
+        // (consecutive dot2 of one chain would wait on each other)
         v2s d[TS];
+        int s0[TS], s1[TS];
 #pragma unroll
         for (int k = 0; k < TS; k++) {
-            int s0 = dot2_first(P0[k], vt[0], 0), s1 = dot2_first(P1[k], vt[0], 0);
-#pragma unroll
-            for (int j = 1; j < TAPS / 2; j++) {
-                s0 = dot2(P0[k + 2 * j], vt[j], s0);
-                s1 = dot2(P1[k + 2 * j], vt[j], s1);
-            }
-            d[k] = as_v2s(cw[k]) - as_v2s(pack_hi16(clip_hi16(s0), clip_hi16(s1)));
+            s0[k] = dot2_first(P0[k], vt[0], 0);
+            s1[k] = dot2_first(P1[k], vt[0], 0);
         }
+#pragma unroll
+        for (int j = 1; j < TAPS / 2; j++)
+#pragma unroll
+            for (int k = 0; k < TS; k++) {
+                s0[k] = dot2(P0[k + 2 * j], vt[j], s0[k]);
+                s1[k] = dot2(P1[k + 2 * j], vt[j], s1[k]);
+            }
+#pragma unroll
+        for (int k = 0; k < TS; k++) d[k] = as_v2s(cw[k]) - as_v2s(pack_hi16(clip_hi16(s0[k]), clip_hi16(s1[k])));
         const int ts = tile_satd2<TS>(d, lane);
         if ((m & (TS / 2 - 1)) == 0) acc += ts;
 #pragma unroll
