@@ -43,7 +43,7 @@ constexpr int kPipeMaxRange = 64;
 constexpr int kPipeQuads = 3;                              // candidate quads per lane in the full search
 // window row: 2 * range + 16 samples plus the over-read of the last lane's quads
 constexpr int kPipeWinW = 2 * kPipeMaxRange + kPB + 4 * kPipeQuads + 2;   // 158
-constexpr int kPipeWinH = 2 * kPipeMaxRange + kPB;        // 144
+constexpr int kPipeWinH = 2 * kPipeMaxRange + kPB + 1;    // 145: a two-row item may read one row past the window
 
 struct PipeArgs {
     const xg_picture *pics;
@@ -133,20 +133,26 @@ __global__ void __launch_bounds__(256) k_pipe_full(PipeArgs a)
     }
     __syncthreads();
     constexpr int ND = 2 * NQ + 8;                  // window dwords per row read by one lane
-    const int ng = (nx + 4 * NQ - 1) / (4 * NQ), items = ny * ng;
+    constexpr int NK = 4 * NQ;
+    // an item: NK adjacent candidates of two search rows; window row i serves current row i of the
+    // first and current row i - 1 of the second, so each window row is read and realigned once
+    const int ng = (nx + NK - 1) / NK, items = ((ny + 1) >> 1) * ng;
     const uint32_t *w32 = reinterpret_cast<const uint32_t *>(win);
     unsigned long long best = ~0ull;
     for (int it = tid; it < items; it += 256) {
-        const int dyi = it / ng, g = it - dyi * ng;
+        const int p = it / ng, g = it - p * ng, dyi = 2 * p;
         const uint32_t *rp = w32 + ((dyi * kPipeWinW) >> 1) + 2 * NQ * g;
-        uint32_t acc[4 * NQ];
+        uint32_t acc0[NK], acc1[NK];
 #pragma unroll
-        for (int k = 0; k < 4 * NQ; k++) acc[k] = 0;
-#pragma unroll 2
-        for (int i = 0; i < kPB; i++) {
-            uint32_t D[ND], F[ND - 1], C[8];
+        for (int k = 0; k < NK; k++) acc0[k] = acc1[k] = 0;
+        uint32_t D[ND], F[ND - 1], C[8], Cp[8];
+        auto load_row = [&](int i) {
 #pragma unroll
             for (int j = 0; j < ND; j++) D[j] = rp[i * (kPipeWinW / 2) + j];
+#pragma unroll
+            for (int j = 0; j < ND - 1; j++) F[j] = align16(D[j + 1], D[j], 16);    // pairs starting at odd samples
+        };
+        auto load_cur = [&](int i) {
             const uint4 *c4 = reinterpret_cast<const uint4 *>(curL + i * (kPB / 2));
 #pragma unroll
             for (int j = 0; j < 2; j++) {
@@ -156,28 +162,52 @@ __global__ void __launch_bounds__(256) k_pipe_full(PipeArgs a)
                 C[4 * j + 2] = v.z;
                 C[4 * j + 3] = v.w;
             }
-#pragma unroll
-            for (int j = 0; j < ND - 1; j++) F[j] = align16(D[j + 1], D[j], 16);    // pairs starting at odd samples
+        };
+        auto sads = [&](const uint32_t *Cr, uint32_t *acc) {
 #pragma unroll
             for (int t = 0; t < NQ; t++)
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    acc[4 * t] = sad2(D[2 * t + j], C[j], acc[4 * t]);
-                    acc[4 * t + 1] = sad2(F[2 * t + j], C[j], acc[4 * t + 1]);
-                    acc[4 * t + 2] = sad2(D[2 * t + j + 1], C[j], acc[4 * t + 2]);
-                    acc[4 * t + 3] = sad2(F[2 * t + j + 1], C[j], acc[4 * t + 3]);
+                    acc[4 * t] = sad2(D[2 * t + j], Cr[j], acc[4 * t]);
+                    acc[4 * t + 1] = sad2(F[2 * t + j], Cr[j], acc[4 * t + 1]);
+                    acc[4 * t + 2] = sad2(D[2 * t + j + 1], Cr[j], acc[4 * t + 2]);
+                    acc[4 * t + 3] = sad2(F[2 * t + j + 1], Cr[j], acc[4 * t + 3]);
                 }
-        }
-        const int my = y0 + dyi;
-        const uint32_t by = eg_bits(4 * my);
+        };
+        load_row(0);
+        load_cur(0);
+        sads(C, acc0);
+#pragma unroll 2
+        for (int i = 1; i < kPB; i++) {
 #pragma unroll
-        for (int k = 0; k < 4 * NQ; k++) {
-            const int ox = 4 * NQ * g + k;
-            if (x0 + ox <= x1) {
-                const uint32_t cost = (acc[k] << 8) + (eg_bits(4 * (x0 + ox)) + by) * a.lambda;
-                best = min_u64(best, ((unsigned long long)cost << 32) | (uint32_t)(dyi * nx + ox));
+            for (int j = 0; j < 8; j++) Cp[j] = C[j];
+            load_row(i);
+            load_cur(i);
+            sads(C, acc0);
+            sads(Cp, acc1);
+        }
+        load_row(kPB);
+        sads(C, acc1);
+        // the lane's candidates come in raster order, so a strict < keeps the lowest index
+        // among equal costs, as the 64-bit key order does
+        uint32_t bc = 0xFFFFFFFFu, bi = 0;
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int dy = dyi + r;
+            if (dy < ny) {
+                const uint32_t by = eg_bits(4 * (y0 + dy));
+#pragma unroll
+                for (int k = 0; k < NK; k++) {
+                    const int ox = NK * g + k;
+                    const uint32_t cost = ((r ? acc1[k] : acc0[k]) << 8) + (eg_bits(4 * (x0 + ox)) + by) * a.lambda;
+                    if (x0 + ox <= x1 && cost < bc) {
+                        bc = cost;
+                        bi = (uint32_t)(dy * nx + ox);
+                    }
+                }
             }
         }
+        if (bc != 0xFFFFFFFFu) best = min_u64(best, ((unsigned long long)bc << 32) | bi);
     }
     best = wave_min_u64(best);
     if (lane == 0) red[wave] = best;
