@@ -19,6 +19,7 @@ import golden_cases as gc
 from tests import np_rebuild as nr
 from tests.helpers import ptr
 from tests.oracle_lib import OrMe, OrRefinePrm, load_oracle
+from tests.test_oracle_properties import check_matrices
 from tests.pipeline_common import oracle_pipeline
 from xin26x_amd import workloads as wl
 
@@ -670,13 +671,17 @@ MUTANTS = {
                                 "", "mctf"),
     "mctf: padding reach without the filter taps": ("if (px - 3 < -pad || py - 3 < -pad ||",
                                                     "if (px < -pad || py < -pad ||", "mctf"),
+    # or_TrMatrix
+    "matrix: one-unit slip in a 32-point constant": ("73, 67, 61, 54, 46", "73, 67, 60, 54, 46", "matrix"),
+    "matrix: DCT-VIII without its alternating row sign": ("((k & 1) ? -1 : 1) * dst7(n, k, n - 1 - i)",
+                                                         "dst7(n, k, n - 1 - i)", "matrix"),
     "amvp: prediction MV not clipped to the CU window": (
         "const int32_t mx = clip3(rounded[c][0], mnx * 16, mxx * 16), my = clip3(rounded[c][1], mny * 16, mxy * 16);",
         "const int32_t mx = rounded[c][0], my = rounded[c][1];", "amvp"),
 }
 CHECKS = {"refine": lambda lib: check_refine(lib, nblocks=160), "subpel": check_subpel, "pipeline": check_pipeline,
           "diamond": check_diamond, "cands": check_pipeline_cands, "search": check_search_blocks, "amvp": check_amvp,
-          "mctf": check_mctf}
+          "mctf": check_mctf, "matrix": check_matrices}
 
 
 @pytest.fixture(scope="module")

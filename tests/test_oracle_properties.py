@@ -279,6 +279,90 @@ def test_matrix_against_float_basis(oracle, typ, n):
     assert np.max(np.abs(off)) <= 0.02 * diag
 
 
+# The integer DCT-II constants H.266 inherits from H.265 (the 32-point core
+# transform; Budagavi et al., "Core Transform Design in the HEVC Standard",
+# IEEE JSTSP 7(6), 2013): the tabulated magnitude of 64 sqrt(2) cos(m pi / 64)
+# for m = 1 .. 32 (m = 16 is the 64 of cos(pi/4)).
+HEVC_COS = {1: 90, 3: 90, 5: 88, 7: 85, 9: 82, 11: 78, 13: 73, 15: 67, 17: 61, 19: 54, 21: 46, 23: 38,
+            25: 31, 27: 22, 29: 13, 31: 4, 2: 90, 6: 87, 10: 80, 14: 70, 18: 57, 22: 43, 26: 25, 30: 9,
+            4: 89, 12: 75, 20: 50, 28: 18, 8: 83, 24: 36, 16: 64, 32: 0}
+# The 4-point DST-VII of H.265 (kept by H.266): 4-point sine basis magnitudes.
+HEVC_DST4 = [29, 55, 74, 84]
+
+
+def _hevc_cos(a):
+    """Entry of 64 sqrt(2) cos(a pi / 64) from the H.265 constants (cos is
+    even with period 128 units and odd about 64 - a)."""
+    a %= 128
+    if a > 64:
+        a = 128 - a
+    return HEVC_COS[a] if a <= 32 else -HEVC_COS[64 - a]
+
+
+def _sin_fold(a, n, row0):
+    """Entry of c sin(a pi / (2n + 1)) given row0[j - 1] = c sin(j pi / (2n + 1)),
+    j = 1..n (sin has period 2 (2n + 1) units, flips sign after 2n + 1 and is
+    symmetric about (2n + 1) / 2)."""
+    p = 2 * n + 1
+    a %= 2 * p
+    s = 1
+    if a >= p:
+        a, s = a - p, -1
+    if a > n:
+        a = p - a
+    return 0 if a == 0 else s * int(row0[a - 1])
+
+
+def check_dct2_exact(oracle):
+    """Every DCT-II matrix up to 32 points, and the even rows of the 64-point
+    one, exactly as the H.265 constants and the cosine's symmetries fix them:
+    row k of the N-point matrix is row k * 32 / N of the 32-point one, whose
+    entry (k, n) is 64 sqrt(2) cos(k (2n + 1) pi / 64) as tabulated (64 for
+    k = 0).  A one-unit change of any entry, a transposition or a sign slip
+    fails here, which the float-basis bound above cannot promise."""
+    for n in (2, 4, 8, 16, 32):
+        T = _matrix(oracle, 0, n)
+        exp = np.array([[64 if k == 0 else _hevc_cos(k * (32 // n) * (2 * i + 1)) for i in range(n)]
+                        for k in range(n)])
+        assert np.array_equal(T, exp), n
+    T64 = _matrix(oracle, 0, 64)
+    exp = np.array([[64 if k == 0 else _hevc_cos(k * (2 * i + 1)) for i in range(64)] for k in range(32)])
+    assert np.array_equal(T64[0::2], exp)
+
+
+def check_dst7_dct8_structure(oracle):
+    """DST-VII entry (k, n) is c sin((2k + 1)(n + 1) pi / (2N + 1)) and DCT-VIII
+    entry (k, n) is c cos((2k + 1)(2n + 1) pi / (4N + 2)) = c sin(b pi / (2N + 1))
+    with b = (2N + 1 - (2k + 1)(2n + 1)) / 2: both matrices follow exactly
+    from the DST-VII's first row, and for N = 4 that row is H.265's
+    29, 55, 74, 84.  The VVC-only first rows (N = 8, 16, 32) stay pinned by the
+    float-basis bound and near-orthogonality above."""
+    for n in (4, 8, 16, 32):
+        S = _matrix(oracle, 2, n)
+        C = _matrix(oracle, 1, n)
+        row0 = S[0]
+        assert (np.diff(row0) >= 0).all() and row0[0] > 0
+        expS = np.array([[_sin_fold((2 * k + 1) * (i + 1), n, row0) for i in range(n)] for k in range(n)])
+        expC = np.array([[_sin_fold((2 * n + 1 - (2 * k + 1) * (2 * i + 1)) // 2, n, row0) for i in range(n)]
+                         for k in range(n)])
+        assert np.array_equal(S, expS), n
+        assert np.array_equal(C, expC), n
+    assert list(_matrix(oracle, 2, 4)[0]) == HEVC_DST4
+
+
+def check_matrices(oracle):
+    check_dct2_exact(oracle)
+    check_dst7_dct8_structure(oracle)
+
+
+def test_dct2_matrices_exact_from_published_constants(oracle):
+    check_dct2_exact(oracle)
+
+
+def test_dst7_dct8_structure_and_4_point_exact(oracle):
+    check_dst7_dct8_structure(oracle)
+
+
 def test_dct2_rows_sum_to_zero(oracle):
     for n in (4, 8, 16, 32, 64):
         T = _matrix(oracle, 0, n)
