@@ -707,7 +707,7 @@ int32_t xg::full_search_frame_on(hipStream_t st, xg_session *timer, const xg_pla
 // predictor (written by the AMVP stage, k_amvp_multi) and window, stages the
 // union of the windows once in LDS (with the CTU's current samples), and
 // spreads the blocks' items over its waves, 64 consecutive items of the
-// group's list per wave step.  An item is K = 8 candidate columns of two
+// group's list per wave step.  An item is K = 10 candidate columns of two
 // candidate rows (blocks up to 16 wide: each window row read serves both) or
 // of one row (wider blocks).  Each block's minimum (cost << 32 | raster index
 // in its own window) is kept by an LDS 64-bit atomic min, so the winner and
