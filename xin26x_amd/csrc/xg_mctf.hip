@@ -55,9 +55,18 @@ struct MctfArgs {
     int nbx;
 };
 
+// Shared first pass of a fractional round of 8x8 blocks: 7 horizontal
+// offsets x 16 row pairs x 8 columns, each dword the first-pass values of two
+// consecutive window rows (low half the upper row) of one column.
+constexpr int kMctfFracBs = 8;
+constexpr int kMctfFracPairs = kMctfFracBs + 8;    // rows bs + 7 + up to 2 of spread, as pairs
+
 struct MctfLds {
     uint16_t win[(2 * kMctfMaxRange + kMctfMaxBs) * (2 * kMctfMaxRange + kMctfMaxBs)];  // 32 x 32
-    int16_t tmp[(kMctfMaxBs + 7) * kMctfMaxBs];
+    union {
+        int16_t tmp[(kMctfMaxBs + 7) * kMctfMaxBs];
+        uint32_t hp[7][kMctfFracPairs][kMctfFracBs];
+    };
     uint16_t blk[kMctfMaxBs * kMctfMaxBs];
 };
 
@@ -197,7 +206,45 @@ __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
         for (int step = 4; step >= 1; step -= 3) {
             const int ox = bmx, oy = bmy;
             unsigned long long k = ~0ull;
-            if (lane < 49) {
+            if (bs == kMctfFracBs) {
+                // the first pass of each of the 7 horizontal offsets, once for the 7 positions
+                // that share it: rows from the highest position's first tap row (spread <= 2)
+                const int rb = ((oy - 3 * step) >> 4) - 3 - y0;
+                const int nr = ((oy + 3 * step) >> 4) - 3 - y0 - rb + bs + 7;
+                __syncthreads();                      // earlier readers of hp are done
+                for (int t = lane; t < 7 * nr * bs; t += 64) {
+                    const int i = t / (nr * bs), rc = t - i * (nr * bs), r = rc / bs, c = rc - r * bs;
+                    const int mx = ox + (i - 3) * step;
+                    const int cx = (mx >> 4) - 3 - x0;
+                    const int u = hpass<8>(&L.win[(rb + r) * 32 + cx + c], taps_for<8>(mx & 15, 0));
+                    uint16_t *h = reinterpret_cast<uint16_t *>(&L.hp[i][0][0]);
+                    if (r < kMctfFracPairs) h[2 * (r * kMctfFracBs + c)] = (uint16_t)u;             // pair (r, r + 1)
+                    if (r >= 1) h[2 * ((r - 1) * kMctfFracBs + c) + 1] = (uint16_t)u;               // pair (r - 1, r)
+                }
+                __syncthreads();
+                if (lane < 49) {
+                    const int i = lane % 7, mx = ox + (i - 3) * step, my = oy + (lane / 7 - 3) * step;
+                    if (mctf_valid(a, x, y, mx, my)) {
+                        const int8_t *vf = taps_for<8>(my & 15, 0);
+                        uint32_t vt[4];
+#pragma unroll
+                        for (int j = 0; j < 4; j++) vt[j] = pack16(vf[2 * j], vf[2 * j + 1]);
+                        const int r0 = (my >> 4) - 3 - y0 - rb;      // the position's first tap row in hp
+                        uint32_t e = 0;
+#pragma unroll
+                        for (int r = 0; r < kMctfFracBs; r++)
+#pragma unroll
+                            for (int c = 0; c < kMctfFracBs; c++) {
+                                int s = 0;
+#pragma unroll
+                                for (int j = 0; j < 4; j++) s = dot2(L.hp[i][r0 + r + 2 * j][c], vt[j], s);
+                                const int d = (int)L.blk[r * kMctfMaxBs + c] - vfinal_pix(s);
+                                e += (uint32_t)(d * d);
+                            }
+                        k = ((unsigned long long)e << 32) | (uint32_t)lane;
+                    }
+                }
+            } else if (lane < 49) {
                 const int mx = ox + (lane % 7 - 3) * step, my = oy + (lane / 7 - 3) * step;
                 if (mctf_valid(a, x, y, mx, my)) {
                     const int8_t *hf = taps_for<8>(mx & 15, 0), *vf = taps_for<8>(my & 15, 0);
