@@ -63,10 +63,7 @@ constexpr int kMctfFracPairs = kMctfFracBs + 8;    // rows bs + 7 + up to 2 of s
 
 struct MctfLds {
     uint16_t win[(2 * kMctfMaxRange + kMctfMaxBs) * (2 * kMctfMaxRange + kMctfMaxBs)];  // 32 x 32
-    union {
-        int16_t tmp[(kMctfMaxBs + 7) * kMctfMaxBs];
-        uint32_t hp[7][kMctfFracPairs][kMctfFracBs];
-    };
+    int16_t tmp[(kMctfMaxBs + 7) * kMctfMaxBs];
     uint16_t blk[kMctfMaxBs * kMctfMaxBs];
 };
 
@@ -129,6 +126,9 @@ __device__ uint32_t mctf_candidate(const MctfArgs &a, MctfLds &L, int x, int y, 
     return (uint32_t)group_sum<64>((int)e);
 }
 
+// FRAC: the finest level's fractional rounds (a separate instantiation, so the
+// integer levels keep the register budget of their own code).
+template <bool FRAC>
 __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
 {
     __shared__ MctfLds L;
@@ -193,7 +193,8 @@ __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
         }
     }
     // fractional rounds: 7x7 at quarter-pel steps, then 7x7 at 1/16 pel
-    if (a.frac) {
+    if constexpr (FRAC) {
+        __shared__ uint32_t hp[7][kMctfFracPairs][kMctfFracBs];
         const int x0 = ((bmx - 15) >> 4) - 3, y0 = ((bmy - 15) >> 4) - 3;   // window covering both rounds
         const int ww = ((bmx + 15) >> 4) - ((bmx - 15) >> 4) + bs + 7;
         const int wh = ((bmy + 15) >> 4) - ((bmy - 15) >> 4) + bs + 7;
@@ -217,7 +218,7 @@ __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
                     const int mx = ox + (i - 3) * step;
                     const int cx = (mx >> 4) - 3 - x0;
                     const int u = hpass<8>(&L.win[(rb + r) * 32 + cx + c], taps_for<8>(mx & 15, 0));
-                    uint16_t *h = reinterpret_cast<uint16_t *>(&L.hp[i][0][0]);
+                    uint16_t *h = reinterpret_cast<uint16_t *>(&hp[i][0][0]);
                     if (r < kMctfFracPairs) h[2 * (r * kMctfFracBs + c)] = (uint16_t)u;             // pair (r, r + 1)
                     if (r >= 1) h[2 * ((r - 1) * kMctfFracBs + c) + 1] = (uint16_t)u;               // pair (r - 1, r)
                 }
@@ -237,7 +238,7 @@ __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
                             for (int c = 0; c < kMctfFracBs; c++) {
                                 int s = 0;
 #pragma unroll
-                                for (int j = 0; j < 4; j++) s = dot2(L.hp[i][r0 + r + 2 * j][c], vt[j], s);
+                                for (int j = 0; j < 4; j++) s = dot2(hp[i][r0 + r + 2 * j][c], vt[j], s);
                                 const int d = (int)L.blk[r * kMctfMaxBs + c] - vfinal_pix(s);
                                 e += (uint32_t)(d * d);
                             }
@@ -328,7 +329,8 @@ int32_t level_launch(hipStream_t st, const xg_plane *cur, const xg_plane *ref, i
     a.nbx = cur->width / bs;
     const int n = a.nbx * (cur->height / bs);
     if (n <= 0) return XG_OK;
-    k_mctf_level<<<n, 64, 0, st>>>(a);
+    if (frac) k_mctf_level<true><<<n, 64, 0, st>>>(a);
+    else k_mctf_level<false><<<n, 64, 0, st>>>(a);
     return XG_HIP(hipGetLastError()) ? XG_OK : XG_ERR_HIP;
 }
 
