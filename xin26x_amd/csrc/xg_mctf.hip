@@ -128,6 +128,29 @@ __device__ uint32_t mctf_candidate(const MctfArgs &a, MctfLds &L, int x, int y, 
 
 // FRAC: the finest level's fractional rounds (a separate instantiation, so the
 // integer levels keep the register budget of their own code).
+// SSD of the staged block against the window at (dx, dy) on packed pairs: the
+// window pairs realigned by one sample for odd dx (alignbit by 16 dx & 16; an
+// even dx also reads the dword past its last pair, inside the LDS struct, and
+// shifts it out), each pair difference squared and summed by one v_dot2.
+template <int BS>
+__device__ __forceinline__ uint32_t int_ssd(const MctfLds &L, int dx, int dy)
+{
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(L.win) + (dx >> 1);
+    const uint32_t *b32 = reinterpret_cast<const uint32_t *>(L.blk);
+    const uint32_t sh = (uint32_t)(dx & 1) << 4;
+    int e = 0;
+#pragma unroll 4
+    for (int r = 0; r < BS; r++) {
+        const uint32_t *wr = w32 + (dy + r) * 16, *br = b32 + r * (kMctfMaxBs / 2);
+#pragma unroll
+        for (int j = 0; j < BS / 2; j++) {
+            const v2s d = as_v2s(br[j]) - as_v2s(align16(wr[j + 1], wr[j], sh));
+            e = dot2(as_u32(d), as_u32(d), e);
+        }
+    }
+    return (uint32_t)e;
+}
+
 template <bool FRAC>
 __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
 {
@@ -175,12 +198,7 @@ __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
         for (int p = lane; p < n; p += 64) {
             const int dy = p / (2 * R + 1), dx = p - dy * (2 * R + 1);
             if (!mctf_valid(a, x, y, 16 * (ix - R + dx), 16 * (iy - R + dy))) continue;
-            uint32_t e = 0;
-            for (int r = 0; r < bs; r++)
-                for (int c = 0; c < bs; c++) {
-                    const int d = (int)L.blk[r * kMctfMaxBs + c] - (int)L.win[(dy + r) * 32 + dx + c];
-                    e += (uint32_t)(d * d);
-                }
+            const uint32_t e = bs == 16 ? int_ssd<16>(L, dx, dy) : int_ssd<8>(L, dx, dy);
             const unsigned long long kk = ((unsigned long long)e << 32) | (uint32_t)p;
             k = kk < k ? kk : k;
         }
