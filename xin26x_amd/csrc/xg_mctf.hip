@@ -167,20 +167,69 @@ __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
     int bmx = 0, bmy = 0;
     // candidates: the 3x3 lower-level MVs (rows then columns), then zero
     if (a.lower) {
+        int cxs[10], cys[10];
+        bool has[10], allInt = true;
+#pragma unroll
         for (int k = 0; k < 10; k++) {
             int cx = 0, cy = 0;
+            has[k] = true;
             if (k < 9) {
                 const int tx = bx / 2 + k % 3 - 1, ty = by / 2 + k / 3 - 1;
-                if (tx < 0 || ty < 0 || tx >= a.lowerBw || ty >= a.lowerBh) continue;
-                const int2 m = a.lower[(intptr_t)ty * a.lowerBw + tx];
-                cx = m.x * a.factor;
-                cy = m.y * a.factor;
+                has[k] = tx >= 0 && ty >= 0 && tx < a.lowerBw && ty < a.lowerBh;
+                if (has[k]) {
+                    const int2 m = a.lower[(intptr_t)ty * a.lowerBw + tx];
+                    cx = m.x * a.factor;
+                    cy = m.y * a.factor;
+                }
             }
-            const uint32_t e = mctf_candidate(a, L, x, y, cx, cy);
-            if (e < best) {
-                best = e;
-                bmx = cx;
-                bmy = cy;
+            cxs[k] = cx;
+            cys[k] = cy;
+            allInt = allInt && (!has[k] || ((cx | cy) & 15) == 0);
+        }
+        if (allInt) {
+            // integer positions (the lower levels' MVs, scaled, always are): the prediction is the
+            // reference itself, so every lane loads its samples of all ten positions at once and
+            // the SSDs need no staging
+            constexpr int SPL = kMctfMaxBs * kMctfMaxBs / 64;
+            bool ok[10];
+            uint16_t v[10][SPL];
+#pragma unroll
+            for (int k = 0; k < 10; k++) {
+                ok[k] = has[k] && mctf_valid(a, x, y, cxs[k], cys[k]);
+                const uint16_t *rp = a.ref + (intptr_t)(y + (cys[k] >> 4)) * a.rs + x + (cxs[k] >> 4);
+#pragma unroll
+                for (int t = 0; t < SPL; t++) {
+                    const int i = lane + 64 * t, r = i / bs, c = i - r * bs;
+                    v[k][t] = (ok[k] && i < bs * bs) ? rp[(intptr_t)r * a.rs + c] : (uint16_t)0;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 10; k++) {
+                int e = 0;
+#pragma unroll
+                for (int t = 0; t < SPL; t++) {
+                    const int i = lane + 64 * t, r = i / bs, c = i - r * bs;
+                    if (i < bs * bs) {
+                        const int d = (int)L.blk[r * kMctfMaxBs + c] - (int)v[k][t];
+                        e += d * d;
+                    }
+                }
+                e = group_sum<64>(e);
+                if (ok[k] && (uint32_t)e < best) {
+                    best = (uint32_t)e;
+                    bmx = cxs[k];
+                    bmy = cys[k];
+                }
+            }
+        } else {
+            for (int k = 0; k < 10; k++) {
+                if (!has[k]) continue;
+                const uint32_t e = mctf_candidate(a, L, x, y, cxs[k], cys[k]);
+                if (e < best) {
+                    best = e;
+                    bmx = cxs[k];
+                    bmy = cys[k];
+                }
             }
         }
     }
@@ -244,23 +293,32 @@ __global__ void __launch_bounds__(64) k_mctf_level(MctfArgs a)
                 if (lane < 49) {
                     const int i = lane % 7, mx = ox + (i - 3) * step, my = oy + (lane / 7 - 3) * step;
                     if (mctf_valid(a, x, y, mx, my)) {
+                        // second pass with the taps x 64 and the rounding offset x 64: clamped to
+                        // [0, (1023 << 16) | 0xFFFF], the upper half of the sum is the clipped sample,
+                        // and two columns' samples pack into one dword for a packed difference
                         const int8_t *vf = taps_for<8>(my & 15, 0);
                         uint32_t vt[4];
 #pragma unroll
-                        for (int j = 0; j < 4; j++) vt[j] = pack16(vf[2 * j], vf[2 * j + 1]);
+                        for (int j = 0; j < 4; j++) vt[j] = pack16(64 * vf[2 * j], 64 * vf[2 * j + 1]);
                         const int r0 = (my >> 4) - 3 - y0 - rb;      // the position's first tap row in hp
-                        uint32_t e = 0;
+                        const uint32_t *b32 = reinterpret_cast<const uint32_t *>(L.blk);
+                        int e = 0;
 #pragma unroll
                         for (int r = 0; r < kMctfFracBs; r++)
 #pragma unroll
-                            for (int c = 0; c < kMctfFracBs; c++) {
-                                int s = 0;
+                            for (int c = 0; c < kMctfFracBs; c += 2) {
+                                int s0 = dot2_first(hp[i][r0 + r][c], vt[0], kVOffset << 6);
+                                int s1 = dot2_first(hp[i][r0 + r][c + 1], vt[0], kVOffset << 6);
 #pragma unroll
-                                for (int j = 0; j < 4; j++) s = dot2(hp[i][r0 + r + 2 * j][c], vt[j], s);
-                                const int d = (int)L.blk[r * kMctfMaxBs + c] - vfinal_pix(s);
-                                e += (uint32_t)(d * d);
+                                for (int j = 1; j < 4; j++) {
+                                    s0 = dot2(hp[i][r0 + r + 2 * j][c], vt[j], s0);
+                                    s1 = dot2(hp[i][r0 + r + 2 * j][c + 1], vt[j], s1);
+                                }
+                                const v2s d = as_v2s(b32[(r * kMctfMaxBs + c) >> 1]) -
+                                              as_v2s(pack_hi16(clip_hi16(s0), clip_hi16(s1)));
+                                e = dot2(as_u32(d), as_u32(d), e);
                             }
-                        k = ((unsigned long long)e << 32) | (uint32_t)lane;
+                        k = ((unsigned long long)(uint32_t)e << 32) | (uint32_t)lane;
                     }
                 }
             } else if (lane < 49) {
