@@ -215,11 +215,13 @@ __device__ int select_mvp(const xg_picture *pics, bool live, int pic, int refSlo
     }
     const bool ok = slots && amvp_block_ok(cur, ref, x, y, w, h, ctu);
     const int prec = amvr_prec(imvIdx), fi = imvIdx == 3 ? 1 : 0;
-    const int nc = min(max(ncand, 1), 2);
     const int m0x = change_prec(change_prec(c0x, 6, prec), prec, 6);
     const int m0y = change_prec(change_prec(c0y, 6, prec), prec, 6);
     const int m1x = change_prec(change_prec(c1x, 6, prec), prec, 6);
     const int m1y = change_prec(change_prec(c1y, 6, prec), prec, 6);
+    // two candidates equal after the rounding predict the same samples: the first wins
+    // their tie, so only it is costed (and not at all when no SAD is asked for)
+    const int nc = min(max(ncand, 1), 2) == 2 && (m0x != m1x || m0y != m1y) ? 2 : 1;
     const int q = threadIdx.x & (kAmvpLanes - 1), c = q >> 2;
     uint32_t part = 0;
     if (ok && (cost || nc > 1) && c < nc)
