@@ -514,7 +514,7 @@ int32_t xg_session_read(xg_session *s, uint32_t *candCost, xg_block_result *best
  * (xg_search_blocks, xg_full_search_frame, the full search of xg_me_pipeline),
  * 4 a whole xg_me_pipeline frame, 5 the reconstruction kernel (xg_recon_run),
  * 6 the DMVR kernel (xg_dmvr_run), 7 the bi-prediction distortion kernel
- * (xg_bipred_run, without the kernel that zeroes the sums before it), 8 the
+ * (xg_bipred_run), 8 the
  * quantisation kernel (xg_quant_run), 9 a whole xg_mctf_me frame, 10 its
  * finest level (8x8 blocks with the fractional rounds). */
 int32_t xg_session_kernel_time(xg_session *s, int32_t stage, float *avgMs, int32_t *count);

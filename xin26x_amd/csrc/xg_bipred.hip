@@ -9,8 +9,11 @@
 // 4x4) -- for every block of a frame in one launch.  The plan flattens the
 // blocks into their Hadamard tiles (8x8 when both sides are multiples of 8,
 // else 4x4; SAD is tile-size independent, so the same tiles serve it): one
-// thread per tile, each adding its tile's cost into its block's sum, so a
-// frame of small blocks still fills the chip.
+// thread per tile, so a frame of small blocks still fills the chip.  The plan
+// keeps each block's tiles inside one workgroup of 256 (a block that would
+// straddle two starts the next one, after empty tiles), so the workgroup sums
+// its blocks in LDS and one thread per block writes the total: one launch,
+// no pass that zeroes the outputs and no global atomics.
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -65,24 +68,29 @@ struct BiTile {
     int32_t n;
 };
 
-__global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
+constexpr int kBiWg = 256;                       // tiles per workgroup
 
-__global__ void __launch_bounds__(256) k_bipred_tiles(BiPic in, BiPic p0, BiPic p1, const BiTile *__restrict__ tiles,
-                                                      int nTiles, int satd, uint32_t *__restrict__ out)
+__global__ void __launch_bounds__(kBiWg) k_bipred_tiles(BiPic in, BiPic p0, BiPic p1,
+                                                        const BiTile *__restrict__ tiles, int nTiles, int satd,
+                                                        uint32_t *__restrict__ out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nTiles) return;
-    const BiTile t = tiles[i];
-    const uint16_t *a = in.data + (intptr_t)t.y * in.stride + t.x;
-    const uint16_t *q0 = p0.data + (intptr_t)t.y * p0.stride + t.x;
-    const uint16_t *q1 = p1.data + (intptr_t)t.y * p1.stride + t.x;
-    const uint32_t c = t.n == 8 ? bi_tile<8>(a, in.stride, q0, p0.stride, q1, p1.stride, satd != 0)
-                                : bi_tile<4>(a, in.stride, q0, p0.stride, q1, p1.stride, satd != 0);
-    atomicAdd(&out[t.block], c);
+    __shared__ uint32_t sum[kBiWg];                  // the workgroup's blocks, from its first tile's
+    const int i = (int)blockIdx.x * kBiWg + (int)threadIdx.x;
+    const BiTile t = i < nTiles ? tiles[i] : BiTile{-1, 0, 0, 0};
+    const int first = tiles[(int)blockIdx.x * kBiWg].block;   // a workgroup never starts with an empty tile
+    sum[threadIdx.x] = 0;
+    __syncthreads();
+    if (t.block >= 0) {
+        const uint16_t *a = in.data + (intptr_t)t.y * in.stride + t.x;
+        const uint16_t *q0 = p0.data + (intptr_t)t.y * p0.stride + t.x;
+        const uint16_t *q1 = p1.data + (intptr_t)t.y * p1.stride + t.x;
+        const uint32_t c = t.n == 8 ? bi_tile<8>(a, in.stride, q0, p0.stride, q1, p1.stride, satd != 0)
+                                    : bi_tile<4>(a, in.stride, q0, p0.stride, q1, p1.stride, satd != 0);
+        atomicAdd(&sum[t.block - first], c);
+    }
+    __syncthreads();
+    // the block's first tile writes its total
+    if (t.block >= 0 && (threadIdx.x == 0 || tiles[i - 1].block != t.block)) out[t.block] = sum[t.block - first];
 }
 
 }  // namespace xg
@@ -91,9 +99,10 @@ using namespace xg;
 
 namespace {
 struct BiPlan {
-    std::vector<xg_block> hBlocks;
     BiTile *dTiles = nullptr;
     int n = 0, nTiles = 0, cap = 0;
+    int maxX = 0, maxY = 0;                     // extent of the blocks: the planes must cover it
+    int widest = -1, tallest = -1;              // a block reaching maxX / maxY (for the error text)
     bool valid = false;
 };
 std::mutex g_bplan_mu;
@@ -130,6 +139,9 @@ int32_t xg_bipred_plan(xg_session *s, const xg_block *blocks, int32_t n)
     for (int i = 0; i < n; i++) {
         const xg_block &b = blocks[i];
         const int t = ((b.w & 7) == 0 && (b.h & 7) == 0) ? 8 : 4;
+        const int k = (b.w / t) * (b.h / t);           // at most 256 (128x128 in 8x8 tiles)
+        const int used = (int)tiles.size() % kBiWg;
+        if (used && used + k > kBiWg) tiles.resize(tiles.size() + (kBiWg - used), BiTile{-1, 0, 0, 0});
         for (int y = 0; y < b.h; y += t)
             for (int x = 0; x < b.w; x += t) tiles.push_back(BiTile{i, (int16_t)(b.x + x), (int16_t)(b.y + y), t});
     }
@@ -142,7 +154,12 @@ int32_t xg_bipred_plan(xg_session *s, const xg_block *blocks, int32_t n)
     if (nt && !XG_HIP(hipMemcpyAsync(p->dTiles, tiles.data(), sizeof(BiTile) * (size_t)nt, hipMemcpyHostToDevice, st)))
         return XG_ERR_HIP;
     if (!XG_HIP(hipStreamSynchronize(st))) return XG_ERR_HIP;
-    p->hBlocks.assign(blocks, blocks + n);
+    p->maxX = p->maxY = 0;
+    p->widest = p->tallest = -1;
+    for (int i = 0; i < n; i++) {
+        if (blocks[i].x + blocks[i].w > p->maxX) p->maxX = blocks[i].x + blocks[i].w, p->widest = i;
+        if (blocks[i].y + blocks[i].h > p->maxY) p->maxY = blocks[i].y + blocks[i].h, p->tallest = i;
+    }
     p->n = n;
     p->nTiles = nt;
     p->valid = true;
@@ -165,22 +182,18 @@ int32_t xg_bipred_run(xg_session *s, const xg_plane *input, const xg_plane *pred
         set_error(XG_ERR_STATE, "xg_bipred_run: no plan (call xg_bipred_plan first; a failed plan leaves none)");
         return XG_ERR_STATE;
     }
-    // every block inside all three planes
-    for (int i = 0; i < p->n; i++) {
-        const xg_block &b = p->hBlocks[i];
-        for (const xg_plane *pl : {input, pred0, pred1})
-            if (b.x + b.w > pl->width || b.y + b.h > pl->height) {
-                set_error(XG_ERR_ARG, "xg_bipred_run: block %d is outside a %dx%d plane", i, pl->width, pl->height);
-                return XG_ERR_ARG;
-            }
-    }
+    // every block inside all three planes (the plan keeps the blocks' extent)
+    for (const xg_plane *pl : {input, pred0, pred1})
+        if (p->maxX > pl->width || p->maxY > pl->height) {
+            set_error(XG_ERR_ARG, "xg_bipred_run: block %d is outside a %dx%d plane",
+                      p->maxX > pl->width ? p->widest : p->tallest, pl->width, pl->height);
+            return XG_ERR_ARG;
+        }
     if (p->n == 0) return XG_OK;
     hipStream_t st = (hipStream_t)xg_session_stream(s);
-    // the sums start at zero (a kernel, not a memset: the launch may be captured)
-    k_zero_u32<<<(p->n + 255) / 256, 256, 0, st>>>(dist, p->n);
     int ev[2];
-    const bool timed = session_timer_begin(s, 7, ev);      // the distortion kernel alone
-    k_bipred_tiles<<<(p->nTiles + 255) / 256, 256, 0, st>>>(BiPic{input->data, input->stride},
+    const bool timed = session_timer_begin(s, 7, ev);
+    k_bipred_tiles<<<(p->nTiles + kBiWg - 1) / kBiWg, kBiWg, 0, st>>>(BiPic{input->data, input->stride},
                                                             BiPic{pred0->data, pred0->stride},
                                                             BiPic{pred1->data, pred1->stride}, p->dTiles, p->nTiles,
                                                             satd, dist);

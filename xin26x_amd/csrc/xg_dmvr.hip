@@ -21,6 +21,7 @@
 // The per-call table slots (bilinear filters, S16 SAD) and the per-call
 // refinement stage host data and run the same device code.
 #include <algorithm>
+#include <climits>
 #include <mutex>
 #include <string.h>
 #include <vector>
@@ -242,7 +243,10 @@ using namespace xg;
 
 namespace {
 struct DmvrPlan {
-    std::vector<DmvrTask> hTasks;     // host copy: the run checks every read against the planes
+    // per list, the extent of every block's source region (clipped MV, -2 pels,
+    // (w + 5) x (h + 5)) and a block reaching each side: the run checks the
+    // planes against it
+    int lo[2][2], hi[2][2], loAt[2][2], hiAt[2][2];
     DmvrTask *dTasks = nullptr;
     int2 *dSubs = nullptr;
     int n = 0, nSub = 0, taskCap = 0, subCap = 0;
@@ -334,26 +338,44 @@ int32_t xg_dmvr_plan(xg_session *s, const xg_dmvr_block *blocks, int32_t n, int3
     if (out && !XG_HIP(hipMemcpyAsync(p->dSubs, subs.data(), sizeof(int2) * (size_t)out, hipMemcpyHostToDevice, st)))
         return XG_ERR_HIP;
     if (!XG_HIP(hipStreamSynchronize(st))) return XG_ERR_HIP;
-    p->hTasks = std::move(tasks);
+    for (int l = 0; l < 2; l++)
+        for (int d = 0; d < 2; d++) {
+            p->lo[l][d] = INT_MAX;
+            p->hi[l][d] = INT_MIN;
+            p->loAt[l][d] = p->hiAt[l][d] = 0;
+        }
+    for (int i = 0; i < n; i++) {
+        const DmvrTask &t = tasks[(size_t)i];
+        for (int l = 0; l < 2; l++) {
+            const int mvx = std::min(std::max(t.mv[l][0], t.minX * 16), t.maxX * 16) - 32;
+            const int mvy = std::min(std::max(t.mv[l][1], t.minY * 16), t.maxY * 16) - 32;
+            const int o[2] = {(t.x * 16 + mvx) >> 4, (t.y * 16 + mvy) >> 4};
+            const int e[2] = {o[0] + (1 << t.lgW) + 5, o[1] + (1 << t.lgH) + 5};
+            for (int d = 0; d < 2; d++) {
+                if (o[d] < p->lo[l][d]) p->lo[l][d] = o[d], p->loAt[l][d] = i;
+                if (e[d] > p->hi[l][d]) p->hi[l][d] = e[d], p->hiAt[l][d] = i;
+            }
+        }
+    }
     p->n = n;
     p->nSub = out;
     p->valid = true;
     return XG_OK;
 }
 
-// every list's source region (clipped MV, -2 pels, (w + 5) x (h + 5)) inside the padded plane
-static bool reads_inside(const std::vector<DmvrTask> &tasks, const xg_plane *ref0, const xg_plane *ref1, int *bad)
+// every list's source region inside the padded plane, from the plan's extents
+static bool reads_inside(const DmvrPlan &p, const xg_plane *ref0, const xg_plane *ref1, int *bad)
 {
-    for (size_t i = 0; i < tasks.size(); i++) {
-        const DmvrTask &t = tasks[i];
-        for (int l = 0; l < 2; l++) {
-            const xg_plane *pl = l ? ref1 : ref0;
-            const int mvx = std::min(std::max(t.mv[l][0], t.minX * 16), t.maxX * 16) - 32;
-            const int mvy = std::min(std::max(t.mv[l][1], t.minY * 16), t.maxY * 16) - 32;
-            const int ix = (t.x * 16 + mvx) >> 4, iy = (t.y * 16 + mvy) >> 4;
-            if (ix < -pl->pad || iy < -pl->pad || ix + (1 << t.lgW) + 5 > pl->width + pl->pad ||
-                iy + (1 << t.lgH) + 5 > pl->height + pl->pad) {
-                *bad = (int)i;
+    for (int l = 0; l < 2; l++) {
+        const xg_plane *pl = l ? ref1 : ref0;
+        const int size[2] = {pl->width, pl->height};
+        for (int d = 0; d < 2; d++) {
+            if (p.lo[l][d] < -pl->pad) {
+                *bad = p.loAt[l][d];
+                return false;
+            }
+            if (p.hi[l][d] > size[d] + pl->pad) {
+                *bad = p.hiAt[l][d];
                 return false;
             }
         }
@@ -378,7 +400,7 @@ int32_t xg_dmvr_run(xg_session *s, const xg_plane *ref0, const xg_plane *ref1, i
     }
     if (p->nSub == 0) return XG_OK;
     int bad = -1;
-    if (!reads_inside(p->hTasks, ref0, ref1, &bad)) {
+    if (!reads_inside(*p, ref0, ref1, &bad)) {
         set_error(XG_ERR_ARG, "xg_dmvr_run: block %d reads outside the padded reference planes", bad);
         return XG_ERR_ARG;
     }
