@@ -35,8 +35,8 @@ def _scene(seed=21):
         hh, ww = (int(v) for v in rng.integers(3, 11, 2))
         field[uy:uy + hh, ux:ux + ww] = rng.integers(-400, 401, 2)
         blocks.append((4 * ux + 4, 4 * uy + 4, 4, 4))
-    for w in (4, 8, 16, 32, 64):
-        for h in (4, 8, 16, 32, 64):
+    for w in (4, 8, 12, 16, 20, 32, 48, 64):       # multiples of 4, not only powers of two
+        for h in (4, 8, 12, 16, 24, 32, 64):
             for x, y in ((0, 0), (W - w, H - h), (0, H - h), (W - w, 0)):
                 blocks.append((x, y, w, h))
             for _ in range(12):
