@@ -128,6 +128,31 @@ def test_mctf_frame_matches_oracle(oracle, env):
     assert np.count_nonzero((m[..., 0] == -144) & (m[..., 1] == 80)) > nbx * nby // 3
 
 
+def test_mctf_fractional_candidates_and_16x16_rounds(oracle, env):
+    """Paths the frame search does not take: lower-level MVs with fractional
+    parts (the finest level's own output used as the candidate field, so each
+    candidate is costed through the staged two-pass filter), and the
+    fractional rounds on 16x16 blocks (one position per lane, no shared first
+    pass); both against the oracle on the same inputs."""
+    lib, s, torch = env
+    cur, ref = _scene()
+    lv = oracle_mctf_levels(oracle, cur, ref, pads=(16, 32, PAD))
+    rp, mv0 = lv["l0"][1], lv["l0"][8]
+    assert np.count_nonzero(mv0 & 15) > 100                    # fractional candidates
+    for bs, rng, lower, factor in ((8, 1, mv0, 1), (16, 2, lv["l2"][8], 2)):
+        h, w = cur.shape
+        nbx, nby = w // bs, h // bs
+        emv, eerr = np.zeros(2 * nbx * nby, np.int32), np.zeros(nbx * nby, np.uint32)
+        low = np.ascontiguousarray(lower.astype(np.int32))
+        ps = (rp.shape[0] - h) // 2
+        oracle.or_MctfMeLevel(ptr(cur), w, ptr(rp, ps * rp.shape[1] + ps), rp.shape[1], w, h, PAD, bs, rng, ptr(low),
+                              low.shape[1], low.shape[0], factor, 1, 0, nby, ptr(emv), ptr(eerr))
+        gmv, gerr = _level(lib, s, torch, cur, rp, PAD, bs, rng, lower, factor, 1)
+        assert np.array_equal(gmv.reshape(-1), emv), bs
+        assert np.array_equal(gerr.reshape(-1), eerr), bs
+        print(f"mctf bs {bs}: {nbx}x{nby} blocks with fractional rounds, bit-exact", flush=True)
+
+
 def test_mctf_padding_rule(oracle, env):
     """Texture in the padding, the current picture moved 6 right: the left
     blocks' true match reads past a padding of 8 and is skipped, as in the
