@@ -119,7 +119,8 @@ def one_and_all_cores(run, seconds):
     out["value_1core"] = one["value"]
     out["sample_1core"] = one["sample"]
     out["cpu_model"] = cpu_model()
-    for k in ("bit_exact_vs_gpu_on_sample", "bit_exact_vs_gpu"):
+    for k in ("bit_exact_vs_gpu_on_sample", "bit_exact_vs_gpu", "bit_exact_vs_oracle_on_sample",
+              "bit_exact_simd_vs_c_on_sample"):
         if k in one:
             out[k] = bool(out.get(k, True)) and bool(one[k])
     return out
@@ -205,25 +206,39 @@ def residual_workload():
 
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libxin_ref.so")
+REF8_LIB = os.path.join(ROOT, "oracle", "_ref", "libxin_ref8.so")
 
 
-def ref_interp_cpu(fp, offs, seconds, threads=None):
+def ref_interp_cpu(fp, offs, seconds, threads=None, simd8=False):
     """The reference's own code on the host: the interpolation share of the
-    same refinement (every block's candidates through the reference's 10-bit
-    h266_luma_mc.c / h266_chroma_mc.c, built from its sources into
-    oracle/_ref by oracle/Makefile, driven by oracle/ref_interp.c) in chunks
-    of blocks on host threads.  Its last luma prediction of each chunk is
-    compared with the oracle's.  None when that library was not built."""
-    if not os.path.exists(REF_LIB):
+    same refinement (every block's candidates through the reference's
+    interpolation, built from its sources into oracle/_ref by
+    oracle/Makefile, driven by oracle/ref_interp.c) in chunks of blocks on
+    host threads.
+
+    simd8=False: the 10-bit build (h266_luma_mc.c / h266_chroma_mc.c, C only);
+    the last luma prediction of each chunk is compared with the oracle's.
+    simd8=True: the reference's default 8-bit build through its encoder
+    tables with the SSSE3 and AVX2 entries, on the same frames shifted to 8
+    bits; one chunk is run again through the C entries of the same build and
+    compared.  None when that library was not built."""
+    path = REF8_LIB if simd8 else REF_LIB
+    if not os.path.exists(path):
         return None
     from tests.oracle_lib import load_oracle
-    ref = ctypes.CDLL(REF_LIB)
+    ref = ctypes.CDLL(path)
     ref.ref_RefineInterpShare.restype = ctypes.c_int64
+    level = None
+    if simd8:
+        ref.ref_SetDispatch.restype = ctypes.c_int32
+        level = ref.ref_SetDispatch(2)
     oracle = load_oracle()
     threads = threads or host_threads()
     pad = 80
     W, H = fp.width, fp.height
-    refp = [np.ascontiguousarray(np.pad(p, pad if i == 0 else pad // 2, mode="edge")) for i, p in enumerate(fp.ref)]
+    planes = [(p >> 2).astype(np.uint8) for p in fp.ref] if simd8 else fp.ref
+    pt = np.uint8 if simd8 else np.uint16
+    refp = [np.ascontiguousarray(np.pad(p, pad if i == 0 else pad // 2, mode="edge")) for i, p in enumerate(planes)]
     rsy, rsc = W + 2 * pad, W // 2 + pad
     blk = np.ascontiguousarray(fp.blocks.astype(np.int32))
     offx = np.ascontiguousarray(np.array([o[0] for o in offs], np.int16))
@@ -231,13 +246,19 @@ def ref_interp_cpu(fp, offs, seconds, threads=None):
     P = lambda a, off=0: ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
     chunk, n, nc = 256, len(blk), len(offs)
 
-    def work(start):
+    def interp(start):
         cnt = min(chunk, n - start)
-        oy, ou, ov = np.zeros(64 * 64, np.uint16), np.zeros(32 * 32, np.uint16), np.zeros(32 * 32, np.uint16)
+        oy, ou, ov = np.zeros(64 * 64, pt), np.zeros(32 * 32, pt), np.zeros(32 * 32, pt)
         px = ref.ref_RefineInterpShare(P(refp[0], pad * rsy + pad), P(refp[1], (pad // 2) * rsc + pad // 2),
                                        P(refp[2], (pad // 2) * rsc + pad // 2), ctypes.c_ssize_t(rsy),
                                        ctypes.c_ssize_t(rsc), W, H, P(blk, start * 8), cnt, nc, P(offx), P(offy), 64,
                                        P(oy), P(ou), P(ov))
+        return px, cnt, (oy, ou, ov)
+
+    def work(start):
+        px, cnt, (oy, _, _) = interp(start)
+        if simd8:
+            return px, True
         x, y, w, h, mx, my = (int(v) for v in blk[start + cnt - 1, :6])
         mnx, mny = np.int16(-x - 72), np.int16(-y - 72)
         mxx, mxy = np.int16(W + (64 - w + 8) - x), np.int16(H + (64 - h + 8) - y)
@@ -260,6 +281,21 @@ def ref_interp_cpu(fp, offs, seconds, threads=None):
                 px += p_
                 exact &= bool(ok)
     dt = time.perf_counter() - t0
+    if simd8:
+        # Every thread has finished: the tables may be rebuilt.  The last
+        # block of one chunk, through the SIMD and then the C entries.
+        simd = interp(0)[2]
+        ref.ref_SetDispatch(0)
+        plain = interp(0)[2]
+        exact = all(np.array_equal(a, b) for a, b in zip(simd, plain))
+        names = {2: "SSSE3 + AVX2", 1: "SSSE3", 0: "C"}
+        return {"value": round(px / dt / 1e6, 3), "unit": "Mpixel/s", "cores": threads, "kind": "reference",
+                "sample": f"the reference's default 8-bit build (gcc -O2, built from its sources) through its "
+                          f"encoder's interpolation tables with the {names[level]} entries, on the workload's "
+                          f"frames shifted to 8 bits: the interpolation share only (luma 8-tap + Cb/Cr 4-tap "
+                          f"prediction of all {nc} candidates) in chunks of {chunk}, {threads} threads, "
+                          f"{dt:.1f} s, {px / 1e6:.1f} Mpixel of blocks. 8-bit samples, not the workload's 10",
+                "bit_exact_simd_vs_c_on_sample": bool(exact)}
     return {"value": round(px / dt / 1e6, 3), "unit": "Mpixel/s", "cores": threads, "kind": "reference",
             "sample": f"the reference's 10-bit h266_luma_mc.c + h266_chroma_mc.c (gcc -O2, built from its sources) "
                       f"on the interpolation share only (luma 8-tap + Cb/Cr 4-tap prediction of all {nc} "
@@ -1359,6 +1395,10 @@ def main():
             log("cpu baseline: the reference's own interpolation")
             cpu["reference_interp"] = one_and_all_cores(lambda sec, t: ref_interp_cpu(fp, offs, sec, threads=t),
                                                         args.cpu_seconds / 2)
+        if os.path.exists(REF8_LIB):
+            log("cpu baseline: the reference's 8-bit SIMD interpolation")
+            cpu["reference_interp_simd8"] = one_and_all_cores(
+                lambda sec, t: ref_interp_cpu(fp, offs, sec, threads=t, simd8=True), args.cpu_seconds / 2)
 
     line = {
         "metric": METRIC,

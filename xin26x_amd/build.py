@@ -55,7 +55,8 @@ def build(verbose=True):
 
 def build_oracle(verbose=True):
     """The CPU checker (oracle/liboracle.so) and, when the xin26x sources are
-    present, oracle/_ref/libxin_ref.so.  Test infrastructure only."""
+    present, oracle/_ref/libxin_ref.so (10-bit C) and libxin_ref8.so (8-bit,
+    with the SSSE3/AVX2 table entries).  Test infrastructure only."""
     od = os.path.join(ROOT, "oracle")
     subprocess.check_call(["make", "-s", "-C", od, "-B", "liboracle.so"])
     ref = os.environ.get("XG_REFERENCE_DIR", "/root/reference")
