@@ -1354,21 +1354,27 @@ def main():
         nfr = args.serve_frames
         log(f"serving: {nfr} distinct frames per stream into HBM (as from a GPU decoder)")
         dev = [serve.DeviceFrames(s_, nfr) for s_ in srcs]
+        cap = serve.two_chain_cap(srcs)
         log(f"serving: 16 streams at 30 fps, {nfr} frames each")
-        rt = serve.run_mix(frames=nfr, fps=30.0, srcs=srcs, mode="device", dev_frames=dev)
+        rt = serve.run_mix(frames=nfr, fps=30.0, srcs=srcs, mode="device", dev_frames=dev, max_blocks=cap)
         log("serving: saturated (every frame offered at once)")
-        sat = serve.run_mix(frames=nfr, fps=0, srcs=srcs, mode="device", dev_frames=dev)
+        sat = serve.run_mix(frames=nfr, fps=0, srcs=srcs, mode="device", dev_frames=dev, max_blocks=cap)
+        log("serving: saturated, every stream's frame in one batch")
+        sat1 = serve.run_mix(frames=nfr, fps=0, srcs=srcs, mode="device", dev_frames=dev)
         del dev
         torch.cuda.empty_cache()
         log("serving: saturated from page-locked host frames (PCIe-inclusive)")
-        pin = serve.run_mix(frames=min(60, nfr), fps=0, srcs=srcs, mode="pinned")
+        pin = serve.run_mix(frames=min(60, nfr), fps=0, srcs=srcs, mode="pinned", max_blocks=cap)
         serving = {"workload": "16 streams (6x 640x360, 6x 1280x720, 4x 1920x1080) 10-bit 4:2:0, "
                                f"{nfr} distinct frames each; per frame: AMVP predictor choice (spatial candidates "
                                "from the stream's resident MV field, its previous frame's final MVs; prediction "
                                "SAD at quarter pel), integer search +/-16 around the predictor, 32-candidate "
-                               "fractional refinement + MC; reference = previous frame; the blocks of all "
-                               "streams' frames in shared launches",
+                               "fractional refinement + MC; reference = previous frame; the blocks of many "
+                               f"streams' frames in shared launches, batches capped at {cap} blocks so the two "
+                               "sessions serve disjoint halves of the streams side by side",
+                   "max_batch_blocks": cap,
                    "realtime_30fps": rt, "saturated": sat, "saturated_pinned_host": pin,
+                   "saturated_one_batch": sat1,
                    "served_vs_plain": round(sat["mpixel_per_s"] / plain, 3) if plain else None,
                    "note": "device mode: frames resident in HBM before the timed region, results to page-locked "
                            "host memory; pinned mode adds the host-to-device upload of every frame"}

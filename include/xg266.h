@@ -550,7 +550,10 @@ typedef struct xg_server xg_server;
 typedef struct xg_server_cfg {
     uint64_t      hbmBudget;       /* bytes of stream state the server may hold      */
     int32_t       maxStreams;      /* <= 32                                            */
-    int32_t       maxBatchBlocks;  /* blocks per batch                                 */
+    int32_t       maxBatchBlocks;  /* blocks per batch.  A batch waits for the other
+                                      session's batch only if they share a stream,
+                                      so a cap that splits the streams in two lets
+                                      both sessions' batches run side by side      */
     int32_t       searchRange;     /* integer search range in pel (<= 32)              */
     uint32_t      searchLambda;    /* sadLambda of the integer search                  */
     int32_t       maxDelayUs;      /* a free session waits up to this long after the

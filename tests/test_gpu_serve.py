@@ -70,37 +70,57 @@ def _run_alone(srv, src, frames=FRAMES, keep_pred=False):
     return out
 
 
-def test_16_stream_mix_alone_equals_mix_equals_oracle(oracle, mix):
+@pytest.fixture(scope="module")
+def alone(mix):
+    """Each stream of the mix served alone, one frame per batch."""
+    srv = Server(server_cfg(sum(len(s.blocks) for s in mix)))
+    try:
+        out = [_run_alone(srv, s, keep_pred=k in SAMPLED) for k, s in enumerate(mix)]
+    finally:
+        srv.destroy()
+    print("alone runs done", flush=True)
+    return out
+
+
+def _serve_mix(srv, mix):
+    """Every frame of every stream offered at once; returns the reports and buffers."""
+    sids = []
+    for s in mix:
+        rc, sid = srv.open(s.w, s.h)
+        assert rc == 0
+        sids.append(sid)
+    bufs = {}
+    for t in range(FRAMES):
+        for k, s in enumerate(mix):
+            res, pred = _bufs(s)
+            bufs[(k, t)] = (res, pred)
+            assert srv.submit(sids[k], s.frame(t), s.blocks, res, pred, tag=t) == 0
+    got = _drain(srv, FRAMES * len(mix))
+    assert all(r.status == 0 for r in got)
+    return got, bufs
+
+
+def _equal_alone(mix, bufs, alone):
+    for k, s in enumerate(mix):
+        for t in range(FRAMES):
+            res, pred = bufs[(k, t)]
+            assert np.array_equal(res.reshape(-1, 4), alone[k][t][0]), (k, t)
+            assert _digest(pred) == alone[k][t][2], (k, t)
+
+
+def test_16_stream_mix_alone_equals_mix_equals_oracle(oracle, mix, alone):
     assert [(s.w, s.h) for s in mix] == MIX_16
     n_all = sum(len(s.blocks) for s in mix)
     print(f"serving mix: {len(mix)} streams, {n_all} blocks per tick, {FRAMES} frames", flush=True)
     # every batch waits for a frame of every stream: deterministic batching
     srv = Server(server_cfg(n_all, min_batch_frames=len(mix)))
     try:
-        alone = [_run_alone(srv, s, keep_pred=k in SAMPLED) for k, s in enumerate(mix)]
-        print("alone runs done", flush=True)
-        sids = []
-        for s in mix:
-            rc, sid = srv.open(s.w, s.h)
-            assert rc == 0
-            sids.append(sid)
-        bufs = {}
-        for t in range(FRAMES):            # every frame of every stream offered at once
-            for k, s in enumerate(mix):
-                res, pred = _bufs(s)
-                bufs[(k, t)] = (res, pred)
-                assert srv.submit(sids[k], s.frame(t), s.blocks, res, pred, tag=t) == 0
-        got = _drain(srv, FRAMES * len(mix))
-        assert all(r.status == 0 for r in got)
+        got, bufs = _serve_mix(srv, mix)
         assert all(r.batchFrames == len(mix) for r in got), sorted({r.batchFrames for r in got})
-        for k, s in enumerate(mix):
-            for t in range(FRAMES):
-                res, pred = bufs[(k, t)]
-                assert np.array_equal(res.reshape(-1, 4), alone[k][t][0]), (k, t)
-                assert _digest(pred) == alone[k][t][2], (k, t)
+        _equal_alone(mix, bufs, alone)
         st = srv.stats()
-        assert st.framesDone == 2 * FRAMES * len(mix) and st.framesDropped == 0
-        assert st.batches == len(mix) * FRAMES + FRAMES
+        assert st.framesDone == FRAMES * len(mix) and st.framesDropped == 0
+        assert st.batches == FRAMES
     finally:
         srv.destroy()
     print("mix == alone", flush=True)
@@ -133,6 +153,24 @@ def test_16_stream_mix_alone_equals_mix_equals_oracle(oracle, mix):
         f3 = next_field(block_array(s), alone[0][u][0][:, :2], f3, s.w, s.h)
     refp = np.ascontiguousarray(np.pad(s.frame(2)[0], PAD, mode="edge"))
     assert np.any(predictors(oracle, block_array(s), f3, s.frame(3)[0], refp, s.w, s.h) != 0)
+
+
+def test_streams_split_over_both_sessions_run_side_by_side(mix, alone):
+    """Batches capped so the 360p and 720p streams fill one batch and the
+    1080p streams the other: the two sessions' batches share no stream, so
+    neither waits for the other, and every frame still equals the stream
+    served alone."""
+    cap = max(sum(len(s.blocks) for s in mix[:12]), sum(len(s.blocks) for s in mix[12:]))
+    srv = Server(server_cfg(cap))
+    try:
+        got, bufs = _serve_mix(srv, mix)
+        assert max(r.batchFrames for r in got) < len(mix)
+        _equal_alone(mix, bufs, alone)
+        st = srv.stats()
+        assert st.framesDone == FRAMES * len(mix) and st.framesDropped == 0
+    finally:
+        srv.destroy()
+    print("split batches == alone", flush=True)
 
 
 def test_device_and_pinned_submissions_match_pageable(small):

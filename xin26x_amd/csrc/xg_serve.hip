@@ -110,6 +110,7 @@ namespace {
 
 constexpr int kPics = 3;                          // pictures per stream, in rotation
 constexpr int kMaxStreams = 32;
+static_assert(kMaxStreams <= 32, "a batch's streams are kept as a 32-bit mask");
 static_assert(kMaxStreams * kPics <= kSessionPicSlots, "picture slots");
 constexpr int kMaxInFlight = 2;                    // frames per stream on the device at once
 constexpr int kSessions = 2;
@@ -192,6 +193,7 @@ T *dev_member(void *base, size_t offset)
 struct Batch {
     bool active = false;
     std::vector<Request *> reqs;
+    uint32_t streamMask = 0;           // the streams with a frame in this batch (kMaxStreams <= 32)
     hipEvent_t done = nullptr;         // after the batch's output copies
     hipEvent_t computed = nullptr;     // after its refinement: what the next batch's kernels depend on
     int64_t startNs = 0;
@@ -392,6 +394,7 @@ struct xg_server {
             return streams[x].pending.front()->submitNs < streams[y].pending.front()->submitNs;
         });
         b.reqs.clear();
+        b.streamMask = 0;
         int nblk = 0;
         for (int id : ids) {
             Stream &st = streams[id];
@@ -404,6 +407,7 @@ struct xg_server {
             st.inFlight++;
             nblk += r->nblocks;
             b.reqs.push_back(r);
+            b.streamMask |= 1u << id;
         }
         xg_session *s = sess[si];
         hipStream_t hs = (hipStream_t)xg_session_stream(s);
@@ -455,11 +459,14 @@ struct xg_server {
             k_upload_pad<<<dim3(rows, nUp), 256, 0, hs>>>(dev_member<UploadPlane>(b.dDesc, offsetof(Descs, up)));
             if (hipGetLastError() != hipSuccess) rc = XG_ERR_HIP;
         }
-        // the other session's batch holds the previous frames of these streams:
-        // their MV fields, references and partitions (outputs rotate with the
-        // pictures, so its output copies need not be waited for)
+        // if the other session's batch holds the previous frames of any of these
+        // streams, their MV fields, references and partitions come from it
+        // (outputs rotate with the pictures, so its output copies need not be
+        // waited for); batches of disjoint streams run side by side
         const Batch &o = batch[si ^ 1];
-        if (rc == XG_OK && o.active && hipStreamWaitEvent(hs, o.computed, 0) != hipSuccess) rc = XG_ERR_HIP;
+        if (rc == XG_OK && o.active && (o.streamMask & b.streamMask) &&
+            hipStreamWaitEvent(hs, o.computed, 0) != hipSuccess)
+            rc = XG_ERR_HIP;
         for (Request *r : b.reqs)
             if (rc == XG_OK) rc = stage_partition(r, hs);
         if (rc == XG_OK) {

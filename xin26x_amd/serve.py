@@ -72,6 +72,16 @@ class StreamSource:
         return out
 
 
+def two_chain_cap(srcs):
+    """The smallest batch cap (blocks) that splits one frame of every stream,
+    taken in order, into two batches.  With every frame offered at once the
+    two sessions then serve disjoint halves of the streams; a batch waits only
+    for the other session's batch when they share a stream, so the two halves
+    run side by side instead of one after the other."""
+    n = [len(s.blocks) for s in srcs]
+    return min(max(sum(n[:k]), sum(n[k:])) for k in range(1, len(n))) if len(n) > 1 else sum(n)
+
+
 def server_cfg(max_blocks, budget=64 << 30, max_streams=16, search_range=16, search_lambda=1024, max_delay_us=0,
                min_batch_frames=0):
     c = capi.XgServerCfg()
@@ -183,7 +193,7 @@ class DeviceFrames:
 
 
 def run_mix(frames=300, fps=30.0, jitter_ms=3.0, srcs=None, seed=5, max_delay_us=0, mode="device", dev_frames=None,
-            pinned_ring=8):
+            pinned_ring=8, max_blocks=None):
     """Arrivals at `fps` per stream with N(0, jitter) timing (fps <= 0: every
     frame offered at once, bounded by a 4-frame result ring per stream);
     returns the latency/throughput report.
@@ -191,10 +201,12 @@ def run_mix(frames=300, fps=30.0, jitter_ms=3.0, srcs=None, seed=5, max_delay_us
         source), results to page-locked host buffers: no staging anywhere;
       mode "pinned": planes in page-locked host memory (a ring of
         `pinned_ring` distinct frames per stream), read by DMA (PCIe-inclusive);
-      mode "host": pageable numpy planes, copied on submit (xg_server_submit)."""
+      mode "host": pageable numpy planes, copied on submit (xg_server_submit).
+    max_blocks caps a batch's blocks (default: every stream's frame fits one
+    batch); a smaller cap splits the streams over the two sessions' batches."""
     import torch
     srcs = srcs or mix_sources(seed=seed)
-    max_blocks = sum(len(s.blocks) for s in srcs)
+    max_blocks = max_blocks or sum(len(s.blocks) for s in srcs)
     srv = Server(server_cfg(max_blocks, max_delay_us=max_delay_us))
     try:
         sids = []
@@ -301,12 +313,17 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=40, help="distinct frames per stream")
     ap.add_argument("--fps", type=float, default=0.0, help="arrival rate per stream (0: every frame at once)")
     ap.add_argument("--mode", choices=("device", "pinned", "host"), default="device")
+    ap.add_argument("--batch-blocks", type=int, default=0,
+                    help="blocks per batch (0: two_chain_cap, half the streams per batch)")
+    ap.add_argument("--one-batch", action="store_true", help="every stream's frame in one batch")
     args = ap.parse_args(argv)
     torch.cuda.set_device(0)
     srcs = mix_sources()
     dev = [DeviceFrames(s, args.frames) for s in srcs] if args.mode == "device" else None
     print(f"serving: {len(srcs)} streams, {args.frames} frames each, mode {args.mode}", flush=True)
-    rep = run_mix(frames=args.frames, fps=args.fps, srcs=srcs, mode=args.mode, dev_frames=dev)
+    cap = None if args.one_batch else (args.batch_blocks or two_chain_cap(srcs))
+    rep = run_mix(frames=args.frames, fps=args.fps, srcs=srcs, mode=args.mode, dev_frames=dev, max_blocks=cap)
+    rep["max_batch_blocks"] = cap or sum(len(s.blocks) for s in srcs)
     print(json.dumps(rep), flush=True)
 
 
