@@ -303,6 +303,72 @@ __device__ __forceinline__ void hrow2(const uint32_t *win, int idx, const HTaps<
     h.filter(T, t0, t1);
 }
 
+// n rows of a lane's dword column, stride csw dwords, walked with one
+// pointer: the empty asm hides the pointer's history, so the compiler does
+// not keep a 64-bit offset per row in scalar registers (they spilled).
+template <int N>
+__device__ __forceinline__ void load_rows(const gdword *g, intptr_t csw, uint32_t *dst)
+{
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        dst[k] = *g;
+        g += csw;
+        asm volatile("" : "+v"(g));
+    }
+}
+
+// One TS-row tile of interp_satd_pass: the first pass of its TS window rows
+// into the new pairs N0/N1 (the pairs before them are the last TAPS - 2 of
+// C0/C1), the second pass, and the SATD against the current rows cw.  When
+// `pre` is set the next tile's current rows (gn) are loaded into cn meanwhile.
+// Returns the tile's SATD on the lane that holds it (column pair 0 of the
+// tile), 0 on the others.
+template <int TAPS, int TS, int LS>
+__device__ __forceinline__ int satd_tile(const uint32_t *&q, HRow<TAPS> &cur_row, HRow<TAPS> &next_row,
+                                         const HTaps<TAPS> &ht, const uint32_t *vt, int &tl0, int &tl1,
+                                         const uint32_t *C0, const uint32_t *C1, uint32_t *N0, uint32_t *N1,
+                                         const uint32_t *cw, uint32_t *cn, bool pre, const gdword *gn,
+                                         intptr_t csw, int m, int lane)
+{
+    constexpr int lw = LS >> 1, CR = TAPS - 2;
+    if (pre) load_rows<TS>(gn, csw, cn);
+#pragma unroll
+    for (int k = 0; k < TS; k++) {
+        int b0, b1;
+        next_row.fetch(q + (k + 1) * lw);           // one row past the window at the very end: unused
+        cur_row.sums(ht, b0, b1);
+        cur_row = next_row;
+        N0[k] = pack_mid16(tl0, b0);
+        N1[k] = pack_mid16(tl1, b1);
+        tl0 = b0;
+        tl1 = b1;
+    }
+    q += TS * lw;
+    // pair v of the tile: the carried ones, then the new ones; the second
+    // pass's 2 TS dot2 chains are interleaved tap by tap (consecutive dot2 of
+    // one chain would wait on each other)
+    auto p0 = [&](int v) { return v < CR ? C0[TS - CR + v] : N0[v - CR]; };
+    auto p1 = [&](int v) { return v < CR ? C1[TS - CR + v] : N1[v - CR]; };
+    v2s d[TS];
+    int s0[TS], s1[TS];
+#pragma unroll
+    for (int k = 0; k < TS; k++) {
+        s0[k] = dot2_first(p0(k), vt[0], 0);
+        s1[k] = dot2_first(p1(k), vt[0], 0);
+    }
+#pragma unroll
+    for (int j = 1; j < TAPS / 2; j++)
+#pragma unroll
+        for (int k = 0; k < TS; k++) {
+            s0[k] = dot2(p0(k + 2 * j), vt[j], s0[k]);
+            s1[k] = dot2(p1(k + 2 * j), vt[j], s1[k]);
+        }
+#pragma unroll
+    for (int k = 0; k < TS; k++) d[k] = as_v2s(cw[k]) - as_v2s(pack_hi16(clip_hi16(s0[k]), clip_hi16(s1[k])));
+    const int ts = tile_satd2<TS>(d, lane);
+    return (m & (TS / 2 - 1)) == 0 ? ts : 0;
+}
+
 // Interpolation + SATD of one unit (a candidate's plane) of W x H samples for
 // the lane's columns 2m, 2m+1:
 //   win, LS: the plane's LDS window and its row stride in samples (even)
@@ -351,56 +417,81 @@ __device__ __forceinline__ int interp_satd_pass(const uint32_t *win, int wx, int
     q += (TAPS - 1) * lw;                           // cur_row holds row TAPS - 1
     const gdword *gc = (const gdword *)(cur + 2 * m);
     const intptr_t csw = cs >> 1;
-    uint32_t cw[TS], cn[TS];
+    if constexpr (TS >= TAPS - 2) {
+        // Two tiles per iteration with the row pairs and current rows in
+        // ping-pong arrays: a tile writes its TS new pairs into one array and
+        // the next tile takes the TAPS - 2 pairs it carries from that
+        // array's end, so no registers are copied between tiles.
+        constexpr int CR = TAPS - 2;
+        uint32_t A0[TS], A1[TS], B0[TS], B1[TS], cwA[TS], cwB[TS];
 #pragma unroll
-    for (int k = 0; k < TS; k++) cw[k] = gc[(intptr_t)k * csw];
-    int acc = 0;
-    for (int y0 = 0; y0 < H; y0 += TS) {
-        if (y0 + TS < H) {                          // the next tile's current rows, in flight meanwhile
-#pragma unroll
-            for (int k = 0; k < TS; k++) cn[k] = gc[(intptr_t)(y0 + TS + k) * csw];
+        for (int k = 0; k < CR; k++) {
+            B0[TS - CR + k] = P0[k];
+            B1[TS - CR + k] = P1[k];
         }
-#pragma unroll
-        for (int k = 0; k < TS; k++) {
-            int b0, b1;
-            next_row.fetch(q + (k + 1) * lw);       // one row past the window at the very end: unused
-            cur_row.sums(ht, b0, b1);
-            cur_row = next_row;
-            P0[TAPS - 2 + k] = pack_mid16(tl0, b0);
-            P1[TAPS - 2 + k] = pack_mid16(tl1, b1);
-            tl0 = b0;
-            tl1 = b1;
+        load_rows<TS>(gc, csw, cwA);
+        int acc = 0;
+        for (int y0 = 0;;) {
+            acc += satd_tile<TAPS, TS, LS>(q, cur_row, next_row, ht, vt, tl0, tl1, B0, B1, A0, A1, cwA, cwB,
+                                           y0 + TS < H, gc + (intptr_t)(y0 + TS) * csw, csw, m, lane);
+            y0 += TS;
+            if (y0 >= H) break;
+            acc += satd_tile<TAPS, TS, LS>(q, cur_row, next_row, ht, vt, tl0, tl1, A0, A1, B0, B1, cwB, cwA,
+                                           y0 + TS < H, gc + (intptr_t)(y0 + TS) * csw, csw, m, lane);
+            y0 += TS;
+            if (y0 >= H) break;
         }
-        q += TS * lw;
-        // second pass of the tile's rows, the 2 TS dot2 chains interleaved tap by tap
-        // (consecutive dot2 of one chain would wait on each other)
-        v2s d[TS];
-        int s0[TS], s1[TS];
-#pragma unroll
-        for (int k = 0; k < TS; k++) {
-            s0[k] = dot2_first(P0[k], vt[0], 0);
-            s1[k] = dot2_first(P1[k], vt[0], 0);
-        }
-#pragma unroll
-        for (int j = 1; j < TAPS / 2; j++)
+        return acc;
+    } else {
+        uint32_t cw[TS], cn[TS];
+        load_rows<TS>(gc, csw, cw);
+        int acc = 0;
+        for (int y0 = 0; y0 < H; y0 += TS) {
+            if (y0 + TS < H)                            // the next tile's current rows, in flight meanwhile
+                load_rows<TS>(gc + (intptr_t)(y0 + TS) * csw, csw, cn);
 #pragma unroll
             for (int k = 0; k < TS; k++) {
-                s0[k] = dot2(P0[k + 2 * j], vt[j], s0[k]);
-                s1[k] = dot2(P1[k + 2 * j], vt[j], s1[k]);
+                int b0, b1;
+                next_row.fetch(q + (k + 1) * lw);       // one row past the window at the very end: unused
+                cur_row.sums(ht, b0, b1);
+                cur_row = next_row;
+                P0[TAPS - 2 + k] = pack_mid16(tl0, b0);
+                P1[TAPS - 2 + k] = pack_mid16(tl1, b1);
+                tl0 = b0;
+                tl1 = b1;
+            }
+            q += TS * lw;
+            // second pass of the tile's rows, the 2 TS dot2 chains interleaved tap by tap
+            // (consecutive dot2 of one chain would wait on each other)
+            v2s d[TS];
+            int s0[TS], s1[TS];
+#pragma unroll
+            for (int k = 0; k < TS; k++) {
+                s0[k] = dot2_first(P0[k], vt[0], 0);
+                s1[k] = dot2_first(P1[k], vt[0], 0);
             }
 #pragma unroll
-        for (int k = 0; k < TS; k++) d[k] = as_v2s(cw[k]) - as_v2s(pack_hi16(clip_hi16(s0[k]), clip_hi16(s1[k])));
-        const int ts = tile_satd2<TS>(d, lane);
-        if ((m & (TS / 2 - 1)) == 0) acc += ts;
+            for (int j = 1; j < TAPS / 2; j++)
 #pragma unroll
-        for (int k = 0; k < TAPS - 2; k++) {
-            P0[k] = P0[TS + k];
-            P1[k] = P1[TS + k];
+                for (int k = 0; k < TS; k++) {
+                    s0[k] = dot2(P0[k + 2 * j], vt[j], s0[k]);
+                    s1[k] = dot2(P1[k + 2 * j], vt[j], s1[k]);
+                }
+#pragma unroll
+            for (int k = 0; k < TS; k++)
+                d[k] = as_v2s(cw[k]) - as_v2s(pack_hi16(clip_hi16(s0[k]), clip_hi16(s1[k])));
+            const int ts = tile_satd2<TS>(d, lane);
+            if ((m & (TS / 2 - 1)) == 0) acc += ts;
+#pragma unroll
+            for (int k = 0; k < TAPS - 2; k++) {
+                P0[k] = P0[TS + k];
+                P1[k] = P1[TS + k];
+            }
+#pragma unroll
+            for (int k = 0; k < TS; k++) cw[k] = cn[k];
         }
-#pragma unroll
-        for (int k = 0; k < TS; k++) cw[k] = cn[k];
+        return acc;
     }
-    return acc;
 }
 
 // Luma SATD of every candidate of a task; lane = unit * (W / 2) + m.
