@@ -546,59 +546,49 @@ __device__ __forceinline__ const gdword *plane_dw(const xg_plane &pl, int x, int
     return (const gdword *)(pl.data + (intptr_t)y * pl.stride + x);
 }
 
-// Global -> LDS copy plan of the windows of up to two planes with one
-// geometry: `rows` rows of `wd` dwords (wd <= 64) from s0/s1 (row pitch sp
-// dwords) to d0/d1 (row pitch dp dwords).  Each lane owns one dword column and
-// walks rows with a stride of 64 / wd; loads go in batches of 8 per lane, all
-// issued before the batch's first store.
+// Global -> LDS copy plan of one plane's window: `rows` rows of `wd` dwords
+// (wd <= 64) from s (row pitch sp dwords) to d (row pitch dp dwords).  Lane
+// (c, r0) = (lane % wd, lane / wd) copies column c of rows r0, r0 + per, ...
+// (per = 64 / wd rows at a time).  Loads go in batches of 8 per lane, all
+// issued before the batch's first store, and a lane loads only its own rows.
 struct StagePlan {
-    const gdword *s0, *s1;
-    uint32_t *d0, *d1;
-    intptr_t sp;
-    int dp, rows, total, per, c, r0, batches;
+    const gdword *s;     // the lane's first source dword
+    uint32_t *d;         // its first LDS dword
+    intptr_t sstep;      // source dwords from one of its rows to the next
+    int dstep;           // LDS dwords likewise
+    int n;               // its rows
+    int batches;         // batches of the lanes with the most rows (wave-uniform)
 };
 
-__device__ __forceinline__ StagePlan stage_plan(uint32_t *d0, uint32_t *d1, int dp, const gdword *s0,
-                                                const gdword *s1, intptr_t sp, int rows, int wd, int planes)
+__device__ __forceinline__ StagePlan stage_plan(uint32_t *d, int dp, const gdword *s, intptr_t sp, int rows, int wd)
 {
+    const int per = 64 / wd, c = (int)threadIdx.x % wd, r0 = (int)threadIdx.x / wd;
     StagePlan P;
-    P.s0 = s0;
-    P.s1 = s1;
-    P.d0 = d0;
-    P.d1 = d1;
-    P.sp = sp;
-    P.dp = dp;
-    P.rows = rows;
-    P.total = planes * rows;
-    P.per = 64 / wd;
-    P.c = (int)threadIdx.x % wd;
-    P.r0 = (int)threadIdx.x / wd;
-    P.batches = (P.total + 8 * P.per - 1) / (8 * P.per);
+    P.s = s + (intptr_t)r0 * sp + c;
+    P.d = d + r0 * dp + c;
+    P.sstep = (intptr_t)per * sp;
+    P.dstep = per * dp;
+    P.n = r0 < per && r0 < rows ? (rows - r0 + per - 1) / per : 0;
+    P.batches = ((rows + per - 1) / per + 7) / 8;
     return P;
 }
 
 __device__ __forceinline__ void stage_load(const StagePlan &P, int k, uint32_t *v)
 {
-    if (P.r0 >= P.per) return;
+    const gdword *g = P.s + (intptr_t)(8 * k) * P.sstep;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        const int rr = min(P.r0 + (8 * k + j) * P.per, P.total - 1);
-        const bool second = rr >= P.rows;
-        v[j] = (second ? P.s1 : P.s0)[(intptr_t)(second ? rr - P.rows : rr) * P.sp + P.c];
+        if (8 * k + j < P.n) v[j] = *g;
+        g += P.sstep;
+        asm volatile("" : "+v"(g));              // one 64-bit add per row, no offsets held per row
     }
 }
 
 __device__ __forceinline__ void stage_store(const StagePlan &P, int k, const uint32_t *v)
 {
-    if (P.r0 >= P.per) return;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int rr = P.r0 + (8 * k + j) * P.per;
-        if (rr < P.total) {
-            const bool second = rr >= P.rows;
-            (second ? P.d1 : P.d0)[(second ? rr - P.rows : rr) * P.dp + P.c] = v[j];
-        }
-    }
+    for (int j = 0; j < 8; j++)
+        if (8 * k + j < P.n) P.d[(8 * k + j) * P.dstep] = v[j];
 }
 
 // Motion compensation of one plane from its LDS window: lanes [lane0,
@@ -664,18 +654,25 @@ __device__ __forceinline__ SubWin stage_sub(const RefineArgs &a, const xg_block 
     w.cx0 = ((qx0 >> 5) - 1) & ~1;
     w.cy0 = (qy0 >> 5) - 1;
     const int CWW = (qx1 >> 5) + (sw >> 1) + 2 - w.cx0, CHH = (qy1 >> 5) - (qy0 >> 5) + (sh >> 1) + 3;
-    const StagePlan pl = stage_plan(L.luma, L.luma, kLumaLW / 2, plane_dw(R.plane[0], w.wx0, w.wy0), nullptr,
-                                    R.plane[0].stride >> 1, HH, (WW + 1) >> 1, 1);
-    const StagePlan pc = stage_plan(L.cb, L.cr, kChromaLW / 2, plane_dw(R.plane[1], w.cx0, w.cy0),
-                                    plane_dw(R.plane[2], w.cx0, w.cy0), R.plane[1].stride >> 1, CHH, (CWW + 1) >> 1,
-                                    2);
-    const int nb = max(pl.batches, pc.batches);
+    const StagePlan pl = stage_plan(L.luma, kLumaLW / 2, plane_dw(R.plane[0], w.wx0, w.wy0), R.plane[0].stride >> 1,
+                                    HH, (WW + 1) >> 1);
+    const StagePlan pu = stage_plan(L.cb, kChromaLW / 2, plane_dw(R.plane[1], w.cx0, w.cy0), R.plane[1].stride >> 1,
+                                    CHH, (CWW + 1) >> 1);
+    const StagePlan pv = stage_plan(L.cr, kChromaLW / 2, plane_dw(R.plane[2], w.cx0, w.cy0), R.plane[2].stride >> 1,
+                                    CHH, (CWW + 1) >> 1);
+    const int nb = max(pl.batches, pu.batches);
     for (int k = 0; k < nb; k++) {
-        uint32_t vl[8], vc[8];
+        uint32_t vl[8], vu[8], vv[8];
         if (k < pl.batches) stage_load(pl, k, vl);
-        if (k < pc.batches) stage_load(pc, k, vc);
+        if (k < pu.batches) {
+            stage_load(pu, k, vu);
+            stage_load(pv, k, vv);
+        }
         if (k < pl.batches) stage_store(pl, k, vl);
-        if (k < pc.batches) stage_store(pc, k, vc);
+        if (k < pu.batches) {
+            stage_store(pu, k, vu);
+            stage_store(pv, k, vv);
+        }
     }
     return w;
 }
