@@ -1437,8 +1437,9 @@ def main():
         "luma_cand_px_per_s": round(cand_px / t_graph, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
-                     "binding": "VALU issue (integer filters + Hadamard): see valu_roofline; HBM traffic is ~1.6x "
-                                "the algorithmic bytes, far from its ceiling",
+                     "binding": "VALU issue (integer filters + Hadamard): see valu_roofline; HBM traffic is "
+                                + (f"{traffic / alg_bytes:.2f}x the algorithmic bytes, " if traffic else "")
+                                + "far from its ceiling",
                      "kernel": "k_refine_persistent", "kernel_ms": round(ms.value, 4), "launches": cnt.value,
                      "alg_bytes_per_launch": alg_bytes},
         "valu_roofline": valu_roofline(PMC_REFINE, ms.value),
