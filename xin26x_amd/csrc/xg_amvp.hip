@@ -59,8 +59,6 @@ __device__ __forceinline__ void taps_x64(const int8_t *t, uint32_t *p)
     for (int m = 0; m < 4; m++) p[m] = pack16(64 * (int)t[2 * m], 64 * (int)t[2 * m + 1]);
 }
 
-typedef const __attribute__((address_space(1))) uint16_t gword;
-
 // The current samples (c, c + 1) and (c + 2, c + 3) of a row as two dwords.
 __device__ __forceinline__ void cur_pairs(const gword *c, bool aligned, uint32_t &p01, uint32_t &p23)
 {

@@ -157,7 +157,10 @@ __device__ __forceinline__ int tile_satd(int *d, int lane)
 // Packed 16-bit pairs: two samples (or two int16 values) per dword.
 // ---------------------------------------------------------------------------
 typedef short v2s __attribute__((ext_vector_type(2)));
+// Global-memory typed pointers: loads through them stay global_load (a
+// generic pointer, e.g. one passed through an empty asm, becomes flat_load).
 typedef const __attribute__((address_space(1))) uint32_t gdword;
+typedef const __attribute__((address_space(1))) uint16_t gword;
 
 __device__ __forceinline__ v2s as_v2s(uint32_t v) { return __builtin_bit_cast(v2s, v); }
 __device__ __forceinline__ uint32_t as_u32(v2s v) { return __builtin_bit_cast(uint32_t, v); }

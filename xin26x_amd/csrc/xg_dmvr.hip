@@ -122,6 +122,15 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
         const int rw = sw + 5, rh = sh + 5;          // source region of the subblock (filter reach included)
         const int bw = sw + 4, bh = sh + 4;          // bilinear prediction of the subblock plus padding
         int fxy[2][2];
+        // Both lists' source regions.  Lane = column (rw <= 21 < 32) + 32 *
+        // row parity, so a lane holds at most kRowsPerLane rows of each list;
+        // it issues all their loads before its first LDS store, and the
+        // subblock waits for one memory latency instead of one per row.  A
+        // walking pointer (hidden from the compiler by an empty asm) keeps
+        // the row offsets out of scalar registers.
+        constexpr int kRowsPerLane = 11;                 // ceil(21 / 2)
+        const int c = lane & 31, r0 = lane >> 5;
+        uint32_t v[2][kRowsPerLane];
 #pragma unroll
         for (int l = 0; l < 2; l++) {
             int ix, iy, fx, fy;
@@ -129,31 +138,44 @@ __global__ void __launch_bounds__(64) k_dmvr(DmvrPic p0, DmvrPic p1, const DmvrT
             fxy[l][0] = fx;
             fxy[l][1] = fy;
             const DmvrPic p = l ? p1 : p0;
-            const uint16_t *s = p.data + (intptr_t)(iy + sy) * p.stride + (ix + sx);
-            // lane = column (rw <= 21 < 32) + 32 * row parity: no divisions
-            const int c = lane & 31;
-            if (c < rw)
-                for (int r = lane >> 5; r < rh; r += 2) {
-                    // the last row and column are read only by the filters that need them
-                    const bool need = (c < bw || fx) && (r < bh || fy);
-                    src[l][r * 21 + c] = need ? s[(intptr_t)r * p.stride + c] : 0;
-                }
+            const gword *g = (const gword *)(p.data + (intptr_t)(iy + sy + r0) * p.stride + (ix + sx + c));
+            const intptr_t step = 2 * p.stride;
+            // the last row and column are read only by the filters that need them
+            const bool col = c < rw && (c < bw || fx);
+#pragma unroll
+            for (int j = 0; j < kRowsPerLane; j++) {
+                const int r = r0 + 2 * j;
+                v[l][j] = 0;
+                if (col && r < rh && (r < bh || fy)) v[l][j] = *g;
+                g += step;
+                asm volatile("" : "+v"(g));
+            }
         }
         if (lane < 25) cost[lane] = 0;
+#pragma unroll
+        for (int l = 0; l < 2; l++)
+#pragma unroll
+            for (int j = 0; j < kRowsPerLane; j++) {
+                const int r = r0 + 2 * j;
+                if (c < rw && r < rh) src[l][r * 21 + c] = (uint16_t)v[l][j];
+            }
         __syncthreads();
+        // bilinear predictions, all of a lane's rows unrolled (bh <= 20)
 #pragma unroll
         for (int l = 0; l < 2; l++) {
             const int fx = fxy[l][0], fy = fxy[l][1];
-            const int c = lane & 31;                      // bw <= 20 < 32
-            if (c < bw)
-                for (int r = lane >> 5; r < bh; r += 2) {
+            if (c < bw)                                   // bw <= 20 < 32
+#pragma unroll
+                for (int j = 0; j < kRowsPerLane - 1; j++) {
+                    const int r = r0 + 2 * j;
+                    if (r >= bh) break;
                     const uint16_t *q = &src[l][r * 21 + c];
-                    int v;
-                    if (fx && fy) v = bili2(bili2(q[0], q[1], fx), bili2(q[21], q[22], fx), fy);
-                    else if (fx) v = bili2(q[0], q[1], fx);
-                    else if (fy) v = bili2(q[0], q[21], fy);
-                    else v = q[0];
-                    bil[l][r * 20 + c] = (int16_t)v;
+                    int b;
+                    if (fx && fy) b = bili2(bili2(q[0], q[1], fx), bili2(q[21], q[22], fx), fy);
+                    else if (fx) b = bili2(q[0], q[1], fx);
+                    else if (fy) b = bili2(q[0], q[21], fy);
+                    else b = q[0];
+                    bil[l][r * 20 + c] = (int16_t)b;
                 }
         }
         __syncthreads();
