@@ -720,6 +720,7 @@ namespace xg {
 
 constexpr int kGroupWaves = 4;
 constexpr int kGroupCurDw = 64 * 32;             // the group's current samples: 64 rows x 32 dwords
+static_assert(kGroupCurDw % (64 * kGroupWaves) == 0, "whole current-sample rounds per thread");
 constexpr int kGroupK = 10;                      // candidates per lane (an odd number of dwords apart: LDS banks)
 
 struct GroupBlk {
@@ -909,14 +910,35 @@ __device__ __forceinline__ void group_units(const uint32_t *win, int ws, const u
 }
 
 // rows x (ws/2) dwords of the reference luma plane from picture column cx
-// (even) and row cy into win, in the win_row layout.
+// (even) and row cy into win, in the win_row layout.  A thread's dwords i,
+// i + blockDim, ... go in batches of 8 loads before their stores (a load per
+// iteration would wait out each load's latency in turn); its (row, column)
+// steps by (blockDim / nd, blockDim % nd) with a carry, with no division per
+// dword.
 __device__ __forceinline__ void stage_win_dw(uint32_t *win, int ws, const xg_plane &pl, int cx, int cy, int rows)
 {
-    const int nd = ws >> 1, n = rows * nd;
+    const int nd = ws >> 1, n = rows * nd, nt = (int)blockDim.x;
+    const int dr = nt / nd, dc = nt - dr * nd;
     const uint16_t *base = pl.data + (intptr_t)cy * pl.stride + cx;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int r = i / nd, c = i - r * nd;
-        win[win_row(r, nd) + c] = *reinterpret_cast<const uint32_t *>(base + (intptr_t)r * pl.stride + 2 * c);
+    int r = (int)threadIdx.x / nd, c = (int)threadIdx.x - r * nd;
+    for (int i0 = threadIdx.x; i0 < n; i0 += 8 * nt) {
+        uint32_t v[8];
+        int at[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool in = i0 + k * nt < n;
+            at[k] = in ? win_row(r, nd) + c : -1;
+            if (in) v[k] = *(const gdword *)(base + (intptr_t)r * pl.stride + 2 * c);
+            r += dr;
+            c += dc;
+            if (c >= nd) {
+                c -= nd;
+                r++;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (at[k] >= 0) win[at[k]] = v[k];
     }
 }
 
@@ -992,13 +1014,20 @@ __global__ void __launch_bounds__(64 * kGroupWaves) __attribute__((amdgpu_waves_
         ibeg[0] = 0;
         for (int j = 0; j < count; j++) ibeg[j + 1] = ibeg[j] + gb[j].items;
     }
-    // the group's current samples (within 64x64 of (cx0, cy0): checked on the host)
-    for (int i = t; i < kGroupCurDw; i += blockDim.x) {
-        const int r = i >> 5, c = i & 31;
-        const bool in = cy0 + r < fh && cx0 + 2 * c + 1 < fw + 1 && cx0 + 2 * c < fw;
-        cur[i] = in ? *reinterpret_cast<const uint32_t *>(P.plane[0].data + (intptr_t)(cy0 + r) * P.plane[0].stride +
-                                                           cx0 + 2 * c)
-                    : 0u;
+    // the group's current samples (within 64x64 of (cx0, cy0): checked on the
+    // host), every load of a thread issued before its first store
+    {
+        constexpr int kPer = kGroupCurDw / (64 * kGroupWaves);
+        uint32_t v[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int i = t + k * 64 * kGroupWaves, r = i >> 5, c = i & 31;
+            const bool in = cy0 + r < fh && cx0 + 2 * c < fw;
+            v[k] = in ? *(const gdword *)(P.plane[0].data + (intptr_t)(cy0 + r) * P.plane[0].stride + cx0 + 2 * c)
+                      : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; k++) cur[t + k * 64 * kGroupWaves] = v[k];
     }
     if (win_row(rows + 1, ws >> 1) <= winDw) {               // an item may read one row past the windows
         stage_win_dw(win, ws, R.plane[0], ux0, uy0, rows);
