@@ -1,8 +1,10 @@
 """The bench's readers of the committed rocprofv3 summaries (CPU only): the
-traffic figure and the SQ counters the rooflines take from them."""
+traffic figure and the SQ counters the rooflines take from them; the serving
+batch cap the bench uses."""
 import os
 
 import bench
+from xin26x_amd import serve
 
 
 def test_profile_readers(tmp_path):
@@ -22,3 +24,25 @@ def test_committed_profiles_carry_the_figures_the_bench_reads():
         assert os.path.exists(path), path
         assert bench.profiled_traffic(path)[0], path
     assert bench.profiled_counter(bench.PMC_REFINE, "SQ_INSTS_VALU") > 0
+
+
+class _Src:
+    def __init__(self, n):
+        self.blocks = [None] * n
+
+
+def test_two_chain_cap_splits_the_streams_in_two():
+    # the smallest cap under which the streams, taken in order, fill exactly two batches
+    srcs = [_Src(n) for n in (3, 3, 3, 10, 10)]
+    cap = serve.two_chain_cap(srcs)
+    assert cap == 19                                    # (3 + 3 + 3 + 10) | 10 beats 9 | 20
+    sizes = [len(s.blocks) for s in srcs]
+    batches, cur = [], 0
+    for n in sizes:                                     # the server's greedy fill in stream order
+        if cur and cur + n > cap:
+            batches.append(cur)
+            cur = 0
+        cur += n
+    batches.append(cur)
+    assert len(batches) == 2 and max(batches) <= cap
+    assert serve.two_chain_cap([_Src(7)]) == 7
