@@ -1280,6 +1280,7 @@ def main():
 
     if args.quick:
         t_graph = timed(capi.MODE_GRAPH)
+        t_direct = timed(capi.MODE_BATCHED)
         t_plain = t_plain_tuned = float("nan")
         exact, lat = None, np.array([float("nan")])
     else:
@@ -1295,17 +1296,20 @@ def main():
             capi.check(lib.xg_session_set_tile(s, sh[0], sh[1], tuned[sh]), "tile")
         log("plain form with the tuned candidates-per-workgroup table")
         t_plain_tuned = timed(capi.MODE_PLAIN)
-        log("batched form (graph)")
+        log("batched form (graph replay)")
         t_graph = timed(capi.MODE_GRAPH)
         out_graph = outputs()
-        exact = all(np.array_equal(a, b) for a, b in zip(out_plain[:2], out_graph[:2])) and \
-            all(np.array_equal(a, b) for a, b in zip(out_plain[2], out_graph[2]))
+        log("batched form (direct launch)")
+        t_direct = timed(capi.MODE_BATCHED)
+        out_direct = outputs()
+        exact = all(all(np.array_equal(a, b) for a, b in zip(out_plain[:2], o[:2])) and
+                    all(np.array_equal(a, b) for a, b in zip(out_plain[2], o[2])) for o in (out_graph, out_direct))
 
-        # per-frame latency: submit one frame, wait for it
+        # per-frame latency: submit one frame, wait for it (the faster form)
         lat = []
         for _ in range(max(args.steps, 20)):
             t0 = time.perf_counter()
-            run(capi.MODE_GRAPH, 1)
+            run(capi.MODE_BATCHED if t_direct < t_graph else capi.MODE_GRAPH, 1)
             capi.check(lib.xg_session_sync(s), "sync")
             lat.append((time.perf_counter() - t0) * 1e3)
         lat = np.array(lat)
@@ -1327,7 +1331,11 @@ def main():
     k_s = ms.value / 1e3
     achieved = alg_bytes / k_s / 1e9
     cand_px = int((fp.blocks[:, 2] * fp.blocks[:, 3]).sum()) * nc
-    value = frame_px / t_graph / 1e6
+    # the same persistent kernel either way: replayed from a captured graph or
+    # launched directly; the faster is the value, both are reported
+    t_best = min(t_graph, t_direct)
+    form = "direct launch" if t_direct < t_graph else "graph replay"
+    value = frame_px / t_best / 1e6
     plain = frame_px / t_plain / 1e6
     plain_tuned = frame_px / t_plain_tuned / 1e6
 
@@ -1416,7 +1424,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_graph * 1e3, 4),
+        "ms_per_step": round(t_best * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -1426,15 +1434,17 @@ def main():
                                f"{n} QTMT blocks, {nc} fractional candidates/block, 8-tap luma + 4-tap "
                                "chroma interpolation + SATD, MC of the best candidate",
                    "frame": f"{W}x{H}", "blocks": n, "candidates": nc, "global_batch": 1, "tiles": tuning,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "form": form},
+        "direct_launch_ms_per_step": round(t_direct * 1e3, 4),
+        "graph_replay_ms_per_step": round(t_graph * 1e3, 4),
         "plain_value": round(plain, 2),
         "plain_tuned_value": round(plain_tuned, 2),
         "batched_vs_plain": round(value / plain, 3),
         "bit_exact_batched_vs_plain": exact,
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
-        "block_cand_evals_per_s": round(n * nc / t_graph, 1),
-        "luma_cand_px_per_s": round(cand_px / t_graph, 1),
+        "block_cand_evals_per_s": round(n * nc / t_best, 1),
+        "luma_cand_px_per_s": round(cand_px / t_best, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                      "binding": "VALU issue (integer filters + Hadamard): see valu_roofline; HBM traffic is "
