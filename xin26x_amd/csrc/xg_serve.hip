@@ -212,7 +212,7 @@ struct xg_server {
     std::condition_variable cvWork, cvDone;
     std::thread thread;
     bool stop = false;
-    xg_session *sess[kSessions] = {nullptr, nullptr};
+    xg_session *sess[kSessions] = {};
     Batch batch[kSessions];
     Stream streams[kMaxStreams];
     std::deque<xg_frame_result> completed;
@@ -459,14 +459,16 @@ struct xg_server {
             k_upload_pad<<<dim3(rows, nUp), 256, 0, hs>>>(dev_member<UploadPlane>(b.dDesc, offsetof(Descs, up)));
             if (hipGetLastError() != hipSuccess) rc = XG_ERR_HIP;
         }
-        // if the other session's batch holds the previous frames of any of these
+        // if another session's batch holds the previous frames of any of these
         // streams, their MV fields, references and partitions come from it
         // (outputs rotate with the pictures, so its output copies need not be
         // waited for); batches of disjoint streams run side by side
-        const Batch &o = batch[si ^ 1];
-        if (rc == XG_OK && o.active && (o.streamMask & b.streamMask) &&
-            hipStreamWaitEvent(hs, o.computed, 0) != hipSuccess)
-            rc = XG_ERR_HIP;
+        for (int oi = 0; oi < kSessions; oi++) {
+            const Batch &o = batch[oi];
+            if (rc == XG_OK && oi != si && o.active && (o.streamMask & b.streamMask) &&
+                hipStreamWaitEvent(hs, o.computed, 0) != hipSuccess)
+                rc = XG_ERR_HIP;
+        }
         for (Request *r : b.reqs)
             if (rc == XG_OK) rc = stage_partition(r, hs);
         if (rc == XG_OK) {
@@ -537,7 +539,8 @@ struct xg_server {
                 else if (q != hipErrorNotReady) fail_batch(i, XG_ERR_HIP);
             }
             const int64_t gather = stop ? 0 : gather_wait_ns();
-            bool anyActive = batch[0].active || batch[1].active;
+            bool anyActive = false;
+            for (int i = 0; i < kSessions; i++) anyActive |= batch[i].active;
             if (gather > 0) {
                 // wake for new frames, or to poll the batches in flight
                 const int64_t w = anyActive ? std::min<int64_t>(gather, 100000) : std::min<int64_t>(gather, 50000000);
